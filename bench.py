@@ -1,0 +1,192 @@
+"""Benchmark of the Chebyshev moment recursion on one MI355X.
+
+Workload (default): graphene honeycomb, 1024 x 1024 cells periodic
+(2,097,152 sites), t = -2.7 eV, on-site disorder uniform in [-0.25, 0.25] eV,
+R = 64 random-phase vectors as one block, N = 2048 moments, fp64 recursion
+(complex128 vectors, real fp64 H).  H and the vector block are resident in
+HBM before the timed region.
+
+A step is one application of H~ = (H - b)/a to the whole 64-vector block,
+with the three-term update and the moment partials fused (one kernel
+launch).  With bras = kets each step yields two moments
+(mu_{2k} = 2<T_k|T_k> - mu_0, mu_{2k+1} = 2<T_{k+1}|T_k> - mu_1), so N = 2048
+moments take 1024 steps.  --steps K times exactly K steps (2K moments).
+
+Prints one JSON line.  roofline.achieved uses the algorithmic bytes of a
+step: nnz*(8+4) + (n+1)*4 (CSR matrix) + 3*n*R*16 (read T_k, read T_{k-1},
+write T_{k+1}).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def build_c2(L=1024, t=-2.7, W=0.5, seed=2026):
+    from tight_binding_amd import lattice as Lm
+
+    cell = Lm.UnitCell.model("honeycomb")
+    cell.add_hopping_nn("A", 1, t)
+    lat = Lm.PartialLattice(cell, (L, L))
+    lat.set_disorder(W, seed)
+    H = lat.get_Hamiltonian_periodic_extend_bare_sparse((1, 1))
+    # Gershgorin bounds, contracted like the reference's (E_max-E_min)/(2-eps)
+    emax = 3 * abs(t) + W / 2
+    a = (2 * emax) / (2 - 0.01)
+    return H, a
+
+
+def cpu_baseline(H, R, a, seconds):
+    """The oracle (C restatement of the reference's CPU recursion, single
+    thread) timed on a bounded number of steps of the same workload."""
+    import oracle
+    from oracle import lattice_oracle as O
+
+    Hs = O.rescale(O.CSC(H.n, H.colptr, H.rowidx, H.values), a, 0.0)
+    kets = np.full((H.n, R), 1.0 / math.sqrt(H.n), dtype=np.complex128)
+    # one timed probe step decides how many fit the time budget
+    t0 = time.perf_counter()
+    oracle.cheb_braket(Hs, 3, kets, kets)  # T1 + one recursion step
+    probe = time.perf_counter() - t0
+    steps = max(1, min(64, int(seconds / max(probe, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    oracle.cheb_braket(Hs, steps + 2, kets, kets)
+    dt = time.perf_counter() - t0
+    total_steps = steps + 1  # T1 and `steps` recursion steps
+    return {
+        "value": total_steps / dt,
+        "unit": "steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "%d Chebyshev steps (T1 + %d recursion steps) of the same 2,097,152-site x 64-vector "
+                  "fp64 workload, oracle/cheb_oracle.c (restatement of numerical.h:363-381, single "
+                  "thread), %.1f s" % (total_steps, steps, dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1024)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--R", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extra", action="store_true", help="also time fp32 and one-moment-per-step variants")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(sharding over R vectors is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from tight_binding_amd import kpm
+
+    t0 = time.perf_counter()
+    H, a = build_c2(args.cells)
+    log("assembled H: n=%d nnz=%d a=%.6f in %.1f s" % (H.n, H.nnz, a, time.perf_counter() - t0))
+
+    ctx = kpm.Context(0)
+    M = ctx.matrix(H)
+    K = ctx.random_block(H.n, args.R, seed=7)
+    ctx.synchronize()
+    log("resident: %.2f GB on device" % (ctx.memory() / 1e9))
+
+    # warmup
+    ctx.moments(M, K, 2 * args.warmup, a=a)
+    ctx.synchronize()
+
+    # timed: exactly K steps (2K moments, doubling path)
+    n_mom = 2 * args.steps
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    mu, st, _ = ctx.moments(M, K, n_mom, a=a, time_launches=True)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    assert st["steps"] == args.steps
+    per_launch_ms = st["kernel_ms"] / st["steps"]
+    achieved = st["bytes_total"] / (st["kernel_ms"] * 1e-3) / 1e9
+    log("fp64: %d steps, wall %.3f s, kernel %.3f ms/step, %.1f GB/s (%.1f%% of 8 TB/s), mu0=%.6f"
+        % (st["steps"], wall, per_launch_ms, achieved, 100 * achieved / HBM_PEAK_GBS, mu[0]))
+    if not (abs(mu[0] - args.R) < 1e-8 and np.all(np.abs(mu) <= mu[0] * (1 + 1e-9))):
+        raise SystemExit("bench: moments fail the mu_0 = R, |mu_n| <= mu_0 check")
+
+    extra = {}
+    if args.extra:
+        mu_p, stp, _ = ctx.moments(M, K, args.steps + 1, a=a, plain=True, time_launches=True)
+        extra["plain_fp64"] = {
+            "steps_per_s": stp["steps"] / (stp["kernel_ms"] * 1e-3),
+            "GBps": stp["bytes_total"] / (stp["kernel_ms"] * 1e-3) / 1e9,
+            "bytes_per_step": stp["bytes_per_step"],
+        }
+        K32 = ctx.random_block(H.n, args.R, seed=7, precision=kpm.FP32)
+        mu32, st32, _ = ctx.moments(M, K32, n_mom, a=a, time_launches=True)
+        extra["fp32_recursion"] = {
+            "steps_per_s": st32["steps"] / (st32["kernel_ms"] * 1e-3),
+            "GBps": st32["bytes_total"] / (st32["kernel_ms"] * 1e-3) / 1e9,
+            "max_abs_dmu_over_max_mu_vs_fp64": float(np.max(np.abs(mu32 - mu)) / np.max(np.abs(mu))),
+        }
+        K32.close()
+        log("extra: %s" % json.dumps(extra))
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle, 1 thread, ~%.0f s)" % args.cpu_seconds)
+        cpu = cpu_baseline(H, args.R, a, args.cpu_seconds)
+        log("cpu: %.4f steps/s" % cpu["value"])
+
+    value = st["steps"] / (st["kernel_ms"] * 1e-3)
+    out = {
+        "metric": "Chebyshev SpMM steps/s + %HBM roofline; sigma_xy solve p50/p99 latency",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": per_launch_ms,
+        "wall_ms_per_step": 1e3 * wall / st["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded honeycomb lattice, site-keyed disorder, device-generated random-phase vectors",
+        "config": {
+            "workload": "KPM DoS moments, honeycomb %dx%d cells periodic (%d sites), t=-2.7 eV, W=0.5 eV, "
+                        "R=%d, N=%d moments, fp64" % (args.cells, args.cells, H.n, args.R, n_mom),
+            "sites": H.n, "nnz": H.nnz, "R": args.R, "moments": n_mom, "moments_per_step": 2,
+            "parallelism": "single GPU",
+        },
+        "moments_per_s": n_mom / (st["kernel_ms"] * 1e-3),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_step": st["bytes_per_step"],
+        },
+        "cpu_baseline": cpu,
+    }
+    if extra:
+        out["variants"] = extra
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

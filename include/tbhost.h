@@ -1,0 +1,75 @@
+/*
+ * tbhost.h - C ABI of the host-side model layer (UnitCell / PartialLattice
+ * and the sparse Hamiltonian builders), for bindings from other languages.
+ *
+ * Each entry wraps the C++ API of the headers in tight_binding_amd/host, which restates
+ * the reference's unit_cell.h / lattice.h.  Return value: 0 or a negative
+ * TBK_E_* code (tbkpm.h); tbh_last_error() gives the message.
+ */
+#ifndef TBHOST_H
+#define TBHOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tbh_cell tbh_cell;
+typedef struct tbh_lattice tbh_lattice;
+typedef struct tbh_csc tbh_csc;
+
+const char* tbh_last_error(void);
+
+/* UnitCell<Dim> ctor (unit_cell.h:188-210): dim lattice vectors (3 doubles
+   each) and n_atoms named atoms with 3-vector positions */
+int tbh_cell_create(int32_t dim, const double* lattice_vectors, int32_t n_atoms, const char* const* names,
+                    const double* positions, tbh_cell** out);
+/* presets of models.h:6-150 by name: "honeycomb", "lieb", "kagome", "square",
+   "chain", "fcc", "cubic", "graphene", "ti_sp" */
+int tbh_cell_model(const char* name, tbh_cell** out);
+int tbh_cell_destroy(tbh_cell* c);
+int tbh_cell_dim(const tbh_cell* c, int32_t* dim);
+/* add_hopping(from, to, lattice_vector, t) (unit_cell.h:212-228) */
+int tbh_cell_add_hopping(tbh_cell* c, const char* from, const char* to, const int32_t* lattice_vector,
+                         double t_re, double t_im);
+/* add_hopping(from, k_NN, t) (unit_cell.h:230-256) */
+int tbh_cell_add_hopping_nn(tbh_cell* c, const char* from, uint64_t k, double t_re, double t_im);
+int tbh_cell_hopping_count(const tbh_cell* c, int32_t* n);
+/* stored hoppings in table order: from[n], to[n], lattice_vector[n*dim], t[2n] */
+int tbh_cell_hoppings(const tbh_cell* c, int32_t* from, int32_t* to, int32_t* lattice_vector, double* t);
+
+/* PartialLattice<Dim> ctor (lattice.h:106-117) */
+int tbh_lattice_create(const tbh_cell* c, const int32_t* span, tbh_lattice** out);
+int tbh_lattice_destroy(tbh_lattice* l);
+int tbh_lattice_total_number(const tbh_lattice* l, int32_t* n);
+/* gauge_field: 0 none, 1 Landau A = (0, B x, 0), 2 symmetric A = B(-y, x, 0)/2 */
+int tbh_lattice_set_gauge(tbh_lattice* l, int32_t kind, double B);
+/* site-keyed on-site disorder V = W (u(seed, site) - 1/2) */
+int tbh_lattice_set_disorder(tbh_lattice* l, double W, uint64_t seed);
+/* remove atom `atom` from the listed cells (n cells, dim ints each) */
+int tbh_lattice_remove_atoms(tbh_lattice* l, const char* atom, const int32_t* cells, int32_t n);
+/* builder: 0 get_Hamiltonian_sparse (open, lattice.h:262-305),
+            1 get_Hamiltonian_periodic_extend_bare_sparse (lattice.h:216-258),
+            2 get_Hamiltonian_bare_sparse (lattice.h:347-377) */
+int tbh_lattice_hamiltonian(tbh_lattice* l, int32_t builder, const int32_t* extend, int32_t is_complex,
+                            tbh_csc** out);
+int tbh_lattice_op_position(tbh_lattice* l, int32_t axis, tbh_csc** out);
+int tbh_lattice_op_velocity(tbh_lattice* l, int32_t axis, int32_t is_complex, tbh_csc** out);
+
+/* H~ = (H - b I)/a (lattice.h:550-559) */
+int tbh_csc_rescale(const tbh_csc* m, double a, double b, tbh_csc** out);
+/* [X, M] for a diagonal X: x_i M_ij - M_ij x_j (snippets.h:316-317) */
+int tbh_csc_commutator_diag(const tbh_csc* x, const tbh_csc* m, tbh_csc** out);
+int tbh_csc_info(const tbh_csc* m, int32_t* n, int64_t* nnz, int32_t* is_complex);
+/* column pointers [n+1], row indices [nnz], values [nnz or 2*nnz] */
+int tbh_csc_copy(const tbh_csc* m, int32_t* colptr, int32_t* rowidx, double* values);
+/* the same matrix row-compressed: row pointers, column indices, values */
+int tbh_csc_copy_csr(const tbh_csc* m, int32_t* rowptr, int32_t* colidx, double* values);
+int tbh_csc_destroy(tbh_csc* m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TBHOST_H */

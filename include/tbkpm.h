@@ -1,0 +1,143 @@
+/*
+ * tbkpm.h - C ABI of the MI355X (gfx950) kernel-polynomial engine.
+ *
+ * Plain pointers and sizes only.  Every function returns TBK_OK (0) or a
+ * negative TBK_E_* code; tbk_last_error() returns the message of the last
+ * failure on the calling thread.  No C++ exception crosses this boundary.
+ *
+ * Dense vector blocks cross the boundary as the reference passes them:
+ * column-major rows x R, complex128 interleaved (re, im) -- the layout of
+ * Eigen::MatrixXcd bras_T / kets in numerics_cuda.cuh:10-13.  On the device a
+ * block is stored row-major [site][Rp] (Rp = R rounded up to a power of two,
+ * padding columns zero) so one wavefront reads a site's whole row of vectors
+ * with one coalesced access.
+ *
+ * Sparse matrices cross as compressed arrays with int32 indices, either
+ * column-compressed (CSC: Eigen's default ColMajor storage that the reference
+ * hands to cuSPARSE, numerics_cuda.cu:52-64) or row-compressed (CSR).
+ *
+ * Threading: a tbk_context owns one HIP stream on one device.  Calls on one
+ * context must not overlap (callers serialise them); distinct contexts may be
+ * used from distinct threads.
+ */
+#ifndef TBKPM_H
+#define TBKPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TBK_OK = 0,
+    TBK_E_INVALID = -1,     /* bad argument: n < 2, R < 1, null pointer, shape mismatch */
+    TBK_E_DEVICE = -2,      /* HIP runtime or launch failure, or no device */
+    TBK_E_OOM = -3,         /* device allocation failed */
+    TBK_E_UNSUPPORTED = -4  /* valid request outside what this build supports */
+};
+
+/* precision of a vector block and of the recursion run on it */
+enum {
+    TBK_FP64 = 0, /* complex128 vectors, fp64 matrix values, fp64 moments */
+    TBK_FP32 = 1  /* complex64 vectors, fp32 matrix values, moments summed in fp64 */
+};
+
+enum { TBK_CSC = 0, TBK_CSR = 1 };
+
+/* flags for tbk_moments */
+enum {
+    TBK_FLAG_PLAIN = 1,        /* bras == kets: still run one step per moment (no doubling) */
+    TBK_FLAG_TIME_LAUNCHES = 2 /* time every step launch with its own HIP event pair */
+};
+
+typedef struct tbk_sparse {
+    int32_t rows, cols;
+    int64_t nnz;
+    const int32_t* outer; /* CSC: column pointers (cols+1); CSR: row pointers (rows+1) */
+    const int32_t* inner; /* CSC: row indices; CSR: column indices */
+    const double* values; /* nnz doubles, or 2*nnz interleaved (re, im) if is_complex */
+    int32_t is_complex;
+    int32_t layout; /* TBK_CSC or TBK_CSR */
+} tbk_sparse;
+
+typedef struct tbk_stats {
+    double kernel_ms;       /* device time of the recursion step launches (HIP events) */
+    double wall_ms;         /* host wall time of the whole call */
+    int64_t steps;          /* step launches (applications of H~ to the block) */
+    int64_t moments;        /* moments produced */
+    double bytes_per_step;  /* algorithmic HBM bytes of one recursion step launch */
+    double bytes_total;     /* algorithmic HBM bytes of all step launches */
+    int32_t doubling;       /* 1 if the Chebyshev product identity produced two moments/step */
+    int32_t blocks;         /* workgroups per step launch */
+    int32_t rows_per_tile;  /* rows one workgroup handles per tile */
+    int32_t padded_R;       /* vector columns stored per site on the device */
+} tbk_stats;
+
+const char* tbk_version(void);
+const char* tbk_last_error(void);
+int tbk_device_count(int32_t* count);
+
+/*
+ * Drop-in for cuda::Cheb_first_kind_seq_braket (numerics_cuda.cuh:10-13,
+ * numerics_cuda.cu:190-323; CPU twin numerical.h:363-381): both overloads
+ * (real and complex M) in one entry selected by M->is_complex.
+ *   mu[i] = sum over sites and columns of Re(conj(bras_T) * T_i(M) kets),
+ *   T_0 = kets, T_1 = M kets, T_i = 2 M T_{i-1} - T_{i-2}, i < n.
+ * Not divided by the number of columns.  M is used as given (already
+ * rescaled).  n < 2 fails with TBK_E_INVALID, as numerics_cuda.cu:194-197.
+ * bras_T and kets: ket_rows x ket_cols column-major complex128.  bras_T may
+ * equal kets (the same pointer), which selects the two-moments-per-step path.
+ * Uses a process-wide context on device 0.
+ */
+int tbk_cheb_first_kind_seq_braket(const tbk_sparse* M, int32_t n, const double* bras_T,
+                                   const double* kets, int32_t ket_rows, int32_t ket_cols,
+                                   double* mu_out);
+
+/* ------------------------------------------------ device-resident objects */
+typedef struct tbk_context tbk_context;
+typedef struct tbk_matrix tbk_matrix;
+typedef struct tbk_block tbk_block;
+
+int tbk_context_create(int32_t device, tbk_context** out);
+int tbk_context_destroy(tbk_context* ctx);
+int tbk_context_synchronize(tbk_context* ctx);
+/* bytes of device memory the context currently holds (matrices, blocks, work) */
+int tbk_context_memory(tbk_context* ctx, int64_t* bytes);
+
+/* upload a square sparse matrix; CSC input is reordered to CSR exactly */
+int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out);
+int tbk_matrix_destroy(tbk_matrix* m);
+
+/* upload rows x R column-major complex128 (NULL: zeros) as a block */
+int tbk_block_create(tbk_context* ctx, const double* colmajor, int32_t rows, int32_t R,
+                     int32_t precision, tbk_block** out);
+/* random-phase block e^{i theta}/sqrt(rows), theta = 2 pi u(seed, site*R + r),
+   generated on the device */
+int tbk_block_random_phase(tbk_context* ctx, int32_t rows, int32_t R, uint64_t seed,
+                           int32_t precision, tbk_block** out);
+/* copy a block back as rows x R column-major complex128 */
+int tbk_block_download(const tbk_block* b, double* colmajor_out);
+int tbk_block_destroy(tbk_block* b);
+
+/*
+ * Chebyshev moments of H~ = (M - b I)/a, rescaling fused into the step
+ * kernel (a = 1, b = 0 uses M as given):
+ *   mu[i] = Re <bras| T_i(H~) |kets>, i = 0 .. n-1.
+ * bras == NULL means bras = kets; then, unless TBK_FLAG_PLAIN, moments come
+ * two per step from T_{m+k} = 2 T_m T_k - T_{|m-k|}:
+ *   mu_{2k} = 2 <T_k|T_k> - mu_0,  mu_{2k+1} = 2 <T_{k+1}|T_k> - mu_1.
+ * The recursion runs in the precision of `kets` (bras must match).
+ * last_out (optional, NULL to skip): the last vector block T_{steps} written by
+ * the recursion, rows x R column-major complex128.
+ */
+int tbk_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b,
+                const tbk_block* bras, const tbk_block* kets, int32_t n, int32_t flags,
+                double* mu_out, double* last_out, tbk_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TBKPM_H */

@@ -1,0 +1,173 @@
+"""ctypes bindings of the package's native libraries.
+
+libtbkpm.so (HIP engine, include/tbkpm.h) and libtbhost.so (host model
+layer, include/tbhost.h) are loaded from this directory.  There is no
+fallback: if a library is missing the import raises, so a caller can never
+silently run something other than the native code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+TBK_OK = 0
+TBK_E_INVALID = -1
+TBK_E_DEVICE = -2
+TBK_E_OOM = -3
+TBK_E_UNSUPPORTED = -4
+TBK_FP64 = 0
+TBK_FP32 = 1
+TBK_CSC = 0
+TBK_CSR = 1
+TBK_FLAG_PLAIN = 1
+TBK_FLAG_TIME_LAUNCHES = 2
+
+
+class NativeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class tbk_sparse(C.Structure):
+    _fields_ = [
+        ("rows", C.c_int32),
+        ("cols", C.c_int32),
+        ("nnz", C.c_int64),
+        ("outer", C.POINTER(C.c_int32)),
+        ("inner", C.POINTER(C.c_int32)),
+        ("values", C.POINTER(C.c_double)),
+        ("is_complex", C.c_int32),
+        ("layout", C.c_int32),
+    ]
+
+
+class tbk_stats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("steps", C.c_int64),
+        ("moments", C.c_int64),
+        ("bytes_per_step", C.c_double),
+        ("bytes_total", C.c_double),
+        ("doubling", C.c_int32),
+        ("blocks", C.c_int32),
+        ("rows_per_tile", C.c_int32),
+        ("padded_R", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _load(name):
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise ImportError(
+            "native library %s is not built; run __graft_entry__.build() or "
+            "`make -C tight_binding_amd`" % path
+        )
+    return C.CDLL(path)
+
+
+_kpm = None
+_host = None
+
+P = C.c_void_p
+PP = C.POINTER(C.c_void_p)
+I32 = C.c_int32
+I64 = C.c_int64
+U64 = C.c_uint64
+D = C.c_double
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int32)
+
+KPM_SIGNATURES = {
+    "tbk_version": ([], C.c_char_p),
+    "tbk_last_error": ([], C.c_char_p),
+    "tbk_device_count": ([IP], I32),
+    "tbk_cheb_first_kind_seq_braket": ([C.POINTER(tbk_sparse), I32, DP, DP, I32, I32, DP], I32),
+    "tbk_context_create": ([I32, PP], I32),
+    "tbk_context_destroy": ([P], I32),
+    "tbk_context_synchronize": ([P], I32),
+    "tbk_context_memory": ([P, C.POINTER(I64)], I32),
+    "tbk_matrix_create": ([P, C.POINTER(tbk_sparse), PP], I32),
+    "tbk_matrix_destroy": ([P], I32),
+    "tbk_block_create": ([P, DP, I32, I32, I32, PP], I32),
+    "tbk_block_random_phase": ([P, I32, I32, U64, I32, PP], I32),
+    "tbk_block_download": ([P, DP], I32),
+    "tbk_block_destroy": ([P], I32),
+    "tbk_moments": ([P, P, D, D, P, P, I32, I32, DP, DP, C.POINTER(tbk_stats)], I32),
+}
+
+HOST_SIGNATURES = {
+    "tbh_last_error": ([], C.c_char_p),
+    "tbh_cell_create": ([I32, DP, I32, C.POINTER(C.c_char_p), DP, PP], I32),
+    "tbh_cell_model": ([C.c_char_p, PP], I32),
+    "tbh_cell_destroy": ([P], I32),
+    "tbh_cell_dim": ([P, IP], I32),
+    "tbh_cell_add_hopping": ([P, C.c_char_p, C.c_char_p, IP, D, D], I32),
+    "tbh_cell_add_hopping_nn": ([P, C.c_char_p, U64, D, D], I32),
+    "tbh_cell_hopping_count": ([P, IP], I32),
+    "tbh_cell_hoppings": ([P, IP, IP, IP, DP], I32),
+    "tbh_lattice_create": ([P, IP, PP], I32),
+    "tbh_lattice_destroy": ([P], I32),
+    "tbh_lattice_total_number": ([P, IP], I32),
+    "tbh_lattice_set_gauge": ([P, I32, D], I32),
+    "tbh_lattice_set_disorder": ([P, D, U64], I32),
+    "tbh_lattice_remove_atoms": ([P, C.c_char_p, IP, I32], I32),
+    "tbh_lattice_hamiltonian": ([P, I32, IP, I32, PP], I32),
+    "tbh_lattice_op_position": ([P, I32, PP], I32),
+    "tbh_lattice_op_velocity": ([P, I32, I32, PP], I32),
+    "tbh_csc_rescale": ([P, D, D, PP], I32),
+    "tbh_csc_commutator_diag": ([P, P, PP], I32),
+    "tbh_csc_info": ([P, IP, C.POINTER(I64), IP], I32),
+    "tbh_csc_copy": ([P, IP, IP, DP], I32),
+    "tbh_csc_copy_csr": ([P, IP, IP, DP], I32),
+    "tbh_csc_destroy": ([P], I32),
+}
+
+
+def _bind(lib, sigs):
+    for name, (args, res) in sigs.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    return lib
+
+
+def kpm():
+    global _kpm
+    if _kpm is None:
+        _kpm = _bind(_load("libtbkpm.so"), KPM_SIGNATURES)
+    return _kpm
+
+
+def host():
+    global _host
+    if _host is None:
+        _host = _bind(_load("libtbhost.so"), HOST_SIGNATURES)
+    return _host
+
+
+def check_kpm(rc):
+    if rc != TBK_OK:
+        raise NativeError(rc, kpm().tbk_last_error().decode())
+    return rc
+
+
+def check_host(rc):
+    if rc != TBK_OK:
+        raise NativeError(rc, host().tbh_last_error().decode())
+    return rc
+
+
+def loaded_paths():
+    """Paths of the native libraries this process has loaded."""
+    out = []
+    for lib in (_kpm, _host):
+        if lib is not None:
+            out.append(lib._name)
+    return out

@@ -1,0 +1,690 @@
+// Host side of the MI355X kernel-polynomial engine: contexts, resident
+// matrices and vector blocks, the moment driver, and the C ABI of tbkpm.h.
+//
+// The drop-in entry tbk_cheb_first_kind_seq_braket replaces the two
+// overloads of cuda::Cheb_first_kind_seq_braket (numerics_cuda.cu:190-323):
+// same inputs (rescaled CSC matrix, bras_T, kets, n) and output (n moments,
+// not divided by R), with the per-step host synchronisation, per-step
+// printing and cuSPARSE/cuBLAS calls of the reference replaced by one fused
+// HIP kernel per step whose moment partials stay on the device until the end.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cheb_kernels.h"
+#include "tbkpm.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void raise(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        raise(TBK_E_OOM, std::string(what) + ": " + hipGetErrorString(e));
+    raise(TBK_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(expr) hip_check((expr), #expr)
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return TBK_OK;
+    } catch (const Error& e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return TBK_E_OOM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return TBK_E_INVALID;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return TBK_E_INVALID;
+    }
+}
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        release();
+        if (n == 0) return;
+        HIPC(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+int pow2_at_least(int v) {
+    int r = 1;
+    while (r < v) r <<= 1;
+    return r;
+}
+
+}  // namespace
+
+struct tbk_context {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    DevBuf work[2];
+    DevBuf partial;
+    DevBuf sums;
+    std::vector<hipEvent_t> events;
+    int64_t held_bytes = 0;  // matrices + blocks owned through this context
+};
+
+struct tbk_matrix {
+    tbk_context* ctx = nullptr;
+    int32_t rows = 0;
+    int64_t nnz = 0;
+    bool is_complex = false;
+    DevBuf rp, ci, val64, val32;
+    bool have32 = false;
+};
+
+struct tbk_block {
+    tbk_context* ctx = nullptr;
+    int32_t rows = 0, R = 0, Rp = 0, precision = TBK_FP64;
+    DevBuf data;
+    size_t elem_bytes() const { return precision == TBK_FP64 ? 16 : 8; }
+};
+
+namespace {
+
+void set_device(const tbk_context* ctx) { HIPC(hipSetDevice(ctx->device)); }
+
+void ensure_events(tbk_context* ctx, size_t n) {
+    while (ctx->events.size() < n) {
+        hipEvent_t e;
+        HIPC(hipEventCreate(&e));
+        ctx->events.push_back(e);
+    }
+}
+
+// --------------------------------------------------------- kernel dispatch
+using StepFn = void (*)(dim3, dim3, hipStream_t, const void* const* args);
+
+struct StepArgs {
+    const int32_t* rp;
+    const int32_t* ci;
+    const void* hv;
+    const void* x;
+    const void* z;
+    void* y;
+    const void* bra;
+    int32_t rows, Rp;
+    double alpha, beta;
+    int32_t shift;
+    int64_t ntiles;
+    int32_t xcd_split;
+    double2* partial;
+};
+
+template <typename VT, bool CH, int LPR, int CPL, int MODE>
+void launch_step(dim3 grid, hipStream_t st, const StepArgs& a) {
+    using V2 = typename tbk::Vec2<VT>::type;
+    using H = typename tbk::HVal<VT, CH>::type;
+    hipLaunchKernelGGL((tbk::cheb_step_kernel<VT, CH, LPR, CPL, MODE>), grid, dim3(tbk::kBlock), 0,
+                       st, a.rp, a.ci, static_cast<const H*>(a.hv), static_cast<const V2*>(a.x),
+                       static_cast<const V2*>(a.z), static_cast<V2*>(a.y),
+                       static_cast<const V2*>(a.bra), a.rows, a.Rp, static_cast<VT>(a.alpha),
+                       static_cast<VT>(a.beta), a.shift, a.ntiles, a.xcd_split, a.partial);
+}
+
+template <typename VT, bool CH, int LPR, int CPL>
+void launch_mode(int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    switch (mode) {
+        case tbk::kFirst: launch_step<VT, CH, LPR, CPL, tbk::kFirst>(grid, st, a); break;
+        case tbk::kRec: launch_step<VT, CH, LPR, CPL, tbk::kRec>(grid, st, a); break;
+        default: launch_step<VT, CH, LPR, CPL, tbk::kRecDbl>(grid, st, a); break;
+    }
+}
+
+template <typename VT, bool CH>
+void launch_rp(int Rp, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    switch (Rp) {
+        case 1: launch_mode<VT, CH, 1, 1>(mode, grid, st, a); break;
+        case 2: launch_mode<VT, CH, 2, 1>(mode, grid, st, a); break;
+        case 4: launch_mode<VT, CH, 4, 1>(mode, grid, st, a); break;
+        case 8: launch_mode<VT, CH, 8, 1>(mode, grid, st, a); break;
+        case 16: launch_mode<VT, CH, 16, 1>(mode, grid, st, a); break;
+        case 32: launch_mode<VT, CH, 32, 1>(mode, grid, st, a); break;
+        case 64: launch_mode<VT, CH, 64, 1>(mode, grid, st, a); break;
+        case 128: launch_mode<VT, CH, 64, 2>(mode, grid, st, a); break;
+        case 256: launch_mode<VT, CH, 64, 4>(mode, grid, st, a); break;
+        default: raise(TBK_E_UNSUPPORTED, "vector block wider than 256 columns");
+    }
+}
+
+void launch_any(int precision, bool cplx, int Rp, int mode, dim3 grid, hipStream_t st,
+                const StepArgs& a) {
+    if (precision == TBK_FP64) {
+        if (cplx)
+            launch_rp<double, true>(Rp, mode, grid, st, a);
+        else
+            launch_rp<double, false>(Rp, mode, grid, st, a);
+    } else {
+        if (cplx)
+            launch_rp<float, true>(Rp, mode, grid, st, a);
+        else
+            launch_rp<float, false>(Rp, mode, grid, st, a);
+    }
+}
+
+int rows_per_tile(int Rp) {
+    const int lpr = std::min(Rp, tbk::kWave);
+    return tbk::kWavesPerBlock * (tbk::kWave / lpr);
+}
+
+// ---------------------------------------------------------- matrix upload
+void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_t>& ci,
+               std::vector<double>& val) {
+    const int vw = M->is_complex ? 2 : 1;
+    const int64_t nnz = M->nnz;
+    if (M->layout == TBK_CSR) {
+        rp.assign(M->outer, M->outer + M->rows + 1);
+        ci.assign(M->inner, M->inner + nnz);
+        val.assign(M->values, M->values + nnz * vw);
+        for (int32_t i = 0; i < M->rows; ++i)
+            if (rp[i + 1] < rp[i]) raise(TBK_E_INVALID, "row pointers not monotone");
+        if (rp[0] != 0 || rp[M->rows] != nnz) raise(TBK_E_INVALID, "row pointers inconsistent with nnz");
+        for (int64_t p = 0; p < nnz; ++p)
+            if (ci[p] < 0 || ci[p] >= M->cols) raise(TBK_E_INVALID, "column index out of range");
+        return;
+    }
+    // CSC -> CSR: exact stable reordering (columns visited ascending)
+    const int32_t* cp = M->outer;
+    const int32_t* ri = M->inner;
+    if (cp[0] != 0 || cp[M->cols] != nnz) raise(TBK_E_INVALID, "column pointers inconsistent with nnz");
+    rp.assign(static_cast<size_t>(M->rows) + 1, 0);
+    for (int32_t j = 0; j < M->cols; ++j) {
+        if (cp[j + 1] < cp[j]) raise(TBK_E_INVALID, "column pointers not monotone");
+    }
+    for (int64_t p = 0; p < nnz; ++p) {
+        if (ri[p] < 0 || ri[p] >= M->rows) raise(TBK_E_INVALID, "row index out of range");
+        rp[ri[p] + 1]++;
+    }
+    for (int32_t i = 0; i < M->rows; ++i) rp[i + 1] += rp[i];
+    ci.resize(nnz);
+    val.resize(static_cast<size_t>(nnz) * vw);
+    std::vector<int32_t> fill(rp.begin(), rp.end() - 1);
+    for (int32_t j = 0; j < M->cols; ++j)
+        for (int32_t p = cp[j]; p < cp[j + 1]; ++p) {
+            const int32_t q = fill[ri[p]]++;
+            ci[q] = j;
+            for (int k = 0; k < vw; ++k) val[static_cast<size_t>(q) * vw + k] = M->values[static_cast<size_t>(p) * vw + k];
+        }
+}
+
+void matrix_fp32(tbk_matrix* m) {
+    if (m->have32) return;
+    const int64_t n = m->nnz * (m->is_complex ? 2 : 1);
+    m->val32.reserve(std::max<size_t>(static_cast<size_t>(n) * 4, 4));
+    if (n > 0) {
+        const int grid = static_cast<int>(std::min<int64_t>((n + tbk::kBlock - 1) / tbk::kBlock, 65536));
+        hipLaunchKernelGGL(tbk::to_fp32_kernel, dim3(grid), dim3(tbk::kBlock), 0, m->ctx->stream,
+                           m->val64.as<const double>(), m->val32.as<float>(), n);
+        HIPC(hipGetLastError());
+    }
+    m->ctx->held_bytes += static_cast<int64_t>(m->val32.bytes);
+    m->have32 = true;
+}
+
+// ------------------------------------------------------------ the driver
+void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* bras,
+                 const tbk_block* kets, int32_t n, int32_t flags, double* mu_out, double* last_out,
+                 tbk_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !M || !kets || !mu_out) raise(TBK_E_INVALID, "null argument");
+    if (n < 2) raise(TBK_E_INVALID, "n must be greater than or equal to 2");
+    if (!(a != 0.0) || !std::isfinite(a) || !std::isfinite(b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
+    if (M->ctx != ctx || kets->ctx != ctx || (bras && bras->ctx != ctx))
+        raise(TBK_E_INVALID, "objects belong to another context");
+    if (kets->rows != M->rows) raise(TBK_E_INVALID, "kets rows do not match the matrix");
+    if (bras && (bras->rows != kets->rows || bras->R != kets->R || bras->precision != kets->precision))
+        raise(TBK_E_INVALID, "bras shape or precision does not match kets");
+    set_device(ctx);
+
+    const int prec = kets->precision;
+    if (prec == TBK_FP32) matrix_fp32(const_cast<tbk_matrix*>(M));
+    const void* hv = prec == TBK_FP64 ? M->val64.p : M->val32.p;
+    const int Rp = kets->Rp;
+    const int rows = kets->rows;
+    const size_t block_bytes = static_cast<size_t>(rows) * Rp * kets->elem_bytes();
+    ctx->work[0].reserve(block_bytes);
+    ctx->work[1].reserve(block_bytes);
+
+    const bool doubling = (bras == nullptr || bras == kets) && !(flags & TBK_FLAG_PLAIN);
+    const tbk_block* bra = bras ? bras : kets;
+    const int64_t steps = doubling ? (n + 1) / 2 : (n - 1);
+
+    const int TR = rows_per_tile(Rp);
+    const int64_t ntiles = (static_cast<int64_t>(rows) + TR - 1) / TR;
+    int64_t cap = static_cast<int64_t>(ctx->num_cu) * 8;
+    int64_t nb = std::min<int64_t>(ntiles, cap);
+    int xcd_split = 0;
+    if (nb >= 64) {
+        nb = (nb / 8) * 8;
+        xcd_split = 1;
+    }
+    nb = std::max<int64_t>(nb, 1);
+    ctx->partial.reserve(static_cast<size_t>(steps) * nb * sizeof(double2));
+    ctx->sums.reserve(static_cast<size_t>(steps) * sizeof(double2));
+
+    const bool time_each = (flags & TBK_FLAG_TIME_LAUNCHES) != 0;
+    ensure_events(ctx, time_each ? static_cast<size_t>(2 * steps) : 2);
+
+    StepArgs A{};
+    A.rp = M->rp.as<const int32_t>();
+    A.ci = M->ci.as<const int32_t>();
+    A.hv = hv;
+    A.rows = rows;
+    A.Rp = Rp;
+    A.shift = (b != 0.0) ? 1 : 0;
+    A.ntiles = ntiles;
+    A.xcd_split = xcd_split;
+    double2* partial = ctx->partial.as<double2>();
+
+    auto buf = [&](int64_t k) -> void* {  // storage of T_k
+        if (k == 0) return kets->data.p;
+        return ctx->work[(k & 1) ? 1 : 0].p;
+    };
+    const dim3 grid(static_cast<unsigned>(nb));
+    if (!time_each) HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+    for (int64_t k = 0; k < steps; ++k) {
+        int mode;
+        if (k == 0) {
+            mode = tbk::kFirst;
+            A.alpha = 1.0 / a;
+            A.beta = -b / a;
+            A.x = buf(0);
+            A.z = nullptr;
+            A.y = buf(1);
+            A.bra = bra->data.p;
+        } else {
+            mode = doubling ? tbk::kRecDbl : tbk::kRec;
+            A.alpha = 2.0 / a;
+            A.beta = -2.0 * b / a;
+            A.x = buf(k);
+            A.z = buf(k - 1);
+            A.y = buf(k + 1);
+            A.bra = bra->data.p;
+        }
+        A.partial = partial + static_cast<size_t>(k) * nb;
+        if (time_each) HIPC(hipEventRecord(ctx->events[2 * k], ctx->stream));
+        launch_any(prec, M->is_complex, Rp, mode, grid, ctx->stream, A);
+        HIPC(hipGetLastError());
+        if (time_each) HIPC(hipEventRecord(ctx->events[2 * k + 1], ctx->stream));
+    }
+    if (!time_each) HIPC(hipEventRecord(ctx->events[1], ctx->stream));
+    hipLaunchKernelGGL(tbk::reduce_partials_kernel, dim3(static_cast<unsigned>(steps)), dim3(tbk::kBlock), 0,
+                       ctx->stream, static_cast<const double2*>(partial), static_cast<int32_t>(nb),
+                       ctx->sums.as<double2>());
+    HIPC(hipGetLastError());
+    std::vector<double2> sums(static_cast<size_t>(steps));
+    HIPC(hipMemcpyAsync(sums.data(), ctx->sums.p, sums.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                        ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+
+    // moments
+    const double mu0 = sums[0].x, mu1 = sums[0].y;
+    mu_out[0] = mu0;
+    mu_out[1] = mu1;
+    if (doubling) {
+        for (int64_t k = 1; k < steps; ++k) {
+            if (2 * k < n) mu_out[2 * k] = 2.0 * sums[k].x - mu0;
+            if (2 * k + 1 < n) mu_out[2 * k + 1] = 2.0 * sums[k].y - mu1;
+        }
+    } else {
+        for (int64_t k = 1; k < steps; ++k) mu_out[k + 1] = sums[k].x;
+    }
+
+    if (last_out) {
+        // download T_steps and transpose to column-major complex128
+        const void* src = buf(steps);
+        const int R = kets->R;
+        if (prec == TBK_FP64) {
+            std::vector<double2> h(static_cast<size_t>(rows) * Rp);
+            HIPC(hipMemcpy(h.data(), src, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+            for (int r = 0; r < R; ++r)
+                for (int i = 0; i < rows; ++i) {
+                    last_out[2 * (static_cast<size_t>(r) * rows + i)] = h[static_cast<size_t>(i) * Rp + r].x;
+                    last_out[2 * (static_cast<size_t>(r) * rows + i) + 1] = h[static_cast<size_t>(i) * Rp + r].y;
+                }
+        } else {
+            std::vector<float2> h(static_cast<size_t>(rows) * Rp);
+            HIPC(hipMemcpy(h.data(), src, h.size() * sizeof(float2), hipMemcpyDeviceToHost));
+            for (int r = 0; r < R; ++r)
+                for (int i = 0; i < rows; ++i) {
+                    last_out[2 * (static_cast<size_t>(r) * rows + i)] = h[static_cast<size_t>(i) * Rp + r].x;
+                    last_out[2 * (static_cast<size_t>(r) * rows + i) + 1] = h[static_cast<size_t>(i) * Rp + r].y;
+                }
+        }
+    }
+
+    if (stats) {
+        float ms = 0.f;
+        double kernel_ms = 0.0;
+        if (time_each) {
+            for (int64_t k = 0; k < steps; ++k) {
+                HIPC(hipEventElapsedTime(&ms, ctx->events[2 * k], ctx->events[2 * k + 1]));
+                kernel_ms += ms;
+            }
+        } else {
+            HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+            kernel_ms = ms;
+        }
+        const double sval = (prec == TBK_FP64 ? 8.0 : 4.0) * (M->is_complex ? 2.0 : 1.0);
+        const double svec = static_cast<double>(kets->elem_bytes());
+        const double mat = static_cast<double>(M->nnz) * (sval + 4.0) + (static_cast<double>(rows) + 1.0) * 4.0;
+        const double blk = static_cast<double>(rows) * kets->R * svec;
+        const double k_rec = doubling ? 3.0 : 4.0;     // x, z, (bra), y
+        const double k_first = doubling ? 2.0 : 3.0;   // x, (bra), y
+        stats->kernel_ms = kernel_ms;
+        stats->steps = steps;
+        stats->moments = n;
+        stats->bytes_per_step = mat + k_rec * blk;
+        stats->bytes_total = (mat + k_first * blk) + static_cast<double>(steps - 1) * (mat + k_rec * blk);
+        stats->doubling = doubling ? 1 : 0;
+        stats->blocks = static_cast<int32_t>(nb);
+        stats->rows_per_tile = TR;
+        stats->padded_R = Rp;
+        stats->wall_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
+std::mutex g_default_mu;
+tbk_context* g_default_ctx = nullptr;
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* tbk_version(void) { return "tbkpm 0.1 gfx950"; }
+const char* tbk_last_error(void) { return g_last_error.c_str(); }
+
+int tbk_device_count(int32_t* count) {
+    return guarded([&] {
+        if (!count) raise(TBK_E_INVALID, "null argument");
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        *count = (e == hipSuccess) ? c : 0;
+    });
+}
+
+int tbk_context_create(int32_t device, tbk_context** out) {
+    return guarded([&] {
+        if (!out) raise(TBK_E_INVALID, "null argument");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) raise(TBK_E_DEVICE, "no HIP device");
+        if (device < 0 || device >= count) raise(TBK_E_INVALID, "device index out of range");
+        auto ctx = std::make_unique<tbk_context>();
+        ctx->device = device;
+        HIPC(hipSetDevice(device));
+        HIPC(hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device));
+        HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        *out = ctx.release();
+    });
+}
+
+int tbk_context_destroy(tbk_context* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto e : ctx->events) (void)hipEventDestroy(e);
+        ctx->work[0].release();
+        ctx->work[1].release();
+        ctx->partial.release();
+        ctx->sums.release();
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int tbk_context_synchronize(tbk_context* ctx) {
+    return guarded([&] {
+        if (!ctx) raise(TBK_E_INVALID, "null argument");
+        set_device(ctx);
+        HIPC(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int tbk_context_memory(tbk_context* ctx, int64_t* bytes) {
+    return guarded([&] {
+        if (!ctx || !bytes) raise(TBK_E_INVALID, "null argument");
+        *bytes = ctx->held_bytes + static_cast<int64_t>(ctx->work[0].bytes + ctx->work[1].bytes +
+                                                        ctx->partial.bytes + ctx->sums.bytes);
+    });
+}
+
+int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out) {
+    return guarded([&] {
+        if (!ctx || !M || !out) raise(TBK_E_INVALID, "null argument");
+        if (M->rows <= 0 || M->rows != M->cols) raise(TBK_E_INVALID, "matrix must be square and non-empty");
+        if (M->nnz < 0 || M->nnz > INT32_MAX) raise(TBK_E_INVALID, "nnz out of range for int32 indices");
+        if (M->layout != TBK_CSC && M->layout != TBK_CSR) raise(TBK_E_INVALID, "unknown layout");
+        if (!M->outer || (M->nnz > 0 && (!M->inner || !M->values))) raise(TBK_E_INVALID, "null array");
+        std::vector<int32_t> rp, ci;
+        std::vector<double> val;
+        build_csr(M, rp, ci, val);
+        set_device(ctx);
+        auto m = std::make_unique<tbk_matrix>();
+        m->ctx = ctx;
+        m->rows = M->rows;
+        m->nnz = M->nnz;
+        m->is_complex = M->is_complex != 0;
+        m->rp.reserve(rp.size() * 4);
+        m->ci.reserve(std::max<size_t>(ci.size() * 4, 4));
+        m->val64.reserve(std::max<size_t>(val.size() * 8, 8));
+        HIPC(hipMemcpy(m->rp.p, rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
+        if (!ci.empty()) {
+            HIPC(hipMemcpy(m->ci.p, ci.data(), ci.size() * 4, hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(m->val64.p, val.data(), val.size() * 8, hipMemcpyHostToDevice));
+        }
+        ctx->held_bytes += static_cast<int64_t>(m->rp.bytes + m->ci.bytes + m->val64.bytes);
+        *out = m.release();
+    });
+}
+
+int tbk_matrix_destroy(tbk_matrix* m) {
+    return guarded([&] {
+        if (!m) return;
+        set_device(m->ctx);
+        HIPC(hipStreamSynchronize(m->ctx->stream));
+        m->ctx->held_bytes -= static_cast<int64_t>(m->rp.bytes + m->ci.bytes + m->val64.bytes + m->val32.bytes);
+        delete m;
+    });
+}
+
+int tbk_block_create(tbk_context* ctx, const double* colmajor, int32_t rows, int32_t R,
+                     int32_t precision, tbk_block** out) {
+    return guarded([&] {
+        if (!ctx || !out) raise(TBK_E_INVALID, "null argument");
+        if (rows <= 0 || R <= 0) raise(TBK_E_INVALID, "block must have rows >= 1 and R >= 1");
+        if (precision != TBK_FP64 && precision != TBK_FP32) raise(TBK_E_INVALID, "unknown precision");
+        if (R > 256) raise(TBK_E_UNSUPPORTED, "vector block wider than 256 columns");
+        set_device(ctx);
+        auto b = std::make_unique<tbk_block>();
+        b->ctx = ctx;
+        b->rows = rows;
+        b->R = R;
+        b->Rp = pow2_at_least(R);
+        b->precision = precision;
+        const size_t elems = static_cast<size_t>(rows) * b->Rp;
+        b->data.reserve(elems * b->elem_bytes());
+        if (precision == TBK_FP64) {
+            std::vector<double2> h(elems, make_double2(0.0, 0.0));
+            if (colmajor)
+                for (int r = 0; r < R; ++r)
+                    for (int i = 0; i < rows; ++i) {
+                        const size_t s = 2 * (static_cast<size_t>(r) * rows + i);
+                        h[static_cast<size_t>(i) * b->Rp + r] = make_double2(colmajor[s], colmajor[s + 1]);
+                    }
+            HIPC(hipMemcpy(b->data.p, h.data(), elems * sizeof(double2), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float2> h(elems, make_float2(0.f, 0.f));
+            if (colmajor)
+                for (int r = 0; r < R; ++r)
+                    for (int i = 0; i < rows; ++i) {
+                        const size_t s = 2 * (static_cast<size_t>(r) * rows + i);
+                        h[static_cast<size_t>(i) * b->Rp + r] =
+                            make_float2(static_cast<float>(colmajor[s]), static_cast<float>(colmajor[s + 1]));
+                    }
+            HIPC(hipMemcpy(b->data.p, h.data(), elems * sizeof(float2), hipMemcpyHostToDevice));
+        }
+        ctx->held_bytes += static_cast<int64_t>(b->data.bytes);
+        *out = b.release();
+    });
+}
+
+int tbk_block_random_phase(tbk_context* ctx, int32_t rows, int32_t R, uint64_t seed, int32_t precision,
+                           tbk_block** out) {
+    return guarded([&] {
+        if (!ctx || !out) raise(TBK_E_INVALID, "null argument");
+        if (rows <= 0 || R <= 0) raise(TBK_E_INVALID, "block must have rows >= 1 and R >= 1");
+        if (precision != TBK_FP64 && precision != TBK_FP32) raise(TBK_E_INVALID, "unknown precision");
+        if (R > 256) raise(TBK_E_UNSUPPORTED, "vector block wider than 256 columns");
+        set_device(ctx);
+        auto b = std::make_unique<tbk_block>();
+        b->ctx = ctx;
+        b->rows = rows;
+        b->R = R;
+        b->Rp = pow2_at_least(R);
+        b->precision = precision;
+        const size_t elems = static_cast<size_t>(rows) * b->Rp;
+        b->data.reserve(elems * b->elem_bytes());
+        const int grid = static_cast<int>(std::min<size_t>((elems + tbk::kBlock - 1) / tbk::kBlock, 65536));
+        const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
+        if (precision == TBK_FP64)
+            hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
+                               b->data.as<double2>(), rows, R, b->Rp, seed, inv);
+        else
+            hipLaunchKernelGGL(tbk::random_phase_kernel<float2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
+                               b->data.as<float2>(), rows, R, b->Rp, seed, inv);
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(ctx->stream));
+        ctx->held_bytes += static_cast<int64_t>(b->data.bytes);
+        *out = b.release();
+    });
+}
+
+int tbk_block_download(const tbk_block* b, double* colmajor_out) {
+    return guarded([&] {
+        if (!b || !colmajor_out) raise(TBK_E_INVALID, "null argument");
+        set_device(b->ctx);
+        HIPC(hipStreamSynchronize(b->ctx->stream));
+        const size_t elems = static_cast<size_t>(b->rows) * b->Rp;
+        if (b->precision == TBK_FP64) {
+            std::vector<double2> h(elems);
+            HIPC(hipMemcpy(h.data(), b->data.p, elems * sizeof(double2), hipMemcpyDeviceToHost));
+            for (int r = 0; r < b->R; ++r)
+                for (int i = 0; i < b->rows; ++i) {
+                    const size_t s = 2 * (static_cast<size_t>(r) * b->rows + i);
+                    colmajor_out[s] = h[static_cast<size_t>(i) * b->Rp + r].x;
+                    colmajor_out[s + 1] = h[static_cast<size_t>(i) * b->Rp + r].y;
+                }
+        } else {
+            std::vector<float2> h(elems);
+            HIPC(hipMemcpy(h.data(), b->data.p, elems * sizeof(float2), hipMemcpyDeviceToHost));
+            for (int r = 0; r < b->R; ++r)
+                for (int i = 0; i < b->rows; ++i) {
+                    const size_t s = 2 * (static_cast<size_t>(r) * b->rows + i);
+                    colmajor_out[s] = h[static_cast<size_t>(i) * b->Rp + r].x;
+                    colmajor_out[s + 1] = h[static_cast<size_t>(i) * b->Rp + r].y;
+                }
+        }
+    });
+}
+
+int tbk_block_destroy(tbk_block* b) {
+    return guarded([&] {
+        if (!b) return;
+        set_device(b->ctx);
+        HIPC(hipStreamSynchronize(b->ctx->stream));
+        b->ctx->held_bytes -= static_cast<int64_t>(b->data.bytes);
+        delete b;
+    });
+}
+
+int tbk_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* bras,
+                const tbk_block* kets, int32_t n, int32_t flags, double* mu_out, double* last_out,
+                tbk_stats* stats) {
+    return guarded([&] { run_moments(ctx, M, a, b, bras, kets, n, flags, mu_out, last_out, stats); });
+}
+
+int tbk_cheb_first_kind_seq_braket(const tbk_sparse* M, int32_t n, const double* bras_T, const double* kets,
+                                   int32_t ket_rows, int32_t ket_cols, double* mu_out) {
+    return guarded([&] {
+        if (!M || !bras_T || !kets || !mu_out) raise(TBK_E_INVALID, "null argument");
+        if (n < 2) raise(TBK_E_INVALID, "n must be greater than or equal to 2");
+        if (ket_rows != M->rows || ket_rows != M->cols) raise(TBK_E_INVALID, "kets rows do not match M");
+        tbk_context* ctx;
+        {
+            std::lock_guard<std::mutex> lk(g_default_mu);
+            if (!g_default_ctx) {
+                int rc = tbk_context_create(0, &g_default_ctx);
+                if (rc != TBK_OK) raise(rc, g_last_error);
+            }
+            ctx = g_default_ctx;
+        }
+        struct Guard {
+            tbk_matrix* m = nullptr;
+            tbk_block* k = nullptr;
+            tbk_block* b = nullptr;
+            ~Guard() {
+                if (b) tbk_block_destroy(b);
+                if (k) tbk_block_destroy(k);
+                if (m) tbk_matrix_destroy(m);
+            }
+        } g;
+        int rc = tbk_matrix_create(ctx, M, &g.m);
+        if (rc != TBK_OK) raise(rc, g_last_error);
+        rc = tbk_block_create(ctx, kets, ket_rows, ket_cols, TBK_FP64, &g.k);
+        if (rc != TBK_OK) raise(rc, g_last_error);
+        if (bras_T != kets) {
+            rc = tbk_block_create(ctx, bras_T, ket_rows, ket_cols, TBK_FP64, &g.b);
+            if (rc != TBK_OK) raise(rc, g_last_error);
+        }
+        run_moments(ctx, g.m, 1.0, 0.0, g.b, g.k, n, 0, mu_out, nullptr, nullptr);
+    });
+}
+
+}  // extern "C"

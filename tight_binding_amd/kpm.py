@@ -1,0 +1,159 @@
+"""Python interface of the HIP moment engine (libtbkpm.so).
+
+Context / Matrix / Block are device-resident objects; Context.moments runs
+the fused Chebyshev recursion.  cheb_first_kind_seq_braket is the drop-in of
+cuda::Cheb_first_kind_seq_braket (numerics_cuda.cuh:10-13).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .sparse import CSC, tbk_sparse_of
+
+FP64 = N.TBK_FP64
+FP32 = N.TBK_FP32
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def device_count() -> int:
+    n = C.c_int32()
+    N.check_kpm(N.kpm().tbk_device_count(C.byref(n)))
+    return n.value
+
+
+def _as_colmajor_c128(a: np.ndarray) -> np.ndarray:
+    """(rows, R) complex array -> column-major complex128 buffer as float64 view."""
+    a = np.asarray(a, dtype=np.complex128)
+    if a.ndim == 1:
+        a = a[:, None]
+    return np.asfortranarray(a).ravel(order="F").view(np.float64)
+
+
+class Matrix:
+    def __init__(self, ctx: "Context", m: CSC):
+        self.ctx = ctx
+        self.n = int(m.n)
+        self.nnz = int(np.asarray(m.rowidx).size)
+        self.is_complex = bool(np.iscomplexobj(m.values))
+        s, keep = tbk_sparse_of(m)
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_matrix_create(ctx._h, C.byref(s), C.byref(h)))
+        del keep
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            N.kpm().tbk_matrix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Block:
+    def __init__(self, ctx: "Context", handle, rows, R, precision):
+        self.ctx, self._h, self.rows, self.R, self.precision = ctx, handle, rows, R, precision
+
+    def download(self) -> np.ndarray:
+        out = np.zeros(2 * self.rows * self.R, dtype=np.float64)
+        N.check_kpm(N.kpm().tbk_block_download(self._h, _dp(out)))
+        return out.view(np.complex128).reshape(self.R, self.rows).T
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            N.kpm().tbk_block_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device, one HIP stream.  Not safe for concurrent use."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_context_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            N.kpm().tbk_context_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def memory(self) -> int:
+        b = C.c_int64()
+        N.check_kpm(N.kpm().tbk_context_memory(self._h, C.byref(b)))
+        return b.value
+
+    def synchronize(self):
+        N.check_kpm(N.kpm().tbk_context_synchronize(self._h))
+
+    def matrix(self, m: CSC) -> Matrix:
+        return Matrix(self, m)
+
+    def block(self, kets: np.ndarray, precision: int = FP64) -> Block:
+        kets = np.asarray(kets)
+        if kets.ndim == 1:
+            kets = kets[:, None]
+        rows, R = kets.shape
+        buf = _as_colmajor_c128(kets)
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_block_create(self._h, _dp(buf), rows, R, int(precision), C.byref(h)))
+        return Block(self, h, rows, R, precision)
+
+    def random_block(self, rows: int, R: int, seed: int, precision: int = FP64) -> Block:
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_block_random_phase(self._h, rows, R, int(seed), int(precision), C.byref(h)))
+        return Block(self, h, rows, R, precision)
+
+    def moments(self, M: Matrix, kets: Block, n: int, bras: Block | None = None, a: float = 1.0,
+                b: float = 0.0, plain: bool = False, time_launches: bool = False, want_last: bool = False):
+        """mu[i] = Re <bras| T_i((M - b)/a) |kets>, i < n.  Returns (mu, stats, last)."""
+        mu = np.zeros(int(n), dtype=np.float64)
+        last = np.zeros(2 * kets.rows * kets.R, dtype=np.float64) if want_last else None
+        st = N.tbk_stats()
+        flags = (N.TBK_FLAG_PLAIN if plain else 0) | (N.TBK_FLAG_TIME_LAUNCHES if time_launches else 0)
+        N.check_kpm(
+            N.kpm().tbk_moments(
+                self._h, M._h, float(a), float(b), bras._h if bras is not None else None, kets._h, int(n), flags,
+                _dp(mu), _dp(last) if last is not None else None, C.byref(st),
+            )
+        )
+        if last is not None:
+            last = last.view(np.complex128).reshape(kets.R, kets.rows).T
+        return mu, st.as_dict(), last
+
+
+def cheb_first_kind_seq_braket(M: CSC, n: int, bras_T: np.ndarray, kets: np.ndarray) -> np.ndarray:
+    """Drop-in of cuda::Cheb_first_kind_seq_braket: M already rescaled (CSC),
+    bras_T and kets (rows, R) complex; passing the same array object for both
+    selects the two-moments-per-step path."""
+    s, keep = tbk_sparse_of(M)
+    same = bras_T is kets
+    kb = _as_colmajor_c128(kets)
+    bb = kb if same else _as_colmajor_c128(bras_T)
+    rows, R = (kets.shape[0], 1) if np.ndim(kets) == 1 else kets.shape
+    mu = np.zeros(int(n), dtype=np.float64)
+    N.check_kpm(N.kpm().tbk_cheb_first_kind_seq_braket(C.byref(s), int(n), _dp(bb), _dp(kb), rows, R, _dp(mu)))
+    del keep
+    return mu
