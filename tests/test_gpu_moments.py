@@ -42,7 +42,7 @@ def test_dropin_c1_doubling_and_general_match_oracle():
     ref = oracle.cheb_braket(Hs, N, kets, kets)
     mu_d = kpm.cheb_first_kind_seq_braket(Hs, N, kets, kets)  # same object -> doubling
     mu_g = kpm.cheb_first_kind_seq_braket(Hs, N, kets.copy(), kets)  # distinct bras -> one moment/step
-    assert abs(ref[0] - 16.0) < 1e-12
+    assert abs(ref[0] - 16.0) < 1e-9  # sequential sum of 160k terms
     assert_moments_close(mu_g, ref)
     assert_moments_close(mu_d, ref)
 
