@@ -86,7 +86,9 @@ def main():
     ap.add_argument("--R", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=3, help="time this many plain block copies")
     ap.add_argument("--extra", action="store_true", help="also time fp32 and one-moment-per-step variants")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default launch shape")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -105,6 +107,24 @@ def main():
     K = ctx.random_block(H.n, args.R, seed=7)
     ctx.synchronize()
     log("resident: %.2f GB on device" % (ctx.memory() / 1e9))
+
+    copy_gbps = None
+    if args.copy_probe > 0:
+        # plain device copy of one vector block: the measured bandwidth ceiling
+        # for streaming traffic, and the counter calibration kernel
+        K2 = ctx.random_block(H.n, args.R, seed=8)
+        nbytes = 2.0 * H.n * args.R * 16  # read + write
+        times = [K.copy_to(K2) for _ in range(args.copy_probe)]
+        copy_gbps = nbytes / (min(times) * 1e-3) / 1e9
+        log("block copy: %.3f ms best of %d, %.1f GB/s (read+write)" % (min(times), len(times), copy_gbps))
+        K2.close()
+
+    # launch-shape autotuning for this matrix and block (outside the timed region)
+    tuning = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning, tms = ctx.autotune(M, K, a=a, steps=16)
+        log("autotune: %.3f ms/step with %s (%.1f s)" % (tms, json.dumps(tuning), time.perf_counter() - t0))
 
     # warmup
     ctx.moments(M, K, 2 * args.warmup, a=a)
@@ -134,7 +154,11 @@ def main():
             "bytes_per_step": stp["bytes_per_step"],
         }
         K32 = ctx.random_block(H.n, args.R, seed=7, precision=kpm.FP32)
+        saved = ctx.get_tuning()
+        if not args.no_autotune:
+            ctx.autotune(M, K32, a=a, steps=16)
         mu32, st32, _ = ctx.moments(M, K32, n_mom, a=a, time_launches=True)
+        ctx.set_tuning(**saved)
         extra["fp32_recursion"] = {
             "steps_per_s": st32["steps"] / (st32["kernel_ms"] * 1e-3),
             "GBps": st32["bytes_total"] / (st32["kernel_ms"] * 1e-3) / 1e9,
@@ -169,6 +193,8 @@ def main():
                         "R=%d, N=%d moments, fp64" % (args.cells, args.cells, H.n, args.R, n_mom),
             "sites": H.n, "nnz": H.nnz, "R": args.R, "moments": n_mom, "moments_per_step": 2,
             "parallelism": "single GPU",
+            "launch_tuning": tuning,
+            "kernel_kind": st["kernel_kind"], "chunk_rows": st["chunk_rows"], "chunk_period": st["chunk_period"],
         },
         "moments_per_s": n_mom / (st["kernel_ms"] * 1e-3),
         "roofline": {
@@ -181,6 +207,7 @@ def main():
             "bytes_per_step": st["bytes_per_step"],
         },
         "cpu_baseline": cpu,
+        "copy_GBps_same_block": copy_gbps,
     }
     if extra:
         out["variants"] = extra

@@ -48,8 +48,9 @@ enum { TBK_CSC = 0, TBK_CSR = 1 };
 
 /* flags for tbk_moments */
 enum {
-    TBK_FLAG_PLAIN = 1,        /* bras == kets: still run one step per moment (no doubling) */
-    TBK_FLAG_TIME_LAUNCHES = 2 /* time every step launch with its own HIP event pair */
+    TBK_FLAG_PLAIN = 1,         /* bras == kets: still run one step per moment (no doubling) */
+    TBK_FLAG_TIME_LAUNCHES = 2, /* time every step launch with its own HIP event pair */
+    TBK_FLAG_ROW_KERNEL = 4     /* use the row kernel even where the tiled kernel applies */
 };
 
 typedef struct tbk_sparse {
@@ -73,7 +74,25 @@ typedef struct tbk_stats {
     int32_t blocks;         /* workgroups per step launch */
     int32_t rows_per_tile;  /* rows one workgroup handles per tile */
     int32_t padded_R;       /* vector columns stored per site on the device */
+    int32_t kernel_kind;    /* 0: row kernel; 1, 2, 4: tiled kernel for rows of <= 4*kind entries */
+    int32_t slab_cols;      /* vector columns one sweep over the rows covers */
+    int32_t chunk_rows;     /* rows per chunk of the visiting order */
+    int32_t chunk_period;   /* period P of the transposed chunk order (1: row order) */
 } tbk_stats;
+
+/* Launch-shape choices of a context (tbk_autotune sets these; zero fields
+   mean the defaults, which are the measured best on the 2.1M-site
+   honeycomb with R = 64). */
+typedef struct tbk_tuning {
+    int32_t slab_cols;       /* columns per sweep of the tiled kernel (power of two <= R) */
+    int32_t blocks_per_cu;   /* workgroups per CU in a step launch (default 4) */
+    int32_t xcd_split;       /* 1: deal row ranges per XCD, -1: never, 0: when >= 64 workgroups */
+    int32_t l2_budget_bytes; /* per-XCD L2 bytes the slab heuristic may plan for (default 3 MiB) */
+    int32_t stream_nt;       /* non-temporal loads/stores for rows used once per step: 0/1 on, -1 off */
+    int32_t use_tiled;       /* 1: LDS-staged tiled kernel where rows have <= 16 entries; 0: row kernel */
+    int32_t chunk_rows;      /* rows per chunk of the visiting order (default 256) */
+    int32_t chunk_period;    /* chunk-grid period P; 0: bandwidth / chunk rows, -1: row order */
+} tbk_tuning;
 
 const char* tbk_version(void);
 const char* tbk_last_error(void);
@@ -103,12 +122,18 @@ typedef struct tbk_block tbk_block;
 int tbk_context_create(int32_t device, tbk_context** out);
 int tbk_context_destroy(tbk_context* ctx);
 int tbk_context_synchronize(tbk_context* ctx);
+int tbk_context_set_tuning(tbk_context* ctx, const tbk_tuning* t);
+int tbk_context_get_tuning(const tbk_context* ctx, tbk_tuning* t);
 /* bytes of device memory the context currently holds (matrices, blocks, work) */
 int tbk_context_memory(tbk_context* ctx, int64_t* bytes);
 
 /* upload a square sparse matrix; CSC input is reordered to CSR exactly */
 int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out);
 int tbk_matrix_destroy(tbk_matrix* m);
+/* shape statistics used by the kernel choice: longest row and the 95th
+   percentile of |row - col| over the entries */
+int tbk_matrix_info(const tbk_matrix* m, int32_t* rows, int64_t* nnz, int32_t* max_row_nnz,
+                    int32_t* bandwidth95);
 
 /* upload rows x R column-major complex128 (NULL: zeros) as a block */
 int tbk_block_create(tbk_context* ctx, const double* colmajor, int32_t rows, int32_t R,
@@ -119,6 +144,9 @@ int tbk_block_random_phase(tbk_context* ctx, int32_t rows, int32_t R, uint64_t s
                            int32_t precision, tbk_block** out);
 /* copy a block back as rows x R column-major complex128 */
 int tbk_block_download(const tbk_block* b, double* colmajor_out);
+/* device copy dst = src (same shape and precision); kernel_ms (optional)
+   receives the copy's device time */
+int tbk_block_copy(const tbk_block* src, tbk_block* dst, float* kernel_ms);
 int tbk_block_destroy(tbk_block* b);
 
 /*
@@ -135,6 +163,17 @@ int tbk_block_destroy(tbk_block* b);
 int tbk_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b,
                 const tbk_block* bras, const tbk_block* kets, int32_t n, int32_t flags,
                 double* mu_out, double* last_out, tbk_stats* stats);
+
+/*
+ * Autotuner: times `steps` recursion steps of tbk_moments for M and kets under
+ * candidate launch shapes (coordinate descent over kernel family, workgroups
+ * per CU, chunk size, XCD split and streaming stores, from the current
+ * tuning), installs the fastest in the context and returns it with its time
+ * per step.  Results do not depend on the tuning beyond rounding of the
+ * moment sums (the recursion vectors are bit-identical across shapes).
+ */
+int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
+                 int32_t steps, tbk_tuning* best, double* best_ms_per_step);
 
 #ifdef __cplusplus
 }

@@ -140,6 +140,50 @@ def test_block_widths(gpu_ctx, R):
     assert_moments_close(mu2, ref)
 
 
+@pytest.mark.parametrize("model,nn,R,prec", [
+    ("honeycomb", [("A", 1, -2.7)], 64, "fp64"),
+    ("square", [("A", 1, 1.0)], 16, "fp64"),
+    ("fcc", [("A", 1, 1.0)], 32, "fp64"),
+    ("cubic", [("A", 1, 1.0), ("A", 2, 0.3), ("A", 3, 0.1)], 8, "fp64"),  # 27 entries/row: row kernel
+    ("honeycomb", [("A", 1, -2.7)], 64, "fp32"),
+])
+def test_tiled_and_row_kernels_and_slabs_agree(gpu_ctx, model, nn, R, prec):
+    """Every kernel family and slab width produces the oracle's T_n bit for
+    bit (fp64) and the same moments; kernel_kind / slab_cols report the
+    choice."""
+    span = (10, 10, 10) if model in ("fcc", "cubic") else (40, 36)
+    p, _ = build_pair(model, span, nn, W=1.0, seed=2)
+    gersh = np.bincount(p.rowidx, weights=np.abs(p.values), minlength=p.n).max()
+    Hs = O.rescale(p, 1.01 * gersh, 0.0)
+    kets = O.random_phase_kets(Hs.n, R, 5)
+    N = 24
+    ref, T = oracle.cheb_braket(Hs, N, kets, kets, last_index=N - 1)
+    precision = kpm.FP64 if prec == "fp64" else kpm.FP32
+    M, K = gpu_ctx.matrix(Hs), gpu_ctx.block(kets, precision=precision)
+    seen = set()
+    try:
+        for slab, chunk_rows, period, nt in [(0, 0, 0, 0), (8, 64, 3, -1), (16, 0, -1, 0), (32, 256, 0, -1),
+                                             (64, 32, 0, 0), (64, 64, 7, 1)]:
+            for row in (False, True):
+                gpu_ctx.set_tuning(slab_cols=slab, chunk_rows=chunk_rows, chunk_period=period, stream_nt=nt,
+                                   use_tiled=0 if row else 1)
+                mu, st, last = gpu_ctx.moments(M, K, N, plain=True, want_last=True, row_kernel=row)
+                seen.add((st["kernel_kind"], st["slab_cols"]))
+                if prec == "fp64":
+                    assert np.array_equal(last, T), (slab, row, st)
+                    assert_moments_close(mu, ref)
+                else:
+                    assert np.max(np.abs(mu - ref)) < 1e-5 * np.max(np.abs(ref))
+                mu2, _, _ = gpu_ctx.moments(M, K, N, row_kernel=row)
+                assert np.max(np.abs(mu2 - mu)) <= 1e-12 * np.max(np.abs(mu)) * (1 if prec == "fp64" else 1e6)
+    finally:
+        gpu_ctx.set_tuning()
+    if model == "cubic":
+        assert all(k == 0 for k, _ in seen)
+    else:
+        assert any(k > 0 for k, _ in seen) and any(k == 0 for k, _ in seen)
+
+
 @pytest.mark.parametrize("N", [2, 3, 4, 5])
 def test_smallest_moment_counts(gpu_ctx, N):
     _, Hs, kets = c1_problem(L=10, R=4)
@@ -242,3 +286,18 @@ def test_large_honeycomb_size_independent_properties(gpu_ctx):
     assert abs(mu_d[0] - 64.0) < 1e-10
     assert np.all(np.abs(mu_d) <= mu_d[0] * (1 + 1e-12))
     assert_moments_close(mu_d, mu_p2)
+
+
+def test_autotune_installs_a_valid_tuning(gpu_ctx):
+    p, _ = build_pair("honeycomb", (96, 96), [("A", 1, -2.7)], W=0.5, seed=1)
+    M = gpu_ctx.matrix(p)
+    K = gpu_ctx.random_block(p.n, 64, seed=3)
+    a = 8.5
+    ref, _, _ = gpu_ctx.moments(M, K, 64, a=a)
+    try:
+        best, ms = gpu_ctx.autotune(M, K, a=a, steps=4)
+        assert ms > 0 and gpu_ctx.get_tuning() == best
+        mu, st, _ = gpu_ctx.moments(M, K, 64, a=a)
+        assert_moments_close(mu, ref, tol=1e-12)
+    finally:
+        gpu_ctx.set_tuning()

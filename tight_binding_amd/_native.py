@@ -23,6 +23,7 @@ TBK_CSC = 0
 TBK_CSR = 1
 TBK_FLAG_PLAIN = 1
 TBK_FLAG_TIME_LAUNCHES = 2
+TBK_FLAG_ROW_KERNEL = 4
 
 
 class NativeError(RuntimeError):
@@ -56,10 +57,27 @@ class tbk_stats(C.Structure):
         ("blocks", C.c_int32),
         ("rows_per_tile", C.c_int32),
         ("padded_R", C.c_int32),
+        ("kernel_kind", C.c_int32),
+        ("slab_cols", C.c_int32),
+        ("chunk_rows", C.c_int32),
+        ("chunk_period", C.c_int32),
     ]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class tbk_tuning(C.Structure):
+    _fields_ = [
+        ("slab_cols", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("xcd_split", C.c_int32),
+        ("l2_budget_bytes", C.c_int32),
+        ("stream_nt", C.c_int32),
+        ("use_tiled", C.c_int32),
+        ("chunk_rows", C.c_int32),
+        ("chunk_period", C.c_int32),
+    ]
 
 
 def _load(name):
@@ -93,13 +111,18 @@ KPM_SIGNATURES = {
     "tbk_context_destroy": ([P], I32),
     "tbk_context_synchronize": ([P], I32),
     "tbk_context_memory": ([P, C.POINTER(I64)], I32),
+    "tbk_context_set_tuning": ([P, C.POINTER(tbk_tuning)], I32),
+    "tbk_context_get_tuning": ([P, C.POINTER(tbk_tuning)], I32),
     "tbk_matrix_create": ([P, C.POINTER(tbk_sparse), PP], I32),
     "tbk_matrix_destroy": ([P], I32),
+    "tbk_matrix_info": ([P, IP, C.POINTER(I64), IP, IP], I32),
     "tbk_block_create": ([P, DP, I32, I32, I32, PP], I32),
     "tbk_block_random_phase": ([P, I32, I32, U64, I32, PP], I32),
     "tbk_block_download": ([P, DP], I32),
+    "tbk_block_copy": ([P, P, C.POINTER(C.c_float)], I32),
     "tbk_block_destroy": ([P], I32),
     "tbk_moments": ([P, P, D, D, P, P, I32, I32, DP, DP, C.POINTER(tbk_stats)], I32),
+    "tbk_autotune": ([P, P, D, D, P, I32, C.POINTER(tbk_tuning), DP], I32),
 }
 
 HOST_SIGNATURES = {

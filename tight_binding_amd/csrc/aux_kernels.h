@@ -1,0 +1,100 @@
+// Small device kernels of the engine other than the recursion step:
+// moment-partial reduction, random-phase block generation, block copy and
+// fp64 -> fp32 value conversion.  Included by tbkpm.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cheb_kernels.h"
+
+namespace tbk {
+
+// out[s] = sum_b partial[s*nb + b], fixed order; one workgroup per step.
+__global__ __launch_bounds__(kBlock) void reduce_partials_kernel(const double2* __restrict__ partial,
+                                                                 int32_t nb,
+                                                                 double2* __restrict__ out) {
+    const int s = blockIdx.x;
+    const double2* ps = partial + static_cast<size_t>(s) * nb;
+    double a0 = 0.0, a1 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+        a0 = a0 + ps[b].x;
+        a1 = a1 + ps[b].y;
+    }
+    __shared__ double r0[kBlock], r1[kBlock];
+    r0[threadIdx.x] = a0;
+    r1[threadIdx.x] = a1;
+    __syncthreads();
+    for (int w = kBlock / 2; w >= 1; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
+            r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = make_double2(r0[0], r1[0]);
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// e^{i 2 pi u(seed, site*R + col)} / sqrt(rows); padding columns zero
+template <typename V2>
+__global__ __launch_bounds__(kBlock) void random_phase_kernel(V2* __restrict__ out, int32_t rows,
+                                                              int32_t R, int32_t Rp, uint64_t seed,
+                                                              double inv_sqrt_n) {
+    const int64_t total = static_cast<int64_t>(rows) * Rp;
+    const uint64_t key = splitmix64(seed);
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; idx < total;
+         idx += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const int64_t i = idx / Rp;
+        const int col = static_cast<int>(idx - i * Rp);
+        V2 v;
+        if (col < R) {
+            const uint64_t h = splitmix64(key ^ static_cast<uint64_t>(i * R + col));
+            const double u = static_cast<double>(h >> 11) * 0x1.0p-53;
+            double sn, cs;
+            sincos(2.0 * 3.141592653589793 * u, &sn, &cs);
+            v.x = static_cast<decltype(v.x)>(cs * inv_sqrt_n);
+            v.y = static_cast<decltype(v.y)>(sn * inv_sqrt_n);
+        } else {
+            v.x = 0;
+            v.y = 0;
+        }
+        out[idx] = v;
+    }
+}
+
+// dst = src, 16-byte elements (a vector block copy; also the reference
+// kernel of known traffic for the counter calibration)
+// Each thread moves kCopyUnroll elements, all loads issued before the stores.
+constexpr int kCopyUnroll = 8;
+__global__ __launch_bounds__(kBlock) void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                        int64_t n) {
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kBlock * kCopyUnroll + threadIdx.x;
+    uint4 v[kCopyUnroll];
+#pragma unroll
+    for (int k = 0; k < kCopyUnroll; ++k) {
+        const int64_t i = base + static_cast<int64_t>(k) * kBlock;
+        if (i < n) v[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kCopyUnroll; ++k) {
+        const int64_t i = base + static_cast<int64_t>(k) * kBlock;
+        if (i < n) dst[i] = v[k];
+    }
+}
+
+// fp64 -> fp32 copy of matrix values
+__global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double* __restrict__ in,
+                                                         float* __restrict__ out, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock)
+        out[i] = static_cast<float>(in[i]);
+}
+
+}  // namespace tbk

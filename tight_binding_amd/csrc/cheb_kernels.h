@@ -81,10 +81,66 @@ __device__ __forceinline__ double redot(float2 a, float2 b) {
            static_cast<double>(a.y) * static_cast<double>(b.y);
 }
 
+// Step options (bit flags of the kernels' `opts` argument).
+constexpr int kOptShift = 1;   // beta*x_r term present (b != 0)
+constexpr int kOptStream = 2;  // non-temporal access for the once-used rows
+
+// Rows read or written once per step (T_{k-1}, bra, T_{k+1}) may bypass
+// cache retention so that L2 keeps the gathered neighbour rows.
+template <typename V2>
+__device__ __forceinline__ V2 ld_stream(const V2* p, int opts) {
+    using E = decltype(p->x);
+    typedef E vec2 __attribute__((ext_vector_type(2)));
+    if (opts & kOptStream) {
+        const vec2 v = __builtin_nontemporal_load(reinterpret_cast<const vec2*>(p));
+        V2 r;
+        r.x = v.x;
+        r.y = v.y;
+        return r;
+    }
+    return *p;
+}
+template <typename V2>
+__device__ __forceinline__ void st_stream(V2* p, V2 v, int opts) {
+    using E = decltype(v.x);
+    typedef E vec2 __attribute__((ext_vector_type(2)));
+    if (opts & kOptStream) {
+        vec2 w;
+        w.x = v.x;
+        w.y = v.y;
+        __builtin_nontemporal_store(w, reinterpret_cast<vec2*>(p));
+    } else {
+        *p = v;
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, kWave);
     return v;
+}
+
+// Visiting order of row tiles.  Row tiles are grouped in chunks of tpc
+// tiles; with P > 1 the chunk grid (Q rows of P chunks) is walked column by
+// column -- chunk q*P + p in the order p-major -- so that chunks P apart in
+// the row numbering, which a matrix of bandwidth ~ P chunks couples, are
+// visited one after the other: a neighbour row fetched for one chunk is
+// reused by the next instead of ~2D rows later.  P = 1 is row order.  Visit
+// indices past the last row tile are empty.  Slabs repeat the order.
+struct TileOrder {
+    uint32_t row_tiles;  // tiles covering the rows once
+    uint32_t tpc;        // tiles per chunk
+    uint32_t P, Q;       // chunk grid: P columns, Q rows (P*Q*tpc >= row_tiles)
+};
+
+__device__ __forceinline__ int64_t visit_row_tile(const TileOrder& o, uint32_t t, uint32_t& slab) {
+    const uint32_t per = o.P * o.Q * o.tpc;
+    slab = t / per;
+    const uint32_t u = t - slab * per;
+    const uint32_t cv = u / o.tpc, off = u - cv * o.tpc;
+    const uint32_t p = cv / o.Q, q = cv - p * o.Q;
+    const uint32_t rt = (q * o.P + p) * o.tpc + off;
+    return rt < o.row_tiles ? static_cast<int64_t>(rt) : -1;
 }
 
 // Tile range of this workgroup (see file comment).
@@ -110,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const typename HVal<VT, CH>::type* __restrict__ hv, const typename Vec2<VT>::type* __restrict__ x,
     const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y,
     const typename Vec2<VT>::type* __restrict__ bra, int32_t rows, int32_t Rp, VT alpha, VT beta,
-    int32_t shift, int64_t ntiles, int32_t xcd_split, double2* __restrict__ partial) {
+    int32_t opts, int64_t ntiles, TileOrder order, int32_t xcd_split, double2* __restrict__ partial) {
     using V2 = typename Vec2<VT>::type;
     using H = typename HVal<VT, CH>::type;
     constexpr int RPW = kWave / LPR;
@@ -127,7 +183,10 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
 
     double p0 = 0.0, p1 = 0.0;
     for (int64_t t = first; t < end; t += stride) {
-        int r = static_cast<int>(t * TR) + wave * RPW + sub;
+        uint32_t slab_unused;
+        const int64_t rt = visit_row_tile(order, static_cast<uint32_t>(t), slab_unused);
+        if (rt < 0) continue;
+        int r = static_cast<int>(rt * TR) + wave * RPW + sub;
         if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
         if (r >= rows) continue;
         const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * CPL;
@@ -142,11 +201,11 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
         }
         if (MODE != kFirst) {
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) zv[c] = z[own + c];
+            for (int c = 0; c < CPL; ++c) zv[c] = ld_stream(z + own + c, opts);
         }
         if (MODE != kRecDbl) {
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) bv[c] = bra[own + c];
+            for (int c = 0; c < CPL; ++c) bv[c] = ld_stream(bra + own + c, opts);
         }
         const int s = rp[r], e = rp[r + 1];
         bool have_self = false;
@@ -202,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             V2 v;
             v.x = alpha * acc[c].x;
             v.y = alpha * acc[c].y;
-            if (shift) {
+            if (opts & kOptShift) {
                 v.x = v.x + beta * xs[c].x;
                 v.y = v.y + beta * xs[c].y;
             }
@@ -210,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
                 v.x = v.x - zv[c].x;
                 v.y = v.y - zv[c].y;
             }
-            y[own + c] = v;
+            st_stream(y + own + c, v, opts);
             if (MODE == kFirst) {
                 p0 = p0 + redot(bv[c], xs[c]);
                 p1 = p1 + redot(bv[c], v);
@@ -242,71 +301,205 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     }
 }
 
-// out[s] = sum_b partial[s*nb + b], fixed order; one workgroup per step.
-__global__ __launch_bounds__(kBlock) void reduce_partials_kernel(const double2* __restrict__ partial,
-                                                                 int32_t nb,
-                                                                 double2* __restrict__ out) {
-    const int s = blockIdx.x;
-    const double2* ps = partial + static_cast<size_t>(s) * nb;
-    double a0 = 0.0, a1 = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) {
-        a0 = a0 + ps[b].x;
-        a1 = a1 + ps[b].y;
-    }
-    __shared__ double r0[kBlock], r1[kBlock];
-    r0[threadIdx.x] = a0;
-    r1[threadIdx.x] = a1;
-    __syncthreads();
-    for (int w = kBlock / 2; w >= 1; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w) {
-            r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
-            r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
+// Tiled variant of cheb_step_kernel for matrices with at most KMAX = 4*KCH
+// entries per row (every lattice Hamiltonian of the presets: 4 to 13).
+// Per tile of TR consecutive rows the workgroup first stages the row
+// pointers, column indices and values of the tile into LDS with coalesced
+// loads; then every lane issues the loads of all RW rows it owns -- the
+// T_{k-1} row, the bra row and KMAX neighbour-row gathers per row (slots past
+// a row's end read the row itself and are discarded) -- before any of them
+// is consumed, so a wave keeps RW*(KMAX+1) row loads in flight.  Results and
+// accumulation order are those of cheb_step_kernel.
+//
+// Column slabs: the Rp columns of a block are processed as Rp/SW slabs of
+// SW = LPR*CPL columns, each a sweep over all rows (tile t -> slab t /
+// row_tiles, row tile t % row_tiles).  A neighbour row gathered for row i is
+// needed again up to the matrix bandwidth D rows later; a narrow slab keeps
+// the rows touched in that window (about 3*2D*SW elements) inside one XCD's
+// L2, so the gathers hit in cache instead of re-reading memory.  The CSR
+// arrays are then read once per slab.
+//
+// Rows per lane-slot and tile size of cheb_tile_kernel: as many rows as the
+// register budget allows (4 rows of 4 gathers of one complex per lane), while
+// the staged CSR of a tile stays within kTileLds bytes of LDS.
+constexpr int kTileLds = 16384;
+constexpr int tile_rw(int lpr, int cpl, int kch, int hbytes) {
+    int rw = 4 / (kch * cpl);
+    if (rw < 1) rw = 1;
+    while (rw > 1 && kWavesPerBlock * (kWave / lpr) * rw * 4 * kch * (4 + hbytes) > kTileLds) rw /= 2;
+    return rw;
+}
+constexpr int tile_rows(int lpr, int cpl, int kch, int hbytes) {
+    return kWavesPerBlock * (kWave / lpr) * tile_rw(lpr, cpl, kch, hbytes);
+}
+constexpr bool tile_fits(int lpr, int cpl, int kch, int hbytes) {
+    return tile_rows(lpr, cpl, kch, hbytes) * 4 * kch * (4 + hbytes) <= kTileLds;
+}
+
+template <typename VT, bool CH, int LPR, int CPL, int MODE, int KCH>
+__global__ __launch_bounds__(kBlock) void cheb_tile_kernel(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const typename HVal<VT, CH>::type* __restrict__ hv, const typename Vec2<VT>::type* __restrict__ x,
+    const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y,
+    const typename Vec2<VT>::type* __restrict__ bra, int32_t rows, int32_t Rp, VT alpha, VT beta,
+    int32_t opts, int64_t ntiles, TileOrder order, int32_t xcd_split, double2* __restrict__ partial) {
+    using V2 = typename Vec2<VT>::type;
+    using H = typename HVal<VT, CH>::type;
+    constexpr int RPW = kWave / LPR;
+    constexpr int SW = LPR * CPL;
+    constexpr int KMAX = 4 * KCH;
+    constexpr int RW = tile_rw(LPR, CPL, KCH, sizeof(H));
+    constexpr int TR = kWavesPerBlock * RPW * RW;
+    constexpr int CAP = TR * KMAX;
+    static_assert(tile_fits(LPR, CPL, KCH, sizeof(H)), "staged tile exceeds its LDS budget");
+
+    __shared__ int32_t s_rp[TR + 1];
+    __shared__ int32_t s_ci[CAP];
+    __shared__ H s_hv[CAP];
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int sub = lane / LPR;
+    const int q = lane % LPR;
+
+    int64_t first, end, stride;
+    tile_range(ntiles, xcd_split, first, end, stride);
+
+    double p0 = 0.0, p1 = 0.0;
+    for (int64_t t = first; t < end; t += stride) {
+        uint32_t slab;
+        const int64_t rt = visit_row_tile(order, static_cast<uint32_t>(t), slab);
+        if (rt < 0) continue;  // uniform over the workgroup
+        const int64_t r0 = rt * TR;
+        const size_t col0 = static_cast<size_t>(slab) * SW + static_cast<size_t>(q) * CPL;
+        for (int i = threadIdx.x; i <= TR; i += kBlock) {
+            const int64_t rr = r0 + i;
+            s_rp[i] = rp[rr < rows ? rr : rows];
         }
         __syncthreads();
-    }
-    if (threadIdx.x == 0) out[s] = make_double2(r0[0], r1[0]);
-}
-
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
-
-// e^{i 2 pi u(seed, site*R + col)} / sqrt(rows); padding columns zero
-template <typename V2>
-__global__ __launch_bounds__(kBlock) void random_phase_kernel(V2* __restrict__ out, int32_t rows,
-                                                              int32_t R, int32_t Rp, uint64_t seed,
-                                                              double inv_sqrt_n) {
-    const int64_t total = static_cast<int64_t>(rows) * Rp;
-    const uint64_t key = splitmix64(seed);
-    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; idx < total;
-         idx += static_cast<int64_t>(gridDim.x) * kBlock) {
-        const int64_t i = idx / Rp;
-        const int col = static_cast<int>(idx - i * Rp);
-        V2 v;
-        if (col < R) {
-            const uint64_t h = splitmix64(key ^ static_cast<uint64_t>(i * R + col));
-            const double u = static_cast<double>(h >> 11) * 0x1.0p-53;
-            double sn, cs;
-            sincos(2.0 * 3.141592653589793 * u, &sn, &cs);
-            v.x = static_cast<decltype(v.x)>(cs * inv_sqrt_n);
-            v.y = static_cast<decltype(v.y)>(sn * inv_sqrt_n);
-        } else {
-            v.x = 0;
-            v.y = 0;
+        const int base = s_rp[0];
+        const int cnt = s_rp[TR] - base;  // <= CAP: every row has <= KMAX entries
+        for (int j = threadIdx.x; j < cnt; j += kBlock) {
+            s_ci[j] = ci[base + j];
+            s_hv[j] = hv[base + j];
         }
-        out[idx] = v;
-    }
-}
+        __syncthreads();
 
-// fp64 -> fp32 copy of matrix values
-__global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double* __restrict__ in,
-                                                         float* __restrict__ out, int64_t n) {
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * kBlock)
-        out[i] = static_cast<float>(in[i]);
+        int loc[RW], r[RW];
+        bool live[RW];
+        size_t own[RW];
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            loc[k] = (k * kWavesPerBlock + wave) * RPW + sub;
+            const int64_t rr = r0 + loc[k];
+            live[k] = rr < rows;
+            r[k] = static_cast<int>(live[k] ? rr : rows - 1);
+            own[k] = static_cast<size_t>(r[k]) * Rp + col0;
+        }
+
+        V2 zv[RW][CPL], bv[RW][CPL], xv[RW][KMAX][CPL];
+        H hh[RW][KMAX];
+        int cc[RW][KMAX];
+        int ne[RW];
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            if (MODE != kFirst) {
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) zv[k][c] = ld_stream(z + own[k] + c, opts);
+            }
+            if (MODE != kRecDbl) {
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) bv[k][c] = ld_stream(bra + own[k] + c, opts);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int s = s_rp[loc[k]] - base;
+            ne[k] = s_rp[loc[k] + 1] - base - s;
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j) {
+                const bool ok = j < ne[k];
+                const int idx = ok ? s + j : 0;  // s + j < cnt <= CAP whenever ok
+                cc[k][j] = ok ? s_ci[idx] : r[k];
+                hh[k][j] = ok ? s_hv[idx] : H{};
+                const V2* xr = x + static_cast<size_t>(cc[k][j]) * Rp + col0;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) xv[k][j][c] = xr[c];
+            }
+        }
+
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            V2 acc[CPL], xs[CPL];
+            bool have_self = false;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                acc[c].x = 0;
+                acc[c].y = 0;
+                xs[c].x = 0;
+                xs[c].y = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j) {
+                if (j < ne[k]) {
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) madd(acc[c], hh[k][j], xv[k][j][c]);
+                    if (cc[k][j] == r[k]) {
+                        have_self = true;
+#pragma unroll
+                        for (int c = 0; c < CPL; ++c) xs[c] = xv[k][j][c];
+                    }
+                }
+            }
+            if (!have_self) {
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) xs[c] = x[own[k] + c];
+            }
+            if (!live[k]) continue;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                V2 v;
+                v.x = alpha * acc[c].x;
+                v.y = alpha * acc[c].y;
+                if (opts & kOptShift) {
+                    v.x = v.x + beta * xs[c].x;
+                    v.y = v.y + beta * xs[c].y;
+                }
+                if (MODE != kFirst) {
+                    v.x = v.x - zv[k][c].x;
+                    v.y = v.y - zv[k][c].y;
+                }
+                st_stream(y + own[k] + c, v, opts);
+                if (MODE == kFirst) {
+                    p0 = p0 + redot(bv[k][c], xs[c]);
+                    p1 = p1 + redot(bv[k][c], v);
+                } else if (MODE == kRec) {
+                    p0 = p0 + redot(bv[k][c], v);
+                } else {
+                    p0 = p0 + redot(xs[c], xs[c]);
+                    p1 = p1 + redot(v, xs[c]);
+                }
+            }
+        }
+        __syncthreads();  // the next tile overwrites the staged CSR
+    }
+
+    __shared__ double red[2][kWavesPerBlock];
+    p0 = wave_sum(p0);
+    p1 = wave_sum(p1);
+    if (lane == 0) {
+        red[0][wave] = p0;
+        red[1][wave] = p1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = red[0][0], s1 = red[1][0];
+        for (int w = 1; w < kWavesPerBlock; ++w) {
+            s0 = s0 + red[0][w];
+            s1 = s1 + red[1][w];
+        }
+        partial[blockIdx.x] = make_double2(s0, s1);
+    }
 }
 
 }  // namespace tbk

@@ -1,0 +1,98 @@
+// Step-kernel launchers: StepArgs and the template switch from runtime
+// (width, kernel family, mode) to a kernel instantiation.  The instantiations
+// are split over four translation units (steps_f64_real.hip, ...) so that
+// they compile in parallel; tbkpm.hip calls launch_steps().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cheb_kernels.h"
+
+namespace tbk {
+
+struct StepArgs {
+    const int32_t* rp;
+    const int32_t* ci;
+    const void* hv;
+    const void* x;
+    const void* z;
+    void* y;
+    const void* bra;
+    int32_t rows, Rp;
+    double alpha, beta;
+    int32_t opts;  // kOptShift | kOptStream
+    int64_t ntiles;
+    TileOrder order;
+    int32_t xcd_split;
+    double2* partial;
+};
+
+// Launch one step; returns false if no kernel exists for the combination.
+bool launch_f64_real(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
+bool launch_f64_cplx(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
+bool launch_f32_real(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
+bool launch_f32_cplx(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
+
+// KCH = 0: cheb_step_kernel (rows of any length); KCH in {1, 2, 4}:
+// cheb_tile_kernel for rows of at most 4*KCH entries.
+template <typename VT, bool CH, int LPR, int CPL, int MODE, int KCH>
+bool launch_step(dim3 grid, hipStream_t st, const StepArgs& a) {
+    using V2 = typename Vec2<VT>::type;
+    using H = typename HVal<VT, CH>::type;
+    if constexpr (KCH == 0) {
+        hipLaunchKernelGGL((cheb_step_kernel<VT, CH, LPR, CPL, MODE>), grid, dim3(kBlock), 0, st, a.rp, a.ci,
+                           static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
+                           static_cast<V2*>(a.y), static_cast<const V2*>(a.bra), a.rows, a.Rp,
+                           static_cast<VT>(a.alpha), static_cast<VT>(a.beta), a.opts, a.ntiles, a.order, a.xcd_split,
+                           a.partial);
+        return true;
+    } else if constexpr (tile_fits(LPR, CPL, KCH, sizeof(H))) {
+        hipLaunchKernelGGL((cheb_tile_kernel<VT, CH, LPR, CPL, MODE, KCH>), grid, dim3(kBlock), 0, st, a.rp, a.ci,
+                           static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
+                           static_cast<V2*>(a.y), static_cast<const V2*>(a.bra), a.rows, a.Rp,
+                           static_cast<VT>(a.alpha), static_cast<VT>(a.beta), a.opts, a.ntiles, a.order,
+                           a.xcd_split, a.partial);
+        return true;
+    } else {
+        return false;
+    }
+}
+
+template <typename VT, bool CH, int LPR, int CPL, int KCH>
+bool launch_mode(int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    switch (mode) {
+        case kFirst: return launch_step<VT, CH, LPR, CPL, kFirst, KCH>(grid, st, a);
+        case kRec: return launch_step<VT, CH, LPR, CPL, kRec, KCH>(grid, st, a);
+        default: return launch_step<VT, CH, LPR, CPL, kRecDbl, KCH>(grid, st, a);
+    }
+}
+
+template <typename VT, bool CH, int LPR, int CPL>
+bool launch_kch(int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    switch (kch) {
+        case 0: return launch_mode<VT, CH, LPR, CPL, 0>(mode, grid, st, a);
+        case 1: return launch_mode<VT, CH, LPR, CPL, 1>(mode, grid, st, a);
+        case 2: return launch_mode<VT, CH, LPR, CPL, 2>(mode, grid, st, a);
+        case 4: return launch_mode<VT, CH, LPR, CPL, 4>(mode, grid, st, a);
+        default: return false;
+    }
+}
+
+// width: padded column count for the row kernel, slab width for the tiled one
+template <typename VT, bool CH>
+bool launch_width(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    switch (width) {
+        case 1: return launch_kch<VT, CH, 1, 1>(kch, mode, grid, st, a);
+        case 2: return launch_kch<VT, CH, 2, 1>(kch, mode, grid, st, a);
+        case 4: return launch_kch<VT, CH, 4, 1>(kch, mode, grid, st, a);
+        case 8: return launch_kch<VT, CH, 8, 1>(kch, mode, grid, st, a);
+        case 16: return launch_kch<VT, CH, 16, 1>(kch, mode, grid, st, a);
+        case 32: return launch_kch<VT, CH, 32, 1>(kch, mode, grid, st, a);
+        case 64: return launch_kch<VT, CH, 64, 1>(kch, mode, grid, st, a);
+        case 128: return launch_kch<VT, CH, 64, 2>(kch, mode, grid, st, a);
+        case 256: return launch_kch<VT, CH, 64, 4>(kch, mode, grid, st, a);
+        default: return false;
+    }
+}
+
+}  // namespace tbk

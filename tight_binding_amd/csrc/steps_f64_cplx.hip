@@ -1,0 +1,8 @@
+// Step-kernel instantiations for fp64 vectors, complex matrix values.
+#include "step_launch.h"
+
+namespace tbk {
+bool launch_f64_cplx(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    return launch_width<double, true>(width, kch, mode, grid, st, a);
+}
+}  // namespace tbk

@@ -1,0 +1,8 @@
+// Step-kernel instantiations for fp64 vectors, real matrix values.
+#include "step_launch.h"
+
+namespace tbk {
+bool launch_f64_real(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    return launch_width<double, false>(width, kch, mode, grid, st, a);
+}
+}  // namespace tbk

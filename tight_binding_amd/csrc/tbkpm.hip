@@ -21,7 +21,9 @@
 #include <string>
 #include <vector>
 
+#include "aux_kernels.h"
 #include "cheb_kernels.h"
+#include "step_launch.h"
 #include "tbkpm.h"
 
 namespace {
@@ -106,6 +108,7 @@ struct tbk_context {
     DevBuf sums;
     std::vector<hipEvent_t> events;
     int64_t held_bytes = 0;  // matrices + blocks owned through this context
+    tbk_tuning tuning{};     // zeros: automatic choices
 };
 
 struct tbk_matrix {
@@ -113,6 +116,8 @@ struct tbk_matrix {
     int32_t rows = 0;
     int64_t nnz = 0;
     bool is_complex = false;
+    int32_t max_row_nnz = 0;
+    int32_t bandwidth95 = 0;  // 95th percentile of |row - col| over the entries
     DevBuf rp, ci, val64, val32;
     bool have32 = false;
 };
@@ -137,81 +142,66 @@ void ensure_events(tbk_context* ctx, size_t n) {
 }
 
 // --------------------------------------------------------- kernel dispatch
-using StepFn = void (*)(dim3, dim3, hipStream_t, const void* const* args);
+using tbk::StepArgs;
 
-struct StepArgs {
-    const int32_t* rp;
-    const int32_t* ci;
-    const void* hv;
-    const void* x;
-    const void* z;
-    void* y;
-    const void* bra;
-    int32_t rows, Rp;
-    double alpha, beta;
-    int32_t shift;
-    int64_t ntiles;
-    int32_t xcd_split;
-    double2* partial;
+void launch_any(int precision, bool cplx, int width, int kch, int mode, dim3 grid, hipStream_t st,
+                const StepArgs& a) {
+    bool ok;
+    if (precision == TBK_FP64)
+        ok = cplx ? tbk::launch_f64_cplx(width, kch, mode, grid, st, a) : tbk::launch_f64_real(width, kch, mode, grid, st, a);
+    else
+        ok = cplx ? tbk::launch_f32_cplx(width, kch, mode, grid, st, a) : tbk::launch_f32_real(width, kch, mode, grid, st, a);
+    if (!ok) raise(TBK_E_UNSUPPORTED, "no step kernel for this block width");
+}
+
+// Kernel family, slab width and rows per tile for a matrix and block shape.
+struct KernelChoice {
+    int kch;            // 0: row kernel
+    int width;          // columns one sweep covers (the slab width; Rp for the row kernel)
+    int rows_per_tile;
 };
 
-template <typename VT, bool CH, int LPR, int CPL, int MODE>
-void launch_step(dim3 grid, hipStream_t st, const StepArgs& a) {
-    using V2 = typename tbk::Vec2<VT>::type;
-    using H = typename tbk::HVal<VT, CH>::type;
-    hipLaunchKernelGGL((tbk::cheb_step_kernel<VT, CH, LPR, CPL, MODE>), grid, dim3(tbk::kBlock), 0,
-                       st, a.rp, a.ci, static_cast<const H*>(a.hv), static_cast<const V2*>(a.x),
-                       static_cast<const V2*>(a.z), static_cast<V2*>(a.y),
-                       static_cast<const V2*>(a.bra), a.rows, a.Rp, static_cast<VT>(a.alpha),
-                       static_cast<VT>(a.beta), a.shift, a.ntiles, a.xcd_split, a.partial);
-}
-
-template <typename VT, bool CH, int LPR, int CPL>
-void launch_mode(int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
-    switch (mode) {
-        case tbk::kFirst: launch_step<VT, CH, LPR, CPL, tbk::kFirst>(grid, st, a); break;
-        case tbk::kRec: launch_step<VT, CH, LPR, CPL, tbk::kRec>(grid, st, a); break;
-        default: launch_step<VT, CH, LPR, CPL, tbk::kRecDbl>(grid, st, a); break;
+KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int Rp, int max_row_nnz,
+                           int bandwidth, bool force_row) {
+    const int hbytes = (precision == TBK_FP64 ? 8 : 4) * (cplx ? 2 : 1);
+    const int ebytes = precision == TBK_FP64 ? 16 : 8;
+    int kch = max_row_nnz <= 4 ? 1 : max_row_nnz <= 8 ? 2 : max_row_nnz <= 16 ? 4 : 0;
+    if (force_row) kch = 0;
+    int sw = Rp;
+    if (kch > 0) {
+        if (tu.slab_cols > 0) {
+            sw = 1;
+            while (sw * 2 <= std::min<int>(tu.slab_cols, Rp)) sw *= 2;
+        } else {
+            // narrowest slab of >= 16 columns (or Rp) whose gather window
+            // 3 * 2D * SW elements fits the L2 budget
+            const double budget = tu.l2_budget_bytes > 0 ? tu.l2_budget_bytes : 3.0 * 1024 * 1024;
+            while (sw > 16 && 6.0 * bandwidth * sw * ebytes > budget) sw /= 2;
+        }
+        const int lpr = std::min(sw, tbk::kWave);
+        if (!tbk::tile_fits(lpr, sw / lpr, kch, hbytes)) {
+            kch = 0;
+            sw = Rp;
+        }
     }
-}
-
-template <typename VT, bool CH>
-void launch_rp(int Rp, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
-    switch (Rp) {
-        case 1: launch_mode<VT, CH, 1, 1>(mode, grid, st, a); break;
-        case 2: launch_mode<VT, CH, 2, 1>(mode, grid, st, a); break;
-        case 4: launch_mode<VT, CH, 4, 1>(mode, grid, st, a); break;
-        case 8: launch_mode<VT, CH, 8, 1>(mode, grid, st, a); break;
-        case 16: launch_mode<VT, CH, 16, 1>(mode, grid, st, a); break;
-        case 32: launch_mode<VT, CH, 32, 1>(mode, grid, st, a); break;
-        case 64: launch_mode<VT, CH, 64, 1>(mode, grid, st, a); break;
-        case 128: launch_mode<VT, CH, 64, 2>(mode, grid, st, a); break;
-        case 256: launch_mode<VT, CH, 64, 4>(mode, grid, st, a); break;
-        default: raise(TBK_E_UNSUPPORTED, "vector block wider than 256 columns");
-    }
-}
-
-void launch_any(int precision, bool cplx, int Rp, int mode, dim3 grid, hipStream_t st,
-                const StepArgs& a) {
-    if (precision == TBK_FP64) {
-        if (cplx)
-            launch_rp<double, true>(Rp, mode, grid, st, a);
-        else
-            launch_rp<double, false>(Rp, mode, grid, st, a);
-    } else {
-        if (cplx)
-            launch_rp<float, true>(Rp, mode, grid, st, a);
-        else
-            launch_rp<float, false>(Rp, mode, grid, st, a);
-    }
-}
-
-int rows_per_tile(int Rp) {
-    const int lpr = std::min(Rp, tbk::kWave);
-    return tbk::kWavesPerBlock * (tbk::kWave / lpr);
+    const int lpr = std::min(sw, tbk::kWave);
+    const int tr = kch ? tbk::tile_rows(lpr, sw / lpr, kch, hbytes) : tbk::kWavesPerBlock * (tbk::kWave / lpr);
+    return {kch, sw, tr};
 }
 
 // ---------------------------------------------------------- matrix upload
+// p-quantile of |row - col| over the stored entries (the distance in rows at
+// which a gathered neighbour row is used again)
+int bandwidth_percentile(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, double p) {
+    if (ci.empty()) return 0;
+    std::vector<int32_t> d(ci.size());
+    for (size_t i = 0; i + 1 < rp.size(); ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) d[k] = std::abs(ci[k] - static_cast<int32_t>(i));
+    const size_t at = std::min(d.size() - 1, static_cast<size_t>(p * static_cast<double>(d.size())));
+    std::nth_element(d.begin(), d.begin() + static_cast<std::ptrdiff_t>(at), d.end());
+    return d[at];
+}
+
 void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_t>& ci,
                std::vector<double>& val) {
     const int vw = M->is_complex ? 2 : 1;
@@ -293,12 +283,36 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
     const tbk_block* bra = bras ? bras : kets;
     const int64_t steps = doubling ? (n + 1) / 2 : (n - 1);
 
-    const int TR = rows_per_tile(Rp);
-    const int64_t ntiles = (static_cast<int64_t>(rows) + TR - 1) / TR;
-    int64_t cap = static_cast<int64_t>(ctx->num_cu) * 8;
+    const KernelChoice kc = choose_kernel(ctx->tuning, prec, M->is_complex, Rp, M->max_row_nnz, M->bandwidth95,
+                                          (flags & TBK_FLAG_ROW_KERNEL) != 0 || ctx->tuning.use_tiled <= 0);
+    const int TR = kc.rows_per_tile;
+    const int64_t row_tiles = (static_cast<int64_t>(rows) + TR - 1) / TR;
+    // chunked visiting order: chunks of ~chunk_rows rows, transposed with the
+    // period P = (matrix bandwidth) / (chunk rows) when that exceeds 1
+    tbk::TileOrder order{};
+    {
+        const int64_t chunk_rows = ctx->tuning.chunk_rows > 0 ? ctx->tuning.chunk_rows : 256;
+        const int64_t tpc = std::max<int64_t>(1, chunk_rows / TR);
+        int64_t P = 1;
+        if (ctx->tuning.chunk_period > 0)
+            P = ctx->tuning.chunk_period;
+        else if (ctx->tuning.chunk_period == 0)
+            P = std::max<int64_t>(1, (M->bandwidth95 + (tpc * TR) / 2) / (tpc * TR));
+        const int64_t nchunks = (row_tiles + tpc - 1) / tpc;
+        P = std::min<int64_t>(P, nchunks);
+        const int64_t Q = (nchunks + P - 1) / P;
+        if (P * Q * tpc * (Rp / kc.width) >= (int64_t(1) << 31)) raise(TBK_E_UNSUPPORTED, "too many row tiles");
+        order.row_tiles = static_cast<uint32_t>(row_tiles);
+        order.tpc = static_cast<uint32_t>(tpc);
+        order.P = static_cast<uint32_t>(P);
+        order.Q = static_cast<uint32_t>(Q);
+    }
+    const int64_t ntiles = static_cast<int64_t>(order.P) * order.Q * order.tpc * (Rp / kc.width);
+    const int bpc = ctx->tuning.blocks_per_cu > 0 ? ctx->tuning.blocks_per_cu : 4;
+    int64_t cap = static_cast<int64_t>(ctx->num_cu) * bpc;
     int64_t nb = std::min<int64_t>(ntiles, cap);
     int xcd_split = 0;
-    if (nb >= 64) {
+    if ((nb >= 64 && ctx->tuning.xcd_split >= 0) || (ctx->tuning.xcd_split > 0 && nb >= 8)) {
         nb = (nb / 8) * 8;
         xcd_split = 1;
     }
@@ -315,8 +329,9 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
     A.hv = hv;
     A.rows = rows;
     A.Rp = Rp;
-    A.shift = (b != 0.0) ? 1 : 0;
+    A.opts = ((b != 0.0) ? tbk::kOptShift : 0) | (ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0);
     A.ntiles = ntiles;
+    A.order = order;
     A.xcd_split = xcd_split;
     double2* partial = ctx->partial.as<double2>();
 
@@ -347,7 +362,7 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         }
         A.partial = partial + static_cast<size_t>(k) * nb;
         if (time_each) HIPC(hipEventRecord(ctx->events[2 * k], ctx->stream));
-        launch_any(prec, M->is_complex, Rp, mode, grid, ctx->stream, A);
+        launch_any(prec, M->is_complex, kc.width, kc.kch, mode, grid, ctx->stream, A);
         HIPC(hipGetLastError());
         if (time_each) HIPC(hipEventRecord(ctx->events[2 * k + 1], ctx->stream));
     }
@@ -424,6 +439,10 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         stats->blocks = static_cast<int32_t>(nb);
         stats->rows_per_tile = TR;
         stats->padded_R = Rp;
+        stats->kernel_kind = kc.kch;
+        stats->slab_cols = kc.width;
+        stats->chunk_period = static_cast<int32_t>(order.P);
+        stats->chunk_rows = static_cast<int32_t>(order.tpc) * TR;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -479,6 +498,32 @@ int tbk_context_destroy(tbk_context* ctx) {
     });
 }
 
+int tbk_context_set_tuning(tbk_context* ctx, const tbk_tuning* t) {
+    return guarded([&] {
+        if (!ctx || !t) raise(TBK_E_INVALID, "null argument");
+        if (t->slab_cols < 0 || t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || t->l2_budget_bytes < 0)
+            raise(TBK_E_INVALID, "tuning value out of range");
+        ctx->tuning = *t;
+    });
+}
+
+int tbk_context_get_tuning(const tbk_context* ctx, tbk_tuning* t) {
+    return guarded([&] {
+        if (!ctx || !t) raise(TBK_E_INVALID, "null argument");
+        *t = ctx->tuning;
+    });
+}
+
+int tbk_matrix_info(const tbk_matrix* m, int32_t* rows, int64_t* nnz, int32_t* max_row_nnz, int32_t* bandwidth95) {
+    return guarded([&] {
+        if (!m) raise(TBK_E_INVALID, "null argument");
+        if (rows) *rows = m->rows;
+        if (nnz) *nnz = m->nnz;
+        if (max_row_nnz) *max_row_nnz = m->max_row_nnz;
+        if (bandwidth95) *bandwidth95 = m->bandwidth95;
+    });
+}
+
 int tbk_context_synchronize(tbk_context* ctx) {
     return guarded([&] {
         if (!ctx) raise(TBK_E_INVALID, "null argument");
@@ -511,6 +556,8 @@ int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out) {
         m->rows = M->rows;
         m->nnz = M->nnz;
         m->is_complex = M->is_complex != 0;
+        for (int32_t i = 0; i < M->rows; ++i) m->max_row_nnz = std::max(m->max_row_nnz, rp[i + 1] - rp[i]);
+        m->bandwidth95 = bandwidth_percentile(rp, ci, 0.95);
         m->rp.reserve(rp.size() * 4);
         m->ci.reserve(std::max<size_t>(ci.size() * 4, 4));
         m->val64.reserve(std::max<size_t>(val.size() * 8, 8));
@@ -634,6 +681,30 @@ int tbk_block_download(const tbk_block* b, double* colmajor_out) {
     });
 }
 
+int tbk_block_copy(const tbk_block* src, tbk_block* dst, float* kernel_ms) {
+    return guarded([&] {
+        if (!src || !dst) raise(TBK_E_INVALID, "null argument");
+        if (src->ctx != dst->ctx) raise(TBK_E_INVALID, "blocks belong to different contexts");
+        if (src->rows != dst->rows || src->Rp != dst->Rp || src->precision != dst->precision)
+            raise(TBK_E_INVALID, "block shapes differ");
+        tbk_context* ctx = src->ctx;
+        set_device(ctx);
+        const int64_t n16 = static_cast<int64_t>(static_cast<size_t>(src->rows) * src->Rp * src->elem_bytes() / 16);
+        const int64_t per_block = static_cast<int64_t>(tbk::kBlock) * tbk::kCopyUnroll;
+        if ((n16 + per_block - 1) / per_block > INT32_MAX) raise(TBK_E_UNSUPPORTED, "block too large to copy");
+        const int grid = static_cast<int>((n16 + per_block - 1) / per_block);
+        ensure_events(ctx, 2);
+        HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+        if (n16 > 0)
+            hipLaunchKernelGGL(tbk::copy16_kernel, dim3(std::max(grid, 1)), dim3(tbk::kBlock), 0, ctx->stream,
+                               src->data.as<const uint4>(), dst->data.as<uint4>(), n16);
+        HIPC(hipGetLastError());
+        HIPC(hipEventRecord(ctx->events[1], ctx->stream));
+        HIPC(hipStreamSynchronize(ctx->stream));
+        if (kernel_ms) HIPC(hipEventElapsedTime(kernel_ms, ctx->events[0], ctx->events[1]));
+    });
+}
+
 int tbk_block_destroy(tbk_block* b) {
     return guarded([&] {
         if (!b) return;
@@ -648,6 +719,64 @@ int tbk_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const
                 const tbk_block* kets, int32_t n, int32_t flags, double* mu_out, double* last_out,
                 tbk_stats* stats) {
     return guarded([&] { run_moments(ctx, M, a, b, bras, kets, n, flags, mu_out, last_out, stats); });
+}
+
+int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
+                 int32_t steps, tbk_tuning* best, double* best_ms_per_step) {
+    return guarded([&] {
+        if (!ctx || !M || !kets) raise(TBK_E_INVALID, "null argument");
+        if (steps < 2) raise(TBK_E_INVALID, "autotune needs at least 2 steps per trial");
+        const int32_t n = 2 * steps;  // two moments per step (bras = kets)
+        std::vector<double> mu(static_cast<size_t>(n));
+        struct Restore {  // a failed trial leaves the caller's tuning in place
+            tbk_context* c;
+            tbk_tuning saved;
+            bool done = false;
+            ~Restore() {
+                if (!done) c->tuning = saved;
+            }
+        } restore{ctx, ctx->tuning};
+        auto trial = [&](const tbk_tuning& t) -> double {
+            ctx->tuning = t;
+            tbk_stats st{};
+            run_moments(ctx, M, a, b, nullptr, kets, 4, 0, mu.data(), nullptr, nullptr);  // warm
+            run_moments(ctx, M, a, b, nullptr, kets, n, TBK_FLAG_TIME_LAUNCHES, mu.data(), nullptr, &st);
+            return st.kernel_ms / static_cast<double>(st.steps);
+        };
+        tbk_tuning cur = ctx->tuning;
+        double cur_ms = trial(cur);
+        // coordinate descent: each knob in turn, keep the fastest value
+        struct Knob {
+            int32_t tbk_tuning::*field;
+            std::vector<int32_t> values;
+        };
+        const std::vector<Knob> knobs = {
+            {&tbk_tuning::use_tiled, {0, 1}},
+            {&tbk_tuning::blocks_per_cu, {2, 4, 8, 16}},
+            {&tbk_tuning::chunk_rows, {64, 128, 256, 512}},
+            {&tbk_tuning::chunk_period, {0, -1}},
+            {&tbk_tuning::xcd_split, {0, -1}},
+            {&tbk_tuning::stream_nt, {1, -1}},
+        };
+        for (int pass = 0; pass < 2; ++pass) {
+            for (const auto& k : knobs) {
+                for (int32_t v : k.values) {
+                    if (cur.*(k.field) == v) continue;
+                    tbk_tuning t = cur;
+                    t.*(k.field) = v;
+                    const double ms = trial(t);
+                    if (ms < cur_ms) {
+                        cur_ms = ms;
+                        cur = t;
+                    }
+                }
+            }
+        }
+        ctx->tuning = cur;
+        restore.done = true;
+        if (best) *best = cur;
+        if (best_ms_per_step) *best_ms_per_step = cur_ms;
+    });
 }
 
 int tbk_cheb_first_kind_seq_braket(const tbk_sparse* M, int32_t n, const double* bras_T, const double* kets,

@@ -47,6 +47,11 @@ class Matrix:
         del keep
         self._h = h
 
+    def info(self) -> dict:
+        rows, nnz, mx, bw = C.c_int32(), C.c_int64(), C.c_int32(), C.c_int32()
+        N.check_kpm(N.kpm().tbk_matrix_info(self._h, C.byref(rows), C.byref(nnz), C.byref(mx), C.byref(bw)))
+        return {"rows": rows.value, "nnz": nnz.value, "max_row_nnz": mx.value, "bandwidth95": bw.value}
+
     def close(self):
         if getattr(self, "_h", None) is not None:
             N.kpm().tbk_matrix_destroy(self._h)
@@ -67,6 +72,12 @@ class Block:
         out = np.zeros(2 * self.rows * self.R, dtype=np.float64)
         N.check_kpm(N.kpm().tbk_block_download(self._h, _dp(out)))
         return out.view(np.complex128).reshape(self.R, self.rows).T
+
+    def copy_to(self, dst: "Block") -> float:
+        """dst = self on the device; returns the copy kernel's time in ms."""
+        ms = C.c_float()
+        N.check_kpm(N.kpm().tbk_block_copy(self._h, dst._h, C.byref(ms)))
+        return ms.value
 
     def close(self):
         if getattr(self, "_h", None) is not None:
@@ -108,6 +119,27 @@ class Context:
     def synchronize(self):
         N.check_kpm(N.kpm().tbk_context_synchronize(self._h))
 
+    def set_tuning(self, slab_cols=0, blocks_per_cu=0, xcd_split=0, l2_budget_bytes=0, stream_nt=0,
+                   use_tiled=0, chunk_rows=0, chunk_period=0):
+        """Launch-shape choices (0 = default)."""
+        t = N.tbk_tuning(int(slab_cols), int(blocks_per_cu), int(xcd_split), int(l2_budget_bytes),
+                         int(stream_nt), int(use_tiled), int(chunk_rows), int(chunk_period))
+        N.check_kpm(N.kpm().tbk_context_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, M: "Matrix", kets: "Block", a=1.0, b=0.0, steps=16):
+        """Search launch shapes for (M, kets); installs and returns the best
+        tuning and its time per step in ms."""
+        t = N.tbk_tuning()
+        ms = C.c_double()
+        N.check_kpm(N.kpm().tbk_autotune(self._h, M._h, float(a), float(b), kets._h, int(steps), C.byref(t),
+                                         C.byref(ms)))
+        return {k: getattr(t, k) for k, _ in t._fields_}, ms.value
+
+    def get_tuning(self) -> dict:
+        t = N.tbk_tuning()
+        N.check_kpm(N.kpm().tbk_context_get_tuning(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
     def matrix(self, m: CSC) -> Matrix:
         return Matrix(self, m)
 
@@ -127,12 +159,14 @@ class Context:
         return Block(self, h, rows, R, precision)
 
     def moments(self, M: Matrix, kets: Block, n: int, bras: Block | None = None, a: float = 1.0,
-                b: float = 0.0, plain: bool = False, time_launches: bool = False, want_last: bool = False):
+                b: float = 0.0, plain: bool = False, time_launches: bool = False, want_last: bool = False,
+                row_kernel: bool = False):
         """mu[i] = Re <bras| T_i((M - b)/a) |kets>, i < n.  Returns (mu, stats, last)."""
         mu = np.zeros(int(n), dtype=np.float64)
         last = np.zeros(2 * kets.rows * kets.R, dtype=np.float64) if want_last else None
         st = N.tbk_stats()
-        flags = (N.TBK_FLAG_PLAIN if plain else 0) | (N.TBK_FLAG_TIME_LAUNCHES if time_launches else 0)
+        flags = ((N.TBK_FLAG_PLAIN if plain else 0) | (N.TBK_FLAG_TIME_LAUNCHES if time_launches else 0)
+                 | (N.TBK_FLAG_ROW_KERNEL if row_kernel else 0))
         N.check_kpm(
             N.kpm().tbk_moments(
                 self._h, M._h, float(a), float(b), bras._h if bras is not None else None, kets._h, int(n), flags,
