@@ -1,0 +1,41 @@
+// A caller written against the reference's API (models, UnitCell,
+// PartialLattice, GreenFunction, density_of_states_seq -- the flow of
+// snippets.h:743-771 / functions.h:16-33), built on this package's headers.
+// Prints the N moments of GreenFunctionImag, then DoS at a few energies.
+//
+// usage: dropin_dos <L> <N>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "green.h"
+#include "models.h"
+
+using namespace tb;
+
+int main(int argc, char** argv) {
+    const int L = argc > 1 ? std::atoi(argv[1]) : 16;
+    const int N = argc > 2 ? std::atoi(argv[2]) : 64;
+    auto d = models::graphene();  // Honeycomb + 1_NN t = 1
+    PartialLattice<2> X{d, LatticeVector<2>{L, L}};
+    X.set_scalar_field_indexed(uniform_disorder(0.2, 99));
+    X.gauge_field = [](Vector3d r) { return 0.01 * Vector3d{-r(1), r(0), 0} / 2; };  // symmetric gauge
+
+    // moments exactly as GreenFunctionImag computes them
+    auto H = X.get_Hamiltonian_periodic_extend_bare_sparse<Complex>(LatticeVector<2>{1, 1});
+    PartialLattice<2>::GreenFunction<Complex> G(N, H, -3.5 - 1e-8, 3.5 + 1e-8);
+    numerical::linalg::seed_random_kets(0x5EED);
+    MatrixXcd kets = numerical::linalg::bases_of_uniform_ampl_with_random_phases(X.get_total_number(), 4);
+    auto f = G.imag(kets, kets);
+    for (double m : f.moments) std::printf("mu %.17g\n", m);
+
+    // the DoS driver (draws the next ket seed)
+    std::vector<double> E = {-0.5, -0.1, 0.0, 0.3, 0.7};
+    auto dos = density_of_states_seq<Complex>(X, N, E, -3.5, 3.5);
+    for (size_t i = 0; i < E.size(); ++i) std::printf("dos %.17g %.17g\n", E[i], dos[i]);
+
+    // n < 2 keeps the reference's behaviour: message and an empty result
+    auto none = cuda::Cheb_first_kind_seq_braket(G.scaled_HE, 1, kets, kets);
+    std::printf("empty %zu\n", none.size());
+    return 0;
+}

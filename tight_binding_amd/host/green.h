@@ -1,0 +1,84 @@
+// KPM front end of PartialLattice: GreenFunction<Scalar> and its
+// GreenFunctionImag functor (lattice.h:532-652 of the reference), and
+// density_of_states_seq (functions.h:15-33).
+//
+// GreenFunction(N, H, E_min, E_max) rescales H~ = (H - b I)/a with
+// a = (E_max - E_min)/2, b = (E_max + E_min)/2 over the union pattern of H
+// and I (lattice.h:553-559).  GreenFunctionImag computes the N Chebyshev
+// moments with the HIP engine (cuda::Cheb_first_kind_seq_braket, the path
+// the reference takes under USE_CUDA, lattice.h:608-612) and the Lorentz
+// kernel with lambda = 4 (lattice.h:613); operator()(e) evaluates
+// sum g_n Im(alpha_n(e)) mu_n (lattice.h:618-624).
+#pragma once
+
+#include <cmath>
+#include <utility>
+#include <vector>
+
+#include "lattice.h"
+#include "numerical.h"
+#include "numerics_cuda.cuh"
+
+namespace tb {
+
+template <int Dim>
+template <typename Scalar>
+struct PartialLattice<Dim>::GreenFunction {
+    using Mat = SparseMatrix<Scalar>;
+    using Kets = MatrixXcd;
+
+    int N;
+    double a, b;
+    double E_min, E_max;
+    Mat scaled_HE;
+
+    GreenFunction(int N, const Mat& HE, double E_min, double E_max) : N(N), E_min(E_min), E_max(E_max) {
+        a = (E_max - E_min) / 2;
+        b = (E_max + E_min) / 2;
+        scaled_HE = rescale(HE, a, b);
+    }
+    double scale(double x) const { return (x - b) / a; }
+    std::pair<double, double> spectral_range() const { return {E_min, E_max}; }
+
+    struct GreenFunctionImag {
+        GreenFunction* G;
+        std::vector<double> moments;
+        std::vector<double> g;  // kernel
+
+        GreenFunctionImag(GreenFunction* G, const Kets& bras_T, const Kets& kets) : G(G) {
+            moments = cuda::Cheb_first_kind_seq_braket(G->scaled_HE, G->N, bras_T, kets);
+            g = numerical::kernels::kernel_seq(G->N, numerical::kernels::Lorentz(G->N, 4));
+        }
+        static double weight(double x) { return 1. / std::sqrt(1. - x * x); }
+        // sum_n g_n Im(alpha_n(e)) mu_n at a rescaled energy e
+        double operator()(Complex e) const {
+            const auto alpha = numerical::Chebyshev::expn_coef_alpha_seq(e, G->N);
+            double sum = 0;
+            for (int n = 0; n < G->N && n < static_cast<int>(moments.size()); ++n)
+                sum += g[n] * alpha[n].imag() * moments[n];
+            return sum;
+        }
+    };
+    GreenFunctionImag imag(const Kets& bras_T, const Kets& kets) { return GreenFunctionImag(this, bras_T, kets); }
+};
+
+// DoS(e) = -f(e)/(R pi) at rescaled energies e (functions.h:16-33): periodic
+// H (extend 1 along every axis), bounds padded by 1e-8, R = 4 random-phase
+// vectors.
+template <typename Scalar, int Dim>
+std::vector<double> density_of_states_seq(PartialLattice<Dim>& X, int N, const std::vector<double>& E_list,
+                                          double E_min, double E_max) {
+    LatticeVector<Dim> ext;
+    for (int d = 0; d < Dim; ++d) ext(d) = 1;
+    auto H = X.template get_Hamiltonian_periodic_extend_bare_sparse<Scalar>(ext);
+    typename PartialLattice<Dim>::template GreenFunction<Scalar> G(N, H, E_min - 1e-8, E_max + 1e-8);
+    const int R = 4;
+    MatrixXcd kets = numerical::linalg::bases_of_uniform_ampl_with_random_phases(X.get_total_number(), R);
+    auto f = G.imag(kets, kets);
+    std::vector<double> dos;
+    dos.reserve(E_list.size());
+    for (double e : E_list) dos.push_back(-1.0 / R / numerical::pi * f(Complex(e, 0.0)));
+    return dos;
+}
+
+}  // namespace tb

@@ -1,0 +1,121 @@
+// Host numerics of the KPM front end (numerical.h:120-144, 304-316,
+// 383-447 of the reference): damping kernels, Chebyshev expansion
+// coefficients of the resolvent, random-phase kets and block traces.
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <numbers>
+#include <vector>
+
+#include "lattice.h"
+#include "linalg.h"
+
+namespace tb::numerical {
+
+using std::numbers::pi;
+
+namespace linalg {
+
+// Generator state behind bases_of_uniform_ampl_with_random_phases: every call
+// draws the next seed, so a program is reproducible run to run (the
+// reference draws from std::rand through Eigen's Random).
+inline uint64_t& ket_seed_state() {
+    static uint64_t s = 0x5EEDu;
+    return s;
+}
+inline void seed_random_kets(uint64_t seed) { ket_seed_state() = seed; }
+
+// kets(i, r) = e^{i 2 pi u(seed, i*m + r)} / sqrt(n)  (numerical.h:120-122)
+inline MatrixXcd bases_of_uniform_ampl_with_random_phases(int n, int m) {
+    const uint64_t seed = ket_seed_state()++;
+    MatrixXcd k(n, m);
+    const double inv = 1.0 / std::sqrt(static_cast<double>(n));
+    for (int r = 0; r < m; ++r)
+        for (int i = 0; i < n; ++i) {
+            const double u = counter_uniform(seed, static_cast<uint64_t>(i) * m + r);
+            const double th = 2.0 * pi * u;
+            k(i, r) = Complex(std::cos(th) * inv, std::sin(th) * inv);
+        }
+    return k;
+}
+
+// Re tr(bras^H kets) (numerical.h:127-135)
+inline double trace_braket(const MatrixXcd& bras_T, const MatrixXcd& kets) {
+    double t = 0.0;
+    for (int64_t i = 0; i < bras_T.rows(); ++i)
+        for (int64_t j = 0; j < bras_T.cols(); ++j)
+            t += bras_T(i, j).real() * kets(i, j).real() + bras_T(i, j).imag() * kets(i, j).imag();
+    return t;
+}
+
+}  // namespace linalg
+
+namespace Chebyshev {
+
+// 1/(z - x) = sum alpha_n(z) T_n(x)  (numerical.h:304-316)
+inline std::vector<Complex> expn_coef_alpha_seq(Complex z, int n) {
+    using namespace std::complex_literals;
+    std::vector<Complex> g(static_cast<size_t>(n));
+    const Complex s1 = std::sqrt(1.0 - z * z);
+    const Complex s2 = z - 1.0i * s1;
+    Complex sn = -2.0i / s1;
+    g[0] = -1.0i / s1;
+    for (int i = 1; i < n; ++i) {
+        sn *= s2;
+        g[i] = sn;
+    }
+    return g;
+}
+
+}  // namespace Chebyshev
+
+namespace kernels {
+
+struct Dirichlet {
+    double operator()(int) const { return 1.0; }
+};
+struct Fejer {
+    int N;
+    explicit Fejer(int N) : N(N) {}
+    double operator()(int n) const { return 1.0 - static_cast<double>(n) / N; }
+};
+struct Lorentz {
+    int N;
+    double lambda;
+    Lorentz(int N, double lambda) : N(N), lambda(lambda) {}
+    double operator()(int n) const { return std::sinh(lambda * (1.0 - static_cast<double>(n) / N)) / std::sinh(lambda); }
+};
+struct Jackson {
+    int N1;
+    explicit Jackson(int N) : N1(N + 1) {}
+    double operator()(int n) const {
+        const double s1 = pi / N1;
+        const double s2 = n * s1;
+        return ((N1 - n) * std::cos(s2) + std::sin(s2) / std::tan(s1)) / N1;
+    }
+};
+// as numerical.h:428-435: (sin(n pi/N) / (pi/N))^M (no 1/n factor)
+struct Lanczos {
+    double pi_N;
+    int M;
+    Lanczos(int N, int M) : pi_N(pi / N), M(M) {}
+    double operator()(int n) const { return std::pow(std::sin(n * pi_N) / pi_N, M); }
+};
+struct Wang_and_Zunger {
+    double alpha_N, beta;
+    Wang_and_Zunger(int N, double alpha, double beta) : alpha_N(alpha / N), beta(beta) {}
+    double operator()(int n) const { return std::exp(-std::pow(n * alpha_N, beta)); }
+};
+
+template <typename K>
+std::vector<double> kernel_seq(int N, const K& kernel) {
+    std::vector<double> g(static_cast<size_t>(N));
+    for (int n = 0; n < N; ++n) g[n] = kernel(n);
+    return g;
+}
+
+}  // namespace kernels
+
+}  // namespace tb::numerical

@@ -46,12 +46,11 @@ def jackson(N):
 
 
 def lanczos(N, M):
+    """As numerical.h:428-435: (sin(n pi/N) / (pi/N))^M -- the reference's
+    formula, without the 1/n of the textbook sigma factor."""
     n = np.arange(int(N), dtype=np.float64)
     pi_N = math.pi / N
-    with np.errstate(invalid="ignore", divide="ignore"):
-        g = np.power(np.sin(n * pi_N) / (n * pi_N), M)
-    g[0] = 1.0
-    return g
+    return np.power(np.sin(n * pi_N) / pi_N, M)
 
 
 def wang_zunger(N, alpha, beta):
