@@ -57,6 +57,9 @@ int tbh_lattice_hamiltonian(tbh_lattice* l, int32_t builder, const int32_t* exte
 int tbh_lattice_op_position(tbh_lattice* l, int32_t axis, tbh_csc** out);
 int tbh_lattice_op_velocity(tbh_lattice* l, int32_t axis, int32_t is_complex, tbh_csc** out);
 
+/* wrap caller arrays (copied) as a matrix handle: n x n column-compressed */
+int tbh_csc_create(int32_t n, int64_t nnz, const int32_t* colptr, const int32_t* rowidx, const double* values,
+                   int32_t is_complex, tbh_csc** out);
 /* H~ = (H - b I)/a (lattice.h:550-559) */
 int tbh_csc_rescale(const tbh_csc* m, double a, double b, tbh_csc** out);
 /* [X, M] for a diagonal X: x_i M_ij - M_ij x_j (snippets.h:316-317) */

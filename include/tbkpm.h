@@ -175,6 +175,44 @@ int tbk_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b,
 int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
                  int32_t steps, tbk_tuning* best, double* best_ms_per_step);
 
+/* ----------------------------------------------- Kubo-Bastin Gamma_nm */
+enum {
+    TBK_GAMMA_FP64 = 0,  /* fp64 matrix-core contraction */
+    TBK_GAMMA_BF16X3 = 1 /* split-bf16 three-term products, fp32 accumulation per chunk, fp64 across */
+};
+
+typedef struct tbk_kubo_stats {
+    double recursion_ms;   /* device time building both vector stacks */
+    double contraction_ms; /* device time of the Gamma contraction and its chunk reduction */
+    double wall_ms;
+    double flops;          /* 8 N^2 K: real flops of the complex contraction, K = sites x columns */
+    double stack_bytes;    /* bytes of the two stacks held during the call */
+    int32_t nchunks;       /* split-K chunks of the contraction */
+    int32_t reserved;
+} tbk_kubo_stats;
+
+/*
+ * Kubo-Bastin moments (snippets.h:306-342 with the double-sum definition of
+ * snippets.h:337-342; stored-block recursion numerical.h:334-344):
+ *   aL_n = g_n T_n(H~) |r>,   aR_m = g_m v_x T_m(H~) v_y |r>,
+ *   gamma[n*N + m] = sum over sites and columns of conj(aL_n) * aR_m,
+ * H~ = (H - b I)/a fused as in tbk_moments, v_x and v_y used as given (the
+ * reference builds them as X H~ - H~ X, lattice.h:338-345 / snippets.h:316-317),
+ * g (length N) the kernel weights or NULL for ones.  gamma_out: N*N complex128
+ * row-major (re, im interleaved).  kets must be fp64.  Both stacks (2 N blocks)
+ * are held in device memory during the call; TBK_E_OOM if they do not fit.
+ */
+int tbk_kubo_gamma(tbk_context* ctx, const tbk_matrix* H, double a, double b, const tbk_matrix* vx,
+                   const tbk_matrix* vy, const tbk_block* kets, int32_t N, int32_t gamma_precision,
+                   const double* g, double* gamma_out, tbk_kubo_stats* stats);
+
+/*
+ * The contraction alone: gamma[n*N + m] = sum_k conj(A[n*K + k]) * B[m*K + k]
+ * for host arrays A, B of N x K complex128 (interleaved).
+ */
+int tbk_stack_contract(tbk_context* ctx, const double* A, const double* B, int32_t N, int64_t K,
+                       int32_t gamma_precision, double* gamma_out, double* contraction_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,25 @@ class tbk_tuning(C.Structure):
     ]
 
 
+class tbk_kubo_stats(C.Structure):
+    _fields_ = [
+        ("recursion_ms", C.c_double),
+        ("contraction_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("flops", C.c_double),
+        ("stack_bytes", C.c_double),
+        ("nchunks", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+TBK_GAMMA_FP64 = 0
+TBK_GAMMA_BF16X3 = 1
+
+
 def _load(name):
     path = os.path.join(_HERE, name)
     if not os.path.exists(path):
@@ -123,6 +142,8 @@ KPM_SIGNATURES = {
     "tbk_block_destroy": ([P], I32),
     "tbk_moments": ([P, P, D, D, P, P, I32, I32, DP, DP, C.POINTER(tbk_stats)], I32),
     "tbk_autotune": ([P, P, D, D, P, I32, C.POINTER(tbk_tuning), DP], I32),
+    "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
+    "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
 }
 
 HOST_SIGNATURES = {
@@ -144,6 +165,7 @@ HOST_SIGNATURES = {
     "tbh_lattice_hamiltonian": ([P, I32, IP, I32, PP], I32),
     "tbh_lattice_op_position": ([P, I32, PP], I32),
     "tbh_lattice_op_velocity": ([P, I32, I32, PP], I32),
+    "tbh_csc_create": ([I32, I64, IP, IP, DP, I32, PP], I32),
     "tbh_csc_rescale": ([P, D, D, PP], I32),
     "tbh_csc_commutator_diag": ([P, P, PP], I32),
     "tbh_csc_info": ([P, IP, C.POINTER(I64), IP], I32),

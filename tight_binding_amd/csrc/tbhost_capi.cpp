@@ -281,6 +281,34 @@ int tbh_lattice_op_velocity(tbh_lattice* l, int32_t axis, int32_t is_complex, tb
     });
 }
 
+int tbh_csc_create(int32_t n, int64_t nnz, const int32_t* colptr, const int32_t* rowidx, const double* values,
+                   int32_t is_complex, tbh_csc** out) {
+    return guarded([&] {
+        if (n < 0 || nnz < 0 || !colptr || !out || (nnz > 0 && (!rowidx || !values))) bad("invalid argument");
+        if (colptr[0] != 0 || colptr[n] != nnz) bad("column pointers inconsistent with nnz");
+        for (int32_t j = 0; j < n; ++j)
+            if (colptr[j + 1] < colptr[j]) bad("column pointers not monotone");
+        for (int64_t p = 0; p < nnz; ++p)
+            if (rowidx[p] < 0 || rowidx[p] >= n) bad("row index out of range");
+        auto fill = [&](auto& M) {
+            using S = typename std::remove_reference_t<decltype(M)>::value_type;
+            M.outer().assign(colptr, colptr + n + 1);
+            M.inner().assign(rowidx, rowidx + nnz);
+            M.values().resize(static_cast<size_t>(nnz));
+            std::memcpy(M.values().data(), values, static_cast<size_t>(nnz) * sizeof(S));
+        };
+        if (is_complex) {
+            SparseMatrix<Complex> M(n, n);
+            fill(M);
+            *out = new tbh_csc{std::move(M)};
+        } else {
+            SparseMatrix<double> M(n, n);
+            fill(M);
+            *out = new tbh_csc{std::move(M)};
+        }
+    });
+}
+
 int tbh_csc_rescale(const tbh_csc* m, double a, double b, tbh_csc** out) {
     return guarded([&] {
         if (!m || !out) bad("invalid argument");

@@ -23,6 +23,7 @@
 
 #include "aux_kernels.h"
 #include "cheb_kernels.h"
+#include "gamma_launch.h"
 #include "step_launch.h"
 #include "tbkpm.h"
 
@@ -256,35 +257,25 @@ void matrix_fp32(tbk_matrix* m) {
 }
 
 // ------------------------------------------------------------ the driver
-void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* bras,
-                 const tbk_block* kets, int32_t n, int32_t flags, double* mu_out, double* last_out,
-                 tbk_stats* stats) {
-    auto t0 = std::chrono::steady_clock::now();
-    if (!ctx || !M || !kets || !mu_out) raise(TBK_E_INVALID, "null argument");
-    if (n < 2) raise(TBK_E_INVALID, "n must be greater than or equal to 2");
-    if (!(a != 0.0) || !std::isfinite(a) || !std::isfinite(b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
-    if (M->ctx != ctx || kets->ctx != ctx || (bras && bras->ctx != ctx))
-        raise(TBK_E_INVALID, "objects belong to another context");
-    if (kets->rows != M->rows) raise(TBK_E_INVALID, "kets rows do not match the matrix");
-    if (bras && (bras->rows != kets->rows || bras->R != kets->R || bras->precision != kets->precision))
-        raise(TBK_E_INVALID, "bras shape or precision does not match kets");
-    set_device(ctx);
+// Launch plan of one step kernel for a matrix and a block shape: kernel
+// family, slab width, visiting order and grid.  Shared by every recursion
+// driver (moments, Kubo stacks).
+struct StepPlan {
+    KernelChoice kc;
+    StepArgs args;  // x, z, y, bra, alpha, beta, partial filled per launch
+    int64_t nb;
+    int precision;
+    bool cplx;
+};
 
-    const int prec = kets->precision;
+StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row) {
+    StepPlan pl{};
+    pl.precision = prec;
+    pl.cplx = M->is_complex;
     if (prec == TBK_FP32) matrix_fp32(const_cast<tbk_matrix*>(M));
-    const void* hv = prec == TBK_FP64 ? M->val64.p : M->val32.p;
-    const int Rp = kets->Rp;
-    const int rows = kets->rows;
-    const size_t block_bytes = static_cast<size_t>(rows) * Rp * kets->elem_bytes();
-    ctx->work[0].reserve(block_bytes);
-    ctx->work[1].reserve(block_bytes);
-
-    const bool doubling = (bras == nullptr || bras == kets) && !(flags & TBK_FLAG_PLAIN);
-    const tbk_block* bra = bras ? bras : kets;
-    const int64_t steps = doubling ? (n + 1) / 2 : (n - 1);
-
-    const KernelChoice kc = choose_kernel(ctx->tuning, prec, M->is_complex, Rp, M->max_row_nnz, M->bandwidth95,
-                                          (flags & TBK_FLAG_ROW_KERNEL) != 0 || ctx->tuning.use_tiled <= 0);
+    pl.kc = choose_kernel(ctx->tuning, prec, M->is_complex, Rp, M->max_row_nnz, M->bandwidth95,
+                          force_row || ctx->tuning.use_tiled <= 0);
+    const KernelChoice& kc = pl.kc;
     const int TR = kc.rows_per_tile;
     const int64_t row_tiles = (static_cast<int64_t>(rows) + TR - 1) / TR;
     // chunked visiting order: chunks of ~chunk_rows rows, transposed with the
@@ -316,54 +307,86 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         nb = (nb / 8) * 8;
         xcd_split = 1;
     }
-    nb = std::max<int64_t>(nb, 1);
-    ctx->partial.reserve(static_cast<size_t>(steps) * nb * sizeof(double2));
-    ctx->sums.reserve(static_cast<size_t>(steps) * sizeof(double2));
-
-    const bool time_each = (flags & TBK_FLAG_TIME_LAUNCHES) != 0;
-    ensure_events(ctx, time_each ? static_cast<size_t>(2 * steps) : 2);
-
-    StepArgs A{};
+    pl.nb = std::max<int64_t>(nb, 1);
+    StepArgs& A = pl.args;
     A.rp = M->rp.as<const int32_t>();
     A.ci = M->ci.as<const int32_t>();
-    A.hv = hv;
+    A.hv = prec == TBK_FP64 ? M->val64.p : M->val32.p;
     A.rows = rows;
     A.Rp = Rp;
     A.opts = ((b != 0.0) ? tbk::kOptShift : 0) | (ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0);
     A.ntiles = ntiles;
     A.order = order;
     A.xcd_split = xcd_split;
+    return pl;
+}
+
+// One step: y = alpha*(M x) + beta*x - z (mode kFirst: no z), partial sums of
+// the mode into partial[0 .. plan.nb).
+void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
+                 const void* z, void* y, const void* bra, double2* partial) {
+    StepArgs A = pl.args;
+    A.alpha = alpha;
+    A.beta = beta;
+    A.x = x;
+    A.z = z;
+    A.y = y;
+    A.bra = bra;
+    A.partial = partial;
+    launch_any(pl.precision, pl.cplx, pl.kc.width, pl.kc.kch, mode, dim3(static_cast<unsigned>(pl.nb)), st, A);
+    HIPC(hipGetLastError());
+}
+
+void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* bras,
+                 const tbk_block* kets, int32_t n, int32_t flags, double* mu_out, double* last_out,
+                 tbk_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !M || !kets || !mu_out) raise(TBK_E_INVALID, "null argument");
+    if (n < 2) raise(TBK_E_INVALID, "n must be greater than or equal to 2");
+    if (!(a != 0.0) || !std::isfinite(a) || !std::isfinite(b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
+    if (M->ctx != ctx || kets->ctx != ctx || (bras && bras->ctx != ctx))
+        raise(TBK_E_INVALID, "objects belong to another context");
+    if (kets->rows != M->rows) raise(TBK_E_INVALID, "kets rows do not match the matrix");
+    if (bras && (bras->rows != kets->rows || bras->R != kets->R || bras->precision != kets->precision))
+        raise(TBK_E_INVALID, "bras shape or precision does not match kets");
+    set_device(ctx);
+
+    const int prec = kets->precision;
+    const int Rp = kets->Rp;
+    const int rows = kets->rows;
+    const size_t block_bytes = static_cast<size_t>(rows) * Rp * kets->elem_bytes();
+    ctx->work[0].reserve(block_bytes);
+    ctx->work[1].reserve(block_bytes);
+
+    const bool doubling = (bras == nullptr || bras == kets) && !(flags & TBK_FLAG_PLAIN);
+    const tbk_block* bra = bras ? bras : kets;
+    const int64_t steps = doubling ? (n + 1) / 2 : (n - 1);
+
+    const StepPlan plan = make_plan(ctx, M, prec, Rp, rows, b, (flags & TBK_FLAG_ROW_KERNEL) != 0);
+    const KernelChoice& kc = plan.kc;
+    const int TR = kc.rows_per_tile;
+    const tbk::TileOrder order = plan.args.order;
+    const int64_t nb = plan.nb;
+    ctx->partial.reserve(static_cast<size_t>(steps) * nb * sizeof(double2));
+    ctx->sums.reserve(static_cast<size_t>(steps) * sizeof(double2));
+
+    const bool time_each = (flags & TBK_FLAG_TIME_LAUNCHES) != 0;
+    ensure_events(ctx, time_each ? static_cast<size_t>(2 * steps) : 2);
     double2* partial = ctx->partial.as<double2>();
 
     auto buf = [&](int64_t k) -> void* {  // storage of T_k
         if (k == 0) return kets->data.p;
         return ctx->work[(k & 1) ? 1 : 0].p;
     };
-    const dim3 grid(static_cast<unsigned>(nb));
     if (!time_each) HIPC(hipEventRecord(ctx->events[0], ctx->stream));
     for (int64_t k = 0; k < steps; ++k) {
-        int mode;
-        if (k == 0) {
-            mode = tbk::kFirst;
-            A.alpha = 1.0 / a;
-            A.beta = -b / a;
-            A.x = buf(0);
-            A.z = nullptr;
-            A.y = buf(1);
-            A.bra = bra->data.p;
-        } else {
-            mode = doubling ? tbk::kRecDbl : tbk::kRec;
-            A.alpha = 2.0 / a;
-            A.beta = -2.0 * b / a;
-            A.x = buf(k);
-            A.z = buf(k - 1);
-            A.y = buf(k + 1);
-            A.bra = bra->data.p;
-        }
-        A.partial = partial + static_cast<size_t>(k) * nb;
         if (time_each) HIPC(hipEventRecord(ctx->events[2 * k], ctx->stream));
-        launch_any(prec, M->is_complex, kc.width, kc.kch, mode, grid, ctx->stream, A);
-        HIPC(hipGetLastError());
+        if (k == 0)
+            launch_plan(plan, ctx->stream, tbk::kFirst, 1.0 / a, -b / a, buf(0), nullptr, buf(1), bra->data.p,
+                        partial);
+        else
+            launch_plan(plan, ctx->stream, doubling ? tbk::kRecDbl : tbk::kRec, 2.0 / a, -2.0 * b / a, buf(k),
+                        buf(k - 1), buf(k + 1), bra->data.p, partial + static_cast<size_t>(k) * nb);
         if (time_each) HIPC(hipEventRecord(ctx->events[2 * k + 1], ctx->stream));
     }
     if (!time_each) HIPC(hipEventRecord(ctx->events[1], ctx->stream));
@@ -443,6 +466,132 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         stats->slab_cols = kc.width;
         stats->chunk_period = static_cast<int32_t>(order.P);
         stats->chunk_rows = static_cast<int32_t>(order.tpc) * TR;
+        stats->wall_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
+
+// ------------------------------------------------------ Kubo-Bastin Gamma
+// Split-K shape of the contraction: enough workgroups to fill the device,
+// chunks a multiple of 16 complex columns.
+void contraction_shape(const tbk_context* ctx, int N, int64_t K, int64_t& kc, int& nchunks) {
+    const int64_t tiles = static_cast<int64_t>((N + tbk::kGammaTile - 1) / tbk::kGammaTile);
+    const int64_t want = std::max<int64_t>(1, (4 * static_cast<int64_t>(ctx->num_cu) + tiles * tiles - 1) / (tiles * tiles));
+    kc = (K + want - 1) / want;
+    kc = std::max<int64_t>(256, ((kc + 15) / 16) * 16);
+    nchunks = static_cast<int>((K + kc - 1) / kc);
+}
+
+// gamma = A^H B on the device (A, B: N x K complex128), into host gamma_out.
+double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int N, int64_t K, int prec,
+                       double* gamma_out, int* nchunks_out) {
+    if (prec != TBK_GAMMA_FP64 && prec != TBK_GAMMA_BF16X3) raise(TBK_E_INVALID, "unknown contraction precision");
+    int64_t kc;
+    int nchunks;
+    contraction_shape(ctx, N, K, kc, nchunks);
+    const int64_t NN = static_cast<int64_t>(N) * N;
+    DevBuf P, out;
+    P.reserve(static_cast<size_t>(nchunks) * NN * sizeof(double2));
+    out.reserve(static_cast<size_t>(NN) * sizeof(double2));
+    ensure_events(ctx, 2);
+    HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+    if (prec == TBK_GAMMA_FP64)
+        tbk::launch_gamma_f64(A, B, N, K, kc, nchunks, P.as<double2>(), ctx->stream);
+    else
+        tbk::launch_gamma_bf16x3(A, B, N, K, kc, nchunks, P.as<double2>(), ctx->stream);
+    HIPC(hipGetLastError());
+    tbk::launch_gamma_reduce(P.as<const double2>(), nchunks, NN, out.as<double2>(), ctx->stream);
+    HIPC(hipGetLastError());
+    HIPC(hipEventRecord(ctx->events[1], ctx->stream));
+    HIPC(hipMemcpyAsync(gamma_out, out.p, static_cast<size_t>(NN) * sizeof(double2), hipMemcpyDeviceToHost,
+                        ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+    if (nchunks_out) *nchunks_out = nchunks;
+    return ms;
+}
+
+void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const tbk_matrix* vx, const tbk_matrix* vy,
+              const tbk_block* kets, int32_t N, int32_t prec, const double* g, double* gamma_out,
+              tbk_kubo_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !H || !vx || !vy || !kets || !gamma_out) raise(TBK_E_INVALID, "null argument");
+    if (N < 1) raise(TBK_E_INVALID, "N must be at least 1");
+    if (!(a != 0.0) || !std::isfinite(a) || !std::isfinite(b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
+    if (H->ctx != ctx || vx->ctx != ctx || vy->ctx != ctx || kets->ctx != ctx)
+        raise(TBK_E_INVALID, "objects belong to another context");
+    if (kets->precision != TBK_FP64) raise(TBK_E_INVALID, "Kubo stacks are built from fp64 kets");
+    if (H->rows != kets->rows || vx->rows != kets->rows || vy->rows != kets->rows)
+        raise(TBK_E_INVALID, "matrix and kets rows differ");
+    set_device(ctx);
+    const int rows = kets->rows, Rp = kets->Rp;
+    const int64_t K = static_cast<int64_t>(rows) * Rp;  // complex columns of one stacked block
+    const size_t blk_bytes = static_cast<size_t>(K) * sizeof(double2);
+    const size_t stack_bytes = static_cast<size_t>(N) * blk_bytes;
+    {
+        size_t free_b = 0, total_b = 0;
+        HIPC(hipMemGetInfo(&free_b, &total_b));
+        const double need = 2.0 * static_cast<double>(stack_bytes) + 2.0 * blk_bytes;
+        if (need > 0.95 * static_cast<double>(free_b))
+            raise(TBK_E_OOM, "Kubo stacks need " + std::to_string(need / 1e9) + " GB, " +
+                                 std::to_string(static_cast<double>(free_b) / 1e9) + " GB free");
+    }
+    DevBuf L, Rs;
+    L.reserve(stack_bytes);
+    Rs.reserve(stack_bytes);
+    ctx->work[0].reserve(blk_bytes);
+    ctx->work[1].reserve(blk_bytes);
+    const StepPlan pH = make_plan(ctx, H, TBK_FP64, Rp, rows, b, false);
+    const StepPlan pX = make_plan(ctx, vx, TBK_FP64, Rp, rows, 0.0, false);
+    const StepPlan pY = make_plan(ctx, vy, TBK_FP64, Rp, rows, 0.0, false);
+    ctx->partial.reserve(static_cast<size_t>(std::max(pH.nb, std::max(pX.nb, pY.nb))) * sizeof(double2));
+    double2* part = ctx->partial.as<double2>();
+    auto Lk = [&](int64_t k) { return static_cast<char*>(L.p) + static_cast<size_t>(k) * blk_bytes; };
+    auto Rk = [&](int64_t k) { return static_cast<char*>(Rs.p) + static_cast<size_t>(k) * blk_bytes; };
+    auto W = [&](int64_t k) { return ctx->work[k & 1].p; };
+
+    ensure_events(ctx, 2);
+    HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+    // left stack: T_n(H~)|r>
+    HIPC(hipMemcpyAsync(Lk(0), kets->data.p, blk_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (N > 1) launch_plan(pH, ctx->stream, tbk::kFirst, 1.0 / a, -b / a, Lk(0), nullptr, Lk(1), Lk(0), part);
+    for (int64_t n = 1; n + 1 < N; ++n)
+        launch_plan(pH, ctx->stream, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, Lk(n), Lk(n - 1), Lk(n + 1), nullptr, part);
+    // right stack: v_x T_m(H~) v_y|r>, T_m in two rotating work blocks
+    launch_plan(pY, ctx->stream, tbk::kFirst, 1.0, 0.0, kets->data.p, nullptr, W(0), kets->data.p, part);
+    launch_plan(pX, ctx->stream, tbk::kFirst, 1.0, 0.0, W(0), nullptr, Rk(0), W(0), part);
+    if (N > 1) {
+        launch_plan(pH, ctx->stream, tbk::kFirst, 1.0 / a, -b / a, W(0), nullptr, W(1), W(0), part);
+        launch_plan(pX, ctx->stream, tbk::kFirst, 1.0, 0.0, W(1), nullptr, Rk(1), W(1), part);
+    }
+    for (int64_t m = 2; m < N; ++m) {
+        launch_plan(pH, ctx->stream, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part);
+        launch_plan(pX, ctx->stream, tbk::kFirst, 1.0, 0.0, W(m), nullptr, Rk(m), W(m), part);
+    }
+    HIPC(hipEventRecord(ctx->events[1], ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    float rec_ms = 0.f;
+    HIPC(hipEventElapsedTime(&rec_ms, ctx->events[0], ctx->events[1]));
+
+    int nchunks = 0;
+    const double con_ms = contract_stacks(ctx, static_cast<const double2*>(L.p), static_cast<const double2*>(Rs.p),
+                                          N, K, prec, gamma_out, &nchunks);
+    if (g) {
+        for (int64_t n = 0; n < N; ++n)
+            for (int64_t m = 0; m < N; ++m) {
+                const double w = g[n] * g[m];
+                gamma_out[2 * (n * N + m)] *= w;
+                gamma_out[2 * (n * N + m) + 1] *= w;
+            }
+    }
+    if (stats) {
+        stats->recursion_ms = rec_ms;
+        stats->contraction_ms = con_ms;
+        stats->flops = 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
+        stats->stack_bytes = 2.0 * static_cast<double>(stack_bytes);
+        stats->nchunks = nchunks;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -776,6 +925,30 @@ int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         restore.done = true;
         if (best) *best = cur;
         if (best_ms_per_step) *best_ms_per_step = cur_ms;
+    });
+}
+
+int tbk_kubo_gamma(tbk_context* ctx, const tbk_matrix* H, double a, double b, const tbk_matrix* vx,
+                   const tbk_matrix* vy, const tbk_block* kets, int32_t N, int32_t gamma_precision,
+                   const double* g, double* gamma_out, tbk_kubo_stats* stats) {
+    return guarded([&] { run_kubo(ctx, H, a, b, vx, vy, kets, N, gamma_precision, g, gamma_out, stats); });
+}
+
+int tbk_stack_contract(tbk_context* ctx, const double* A, const double* B, int32_t N, int64_t K,
+                       int32_t gamma_precision, double* gamma_out, double* contraction_ms) {
+    return guarded([&] {
+        if (!ctx || !A || !B || !gamma_out) raise(TBK_E_INVALID, "null argument");
+        if (N < 1 || K < 1) raise(TBK_E_INVALID, "N and K must be positive");
+        set_device(ctx);
+        const size_t bytes = static_cast<size_t>(N) * static_cast<size_t>(K) * sizeof(double2);
+        DevBuf dA, dB;
+        dA.reserve(bytes);
+        dB.reserve(bytes);
+        HIPC(hipMemcpy(dA.p, A, bytes, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(dB.p, B, bytes, hipMemcpyHostToDevice));
+        const double ms = contract_stacks(ctx, dA.as<const double2>(), dB.as<const double2>(), N, K,
+                                          gamma_precision, gamma_out, nullptr);
+        if (contraction_ms) *contraction_ms = ms;
     });
 }
 

@@ -178,6 +178,37 @@ class Context:
         return mu, st.as_dict(), last
 
 
+GAMMA_FP64 = N.TBK_GAMMA_FP64
+GAMMA_BF16X3 = N.TBK_GAMMA_BF16X3
+
+
+def kubo_gamma(ctx: Context, H: Matrix, vx: Matrix, vy: Matrix, kets: Block, n: int, a: float = 1.0,
+               b: float = 0.0, precision: int = GAMMA_FP64, g=None):
+    """Gamma[n, m] = sum conj(g_n T_n(H~)|r>) * (g_m v_x T_m(H~) v_y|r>) over sites
+    and vectors (tbkpm.h tbk_kubo_gamma).  Returns (Gamma (n, n) complex, stats)."""
+    out = np.zeros(2 * n * n, dtype=np.float64)
+    gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+    st = N.tbk_kubo_stats()
+    N.check_kpm(N.kpm().tbk_kubo_gamma(ctx._h, H._h, float(a), float(b), vx._h, vy._h, kets._h, int(n),
+                                       int(precision), _dp(gg) if gg is not None else None, _dp(out),
+                                       C.byref(st)))
+    return out.view(np.complex128).reshape(n, n), st.as_dict()
+
+
+def stack_contract(ctx: Context, A: np.ndarray, B: np.ndarray, precision: int = GAMMA_FP64):
+    """Gamma = conj(A) @ B.T for A, B of shape (N, K) complex, on matrix cores.
+    Returns (Gamma, milliseconds)."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    B = np.ascontiguousarray(B, dtype=np.complex128)
+    assert A.shape == B.shape and A.ndim == 2
+    n, k = A.shape
+    out = np.zeros(2 * n * n, dtype=np.float64)
+    ms = C.c_double()
+    N.check_kpm(N.kpm().tbk_stack_contract(ctx._h, _dp(A.view(np.float64)), _dp(B.view(np.float64)), n, k,
+                                           int(precision), _dp(out), C.byref(ms)))
+    return out.view(np.complex128).reshape(n, n), ms.value
+
+
 def cheb_first_kind_seq_braket(M: CSC, n: int, bras_T: np.ndarray, kets: np.ndarray) -> np.ndarray:
     """Drop-in of cuda::Cheb_first_kind_seq_braket: M already rescaled (CSC),
     bras_T and kets (rows, R) complex; passing the same array object for both

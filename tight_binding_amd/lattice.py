@@ -43,6 +43,44 @@ def _csc_from_handle(h) -> CSC:
         lib.tbh_csc_destroy(h)
 
 
+def _csc_handle(m):
+    lib = N.host()
+    cp = np.ascontiguousarray(m.colptr, dtype=np.int32)
+    ri = np.ascontiguousarray(m.rowidx, dtype=np.int32)
+    cplx = np.iscomplexobj(m.values)
+    v = np.ascontiguousarray(m.values, dtype=np.complex128 if cplx else np.float64)
+    vv = v.view(np.float64) if cplx else v
+    h = C.c_void_p()
+    N.check_host(lib.tbh_csc_create(int(m.n), int(ri.size), _ip(cp), _ip(ri), _dp(vv), int(cplx), C.byref(h)))
+    return h
+
+
+def rescale(m, a, b) -> CSC:
+    """H~ = (H - b I)/a over the union pattern of H and I (lattice.h:550-559)."""
+    lib = N.host()
+    h = _csc_handle(m)
+    try:
+        h2 = C.c_void_p()
+        N.check_host(lib.tbh_csc_rescale(h, float(a), float(b), C.byref(h2)))
+    finally:
+        lib.tbh_csc_destroy(h)
+    return _csc_from_handle(h2)
+
+
+def commutator_diag(X, M) -> CSC:
+    """X M - M X for a diagonal real X: x_i M_ij - M_ij x_j on M's pattern --
+    the velocity operator v = [x, H~] of snippets.h:316-317."""
+    lib = N.host()
+    hx, hm = _csc_handle(X), _csc_handle(M)
+    try:
+        h2 = C.c_void_p()
+        N.check_host(lib.tbh_csc_commutator_diag(hx, hm, C.byref(h2)))
+    finally:
+        lib.tbh_csc_destroy(hx)
+        lib.tbh_csc_destroy(hm)
+    return _csc_from_handle(h2)
+
+
 class UnitCell:
     def __init__(self, dim, lattice_vectors, atoms, _handle=None):
         lib = N.host()

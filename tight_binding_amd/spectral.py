@@ -116,3 +116,48 @@ def linspace(start, end, n, endpoint=False):
     """utils.h:15-21: endpoint excluded by default."""
     step = (end - start) / n if not endpoint else (end - start) / (n - 1)
     return start + step * np.arange(int(n))
+
+
+# ------------------------------------------------------------ Kubo-Bastin
+def _kubo_vectors(N, e):
+    """u, v, p, q of zeta(e) = u^T G v + p^T G q (the separable form of the
+    double sum of snippets.h:343-349, 406-415 with a_0 = 1, a_{n>0} = 2)."""
+    e = np.atleast_1d(np.asarray(e, dtype=np.float64))
+    n = np.arange(N, dtype=np.float64)[:, None]
+    ad = np.where(n == 0, 1.0, 2.0)
+    th = np.arccos(e)[None, :]
+    s = np.sqrt(1.0 - e * e)[None, :]
+    T = np.cos(n * th)
+    u = ad * T
+    v = ad * (e[None, :] - 1j * n * s) * np.exp(1j * n * th)
+    p = ad * (e[None, :] + 1j * n * s) * np.exp(-1j * n * th)
+    return u, v, p, T * ad
+
+
+def kubo_zeta(gamma, e, volume=1.0):
+    """zeta(e) = sum_ij Gamma_ij g_mn(e, i, j) / (1 - e^2)^2 / Volume at rescaled
+    energies e (snippets.h:400-405; no 1/R and no e^2/h prefactor, as there)."""
+    gamma = np.asarray(gamma)
+    N = gamma.shape[0]
+    e = np.atleast_1d(np.asarray(e, dtype=np.float64))
+    u, v, p, q = _kubo_vectors(N, e)
+    z = np.sum(u * (gamma @ v), axis=0) + np.sum(p * (gamma @ q), axis=0)
+    return z / (1.0 - e * e) ** 2 / volume
+
+
+def kubo_sigma(gamma, e_fermi, volume=1.0, trim=1.0 / 24, points_per_moment=8):
+    """sigma(mu) = integral of Re zeta from the lower edge of the trimmed
+    spectral interval to mu, for rescaled Fermi energies mu (T = 0).  The
+    interval edges e = +-cos(trim*pi) are cut as the reference cuts N/24
+    Chebyshev nodes at each end (snippets.h:421-426), where (1 - e^2)^-2
+    diverges.  Trapezoid rule on a uniform grid in theta = arccos(e).
+    Returns (sigma at e_fermi, the grid e, Re zeta on the grid)."""
+    gamma = np.asarray(gamma)
+    N = gamma.shape[0]
+    M = max(64, points_per_moment * N)
+    th = np.linspace(math.pi * (1 - trim), math.pi * trim, M)  # e ascending
+    e = np.cos(th)
+    zr = np.real(kubo_zeta(gamma, e, volume))
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (zr[1:] + zr[:-1]) * np.diff(e))])
+    ef = np.atleast_1d(np.asarray(e_fermi, dtype=np.float64))
+    return np.interp(ef, e, cum), e, zr
