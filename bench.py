@@ -49,6 +49,77 @@ def build_c2(L=1024, t=-2.7, W=0.5, seed=2026):
     return H, a
 
 
+def build_c3(L=512, W=2.0, flux=1.0 / 64, seed=2027):
+    """Square L x L open, |t| = 1, Landau-gauge field of `flux` quanta per
+    plaquette, on-site disorder uniform in [-W/2, W/2]; complex H and the
+    position operators X, Y."""
+    from tight_binding_amd import lattice as Lm
+
+    cell = Lm.UnitCell.model("square")
+    cell.add_hopping_nn("A", 1, 1.0)
+    lat = Lm.PartialLattice(cell, (L, L))
+    lat.set_gauge(Lm.GAUGE_LANDAU, 2 * math.pi * flux)
+    lat.set_disorder(W, seed)
+    H = lat.get_Hamiltonian_sparse(is_complex=True)
+    return H, lat.op_position(0), lat.op_position(1), 4.0 + W / 2
+
+
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "bf16": 2500.0}
+
+
+def kubo_leg(ctx, cells, N, R, repeats):
+    """sigma_xy solves of the Kubo-Bastin workload with H, v_x, v_y resident:
+    each solve = random vectors, both stacks, the Gamma contraction, zeta at
+    the N Chebyshev nodes (device) and the integral to 201 Fermi energies."""
+    from tight_binding_amd import kpm
+    from tight_binding_amd import kubo
+
+    t0 = time.perf_counter()
+    H, X, Y, emax = build_c3(cells)
+    solver = kubo.KuboSolver(ctx, H, X, Y, -1.02 * emax, 1.02 * emax, volume=float(H.n), lam=6.0)
+    log("kubo: assembled and uploaded n=%d nnz=%d a=%.4f in %.1f s" % (H.n, H.nnz, solver.a, time.perf_counter() - t0))
+    ef = np.linspace(-4.0, 4.0, 201)
+    solver.solve(16, R=R, seed=1, fermi_energies=ef)  # loads the kernels
+    out = {
+        "workload": "Kubo-Bastin sigma_xy, square %dx%d open (%d sites), Landau flux 1/64 per plaquette, W=2, "
+                    "R=%d, N=%d, Lorentz lambda=6, 201 Fermi energies in [-4, 4]" % (cells, cells, H.n, R, N),
+        "sites": H.n, "R": R, "N": N, "solves_per_precision": repeats,
+    }
+    sigmas = {}
+    for prec, name, unit in ((kpm.GAMMA_FP64, "fp64", "fp64"), (kpm.GAMMA_BF16X3, "bf16x3", "bf16")):
+        lat, rec_ms, con_ms, sig_s = [], [], [], []
+        for r in range(repeats):
+            res = solver.solve(N, R=R, seed=100 + r, precision=prec, fermi_energies=ef)
+            st = res["stats"]
+            lat.append(res["timings"]["total_s"])
+            rec_ms.append(st["recursion_ms"])
+            con_ms.append(st["contraction_ms"])
+            sig_s.append(res["timings"]["sigma_s"])
+        sigmas[name] = res["sigma"]
+        flops = st["flops"] * (3.0 if name == "bf16x3" else 1.0)  # split-bf16: three products per real product
+        c = float(np.median(con_ms))
+        r_ms = float(np.median(rec_ms))
+        out[name] = {
+            "latency_s_p50": float(np.percentile(lat, 50)),
+            "latency_s_p99": float(np.percentile(lat, 99)),
+            "latency_s_all": lat,
+            "recursion_ms": r_ms,
+            "recursion_GBps": st["recursion_bytes"] / (r_ms * 1e-3) / 1e9,
+            "recursion_frac_hbm": st["recursion_bytes"] / (r_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "contraction_ms": c,
+            "contraction_TFLOPs_issued": flops / (c * 1e-3) / 1e12,
+            "mfma_frac": flops / (c * 1e-3) / 1e12 / MFMA_PEAK_TFLOPS[unit],
+            "zeta_and_integral_ms": 1e3 * float(np.median(sig_s)),
+            "stack_GB": st["stack_bytes"] / 1e9,
+        }
+        log("kubo %s: %s" % (name, json.dumps(out[name])))
+    s64, sbf = sigmas["fp64"], sigmas["bf16x3"]
+    out["bf16x3"]["sigma_rel_err_vs_fp64"] = float(np.max(np.abs(sbf - s64)) / np.max(np.abs(s64)))
+    out["sigma_fp64_sample"] = [float(x) for x in s64[::25]]
+    solver.close()
+    return out
+
+
 def cpu_baseline(H, R, a, seconds):
     """The oracle (C restatement of the reference's CPU recursion, single
     thread) timed on a bounded number of steps of the same workload."""
@@ -89,6 +160,11 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=3, help="time this many plain block copies")
     ap.add_argument("--extra", action="store_true", help="also time fp32 and one-moment-per-step variants")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch shape")
+    ap.add_argument("--kubo-repeats", type=int, default=3, help="sigma_xy solves per precision (0: skip)")
+    ap.add_argument("--kubo-cells", type=int, default=512)
+    ap.add_argument("--kubo-N", type=int, default=512)
+    ap.add_argument("--kubo-R", type=int, default=32)
+    ap.add_argument("--kubo-only", action="store_true", help="run only the sigma_xy leg")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -97,6 +173,13 @@ def main():
         sys.exit(2)
 
     from tight_binding_amd import kpm
+
+    if args.kubo_only:
+        ctx = kpm.Context(0)
+        print(json.dumps({"kubo": kubo_leg(ctx, args.kubo_cells, args.kubo_N, args.kubo_R,
+                                           max(1, args.kubo_repeats))}), flush=True)
+        ctx.close()
+        return
 
     t0 = time.perf_counter()
     H, a = build_c2(args.cells)
@@ -173,6 +256,13 @@ def main():
         cpu = cpu_baseline(H, args.R, a, args.cpu_seconds)
         log("cpu: %.4f steps/s" % cpu["value"])
 
+    kubo_out = None
+    if args.kubo_repeats > 0:
+        # the DoS blocks are released first: the two Kubo stacks take 137 GB
+        M.close()
+        K.close()
+        kubo_out = kubo_leg(ctx, args.kubo_cells, args.kubo_N, args.kubo_R, args.kubo_repeats)
+
     value = st["steps"] / (st["kernel_ms"] * 1e-3)
     out = {
         "metric": "Chebyshev SpMM steps/s + %HBM roofline; sigma_xy solve p50/p99 latency",
@@ -211,6 +301,8 @@ def main():
     }
     if extra:
         out["variants"] = extra
+    if kubo_out:
+        out["kubo_sigma_xy"] = kubo_out
     print(json.dumps(out), flush=True)
     ctx.close()
 

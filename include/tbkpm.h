@@ -126,6 +126,9 @@ int tbk_context_set_tuning(tbk_context* ctx, const tbk_tuning* t);
 int tbk_context_get_tuning(const tbk_context* ctx, tbk_tuning* t);
 /* bytes of device memory the context currently holds (matrices, blocks, work) */
 int tbk_context_memory(tbk_context* ctx, int64_t* bytes);
+/* free the work buffers the context keeps between calls (recursion blocks,
+   partial sums, Kubo stacks and Gamma buffers); matrices and blocks stay */
+int tbk_context_release_work(tbk_context* ctx);
 
 /* upload a square sparse matrix; CSC input is reordered to CSR exactly */
 int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out);
@@ -189,6 +192,7 @@ typedef struct tbk_kubo_stats {
     double stack_bytes;    /* bytes of the two stacks held during the call */
     int32_t nchunks;       /* split-K chunks of the contraction */
     int32_t reserved;
+    double recursion_bytes; /* algorithmic HBM bytes of the stack-building launches and copies */
 } tbk_kubo_stats;
 
 /*
@@ -212,6 +216,19 @@ int tbk_kubo_gamma(tbk_context* ctx, const tbk_matrix* H, double a, double b, co
  */
 int tbk_stack_contract(tbk_context* ctx, const double* A, const double* B, int32_t N, int64_t K,
                        int32_t gamma_precision, double* gamma_out, double* contraction_ms);
+
+/*
+ * Kubo-Bastin integrand at rescaled energies e[0..n_e), |e_j| < 1
+ * (snippets.h:343-349 g_mn, normalised as snippets.h:400-405):
+ *   zeta_j = sum_{n,m} gamma[n][m] g_mn(e_j) / (1 - e_j^2)^2 / volume,
+ *   g_mn(e) = a_n a_m [(e - i m s) e^{i m theta} T_n(e) + (e + i n s) e^{-i n theta} T_m(e)],
+ * e = cos(theta), s = sqrt(1 - e^2), a_0 = 1, a_{k>0} = 2; computed on the
+ * device.  gamma: N x N complex128 row-major (kernel weights applied, as
+ * tbk_kubo_gamma returns it); zeta_out: n_e complex128.  N <= 2730 (the
+ * per-energy tables must fit in LDS), else TBK_E_UNSUPPORTED.
+ */
+int tbk_kubo_zeta(tbk_context* ctx, const double* gamma, int32_t N, const double* e, int32_t n_e, double volume,
+                  double* zeta_out, double* kernel_ms);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,10 @@ Restated (reference file:line):
   Kubo-Bastin           snippets.h:337-349, 406-415: Gamma_ij = sum over sites
                          and vectors of conj(aL_i) aR_j, and the direct
                          double sum zeta(e) = sum_ij Gamma_ij g_mn(e, i, j)
+  sigma integration     snippets.h:397-402, 420-449: Chebyshev-node grid, N/24
+                         nodes cut at each end, integral of Re zeta anchored
+                         at e = 0 (3/4-1/4 split of the bracketing interval)
+                         and continued outward by the trapezoid rule
 """
 from __future__ import annotations
 
@@ -88,3 +92,28 @@ def kubo_zeta(gamma, e, volume=1.0):
                 s += gamma[i, j] * g_mn(ee, i, j)
         out.append(s / (1 - ee * ee) ** 2 / volume)
     return np.array(out)
+
+
+def kubo_nodes(N):
+    """epsilon_k = cos((k + 0.5)/N pi), k < N (snippets.h:397-402)."""
+    return np.array([math.cos((k + 0.5) / N * math.pi) for k in range(N)])
+
+
+def kubo_sigma_reference(epsilon, zeta):
+    """snippets.h:420-449: drop N/24 nodes at each end, reverse to ascending,
+    find E0_i = upper_bound(E, 0.0), seed the two entries around e = 0 with
+    the 3/4-1/4 rule, then trapezoid outward.  Returns (E, Re-zeta integral).
+    Only the real part is restated (sigma_xy = integral of Re zeta)."""
+    N = len(epsilon)
+    cut = N // 24
+    E = list(epsilon[cut:cut + (N - 2 * cut)])[::-1]
+    zr = [complex(z).real for z in zeta[cut:cut + (N - 2 * cut)]][::-1]
+    E0 = next(i for i, x in enumerate(E) if x > 0.0)
+    out = [0.0] * len(E)
+    out[E0] = (zr[E0] * 3 / 4 + zr[E0 - 1] / 4) * (E[E0] - E[E0 - 1]) / 2
+    out[E0 - 1] = -(zr[E0] / 4 + zr[E0 - 1] * 3 / 4) * (E[E0] - E[E0 - 1]) / 2
+    for i in range(E0 + 1, len(E)):
+        out[i] = out[i - 1] + (zr[i] + zr[i - 1]) / 2 * (E[i] - E[i - 1])
+    for i in range(E0 - 2, -1, -1):
+        out[i] = out[i + 1] - (zr[i] + zr[i + 1]) / 2 * (E[i + 1] - E[i])
+    return np.array(E), np.array(out)

@@ -17,7 +17,8 @@ from tight_binding_amd import spectral as S
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N,K", [(37, 1000), (64, 4096), (130, 777), (1, 5)])
+@pytest.mark.parametrize("N,K", [(37, 1000), (64, 4096), (130, 777), (1, 5), (256, 3000), (300, 77), (512, 2049),
+                                 (383, 20001)])
 def test_stack_contraction_matches_numpy(gpu_ctx, N, K):
     rng = np.random.default_rng(N * 7 + K)
     A = rng.standard_normal((N, K)) + 1j * rng.standard_normal((N, K))
@@ -95,8 +96,68 @@ def test_kubo_bf16x3_error_against_fp64(gpu_ctx):
     gbf, stbf = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=kpm.GAMMA_BF16X3, g=g)
     gerr = np.max(np.abs(gbf - g64)) / np.max(np.abs(g64))
     ef = np.linspace(-0.6, 0.6, 41)
-    s64, _, _ = S.kubo_sigma(g64, ef)
-    sbf, _, _ = S.kubo_sigma(gbf, ef)
+    s64 = S.kubo_sigma(g64, ef)[0]
+    sbf = S.kubo_sigma(gbf, ef)[0]
     serr = np.max(np.abs(sbf - s64)) / np.max(np.abs(s64))
     print("bf16x3 vs fp64: Gamma %.2e, sigma %.2e" % (gerr, serr))
     assert gerr < 1e-4 and serr < 1e-4
+
+
+@pytest.mark.parametrize("N", [1, 7, 48])
+def test_device_zeta_matches_direct_double_sum(gpu_ctx, N):
+    rng = np.random.default_rng(N)
+    G = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    e = np.concatenate([SO.kubo_nodes(max(N, 2)), rng.uniform(-0.95, 0.95, 9)])
+    z, _ = kpm.kubo_zeta(gpu_ctx, G, e, 1.7)
+    ref = SO.kubo_zeta(G, e, 1.7)
+    assert np.max(np.abs(z - ref)) <= 1e-11 * np.max(np.abs(ref)), np.max(np.abs(z - ref))
+
+
+@pytest.mark.parametrize("N", [600, 1000, 2048])
+def test_device_zeta_large_orders_match_host_form(gpu_ctx, N):
+    """N = 600 and 1000 run two energies per workgroup, N = 2048 one."""
+    rng = np.random.default_rng(N)
+    G = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    e = np.concatenate([rng.uniform(-0.9, 0.9, 13), [0.0]])
+    z, _ = kpm.kubo_zeta(gpu_ctx, G, e)
+    ref = S.kubo_zeta(G, e)
+    assert np.max(np.abs(z - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+def test_device_zeta_errors(gpu_ctx):
+    from tight_binding_amd import _native as Nn
+
+    G = np.eye(3, dtype=np.complex128)
+    with pytest.raises(Nn.NativeError) as e:
+        kpm.kubo_zeta(gpu_ctx, G, [0.2, 1.0])
+    assert e.value.code == Nn.TBK_E_INVALID
+    with pytest.raises(Nn.NativeError) as e:
+        kpm.kubo_zeta(gpu_ctx, np.zeros((3000, 3000), np.complex128), [0.1])
+    assert e.value.code == Nn.TBK_E_UNSUPPORTED
+
+
+def test_sigma_full_basis_equals_exact_diagonalisation(gpu_ctx):
+    """sigma_xy from the device path (full-basis Gamma, device zeta, zero-anchored
+    integral) equals sigma_xy built from dense exact diagonalisation with the
+    same Lorentz damping and the oracle's direct double sum and integration."""
+    Hs, vx, vy = landau_problem(6, 2 * math.pi / 5, 0.5, 2, 5.0)
+    n = Hs.n
+    N = 48
+    g = S.lorentz(N, 6.0)
+    e_all, V = np.linalg.eigh(Hs.to_dense())
+    VX = V.conj().T @ vx.to_dense() @ V
+    VY = V.conj().T @ vy.to_dense() @ V
+    T = cheb_T(N, e_all)
+    G_ed = np.einsum("na,ab,mb,ba->nm", T, VX, T, VY) * np.outer(g, g)
+    eps = SO.kubo_nodes(N)
+    E_ref, I_ref = SO.kubo_sigma_reference(eps, SO.kubo_zeta(G_ed, eps, float(n)))
+    ef = np.linspace(-0.5, 0.5, 11)
+    sig_ref = np.interp(ef, E_ref, I_ref)
+
+    got, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
+                            gpu_ctx.block(np.eye(n, dtype=np.complex128)), N, g=g)
+    sig, E, _, I = S.kubo_sigma(got, ef, float(n), zeta=lambda G, e, v: kpm.kubo_zeta(gpu_ctx, G, e, v)[0])
+    scale = np.max(np.abs(I_ref))
+    assert scale > 0
+    assert np.max(np.abs(I - I_ref)) <= 1e-9 * scale
+    assert np.max(np.abs(sig - sig_ref)) <= 1e-9 * scale

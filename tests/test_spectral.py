@@ -74,3 +74,52 @@ def test_dos_of_exact_moments_integrates_and_matches_eigenvalue_histogram():
         m = (e >= lo) & (e < hi)
         count = np.sum((evals >= lo) & (evals < hi)) / n
         assert abs(np.sum(dos[m]) * de - count) < 0.02
+
+
+# ---------------------------------------------------------- Kubo-Bastin sigma
+@pytest.mark.parametrize("N", [48, 64, 512])
+def test_sigma_integration_oracle_closed_forms(N):
+    """The oracle's zero-anchored trapezoid (snippets.h:420-449) integrates
+    degree-1 integrands exactly: the Chebyshev nodes are symmetric about 0,
+    so the split point of the bracketing interval is e = 0 itself."""
+    eps = SO.kubo_nodes(N)
+    E, I1 = SO.kubo_sigma_reference(eps, np.ones(N))
+    assert np.allclose(I1, E, rtol=0, atol=1e-14)
+    E, Ie = SO.kubo_sigma_reference(eps, eps * 3.0 - 0.5)
+    assert np.allclose(Ie, 1.5 * E * E - 0.5 * E, rtol=0, atol=1e-14)
+    cut = N // 24
+    assert E.size == N - 2 * cut and np.all(np.diff(E) > 0)
+
+
+def test_sigma_integration_product_matches_oracle():
+    rng = np.random.default_rng(11)
+    for N in (48, 100, 513):
+        eps = S.kubo_nodes(N)
+        assert np.array_equal(eps, SO.kubo_nodes(N))
+        z = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+        E_ref, I_ref = SO.kubo_sigma_reference(eps, z)
+        cut = N // 24
+        E = eps[cut:N - cut][::-1]
+        I = S.integrate_from_zero(E, np.real(z)[cut:N - cut][::-1])
+        assert np.array_equal(E, E_ref) and np.array_equal(I, I_ref)
+
+
+def test_host_zeta_separable_form_matches_direct_double_sum():
+    rng = np.random.default_rng(5)
+    N = 24
+    G = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    e = np.array([-0.93, -0.4, 0.0, 0.17, 0.8])
+    assert np.allclose(S.kubo_zeta(G, e, 2.5), SO.kubo_zeta(G, e, 2.5), rtol=1e-12, atol=1e-10)
+
+
+def test_kubo_sigma_matches_oracle_pipeline():
+    rng = np.random.default_rng(8)
+    N = 48
+    G = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    ef = np.linspace(-0.7, 0.7, 21)
+    sig, E, zr, I = S.kubo_sigma(G, ef, 3.0)
+    eps = SO.kubo_nodes(N)
+    E_ref, I_ref = SO.kubo_sigma_reference(eps, SO.kubo_zeta(G, eps, 3.0))
+    assert np.allclose(E, E_ref, rtol=0, atol=0)
+    assert np.allclose(I, I_ref, rtol=1e-11, atol=1e-11 * np.abs(I_ref).max())
+    assert np.allclose(sig, np.interp(ef, E_ref, I_ref), rtol=1e-11, atol=1e-11 * np.abs(I_ref).max())

@@ -89,6 +89,7 @@ class tbk_kubo_stats(C.Structure):
         ("stack_bytes", C.c_double),
         ("nchunks", C.c_int32),
         ("reserved", C.c_int32),
+        ("recursion_bytes", C.c_double),
     ]
 
     def as_dict(self):
@@ -130,6 +131,7 @@ KPM_SIGNATURES = {
     "tbk_context_destroy": ([P], I32),
     "tbk_context_synchronize": ([P], I32),
     "tbk_context_memory": ([P, C.POINTER(I64)], I32),
+    "tbk_context_release_work": ([P], I32),
     "tbk_context_set_tuning": ([P, C.POINTER(tbk_tuning)], I32),
     "tbk_context_get_tuning": ([P, C.POINTER(tbk_tuning)], I32),
     "tbk_matrix_create": ([P, C.POINTER(tbk_sparse), PP], I32),
@@ -144,6 +146,7 @@ KPM_SIGNATURES = {
     "tbk_autotune": ([P, P, D, D, P, I32, C.POINTER(tbk_tuning), DP], I32),
     "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
     "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
+    "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),
 }
 
 HOST_SIGNATURES = {

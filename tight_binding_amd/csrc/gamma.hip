@@ -31,6 +31,8 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 namespace {
 
+constexpr int kWavesPerGroup = 4;
+
 __device__ __forceinline__ double2 ld_or_zero(const double2* p, bool ok) {
     return ok ? *p : make_double2(0.0, 0.0);
 }
@@ -107,6 +109,130 @@ __global__ __launch_bounds__(256) void gamma_f64_kernel(const double2* __restric
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int row = n0 + 16 * i + 4 * t + lk, col = m0 + 16 * j + li;
+                if (row < N && col < N) Pc[static_cast<int64_t>(row) * N + col] = make_double2(re[i][j][t], im[i][j][t]);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Large-N fp64 contraction: a 128 x 128 Gamma tile per workgroup, each wave a
+// 64 x 64 sub-tile (4 x 4 blocks of 16 x 16).  Tiles of A and B (128 rows x 4
+// complex columns) are staged through LDS, double-buffered, rows padded to 5
+// double2 to spread the fragment reads over the banks; the next tile's global
+// loads are in flight while the matrix cores work on the current one.
+//
+// Work item w = tile + tiles * chunk.  Workgroup b runs on XCD b % 8; the
+// items are dealt so that XCD x takes w in [x * per_xcd, (x + 1) * per_xcd):
+// all tiles of one k chunk then share that XCD's L2, and every A and B row of
+// the chunk is fetched from HBM about once instead of once per tile.
+__device__ __forceinline__ bool xcd_work_item(const GammaMap& m, int& tn, int& tm, int64_t& c) {
+    const int x = static_cast<int>(blockIdx.x & 7u);
+    const int64_t q = static_cast<int64_t>(blockIdx.x >> 3);
+    if (q >= m.per_xcd) return false;
+    const int64_t w = static_cast<int64_t>(x) * m.per_xcd + q;
+    const int64_t tiles = static_cast<int64_t>(m.tiles_n) * m.tiles_n;
+    if (w >= tiles * m.chunks) return false;
+    c = w / tiles;
+    const int t = static_cast<int>(w - c * tiles);
+    tn = t / m.tiles_n;
+    tm = t - tn * m.tiles_n;
+    return true;
+}
+
+constexpr int kBigKt = 4;               // complex k per LDS stage
+constexpr int kBigStride = kBigKt + 1;  // padded LDS row stride (double2)
+
+__global__ __launch_bounds__(256) void gamma_f64_big_kernel(const double2* __restrict__ A,
+                                                            const double2* __restrict__ B, int N, int64_t K,
+                                                            int64_t kc, GammaMap map, double2* __restrict__ P) {
+    __shared__ double2 As[2][kBigTile * kBigStride];
+    __shared__ double2 Bs[2][kBigTile * kBigStride];
+    int tn, tm;
+    int64_t c;
+    if (!xcd_work_item(map, tn, tm, c)) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wn = wave >> 1, wm = wave & 1;
+    const int n0 = tn * kBigTile, m0 = tm * kBigTile;
+    const int64_t kb = c * kc;
+    const int64_t ke = (kb + kc < K) ? kb + kc : K;
+
+    // staging slots: thread t loads rows (t + 256 i) / 4, column (t % 4)
+    const double2* ga[2];
+    const double2* gb[2];
+    int soff[2];
+    const int cc = tid & 3;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = (tid + 256 * i) >> 2;
+        const int rn = (n0 + r < N) ? n0 + r : N - 1;
+        const int rm = (m0 + r < N) ? m0 + r : N - 1;
+        ga[i] = A + static_cast<int64_t>(rn) * K + cc;
+        gb[i] = B + static_cast<int64_t>(rm) * K + cc;
+        soff[i] = r * kBigStride + cc;
+    }
+    double2 ra[2], rb[2];
+    auto gload = [&](int64_t k0) {
+        const bool ok = k0 + cc < ke;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            ra[i] = ld_or_zero(ga[i] + k0, ok);
+            rb[i] = ld_or_zero(gb[i] + k0, ok);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            As[buf][soff[i]] = ra[i];
+            Bs[buf][soff[i]] = rb[i];
+        }
+    };
+
+    v4d re[4][4], im[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            re[i][j] = v4d{0, 0, 0, 0};
+            im[i][j] = v4d{0, 0, 0, 0};
+        }
+
+    gload(kb);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t k0 = kb; k0 < ke; k0 += kBigKt) {
+        const bool more = k0 + kBigKt < ke;
+        if (more) gload(k0 + kBigKt);
+        double2 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = As[buf][(wn * 64 + 16 * i + li) * kBigStride + lk];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Bs[buf][(wm * 64 + 16 * j + li) * kBigStride + lk];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double ay_neg = -a[i].y;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, re[i][j], 0, 0, 0);
+                im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].y, im[i][j], 0, 0, 0);
+                re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, re[i][j], 0, 0, 0);
+                im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ay_neg, b[j].x, im[i][j], 0, 0, 0);
+            }
+        }
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    double2* Pc = P + c * static_cast<int64_t>(N) * N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = n0 + wn * 64 + 16 * i + 4 * t + lk;
+                const int col = m0 + wm * 64 + 16 * j + li;
                 if (row < N && col < N) Pc[static_cast<int64_t>(row) * N + col] = make_double2(re[i][j][t], im[i][j][t]);
             }
 }
@@ -223,6 +349,81 @@ __global__ __launch_bounds__(256) void gamma_reduce_kernel(const double2* __rest
     }
 }
 
+// zeta(e_j) = sum_{n,m} Gamma[n][m] (u_n v_m + conj(v_n) u_m) / (1 - e_j^2)^2 / volume
+// with e = cos(theta), s = sqrt(1 - e^2), a_0 = 1, a_{n>0} = 2,
+//   u_n = a_n cos(n theta),  v_n = a_n (e - i n s) e^{i n theta}:
+// the double sum of Gamma_nm g_mn(e) (g_mn of the Kubo-Bastin integrand) in
+// separable form.  One workgroup takes EB energies; their u, v tables sit
+// in LDS; wave w takes rows n = w, w+4, ..., lane l columns m = l, l+64, ...
+// (a coalesced read of each Gamma row).  Partial sums are reduced in a
+// fixed order.
+template <int EB>
+__global__ __launch_bounds__(256) void kubo_zeta_kernel(const double2* __restrict__ G, int N,
+                                                        const double* __restrict__ e, int ne, double inv_volume,
+                                                        double2* __restrict__ out) {
+    extern __shared__ double tab[];  // [EB][N] u, then [EB][N] v.re, then [EB][N] v.im
+    double* tu = tab;
+    double* tvr = tab + EB * N;
+    double* tvi = tab + 2 * EB * N;
+    const int j0 = blockIdx.x * EB;
+    for (int idx = threadIdx.x; idx < EB * N; idx += 256) {
+        const int j = idx / N, n = idx - j * N;
+        const int jj = (j0 + j < ne) ? j0 + j : ne - 1;
+        const double ee = e[jj];
+        const double th = acos(ee), es = sqrt(1.0 - ee * ee);
+        double sn, cn;
+        sincos(static_cast<double>(n) * th, &sn, &cn);
+        const double an = (n == 0) ? 1.0 : 2.0;
+        const double ns = static_cast<double>(n) * es;
+        tu[idx] = an * cn;
+        tvr[idx] = an * (ee * cn + ns * sn);
+        tvi[idx] = an * (ee * sn - ns * cn);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double2 acc[EB];
+#pragma unroll
+    for (int j = 0; j < EB; ++j) acc[j] = make_double2(0.0, 0.0);
+    for (int n = wave; n < N; n += kWavesPerGroup) {
+        const double2* row = G + static_cast<int64_t>(n) * N;
+        for (int m = lane; m < N; m += 64) {
+            const double2 g = row[m];
+#pragma unroll
+            for (int j = 0; j < EB; ++j) {
+                const double un = tu[j * N + n], vnr = tvr[j * N + n], vni = tvi[j * N + n];
+                const double um = tu[j * N + m], vmr = tvr[j * N + m], vmi = tvi[j * N + m];
+                const double cr = un * vmr + vnr * um;
+                const double ci = un * vmi - vni * um;
+                acc[j].x = acc[j].x + (g.x * cr - g.y * ci);
+                acc[j].y = acc[j].y + (g.x * ci + g.y * cr);
+            }
+        }
+    }
+    __shared__ double2 red[kWavesPerGroup][EB];
+#pragma unroll
+    for (int j = 0; j < EB; ++j) {
+        double x = acc[j].x, y = acc[j].y;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            x += __shfl_xor(x, off);
+            y += __shfl_xor(y, off);
+        }
+        if (lane == 0) red[wave][j] = make_double2(x, y);
+    }
+    __syncthreads();
+    if (threadIdx.x < EB && j0 + static_cast<int>(threadIdx.x) < ne) {
+        const int j = threadIdx.x;
+        double2 z = red[0][j];
+        for (int w = 1; w < kWavesPerGroup; ++w) {
+            z.x = z.x + red[w][j].x;
+            z.y = z.y + red[w][j].y;
+        }
+        const double ee = e[j0 + j];
+        const double d = (1.0 - ee * ee) * (1.0 - ee * ee);
+        out[j0 + j] = make_double2(z.x / d * inv_volume, z.y / d * inv_volume);
+    }
+}
+
 dim3 gamma_grid(int N, int nchunks) {
     const unsigned t = static_cast<unsigned>((N + kGammaTile - 1) / kGammaTile);
     return dim3(t, t, static_cast<unsigned>(nchunks));
@@ -244,6 +445,34 @@ void launch_gamma_reduce(const double2* P, int nchunks, int64_t NN, double2* out
     const int64_t blocks = (NN + 255) / 256;
     hipLaunchKernelGGL(gamma_reduce_kernel, dim3(static_cast<unsigned>(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
                        st, P, nchunks, NN, out);
+}
+
+
+void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
+                          double2* P, hipStream_t st) {
+    hipLaunchKernelGGL(gamma_f64_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(256), 0, st, A, B, N,
+                       K, kc, map, P);
+}
+
+int kubo_zeta_energies_per_group(int N) {
+    const int64_t per = static_cast<int64_t>(N) * 3 * sizeof(double);
+    if (per * 4 <= kZetaLdsBytes) return 4;
+    if (per * 2 <= kZetaLdsBytes) return 2;
+    if (per <= kZetaLdsBytes) return 1;
+    return 0;
+}
+
+void launch_kubo_zeta(const double2* G, int N, const double* e, int ne, double inv_volume, double2* out,
+                      hipStream_t st) {
+    const int eb = kubo_zeta_energies_per_group(N);
+    const size_t lds = static_cast<size_t>(eb) * N * 3 * sizeof(double);
+    const dim3 grid(static_cast<unsigned>((ne + eb - 1) / eb));
+    if (eb == 4)
+        hipLaunchKernelGGL(kubo_zeta_kernel<4>, grid, dim3(256), lds, st, G, N, e, ne, inv_volume, out);
+    else if (eb == 2)
+        hipLaunchKernelGGL(kubo_zeta_kernel<2>, grid, dim3(256), lds, st, G, N, e, ne, inv_volume, out);
+    else
+        hipLaunchKernelGGL(kubo_zeta_kernel<1>, grid, dim3(256), lds, st, G, N, e, ne, inv_volume, out);
 }
 
 }  // namespace tbk

@@ -10,11 +10,25 @@
 namespace tbk {
 
 constexpr int kGammaTile = 64;  // Gamma rows/cols per workgroup
+constexpr int kBigTile = 128;   // Gamma rows/cols per workgroup of the large-N kernels
+
+// Work placement of the large-N kernels: tiles_n^2 tiles x `chunks` k chunks,
+// per_xcd work items per XCD (the grid is 8 * per_xcd workgroups).
+struct GammaMap {
+    int32_t tiles_n;
+    int32_t chunks;
+    int64_t per_xcd;
+};
 
 // Split-K partial sums P[c][n][m] for k in chunk c = [c*kc, min(K, (c+1)*kc)),
 // fp64 matrix cores.  A, B: N rows of K complex128 each.
 void launch_gamma_f64(const double2* A, const double2* B, int N, int64_t K, int64_t kc, int nchunks, double2* P,
                       hipStream_t st);
+
+// Same partial sums with 128 x 128 tiles staged through LDS and the chunks
+// placed per XCD (for N >= 256).
+void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
+                          double2* P, hipStream_t st);
 
 // Split-BF16 three-term variant: each fp64 operand value x is split as
 // hi = bf16(x), lo = bf16(x - hi); products hi*hi + hi*lo + lo*hi are
@@ -25,5 +39,15 @@ void launch_gamma_bf16x3(const double2* A, const double2* B, int N, int64_t K, i
 
 // out[i] = sum_{c < nchunks} P[c*NN + i], chunks in order.
 void launch_gamma_reduce(const double2* P, int nchunks, int64_t NN, double2* out, hipStream_t st);
+
+// Kubo-Bastin integrand at rescaled energies e[0..ne) (|e| < 1) from the
+// N x N complex Gamma (row-major, kernel weights already applied):
+//   out[j] = sum_{n,m} Gamma[n][m] g_mn(e_j) / (1 - e_j^2)^2 * inv_volume.
+// Tables of 3 N doubles per energy live in LDS; N must satisfy
+// kubo_zeta_energies_per_group(N) > 0.
+constexpr int64_t kZetaLdsBytes = 64 * 1024;
+int kubo_zeta_energies_per_group(int N);
+void launch_kubo_zeta(const double2* G, int N, const double* e, int ne, double inv_volume, double2* out,
+                      hipStream_t st);
 
 }  // namespace tbk

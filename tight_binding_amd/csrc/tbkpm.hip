@@ -107,6 +107,8 @@ struct tbk_context {
     DevBuf work[2];
     DevBuf partial;
     DevBuf sums;
+    DevBuf stack[2];                 // Kubo left/right vector stacks, kept between calls
+    DevBuf gamma_part, gamma_dev;    // split-K Gamma partials and the reduced Gamma
     std::vector<hipEvent_t> events;
     int64_t held_bytes = 0;  // matrices + blocks owned through this context
     tbk_tuning tuning{};     // zeros: automatic choices
@@ -483,20 +485,43 @@ void contraction_shape(const tbk_context* ctx, int N, int64_t K, int64_t& kc, in
     nchunks = static_cast<int>((K + kc - 1) / kc);
 }
 
+// Large-N kernels: about one 128 x 128 tile work item per CU, chunks a
+// multiple of the LDS stage, and the item count a multiple of the 8 XCDs
+// where the chunk count allows it.
+void contraction_shape_big(const tbk_context* ctx, int N, int64_t K, int64_t& kc, tbk::GammaMap& map) {
+    map.tiles_n = (N + tbk::kBigTile - 1) / tbk::kBigTile;
+    const int64_t tiles = static_cast<int64_t>(map.tiles_n) * map.tiles_n;
+    int64_t chunks = std::max<int64_t>(1, (static_cast<int64_t>(ctx->num_cu) + tiles / 2) / tiles);
+    while ((tiles * chunks) % 8 != 0 && chunks < 8) ++chunks;
+    kc = (K + chunks - 1) / chunks;
+    kc = std::max<int64_t>(64, ((kc + 3) / 4) * 4);
+    map.chunks = static_cast<int32_t>((K + kc - 1) / kc);
+    map.per_xcd = (tiles * map.chunks + 7) / 8;
+}
+
 // gamma = A^H B on the device (A, B: N x K complex128), into host gamma_out.
 double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int N, int64_t K, int prec,
                        double* gamma_out, int* nchunks_out) {
     if (prec != TBK_GAMMA_FP64 && prec != TBK_GAMMA_BF16X3) raise(TBK_E_INVALID, "unknown contraction precision");
     int64_t kc;
     int nchunks;
-    contraction_shape(ctx, N, K, kc, nchunks);
+    tbk::GammaMap map{};
+    const bool big = prec == TBK_GAMMA_FP64 && N >= 2 * tbk::kBigTile;
+    if (big)
+        contraction_shape_big(ctx, N, K, kc, map);
+    else
+        contraction_shape(ctx, N, K, kc, nchunks);
+    if (big) nchunks = map.chunks;
     const int64_t NN = static_cast<int64_t>(N) * N;
-    DevBuf P, out;
+    DevBuf& P = ctx->gamma_part;
+    DevBuf& out = ctx->gamma_dev;
     P.reserve(static_cast<size_t>(nchunks) * NN * sizeof(double2));
     out.reserve(static_cast<size_t>(NN) * sizeof(double2));
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    if (prec == TBK_GAMMA_FP64)
+    if (prec == TBK_GAMMA_FP64 && big)
+        tbk::launch_gamma_f64_big(A, B, N, K, kc, map, P.as<double2>(), ctx->stream);
+    else if (prec == TBK_GAMMA_FP64)
         tbk::launch_gamma_f64(A, B, N, K, kc, nchunks, P.as<double2>(), ctx->stream);
     else
         tbk::launch_gamma_bf16x3(A, B, N, K, kc, nchunks, P.as<double2>(), ctx->stream);
@@ -530,17 +555,23 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     const int64_t K = static_cast<int64_t>(rows) * Rp;  // complex columns of one stacked block
     const size_t blk_bytes = static_cast<size_t>(K) * sizeof(double2);
     const size_t stack_bytes = static_cast<size_t>(N) * blk_bytes;
-    {
+    DevBuf& L = ctx->stack[0];
+    DevBuf& Rs = ctx->stack[1];
+    if (L.bytes < stack_bytes || Rs.bytes < stack_bytes) {
+        // the stacks are reallocated at the new size; what the context already
+        // holds for them is freed first and counts as available
+        HIPC(hipStreamSynchronize(ctx->stream));
+        L.release();
+        Rs.release();
         size_t free_b = 0, total_b = 0;
         HIPC(hipMemGetInfo(&free_b, &total_b));
         const double need = 2.0 * static_cast<double>(stack_bytes) + 2.0 * blk_bytes;
         if (need > 0.95 * static_cast<double>(free_b))
             raise(TBK_E_OOM, "Kubo stacks need " + std::to_string(need / 1e9) + " GB, " +
                                  std::to_string(static_cast<double>(free_b) / 1e9) + " GB free");
+        L.reserve(stack_bytes);
+        Rs.reserve(stack_bytes);
     }
-    DevBuf L, Rs;
-    L.reserve(stack_bytes);
-    Rs.reserve(stack_bytes);
     ctx->work[0].reserve(blk_bytes);
     ctx->work[1].reserve(blk_bytes);
     const StepPlan pH = make_plan(ctx, H, TBK_FP64, Rp, rows, b, false);
@@ -592,6 +623,19 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         stats->flops = 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
         stats->stack_bytes = 2.0 * static_cast<double>(stack_bytes);
         stats->nchunks = nchunks;
+        // algorithmic bytes: matrix arrays once per launch; vector blocks
+        // read x (+ z) and write y; v applications read x and write y
+        auto mat = [&](const tbk_matrix* m) {
+            return static_cast<double>(m->nnz) * ((m->is_complex ? 16.0 : 8.0) + 4.0) +
+                   (static_cast<double>(rows) + 1.0) * 4.0;
+        };
+        const double blk = static_cast<double>(rows) * kets->R * 16.0;
+        const double Nm2 = static_cast<double>(std::max<int64_t>(0, N - 2));
+        double bytes = 2.0 * blk;                                   // |r> into the left stack
+        bytes += mat(vy) + 2.0 * blk + mat(vx) + 2.0 * blk;         // v_x v_y |r>
+        if (N > 1) bytes += 2.0 * (mat(H) + 2.0 * blk) + mat(vx) + 2.0 * blk;
+        bytes += Nm2 * (2.0 * (mat(H) + 3.0 * blk) + mat(vx) + 2.0 * blk);
+        stats->recursion_bytes = bytes;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -685,7 +729,20 @@ int tbk_context_memory(tbk_context* ctx, int64_t* bytes) {
     return guarded([&] {
         if (!ctx || !bytes) raise(TBK_E_INVALID, "null argument");
         *bytes = ctx->held_bytes + static_cast<int64_t>(ctx->work[0].bytes + ctx->work[1].bytes +
-                                                        ctx->partial.bytes + ctx->sums.bytes);
+                                                        ctx->partial.bytes + ctx->sums.bytes +
+                                                        ctx->stack[0].bytes + ctx->stack[1].bytes +
+                                                        ctx->gamma_part.bytes + ctx->gamma_dev.bytes);
+    });
+}
+
+int tbk_context_release_work(tbk_context* ctx) {
+    return guarded([&] {
+        if (!ctx) raise(TBK_E_INVALID, "null argument");
+        set_device(ctx);
+        HIPC(hipStreamSynchronize(ctx->stream));
+        for (DevBuf* b : {&ctx->work[0], &ctx->work[1], &ctx->partial, &ctx->sums, &ctx->stack[0], &ctx->stack[1],
+                          &ctx->gamma_part, &ctx->gamma_dev})
+            b->release();
     });
 }
 
@@ -949,6 +1006,39 @@ int tbk_stack_contract(tbk_context* ctx, const double* A, const double* B, int32
         const double ms = contract_stacks(ctx, dA.as<const double2>(), dB.as<const double2>(), N, K,
                                           gamma_precision, gamma_out, nullptr);
         if (contraction_ms) *contraction_ms = ms;
+    });
+}
+
+int tbk_kubo_zeta(tbk_context* ctx, const double* gamma, int32_t N, const double* e, int32_t n_e, double volume,
+                  double* zeta_out, double* kernel_ms) {
+    return guarded([&] {
+        if (!ctx || !gamma || !e || !zeta_out) raise(TBK_E_INVALID, "null argument");
+        if (N < 1 || n_e < 1) raise(TBK_E_INVALID, "N and n_e must be positive");
+        if (!(volume != 0.0) || !std::isfinite(volume)) raise(TBK_E_INVALID, "invalid volume");
+        for (int32_t j = 0; j < n_e; ++j)
+            if (!(e[j] > -1.0 && e[j] < 1.0)) raise(TBK_E_INVALID, "energies must lie in (-1, 1)");
+        if (tbk::kubo_zeta_energies_per_group(N) == 0)
+            raise(TBK_E_UNSUPPORTED, "N too large for the zeta kernel's LDS tables");
+        set_device(ctx);
+        const size_t gbytes = static_cast<size_t>(N) * N * sizeof(double2);
+        DevBuf dG, dE, dZ;
+        dG.reserve(gbytes);
+        dE.reserve(static_cast<size_t>(n_e) * sizeof(double));
+        dZ.reserve(static_cast<size_t>(n_e) * sizeof(double2));
+        HIPC(hipMemcpyAsync(dG.p, gamma, gbytes, hipMemcpyHostToDevice, ctx->stream));
+        HIPC(hipMemcpyAsync(dE.p, e, static_cast<size_t>(n_e) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        ensure_events(ctx, 2);
+        HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+        tbk::launch_kubo_zeta(dG.as<const double2>(), N, dE.as<const double>(), n_e, 1.0 / volume,
+                              dZ.as<double2>(), ctx->stream);
+        HIPC(hipGetLastError());
+        HIPC(hipEventRecord(ctx->events[1], ctx->stream));
+        HIPC(hipMemcpyAsync(zeta_out, dZ.p, static_cast<size_t>(n_e) * sizeof(double2), hipMemcpyDeviceToHost,
+                            ctx->stream));
+        HIPC(hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+        if (kernel_ms) *kernel_ms = ms;
     });
 }
 

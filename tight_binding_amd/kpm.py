@@ -116,6 +116,11 @@ class Context:
         N.check_kpm(N.kpm().tbk_context_memory(self._h, C.byref(b)))
         return b.value
 
+    def release_work(self):
+        """Free the work buffers kept between calls (recursion blocks, Kubo
+        stacks, Gamma buffers)."""
+        N.check_kpm(N.kpm().tbk_context_release_work(self._h))
+
     def synchronize(self):
         N.check_kpm(N.kpm().tbk_context_synchronize(self._h))
 
@@ -207,6 +212,21 @@ def stack_contract(ctx: Context, A: np.ndarray, B: np.ndarray, precision: int = 
     N.check_kpm(N.kpm().tbk_stack_contract(ctx._h, _dp(A.view(np.float64)), _dp(B.view(np.float64)), n, k,
                                            int(precision), _dp(out), C.byref(ms)))
     return out.view(np.complex128).reshape(n, n), ms.value
+
+
+def kubo_zeta(ctx: Context, gamma: np.ndarray, e, volume: float = 1.0):
+    """Kubo-Bastin integrand zeta(e_j) = sum Gamma_nm g_mn(e_j) / (1 - e_j^2)^2 /
+    volume at rescaled energies |e_j| < 1, on the device (tbk_kubo_zeta).
+    Returns (zeta complex array, kernel milliseconds)."""
+    G = np.ascontiguousarray(gamma, dtype=np.complex128)
+    n = G.shape[0]
+    assert G.shape == (n, n)
+    e = np.ascontiguousarray(np.atleast_1d(e), dtype=np.float64)
+    out = np.zeros(2 * e.size, dtype=np.float64)
+    ms = C.c_double()
+    N.check_kpm(N.kpm().tbk_kubo_zeta(ctx._h, _dp(G.view(np.float64)), n, _dp(e), int(e.size), float(volume),
+                                      _dp(out), C.byref(ms)))
+    return out.view(np.complex128), ms.value
 
 
 def cheb_first_kind_seq_braket(M: CSC, n: int, bras_T: np.ndarray, kets: np.ndarray) -> np.ndarray:

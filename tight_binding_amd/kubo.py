@@ -6,13 +6,16 @@ conductivity_Kubo_Bastin_test (snippets.h:306-456).
   R random-phase vectors, Lorentz kernel g_n (lambda = 6)       (snippets.h:318-320)
   Gamma_nm = sum conj(g_n T_n(H~)|r>) (g_m v_x T_m(H~) v_y|r>)  (snippets.h:326-342)
   zeta(e) = sum Gamma_nm g_mn(e) / (1 - e^2)^2 / Volume          (snippets.h:343-349, 400-405)
-  sigma(mu) = integral of Re zeta up to mu                       (snippets.h:436-449)
+  sigma(mu) = integral of Re zeta from e = 0 to mu               (snippets.h:420-449)
 
 Gamma is computed on the GPU (two vector stacks in HBM and a complex
-contraction on matrix cores, fp64 or split-bf16 x3); zeta and sigma are
-evaluated on the host from the N x N Gamma.
+contraction on matrix cores, fp64 or split-bf16 x3); zeta at the Chebyshev
+nodes is evaluated on the GPU from the N x N Gamma, and the integral over
+those N nodes on the host.
 """
 from __future__ import annotations
+
+import time
 
 import numpy as np
 
@@ -25,23 +28,55 @@ def velocity_operators(Hs, X, Y):
     return L.commutator_diag(X, Hs), L.commutator_diag(Y, Hs)
 
 
+class KuboSolver:
+    """One Hamiltonian resident on the device with its velocity operators;
+    solve() runs one sigma_xy computation from fresh random vectors.
+    H: complex CSC Hamiltonian; X, Y: diagonal position operators (CSC)."""
+
+    def __init__(self, ctx: kpm.Context, H, X, Y, E_min, E_max, volume=1.0, lam=6.0):
+        lo, hi = E_min - 1e-8, E_max + 1e-8
+        self.a, self.b = (hi - lo) / 2, (hi + lo) / 2
+        self.ctx, self.volume, self.lam = ctx, float(volume), float(lam)
+        self.Hs = L.rescale(H, self.a, self.b)
+        vx, vy = velocity_operators(self.Hs, X, Y)
+        self.n = self.Hs.n
+        self.MH, self.MX, self.MY = ctx.matrix(self.Hs), ctx.matrix(vx), ctx.matrix(vy)
+
+    def close(self):
+        for m in (self.MH, self.MX, self.MY):
+            m.close()
+
+    def zeta(self, gamma, e, volume):
+        return kpm.kubo_zeta(self.ctx, gamma, e, volume)[0]
+
+    def solve(self, N, R=4, seed=1, precision=kpm.GAMMA_FP64, fermi_energies=None, kets=None):
+        """Returns dict(gamma, a, b, sigma, sigma_grid, zeta_e, zeta,
+        fermi_energies, stats, timings); fermi_energies in the units of H
+        (None skips sigma)."""
+        t0 = time.perf_counter()
+        g = spectral.lorentz(N, self.lam)
+        K = self.ctx.block(kets) if kets is not None else self.ctx.random_block(self.n, R, seed=seed)
+        try:
+            gamma, st = kpm.kubo_gamma(self.ctx, self.MH, self.MX, self.MY, K, N, precision=precision, g=g)
+        finally:
+            K.close()
+        t1 = time.perf_counter()
+        out = dict(gamma=gamma, a=self.a, b=self.b, stats=st)
+        if fermi_energies is not None:
+            ef = (np.asarray(fermi_energies, dtype=np.float64) - self.b) / self.a
+            sigma, e, zr, integral = spectral.kubo_sigma(gamma, ef, self.volume, zeta=self.zeta)
+            out.update(fermi_energies=np.asarray(fermi_energies), sigma=sigma, zeta_e=e, zeta=zr,
+                       sigma_grid=integral)
+        t2 = time.perf_counter()
+        out["timings"] = {"gamma_s": t1 - t0, "sigma_s": t2 - t1, "total_s": t2 - t0}
+        return out
+
+
 def conductivity_kubo_bastin(ctx, H, X, Y, N, E_min, E_max, volume=1.0, R=4, lam=6.0, seed=1,
                              precision=kpm.GAMMA_FP64, fermi_energies=None, kets=None):
-    """Returns dict(gamma, a, b, zeta_e, zeta, sigma, fermi_energies, stats).
-    H: complex CSC Hamiltonian; X, Y: diagonal position operators."""
-    lo, hi = E_min - 1e-8, E_max + 1e-8
-    a, b = (hi - lo) / 2, (hi + lo) / 2
-    Hs = L.rescale(H, a, b)
-    vx, vy = velocity_operators(Hs, X, Y)
-    g = spectral.lorentz(N, lam)
-    MH, MX, MY = ctx.matrix(Hs), ctx.matrix(vx), ctx.matrix(vy)
-    K = ctx.block(kets) if kets is not None else ctx.random_block(Hs.n, R, seed=seed)
-    gamma, st = kpm.kubo_gamma(ctx, MH, MX, MY, K, N, precision=precision, g=g)
-    out = dict(gamma=gamma, a=a, b=b, stats=st)
-    if fermi_energies is not None:
-        ef = (np.asarray(fermi_energies, dtype=np.float64) - b) / a
-        sigma, e, zr = spectral.kubo_sigma(gamma, ef, volume)
-        out.update(fermi_energies=np.asarray(fermi_energies), sigma=sigma, zeta_e=e, zeta=zr)
-    for m in (MH, MX, MY, K):
-        m.close()
-    return out
+    """One-shot form of KuboSolver.solve; returns the same dict."""
+    s = KuboSolver(ctx, H, X, Y, E_min, E_max, volume, lam)
+    try:
+        return s.solve(N, R, seed, precision, fermi_energies, kets)
+    finally:
+        s.close()

@@ -145,19 +145,54 @@ def kubo_zeta(gamma, e, volume=1.0):
     return z / (1.0 - e * e) ** 2 / volume
 
 
-def kubo_sigma(gamma, e_fermi, volume=1.0, trim=1.0 / 24, points_per_moment=8):
-    """sigma(mu) = integral of Re zeta from the lower edge of the trimmed
-    spectral interval to mu, for rescaled Fermi energies mu (T = 0).  The
-    interval edges e = +-cos(trim*pi) are cut as the reference cuts N/24
-    Chebyshev nodes at each end (snippets.h:421-426), where (1 - e^2)^-2
-    diverges.  Trapezoid rule on a uniform grid in theta = arccos(e).
-    Returns (sigma at e_fermi, the grid e, Re zeta on the grid)."""
+def kubo_nodes(N):
+    """Chebyshev nodes e_k = cos((k + 1/2) pi / N), k < N (descending), where
+    the reference evaluates the Kubo-Bastin integrand (snippets.h:397-402)."""
+    k = np.arange(int(N), dtype=np.float64)
+    return np.cos((k + 0.5) / N * math.pi)
+
+
+def integrate_from_zero(E, z):
+    """Cumulative integral of z over the ascending grid E, anchored at e = 0
+    (snippets.h:436-449): the interval [E[i0-1], E[i0]] that brackets 0
+    (i0 = first index with E > 0) is split at its midpoint, with the midpoint
+    value taken as the mean of its ends; from there the trapezoid rule runs
+    outward in both directions."""
+    E = np.asarray(E, dtype=np.float64)
+    z = np.asarray(z, dtype=np.float64)
+    i0 = int(np.searchsorted(E, 0.0, side="right"))
+    if i0 < 1 or i0 >= E.size:
+        raise ValueError("the energy grid must bracket e = 0")
+    out = np.zeros(E.size)
+    h = E[i0] - E[i0 - 1]
+    out[i0] = (z[i0] * 3 / 4 + z[i0 - 1] / 4) * h / 2
+    out[i0 - 1] = -(z[i0] / 4 + z[i0 - 1] * 3 / 4) * h / 2
+    acc = out[i0]
+    for i in range(i0 + 1, E.size):
+        acc = acc + (z[i] + z[i - 1]) / 2 * (E[i] - E[i - 1])
+        out[i] = acc
+    acc = out[i0 - 1]
+    for i in range(i0 - 2, -1, -1):
+        acc = acc - (z[i] + z[i + 1]) / 2 * (E[i + 1] - E[i])
+        out[i] = acc
+    return out
+
+
+def kubo_sigma(gamma, e_fermi, volume=1.0, zeta=None):
+    """sigma_xy at rescaled Fermi energies, by the reference's scheme
+    (snippets.h:397-449): zeta at the N Chebyshev nodes, N/24 nodes cut at
+    each end (where (1 - e^2)^-2 diverges), Re zeta integrated from e = 0
+    outward (integrate_from_zero), then interpolated linearly to e_fermi.
+    zeta: callable (gamma, energies, volume) -> complex zeta (default: the
+    host evaluation kubo_zeta; kpm.kubo_zeta evaluates it on the device).
+    Returns (sigma at e_fermi, grid E ascending, Re zeta on E, integral on E)."""
     gamma = np.asarray(gamma)
     N = gamma.shape[0]
-    M = max(64, points_per_moment * N)
-    th = np.linspace(math.pi * (1 - trim), math.pi * trim, M)  # e ascending
-    e = np.cos(th)
-    zr = np.real(kubo_zeta(gamma, e, volume))
-    cum = np.concatenate([[0.0], np.cumsum(0.5 * (zr[1:] + zr[:-1]) * np.diff(e))])
+    eps = kubo_nodes(N)
+    z = (zeta or kubo_zeta)(gamma, eps, volume)
+    cut = N // 24
+    E = eps[cut:N - cut][::-1]
+    zr = np.real(z)[cut:N - cut][::-1]
+    integral = integrate_from_zero(E, zr)
     ef = np.atleast_1d(np.asarray(e_fermi, dtype=np.float64))
-    return np.interp(ef, e, cum), e, zr
+    return np.interp(ef, E, integral), E, zr, integral
