@@ -335,6 +335,156 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_kernel(const double2* __rest
             }
 }
 
+// ---------------------------------------------------------------------------
+// Large-N split-bf16 x3 contraction, same 128 x 128 tiling and XCD placement
+// as gamma_f64_big_kernel.  Each double is split once, when its tile is
+// staged: hi = bf16(x), lo = bf16(x - hi), a complex value becoming one
+// 32-bit word of (re, im) bf16 pairs per part.  An LDS row holds the 16
+// complex columns of one stage as 16 hi words, 16 lo words and 4 words of
+// padding (144 bytes: the 16 rows of a fragment read fall on distinct bank
+// groups).  Lane l of a fragment reads 4 consecutive words = 8 bf16 = the k
+// range 8*(l/16) .. +7 of its row.  The imaginary part uses B words turned
+// into (im, -re) in registers.  Chunks are short (about 32K complex columns)
+// so fp32 accumulation never runs long; chunk partials are summed in fp64.
+constexpr int kSplitKt = 16;        // complex k per stage = one bf16 MFMA k-step
+constexpr int kSplitRowWords = 36;  // 16 hi + 16 lo + 4 pad
+
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+    return static_cast<uint32_t>(__builtin_bit_cast(unsigned short, static_cast<__bf16>(f)));
+}
+
+// (hi, lo) words of one complex value
+__device__ __forceinline__ uint2 split_word(double2 v) {
+    const float hr = static_cast<float>(v.x), hi_ = static_cast<float>(v.y);
+    const __bf16 br = static_cast<__bf16>(hr), bi = static_cast<__bf16>(hi_);
+    const float lr = static_cast<float>(v.x - static_cast<double>(static_cast<float>(br)));
+    const float li = static_cast<float>(v.y - static_cast<double>(static_cast<float>(bi)));
+    uint2 w;
+    w.x = static_cast<uint32_t>(__builtin_bit_cast(unsigned short, br)) |
+          (static_cast<uint32_t>(__builtin_bit_cast(unsigned short, bi)) << 16);
+    w.y = bf16_bits(lr) | (bf16_bits(li) << 16);
+    return w;
+}
+
+// (re, im) -> (im, -re) on a packed bf16 pair
+__device__ __forceinline__ uint32_t swap_neg(uint32_t w) { return ((w >> 16) | (w << 16)) ^ 0x80000000u; }
+
+__device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
+
+__global__ __launch_bounds__(256) void gamma_bf16x3_big_kernel(const double2* __restrict__ A,
+                                                               const double2* __restrict__ B, int N, int64_t K,
+                                                               int64_t kc, GammaMap map, double2* __restrict__ P) {
+    __shared__ __align__(16) uint32_t As[2][kBigTile * kSplitRowWords];
+    __shared__ __align__(16) uint32_t Bs[2][kBigTile * kSplitRowWords];
+    int tn, tm;
+    int64_t c;
+    if (!xcd_work_item(map, tn, tm, c)) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wn = wave >> 1, wm = wave & 1;
+    const int n0 = tn * kBigTile, m0 = tm * kBigTile;
+    const int64_t kb = c * kc;
+    const int64_t ke = (kb + kc < K) ? kb + kc : K;
+
+    // staging: thread t takes complex column t % 16 of rows t / 16 + 16 i, i < 8
+    const int col = tid & 15;
+    const int r0 = tid >> 4;
+    const double2* ga[8];
+    const double2* gb[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int r = r0 + 16 * i;
+        const int rn = (n0 + r < N) ? n0 + r : N - 1;
+        const int rm = (m0 + r < N) ? m0 + r : N - 1;
+        ga[i] = A + static_cast<int64_t>(rn) * K + col;
+        gb[i] = B + static_cast<int64_t>(rm) * K + col;
+    }
+    double2 ra[8], rb[8];
+    auto gload = [&](int64_t k0) {
+        const bool ok = k0 + col < ke;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            ra[i] = ld_or_zero(ga[i] + k0, ok);
+            rb[i] = ld_or_zero(gb[i] + k0, ok);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int o = (r0 + 16 * i) * kSplitRowWords + col;
+            const uint2 wa = split_word(ra[i]);
+            const uint2 wb = split_word(rb[i]);
+            As[buf][o] = wa.x;
+            As[buf][o + 16] = wa.y;
+            Bs[buf][o] = wb.x;
+            Bs[buf][o + 16] = wb.y;
+        }
+    };
+
+    v4f re[4][4], im[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            re[i][j] = v4f{0, 0, 0, 0};
+            im[i][j] = v4f{0, 0, 0, 0};
+        }
+
+    gload(kb);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t k0 = kb; k0 < ke; k0 += kSplitKt) {
+        const bool more = k0 + kSplitKt < ke;
+        if (more) gload(k0 + kSplitKt);
+        v8bf ah[4], al[4], bh[4], bl[4], bih[4], bil[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t* row = &As[buf][(wn * 64 + 16 * i + li) * kSplitRowWords + 4 * lk];
+            ah[i] = as_v8bf(*reinterpret_cast<const uint4*>(row));
+            al[i] = as_v8bf(*reinterpret_cast<const uint4*>(row + 16));
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t* row = &Bs[buf][(wm * 64 + 16 * j + li) * kSplitRowWords + 4 * lk];
+            const uint4 h = *reinterpret_cast<const uint4*>(row);
+            const uint4 l = *reinterpret_cast<const uint4*>(row + 16);
+            bh[j] = as_v8bf(h);
+            bl[j] = as_v8bf(l);
+            bih[j] = as_v8bf(make_uint4(swap_neg(h.x), swap_neg(h.y), swap_neg(h.z), swap_neg(h.w)));
+            bil[j] = as_v8bf(make_uint4(swap_neg(l.x), swap_neg(l.y), swap_neg(l.z), swap_neg(l.w)));
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], re[i][j], 0, 0, 0);
+                im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bih[j], im[i][j], 0, 0, 0);
+                re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], re[i][j], 0, 0, 0);
+                im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bil[j], im[i][j], 0, 0, 0);
+                re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], re[i][j], 0, 0, 0);
+                im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bih[j], im[i][j], 0, 0, 0);
+            }
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    double2* Pc = P + c * static_cast<int64_t>(N) * N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = n0 + wn * 64 + 16 * i + 4 * lk + t;
+                const int cl = m0 + wm * 64 + 16 * j + li;
+                if (row < N && cl < N)
+                    Pc[static_cast<int64_t>(row) * N + cl] =
+                        make_double2(static_cast<double>(re[i][j][t]), static_cast<double>(im[i][j][t]));
+            }
+}
+
 __global__ __launch_bounds__(256) void gamma_reduce_kernel(const double2* __restrict__ P, int nchunks, int64_t NN,
                                                            double2* __restrict__ out) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < NN;
@@ -452,6 +602,12 @@ void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, 
                           double2* P, hipStream_t st) {
     hipLaunchKernelGGL(gamma_f64_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(256), 0, st, A, B, N,
                        K, kc, map, P);
+}
+
+void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
+                             double2* P, hipStream_t st) {
+    hipLaunchKernelGGL(gamma_bf16x3_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(256), 0, st, A, B,
+                       N, K, kc, map, P);
 }
 
 int kubo_zeta_energies_per_group(int N) {

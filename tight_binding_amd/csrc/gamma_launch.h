@@ -37,6 +37,11 @@ void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, 
 void launch_gamma_bf16x3(const double2* A, const double2* B, int N, int64_t K, int64_t kc, int nchunks, double2* P,
                          hipStream_t st);
 
+// Split-bf16 x3 partial sums with 128 x 128 tiles (operands split once, at
+// LDS staging) and the chunks placed per XCD (for N >= 256).
+void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
+                             double2* P, hipStream_t st);
+
 // out[i] = sum_{c < nchunks} P[c*NN + i], chunks in order.
 void launch_gamma_reduce(const double2* P, int nchunks, int64_t NN, double2* out, hipStream_t st);
 

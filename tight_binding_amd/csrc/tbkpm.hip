@@ -486,15 +486,19 @@ void contraction_shape(const tbk_context* ctx, int N, int64_t K, int64_t& kc, in
 }
 
 // Large-N kernels: about one 128 x 128 tile work item per CU, chunks a
-// multiple of the LDS stage, and the item count a multiple of the 8 XCDs
-// where the chunk count allows it.
-void contraction_shape_big(const tbk_context* ctx, int N, int64_t K, int64_t& kc, tbk::GammaMap& map) {
+// multiple of the LDS stage (16 complex columns covers both kernels), and the
+// item count a multiple of the 8 XCDs where the chunk count allows it.
+// max_cols > 0 caps the chunk length (the fp32-accumulating kernel).
+constexpr int64_t kSplitChunkCols = 32768;
+void contraction_shape_big(const tbk_context* ctx, int N, int64_t K, int64_t max_cols, int64_t& kc,
+                           tbk::GammaMap& map) {
     map.tiles_n = (N + tbk::kBigTile - 1) / tbk::kBigTile;
     const int64_t tiles = static_cast<int64_t>(map.tiles_n) * map.tiles_n;
     int64_t chunks = std::max<int64_t>(1, (static_cast<int64_t>(ctx->num_cu) + tiles / 2) / tiles);
     while ((tiles * chunks) % 8 != 0 && chunks < 8) ++chunks;
+    if (max_cols > 0) chunks = std::max<int64_t>(chunks, (K + max_cols - 1) / max_cols);
     kc = (K + chunks - 1) / chunks;
-    kc = std::max<int64_t>(64, ((kc + 3) / 4) * 4);
+    kc = std::max<int64_t>(64, ((kc + 15) / 16) * 16);
     map.chunks = static_cast<int32_t>((K + kc - 1) / kc);
     map.per_xcd = (tiles * map.chunks + 7) / 8;
 }
@@ -506,9 +510,9 @@ double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int
     int64_t kc;
     int nchunks;
     tbk::GammaMap map{};
-    const bool big = prec == TBK_GAMMA_FP64 && N >= 2 * tbk::kBigTile;
+    const bool big = N >= 2 * tbk::kBigTile;
     if (big)
-        contraction_shape_big(ctx, N, K, kc, map);
+        contraction_shape_big(ctx, N, K, prec == TBK_GAMMA_FP64 ? 0 : kSplitChunkCols, kc, map);
     else
         contraction_shape(ctx, N, K, kc, nchunks);
     if (big) nchunks = map.chunks;
@@ -523,6 +527,8 @@ double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int
         tbk::launch_gamma_f64_big(A, B, N, K, kc, map, P.as<double2>(), ctx->stream);
     else if (prec == TBK_GAMMA_FP64)
         tbk::launch_gamma_f64(A, B, N, K, kc, nchunks, P.as<double2>(), ctx->stream);
+    else if (big)
+        tbk::launch_gamma_bf16x3_big(A, B, N, K, kc, map, P.as<double2>(), ctx->stream);
     else
         tbk::launch_gamma_bf16x3(A, B, N, K, kc, nchunks, P.as<double2>(), ctx->stream);
     HIPC(hipGetLastError());
