@@ -230,6 +230,48 @@ int tbk_stack_contract(tbk_context* ctx, const double* A, const double* B, int32
 int tbk_kubo_zeta(tbk_context* ctx, const double* gamma, int32_t N, const double* e, int32_t n_e, double volume,
                   double* zeta_out, double* kernel_ms);
 
+/* ------------------------------------------- batched ragged moment problems */
+typedef struct tbk_batch tbk_batch;
+
+enum { TBK_BATCH_GRAPHS = 1 /* capture each chunk of 64 steps in a hipGraph, replay on later runs */ };
+
+typedef struct tbk_batch_stats {
+    double kernel_ms;      /* device time from the first step launch to the last moment launch */
+    double wall_ms;        /* host wall time of the call */
+    int64_t launches;      /* batched step launches (the largest step count in the batch) */
+    int64_t graphs;        /* chunk graphs launched (0 without TBK_BATCH_GRAPHS) */
+    double bytes_total;    /* algorithmic HBM bytes of every problem's steps */
+    int64_t rows_total;    /* rows of the block-diagonal matrix, tile padding included */
+    int32_t tile_rows;     /* rows per tile (problems start on tile boundaries) */
+    int32_t cpl;           /* complex values per lane (lanes per row = padded R / cpl) */
+    int32_t blocks_per_cu; /* workgroups per CU of a step launch */
+    int32_t xcd_split;     /* 1: tile ranges dealt per XCD */
+} tbk_batch_stats;
+
+/*
+ * A batch of independent moment problems run as one block-diagonal
+ * recursion (no reference counterpart: the reference solves one Hamiltonian
+ * per call, numerics_cuda.cu:190-323).  Problem p: matrix mats[p] (CSC or CSR,
+ * as tbk_matrix_create), rescaling H~ = (H - b[p])/a[p] fused into the step,
+ * n_moments[p] >= 2 moments, R random-phase vectors generated on the device
+ * from seeds[p] exactly as tbk_block_random_phase(rows_p, R, seeds[p]).  All
+ * problems share R, the precision and real-or-complex values.  Bras = kets:
+ * two moments per step.  Each problem's moments are independent of the other
+ * problems in the batch (bit for bit) and its vectors equal those of a
+ * single-problem run.
+ */
+int tbk_batch_create(tbk_context* ctx, int32_t n_problems, const tbk_sparse* mats, const double* a,
+                     const double* b, const int32_t* n_moments, int32_t R, int32_t precision,
+                     const uint64_t* seeds, tbk_batch** out);
+/* run the whole batch from fresh kets; mu_out: the problems' moments
+   concatenated in problem order (sum of n_moments doubles) */
+int tbk_batch_run(tbk_batch* batch, int32_t flags, double* mu_out, tbk_batch_stats* stats);
+/* launch shape: complex values per lane (1, 2 or 4), workgroups per CU, XCD split */
+int tbk_batch_set_shape(tbk_batch* batch, int32_t cpl, int32_t blocks_per_cu, int32_t xcd_split);
+/* time `steps` steps per launch shape, keep the fastest, return its time per step */
+int tbk_batch_autotune(tbk_batch* batch, int32_t steps, double* best_ms_per_step);
+int tbk_batch_destroy(tbk_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,174 @@
+"""Batched ragged moment problems (tbk_batch_*) on the GPU.
+
+Each problem of a batch must give the moments of the single-problem path
+(same vectors T_n, moment sums in another order: within 1e-12 relative in
+fp64), the oracle's moments within 1e-10, the same bits alone and in any
+mix, the same bits with and without hipGraph capture and across graph
+replays, and agree across launch shapes."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import lattice_oracle as O
+from tight_binding_amd import kpm
+from tight_binding_amd import lattice as L
+
+pytestmark = pytest.mark.gpu
+
+COORD = {"square": 4, "honeycomb": 3, "lieb": 4, "kagome": 4}
+
+
+def model_problem(model, Lc, W, seed, t=1.0):
+    c = L.UnitCell.model(model)
+    c.add_hopping_nn("A", 1, t)
+    if model == "kagome":
+        c.add_hopping_nn("B", 1, t)  # the B-C bonds (A's shell holds A-B and A-C)
+    lat = L.PartialLattice(c, (Lc, Lc))
+    if W:
+        lat.set_disorder(W, seed)
+    H = lat.get_Hamiltonian_periodic_extend_bare_sparse((1, 1))
+    a = 1.02 * (COORD[model] * abs(t) + W / 2)
+    return H, a
+
+
+def mixed_batch(n=7, base_seed=11):
+    rng = np.random.default_rng(base_seed)
+    models = ["square", "honeycomb", "lieb", "kagome"]
+    probs = []
+    for i in range(n):
+        m = models[i % 4]
+        H, a = model_problem(m, int(rng.integers(5, 23)), float(rng.uniform(0, 3)), seed=100 + i)
+        probs.append((H, a, 0.1 * (i % 3) - 0.1, int(rng.choice([2, 3, 30, 64, 130, 257])), 1000 + i))
+    return probs
+
+
+def make_batch(ctx, probs, R, precision=kpm.FP64):
+    return kpm.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs],
+                     [p[3] for p in probs], R, precision, seeds=[p[4] for p in probs])
+
+
+def single(ctx, H, a, b, n, seed, R, precision=kpm.FP64):
+    M = ctx.matrix(H)
+    K = ctx.random_block(H.n, R, seed=seed, precision=precision)
+    mu, _, _ = ctx.moments(M, K, n, a=a, b=b)
+    return mu, K
+
+
+def test_batch_matches_single_problem_path(gpu_ctx):
+    probs = mixed_batch()
+    B = make_batch(gpu_ctx, probs, R=4)
+    mus, st = B.run()
+    assert st["launches"] == max((p[3] + 1) // 2 for p in probs)
+    for (H, a, b, n, seed), mu in zip(probs, mus):
+        ref, _ = single(gpu_ctx, H, a, b, n, seed, 4)
+        assert mu.size == n
+        assert np.max(np.abs(mu - ref)) <= 1e-12 * np.max(np.abs(ref)), (n, np.max(np.abs(mu - ref)))
+    B.close()
+
+
+def test_batch_matches_oracle(gpu_ctx):
+    probs = mixed_batch(5, base_seed=3)
+    B = make_batch(gpu_ctx, probs, R=3)
+    mus, _ = B.run()
+    for (H, a, b, n, seed), mu in zip(probs, mus):
+        K = gpu_ctx.random_block(H.n, 3, seed=seed)
+        kets = K.download()
+        Hs = O.rescale(O.CSC(H.n, H.colptr, H.rowidx, H.values), a, b)
+        ref = oracle.cheb_braket(Hs, n, kets, kets)
+        assert np.max(np.abs(mu - ref)) <= 1e-10 * np.max(np.abs(ref))
+    B.close()
+
+
+def test_alone_and_in_mix_are_bit_identical(gpu_ctx):
+    probs = mixed_batch(6, base_seed=5)
+    Bm = make_batch(gpu_ctx, probs, R=8)
+    mix, _ = Bm.run()
+    Bm.close()
+    for i in (0, 3, 5):
+        Ba = make_batch(gpu_ctx, [probs[i]], R=8)
+        alone, _ = Ba.run()
+        Ba.close()
+        assert np.array_equal(alone[0], mix[i])
+    # another mix with the problems reordered and one repeated
+    order = [4, 1, 1, 0, 3]
+    Br = make_batch(gpu_ctx, [probs[j] for j in order], R=8)
+    mix2, _ = Br.run()
+    Br.close()
+    for k, j in enumerate(order):
+        assert np.array_equal(mix2[k], mix[j])
+
+
+def test_graph_replays_match_eager(gpu_ctx):
+    """Graph capture per 64-step chunk: the first (capturing) run, two
+    replays with eager work and a synchronisation between them, and the eager
+    run all give the same bits."""
+    probs = mixed_batch(5, base_seed=9)
+    B = make_batch(gpu_ctx, probs, R=4)
+    eager, st0 = B.run(graphs=False)
+    assert st0["graphs"] == 0
+    first, st1 = B.run(graphs=True)
+    assert st1["graphs"] == (st1["launches"] + 63) // 64
+    for rep in range(2):
+        # unrelated eager work on the same stream between replays
+        H, a = model_problem("square", 9, 1.0, 2)
+        single(gpu_ctx, H, a, 0.0, 20, 3, 4)
+        gpu_ctx.synchronize()
+        again, _ = B.run(graphs=True)
+        for x, y, z in zip(eager, first, again):
+            assert np.array_equal(x, y) and np.array_equal(x, z)
+    B.close()
+
+
+@pytest.mark.parametrize("R", [1, 8, 16, 32])
+def test_launch_shapes_agree(gpu_ctx, R):
+    probs = mixed_batch(4, base_seed=R)
+    B = make_batch(gpu_ctx, probs, R=R)
+    ref, _ = B.run()
+    Rp = 1 << (R - 1).bit_length()
+    for cpl in (1, 2, 4):
+        if Rp % cpl or Rp // cpl > 64 or (cpl == 4 and Rp < 8) or (cpl == 2 and Rp < 2):
+            continue
+        for bpc, xcd in ((2, 0), (8, 1)):
+            B.set_shape(cpl, bpc, xcd)
+            mus, st = B.run(graphs=True)
+            assert st["cpl"] == cpl and st["blocks_per_cu"] == bpc
+            for m, r in zip(mus, ref):
+                assert np.max(np.abs(m - r)) <= 1e-12 * np.max(np.abs(r))
+    B.close()
+
+
+def test_autotune_and_fp32_batch(gpu_ctx):
+    probs = mixed_batch(6, base_seed=21)
+    B64 = make_batch(gpu_ctx, probs, R=8)
+    ref, _ = B64.run()
+    B64.close()
+    B32 = make_batch(gpu_ctx, probs, R=8, precision=kpm.FP32)
+    ms = B32.autotune(steps=4)
+    assert ms > 0
+    mus, st = B32.run(graphs=True)
+    assert st["cpl"] in (1, 2, 4)
+    for m, r in zip(mus, ref):
+        assert np.max(np.abs(m - r)) <= 1e-5 * np.max(np.abs(r))
+    # fp32 batch = fp32 single-problem path to moment-sum rounding
+    H, a, b, n, seed = probs[2]
+    s32, _ = single(gpu_ctx, H, a, b, n, seed, 8, kpm.FP32)
+    assert np.max(np.abs(mus[2] - s32)) <= 1e-12 * np.max(np.abs(s32))
+    B32.close()
+
+
+def test_batch_errors(gpu_ctx):
+    from tight_binding_amd import _native as N
+
+    H, a = model_problem("square", 6, 0.0, 1)
+    Hc = L.PartialLattice(L.UnitCell.model("square").add_hopping_nn("A", 1, 1.0), (6, 6)).get_Hamiltonian_sparse(True)
+    with pytest.raises(N.NativeError) as e:
+        kpm.Batch(gpu_ctx, [H, Hc], [a, a], [0, 0], [8, 8], 4)
+    assert e.value.code == N.TBK_E_INVALID
+    with pytest.raises(N.NativeError) as e:
+        kpm.Batch(gpu_ctx, [H], [a], [0], [1], 4)
+    assert e.value.code == N.TBK_E_INVALID
+    B = kpm.Batch(gpu_ctx, [H], [a], [0], [8], 8)
+    with pytest.raises(N.NativeError) as e:
+        B.set_shape(3, 4, 1)
+    assert e.value.code == N.TBK_E_UNSUPPORTED
+    B.close()

@@ -96,6 +96,26 @@ class tbk_kubo_stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class tbk_batch_stats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("launches", C.c_int64),
+        ("graphs", C.c_int64),
+        ("bytes_total", C.c_double),
+        ("rows_total", C.c_int64),
+        ("tile_rows", C.c_int32),
+        ("cpl", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("xcd_split", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+TBK_BATCH_GRAPHS = 1
+
 TBK_GAMMA_FP64 = 0
 TBK_GAMMA_BF16X3 = 1
 
@@ -147,6 +167,11 @@ KPM_SIGNATURES = {
     "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
     "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
     "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),
+    "tbk_batch_create": ([P, I32, C.POINTER(tbk_sparse), DP, DP, IP, I32, I32, C.POINTER(U64), PP], I32),
+    "tbk_batch_run": ([P, I32, DP, C.POINTER(tbk_batch_stats)], I32),
+    "tbk_batch_set_shape": ([P, I32, I32, I32], I32),
+    "tbk_batch_autotune": ([P, I32, DP], I32),
+    "tbk_batch_destroy": ([P], I32),
 }
 
 HOST_SIGNATURES = {

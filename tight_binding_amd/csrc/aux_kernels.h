@@ -1,6 +1,7 @@
 // Small device kernels of the engine other than the recursion step:
 // moment-partial reduction, random-phase block generation, block copy and
-// fp64 -> fp32 value conversion.  Included by tbkpm.hip only.
+// fp64 -> fp32 value conversion.  The kernels have internal linkage: each
+// translation unit that includes this header launches its own copy.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,6 +10,7 @@
 #include "cheb_kernels.h"
 
 namespace tbk {
+namespace {
 
 // out[s] = sum_b partial[s*nb + b], fixed order; one workgroup per step.
 __global__ __launch_bounds__(kBlock) void reduce_partials_kernel(const double2* __restrict__ partial,
@@ -97,4 +99,5 @@ __global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double* __restric
         out[i] = static_cast<float>(in[i]);
 }
 
+}  // namespace
 }  // namespace tbk

@@ -160,6 +160,111 @@ __device__ __forceinline__ void tile_range(int64_t ntiles, int xcd_split, int64_
     }
 }
 
+// One site row r of a step (q: this lane's column group): the neighbour
+// gathers, y_r = alpha*acc + beta*x_r - z_r, and the moment partials of the
+// mode added to p0, p1.
+template <typename VT, bool CH, int LPR, int CPL, int MODE>
+__device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                         const typename HVal<VT, CH>::type* __restrict__ hv,
+                                         const typename Vec2<VT>::type* __restrict__ x,
+                                         const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y,
+                                         const typename Vec2<VT>::type* __restrict__ bra, int r, int q, int32_t Rp,
+                                         VT alpha, VT beta, int32_t opts, double& p0, double& p1) {
+    using V2 = typename Vec2<VT>::type;
+    using H = typename HVal<VT, CH>::type;
+    const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * CPL;
+
+    V2 zv[CPL], bv[CPL], acc[CPL], xs[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        acc[c].x = 0;
+        acc[c].y = 0;
+        xs[c].x = 0;
+        xs[c].y = 0;
+    }
+    if (MODE != kFirst) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) zv[c] = ld_stream(z + own + c, opts);
+    }
+    if (MODE != kRecDbl) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) bv[c] = ld_stream(bra + own + c, opts);
+    }
+    const int s = rp[r], e = rp[r + 1];
+    bool have_self = false;
+    int p = s;
+    for (; p + 4 <= e; p += 4) {
+        int cc[4];
+        H hh[4];
+        V2 xv[4][CPL];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            cc[k] = ci[p + k];
+            hh[k] = hv[p + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const V2* xr = x + static_cast<size_t>(cc[k]) * Rp + static_cast<size_t>(q) * CPL;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) xv[k][c] = xr[c];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) madd(acc[c], hh[k], xv[k][c]);
+            if (cc[k] == r) {
+                have_self = true;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) xs[c] = xv[k][c];
+            }
+        }
+    }
+    for (; p < e; ++p) {
+        const int cc = ci[p];
+        const H hh = hv[p];
+        const V2* xr = x + static_cast<size_t>(cc) * Rp + static_cast<size_t>(q) * CPL;
+        V2 xv[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) xv[c] = xr[c];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) madd(acc[c], hh, xv[c]);
+        if (cc == r) {
+            have_self = true;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) xs[c] = xv[c];
+        }
+    }
+    if (!have_self) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) xs[c] = x[own + c];
+    }
+
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        V2 v;
+        v.x = alpha * acc[c].x;
+        v.y = alpha * acc[c].y;
+        if (opts & kOptShift) {
+            v.x = v.x + beta * xs[c].x;
+            v.y = v.y + beta * xs[c].y;
+        }
+        if (MODE != kFirst) {
+            v.x = v.x - zv[c].x;
+            v.y = v.y - zv[c].y;
+        }
+        st_stream(y + own + c, v, opts);
+        if (MODE == kFirst) {
+            p0 = p0 + redot(bv[c], xs[c]);
+            p1 = p1 + redot(bv[c], v);
+        } else if (MODE == kRec) {
+            p0 = p0 + redot(bv[c], v);
+        } else {
+            p0 = p0 + redot(xs[c], xs[c]);
+            p1 = p1 + redot(v, xs[c]);
+        }
+    }
+}
+
 template <typename VT, bool CH, int LPR, int CPL, int MODE>
 __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
@@ -167,8 +272,6 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y,
     const typename Vec2<VT>::type* __restrict__ bra, int32_t rows, int32_t Rp, VT alpha, VT beta,
     int32_t opts, int64_t ntiles, TileOrder order, int32_t xcd_split, double2* __restrict__ partial) {
-    using V2 = typename Vec2<VT>::type;
-    using H = typename HVal<VT, CH>::type;
     constexpr int RPW = kWave / LPR;
     constexpr int TR = kWavesPerBlock * RPW;
     static_assert(LPR >= 1 && LPR <= kWave && (kWave % LPR) == 0, "bad lanes per row");
@@ -189,97 +292,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
         int r = static_cast<int>(rt * TR) + wave * RPW + sub;
         if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
         if (r >= rows) continue;
-        const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * CPL;
-
-        V2 zv[CPL], bv[CPL], acc[CPL], xs[CPL];
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            acc[c].x = 0;
-            acc[c].y = 0;
-            xs[c].x = 0;
-            xs[c].y = 0;
-        }
-        if (MODE != kFirst) {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) zv[c] = ld_stream(z + own + c, opts);
-        }
-        if (MODE != kRecDbl) {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) bv[c] = ld_stream(bra + own + c, opts);
-        }
-        const int s = rp[r], e = rp[r + 1];
-        bool have_self = false;
-        int p = s;
-        for (; p + 4 <= e; p += 4) {
-            int cc[4];
-            H hh[4];
-            V2 xv[4][CPL];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                cc[k] = ci[p + k];
-                hh[k] = hv[p + k];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const V2* xr = x + static_cast<size_t>(cc[k]) * Rp + static_cast<size_t>(q) * CPL;
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) xv[k][c] = xr[c];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) madd(acc[c], hh[k], xv[k][c]);
-                if (cc[k] == r) {
-                    have_self = true;
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c) xs[c] = xv[k][c];
-                }
-            }
-        }
-        for (; p < e; ++p) {
-            const int cc = ci[p];
-            const H hh = hv[p];
-            const V2* xr = x + static_cast<size_t>(cc) * Rp + static_cast<size_t>(q) * CPL;
-            V2 xv[CPL];
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) xv[c] = xr[c];
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) madd(acc[c], hh, xv[c]);
-            if (cc == r) {
-                have_self = true;
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) xs[c] = xv[c];
-            }
-        }
-        if (!have_self) {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) xs[c] = x[own + c];
-        }
-
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            V2 v;
-            v.x = alpha * acc[c].x;
-            v.y = alpha * acc[c].y;
-            if (opts & kOptShift) {
-                v.x = v.x + beta * xs[c].x;
-                v.y = v.y + beta * xs[c].y;
-            }
-            if (MODE != kFirst) {
-                v.x = v.x - zv[c].x;
-                v.y = v.y - zv[c].y;
-            }
-            st_stream(y + own + c, v, opts);
-            if (MODE == kFirst) {
-                p0 = p0 + redot(bv[c], xs[c]);
-                p1 = p1 + redot(bv[c], v);
-            } else if (MODE == kRec) {
-                p0 = p0 + redot(bv[c], v);
-            } else {
-                p0 = p0 + redot(xs[c], xs[c]);
-                p1 = p1 + redot(v, xs[c]);
-            }
-        }
+        step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q, Rp, alpha, beta, opts, p0, p1);
     }
 
     // deterministic block reduction: butterfly in the wave, then waves in order
@@ -298,6 +311,54 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
             s1 = s1 + red[1][w];
         }
         partial[blockIdx.x] = make_double2(s0, s1);
+    }
+}
+
+// Batched step over many independent problems held as one block-diagonal
+// matrix (global row and column numbering) in which every problem starts on
+// a tile boundary; a tile is tile_rows consecutive rows (a multiple of the
+// rows one workgroup pass covers).  Tile t belongs to problem tile_prob[t],
+// whose step coefficients are coef[4p .. 4p+3] = (1/a, -b/a, 2/a, -2b/a)
+// (first step, recursion); the shift term is applied where b != 0, as in a
+// single-problem step.  Each wave writes its partial sums of a tile to
+// partial[4t + wave]: a problem's moments depend on its own tiles only, not
+// on the other problems of the batch or on which workgroup ran which tile.
+// Only tiles t < ntiles are visited (the problems still active).
+template <typename VT, bool CH, int LPR, int CPL, int MODE>
+__global__ __launch_bounds__(kBlock) void cheb_batch_kernel(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const typename HVal<VT, CH>::type* __restrict__ hv, const typename Vec2<VT>::type* __restrict__ x,
+    const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y, int32_t rows, int32_t Rp,
+    const int32_t* __restrict__ tile_prob, const double* __restrict__ coef, int32_t tile_rows, int32_t opts,
+    int64_t ntiles, int32_t xcd_split, double2* __restrict__ partial) {
+    constexpr int RPW = kWave / LPR;
+    constexpr int TR = kWavesPerBlock * RPW;
+    static_assert(LPR >= 1 && LPR <= kWave && (kWave % LPR) == 0, "bad lanes per row");
+    static_assert(MODE != kRec, "batched steps use the bra = ket modes");
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int sub = lane / LPR;
+    const int q = lane % LPR;
+
+    int64_t first, end, stride;
+    tile_range(ntiles, xcd_split, first, end, stride);
+    for (int64_t t = first; t < end; t += stride) {
+        const int p = tile_prob[t];
+        const double ca = coef[4 * p + (MODE == kFirst ? 0 : 2)];
+        const double cb = coef[4 * p + (MODE == kFirst ? 1 : 3)];
+        const VT alpha = static_cast<VT>(ca), beta = static_cast<VT>(cb);
+        const int32_t o = (cb != 0.0) ? (opts | kOptShift) : (opts & ~kOptShift);
+        double p0 = 0.0, p1 = 0.0;
+        const int64_t base = t * tile_rows;
+        for (int g = 0; g < tile_rows; g += TR) {
+            int r = static_cast<int>(base + g) + wave * RPW + sub;
+            if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
+            if (r < rows) step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
+        }
+        p0 = wave_sum(p0);
+        p1 = wave_sum(p1);
+        if (lane == 0) partial[4 * t + wave] = make_double2(p0, p1);
     }
 }
 

@@ -5,4 +5,7 @@ namespace tbk {
 bool launch_f32_real(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
     return launch_width<float, false>(width, kch, mode, grid, st, a);
 }
+bool launch_batch_f32_real(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const BatchArgs& a) {
+    return launch_batch<float, false>(Rp, cpl, mode, grid, st, a);
+}
 }  // namespace tbk

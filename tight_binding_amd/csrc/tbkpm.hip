@@ -24,128 +24,13 @@
 #include "aux_kernels.h"
 #include "cheb_kernels.h"
 #include "gamma_launch.h"
+#include "engine.h"
 #include "step_launch.h"
 #include "tbkpm.h"
 
-namespace {
-
-thread_local std::string g_last_error;
-
-struct Error {
-    int code;
-    std::string msg;
-};
-
-[[noreturn]] void raise(int code, const std::string& msg) { throw Error{code, msg}; }
-
-void hip_check(hipError_t e, const char* what) {
-    if (e == hipSuccess) return;
-    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
-        raise(TBK_E_OOM, std::string(what) + ": " + hipGetErrorString(e));
-    raise(TBK_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPC(expr) hip_check((expr), #expr)
-
-template <typename F>
-int guarded(F&& f) {
-    try {
-        f();
-        return TBK_OK;
-    } catch (const Error& e) {
-        g_last_error = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return TBK_E_OOM;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return TBK_E_INVALID;
-    } catch (...) {
-        g_last_error = "unknown error";
-        return TBK_E_INVALID;
-    }
-}
-
-// Grow-only device buffer.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void reserve(size_t n) {
-        if (n <= bytes) return;
-        release();
-        if (n == 0) return;
-        HIPC(hipMalloc(&p, n));
-        bytes = n;
-    }
-    template <typename T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-int pow2_at_least(int v) {
-    int r = 1;
-    while (r < v) r <<= 1;
-    return r;
-}
-
-}  // namespace
-
-struct tbk_context {
-    int device = 0;
-    int num_cu = 0;
-    hipStream_t stream = nullptr;
-    DevBuf work[2];
-    DevBuf partial;
-    DevBuf sums;
-    DevBuf stack[2];                 // Kubo left/right vector stacks, kept between calls
-    DevBuf gamma_part, gamma_dev;    // split-K Gamma partials and the reduced Gamma
-    std::vector<hipEvent_t> events;
-    int64_t held_bytes = 0;  // matrices + blocks owned through this context
-    tbk_tuning tuning{};     // zeros: automatic choices
-};
-
-struct tbk_matrix {
-    tbk_context* ctx = nullptr;
-    int32_t rows = 0;
-    int64_t nnz = 0;
-    bool is_complex = false;
-    int32_t max_row_nnz = 0;
-    int32_t bandwidth95 = 0;  // 95th percentile of |row - col| over the entries
-    DevBuf rp, ci, val64, val32;
-    bool have32 = false;
-};
-
-struct tbk_block {
-    tbk_context* ctx = nullptr;
-    int32_t rows = 0, R = 0, Rp = 0, precision = TBK_FP64;
-    DevBuf data;
-    size_t elem_bytes() const { return precision == TBK_FP64 ? 16 : 8; }
-};
-
-namespace {
-
-void set_device(const tbk_context* ctx) { HIPC(hipSetDevice(ctx->device)); }
-
-void ensure_events(tbk_context* ctx, size_t n) {
-    while (ctx->events.size() < n) {
-        hipEvent_t e;
-        HIPC(hipEventCreate(&e));
-        ctx->events.push_back(e);
-    }
-}
+namespace engine {
 
 // --------------------------------------------------------- kernel dispatch
-using tbk::StepArgs;
 
 void launch_any(int precision, bool cplx, int width, int kch, int mode, dim3 grid, hipStream_t st,
                 const StepArgs& a) {
@@ -156,13 +41,6 @@ void launch_any(int precision, bool cplx, int width, int kch, int mode, dim3 gri
         ok = cplx ? tbk::launch_f32_cplx(width, kch, mode, grid, st, a) : tbk::launch_f32_real(width, kch, mode, grid, st, a);
     if (!ok) raise(TBK_E_UNSUPPORTED, "no step kernel for this block width");
 }
-
-// Kernel family, slab width and rows per tile for a matrix and block shape.
-struct KernelChoice {
-    int kch;            // 0: row kernel
-    int width;          // columns one sweep covers (the slab width; Rp for the row kernel)
-    int rows_per_tile;
-};
 
 KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int Rp, int max_row_nnz,
                            int bandwidth, bool force_row) {
@@ -258,18 +136,6 @@ void matrix_fp32(tbk_matrix* m) {
     m->have32 = true;
 }
 
-// ------------------------------------------------------------ the driver
-// Launch plan of one step kernel for a matrix and a block shape: kernel
-// family, slab width, visiting order and grid.  Shared by every recursion
-// driver (moments, Kubo stacks).
-struct StepPlan {
-    KernelChoice kc;
-    StepArgs args;  // x, z, y, bra, alpha, beta, partial filled per launch
-    int64_t nb;
-    int precision;
-    bool cplx;
-};
-
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row) {
     StepPlan pl{};
     pl.precision = prec;
@@ -338,6 +204,12 @@ void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, dou
     launch_any(pl.precision, pl.cplx, pl.kc.width, pl.kc.kch, mode, dim3(static_cast<unsigned>(pl.nb)), st, A);
     HIPC(hipGetLastError());
 }
+
+}  // namespace engine
+
+namespace {
+
+using namespace engine;
 
 void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* bras,
                  const tbk_block* kets, int32_t n, int32_t flags, double* mu_out, double* last_out,

@@ -229,6 +229,60 @@ def kubo_zeta(ctx: Context, gamma: np.ndarray, e, volume: float = 1.0):
     return out.view(np.complex128), ms.value
 
 
+class Batch:
+    """Independent moment problems run as one block-diagonal recursion
+    (tbk_batch_create): mats[p] with rescaling a[p], b[p], n_moments[p]
+    moments, R device-generated random-phase vectors from seeds[p].  All
+    problems share R, the precision and real-or-complex values."""
+
+    def __init__(self, ctx: Context, mats, a, b, n_moments, R: int, precision: int = FP64, seeds=None):
+        n = len(mats)
+        arr = (N.tbk_sparse * n)()
+        keep = []
+        for i, m in enumerate(mats):
+            s_, k_ = tbk_sparse_of(m)
+            arr[i] = s_
+            keep.append(k_)
+        self.n_moments = np.ascontiguousarray(n_moments, dtype=np.int32)
+        assert self.n_moments.size == n
+        aa = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,)))
+        bb = np.ascontiguousarray(np.broadcast_to(np.asarray(b, dtype=np.float64), (n,)))
+        sd = np.ascontiguousarray(seeds if seeds is not None else np.arange(n), dtype=np.uint64)
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_batch_create(ctx._h, n, arr, _dp(aa), _dp(bb),
+                                             self.n_moments.ctypes.data_as(C.POINTER(C.c_int32)), int(R),
+                                             int(precision), sd.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(h)))
+        del keep
+        self.ctx, self._h, self.R, self.precision = ctx, h, int(R), int(precision)
+        self.offsets = np.concatenate([[0], np.cumsum(self.n_moments)])
+
+    def run(self, graphs: bool = False):
+        """Returns (list of moment arrays in problem order, stats)."""
+        mu = np.zeros(int(self.offsets[-1]), dtype=np.float64)
+        st = N.tbk_batch_stats()
+        N.check_kpm(N.kpm().tbk_batch_run(self._h, N.TBK_BATCH_GRAPHS if graphs else 0, _dp(mu), C.byref(st)))
+        return [mu[self.offsets[i]:self.offsets[i + 1]] for i in range(self.n_moments.size)], st.as_dict()
+
+    def set_shape(self, cpl=1, blocks_per_cu=4, xcd_split=1):
+        N.check_kpm(N.kpm().tbk_batch_set_shape(self._h, int(cpl), int(blocks_per_cu), int(xcd_split)))
+
+    def autotune(self, steps=8) -> float:
+        ms = C.c_double()
+        N.check_kpm(N.kpm().tbk_batch_autotune(self._h, int(steps), C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            N.kpm().tbk_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def cheb_first_kind_seq_braket(M: CSC, n: int, bras_T: np.ndarray, kets: np.ndarray) -> np.ndarray:
     """Drop-in of cuda::Cheb_first_kind_seq_braket: M already rescaled (CSC),
     bras_T and kets (rows, R) complex; passing the same array object for both
