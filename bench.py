@@ -120,6 +120,103 @@ def kubo_leg(ctx, cells, N, R, repeats):
     return out
 
 
+BATCH_COORD = {"square": 4, "honeycomb": 3, "lieb": 4, "kagome": 4}
+
+
+def build_batch_problems(n, seed=2028, lmin=32, lmax=512):
+    """n problems: model uniform over square, honeycomb, Lieb, Kagome
+    (nearest-neighbour t = 1, periodic), L ~ U{lmin..lmax} cells per side,
+    on-site disorder uniform in [-W/2, W/2] with W ~ U[0, 3], N moments from
+    {256, 512, 1024}; a from the Gershgorin bound.  Returns
+    [(H, a, N, seed, model, L, W)]."""
+    from tight_binding_amd import lattice as Lm
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        model = ["square", "honeycomb", "lieb", "kagome"][int(rng.integers(0, 4))]
+        Lc = int(rng.integers(lmin, lmax + 1))
+        W = float(rng.uniform(0.0, 3.0))
+        N = int(rng.choice([256, 512, 1024]))
+        c = Lm.UnitCell.model(model)
+        c.add_hopping_nn("A", 1, 1.0)
+        if model == "kagome":
+            c.add_hopping_nn("B", 1, 1.0)
+        lat = Lm.PartialLattice(c, (Lc, Lc))
+        lat.set_disorder(W, seed + i)
+        H = lat.get_Hamiltonian_periodic_extend_bare_sparse((1, 1))
+        out.append((H, 1.02 * (BATCH_COORD[model] + W / 2), N, 5000 + i, model, Lc, W))
+    return out
+
+
+def batch_leg(ctx, nprob, R=8):
+    """Batched ragged DoS: nprob problems in one fp32 batch (fp64 moments),
+    autotuned launch shape, each 64-step chunk a hipGraph; against the plain
+    per-solve GPU path (one tbk_moments call per problem) on the same inputs."""
+    from tight_binding_amd import kpm
+
+    t0 = time.perf_counter()
+    probs = build_batch_problems(nprob)
+    sites = sum(p[0].n for p in probs)
+    moments = sum(p[2] for p in probs)
+    log("batch: %d problems, %d sites, %d moments, assembled in %.1f s" % (nprob, sites, moments,
+                                                                          time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    B = kpm.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], 0.0, [p[2] for p in probs], R, kpm.FP32,
+                  seeds=[p[3] for p in probs])
+    create_s = time.perf_counter() - t0
+    tune_ms = B.autotune(steps=6)
+    B.run(graphs=True)  # captures the chunk graphs
+    runs = []
+    for _ in range(2):
+        mus, st = B.run(graphs=True)
+        runs.append(st)
+    st = min(runs, key=lambda d: d["kernel_ms"])
+    gbps = st["bytes_total"] / (st["kernel_ms"] * 1e-3) / 1e9
+    log("batch: kernel %.1f ms, wall %.1f ms, %.1f GB/s, shape cpl=%d bpc=%d xcd=%d (create %.1f s, tune %.3f ms/step)"
+        % (st["kernel_ms"], st["wall_ms"], gbps, st["cpl"], st["blocks_per_cu"], st["xcd_split"], create_s, tune_ms))
+    B.close()
+
+    # plain per-solve path: one moments call per problem, matrices resident
+    mats = [ctx.matrix(p[0]) for p in probs]
+    ctx.synchronize()
+    k_ms, w_s = 0.0, 0.0
+    errs = []
+    for i, (p, M) in enumerate(zip(probs, mats)):
+        K = ctx.random_block(p[0].n, R, seed=p[3], precision=kpm.FP32)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        mu1, s1, _ = ctx.moments(M, K, p[2], a=p[1])
+        w_s += time.perf_counter() - t1
+        k_ms += s1["kernel_ms"]
+        K.close()
+        if i < 4:  # fp32 error against the fp64 recursion on the same vectors
+            K64 = ctx.random_block(p[0].n, R, seed=p[3])
+            mu64, _, _ = ctx.moments(M, K64, p[2], a=p[1])
+            errs.append(float(np.max(np.abs(mus[i] - mu64)) / np.max(np.abs(mu64))))
+            assert np.max(np.abs(mus[i] - mu1)) <= 1e-9 * np.max(np.abs(mu1))
+            K64.close()
+    for M in mats:
+        M.close()
+    out = {
+        "workload": "%d independent DoS problems (square/honeycomb/Lieb/Kagome, L~U{32..512}, W~U[0,3], "
+                    "N in {256,512,1024}), R=%d, fp32 recursion with fp64 moments" % (nprob, R),
+        "sites_total": sites, "moments_total": moments,
+        "batched_kernel_ms": st["kernel_ms"], "batched_wall_ms": st["wall_ms"],
+        "batched_moments_per_s": moments / (st["wall_ms"] * 1e-3),
+        "batched_GBps": gbps, "batched_frac_hbm": gbps / HBM_PEAK_GBS,
+        "launch_shape": {"cpl": st["cpl"], "blocks_per_cu": st["blocks_per_cu"], "xcd_split": st["xcd_split"],
+                         "tile_rows": st["tile_rows"]},
+        "graphs_per_run": st["graphs"], "step_launches": st["launches"],
+        "per_solve_kernel_ms": k_ms, "per_solve_wall_ms": 1e3 * w_s,
+        "per_solve_moments_per_s": moments / w_s,
+        "speedup_wall": (1e3 * w_s) / st["wall_ms"],
+        "fp32_max_rel_err_vs_fp64_sample": max(errs) if errs else None,
+    }
+    log("batch: %s" % json.dumps(out))
+    return out
+
+
 def cpu_baseline(H, R, a, seconds):
     """The oracle (C restatement of the reference's CPU recursion, single
     thread) timed on a bounded number of steps of the same workload."""
@@ -165,6 +262,8 @@ def main():
     ap.add_argument("--kubo-N", type=int, default=512)
     ap.add_argument("--kubo-R", type=int, default=32)
     ap.add_argument("--kubo-only", action="store_true", help="run only the sigma_xy leg")
+    ap.add_argument("--batch-problems", type=int, default=256, help="problems of the batched leg (0: skip)")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -173,6 +272,12 @@ def main():
         sys.exit(2)
 
     from tight_binding_amd import kpm
+
+    if args.batch_only:
+        ctx = kpm.Context(0)
+        print(json.dumps({"batch": batch_leg(ctx, max(1, args.batch_problems))}), flush=True)
+        ctx.close()
+        return
 
     if args.kubo_only:
         ctx = kpm.Context(0)
@@ -262,6 +367,10 @@ def main():
         M.close()
         K.close()
         kubo_out = kubo_leg(ctx, args.kubo_cells, args.kubo_N, args.kubo_R, args.kubo_repeats)
+    batch_out = None
+    if args.batch_problems > 0:
+        ctx.release_work()
+        batch_out = batch_leg(ctx, args.batch_problems)
 
     value = st["steps"] / (st["kernel_ms"] * 1e-3)
     out = {
@@ -303,6 +412,8 @@ def main():
         out["variants"] = extra
     if kubo_out:
         out["kubo_sigma_xy"] = kubo_out
+    if batch_out:
+        out["batched_ragged"] = batch_out
     print(json.dumps(out), flush=True)
     ctx.close()
 
