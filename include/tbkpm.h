@@ -272,6 +272,76 @@ int tbk_batch_set_shape(tbk_batch* batch, int32_t cpl, int32_t blocks_per_cu, in
 int tbk_batch_autotune(tbk_batch* batch, int32_t steps, double* best_ms_per_step);
 int tbk_batch_destroy(tbk_batch* batch);
 
+/* ------------------------------------------------------ request server */
+/*
+ * A serving loop on one device (no reference counterpart: the reference
+ * runs one solve per call).  Requests are copied at submit and queued; a
+ * worker thread admits them while others run, as soon as their device-memory
+ * footprint fits the budget and a slot is free; DoS requests of one class
+ * (precision, real/complex, padded R) advance together in batched chunks,
+ * sigma_xy requests run on a second stream beside them.  A DoS result is the
+ * same bits alone and in any mix of requests.  All functions are thread-safe.
+ */
+typedef struct tbk_server tbk_server;
+
+enum { TBK_REQ_DOS = 0, TBK_REQ_KUBO = 1 };
+enum { TBK_REQ_QUEUED = 0, TBK_REQ_RUNNING = 1, TBK_REQ_DONE = 2, TBK_REQ_FAILED = 3 };
+
+typedef struct tbk_server_config {
+    int64_t hbm_budget_bytes;  /* device memory for requests (0: 85% of the free memory at creation) */
+    int32_t max_slots;         /* DoS requests on the device at once (0: 512) */
+    int32_t max_kubo;          /* sigma_xy requests on the device at once (0: 1) */
+    int32_t chunk_steps;       /* steps per batched chunk (0: 32) */
+    int32_t cpl;               /* complex values per lane of the batched launches (0: automatic) */
+    int32_t max_moments;       /* largest DoS moment count accepted (0: 4096) */
+    int32_t max_kubo_moments;  /* largest sigma_xy N accepted (0: 1024) */
+    double max_wait_bypass_ms; /* a request queued longer than this is not overtaken (0: 1000) */
+} tbk_server_config;
+
+typedef struct tbk_request {
+    int32_t kind;      /* TBK_REQ_DOS or TBK_REQ_KUBO */
+    int32_t precision; /* DoS: TBK_FP64 or TBK_FP32; sigma_xy: TBK_GAMMA_FP64 or TBK_GAMMA_BF16X3 */
+    int32_t n_moments; /* DoS: moments (>= 2); sigma_xy: N of the N x N Gamma */
+    int32_t R;         /* random-phase vectors, generated as tbk_block_random_phase(rows, R, seed) */
+    uint64_t seed;
+    double a, b;       /* H~ = (H - b)/a, fused into the steps */
+    const tbk_sparse* H;
+    const tbk_sparse* vx; /* sigma_xy: velocity operators, used as given */
+    const tbk_sparse* vy;
+    const double* g;   /* sigma_xy: kernel weights g_n (N of them) or NULL for ones */
+} tbk_request;
+
+typedef struct tbk_request_info {
+    int32_t state;          /* TBK_REQ_* */
+    int32_t error_code;     /* TBK_E_* of a failed request */
+    double queue_ms;        /* submit to admission */
+    double run_ms;          /* admission to completion */
+    double latency_ms;      /* submit to completion (so far, if not finished) */
+    int64_t footprint_bytes;
+    int64_t result_doubles; /* DoS: n moments; sigma_xy: 2N (zeta at the N Chebyshev nodes) + 2N^2 (Gamma) */
+} tbk_request_info;
+
+typedef struct tbk_server_stats {
+    int64_t submitted, completed, failed, queued;
+    int64_t running_dos, running_kubo, kubo_done, chunks;
+    int64_t used_bytes, budget_bytes;
+} tbk_server_stats;
+
+int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server** out);
+/* copy the request's inputs and queue it; TBK_E_OOM if its footprint exceeds the budget */
+int tbk_server_submit(tbk_server* s, const tbk_request* r, int64_t* ticket);
+int tbk_server_info(tbk_server* s, int64_t ticket, tbk_request_info* info);
+/* block until the request is done or failed, or timeout_s passes (< 0: no limit) */
+int tbk_server_wait(tbk_server* s, int64_t ticket, double timeout_s, tbk_request_info* info);
+/* copy a finished request's result (DoS: the n moments; sigma_xy: zeta_k at
+   e_k = cos((k + 1/2) pi / N), k < N, as complex128, then the weighted N x N
+   Gamma row-major complex128) and forget the ticket; a failed request returns
+   its error code */
+int tbk_server_take(tbk_server* s, int64_t ticket, double* out, int64_t capacity, tbk_request_info* info);
+int tbk_server_get_stats(tbk_server* s, tbk_server_stats* st);
+/* finish every accepted request, then stop */
+int tbk_server_destroy(tbk_server* s);
+
 #ifdef __cplusplus
 }
 #endif

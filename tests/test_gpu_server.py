@@ -1,0 +1,179 @@
+"""The request server (tbk_server_*): continuous batching of DoS and sigma_xy
+requests on one GPU.  A DoS result must equal the single-problem path (same
+vectors; moment sums in another order: 1e-12 relative) and be the same bits
+alone and in any mix; a sigma_xy result must be the direct path's Gamma bit
+for bit; admission must respect the memory budget."""
+import math
+import time
+
+import numpy as np
+import pytest
+
+from tests.test_gpu_batch import model_problem
+from tight_binding_amd import _native as Nn
+from tight_binding_amd import kpm
+from tight_binding_amd import lattice as L
+from tight_binding_amd import spectral as S
+
+pytestmark = pytest.mark.gpu
+
+
+def dos_requests(k=8, seed=4):
+    rng = np.random.default_rng(seed)
+    models = ["square", "honeycomb", "lieb", "kagome"]
+    out = []
+    for i in range(k):
+        H, a = model_problem(models[i % 4], int(rng.integers(6, 40)), float(rng.uniform(0, 2)), seed=50 + i)
+        out.append(dict(H=H, n=int(rng.choice([2, 5, 40, 64, 101, 200])), a=a, b=0.05 * (i % 2),
+                        R=int(rng.choice([8, 16])), seed=900 + i,
+                        precision=kpm.FP64 if i % 3 else kpm.FP32))
+    return out
+
+
+def run_all(srv, reqs, stagger=0.0):
+    tickets = []
+    for r in reqs:
+        tickets.append(srv.submit_dos(r["H"], r["n"], r["a"], r["b"], r["R"], r["seed"], r["precision"]))
+        if stagger:
+            time.sleep(stagger)
+    out = []
+    for t in tickets:
+        info = srv.wait(t, 60.0)
+        assert info["state"] == Nn.TBK_REQ_DONE, info
+        mu, info = srv.take(t)
+        out.append(mu)
+    return out
+
+
+def test_dos_requests_match_single_path(gpu_ctx):
+    reqs = dos_requests()
+    with kpm.Server(0, chunk_steps=8) as srv:
+        res = run_all(srv, reqs)
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0 and st["used_bytes"] == 0
+    for r, mu in zip(reqs, res):
+        M = gpu_ctx.matrix(r["H"])
+        K = gpu_ctx.random_block(r["H"].n, r["R"], seed=r["seed"], precision=r["precision"])
+        ref, _, _ = gpu_ctx.moments(M, K, r["n"], a=r["a"], b=r["b"])
+        assert mu.size == r["n"]
+        assert np.max(np.abs(mu - ref)) <= 1e-12 * np.max(np.abs(ref)), (r["n"], np.max(np.abs(mu - ref)))
+
+
+def test_alone_and_in_mix_are_bit_identical():
+    reqs = dos_requests(10, seed=8)
+    target = 6
+    with kpm.Server(0, chunk_steps=4) as srv:
+        alone = run_all(srv, [reqs[target]])[0]
+    # the same request amid others, submitted while they are running
+    with kpm.Server(0, chunk_steps=4) as srv:
+        mixed = run_all(srv, reqs, stagger=0.002)
+    assert np.array_equal(alone, mixed[target])
+    # and with another chunk length and slot width
+    with kpm.Server(0, chunk_steps=16, cpl=1) as srv:
+        again = run_all(srv, reqs[::-1])
+    for i, mu in enumerate(again[::-1]):
+        with kpm.Server(0, chunk_steps=16, cpl=1) as s2:
+            solo = run_all(s2, [reqs[i]])[0]
+        assert np.array_equal(solo, mu)
+        if i >= 2:
+            break
+
+
+def landau_kubo(L0=8, N=24, R=4):
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (L0, L0))
+    lat.set_gauge(L.GAUGE_LANDAU, 2 * math.pi / 8)
+    lat.set_disorder(1.0, 3)
+    H = lat.get_Hamiltonian_sparse(True)
+    Hs = L.rescale(H, 5.2, 0.0)
+    X, Y = lat.op_position(0), lat.op_position(1)
+    vx, vy = L.commutator_diag(X, Hs), L.commutator_diag(Y, Hs)
+    return Hs, vx, vy, S.lorentz(N, 6.0)
+
+
+@pytest.mark.parametrize("prec", [kpm.GAMMA_FP64, kpm.GAMMA_BF16X3])
+def test_kubo_request_matches_direct_path(gpu_ctx, prec):
+    Hs, vx, vy, g = landau_kubo()
+    N = g.size
+    ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
+                            gpu_ctx.random_block(Hs.n, 4, seed=77), N, precision=prec, g=g)
+    with kpm.Server(0) as srv:
+        t = srv.submit_kubo(Hs, vx, vy, N, R=4, seed=77, precision=prec, g=g)
+        assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
+        (z, G), info = srv.take(t)
+    assert info["result_doubles"] == 2 * N + 2 * N * N
+    assert np.array_equal(G, ref)
+    zref, _ = kpm.kubo_zeta(gpu_ctx, ref, S.kubo_nodes(N))
+    assert np.max(np.abs(z - zref)) <= 1e-12 * np.max(np.abs(zref))
+
+
+def test_mixed_dos_and_kubo_overlap_and_results_unchanged():
+    reqs = dos_requests(6, seed=13)
+    Hs, vx, vy, g = landau_kubo(10, 32, 4)
+    with kpm.Server(0, chunk_steps=4) as srv:
+        alone = run_all(srv, reqs)
+        tk = srv.submit_kubo(Hs, vx, vy, g.size, R=4, seed=5, g=g)
+        assert srv.wait(tk, 60.0)["state"] == Nn.TBK_REQ_DONE
+        (z_alone, G_alone), _ = srv.take(tk)
+    with kpm.Server(0, chunk_steps=4, max_kubo=2) as srv:
+        tk = srv.submit_kubo(Hs, vx, vy, g.size, R=4, seed=5, g=g)
+        mixed = run_all(srv, reqs)
+        assert srv.wait(tk, 60.0)["state"] == Nn.TBK_REQ_DONE
+        (z_mix, G_mix), _ = srv.take(tk)
+        st = srv.stats()
+    assert st["kubo_done"] == 1 and st["completed"] == len(reqs) + 1
+    assert np.array_equal(G_alone, G_mix) and np.array_equal(z_alone, z_mix)
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a, b)
+
+
+def test_admission_respects_the_memory_budget():
+    reqs = dos_requests(8, seed=21)
+    # footprint of the largest request, from a probe server
+    with kpm.Server(0) as srv:
+        fp = []
+        for r in reqs:
+            t = srv.submit_dos(r["H"], r["n"], r["a"], r["b"], r["R"], r["seed"], r["precision"])
+            fp.append(srv.wait(t, 60.0)["footprint_bytes"])
+            srv.take(t)
+    budget = int(2.2 * max(fp))
+    with kpm.Server(0, hbm_budget_bytes=budget, chunk_steps=2) as srv:
+        tickets = [srv.submit_dos(r["H"], r["n"], r["a"], r["b"], r["R"], r["seed"], r["precision"]) for r in reqs]
+        peak = 0
+        while True:
+            st = srv.stats()
+            peak = max(peak, st["used_bytes"])
+            if st["completed"] + st["failed"] == len(reqs):
+                break
+            time.sleep(0.0005)
+        for t in tickets:
+            srv.take(t)
+        assert st["failed"] == 0
+    assert peak <= budget
+    # a request larger than the whole budget is refused at submission
+    with kpm.Server(0, hbm_budget_bytes=max(fp) // 2) as srv:
+        big = reqs[int(np.argmax(fp))]
+        with pytest.raises(Nn.NativeError) as e:
+            srv.submit_dos(big["H"], big["n"], big["a"], big["b"], big["R"], big["seed"], big["precision"])
+        assert e.value.code == Nn.TBK_E_OOM
+
+
+def test_server_errors():
+    H, a = model_problem("square", 6, 0.0, 1)
+    with kpm.Server(0, max_moments=64) as srv:
+        with pytest.raises(Nn.NativeError) as e:
+            srv.submit_dos(H, 1, a)
+        assert e.value.code == Nn.TBK_E_INVALID
+        with pytest.raises(Nn.NativeError) as e:
+            srv.submit_dos(H, 100, a)
+        assert e.value.code == Nn.TBK_E_UNSUPPORTED
+        with pytest.raises(Nn.NativeError) as e:
+            srv.submit_dos(H, 10, 0.0)
+        assert e.value.code == Nn.TBK_E_INVALID
+        t = srv.submit_dos(H, 10, a)
+        with pytest.raises(Nn.NativeError):
+            srv.info(t + 100)
+        srv.wait(t)
+        mu, info = srv.take(t)
+        assert info["state"] == Nn.TBK_REQ_DONE and mu.size == 10

@@ -116,6 +116,61 @@ class tbk_batch_stats(C.Structure):
 
 TBK_BATCH_GRAPHS = 1
 
+TBK_REQ_DOS, TBK_REQ_KUBO = 0, 1
+TBK_REQ_QUEUED, TBK_REQ_RUNNING, TBK_REQ_DONE, TBK_REQ_FAILED = 0, 1, 2, 3
+
+
+class tbk_server_config(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_int64),
+        ("max_slots", C.c_int32),
+        ("max_kubo", C.c_int32),
+        ("chunk_steps", C.c_int32),
+        ("cpl", C.c_int32),
+        ("max_moments", C.c_int32),
+        ("max_kubo_moments", C.c_int32),
+        ("max_wait_bypass_ms", C.c_double),
+    ]
+
+
+class tbk_request(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("precision", C.c_int32),
+        ("n_moments", C.c_int32),
+        ("R", C.c_int32),
+        ("seed", C.c_uint64),
+        ("a", C.c_double),
+        ("b", C.c_double),
+        ("H", C.POINTER(tbk_sparse)),
+        ("vx", C.POINTER(tbk_sparse)),
+        ("vy", C.POINTER(tbk_sparse)),
+        ("g", C.POINTER(C.c_double)),
+    ]
+
+
+class tbk_request_info(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("error_code", C.c_int32),
+        ("queue_ms", C.c_double),
+        ("run_ms", C.c_double),
+        ("latency_ms", C.c_double),
+        ("footprint_bytes", C.c_int64),
+        ("result_doubles", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class tbk_server_stats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "queued", "running_dos",
+                                          "running_kubo", "kubo_done", "chunks", "used_bytes", "budget_bytes")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
 TBK_GAMMA_FP64 = 0
 TBK_GAMMA_BF16X3 = 1
 
@@ -172,6 +227,13 @@ KPM_SIGNATURES = {
     "tbk_batch_set_shape": ([P, I32, I32, I32], I32),
     "tbk_batch_autotune": ([P, I32, DP], I32),
     "tbk_batch_destroy": ([P], I32),
+    "tbk_server_create": ([I32, C.POINTER(tbk_server_config), PP], I32),
+    "tbk_server_submit": ([P, C.POINTER(tbk_request), C.POINTER(I64)], I32),
+    "tbk_server_info": ([P, I64, C.POINTER(tbk_request_info)], I32),
+    "tbk_server_wait": ([P, I64, D, C.POINTER(tbk_request_info)], I32),
+    "tbk_server_take": ([P, I64, DP, I64, C.POINTER(tbk_request_info)], I32),
+    "tbk_server_get_stats": ([P, C.POINTER(tbk_server_stats)], I32),
+    "tbk_server_destroy": ([P], I32),
 }
 
 HOST_SIGNATURES = {

@@ -362,6 +362,75 @@ __global__ __launch_bounds__(kBlock) void cheb_batch_kernel(
     }
 }
 
+// Resident problems of a serving loop ("slots"), each with its own CSR
+// arrays and three rotating vector blocks (T_k in blk[k % 3]); the slots
+// of one class (precision, value type, padded width) advance together, one
+// launch per step.  tile0/ntiles number the slot's row tiles within the
+// launch; k0 is the slot's step index at the chunk's first launch, so the
+// launch with offset s performs step k0 + s of every slot that has not
+// finished (k0 + s < steps).  Step 0 is the first step (T_1, moments 0, 1).
+struct SlotDesc {
+    const int32_t* rp;
+    const int32_t* ci;
+    const void* hv;
+    void* blk[3];
+    double coef[4];  // (1/a, -b/a, 2/a, -2b/a)
+    double* mu;      // n moments
+    int64_t k0;
+    int64_t steps;   // (n + 1) / 2
+    int64_t tile0;
+    int32_t rows;
+    int32_t ntiles;
+    int32_t n;
+    int32_t pad_;
+};
+
+template <typename VT, bool CH, int LPR, int CPL>
+__global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __restrict__ slots,
+                                                            const int32_t* __restrict__ tile_slot, int64_t ntiles,
+                                                            int32_t s, int32_t Rp, int32_t tile_rows, int32_t opts,
+                                                            int32_t xcd_split, double2* __restrict__ partial) {
+    using V2 = typename Vec2<VT>::type;
+    using H = typename HVal<VT, CH>::type;
+    constexpr int RPW = kWave / LPR;
+    constexpr int TR = kWavesPerBlock * RPW;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int sub = lane / LPR;
+    const int q = lane % LPR;
+
+    int64_t first, end, stride;
+    tile_range(ntiles, xcd_split, first, end, stride);
+    for (int64_t t = first; t < end; t += stride) {
+        const SlotDesc& d = slots[tile_slot[t]];
+        const int64_t k = d.k0 + s;
+        if (k >= d.steps) continue;
+        const int64_t lt = t - d.tile0;
+        const V2* x = static_cast<const V2*>(d.blk[k % 3]);
+        const V2* z = static_cast<const V2*>(d.blk[(k + 2) % 3]);
+        V2* y = static_cast<V2*>(d.blk[(k + 1) % 3]);
+        const int32_t* rp = d.rp;
+        const int32_t* ci = d.ci;
+        const H* hv = static_cast<const H*>(d.hv);
+        const double ca = d.coef[k == 0 ? 0 : 2], cb = d.coef[k == 0 ? 1 : 3];
+        const VT alpha = static_cast<VT>(ca), beta = static_cast<VT>(cb);
+        const int32_t o = (cb != 0.0) ? (opts | kOptShift) : (opts & ~kOptShift);
+        double p0 = 0.0, p1 = 0.0;
+        for (int g = 0; g < tile_rows; g += TR) {
+            int r = static_cast<int>(lt * tile_rows + g) + wave * RPW + sub;
+            if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
+            if (r >= d.rows) continue;
+            if (k == 0)
+                step_row<VT, CH, LPR, CPL, kFirst>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
+            else
+                step_row<VT, CH, LPR, CPL, kRecDbl>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
+        }
+        p0 = wave_sum(p0);
+        p1 = wave_sum(p1);
+        if (lane == 0) partial[4 * t + wave] = make_double2(p0, p1);
+    }
+}
+
 // Tiled variant of cheb_step_kernel for matrices with at most KMAX = 4*KCH
 // entries per row (every lattice Hamiltonian of the presets: 4 to 13).
 // Per tile of TR consecutive rows the workgroup first stages the row

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "gamma_launch.h"
 #include "step_launch.h"
 #include "tbkpm.h"
 
@@ -162,5 +163,21 @@ void matrix_fp32(tbk_matrix* m);
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial);
+
+// Kubo-Bastin contraction shape and launches (gamma.hip kernels).
+struct ContractionShape {
+    bool big;
+    int64_t kc;
+    int nchunks;
+    tbk::GammaMap map;
+};
+ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int prec);
+void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int prec,
+                         double2* P, double2* out, hipStream_t st);
+void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
+                         const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int N,
+                         double2* part, hipStream_t st);
+double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
+                            int64_t N);
 
 }  // namespace engine

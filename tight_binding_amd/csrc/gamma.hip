@@ -574,6 +574,22 @@ __global__ __launch_bounds__(256) void kubo_zeta_kernel(const double2* __restric
     }
 }
 
+// Gamma[n][m] *= g_n g_m (kernel damping; the product is formed first, as
+// the host-side weighting does).
+__global__ __launch_bounds__(256) void gamma_weight_kernel(double2* __restrict__ G, const double* __restrict__ g,
+                                                           int N) {
+    const int64_t NN = static_cast<int64_t>(N) * N;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < NN;
+         i += static_cast<int64_t>(gridDim.x) * 256) {
+        const int64_t n = i / N, m = i - n * N;
+        const double w = g[n] * g[m];
+        double2 v = G[i];
+        v.x = v.x * w;
+        v.y = v.y * w;
+        G[i] = v;
+    }
+}
+
 dim3 gamma_grid(int N, int nchunks) {
     const unsigned t = static_cast<unsigned>((N + kGammaTile - 1) / kGammaTile);
     return dim3(t, t, static_cast<unsigned>(nchunks));
@@ -608,6 +624,13 @@ void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t 
                              double2* P, hipStream_t st) {
     hipLaunchKernelGGL(gamma_bf16x3_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(256), 0, st, A, B,
                        N, K, kc, map, P);
+}
+
+void launch_gamma_weight(double2* G, const double* g, int N, hipStream_t st) {
+    const int64_t NN = static_cast<int64_t>(N) * N;
+    const int64_t blocks = (NN + 255) / 256;
+    hipLaunchKernelGGL(gamma_weight_kernel, dim3(static_cast<unsigned>(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
+                       st, G, g, N);
 }
 
 int kubo_zeta_energies_per_group(int N) {

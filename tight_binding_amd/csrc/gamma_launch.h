@@ -45,6 +45,9 @@ void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t 
 // out[i] = sum_{c < nchunks} P[c*NN + i], chunks in order.
 void launch_gamma_reduce(const double2* P, int nchunks, int64_t NN, double2* out, hipStream_t st);
 
+// Gamma[n][m] *= g[n] * g[m] in place.
+void launch_gamma_weight(double2* G, const double* g, int N, hipStream_t st);
+
 // Kubo-Bastin integrand at rescaled energies e[0..ne) (|e| < 1) from the
 // N x N complex Gamma (row-major, kernel weights already applied):
 //   out[j] = sum_{n,m} Gamma[n][m] g_mn(e_j) / (1 - e_j^2)^2 * inv_volume.

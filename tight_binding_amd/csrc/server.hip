@@ -1,0 +1,849 @@
+// Request server: DoS and sigma_xy requests of mixed models and sizes,
+// continuously batched on one GPU.
+//
+// A worker thread owns the device work.  Requests wait in a queue until
+// their device-memory footprint fits the budget (and a slot is free); then
+// they are admitted while other requests are running:
+//
+//  * DoS request -> a slot: its matrix, three rotating vector blocks and a
+//    moment buffer stay resident until it finishes.  All slots of one class
+//    (precision, real/complex values, padded vector width) advance together:
+//    a chunk of S steps is S launches of cheb_slots_kernel over the tiles of
+//    every slot of the class, each slot at its own step index, followed by a
+//    moment launch per step.  A slot joins at the next chunk after admission
+//    and leaves after the chunk in which it finishes.  Its tiles, partial
+//    sums and their reduction order depend only on the slot itself, so its
+//    moments are the same bits alone and in any mix.
+//  * sigma_xy request -> a Kubo job on a second stream: both vector stacks,
+//    the Gamma contraction on matrix cores, the kernel weights and zeta at
+//    the N Chebyshev nodes, overlapping the DoS chunks on the device.
+//
+// Two chunks are in flight at most: while the device runs chunk c, the host
+// completes chunk c-1 (results to the requests, memory back to the budget)
+// and admits new requests for chunk c+1.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "aux_kernels.h"
+#include "engine.h"
+#include "gamma_launch.h"
+
+namespace {
+using namespace engine;
+
+constexpr int32_t kSlotTileRows = 256;
+
+// Moments of step k0 + s of every slot (one workgroup per slot): its wave
+// partials summed in a fixed order, then mu_0, mu_1 (step 0) or the doubling
+// identities mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
+__global__ __launch_bounds__(tbk::kBlock) void slots_moments_kernel(const tbk::SlotDesc* __restrict__ slots,
+                                                                    int32_t s, const double2* __restrict__ partial) {
+    const tbk::SlotDesc& d = slots[blockIdx.x];
+    const int64_t k = d.k0 + s;
+    if (k >= d.steps) return;  // the whole workgroup
+    const int64_t s0 = 4 * d.tile0, s1 = 4 * (d.tile0 + d.ntiles);
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t i = s0 + threadIdx.x; i < s1; i += tbk::kBlock) {
+        a0 = a0 + partial[i].x;
+        a1 = a1 + partial[i].y;
+    }
+    __shared__ double r0[tbk::kBlock], r1[tbk::kBlock];
+    r0[threadIdx.x] = a0;
+    r1[threadIdx.x] = a1;
+    __syncthreads();
+    for (int w = tbk::kBlock / 2; w >= 1; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
+            r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* m = d.mu;
+        if (k == 0) {
+            m[0] = r0[0];
+            m[1] = r1[0];
+        } else {
+            if (2 * k < d.n) m[2 * k] = 2.0 * r0[0] - m[0];
+            if (2 * k + 1 < d.n) m[2 * k + 1] = 2.0 * r1[0] - m[1];
+        }
+    }
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Host copy of a sparse matrix handed to submit.
+struct HostSparse {
+    tbk_sparse hdr{};
+    std::vector<int32_t> outer, inner;
+    std::vector<double> values;
+    bool present = false;
+    void copy_from(const tbk_sparse* M) {
+        if (!M) return;
+        if (M->rows <= 0 || M->rows != M->cols) raise(TBK_E_INVALID, "matrix must be square and non-empty");
+        if (M->layout != TBK_CSC && M->layout != TBK_CSR) raise(TBK_E_INVALID, "unknown layout");
+        if (M->nnz < 0 || M->nnz > INT32_MAX) raise(TBK_E_INVALID, "nnz out of range for int32 indices");
+        if (!M->outer || (M->nnz > 0 && (!M->inner || !M->values))) raise(TBK_E_INVALID, "null array");
+        hdr = *M;
+        const int32_t no = (M->layout == TBK_CSC ? M->cols : M->rows) + 1;
+        outer.assign(M->outer, M->outer + no);
+        inner.assign(M->inner, M->inner + M->nnz);
+        values.assign(M->values, M->values + M->nnz * (M->is_complex ? 2 : 1));
+        hdr.outer = outer.data();
+        hdr.inner = inner.data();
+        hdr.values = values.data();
+        present = true;
+    }
+    void clear() {
+        std::vector<int32_t>().swap(outer);
+        std::vector<int32_t>().swap(inner);
+        std::vector<double>().swap(values);
+        present = false;
+    }
+};
+
+struct Request {
+    int64_t ticket = 0;
+    int32_t kind = TBK_REQ_DOS, precision = TBK_FP64, n = 0, R = 0, Rp = 0;
+    uint64_t seed = 0;
+    double a = 1.0, b = 0.0;
+    HostSparse H, vx, vy;
+    std::vector<double> g;
+    int64_t footprint = 0;
+    int32_t state = TBK_REQ_QUEUED;
+    int32_t error_code = 0;
+    std::string error;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    std::vector<double> result;
+};
+
+struct Slot {
+    Request* req = nullptr;
+    tbk_matrix* M = nullptr;
+    DevBuf blk[3], mu;
+    int64_t k = 0, steps = 0;
+    int32_t ntiles = 0;
+    int32_t pin = -1;  // row of the pinned moment staging
+    ~Slot() {
+        if (M) tbk_matrix_destroy(M);
+    }
+};
+
+struct KuboJob {
+    Request* req = nullptr;
+    tbk_matrix *H = nullptr, *vx = nullptr, *vy = nullptr;
+    DevBuf kets, L, Rs, W0, W1, part, P, G, g, e, z;
+    hipEvent_t done = nullptr;
+    int32_t pin = -1;
+    ~KuboJob() {
+        if (done) (void)hipEventDestroy(done);
+        for (tbk_matrix* m : {H, vx, vy})
+            if (m) tbk_matrix_destroy(m);
+    }
+};
+
+struct ClassKey {
+    int32_t prec, cplx, Rp;
+    bool operator<(const ClassKey& o) const {
+        return std::tie(prec, cplx, Rp) < std::tie(o.prec, o.cplx, o.Rp);
+    }
+};
+
+// Per-class device arrays of a chunk (descriptors, tile map, partials) and
+// their pinned staging, one staging set per in-flight chunk parity.
+struct ClassState {
+    DevBuf desc, tile_slot, partial;
+    tbk::SlotDesc* stage_desc[2] = {nullptr, nullptr};
+    int32_t* stage_tiles[2] = {nullptr, nullptr};
+    size_t cap_desc[2] = {0, 0}, cap_tiles[2] = {0, 0};
+};
+
+struct Chunk {
+    hipEvent_t ev = nullptr;
+    std::vector<Slot*> finishing;
+    std::vector<void*> graveyard;  // device allocations replaced while this chunk may use them
+    int parity = 0;
+};
+
+}  // namespace
+
+struct tbk_server {
+    tbk_context* ctx = nullptr;
+    hipStream_t kstream = nullptr;
+    tbk_server_config cfg{};
+    int64_t budget = 0;
+
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::deque<Request*> queue;
+    std::map<int64_t, std::unique_ptr<Request>> reqs;
+    int64_t next_ticket = 1;
+    int64_t used = 0;
+    bool stop = false;
+    std::string fatal;
+    tbk_server_stats st{};
+
+    std::thread worker;
+    // worker-only state
+    std::vector<Slot*> slots;
+    std::vector<KuboJob*> kjobs;
+    std::map<ClassKey, ClassState> classes;
+    std::deque<Chunk> inflight;
+    std::vector<hipEvent_t> free_events;
+    int64_t chunk_seq = 0;
+    double* pin_mu = nullptr;  // max_slots x max_moments doubles
+    std::vector<int32_t> free_pin;
+    double* pin_kubo = nullptr;  // max_kubo x (2 max_kubo_N + 2 max_kubo_N^2) doubles
+    std::vector<int32_t> free_kpin;
+    std::vector<void*> pinned_extra;  // grown staging buffers, freed at destroy
+
+    int64_t dos_footprint(const Request& r) const;
+    int64_t kubo_footprint(const Request& r) const;
+    void run();
+    std::vector<Request*> pick_admissions();
+    void start_dos(Request* r);
+    void start_kubo(Request* r);
+    void enqueue_chunk();
+    void complete_chunk(Chunk& c);
+    void poll_kubo(bool block);
+    void finish(Request* r, int32_t code, const std::string& msg);
+    hipEvent_t get_event();
+    void grow(DevBuf& b, size_t bytes);
+    template <typename T>
+    T* stage(T*& p, size_t& cap, size_t n);
+};
+
+namespace {
+
+int64_t mat_bytes(const HostSparse& m, bool fp32_copy) {
+    const int64_t vw = m.hdr.is_complex ? 2 : 1;
+    int64_t b = (static_cast<int64_t>(m.hdr.rows) + 1) * 4 + m.hdr.nnz * 4 + m.hdr.nnz * 8 * vw;
+    if (fp32_copy) b += m.hdr.nnz * 4 * vw;
+    return b;
+}
+
+}  // namespace
+
+int64_t tbk_server::dos_footprint(const Request& r) const {
+    const int64_t rows = r.H.hdr.rows;
+    const int64_t elem = r.precision == TBK_FP64 ? 16 : 8;
+    return mat_bytes(r.H, r.precision == TBK_FP32) + 3 * rows * r.Rp * elem + static_cast<int64_t>(r.n) * 8;
+}
+
+int64_t tbk_server::kubo_footprint(const Request& r) const {
+    const int64_t rows = r.H.hdr.rows;
+    const int64_t blk = rows * r.Rp * 16;
+    const int64_t N = r.n;
+    const ContractionShape cs = contraction_plan(ctx, r.n, rows * r.Rp, r.precision);
+    return mat_bytes(r.H, false) + mat_bytes(r.vx, false) + mat_bytes(r.vy, false) + blk /* kets */ +
+           2 * N * blk + 2 * blk + static_cast<int64_t>(ctx->num_cu) * 32 * 16 /* step partials */ +
+           static_cast<int64_t>(cs.nchunks) * N * N * 16 + N * N * 16 + N * 8 * 2 + N * 16;
+}
+
+hipEvent_t tbk_server::get_event() {
+    if (!free_events.empty()) {
+        hipEvent_t e = free_events.back();
+        free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+}
+
+// grow a per-class device array; the old allocation may still be read by an
+// in-flight chunk, so it is released when the newest in-flight chunk completes
+void tbk_server::grow(DevBuf& b, size_t bytes) {
+    if (bytes <= b.bytes) return;
+    if (b.p) {
+        if (!inflight.empty())
+            inflight.back().graveyard.push_back(b.p);
+        else
+            (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    b.reserve(std::max(bytes, static_cast<size_t>(4096)));
+}
+
+template <typename T>
+T* tbk_server::stage(T*& p, size_t& cap, size_t n) {
+    if (n > cap) {
+        // staging of the chunk that used this parity has completed; the old
+        // buffer is kept until destroy (pinned frees may synchronise)
+        if (p) pinned_extra.push_back(p);
+        const size_t c = std::max<size_t>(n, 2 * cap);
+        void* q = nullptr;
+        HIPC(hipHostMalloc(&q, c * sizeof(T), hipHostMallocDefault));
+        p = static_cast<T*>(q);
+        cap = c;
+    }
+    return p;
+}
+
+void tbk_server::finish(Request* r, int32_t code, const std::string& msg) {
+    std::lock_guard<std::mutex> lk(mu);
+    r->t_done = now_s();
+    r->state = code == TBK_OK ? TBK_REQ_DONE : TBK_REQ_FAILED;
+    r->error_code = code;
+    r->error = msg;
+    used -= r->footprint;
+    if (code == TBK_OK)
+        st.completed++;
+    else
+        st.failed++;
+    if (r->kind == TBK_REQ_DOS) st.running_dos--;
+    else st.running_kubo--;
+    r->H.clear();
+    r->vx.clear();
+    r->vy.clear();
+    done_cv.notify_all();
+}
+
+// First-fit over the queue in arrival order; a request that has waited more
+// than max_wait_bypass_ms at the front stops later ones from passing it.
+std::vector<Request*> tbk_server::pick_admissions() {
+    std::vector<Request*> out;
+    const double t = now_s();
+    // slots and jobs in use, counted by their pinned result rows (a slot that
+    // finishes in an in-flight chunk keeps its row until the chunk completes)
+    int32_t dos_active = cfg.max_slots - static_cast<int32_t>(free_pin.size());
+    int32_t kubo_active = cfg.max_kubo - static_cast<int32_t>(free_kpin.size());
+    for (auto it = queue.begin(); it != queue.end();) {
+        Request* r = *it;
+        const bool is_dos = r->kind == TBK_REQ_DOS;
+        const bool room = is_dos ? dos_active < cfg.max_slots : kubo_active < cfg.max_kubo;
+        if (room && used + r->footprint <= budget) {
+            used += r->footprint;
+            r->state = TBK_REQ_RUNNING;
+            r->t_admit = t;
+            if (is_dos) {
+                ++dos_active;
+                st.running_dos++;
+            } else {
+                ++kubo_active;
+                st.running_kubo++;
+            }
+            out.push_back(r);
+            it = queue.erase(it);
+            continue;
+        }
+        if (1e3 * (t - r->t_submit) > cfg.max_wait_bypass_ms) break;
+        ++it;
+    }
+    st.queued = static_cast<int64_t>(queue.size());
+    st.used_bytes = used;
+    return out;
+}
+
+void tbk_server::start_dos(Request* r) {
+    auto s = std::make_unique<Slot>();
+    s->req = r;
+    int rc = tbk_matrix_create(ctx, &r->H.hdr, &s->M);
+    if (rc != TBK_OK) raise(rc, g_last_error);
+    if (r->precision == TBK_FP32) matrix_fp32(s->M);
+    const int64_t rows = r->H.hdr.rows;
+    const size_t elem = r->precision == TBK_FP64 ? 16 : 8;
+    for (auto& b : s->blk) b.reserve(static_cast<size_t>(rows) * r->Rp * elem);
+    s->mu.reserve(static_cast<size_t>(r->n) * 8);
+    s->steps = (r->n + 1) / 2;
+    s->ntiles = static_cast<int32_t>((rows + kSlotTileRows - 1) / kSlotTileRows);
+    const size_t total = static_cast<size_t>(rows) * r->Rp;
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>((total + tbk::kBlock - 1) / tbk::kBlock, 65536));
+    const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
+    if (r->precision == TBK_FP64)
+        hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
+                           s->blk[0].as<double2>(), static_cast<int32_t>(rows), r->R, r->Rp, r->seed, inv);
+    else
+        hipLaunchKernelGGL(tbk::random_phase_kernel<float2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
+                           s->blk[0].as<float2>(), static_cast<int32_t>(rows), r->R, r->Rp, r->seed, inv);
+    HIPC(hipGetLastError());
+    s->pin = free_pin.back();
+    free_pin.pop_back();
+    slots.push_back(s.release());
+}
+
+void tbk_server::start_kubo(Request* r) {
+    auto j = std::make_unique<KuboJob>();
+    j->req = r;
+    int rc = tbk_matrix_create(ctx, &r->H.hdr, &j->H);
+    if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vx.hdr, &j->vx);
+    if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vy.hdr, &j->vy);
+    if (rc != TBK_OK) raise(rc, g_last_error);
+    const int rows = r->H.hdr.rows, Rp = r->Rp, N = r->n;
+    const int64_t K = static_cast<int64_t>(rows) * Rp;
+    const size_t blk = static_cast<size_t>(K) * sizeof(double2);
+    j->kets.reserve(blk);
+    j->L.reserve(static_cast<size_t>(N) * blk);
+    j->Rs.reserve(static_cast<size_t>(N) * blk);
+    j->W0.reserve(blk);
+    j->W1.reserve(blk);
+    const StepPlan pH = make_plan(ctx, j->H, TBK_FP64, Rp, rows, r->b, false);
+    const StepPlan pX = make_plan(ctx, j->vx, TBK_FP64, Rp, rows, 0.0, false);
+    const StepPlan pY = make_plan(ctx, j->vy, TBK_FP64, Rp, rows, 0.0, false);
+    j->part.reserve(static_cast<size_t>(std::max(pH.nb, std::max(pX.nb, pY.nb))) * sizeof(double2));
+    const ContractionShape cs = contraction_plan(ctx, N, K, r->precision);
+    const int64_t NN = static_cast<int64_t>(N) * N;
+    j->P.reserve(static_cast<size_t>(cs.nchunks) * NN * sizeof(double2));
+    j->G.reserve(static_cast<size_t>(NN) * sizeof(double2));
+    j->g.reserve(static_cast<size_t>(N) * sizeof(double));
+    j->e.reserve(static_cast<size_t>(N) * sizeof(double));
+    j->z.reserve(static_cast<size_t>(N) * sizeof(double2));
+    j->pin = free_kpin.back();
+    free_kpin.pop_back();
+    struct PinGuard {
+        tbk_server* s;
+        KuboJob* j;
+        bool armed = true;
+        ~PinGuard() {
+            if (armed) s->free_kpin.push_back(j->pin);
+        }
+    } guard{this, j.get()};
+    const int64_t Ncap = cfg.max_kubo_moments;
+    double* pz = pin_kubo + static_cast<size_t>(j->pin) * static_cast<size_t>(2 * Ncap + 2 * Ncap * Ncap);
+    double* pg = pz + 2 * N;  // Gamma region; the weights and nodes are staged there first
+    for (int i = 0; i < N; ++i) {
+        pg[i] = r->g.empty() ? 1.0 : r->g[i];
+        pg[N + i] = std::cos((i + 0.5) / N * M_PI);
+    }
+    hipStream_t st_ = kstream;
+    HIPC(hipMemcpyAsync(j->g.p, pg, static_cast<size_t>(N) * 8, hipMemcpyHostToDevice, st_));
+    HIPC(hipMemcpyAsync(j->e.p, pg + N, static_cast<size_t>(N) * 8, hipMemcpyHostToDevice, st_));
+    const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((K + tbk::kBlock - 1) / tbk::kBlock, 65536));
+    hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st_,
+                       j->kets.as<double2>(), rows, r->R, Rp, r->seed, inv);
+    HIPC(hipGetLastError());
+    enqueue_kubo_stacks(pH, pX, pY, r->a, r->b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, N,
+                        j->part.as<double2>(), st_);
+    enqueue_contraction(cs, j->L.as<const double2>(), j->Rs.as<const double2>(), N, K, r->precision,
+                        j->P.as<double2>(), j->G.as<double2>(), st_);
+    tbk::launch_gamma_weight(j->G.as<double2>(), j->g.as<const double>(), N, st_);
+    HIPC(hipGetLastError());
+    tbk::launch_kubo_zeta(j->G.as<const double2>(), N, j->e.as<const double>(), N, 1.0, j->z.as<double2>(), st_);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(pz, j->z.p, static_cast<size_t>(N) * 16, hipMemcpyDeviceToHost, st_));
+    HIPC(hipMemcpyAsync(pg, j->G.p, static_cast<size_t>(NN) * 16, hipMemcpyDeviceToHost, st_));
+    HIPC(hipEventCreateWithFlags(&j->done, hipEventDisableTiming));
+    HIPC(hipEventRecord(j->done, st_));
+    guard.armed = false;
+    kjobs.push_back(j.release());
+}
+
+void tbk_server::poll_kubo(bool block) {
+    for (size_t i = 0; i < kjobs.size();) {
+        KuboJob* j = kjobs[i];
+        hipError_t q = block ? hipEventSynchronize(j->done) : hipEventQuery(j->done);
+        if (q == hipErrorNotReady) {
+            ++i;
+            continue;
+        }
+        HIPC(q);
+        Request* r = j->req;
+        const int64_t N = r->n, Ncap = cfg.max_kubo_moments;
+        const double* pz = pin_kubo + static_cast<size_t>(j->pin) * static_cast<size_t>(2 * Ncap + 2 * Ncap * Ncap);
+        r->result.assign(pz, pz + 2 * N + 2 * N * N);
+        free_kpin.push_back(j->pin);
+        kjobs.erase(kjobs.begin() + static_cast<std::ptrdiff_t>(i));
+        delete j;
+        finish(r, TBK_OK, "");
+        std::lock_guard<std::mutex> lk(mu);
+        st.kubo_done++;
+    }
+}
+
+void tbk_server::enqueue_chunk() {
+    const int32_t S = cfg.chunk_steps;
+    Chunk c;
+    c.parity = static_cast<int>(chunk_seq & 1);
+    hipStream_t st_ = ctx->stream;
+    // group the slots by class
+    std::map<ClassKey, std::vector<Slot*>> groups;
+    for (Slot* s : slots) {
+        const Request* r = s->req;
+        groups[ClassKey{r->precision, r->H.hdr.is_complex ? 1 : 0, r->Rp}].push_back(s);
+    }
+    for (auto& kv : groups) {
+        const ClassKey& key = kv.first;
+        std::vector<Slot*>& v = kv.second;
+        ClassState& cs = classes[key];
+        int64_t ntiles = 0;
+        for (Slot* s : v) ntiles += s->ntiles;
+        tbk::SlotDesc* hd = stage(cs.stage_desc[c.parity], cs.cap_desc[c.parity], v.size());
+        int32_t* ht = stage(cs.stage_tiles[c.parity], cs.cap_tiles[c.parity], static_cast<size_t>(ntiles));
+        int64_t t0 = 0;
+        for (size_t i = 0; i < v.size(); ++i) {
+            Slot* s = v[i];
+            const Request* r = s->req;
+            tbk::SlotDesc& d = hd[i];
+            std::memset(&d, 0, sizeof(d));
+            d.rp = s->M->rp.as<const int32_t>();
+            d.ci = s->M->ci.as<const int32_t>();
+            d.hv = r->precision == TBK_FP64 ? s->M->val64.p : s->M->val32.p;
+            for (int q = 0; q < 3; ++q) d.blk[q] = s->blk[q].p;
+            d.coef[0] = 1.0 / r->a;
+            d.coef[1] = -r->b / r->a;
+            d.coef[2] = 2.0 / r->a;
+            d.coef[3] = -2.0 * r->b / r->a;
+            d.mu = s->mu.as<double>();
+            d.k0 = s->k;
+            d.steps = s->steps;
+            d.tile0 = t0;
+            d.rows = r->H.hdr.rows;
+            d.ntiles = s->ntiles;
+            d.n = r->n;
+            for (int32_t t = 0; t < s->ntiles; ++t) ht[t0 + t] = static_cast<int32_t>(i);
+            t0 += s->ntiles;
+        }
+        grow(cs.desc, v.size() * sizeof(tbk::SlotDesc));
+        grow(cs.tile_slot, static_cast<size_t>(ntiles) * sizeof(int32_t));
+        grow(cs.partial, static_cast<size_t>(4 * ntiles) * sizeof(double2));
+        HIPC(hipMemcpyAsync(cs.desc.p, hd, v.size() * sizeof(tbk::SlotDesc), hipMemcpyHostToDevice, st_));
+        HIPC(hipMemcpyAsync(cs.tile_slot.p, ht, static_cast<size_t>(ntiles) * sizeof(int32_t), hipMemcpyHostToDevice,
+                            st_));
+        tbk::SlotArgs A{};
+        A.slots = cs.desc.as<const tbk::SlotDesc>();
+        A.tile_slot = cs.tile_slot.as<const int32_t>();
+        A.ntiles = ntiles;
+        A.Rp = key.Rp;
+        A.tile_rows = kSlotTileRows;
+        A.opts = ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0;
+        int64_t nb = std::min<int64_t>(ntiles, static_cast<int64_t>(ctx->num_cu) * 4);
+        A.xcd_split = 0;
+        if (nb >= 64) {
+            nb = (nb / 8) * 8;
+            A.xcd_split = 1;
+        }
+        A.partial = cs.partial.as<double2>();
+        int cpl = cfg.cpl > 0 ? cfg.cpl : (key.Rp >= 2 ? 2 : 1);
+        if (key.Rp / cpl > tbk::kWave || (cpl == 4 && key.Rp < 8)) cpl = key.Rp > tbk::kWave ? key.Rp / tbk::kWave : 1;
+        for (int32_t s = 0; s < S; ++s) {
+            A.s = s;
+            bool ok;
+            const dim3 grid(static_cast<unsigned>(std::max<int64_t>(nb, 1)));
+            if (key.prec == TBK_FP64)
+                ok = key.cplx ? tbk::launch_slots_f64_cplx(key.Rp, cpl, grid, st_, A)
+                              : tbk::launch_slots_f64_real(key.Rp, cpl, grid, st_, A);
+            else
+                ok = key.cplx ? tbk::launch_slots_f32_cplx(key.Rp, cpl, grid, st_, A)
+                              : tbk::launch_slots_f32_real(key.Rp, cpl, grid, st_, A);
+            if (!ok) raise(TBK_E_UNSUPPORTED, "no slot kernel for this width");
+            HIPC(hipGetLastError());
+            hipLaunchKernelGGL(slots_moments_kernel, dim3(static_cast<unsigned>(v.size())), dim3(tbk::kBlock), 0, st_,
+                               cs.desc.as<const tbk::SlotDesc>(), s, cs.partial.as<const double2>());
+            HIPC(hipGetLastError());
+        }
+    }
+    // slots finishing in this chunk: moments to pinned staging, then leave
+    for (size_t i = 0; i < slots.size();) {
+        Slot* s = slots[i];
+        s->k += S;
+        if (s->k >= s->steps) {
+            double* dst = pin_mu + static_cast<size_t>(s->pin) * static_cast<size_t>(cfg.max_moments);
+            HIPC(hipMemcpyAsync(dst, s->mu.p, static_cast<size_t>(s->req->n) * 8, hipMemcpyDeviceToHost, st_));
+            c.finishing.push_back(s);
+            slots.erase(slots.begin() + static_cast<std::ptrdiff_t>(i));
+        } else {
+            ++i;
+        }
+    }
+    c.ev = get_event();
+    HIPC(hipEventRecord(c.ev, st_));
+    inflight.push_back(std::move(c));
+    ++chunk_seq;
+    std::lock_guard<std::mutex> lk(mu);
+    st.chunks++;
+}
+
+void tbk_server::complete_chunk(Chunk& c) {
+    HIPC(hipEventSynchronize(c.ev));
+    free_events.push_back(c.ev);
+    for (void* p : c.graveyard) (void)hipFree(p);
+    for (Slot* s : c.finishing) {
+        Request* r = s->req;
+        const double* src = pin_mu + static_cast<size_t>(s->pin) * static_cast<size_t>(cfg.max_moments);
+        r->result.assign(src, src + r->n);
+        free_pin.push_back(s->pin);
+        delete s;
+        finish(r, TBK_OK, "");
+    }
+}
+
+void tbk_server::run() {
+    try {
+        set_device(ctx);
+        for (;;) {
+            std::vector<Request*> adm;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                const bool idle = slots.empty() && kjobs.empty() && inflight.empty();
+                if (idle && queue.empty()) {
+                    if (stop) break;
+                    cv.wait(lk, [&] { return stop || !queue.empty(); });
+                    continue;
+                }
+                adm = pick_admissions();
+            }
+            for (Request* r : adm) {
+                try {
+                    if (r->kind == TBK_REQ_DOS)
+                        start_dos(r);
+                    else
+                        start_kubo(r);
+                } catch (const Error& e) {
+                    finish(r, e.code, e.msg);
+                }
+            }
+            if (!slots.empty() && inflight.size() < 2) enqueue_chunk();
+            if (inflight.size() >= 2 || (slots.empty() && !inflight.empty())) {
+                complete_chunk(inflight.front());
+                inflight.pop_front();
+            }
+            poll_kubo(false);
+            if (slots.empty() && inflight.empty() && !kjobs.empty()) {
+                // only sigma_xy jobs on the device: wait for news, briefly
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || !queue.empty(); });
+            }
+        }
+    } catch (const Error& e) {
+        std::lock_guard<std::mutex> lk(mu);
+        fatal = e.msg;
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu);
+        fatal = e.what();
+    }
+    // on a fatal error every request still pending fails
+    std::vector<Request*> pending;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (fatal.empty()) return;
+        for (auto& kv : reqs)
+            if (kv.second->state == TBK_REQ_QUEUED || kv.second->state == TBK_REQ_RUNNING) pending.push_back(kv.second.get());
+        queue.clear();
+    }
+    for (Request* r : pending) {
+        std::lock_guard<std::mutex> lk(mu);
+        r->state = TBK_REQ_FAILED;
+        r->error_code = TBK_E_DEVICE;
+        r->error = fatal;
+        r->t_done = now_s();
+        st.failed++;
+    }
+    done_cv.notify_all();
+}
+
+namespace {
+
+void fill_info(const Request& r, tbk_request_info* info) {
+    if (!info) return;
+    info->state = r.state;
+    info->error_code = r.error_code;
+    const double t = now_s();
+    info->queue_ms = 1e3 * ((r.t_admit > 0 ? r.t_admit : t) - r.t_submit);
+    info->run_ms = r.t_admit > 0 ? 1e3 * ((r.t_done > 0 ? r.t_done : t) - r.t_admit) : 0.0;
+    info->latency_ms = 1e3 * ((r.t_done > 0 ? r.t_done : t) - r.t_submit);
+    info->footprint_bytes = r.footprint;
+    info->result_doubles = static_cast<int64_t>(r.result.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server** out) {
+    return guarded([&] {
+        if (!out) raise(TBK_E_INVALID, "null argument");
+        auto s = std::make_unique<tbk_server>();
+        if (cfg) s->cfg = *cfg;
+        tbk_server_config& c = s->cfg;
+        if (c.max_slots <= 0) c.max_slots = 512;
+        if (c.max_kubo <= 0) c.max_kubo = 1;
+        if (c.chunk_steps <= 0) c.chunk_steps = 32;
+        if (c.max_moments <= 0) c.max_moments = 4096;
+        if (c.max_kubo_moments <= 0) c.max_kubo_moments = 1024;
+        if (c.max_wait_bypass_ms <= 0) c.max_wait_bypass_ms = 1000.0;
+        if (c.cpl != 0 && c.cpl != 1 && c.cpl != 2 && c.cpl != 4) raise(TBK_E_INVALID, "cpl must be 0, 1, 2 or 4");
+        int rc = tbk_context_create(device, &s->ctx);
+        if (rc != TBK_OK) raise(rc, g_last_error);
+        set_device(s->ctx);
+        HIPC(hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking));
+        size_t free_b = 0, total_b = 0;
+        HIPC(hipMemGetInfo(&free_b, &total_b));
+        s->budget = c.hbm_budget_bytes > 0 ? c.hbm_budget_bytes : static_cast<int64_t>(0.85 * static_cast<double>(free_b));
+        void* p = nullptr;
+        HIPC(hipHostMalloc(&p, static_cast<size_t>(c.max_slots) * c.max_moments * sizeof(double), hipHostMallocDefault));
+        s->pin_mu = static_cast<double*>(p);
+        for (int32_t i = c.max_slots - 1; i >= 0; --i) s->free_pin.push_back(i);
+        const size_t kb = static_cast<size_t>(2 * c.max_kubo_moments + 2 * static_cast<int64_t>(c.max_kubo_moments) *
+                                                                             c.max_kubo_moments);
+        HIPC(hipHostMalloc(&p, static_cast<size_t>(c.max_kubo) * kb * sizeof(double), hipHostMallocDefault));
+        s->pin_kubo = static_cast<double*>(p);
+        for (int32_t i = c.max_kubo - 1; i >= 0; --i) s->free_kpin.push_back(i);
+        s->st.budget_bytes = s->budget;
+        tbk_server* raw = s.get();
+        s->worker = std::thread([raw] { raw->run(); });
+        *out = s.release();
+    });
+}
+
+int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
+    return guarded([&] {
+        if (!s || !q || !ticket) raise(TBK_E_INVALID, "null argument");
+        if (q->kind != TBK_REQ_DOS && q->kind != TBK_REQ_KUBO) raise(TBK_E_INVALID, "unknown request kind");
+        if (!q->H) raise(TBK_E_INVALID, "request without a Hamiltonian");
+        if (q->R < 1) raise(TBK_E_INVALID, "R must be positive");
+        if (!(q->a != 0.0) || !std::isfinite(q->a) || !std::isfinite(q->b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
+        auto r = std::make_unique<Request>();
+        r->kind = q->kind;
+        r->precision = q->precision;
+        r->n = q->n_moments;
+        r->R = q->R;
+        r->Rp = pow2_at_least(q->R);
+        r->seed = q->seed;
+        r->a = q->a;
+        r->b = q->b;
+        if (r->Rp > 256) raise(TBK_E_UNSUPPORTED, "R above 256 is not supported");
+        if (q->kind == TBK_REQ_DOS) {
+            if (q->precision != TBK_FP64 && q->precision != TBK_FP32) raise(TBK_E_INVALID, "unknown precision");
+            if (q->n_moments < 2) raise(TBK_E_INVALID, "n_moments must be at least 2");
+            if (q->n_moments > s->cfg.max_moments) raise(TBK_E_UNSUPPORTED, "more moments than the server allows");
+        } else {
+            if (q->precision != TBK_GAMMA_FP64 && q->precision != TBK_GAMMA_BF16X3)
+                raise(TBK_E_INVALID, "unknown contraction precision");
+            if (!q->vx || !q->vy) raise(TBK_E_INVALID, "sigma_xy request without velocity operators");
+            if (q->n_moments < 1) raise(TBK_E_INVALID, "n_moments must be at least 1");
+            if (q->n_moments > s->cfg.max_kubo_moments) raise(TBK_E_UNSUPPORTED, "more moments than the server allows");
+            if (tbk::kubo_zeta_energies_per_group(q->n_moments) == 0) raise(TBK_E_UNSUPPORTED, "N too large for zeta");
+        }
+        r->H.copy_from(q->H);
+        if (q->kind == TBK_REQ_KUBO) {
+            r->vx.copy_from(q->vx);
+            r->vy.copy_from(q->vy);
+            if (r->vx.hdr.rows != r->H.hdr.rows || r->vy.hdr.rows != r->H.hdr.rows)
+                raise(TBK_E_INVALID, "velocity operators and H differ in size");
+            if (q->g) r->g.assign(q->g, q->g + q->n_moments);
+            r->footprint = s->kubo_footprint(*r);
+        } else {
+            r->footprint = s->dos_footprint(*r);
+        }
+        if (r->footprint > s->budget) raise(TBK_E_OOM, "request needs more device memory than the server budget");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) raise(TBK_E_INVALID, "server is shutting down");
+        if (!s->fatal.empty()) raise(TBK_E_DEVICE, s->fatal);
+        r->ticket = s->next_ticket++;
+        r->t_submit = now_s();
+        *ticket = r->ticket;
+        s->queue.push_back(r.get());
+        s->reqs[r->ticket] = std::move(r);
+        s->st.submitted++;
+        s->st.queued = static_cast<int64_t>(s->queue.size());
+        s->cv.notify_all();
+    });
+}
+
+int tbk_server_info(tbk_server* s, int64_t ticket, tbk_request_info* info) {
+    return guarded([&] {
+        if (!s || !info) raise(TBK_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) raise(TBK_E_INVALID, "unknown ticket");
+        fill_info(*it->second, info);
+    });
+}
+
+int tbk_server_wait(tbk_server* s, int64_t ticket, double timeout_s, tbk_request_info* info) {
+    return guarded([&] {
+        if (!s) raise(TBK_E_INVALID, "null argument");
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) raise(TBK_E_INVALID, "unknown ticket");
+        Request* r = it->second.get();
+        auto pred = [&] { return r->state == TBK_REQ_DONE || r->state == TBK_REQ_FAILED; };
+        if (timeout_s < 0)
+            s->done_cv.wait(lk, pred);
+        else
+            s->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+        fill_info(*r, info);
+    });
+}
+
+int tbk_server_take(tbk_server* s, int64_t ticket, double* out, int64_t capacity, tbk_request_info* info) {
+    return guarded([&] {
+        if (!s) raise(TBK_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) raise(TBK_E_INVALID, "unknown ticket");
+        Request* r = it->second.get();
+        if (r->state == TBK_REQ_QUEUED || r->state == TBK_REQ_RUNNING) raise(TBK_E_INVALID, "request not finished");
+        fill_info(*r, info);
+        if (r->state == TBK_REQ_FAILED) {
+            const int code = r->error_code;
+            const std::string msg = r->error;
+            s->reqs.erase(it);
+            raise(code ? code : TBK_E_DEVICE, msg);
+        }
+        if (static_cast<int64_t>(r->result.size()) > capacity) raise(TBK_E_INVALID, "output buffer too small");
+        if (out && !r->result.empty()) std::memcpy(out, r->result.data(), r->result.size() * sizeof(double));
+        s->reqs.erase(it);
+    });
+}
+
+int tbk_server_get_stats(tbk_server* s, tbk_server_stats* out) {
+    return guarded([&] {
+        if (!s || !out) raise(TBK_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        *out = s->st;
+        out->used_bytes = s->used;
+        out->queued = static_cast<int64_t>(s->queue.size());
+    });
+}
+
+int tbk_server_destroy(tbk_server* s) {
+    return guarded([&] {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+            s->cv.notify_all();
+        }
+        if (s->worker.joinable()) s->worker.join();
+        set_device(s->ctx);
+        (void)hipDeviceSynchronize();
+        for (Slot* x : s->slots) delete x;
+        for (KuboJob* x : s->kjobs) delete x;
+        for (Chunk& c : s->inflight) {
+            for (Slot* x : c.finishing) delete x;
+            for (void* p : c.graveyard) (void)hipFree(p);
+        }
+        for (auto& kv : s->classes)
+            for (int q = 0; q < 2; ++q) {
+                if (kv.second.stage_desc[q]) (void)hipHostFree(kv.second.stage_desc[q]);
+                if (kv.second.stage_tiles[q]) (void)hipHostFree(kv.second.stage_tiles[q]);
+            }
+        s->classes.clear();
+        for (void* p : s->pinned_extra) (void)hipHostFree(p);
+        for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+        if (s->pin_mu) (void)hipHostFree(s->pin_mu);
+        if (s->pin_kubo) (void)hipHostFree(s->pin_kubo);
+        if (s->kstream) (void)hipStreamDestroy(s->kstream);
+        tbk_context* ctx = s->ctx;
+        delete s;
+        tbk_context_destroy(ctx);
+    });
+}
+
+}  // extern "C"

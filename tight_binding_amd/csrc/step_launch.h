@@ -98,6 +98,55 @@ bool launch_batch(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const Ba
     return false;
 }
 
+// Arguments of cheb_slots_kernel (one step of every slot of a class).
+struct SlotArgs {
+    const SlotDesc* slots;
+    const int32_t* tile_slot;
+    int64_t ntiles;
+    int32_t s;
+    int32_t Rp;
+    int32_t tile_rows;
+    int32_t opts;  // kOptStream
+    int32_t xcd_split;
+    double2* partial;
+};
+
+bool launch_slots_f64_real(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a);
+bool launch_slots_f64_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a);
+bool launch_slots_f32_real(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a);
+bool launch_slots_f32_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a);
+
+template <typename VT, bool CH>
+bool launch_slots(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a) {
+#define TBK_SLOT_CASE(RP, C)                                                                                 \
+    if (Rp == RP && cpl == C) {                                                                              \
+        hipLaunchKernelGGL((cheb_slots_kernel<VT, CH, RP / C, C>), grid, dim3(kBlock), 0, st, a.slots,       \
+                           a.tile_slot, a.ntiles, a.s, a.Rp, a.tile_rows, a.opts, a.xcd_split, a.partial);   \
+        return true;                                                                                         \
+    }
+    TBK_SLOT_CASE(1, 1)
+    TBK_SLOT_CASE(2, 1)
+    TBK_SLOT_CASE(2, 2)
+    TBK_SLOT_CASE(4, 1)
+    TBK_SLOT_CASE(4, 2)
+    TBK_SLOT_CASE(8, 1)
+    TBK_SLOT_CASE(8, 2)
+    TBK_SLOT_CASE(8, 4)
+    TBK_SLOT_CASE(16, 1)
+    TBK_SLOT_CASE(16, 2)
+    TBK_SLOT_CASE(16, 4)
+    TBK_SLOT_CASE(32, 1)
+    TBK_SLOT_CASE(32, 2)
+    TBK_SLOT_CASE(32, 4)
+    TBK_SLOT_CASE(64, 1)
+    TBK_SLOT_CASE(64, 2)
+    TBK_SLOT_CASE(128, 2)
+    TBK_SLOT_CASE(128, 4)
+    TBK_SLOT_CASE(256, 4)
+#undef TBK_SLOT_CASE
+    return false;
+}
+
 // Launch one step; returns false if no kernel exists for the combination.
 bool launch_f64_real(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
 bool launch_f64_cplx(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);

@@ -375,44 +375,103 @@ void contraction_shape_big(const tbk_context* ctx, int N, int64_t K, int64_t max
     map.per_xcd = (tiles * map.chunks + 7) / 8;
 }
 
+}  // namespace
+
+namespace engine {
+
+ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int prec) {
+    if (prec != TBK_GAMMA_FP64 && prec != TBK_GAMMA_BF16X3) raise(TBK_E_INVALID, "unknown contraction precision");
+    ContractionShape cs{};
+    cs.big = N >= 2 * tbk::kBigTile;
+    if (cs.big) {
+        contraction_shape_big(ctx, N, K, prec == TBK_GAMMA_FP64 ? 0 : kSplitChunkCols, cs.kc, cs.map);
+        cs.nchunks = cs.map.chunks;
+    } else {
+        contraction_shape(ctx, N, K, cs.kc, cs.nchunks);
+    }
+    return cs;
+}
+
+// split-K partial tiles into P (nchunks x N x N), then their ordered sum into out
+void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int prec,
+                         double2* P, double2* out, hipStream_t st) {
+    if (prec == TBK_GAMMA_FP64 && cs.big)
+        tbk::launch_gamma_f64_big(A, B, N, K, cs.kc, cs.map, P, st);
+    else if (prec == TBK_GAMMA_FP64)
+        tbk::launch_gamma_f64(A, B, N, K, cs.kc, cs.nchunks, P, st);
+    else if (cs.big)
+        tbk::launch_gamma_bf16x3_big(A, B, N, K, cs.kc, cs.map, P, st);
+    else
+        tbk::launch_gamma_bf16x3(A, B, N, K, cs.kc, cs.nchunks, P, st);
+    HIPC(hipGetLastError());
+    tbk::launch_gamma_reduce(P, cs.nchunks, static_cast<int64_t>(N) * N, out, st);
+    HIPC(hipGetLastError());
+}
+
+// Both Kubo stacks: L[n] = T_n(H~)|r>, R[m] = v_x T_m(H~) v_y|r> (n, m < N),
+// blocks of blk_bytes each; T_m of the right side rotates through W0, W1.
+void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
+                         const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int N,
+                         double2* part, hipStream_t st) {
+    auto Lk = [&](int64_t k) { return static_cast<char*>(L) + static_cast<size_t>(k) * blk_bytes; };
+    auto Rk = [&](int64_t k) { return static_cast<char*>(Rs) + static_cast<size_t>(k) * blk_bytes; };
+    auto W = [&](int64_t k) { return (k & 1) ? W1 : W0; };
+    HIPC(hipMemcpyAsync(Lk(0), kets, blk_bytes, hipMemcpyDeviceToDevice, st));
+    if (N > 1) launch_plan(pH, st, tbk::kFirst, 1.0 / a, -b / a, Lk(0), nullptr, Lk(1), Lk(0), part);
+    for (int64_t n = 1; n + 1 < N; ++n)
+        launch_plan(pH, st, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, Lk(n), Lk(n - 1), Lk(n + 1), nullptr, part);
+    launch_plan(pY, st, tbk::kFirst, 1.0, 0.0, kets, nullptr, W(0), kets, part);
+    launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(0), nullptr, Rk(0), W(0), part);
+    if (N > 1) {
+        launch_plan(pH, st, tbk::kFirst, 1.0 / a, -b / a, W(0), nullptr, W(1), W(0), part);
+        launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(1), nullptr, Rk(1), W(1), part);
+    }
+    for (int64_t m = 2; m < N; ++m) {
+        launch_plan(pH, st, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part);
+        launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(m), nullptr, Rk(m), W(m), part);
+    }
+}
+
+double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
+                            int64_t N) {
+    // matrix arrays once per launch; vector blocks: x (+ z) read and y
+    // written per step, x read and y written per velocity application
+    auto mat = [&](const tbk_matrix* m) {
+        return static_cast<double>(m->nnz) * ((m->is_complex ? 16.0 : 8.0) + 4.0) +
+               (static_cast<double>(rows) + 1.0) * 4.0;
+    };
+    const double blk = static_cast<double>(rows) * R * 16.0;
+    const double Nm2 = static_cast<double>(std::max<int64_t>(0, N - 2));
+    double bytes = 2.0 * blk;                              // |r> into the left stack
+    bytes += mat(vy) + 2.0 * blk + mat(vx) + 2.0 * blk;    // v_x v_y |r>
+    if (N > 1) bytes += 2.0 * (mat(H) + 2.0 * blk) + mat(vx) + 2.0 * blk;
+    bytes += Nm2 * (2.0 * (mat(H) + 3.0 * blk) + mat(vx) + 2.0 * blk);
+    return bytes;
+}
+
+}  // namespace engine
+
+namespace {
+
 // gamma = A^H B on the device (A, B: N x K complex128), into host gamma_out.
 double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int N, int64_t K, int prec,
                        double* gamma_out, int* nchunks_out) {
-    if (prec != TBK_GAMMA_FP64 && prec != TBK_GAMMA_BF16X3) raise(TBK_E_INVALID, "unknown contraction precision");
-    int64_t kc;
-    int nchunks;
-    tbk::GammaMap map{};
-    const bool big = N >= 2 * tbk::kBigTile;
-    if (big)
-        contraction_shape_big(ctx, N, K, prec == TBK_GAMMA_FP64 ? 0 : kSplitChunkCols, kc, map);
-    else
-        contraction_shape(ctx, N, K, kc, nchunks);
-    if (big) nchunks = map.chunks;
+    const ContractionShape cs = contraction_plan(ctx, N, K, prec);
     const int64_t NN = static_cast<int64_t>(N) * N;
     DevBuf& P = ctx->gamma_part;
     DevBuf& out = ctx->gamma_dev;
-    P.reserve(static_cast<size_t>(nchunks) * NN * sizeof(double2));
+    P.reserve(static_cast<size_t>(cs.nchunks) * NN * sizeof(double2));
     out.reserve(static_cast<size_t>(NN) * sizeof(double2));
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    if (prec == TBK_GAMMA_FP64 && big)
-        tbk::launch_gamma_f64_big(A, B, N, K, kc, map, P.as<double2>(), ctx->stream);
-    else if (prec == TBK_GAMMA_FP64)
-        tbk::launch_gamma_f64(A, B, N, K, kc, nchunks, P.as<double2>(), ctx->stream);
-    else if (big)
-        tbk::launch_gamma_bf16x3_big(A, B, N, K, kc, map, P.as<double2>(), ctx->stream);
-    else
-        tbk::launch_gamma_bf16x3(A, B, N, K, kc, nchunks, P.as<double2>(), ctx->stream);
-    HIPC(hipGetLastError());
-    tbk::launch_gamma_reduce(P.as<const double2>(), nchunks, NN, out.as<double2>(), ctx->stream);
-    HIPC(hipGetLastError());
+    enqueue_contraction(cs, A, B, N, K, prec, P.as<double2>(), out.as<double2>(), ctx->stream);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipMemcpyAsync(gamma_out, out.p, static_cast<size_t>(NN) * sizeof(double2), hipMemcpyDeviceToHost,
                         ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
-    if (nchunks_out) *nchunks_out = nchunks;
+    if (nchunks_out) *nchunks_out = cs.nchunks;
     return ms;
 }
 
@@ -457,28 +516,10 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     const StepPlan pY = make_plan(ctx, vy, TBK_FP64, Rp, rows, 0.0, false);
     ctx->partial.reserve(static_cast<size_t>(std::max(pH.nb, std::max(pX.nb, pY.nb))) * sizeof(double2));
     double2* part = ctx->partial.as<double2>();
-    auto Lk = [&](int64_t k) { return static_cast<char*>(L.p) + static_cast<size_t>(k) * blk_bytes; };
-    auto Rk = [&](int64_t k) { return static_cast<char*>(Rs.p) + static_cast<size_t>(k) * blk_bytes; };
-    auto W = [&](int64_t k) { return ctx->work[k & 1].p; };
-
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    // left stack: T_n(H~)|r>
-    HIPC(hipMemcpyAsync(Lk(0), kets->data.p, blk_bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    if (N > 1) launch_plan(pH, ctx->stream, tbk::kFirst, 1.0 / a, -b / a, Lk(0), nullptr, Lk(1), Lk(0), part);
-    for (int64_t n = 1; n + 1 < N; ++n)
-        launch_plan(pH, ctx->stream, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, Lk(n), Lk(n - 1), Lk(n + 1), nullptr, part);
-    // right stack: v_x T_m(H~) v_y|r>, T_m in two rotating work blocks
-    launch_plan(pY, ctx->stream, tbk::kFirst, 1.0, 0.0, kets->data.p, nullptr, W(0), kets->data.p, part);
-    launch_plan(pX, ctx->stream, tbk::kFirst, 1.0, 0.0, W(0), nullptr, Rk(0), W(0), part);
-    if (N > 1) {
-        launch_plan(pH, ctx->stream, tbk::kFirst, 1.0 / a, -b / a, W(0), nullptr, W(1), W(0), part);
-        launch_plan(pX, ctx->stream, tbk::kFirst, 1.0, 0.0, W(1), nullptr, Rk(1), W(1), part);
-    }
-    for (int64_t m = 2; m < N; ++m) {
-        launch_plan(pH, ctx->stream, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part);
-        launch_plan(pX, ctx->stream, tbk::kFirst, 1.0, 0.0, W(m), nullptr, Rk(m), W(m), part);
-    }
+    enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, N, part,
+                        ctx->stream);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
     float rec_ms = 0.f;
@@ -501,18 +542,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         stats->flops = 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
         stats->stack_bytes = 2.0 * static_cast<double>(stack_bytes);
         stats->nchunks = nchunks;
-        // algorithmic bytes: matrix arrays once per launch; vector blocks
-        // read x (+ z) and write y; v applications read x and write y
-        auto mat = [&](const tbk_matrix* m) {
-            return static_cast<double>(m->nnz) * ((m->is_complex ? 16.0 : 8.0) + 4.0) +
-                   (static_cast<double>(rows) + 1.0) * 4.0;
-        };
-        const double blk = static_cast<double>(rows) * kets->R * 16.0;
-        const double Nm2 = static_cast<double>(std::max<int64_t>(0, N - 2));
-        double bytes = 2.0 * blk;                                   // |r> into the left stack
-        bytes += mat(vy) + 2.0 * blk + mat(vx) + 2.0 * blk;         // v_x v_y |r>
-        if (N > 1) bytes += 2.0 * (mat(H) + 2.0 * blk) + mat(vx) + 2.0 * blk;
-        bytes += Nm2 * (2.0 * (mat(H) + 3.0 * blk) + mat(vx) + 2.0 * blk);
+        const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N);
         stats->recursion_bytes = bytes;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

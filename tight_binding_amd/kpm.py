@@ -283,6 +283,95 @@ class Batch:
             pass
 
 
+class Server:
+    """Continuous-batching request server on one device (tbk_server_*).
+    Requests are copied at submission; results are taken by ticket."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_slots: int = 0, max_kubo: int = 0,
+                 chunk_steps: int = 0, cpl: int = 0, max_moments: int = 0, max_kubo_moments: int = 0,
+                 max_wait_bypass_ms: float = 0.0):
+        cfg = N.tbk_server_config(int(hbm_budget_bytes), int(max_slots), int(max_kubo), int(chunk_steps), int(cpl),
+                                  int(max_moments), int(max_kubo_moments), float(max_wait_bypass_ms))
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_server_create(int(device), C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._kind = {}
+
+    def _submit(self, req, keep):
+        t = C.c_int64()
+        N.check_kpm(N.kpm().tbk_server_submit(self._h, C.byref(req), C.byref(t)))
+        del keep
+        return t.value
+
+    def submit_dos(self, H: CSC, n: int, a: float, b: float = 0.0, R: int = 8, seed: int = 0,
+                   precision: int = FP64) -> int:
+        """Moments mu_0..mu_{n-1} of (H - b)/a from R device-generated random-phase vectors."""
+        sH, kH = tbk_sparse_of(H)
+        req = N.tbk_request(N.TBK_REQ_DOS, int(precision), int(n), int(R), int(seed), float(a), float(b),
+                            C.pointer(sH), None, None, None)
+        t = self._submit(req, (sH, kH))
+        self._kind[t] = ("dos", int(n))
+        return t
+
+    def submit_kubo(self, H: CSC, vx: CSC, vy: CSC, N_: int, a: float = 1.0, b: float = 0.0, R: int = 8,
+                    seed: int = 0, precision: int = N.TBK_GAMMA_FP64, g=None) -> int:
+        """Kubo-Bastin Gamma (N_ x N_, weighted by g) and zeta at the N_ Chebyshev nodes."""
+        sH, kH = tbk_sparse_of(H)
+        sx, kx = tbk_sparse_of(vx)
+        sy, ky = tbk_sparse_of(vy)
+        gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        req = N.tbk_request(N.TBK_REQ_KUBO, int(precision), int(N_), int(R), int(seed), float(a), float(b),
+                            C.pointer(sH), C.pointer(sx), C.pointer(sy), _dp(gg) if gg is not None else None)
+        t = self._submit(req, (sH, kH, sx, kx, sy, ky, gg))
+        self._kind[t] = ("kubo", int(N_))
+        return t
+
+    def info(self, ticket: int) -> dict:
+        inf = N.tbk_request_info()
+        N.check_kpm(N.kpm().tbk_server_info(self._h, int(ticket), C.byref(inf)))
+        return inf.as_dict()
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> dict:
+        inf = N.tbk_request_info()
+        N.check_kpm(N.kpm().tbk_server_wait(self._h, int(ticket), float(timeout), C.byref(inf)))
+        return inf.as_dict()
+
+    def take(self, ticket: int):
+        """(result, info): DoS -> moments (n,); sigma_xy -> (zeta (N,), Gamma (N, N)) complex."""
+        kind, n = self._kind.pop(int(ticket))
+        size = n if kind == "dos" else 2 * n + 2 * n * n
+        out = np.zeros(size, dtype=np.float64)
+        inf = N.tbk_request_info()
+        N.check_kpm(N.kpm().tbk_server_take(self._h, int(ticket), _dp(out), int(size), C.byref(inf)))
+        if kind == "dos":
+            return out, inf.as_dict()
+        z = out[:2 * n].view(np.complex128)
+        G = out[2 * n:].view(np.complex128).reshape(n, n)
+        return (z, G), inf.as_dict()
+
+    def stats(self) -> dict:
+        st = N.tbk_server_stats()
+        N.check_kpm(N.kpm().tbk_server_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            N.kpm().tbk_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def cheb_first_kind_seq_braket(M: CSC, n: int, bras_T: np.ndarray, kets: np.ndarray) -> np.ndarray:
     """Drop-in of cuda::Cheb_first_kind_seq_braket: M already rescaled (CSC),
     bras_T and kets (rows, R) complex; passing the same array object for both
