@@ -217,6 +217,227 @@ def batch_leg(ctx, nprob, R=8):
     return out
 
 
+SERVE_MODELS = {"square": (2, 4), "honeycomb": (2, 3), "lieb": (2, 4), "kagome": (2, 4), "cubic": (3, 6),
+                "fcc": (3, 12)}
+
+
+def build_serve_pool(n_dos, n_kubo, seed=2029):
+    """Hamiltonians the served requests draw from.  DoS: model uniform over
+    square, honeycomb, Lieb, Kagome, simple cubic and FCC (nearest neighbours,
+    t = 1, periodic), sites log-uniform in [1e3, 4e6], W ~ U[0, 2].  sigma_xy:
+    square or honeycomb, open, Landau field of flux 1/16 per plaquette (square)
+    or per cell (honeycomb), sites log-uniform in [1e3, 6.5e4], W ~ U[0, 2],
+    with v_x, v_y = [X, H~], [Y, H~] of the rescaled H~."""
+    from tight_binding_amd import kubo as Kb
+    from tight_binding_amd import lattice as Lm
+
+    rng = np.random.default_rng(seed)
+    dos, kub = [], []
+    for i in range(n_dos):
+        model = list(SERVE_MODELS)[int(rng.integers(0, len(SERVE_MODELS)))]
+        dim, z = SERVE_MODELS[model]
+        c = Lm.UnitCell.model(model)
+        c.add_hopping_nn("A", 1, 1.0)
+        if model == "kagome":
+            c.add_hopping_nn("B", 1, 1.0)
+        atoms = {"honeycomb": 2, "lieb": 3, "kagome": 3}.get(model, 1)
+        sites = math.exp(rng.uniform(math.log(1e3), math.log(4e6)))
+        Lc = max(2, int(round((sites / atoms) ** (1.0 / dim))))
+        W = float(rng.uniform(0.0, 2.0))
+        lat = Lm.PartialLattice(c, (Lc,) * dim)
+        lat.set_disorder(W, seed + i)
+        H = lat.get_Hamiltonian_periodic_extend_bare_sparse((1,) * dim)
+        dos.append(dict(H=H, a=1.02 * (z + W / 2), model=model, sites=H.n))
+    for i in range(n_kubo):
+        model = ["square", "honeycomb"][i % 2]
+        c = Lm.UnitCell.model(model)
+        c.add_hopping_nn("A", 1, 1.0)
+        atoms = 2 if model == "honeycomb" else 1
+        sites = math.exp(rng.uniform(math.log(1e3), math.log(6.5e4)))
+        Lc = max(4, int(round(math.sqrt(sites / atoms))))
+        W = float(rng.uniform(0.0, 2.0))
+        lat = Lm.PartialLattice(c, (Lc, Lc))
+        lat.set_gauge(Lm.GAUGE_LANDAU, 2 * math.pi / 16)
+        lat.set_disorder(W, seed + 1000 + i)
+        H = lat.get_Hamiltonian_sparse(True)
+        emax = 1.02 * ((4 if model == "square" else 3) + W / 2)
+        lo, hi = -emax - 1e-8, emax + 1e-8
+        a = (hi - lo) / 2
+        Hs = Lm.rescale(H, a, (hi + lo) / 2)
+        vx, vy = Kb.velocity_operators(Hs, lat.op_position(0), lat.op_position(1))
+        kub.append(dict(H=Hs, vx=vx, vy=vy, model=model, sites=Hs.n))
+    return dos, kub
+
+
+def make_requests(n, dos_pool, kubo_pool, qps, seed=2030):
+    """n requests: Poisson arrivals at `qps`, 70% DoS (N ~ U{256..2048}) and
+    30% sigma_xy (N ~ U{256..1024}), R in {8, 16, 32}, half of each kind in
+    fp64 and half mixed precision (fp32 recursion / split-bf16 Gamma)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / qps))
+        dos = rng.uniform() < 0.7
+        mixed = bool(rng.uniform() < 0.5)
+        if dos:
+            out.append(dict(kind="dos", t=t, p=int(rng.integers(0, len(dos_pool))), N=int(rng.integers(256, 2049)),
+                            R=int(rng.choice([8, 16, 32])), mixed=mixed, seed=7000 + i))
+        else:
+            out.append(dict(kind="kubo", t=t, p=int(rng.integers(0, len(kubo_pool))), N=int(rng.integers(256, 1025)),
+                            R=int(rng.choice([8, 16, 32])), mixed=mixed, seed=7000 + i))
+    return out
+
+
+def serve_leg(qps, n_requests, pool_dos=24, pool_kubo=6):
+    """Served mixed workload: continuous batching in the request server, with
+    latency percentiles at a fixed Poisson arrival rate, the makespan of the
+    same requests all submitted at once, and the plain per-solve GPU path on
+    the same requests (one call per request) for comparison."""
+    import threading
+
+    from tight_binding_amd import kpm
+    from tight_binding_amd import spectral as S
+
+    t0 = time.perf_counter()
+    dpool, kpool = build_serve_pool(pool_dos, pool_kubo)
+    reqs = make_requests(n_requests, dpool, kpool, qps)
+    log("serve: pool of %d DoS + %d sigma_xy Hamiltonians (%.1fM sites) assembled in %.1f s; %d requests at %.1f QPS"
+        % (len(dpool), len(kpool), sum(d["sites"] for d in dpool) / 1e6, time.perf_counter() - t0, len(reqs), qps))
+    g_of = {}
+
+    def submit(srv, r):
+        if r["kind"] == "dos":
+            d = dpool[r["p"]]
+            return srv.submit_dos(d["H"], r["N"], d["a"], 0.0, r["R"], r["seed"], kpm.FP32 if r["mixed"] else kpm.FP64)
+        k = kpool[r["p"]]
+        g = g_of.setdefault(r["N"], S.lorentz(r["N"], 6.0))
+        return srv.submit_kubo(k["H"], k["vx"], k["vy"], r["N"], R=r["R"], seed=r["seed"],
+                               precision=kpm.GAMMA_BF16X3 if r["mixed"] else kpm.GAMMA_FP64, g=g)
+
+    def serve(arrivals):
+        srv = kpm.Server(0, max_kubo=2)
+        tickets = [None] * len(reqs)
+        start = time.perf_counter()
+
+        def feeder():
+            for i, r in enumerate(reqs):
+                if arrivals:
+                    dt = r["t"] - (time.perf_counter() - start)
+                    if dt > 0:
+                        time.sleep(dt)
+                tickets[i] = submit(srv, r)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        results, lat = [], []
+        for i, t in enumerate(tickets):
+            info = srv.wait(t, 300.0)
+            if info["state"] != 2:
+                raise SystemExit("bench: served request %d did not finish: %s" % (i, info))
+            res, info = srv.take(t)
+            results.append(res)
+            lat.append(info["latency_ms"])
+        span = time.perf_counter() - start
+        st = srv.stats()
+        srv.close()
+        return results, np.asarray(lat), span, st
+
+    res_p, lat_p, span_p, st_p = serve(True)
+    log("serve: Poisson run %.1f s, latency p50 %.1f ms p99 %.1f ms" % (span_p, np.percentile(lat_p, 50),
+                                                                       np.percentile(lat_p, 99)))
+    res_s, lat_s, span_s, st_s = serve(False)
+    log("serve: saturation run (all submitted at once) %.2f s" % span_s)
+
+    # plain per-solve GPU path: one call per request, fp64 and mixed as requested,
+    # plus the fp64 answer of every mixed-precision request for its error
+    ctx = kpm.Context(0)
+    t1 = time.perf_counter()
+    plain = []
+    for r in reqs:
+        if r["kind"] == "dos":
+            d = dpool[r["p"]]
+            M = ctx.matrix(d["H"])
+            K = ctx.random_block(d["H"].n, r["R"], seed=r["seed"], precision=kpm.FP32 if r["mixed"] else kpm.FP64)
+            mu, _, _ = ctx.moments(M, K, r["N"], a=d["a"])
+            plain.append(mu)
+            K.close()
+            M.close()
+        else:
+            k = kpool[r["p"]]
+            Ms = [ctx.matrix(k[x]) for x in ("H", "vx", "vy")]
+            K = ctx.random_block(k["H"].n, r["R"], seed=r["seed"])
+            G, _ = kpm.kubo_gamma(ctx, *Ms, K, r["N"], precision=kpm.GAMMA_BF16X3 if r["mixed"] else kpm.GAMMA_FP64,
+                                  g=g_of[r["N"]])
+            z, _ = kpm.kubo_zeta(ctx, G, S.kubo_nodes(r["N"]))
+            plain.append((z, G))
+            K.close()
+            for m in Ms:
+                m.close()
+    span_plain = time.perf_counter() - t1
+    ctx.release_work()
+    log("serve: plain per-solve path %.2f s" % span_plain)
+
+    # checks: served == plain (same requests, same precision); mixed vs fp64
+    err_dos, err_sig = [], []
+    for r, a, b in zip(reqs, res_s, plain):
+        if r["kind"] == "dos":
+            assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
+        else:
+            assert np.max(np.abs(a[1] - b[1])) <= 1e-12 * np.max(np.abs(b[1]))
+    for r, a in zip(reqs, res_s):
+        if not r["mixed"]:
+            continue
+        if r["kind"] == "dos":
+            d = dpool[r["p"]]
+            M = ctx.matrix(d["H"])
+            K = ctx.random_block(d["H"].n, r["R"], seed=r["seed"])
+            mu64, _, _ = ctx.moments(M, K, r["N"], a=d["a"])
+            err_dos.append(float(np.max(np.abs(a - mu64)) / np.max(np.abs(mu64))))
+            K.close()
+            M.close()
+        elif len(err_sig) < 4:
+            k = kpool[r["p"]]
+            Ms = [ctx.matrix(k[x]) for x in ("H", "vx", "vy")]
+            K = ctx.random_block(k["H"].n, r["R"], seed=r["seed"])
+            G64, _ = kpm.kubo_gamma(ctx, *Ms, K, r["N"], g=g_of[r["N"]])
+            ef = np.linspace(-0.7, 0.7, 201)
+            s64 = S.kubo_sigma(G64, ef)[0]
+            smx = S.kubo_sigma(a[1], ef)[0]
+            err_sig.append(float(np.max(np.abs(smx - s64)) / np.max(np.abs(s64))))
+            K.close()
+            for m in Ms:
+                m.close()
+    ctx.close()
+    kinds = np.array([r["kind"] for r in reqs])
+    moments = sum(r["N"] for r in reqs if r["kind"] == "dos")
+    out = {
+        "workload": "%d requests, Poisson arrivals at %.1f QPS, 70%% DoS (6 models, 1e3-4e6 sites, N 256-2048) / 30%% "
+                    "sigma_xy (1e3-6.5e4 sites, N 256-1024), R in {8,16,32}, half fp64 and half mixed precision; "
+                    "Hamiltonians drawn from a pool of %d + %d pre-assembled lattices" % (len(reqs), qps, len(dpool),
+                                                                                        len(kpool)),
+        "latency_ms_p50": float(np.percentile(lat_p, 50)),
+        "latency_ms_p99": float(np.percentile(lat_p, 99)),
+        "latency_ms_p50_dos": float(np.percentile(lat_p[kinds == "dos"], 50)),
+        "latency_ms_p99_dos": float(np.percentile(lat_p[kinds == "dos"], 99)),
+        "latency_ms_p50_sigma": float(np.percentile(lat_p[kinds == "kubo"], 50)) if (kinds == "kubo").any() else None,
+        "latency_ms_p99_sigma": float(np.percentile(lat_p[kinds == "kubo"], 99)) if (kinds == "kubo").any() else None,
+        "poisson_run_s": span_p,
+        "served_makespan_s": span_s,
+        "served_requests_per_s": len(reqs) / span_s,
+        "served_dos_moments_per_s": moments / span_s,
+        "plain_per_solve_s": span_plain,
+        "plain_requests_per_s": len(reqs) / span_plain,
+        "throughput_ratio_served_over_plain": span_plain / span_s,
+        "chunks": st_s["chunks"],
+        "mixed_dos_max_rel_err_vs_fp64": max(err_dos) if err_dos else None,
+        "mixed_sigma_max_rel_err_vs_fp64": max(err_sig) if err_sig else None,
+    }
+    log("serve: %s" % json.dumps(out))
+    return out
+
+
 def cpu_baseline(H, R, a, seconds):
     """The oracle (C restatement of the reference's CPU recursion, single
     thread) timed on a bounded number of steps of the same workload."""
@@ -264,6 +485,9 @@ def main():
     ap.add_argument("--kubo-only", action="store_true", help="run only the sigma_xy leg")
     ap.add_argument("--batch-problems", type=int, default=256, help="problems of the batched leg (0: skip)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
+    ap.add_argument("--serve-requests", type=int, default=96, help="requests of the served leg (0: skip)")
+    ap.add_argument("--serve-qps", type=float, default=8.0)
+    ap.add_argument("--serve-only", action="store_true", help="run only the served leg")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -272,6 +496,10 @@ def main():
         sys.exit(2)
 
     from tight_binding_amd import kpm
+
+    if args.serve_only:
+        print(json.dumps({"serve": serve_leg(args.serve_qps, max(1, args.serve_requests))}), flush=True)
+        return
 
     if args.batch_only:
         ctx = kpm.Context(0)
@@ -371,6 +599,10 @@ def main():
     if args.batch_problems > 0:
         ctx.release_work()
         batch_out = batch_leg(ctx, args.batch_problems)
+    serve_out = None
+    if args.serve_requests > 0:
+        ctx.release_work()
+        serve_out = serve_leg(args.serve_qps, args.serve_requests)
 
     value = st["steps"] / (st["kernel_ms"] * 1e-3)
     out = {
@@ -414,6 +646,8 @@ def main():
         out["kubo_sigma_xy"] = kubo_out
     if batch_out:
         out["batched_ragged"] = batch_out
+    if serve_out:
+        out["served_mixed"] = serve_out
     print(json.dumps(out), flush=True)
     ctx.close()
 

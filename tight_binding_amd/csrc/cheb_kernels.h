@@ -365,10 +365,13 @@ __global__ __launch_bounds__(kBlock) void cheb_batch_kernel(
 // Resident problems of a serving loop ("slots"), each with its own CSR
 // arrays and three rotating vector blocks (T_k in blk[k % 3]); the slots
 // of one class (precision, value type, padded width) advance together, one
-// launch per step.  tile0/ntiles number the slot's row tiles within the
-// launch; k0 is the slot's step index at the chunk's first launch, so the
-// launch with offset s performs step k0 + s of every slot that has not
-// finished (k0 + s < steps).  Step 0 is the first step (T_1, moments 0, 1).
+// launch per step.  k0 is the slot's step index at the chunk's first launch,
+// so the launch with offset s performs step k0 + s of every slot that has
+// not finished (k0 + s < steps).  Step 0 is the first step (T_1, moments 0
+// and 1).  A slot's ntiles row tiles are visited in the transposed chunk
+// order of period P (Q = ceil(ntiles / P) rows of the chunk grid, one tile
+// per chunk): visit v of the slot is launch tile visit0 + v, v < P*Q, and
+// row tile rt writes its partial sums to slots part0 + rt.
 struct SlotDesc {
     const int32_t* rp;
     const int32_t* ci;
@@ -378,10 +381,12 @@ struct SlotDesc {
     double* mu;      // n moments
     int64_t k0;
     int64_t steps;   // (n + 1) / 2
-    int64_t tile0;
+    int64_t visit0;
+    int64_t part0;
     int32_t rows;
     int32_t ntiles;
     int32_t n;
+    uint32_t P, Q;
     int32_t pad_;
 };
 
@@ -405,7 +410,10 @@ __global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __re
         const SlotDesc& d = slots[tile_slot[t]];
         const int64_t k = d.k0 + s;
         if (k >= d.steps) continue;
-        const int64_t lt = t - d.tile0;
+        const uint32_t v = static_cast<uint32_t>(t - d.visit0);
+        const uint32_t pc = v / d.Q, qr = v - pc * d.Q;
+        const int64_t lt = static_cast<int64_t>(qr) * d.P + pc;
+        if (lt >= d.ntiles) continue;
         const V2* x = static_cast<const V2*>(d.blk[k % 3]);
         const V2* z = static_cast<const V2*>(d.blk[(k + 2) % 3]);
         V2* y = static_cast<V2*>(d.blk[(k + 1) % 3]);
@@ -427,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __re
         }
         p0 = wave_sum(p0);
         p1 = wave_sum(p1);
-        if (lane == 0) partial[4 * t + wave] = make_double2(p0, p1);
+        if (lane == 0) partial[4 * (d.part0 + lt) + wave] = make_double2(p0, p1);
     }
 }
 

@@ -45,24 +45,37 @@ using namespace engine;
 constexpr int32_t kSlotTileRows = 256;
 
 // Moments of step k0 + s of every slot (one workgroup per slot): its wave
-// partials summed in a fixed order, then mu_0, mu_1 (step 0) or the doubling
-// identities mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
-__global__ __launch_bounds__(tbk::kBlock) void slots_moments_kernel(const tbk::SlotDesc* __restrict__ slots,
-                                                                    int32_t s, const double2* __restrict__ partial) {
+// partials summed in a fixed order (four interleaved accumulators per
+// thread, then a tree), then mu_0, mu_1 (step 0) or the doubling identities
+// mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
+constexpr int kMomThreads = 1024;
+__global__ __launch_bounds__(kMomThreads) void slots_moments_kernel(const tbk::SlotDesc* __restrict__ slots, int32_t s,
+                                                                    const double2* __restrict__ partial) {
     const tbk::SlotDesc& d = slots[blockIdx.x];
     const int64_t k = d.k0 + s;
     if (k >= d.steps) return;  // the whole workgroup
-    const int64_t s0 = 4 * d.tile0, s1 = 4 * (d.tile0 + d.ntiles);
-    double a0 = 0.0, a1 = 0.0;
-    for (int64_t i = s0 + threadIdx.x; i < s1; i += tbk::kBlock) {
-        a0 = a0 + partial[i].x;
-        a1 = a1 + partial[i].y;
+    const double2* p = partial + 4 * d.part0;
+    const int64_t cnt = 4 * static_cast<int64_t>(d.ntiles);
+    double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t i = threadIdx.x;
+    for (; i + 3 * kMomThreads < cnt; i += 4 * kMomThreads) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double2 v = p[i + u * kMomThreads];
+            a0[u] = a0[u] + v.x;
+            a1[u] = a1[u] + v.y;
+        }
     }
-    __shared__ double r0[tbk::kBlock], r1[tbk::kBlock];
-    r0[threadIdx.x] = a0;
-    r1[threadIdx.x] = a1;
+    for (int u = 0; i < cnt; i += kMomThreads, ++u) {
+        const double2 v = p[i];
+        a0[u] = a0[u] + v.x;
+        a1[u] = a1[u] + v.y;
+    }
+    __shared__ double r0[kMomThreads], r1[kMomThreads];
+    r0[threadIdx.x] = (a0[0] + a0[1]) + (a0[2] + a0[3]);
+    r1[threadIdx.x] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
     __syncthreads();
-    for (int w = tbk::kBlock / 2; w >= 1; w >>= 1) {
+    for (int w = kMomThreads / 2; w >= 1; w >>= 1) {
         if (static_cast<int>(threadIdx.x) < w) {
             r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
             r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
@@ -136,7 +149,8 @@ struct Slot {
     DevBuf blk[3], mu;
     int64_t k = 0, steps = 0;
     int32_t ntiles = 0;
-    int32_t pin = -1;  // row of the pinned moment staging
+    uint32_t P = 1, Q = 1;  // transposed tile order (period P = bandwidth / tile rows)
+    int32_t pin = -1;       // row of the pinned moment staging
     ~Slot() {
         if (M) tbk_matrix_destroy(M);
     }
@@ -186,40 +200,50 @@ struct tbk_server {
     tbk_server_config cfg{};
     int64_t budget = 0;
 
+    // shared state, under mu
     std::mutex mu;
-    std::condition_variable cv, done_cv;
-    std::deque<Request*> queue;
+    std::condition_variable cv_prep, cv_work, done_cv;
+    std::deque<Request*> queue;          // submitted, waiting for admission
     std::map<int64_t, std::unique_ptr<Request>> reqs;
+    std::vector<Slot*> ready;            // prepared slots, joining at the next chunk
+    std::vector<Slot*> trash;            // finished slots, freed by the preparation thread
+    std::vector<void*> trash_raw;        // replaced per-class arrays, freed by the preparation thread
+    int64_t prep_gen = 0;                // bumped by every change the preparation thread acts on
+    std::vector<int32_t> free_pin, free_kpin;
     int64_t next_ticket = 1;
     int64_t used = 0;
-    bool stop = false;
+    int32_t preparing = 0;               // admitted requests still being prepared
+    bool stop = false, worker_done = false;
     std::string fatal;
     tbk_server_stats st{};
 
-    std::thread worker;
+    std::thread worker, prep;
     // worker-only state
     std::vector<Slot*> slots;
-    std::vector<KuboJob*> kjobs;
     std::map<ClassKey, ClassState> classes;
     std::deque<Chunk> inflight;
     std::vector<hipEvent_t> free_events;
     int64_t chunk_seq = 0;
-    double* pin_mu = nullptr;  // max_slots x max_moments doubles
-    std::vector<int32_t> free_pin;
+    std::vector<void*> pinned_extra;     // grown staging buffers, freed at destroy
+    // preparation-thread-only state
+    std::vector<KuboJob*> kjobs;
+    double* pin_mu = nullptr;    // max_slots x max_moments doubles
     double* pin_kubo = nullptr;  // max_kubo x (2 max_kubo_N + 2 max_kubo_N^2) doubles
-    std::vector<int32_t> free_kpin;
-    std::vector<void*> pinned_extra;  // grown staging buffers, freed at destroy
+
+    static constexpr size_t kMaxInflight = 2;
 
     int64_t dos_footprint(const Request& r) const;
     int64_t kubo_footprint(const Request& r) const;
     void run();
-    std::vector<Request*> pick_admissions();
-    void start_dos(Request* r);
-    void start_kubo(Request* r);
+    void run_prep();
+    std::vector<std::pair<Request*, int32_t>> pick_admissions();
+    Slot* start_dos(Request* r, int32_t pin);
+    void start_kubo(Request* r, int32_t pin);
     void enqueue_chunk();
     void complete_chunk(Chunk& c);
     void poll_kubo(bool block);
     void finish(Request* r, int32_t code, const std::string& msg);
+    void fail_all(const std::string& msg);
     hipEvent_t get_event();
     void grow(DevBuf& b, size_t bytes);
     template <typename T>
@@ -269,10 +293,14 @@ hipEvent_t tbk_server::get_event() {
 void tbk_server::grow(DevBuf& b, size_t bytes) {
     if (bytes <= b.bytes) return;
     if (b.p) {
-        if (!inflight.empty())
+        if (!inflight.empty()) {
             inflight.back().graveyard.push_back(b.p);
-        else
-            (void)hipFree(b.p);
+        } else {
+            std::lock_guard<std::mutex> lk(mu);
+            trash_raw.push_back(b.p);
+            prep_gen++;
+            cv_prep.notify_all();
+        }
         b.p = nullptr;
         b.bytes = 0;
     }
@@ -311,47 +339,46 @@ void tbk_server::finish(Request* r, int32_t code, const std::string& msg) {
     r->vx.clear();
     r->vy.clear();
     done_cv.notify_all();
+    prep_gen++;
+    cv_prep.notify_all();
 }
 
-// First-fit over the queue in arrival order; a request that has waited more
-// than max_wait_bypass_ms at the front stops later ones from passing it.
-std::vector<Request*> tbk_server::pick_admissions() {
-    std::vector<Request*> out;
+// First-fit over the queue in arrival order (under mu); a request that has
+// waited more than max_wait_bypass_ms stops later ones from passing it.
+// Each admitted request takes its pinned result row.
+std::vector<std::pair<Request*, int32_t>> tbk_server::pick_admissions() {
+    std::vector<std::pair<Request*, int32_t>> out;
     const double t = now_s();
-    // slots and jobs in use, counted by their pinned result rows (a slot that
-    // finishes in an in-flight chunk keeps its row until the chunk completes)
-    int32_t dos_active = cfg.max_slots - static_cast<int32_t>(free_pin.size());
-    int32_t kubo_active = cfg.max_kubo - static_cast<int32_t>(free_kpin.size());
     for (auto it = queue.begin(); it != queue.end();) {
         Request* r = *it;
         const bool is_dos = r->kind == TBK_REQ_DOS;
-        const bool room = is_dos ? dos_active < cfg.max_slots : kubo_active < cfg.max_kubo;
-        if (room && used + r->footprint <= budget) {
+        std::vector<int32_t>& pins = is_dos ? free_pin : free_kpin;
+        if (!pins.empty() && used + r->footprint <= budget) {
             used += r->footprint;
             r->state = TBK_REQ_RUNNING;
             r->t_admit = t;
-            if (is_dos) {
-                ++dos_active;
+            if (is_dos)
                 st.running_dos++;
-            } else {
-                ++kubo_active;
+            else
                 st.running_kubo++;
-            }
-            out.push_back(r);
+            out.emplace_back(r, pins.back());
+            pins.pop_back();
             it = queue.erase(it);
             continue;
         }
         if (1e3 * (t - r->t_submit) > cfg.max_wait_bypass_ms) break;
         ++it;
     }
+    preparing += static_cast<int32_t>(out.size());
     st.queued = static_cast<int64_t>(queue.size());
     st.used_bytes = used;
     return out;
 }
 
-void tbk_server::start_dos(Request* r) {
+Slot* tbk_server::start_dos(Request* r, int32_t pin) {
     auto s = std::make_unique<Slot>();
     s->req = r;
+    s->pin = pin;
     int rc = tbk_matrix_create(ctx, &r->H.hdr, &s->M);
     if (rc != TBK_OK) raise(rc, g_last_error);
     if (r->precision == TBK_FP32) matrix_fp32(s->M);
@@ -361,6 +388,11 @@ void tbk_server::start_dos(Request* r) {
     s->mu.reserve(static_cast<size_t>(r->n) * 8);
     s->steps = (r->n + 1) / 2;
     s->ntiles = static_cast<int32_t>((rows + kSlotTileRows - 1) / kSlotTileRows);
+    // visiting order: tiles a matrix bandwidth apart one after the other, so
+    // a gathered neighbour row is reused from L2 by the next tile
+    const int64_t period = std::max<int64_t>(1, (s->M->bandwidth95 + kSlotTileRows / 2) / kSlotTileRows);
+    s->P = static_cast<uint32_t>(std::min<int64_t>(period, s->ntiles));
+    s->Q = static_cast<uint32_t>((s->ntiles + s->P - 1) / s->P);
     const size_t total = static_cast<size_t>(rows) * r->Rp;
     const unsigned grid = static_cast<unsigned>(std::min<size_t>((total + tbk::kBlock - 1) / tbk::kBlock, 65536));
     const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
@@ -371,14 +403,13 @@ void tbk_server::start_dos(Request* r) {
         hipLaunchKernelGGL(tbk::random_phase_kernel<float2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
                            s->blk[0].as<float2>(), static_cast<int32_t>(rows), r->R, r->Rp, r->seed, inv);
     HIPC(hipGetLastError());
-    s->pin = free_pin.back();
-    free_pin.pop_back();
-    slots.push_back(s.release());
+    return s.release();
 }
 
-void tbk_server::start_kubo(Request* r) {
+void tbk_server::start_kubo(Request* r, int32_t pin) {
     auto j = std::make_unique<KuboJob>();
     j->req = r;
+    j->pin = pin;
     int rc = tbk_matrix_create(ctx, &r->H.hdr, &j->H);
     if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vx.hdr, &j->vx);
     if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vy.hdr, &j->vy);
@@ -402,16 +433,6 @@ void tbk_server::start_kubo(Request* r) {
     j->g.reserve(static_cast<size_t>(N) * sizeof(double));
     j->e.reserve(static_cast<size_t>(N) * sizeof(double));
     j->z.reserve(static_cast<size_t>(N) * sizeof(double2));
-    j->pin = free_kpin.back();
-    free_kpin.pop_back();
-    struct PinGuard {
-        tbk_server* s;
-        KuboJob* j;
-        bool armed = true;
-        ~PinGuard() {
-            if (armed) s->free_kpin.push_back(j->pin);
-        }
-    } guard{this, j.get()};
     const int64_t Ncap = cfg.max_kubo_moments;
     double* pz = pin_kubo + static_cast<size_t>(j->pin) * static_cast<size_t>(2 * Ncap + 2 * Ncap * Ncap);
     double* pg = pz + 2 * N;  // Gamma region; the weights and nodes are staged there first
@@ -439,7 +460,6 @@ void tbk_server::start_kubo(Request* r) {
     HIPC(hipMemcpyAsync(pg, j->G.p, static_cast<size_t>(NN) * 16, hipMemcpyDeviceToHost, st_));
     HIPC(hipEventCreateWithFlags(&j->done, hipEventDisableTiming));
     HIPC(hipEventRecord(j->done, st_));
-    guard.armed = false;
     kjobs.push_back(j.release());
 }
 
@@ -456,12 +476,15 @@ void tbk_server::poll_kubo(bool block) {
         const int64_t N = r->n, Ncap = cfg.max_kubo_moments;
         const double* pz = pin_kubo + static_cast<size_t>(j->pin) * static_cast<size_t>(2 * Ncap + 2 * Ncap * Ncap);
         r->result.assign(pz, pz + 2 * N + 2 * N * N);
-        free_kpin.push_back(j->pin);
+        const int32_t pin = j->pin;
         kjobs.erase(kjobs.begin() + static_cast<std::ptrdiff_t>(i));
         delete j;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            free_kpin.push_back(pin);
+            st.kubo_done++;
+        }
         finish(r, TBK_OK, "");
-        std::lock_guard<std::mutex> lk(mu);
-        st.kubo_done++;
     }
 }
 
@@ -480,11 +503,14 @@ void tbk_server::enqueue_chunk() {
         const ClassKey& key = kv.first;
         std::vector<Slot*>& v = kv.second;
         ClassState& cs = classes[key];
-        int64_t ntiles = 0;
-        for (Slot* s : v) ntiles += s->ntiles;
+        int64_t ntiles = 0, nvisits = 0;
+        for (Slot* s : v) {
+            ntiles += s->ntiles;
+            nvisits += static_cast<int64_t>(s->P) * s->Q;
+        }
         tbk::SlotDesc* hd = stage(cs.stage_desc[c.parity], cs.cap_desc[c.parity], v.size());
-        int32_t* ht = stage(cs.stage_tiles[c.parity], cs.cap_tiles[c.parity], static_cast<size_t>(ntiles));
-        int64_t t0 = 0;
+        int32_t* ht = stage(cs.stage_tiles[c.parity], cs.cap_tiles[c.parity], static_cast<size_t>(nvisits));
+        int64_t t0 = 0, v0 = 0;
         for (size_t i = 0; i < v.size(); ++i) {
             Slot* s = v[i];
             const Request* r = s->req;
@@ -501,34 +527,40 @@ void tbk_server::enqueue_chunk() {
             d.mu = s->mu.as<double>();
             d.k0 = s->k;
             d.steps = s->steps;
-            d.tile0 = t0;
+            d.visit0 = v0;
+            d.part0 = t0;
             d.rows = r->H.hdr.rows;
             d.ntiles = s->ntiles;
             d.n = r->n;
-            for (int32_t t = 0; t < s->ntiles; ++t) ht[t0 + t] = static_cast<int32_t>(i);
+            d.P = s->P;
+            d.Q = s->Q;
+            const int64_t nv = static_cast<int64_t>(s->P) * s->Q;
+            for (int64_t t = 0; t < nv; ++t) ht[v0 + t] = static_cast<int32_t>(i);
             t0 += s->ntiles;
+            v0 += nv;
         }
         grow(cs.desc, v.size() * sizeof(tbk::SlotDesc));
-        grow(cs.tile_slot, static_cast<size_t>(ntiles) * sizeof(int32_t));
+        grow(cs.tile_slot, static_cast<size_t>(nvisits) * sizeof(int32_t));
         grow(cs.partial, static_cast<size_t>(4 * ntiles) * sizeof(double2));
         HIPC(hipMemcpyAsync(cs.desc.p, hd, v.size() * sizeof(tbk::SlotDesc), hipMemcpyHostToDevice, st_));
-        HIPC(hipMemcpyAsync(cs.tile_slot.p, ht, static_cast<size_t>(ntiles) * sizeof(int32_t), hipMemcpyHostToDevice,
+        HIPC(hipMemcpyAsync(cs.tile_slot.p, ht, static_cast<size_t>(nvisits) * sizeof(int32_t), hipMemcpyHostToDevice,
                             st_));
         tbk::SlotArgs A{};
         A.slots = cs.desc.as<const tbk::SlotDesc>();
         A.tile_slot = cs.tile_slot.as<const int32_t>();
-        A.ntiles = ntiles;
+        A.ntiles = nvisits;
         A.Rp = key.Rp;
         A.tile_rows = kSlotTileRows;
         A.opts = ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0;
-        int64_t nb = std::min<int64_t>(ntiles, static_cast<int64_t>(ctx->num_cu) * 4);
+        int64_t nb = std::min<int64_t>(nvisits, static_cast<int64_t>(ctx->num_cu) * 4);
         A.xcd_split = 0;
         if (nb >= 64) {
             nb = (nb / 8) * 8;
             A.xcd_split = 1;
         }
         A.partial = cs.partial.as<double2>();
-        int cpl = cfg.cpl > 0 ? cfg.cpl : (key.Rp >= 2 ? 2 : 1);
+        // default: 16-byte lanes (one complex128 or two complex64 per lane)
+        int cpl = cfg.cpl > 0 ? cfg.cpl : (key.prec == TBK_FP64 ? 1 : std::min(2, key.Rp));
         if (key.Rp / cpl > tbk::kWave || (cpl == 4 && key.Rp < 8)) cpl = key.Rp > tbk::kWave ? key.Rp / tbk::kWave : 1;
         for (int32_t s = 0; s < S; ++s) {
             A.s = s;
@@ -542,7 +574,7 @@ void tbk_server::enqueue_chunk() {
                               : tbk::launch_slots_f32_real(key.Rp, cpl, grid, st_, A);
             if (!ok) raise(TBK_E_UNSUPPORTED, "no slot kernel for this width");
             HIPC(hipGetLastError());
-            hipLaunchKernelGGL(slots_moments_kernel, dim3(static_cast<unsigned>(v.size())), dim3(tbk::kBlock), 0, st_,
+            hipLaunchKernelGGL(slots_moments_kernel, dim3(static_cast<unsigned>(v.size())), dim3(kMomThreads), 0, st_,
                                cs.desc.as<const tbk::SlotDesc>(), s, cs.partial.as<const double2>());
             HIPC(hipGetLastError());
         }
@@ -571,79 +603,146 @@ void tbk_server::enqueue_chunk() {
 void tbk_server::complete_chunk(Chunk& c) {
     HIPC(hipEventSynchronize(c.ev));
     free_events.push_back(c.ev);
-    for (void* p : c.graveyard) (void)hipFree(p);
     for (Slot* s : c.finishing) {
         Request* r = s->req;
         const double* src = pin_mu + static_cast<size_t>(s->pin) * static_cast<size_t>(cfg.max_moments);
         r->result.assign(src, src + r->n);
-        free_pin.push_back(s->pin);
-        delete s;
         finish(r, TBK_OK, "");
+    }
+    {
+        // device memory goes back through the preparation thread, whose frees
+        // may wait for the device
+        std::lock_guard<std::mutex> lk(mu);
+        for (Slot* s : c.finishing) {
+            free_pin.push_back(s->pin);
+            trash.push_back(s);
+        }
+        for (void* p : c.graveyard) trash_raw.push_back(p);
+        prep_gen++;
+        cv_prep.notify_all();
     }
 }
 
+void tbk_server::fail_all(const std::string& msg) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (fatal.empty()) fatal = msg;
+    for (auto& kv : reqs) {
+        Request* r = kv.second.get();
+        if (r->state == TBK_REQ_QUEUED || r->state == TBK_REQ_RUNNING) {
+            r->state = TBK_REQ_FAILED;
+            r->error_code = TBK_E_DEVICE;
+            r->error = fatal;
+            r->t_done = now_s();
+            st.failed++;
+        }
+    }
+    queue.clear();
+    done_cv.notify_all();
+    prep_gen++;
+    cv_prep.notify_all();
+    cv_work.notify_all();
+}
+
+// Worker: batched chunks of the DoS slots, at most kMaxInflight in flight.
 void tbk_server::run() {
     try {
         set_device(ctx);
         for (;;) {
-            std::vector<Request*> adm;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                const bool idle = slots.empty() && kjobs.empty() && inflight.empty();
-                if (idle && queue.empty()) {
-                    if (stop) break;
-                    cv.wait(lk, [&] { return stop || !queue.empty(); });
-                    continue;
-                }
-                adm = pick_admissions();
-            }
-            for (Request* r : adm) {
-                try {
-                    if (r->kind == TBK_REQ_DOS)
-                        start_dos(r);
-                    else
-                        start_kubo(r);
-                } catch (const Error& e) {
-                    finish(r, e.code, e.msg);
+                for (;;) {
+                    for (Slot* x : ready) slots.push_back(x);
+                    ready.clear();
+                    if (!slots.empty() || !inflight.empty()) break;
+                    if (!fatal.empty() || (stop && queue.empty() && preparing == 0)) {
+                        worker_done = true;
+                        prep_gen++;
+                        cv_prep.notify_all();
+                        return;
+                    }
+                    cv_work.wait(lk);
                 }
             }
-            if (!slots.empty() && inflight.size() < 2) enqueue_chunk();
-            if (inflight.size() >= 2 || (slots.empty() && !inflight.empty())) {
+            if (!slots.empty() && inflight.size() < kMaxInflight) enqueue_chunk();
+            if (inflight.size() >= kMaxInflight || (slots.empty() && !inflight.empty())) {
                 complete_chunk(inflight.front());
                 inflight.pop_front();
             }
-            poll_kubo(false);
-            if (slots.empty() && inflight.empty() && !kjobs.empty()) {
-                // only sigma_xy jobs on the device: wait for news, briefly
-                std::unique_lock<std::mutex> lk(mu);
-                cv.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || !queue.empty(); });
-            }
         }
     } catch (const Error& e) {
-        std::lock_guard<std::mutex> lk(mu);
-        fatal = e.msg;
+        fail_all(e.msg);
     } catch (const std::exception& e) {
-        std::lock_guard<std::mutex> lk(mu);
-        fatal = e.what();
+        fail_all(e.what());
     }
-    // on a fatal error every request still pending fails
-    std::vector<Request*> pending;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        if (fatal.empty()) return;
-        for (auto& kv : reqs)
-            if (kv.second->state == TBK_REQ_QUEUED || kv.second->state == TBK_REQ_RUNNING) pending.push_back(kv.second.get());
-        queue.clear();
+    std::lock_guard<std::mutex> lk(mu);
+    worker_done = true;
+    prep_gen++;
+    cv_prep.notify_all();
+}
+
+// Preparation thread: admission, uploads and kets of new slots, the whole
+// enqueue of sigma_xy jobs and their completion, and every device free.
+void tbk_server::run_prep() {
+    int64_t seen_gen = -1;
+    try {
+        set_device(ctx);
+        for (;;) {
+            std::vector<std::pair<Request*, int32_t>> adm;
+            std::vector<Slot*> dead;
+            std::vector<void*> dead_raw;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                // act on news (submission, completion, freed memory, stop);
+                // poll the sigma_xy jobs every 200 us while any is running;
+                // once stopping with the worker gone, run to the exit test
+                auto news = [&] { return prep_gen != seen_gen || (stop && worker_done); };
+                if (kjobs.empty())
+                    cv_prep.wait(lk, news);
+                else
+                    cv_prep.wait_for(lk, std::chrono::microseconds(200), news);
+                seen_gen = prep_gen;
+                if (!fatal.empty()) break;
+                dead.swap(trash);
+                dead_raw.swap(trash_raw);
+                adm = pick_admissions();
+                if (stop && worker_done && queue.empty() && kjobs.empty() && adm.empty() && dead.empty() &&
+                    dead_raw.empty())
+                    break;
+            }
+            for (Slot* x : dead) delete x;
+            for (void* p : dead_raw) (void)hipFree(p);
+            for (auto& pr : adm) {
+                Request* r = pr.first;
+                try {
+                    if (r->kind == TBK_REQ_DOS) {
+                        Slot* x = start_dos(r, pr.second);
+                        std::lock_guard<std::mutex> lk(mu);
+                        ready.push_back(x);
+                        preparing--;
+                        cv_work.notify_all();
+                    } else {
+                        start_kubo(r, pr.second);
+                        std::lock_guard<std::mutex> lk(mu);
+                        preparing--;
+                        cv_work.notify_all();
+                    }
+                } catch (const Error& e) {
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        (r->kind == TBK_REQ_DOS ? free_pin : free_kpin).push_back(pr.second);
+                        preparing--;
+                        cv_work.notify_all();
+                    }
+                    finish(r, e.code, e.msg);
+                }
+            }
+            poll_kubo(false);
+        }
+    } catch (const Error& e) {
+        fail_all(e.msg);
+    } catch (const std::exception& e) {
+        fail_all(e.what());
     }
-    for (Request* r : pending) {
-        std::lock_guard<std::mutex> lk(mu);
-        r->state = TBK_REQ_FAILED;
-        r->error_code = TBK_E_DEVICE;
-        r->error = fatal;
-        r->t_done = now_s();
-        st.failed++;
-    }
-    done_cv.notify_all();
 }
 
 namespace {
@@ -672,7 +771,7 @@ int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server**
         tbk_server_config& c = s->cfg;
         if (c.max_slots <= 0) c.max_slots = 512;
         if (c.max_kubo <= 0) c.max_kubo = 1;
-        if (c.chunk_steps <= 0) c.chunk_steps = 32;
+        if (c.chunk_steps <= 0) c.chunk_steps = 64;
         if (c.max_moments <= 0) c.max_moments = 4096;
         if (c.max_kubo_moments <= 0) c.max_kubo_moments = 1024;
         if (c.max_wait_bypass_ms <= 0) c.max_wait_bypass_ms = 1000.0;
@@ -696,6 +795,7 @@ int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server**
         s->st.budget_bytes = s->budget;
         tbk_server* raw = s.get();
         s->worker = std::thread([raw] { raw->run(); });
+        s->prep = std::thread([raw] { raw->run_prep(); });
         *out = s.release();
     });
 }
@@ -751,7 +851,8 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
         s->reqs[r->ticket] = std::move(r);
         s->st.submitted++;
         s->st.queued = static_cast<int64_t>(s->queue.size());
-        s->cv.notify_all();
+        s->prep_gen++;
+        s->cv_prep.notify_all();
     });
 }
 
@@ -818,12 +919,18 @@ int tbk_server_destroy(tbk_server* s) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->stop = true;
-            s->cv.notify_all();
+            s->prep_gen++;
+            s->cv_prep.notify_all();
+            s->cv_work.notify_all();
         }
         if (s->worker.joinable()) s->worker.join();
+        if (s->prep.joinable()) s->prep.join();
         set_device(s->ctx);
         (void)hipDeviceSynchronize();
         for (Slot* x : s->slots) delete x;
+        for (Slot* x : s->ready) delete x;
+        for (Slot* x : s->trash) delete x;
+        for (void* p : s->trash_raw) (void)hipFree(p);
         for (KuboJob* x : s->kjobs) delete x;
         for (Chunk& c : s->inflight) {
             for (Slot* x : c.finishing) delete x;
