@@ -368,10 +368,10 @@ __global__ __launch_bounds__(kBlock) void cheb_batch_kernel(
 // launch per step.  k0 is the slot's step index at the chunk's first launch,
 // so the launch with offset s performs step k0 + s of every slot that has
 // not finished (k0 + s < steps).  Step 0 is the first step (T_1, moments 0
-// and 1).  A slot's ntiles row tiles are visited in the transposed chunk
-// order of period P (Q = ceil(ntiles / P) rows of the chunk grid, one tile
-// per chunk): visit v of the slot is launch tile visit0 + v, v < P*Q, and
-// row tile rt writes its partial sums to slots part0 + rt.
+// and 1).  Slot d owns the nwg workgroups wg0 .. wg0 + nwg - 1 of a launch
+// (wg0 and nwg multiples of 8, so its workgroup j runs on XCD j % 8) and
+// splits its row tiles over them exactly as cheb_step_kernel splits a
+// matrix: tile ranges dealt per XCD, visited in the slot's chunk order.
 struct SlotDesc {
     const int32_t* rp;
     const int32_t* ci;
@@ -381,20 +381,18 @@ struct SlotDesc {
     double* mu;      // n moments
     int64_t k0;
     int64_t steps;   // (n + 1) / 2
-    int64_t visit0;
-    int64_t part0;
+    int64_t wg0;
+    int32_t nwg;
     int32_t rows;
-    int32_t ntiles;
     int32_t n;
-    uint32_t P, Q;
     int32_t pad_;
+    TileOrder order;
 };
 
 template <typename VT, bool CH, int LPR, int CPL>
 __global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __restrict__ slots,
-                                                            const int32_t* __restrict__ tile_slot, int64_t ntiles,
-                                                            int32_t s, int32_t Rp, int32_t tile_rows, int32_t opts,
-                                                            int32_t xcd_split, double2* __restrict__ partial) {
+                                                            const int32_t* __restrict__ wg_slot, int32_t s,
+                                                            int32_t Rp, int32_t opts, double2* __restrict__ partial) {
     using V2 = typename Vec2<VT>::type;
     using H = typename HVal<VT, CH>::type;
     constexpr int RPW = kWave / LPR;
@@ -404,38 +402,56 @@ __global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __re
     const int sub = lane / LPR;
     const int q = lane % LPR;
 
-    int64_t first, end, stride;
-    tile_range(ntiles, xcd_split, first, end, stride);
-    for (int64_t t = first; t < end; t += stride) {
-        const SlotDesc& d = slots[tile_slot[t]];
-        const int64_t k = d.k0 + s;
-        if (k >= d.steps) continue;
-        const uint32_t v = static_cast<uint32_t>(t - d.visit0);
-        const uint32_t pc = v / d.Q, qr = v - pc * d.Q;
-        const int64_t lt = static_cast<int64_t>(qr) * d.P + pc;
-        if (lt >= d.ntiles) continue;
-        const V2* x = static_cast<const V2*>(d.blk[k % 3]);
-        const V2* z = static_cast<const V2*>(d.blk[(k + 2) % 3]);
-        V2* y = static_cast<V2*>(d.blk[(k + 1) % 3]);
-        const int32_t* rp = d.rp;
-        const int32_t* ci = d.ci;
-        const H* hv = static_cast<const H*>(d.hv);
-        const double ca = d.coef[k == 0 ? 0 : 2], cb = d.coef[k == 0 ? 1 : 3];
-        const VT alpha = static_cast<VT>(ca), beta = static_cast<VT>(cb);
-        const int32_t o = (cb != 0.0) ? (opts | kOptShift) : (opts & ~kOptShift);
-        double p0 = 0.0, p1 = 0.0;
-        for (int g = 0; g < tile_rows; g += TR) {
-            int r = static_cast<int>(lt * tile_rows + g) + wave * RPW + sub;
-            if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
-            if (r >= d.rows) continue;
-            if (k == 0)
-                step_row<VT, CH, LPR, CPL, kFirst>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
-            else
-                step_row<VT, CH, LPR, CPL, kRecDbl>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
+    const SlotDesc& d = slots[wg_slot[blockIdx.x]];
+    const int64_t k = d.k0 + s;
+    if (k >= d.steps) return;  // the whole workgroup, before any barrier
+    const V2* x = static_cast<const V2*>(d.blk[k % 3]);
+    const V2* z = static_cast<const V2*>(d.blk[(k + 2) % 3]);
+    V2* y = static_cast<V2*>(d.blk[(k + 1) % 3]);
+    const int32_t* rp = d.rp;
+    const int32_t* ci = d.ci;
+    const H* hv = static_cast<const H*>(d.hv);
+    const double ca = d.coef[k == 0 ? 0 : 2], cb = d.coef[k == 0 ? 1 : 3];
+    const VT alpha = static_cast<VT>(ca), beta = static_cast<VT>(cb);
+    const int32_t o = (cb != 0.0) ? (opts | kOptShift) : (opts & ~kOptShift);
+    const TileOrder order = d.order;
+    const int32_t rows = d.rows;
+
+    // this workgroup's tiles: as tile_range with xcd_split, within the slot
+    const int64_t j = static_cast<int64_t>(blockIdx.x) - d.wg0;
+    const int64_t ntiles = static_cast<int64_t>(order.P) * order.Q * order.tpc;
+    const int64_t xcd = j & 7, jj = j >> 3, per = d.nwg >> 3;
+    const int64_t first = (ntiles * xcd) / 8 + jj, end = (ntiles * (xcd + 1)) / 8;
+
+    double p0 = 0.0, p1 = 0.0;
+    for (int64_t t = first; t < end; t += per) {
+        uint32_t slab_unused;
+        const int64_t rt = visit_row_tile(order, static_cast<uint32_t>(t), slab_unused);
+        if (rt < 0) continue;
+        int r = static_cast<int>(rt * TR) + wave * RPW + sub;
+        if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
+        if (r >= rows) continue;
+        if (k == 0)
+            step_row<VT, CH, LPR, CPL, kFirst>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
+        else
+            step_row<VT, CH, LPR, CPL, kRecDbl>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
+    }
+
+    __shared__ double red[2][kWavesPerBlock];
+    p0 = wave_sum(p0);
+    p1 = wave_sum(p1);
+    if (lane == 0) {
+        red[0][wave] = p0;
+        red[1][wave] = p1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = red[0][0], s1 = red[1][0];
+        for (int w = 1; w < kWavesPerBlock; ++w) {
+            s0 = s0 + red[0][w];
+            s1 = s1 + red[1][w];
         }
-        p0 = wave_sum(p0);
-        p1 = wave_sum(p1);
-        if (lane == 0) partial[4 * (d.part0 + lt) + wave] = make_double2(p0, p1);
+        partial[blockIdx.x] = make_double2(s0, s1);
     }
 }
 

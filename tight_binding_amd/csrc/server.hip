@@ -42,40 +42,27 @@
 namespace {
 using namespace engine;
 
-constexpr int32_t kSlotTileRows = 256;
+constexpr int64_t kSlotChunkRows = 256;
 
-// Moments of step k0 + s of every slot (one workgroup per slot): its wave
-// partials summed in a fixed order (four interleaved accumulators per
-// thread, then a tree), then mu_0, mu_1 (step 0) or the doubling identities
-// mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
-constexpr int kMomThreads = 1024;
-__global__ __launch_bounds__(kMomThreads) void slots_moments_kernel(const tbk::SlotDesc* __restrict__ slots, int32_t s,
-                                                                    const double2* __restrict__ partial) {
+// Moments of step k0 + s of every slot (one workgroup per slot): the
+// slot's workgroup partials summed in a fixed order, then mu_0, mu_1 (step
+// 0) or the doubling identities mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
+__global__ __launch_bounds__(tbk::kBlock) void slots_moments_kernel(const tbk::SlotDesc* __restrict__ slots,
+                                                                    int32_t s, const double2* __restrict__ partial) {
     const tbk::SlotDesc& d = slots[blockIdx.x];
     const int64_t k = d.k0 + s;
     if (k >= d.steps) return;  // the whole workgroup
-    const double2* p = partial + 4 * d.part0;
-    const int64_t cnt = 4 * static_cast<int64_t>(d.ntiles);
-    double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
-    int64_t i = threadIdx.x;
-    for (; i + 3 * kMomThreads < cnt; i += 4 * kMomThreads) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const double2 v = p[i + u * kMomThreads];
-            a0[u] = a0[u] + v.x;
-            a1[u] = a1[u] + v.y;
-        }
+    const double2* p = partial + d.wg0;
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < d.nwg; i += tbk::kBlock) {
+        a0 = a0 + p[i].x;
+        a1 = a1 + p[i].y;
     }
-    for (int u = 0; i < cnt; i += kMomThreads, ++u) {
-        const double2 v = p[i];
-        a0[u] = a0[u] + v.x;
-        a1[u] = a1[u] + v.y;
-    }
-    __shared__ double r0[kMomThreads], r1[kMomThreads];
-    r0[threadIdx.x] = (a0[0] + a0[1]) + (a0[2] + a0[3]);
-    r1[threadIdx.x] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
+    __shared__ double r0[tbk::kBlock], r1[tbk::kBlock];
+    r0[threadIdx.x] = a0;
+    r1[threadIdx.x] = a1;
     __syncthreads();
-    for (int w = kMomThreads / 2; w >= 1; w >>= 1) {
+    for (int w = tbk::kBlock / 2; w >= 1; w >>= 1) {
         if (static_cast<int>(threadIdx.x) < w) {
             r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
             r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
@@ -92,6 +79,14 @@ __global__ __launch_bounds__(kMomThreads) void slots_moments_kernel(const tbk::S
             if (2 * k + 1 < d.n) m[2 * k + 1] = 2.0 * r1[0] - m[1];
         }
     }
+}
+
+// Complex values per lane of a class: 16-byte lanes (one complex128 or two
+// complex64) unless configured, within 64 lanes per row.
+int class_cpl(int cfg_cpl, int prec, int Rp) {
+    int cpl = cfg_cpl > 0 ? cfg_cpl : (prec == TBK_FP64 ? 1 : std::min(2, Rp));
+    if (Rp / cpl > tbk::kWave || (cpl == 4 && Rp < 8) || cpl > Rp) cpl = Rp > tbk::kWave ? Rp / tbk::kWave : 1;
+    return cpl;
 }
 
 double now_s() {
@@ -148,9 +143,9 @@ struct Slot {
     tbk_matrix* M = nullptr;
     DevBuf blk[3], mu;
     int64_t k = 0, steps = 0;
-    int32_t ntiles = 0;
-    uint32_t P = 1, Q = 1;  // transposed tile order (period P = bandwidth / tile rows)
-    int32_t pin = -1;       // row of the pinned moment staging
+    tbk::TileOrder order{};  // row tiles of one workgroup pass, chunks of 256 rows, transposed
+    int32_t nwg = 8;         // workgroups of the slot in a launch (multiple of 8)
+    int32_t pin = -1;        // row of the pinned moment staging
     ~Slot() {
         if (M) tbk_matrix_destroy(M);
     }
@@ -387,12 +382,26 @@ Slot* tbk_server::start_dos(Request* r, int32_t pin) {
     for (auto& b : s->blk) b.reserve(static_cast<size_t>(rows) * r->Rp * elem);
     s->mu.reserve(static_cast<size_t>(r->n) * 8);
     s->steps = (r->n + 1) / 2;
-    s->ntiles = static_cast<int32_t>((rows + kSlotTileRows - 1) / kSlotTileRows);
-    // visiting order: tiles a matrix bandwidth apart one after the other, so
-    // a gathered neighbour row is reused from L2 by the next tile
-    const int64_t period = std::max<int64_t>(1, (s->M->bandwidth95 + kSlotTileRows / 2) / kSlotTileRows);
-    s->P = static_cast<uint32_t>(std::min<int64_t>(period, s->ntiles));
-    s->Q = static_cast<uint32_t>((s->ntiles + s->P - 1) / s->P);
+    {
+        // the split of cheb_step_kernel: tiles of one workgroup pass, chunks of
+        // 256 rows transposed with period P = bandwidth / chunk rows, up to 4
+        // workgroups per CU
+        const int lpr = r->Rp / class_cpl(cfg.cpl, r->precision, r->Rp);
+        const int64_t TR = static_cast<int64_t>(tbk::kWavesPerBlock) * (tbk::kWave / lpr);
+        const int64_t row_tiles = (rows + TR - 1) / TR;
+        const int64_t tpc = std::max<int64_t>(1, kSlotChunkRows / TR);
+        const int64_t chunks = (row_tiles + tpc - 1) / tpc;
+        int64_t P = std::max<int64_t>(1, (s->M->bandwidth95 + (tpc * TR) / 2) / (tpc * TR));
+        P = std::min<int64_t>(P, chunks);
+        const int64_t Q = (chunks + P - 1) / P;
+        s->order.row_tiles = static_cast<uint32_t>(row_tiles);
+        s->order.tpc = static_cast<uint32_t>(tpc);
+        s->order.P = static_cast<uint32_t>(P);
+        s->order.Q = static_cast<uint32_t>(Q);
+        const int64_t visits = P * Q * tpc;
+        const int64_t cap = std::max<int64_t>(8, (static_cast<int64_t>(ctx->num_cu) * 4 / 8) * 8);
+        s->nwg = static_cast<int32_t>(std::min<int64_t>(cap, ((visits + 7) / 8) * 8));
+    }
     const size_t total = static_cast<size_t>(rows) * r->Rp;
     const unsigned grid = static_cast<unsigned>(std::min<size_t>((total + tbk::kBlock - 1) / tbk::kBlock, 65536));
     const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
@@ -503,69 +512,52 @@ void tbk_server::enqueue_chunk() {
         const ClassKey& key = kv.first;
         std::vector<Slot*>& v = kv.second;
         ClassState& cs = classes[key];
-        int64_t ntiles = 0, nvisits = 0;
-        for (Slot* s : v) {
-            ntiles += s->ntiles;
-            nvisits += static_cast<int64_t>(s->P) * s->Q;
-        }
+        int64_t nwg = 0;
+        for (Slot* x : v) nwg += x->nwg;
         tbk::SlotDesc* hd = stage(cs.stage_desc[c.parity], cs.cap_desc[c.parity], v.size());
-        int32_t* ht = stage(cs.stage_tiles[c.parity], cs.cap_tiles[c.parity], static_cast<size_t>(nvisits));
-        int64_t t0 = 0, v0 = 0;
+        int32_t* ht = stage(cs.stage_tiles[c.parity], cs.cap_tiles[c.parity], static_cast<size_t>(nwg));
+        int64_t w0 = 0;
         for (size_t i = 0; i < v.size(); ++i) {
-            Slot* s = v[i];
-            const Request* r = s->req;
+            Slot* x = v[i];
+            const Request* r = x->req;
             tbk::SlotDesc& d = hd[i];
             std::memset(&d, 0, sizeof(d));
-            d.rp = s->M->rp.as<const int32_t>();
-            d.ci = s->M->ci.as<const int32_t>();
-            d.hv = r->precision == TBK_FP64 ? s->M->val64.p : s->M->val32.p;
-            for (int q = 0; q < 3; ++q) d.blk[q] = s->blk[q].p;
+            d.rp = x->M->rp.as<const int32_t>();
+            d.ci = x->M->ci.as<const int32_t>();
+            d.hv = r->precision == TBK_FP64 ? x->M->val64.p : x->M->val32.p;
+            for (int q = 0; q < 3; ++q) d.blk[q] = x->blk[q].p;
             d.coef[0] = 1.0 / r->a;
             d.coef[1] = -r->b / r->a;
             d.coef[2] = 2.0 / r->a;
             d.coef[3] = -2.0 * r->b / r->a;
-            d.mu = s->mu.as<double>();
-            d.k0 = s->k;
-            d.steps = s->steps;
-            d.visit0 = v0;
-            d.part0 = t0;
+            d.mu = x->mu.as<double>();
+            d.k0 = x->k;
+            d.steps = x->steps;
+            d.wg0 = w0;
+            d.nwg = x->nwg;
             d.rows = r->H.hdr.rows;
-            d.ntiles = s->ntiles;
             d.n = r->n;
-            d.P = s->P;
-            d.Q = s->Q;
-            const int64_t nv = static_cast<int64_t>(s->P) * s->Q;
-            for (int64_t t = 0; t < nv; ++t) ht[v0 + t] = static_cast<int32_t>(i);
-            t0 += s->ntiles;
-            v0 += nv;
+            d.order = x->order;
+            for (int32_t t = 0; t < x->nwg; ++t) ht[w0 + t] = static_cast<int32_t>(i);
+            w0 += x->nwg;
         }
         grow(cs.desc, v.size() * sizeof(tbk::SlotDesc));
-        grow(cs.tile_slot, static_cast<size_t>(nvisits) * sizeof(int32_t));
-        grow(cs.partial, static_cast<size_t>(4 * ntiles) * sizeof(double2));
+        grow(cs.tile_slot, static_cast<size_t>(nwg) * sizeof(int32_t));
+        grow(cs.partial, static_cast<size_t>(nwg) * sizeof(double2));
         HIPC(hipMemcpyAsync(cs.desc.p, hd, v.size() * sizeof(tbk::SlotDesc), hipMemcpyHostToDevice, st_));
-        HIPC(hipMemcpyAsync(cs.tile_slot.p, ht, static_cast<size_t>(nvisits) * sizeof(int32_t), hipMemcpyHostToDevice,
+        HIPC(hipMemcpyAsync(cs.tile_slot.p, ht, static_cast<size_t>(nwg) * sizeof(int32_t), hipMemcpyHostToDevice,
                             st_));
         tbk::SlotArgs A{};
         A.slots = cs.desc.as<const tbk::SlotDesc>();
-        A.tile_slot = cs.tile_slot.as<const int32_t>();
-        A.ntiles = nvisits;
+        A.wg_slot = cs.tile_slot.as<const int32_t>();
         A.Rp = key.Rp;
-        A.tile_rows = kSlotTileRows;
         A.opts = ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0;
-        int64_t nb = std::min<int64_t>(nvisits, static_cast<int64_t>(ctx->num_cu) * 4);
-        A.xcd_split = 0;
-        if (nb >= 64) {
-            nb = (nb / 8) * 8;
-            A.xcd_split = 1;
-        }
         A.partial = cs.partial.as<double2>();
-        // default: 16-byte lanes (one complex128 or two complex64 per lane)
-        int cpl = cfg.cpl > 0 ? cfg.cpl : (key.prec == TBK_FP64 ? 1 : std::min(2, key.Rp));
-        if (key.Rp / cpl > tbk::kWave || (cpl == 4 && key.Rp < 8)) cpl = key.Rp > tbk::kWave ? key.Rp / tbk::kWave : 1;
+        const int cpl = class_cpl(cfg.cpl, key.prec, key.Rp);
+        const dim3 grid(static_cast<unsigned>(nwg));
         for (int32_t s = 0; s < S; ++s) {
             A.s = s;
             bool ok;
-            const dim3 grid(static_cast<unsigned>(std::max<int64_t>(nb, 1)));
             if (key.prec == TBK_FP64)
                 ok = key.cplx ? tbk::launch_slots_f64_cplx(key.Rp, cpl, grid, st_, A)
                               : tbk::launch_slots_f64_real(key.Rp, cpl, grid, st_, A);
@@ -574,7 +566,7 @@ void tbk_server::enqueue_chunk() {
                               : tbk::launch_slots_f32_real(key.Rp, cpl, grid, st_, A);
             if (!ok) raise(TBK_E_UNSUPPORTED, "no slot kernel for this width");
             HIPC(hipGetLastError());
-            hipLaunchKernelGGL(slots_moments_kernel, dim3(static_cast<unsigned>(v.size())), dim3(kMomThreads), 0, st_,
+            hipLaunchKernelGGL(slots_moments_kernel, dim3(static_cast<unsigned>(v.size())), dim3(tbk::kBlock), 0, st_,
                                cs.desc.as<const tbk::SlotDesc>(), s, cs.partial.as<const double2>());
             HIPC(hipGetLastError());
         }

@@ -98,16 +98,14 @@ bool launch_batch(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const Ba
     return false;
 }
 
-// Arguments of cheb_slots_kernel (one step of every slot of a class).
+// Arguments of cheb_slots_kernel (one step of every slot of a class; the
+// grid is the total of the slots' workgroups).
 struct SlotArgs {
     const SlotDesc* slots;
-    const int32_t* tile_slot;
-    int64_t ntiles;
+    const int32_t* wg_slot;
     int32_t s;
     int32_t Rp;
-    int32_t tile_rows;
     int32_t opts;  // kOptStream
-    int32_t xcd_split;
     double2* partial;
 };
 
@@ -121,7 +119,7 @@ bool launch_slots(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a)
 #define TBK_SLOT_CASE(RP, C)                                                                                 \
     if (Rp == RP && cpl == C) {                                                                              \
         hipLaunchKernelGGL((cheb_slots_kernel<VT, CH, RP / C, C>), grid, dim3(kBlock), 0, st, a.slots,       \
-                           a.tile_slot, a.ntiles, a.s, a.Rp, a.tile_rows, a.opts, a.xcd_split, a.partial);   \
+                           a.wg_slot, a.s, a.Rp, a.opts, a.partial);                                         \
         return true;                                                                                         \
     }
     TBK_SLOT_CASE(1, 1)
