@@ -173,8 +173,9 @@ def batch_leg(ctx, nprob, R=8):
         runs.append(st)
     st = min(runs, key=lambda d: d["kernel_ms"])
     gbps = st["bytes_total"] / (st["kernel_ms"] * 1e-3) / 1e9
-    log("batch: kernel %.1f ms, wall %.1f ms, %.1f GB/s, shape cpl=%d bpc=%d xcd=%d (create %.1f s, tune %.3f ms/step)"
-        % (st["kernel_ms"], st["wall_ms"], gbps, st["cpl"], st["blocks_per_cu"], st["xcd_split"], create_s, tune_ms))
+    log("batch: kernel %.1f ms, wall %.1f ms, %.1f GB/s, shape cpl=%d bpc=%d, %d workgroups (create %.1f s, "
+        "tune %.3f ms/step)" % (st["kernel_ms"], st["wall_ms"], gbps, st["cpl"], st["blocks_per_cu"], st["workgroups"],
+                                create_s, tune_ms))
     B.close()
 
     # plain per-solve path: one moments call per problem, matrices resident
@@ -205,8 +206,7 @@ def batch_leg(ctx, nprob, R=8):
         "batched_kernel_ms": st["kernel_ms"], "batched_wall_ms": st["wall_ms"],
         "batched_moments_per_s": moments / (st["wall_ms"] * 1e-3),
         "batched_GBps": gbps, "batched_frac_hbm": gbps / HBM_PEAK_GBS,
-        "launch_shape": {"cpl": st["cpl"], "blocks_per_cu": st["blocks_per_cu"], "xcd_split": st["xcd_split"],
-                         "tile_rows": st["tile_rows"]},
+        "launch_shape": {"cpl": st["cpl"], "blocks_per_cu": st["blocks_per_cu"], "workgroups": st["workgroups"]},
         "graphs_per_run": st["graphs"], "step_launches": st["launches"],
         "per_solve_kernel_ms": k_ms, "per_solve_wall_ms": 1e3 * w_s,
         "per_solve_moments_per_s": moments / w_s,

@@ -241,11 +241,10 @@ typedef struct tbk_batch_stats {
     int64_t launches;      /* batched step launches (the largest step count in the batch) */
     int64_t graphs;        /* chunk graphs launched (0 without TBK_BATCH_GRAPHS) */
     double bytes_total;    /* algorithmic HBM bytes of every problem's steps */
-    int64_t rows_total;    /* rows of the block-diagonal matrix, tile padding included */
-    int32_t tile_rows;     /* rows per tile (problems start on tile boundaries) */
+    int64_t rows_total;    /* rows of all problems */
+    int64_t workgroups;    /* workgroups of a step launch with every problem active */
     int32_t cpl;           /* complex values per lane (lanes per row = padded R / cpl) */
-    int32_t blocks_per_cu; /* workgroups per CU of a step launch */
-    int32_t xcd_split;     /* 1: tile ranges dealt per XCD */
+    int32_t blocks_per_cu; /* workgroups per CU a large problem gets */
 } tbk_batch_stats;
 
 /*
@@ -266,8 +265,10 @@ int tbk_batch_create(tbk_context* ctx, int32_t n_problems, const tbk_sparse* mat
 /* run the whole batch from fresh kets; mu_out: the problems' moments
    concatenated in problem order (sum of n_moments doubles) */
 int tbk_batch_run(tbk_batch* batch, int32_t flags, double* mu_out, tbk_batch_stats* stats);
-/* launch shape: complex values per lane (1, 2 or 4), workgroups per CU, XCD split */
-int tbk_batch_set_shape(tbk_batch* batch, int32_t cpl, int32_t blocks_per_cu, int32_t xcd_split);
+/* launch shape: complex values per lane (1, 2 or 4) and the workgroups per CU
+   a large problem gets (each problem splits its rows over its own workgroups,
+   dealt per XCD, as the single-problem step does) */
+int tbk_batch_set_shape(tbk_batch* batch, int32_t cpl, int32_t blocks_per_cu);
 /* time `steps` steps per launch shape, keep the fastest, return its time per step */
 int tbk_batch_autotune(tbk_batch* batch, int32_t steps, double* best_ms_per_step);
 int tbk_batch_destroy(tbk_batch* batch);

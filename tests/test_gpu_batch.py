@@ -128,8 +128,8 @@ def test_launch_shapes_agree(gpu_ctx, R):
     for cpl in (1, 2, 4):
         if Rp % cpl or Rp // cpl > 64 or (cpl == 4 and Rp < 8) or (cpl == 2 and Rp < 2):
             continue
-        for bpc, xcd in ((2, 0), (8, 1)):
-            B.set_shape(cpl, bpc, xcd)
+        for bpc in (2, 8):
+            B.set_shape(cpl, bpc)
             mus, st = B.run(graphs=True)
             assert st["cpl"] == cpl and st["blocks_per_cu"] == bpc
             for m, r in zip(mus, ref):
@@ -169,6 +169,6 @@ def test_batch_errors(gpu_ctx):
     assert e.value.code == N.TBK_E_INVALID
     B = kpm.Batch(gpu_ctx, [H], [a], [0], [8], 8)
     with pytest.raises(N.NativeError) as e:
-        B.set_shape(3, 4, 1)
+        B.set_shape(3, 4)
     assert e.value.code == N.TBK_E_UNSUPPORTED
     B.close()

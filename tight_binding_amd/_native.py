@@ -104,10 +104,9 @@ class tbk_batch_stats(C.Structure):
         ("graphs", C.c_int64),
         ("bytes_total", C.c_double),
         ("rows_total", C.c_int64),
-        ("tile_rows", C.c_int32),
+        ("workgroups", C.c_int64),
         ("cpl", C.c_int32),
         ("blocks_per_cu", C.c_int32),
-        ("xcd_split", C.c_int32),
     ]
 
     def as_dict(self):
@@ -224,7 +223,7 @@ KPM_SIGNATURES = {
     "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),
     "tbk_batch_create": ([P, I32, C.POINTER(tbk_sparse), DP, DP, IP, I32, I32, C.POINTER(U64), PP], I32),
     "tbk_batch_run": ([P, I32, DP, C.POINTER(tbk_batch_stats)], I32),
-    "tbk_batch_set_shape": ([P, I32, I32, I32], I32),
+    "tbk_batch_set_shape": ([P, I32, I32], I32),
     "tbk_batch_autotune": ([P, I32, DP], I32),
     "tbk_batch_destroy": ([P], I32),
     "tbk_server_create": ([I32, C.POINTER(tbk_server_config), PP], I32),

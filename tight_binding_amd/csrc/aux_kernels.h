@@ -1,5 +1,5 @@
 // Small device kernels of the engine other than the recursion step:
-// moment-partial reduction, random-phase block generation, block copy and
+// moment-partial reductions, random-phase block generation, block copy and
 // fp64 -> fp32 value conversion.  The kernels have internal linkage: each
 // translation unit that includes this header launches its own copy.
 #pragma once
@@ -97,6 +97,43 @@ __global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double* __restric
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kBlock)
         out[i] = static_cast<float>(in[i]);
+}
+
+// Moments of step k0 + s of every slot (one workgroup per slot): the
+// slot's workgroup partials summed in a fixed order, then mu_0, mu_1 (step
+// 0) or the doubling identities mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
+__global__ __launch_bounds__(kBlock) void slots_moments_kernel(const SlotDesc* __restrict__ slots,
+                                                                    int32_t s, const double2* __restrict__ partial) {
+    const SlotDesc& d = slots[blockIdx.x];
+    const int64_t k = d.k0 + s;
+    if (k >= d.steps) return;  // the whole workgroup
+    const double2* p = partial + d.wg0;
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < d.nwg; i += kBlock) {
+        a0 = a0 + p[i].x;
+        a1 = a1 + p[i].y;
+    }
+    __shared__ double r0[kBlock], r1[kBlock];
+    r0[threadIdx.x] = a0;
+    r1[threadIdx.x] = a1;
+    __syncthreads();
+    for (int w = kBlock / 2; w >= 1; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
+            r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* m = d.mu;
+        if (k == 0) {
+            m[0] = r0[0];
+            m[1] = r1[0];
+        } else {
+            if (2 * k < d.n) m[2 * k] = 2.0 * r0[0] - m[0];
+            if (2 * k + 1 < d.n) m[2 * k + 1] = 2.0 * r1[0] - m[1];
+        }
+    }
 }
 
 }  // namespace

@@ -1,20 +1,19 @@
 // Batched ragged moment problems: many independent Hamiltonians of different
-// sizes and moment counts run as one block-diagonal recursion, one batched
-// step launch per Chebyshev step for all problems still active.
+// sizes and moment counts run together, one launch of cheb_slots_kernel per
+// Chebyshev step for all problems still active.
 //
-// Layout.  Problem p occupies rows [row_begin_p, row_begin_p + rows_p) of the
-// concatenated CSR (column indices offset by row_begin_p), padded with empty
-// rows to a multiple of kBatchTileRows, so every tile of kBatchTileRows rows
-// belongs to one problem.  Problems are laid out in order of decreasing step
-// count: the problems still active at step k are a prefix of the layout, and
-// a step launch covers the tiles of that prefix only.  Vector blocks are
-// row-major [row][Rp] over the concatenated rows (padding rows stay zero).
-//
-// Moments.  Each wave writes its partial sums of a tile to its own slot;
-// batch_moments_kernel adds a problem's slots in a fixed order and applies
-// the doubling identities, so a problem's moments do not depend on the other
-// problems in the batch.  Its vectors T_n are those of a single-problem run
-// (same per-row arithmetic and rescaling).
+// Layout.  The problems' CSR arrays (local column indices) and their vector
+// blocks (row-major [row][Rp]) are concatenated; each problem is a slot
+// whose descriptor points at its own pieces.  Problems are laid out in order
+// of decreasing step count, so the problems still active at step k are a
+// prefix, and so are their workgroups: a step launch covers exactly the
+// workgroups of that prefix.  Each problem owns a fixed set of workgroups
+// that splits its rows as the single-problem step kernel does (tiles dealt
+// per XCD in the matrix's chunk order), and each workgroup writes one
+// partial; slots_moments_kernel adds a problem's partials in a fixed order
+// and applies the doubling identities.  A problem's moments therefore do
+// not depend on the other problems in the batch, and its vectors T_n are
+// those of a single-problem run.
 //
 // Graphs.  With TBK_BATCH_GRAPHS the launches of each chunk of 64 steps are
 // captured once into a hipGraph (no memset nodes) and replayed on later runs.
@@ -34,48 +33,7 @@
 namespace {
 using namespace engine;
 
-constexpr int32_t kBatchTileRows = 256;  // multiple of every workgroup pass (4 waves x 64/LPR rows)
 constexpr int64_t kChunkSteps = 64;
-
-// One workgroup per active problem: s = sum of its wave slots in a fixed
-// order; step 0 gives mu_0, mu_1; step k >= 1 gives mu_2k = 2 s.x - mu_0 and
-// mu_2k+1 = 2 s.y - mu_1 (indices below the problem's moment count).
-__global__ __launch_bounds__(tbk::kBlock) void batch_moments_kernel(const double2* __restrict__ partial,
-                                                                    const int64_t* __restrict__ tile_begin,
-                                                                    const int64_t* __restrict__ tile_end,
-                                                                    const int64_t* __restrict__ mu_off,
-                                                                    const int32_t* __restrict__ n_mom, int64_t step,
-                                                                    double* __restrict__ mu) {
-    const int pos = blockIdx.x;
-    const int64_t s0 = 4 * tile_begin[pos], s1 = 4 * tile_end[pos];
-    double a0 = 0.0, a1 = 0.0;
-    for (int64_t i = s0 + threadIdx.x; i < s1; i += tbk::kBlock) {
-        a0 = a0 + partial[i].x;
-        a1 = a1 + partial[i].y;
-    }
-    __shared__ double r0[tbk::kBlock], r1[tbk::kBlock];
-    r0[threadIdx.x] = a0;
-    r1[threadIdx.x] = a1;
-    __syncthreads();
-    for (int w = tbk::kBlock / 2; w >= 1; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w) {
-            r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
-            r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double* m = mu + mu_off[pos];
-        const int64_t n = n_mom[pos];
-        if (step == 0) {
-            m[0] = r0[0];
-            m[1] = r1[0];
-        } else {
-            if (2 * step < n) m[2 * step] = 2.0 * r0[0] - m[0];
-            if (2 * step + 1 < n) m[2 * step + 1] = 2.0 * r1[0] - m[1];
-        }
-    }
-}
 
 }  // namespace
 
@@ -83,19 +41,21 @@ struct tbk_batch {
     tbk_context* ctx = nullptr;
     int32_t np = 0, R = 0, Rp = 0, precision = TBK_FP64;
     bool cplx = false;
-    int64_t rows_total = 0, ntiles = 0, max_steps = 0;
+    int64_t rows_total = 0, max_steps = 0;
     // per layout position (problems sorted by decreasing steps)
-    std::vector<int32_t> prob;          // problem index of each position
-    std::vector<int64_t> row_begin, rows, tile_begin, tile_end, steps, nnz;
+    std::vector<int32_t> prob;  // problem index of each position
+    std::vector<int64_t> row_begin, rows, rp_begin, nz_begin, nnz, steps, bandwidth, mu_off;
     std::vector<uint64_t> seed;
     std::vector<int32_t> nmom;
-    std::vector<int64_t> mu_off;        // offset of the problem's moments in mu_out (problem order)
+    std::vector<double> coef;     // per position: (1/a, -b/a, 2/a, -2b/a)
+    std::vector<int64_t> wg_end;  // cumulative workgroups of positions 0..i
     int64_t mu_total = 0;
-    DevBuf rp, ci, val, tile_prob, coef, d_tb, d_te, d_off, d_n, mu, partial, buf[3];
-    // launch shape
-    int32_t cpl = 1, bpc = 4, xcd = 1;
+    DevBuf rp, ci, val, mu, partial, desc, wg_slot, buf[3];
+    // launch shape: complex values per lane (0: 16-byte lanes), workgroups per CU
+    int32_t cpl = 0, bpc = 4;
     std::vector<hipGraphExec_t> graphs;  // per chunk, captured on first use
     size_t elem() const { return precision == TBK_FP64 ? 16 : 8; }
+    int32_t cpl_used() const { return slot_cpl(cpl, precision, Rp); }
 
     void drop_graphs() {
         for (hipGraphExec_t g : graphs)
@@ -115,46 +75,63 @@ struct tbk_batch {
         return lo;
     }
 
-    int64_t grid_blocks(int64_t nt) const {
-        int64_t nb = std::min<int64_t>(nt, static_cast<int64_t>(ctx->num_cu) * bpc);
-        if (xcd && nb >= 8) nb = (nb / 8) * 8;
-        return std::max<int64_t>(nb, 1);
+    // descriptors and workgroup map for the current launch shape
+    void build_descriptors(hipStream_t st) {
+        const int lpr = Rp / cpl_used();
+        std::vector<tbk::SlotDesc> d(static_cast<size_t>(np));
+        std::vector<int32_t> map;
+        wg_end.assign(static_cast<size_t>(np), 0);
+        const size_t hv_elem = (precision == TBK_FP64 ? 8 : 4) * (cplx ? 2 : 1);
+        int64_t w0 = 0;
+        for (int32_t i = 0; i < np; ++i) {
+            tbk::SlotDesc& s = d[i];
+            std::memset(&s, 0, sizeof(s));
+            s.rp = rp.as<const int32_t>() + rp_begin[i];
+            s.ci = ci.as<const int32_t>() + nz_begin[i];
+            s.hv = static_cast<const char*>(val.p) + static_cast<size_t>(nz_begin[i]) * hv_elem;
+            for (int q = 0; q < 3; ++q)
+                s.blk[q] = static_cast<char*>(buf[q].p) + static_cast<size_t>(row_begin[i]) * Rp * elem();
+            for (int c = 0; c < 4; ++c) s.coef[c] = coef[4 * static_cast<size_t>(i) + c];
+            s.mu = mu.as<double>() + mu_off[i];
+            s.k0 = 0;
+            s.steps = steps[i];
+            s.rows = static_cast<int32_t>(rows[i]);
+            s.n = nmom[i];
+            slot_split(ctx->num_cu, bpc, lpr, rows[i], bandwidth[i], s.order, s.nwg);
+            s.wg0 = w0;
+            map.insert(map.end(), static_cast<size_t>(s.nwg), i);
+            w0 += s.nwg;
+            wg_end[i] = w0;
+        }
+        desc.reserve(d.size() * sizeof(tbk::SlotDesc));
+        wg_slot.reserve(map.size() * sizeof(int32_t));
+        partial.reserve(static_cast<size_t>(w0) * sizeof(double2));
+        HIPC(hipMemcpyAsync(desc.p, d.data(), d.size() * sizeof(tbk::SlotDesc), hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(wg_slot.p, map.data(), map.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPC(hipStreamSynchronize(st));
     }
 
     void launch_step(int64_t k, hipStream_t st) {
         const int32_t na = active(k);
         if (na == 0) return;
-        tbk::BatchArgs A{};
-        A.rp = rp.as<const int32_t>();
-        A.ci = ci.as<const int32_t>();
-        A.hv = val.p;
-        A.x = buf[k % 3].p;
-        A.z = buf[(k + 2) % 3].p;
-        A.y = buf[(k + 1) % 3].p;
-        A.rows = static_cast<int32_t>(rows_total);
+        tbk::SlotArgs A{};
+        A.slots = desc.as<const tbk::SlotDesc>();
+        A.wg_slot = wg_slot.as<const int32_t>();
+        A.s = static_cast<int32_t>(k);
         A.Rp = Rp;
-        A.tile_prob = tile_prob.as<const int32_t>();
-        A.coef = coef.as<const double>();
-        A.tile_rows = kBatchTileRows;
-        A.opts = (ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0);
-        A.ntiles = tile_end[na - 1];
-        const int64_t nb = grid_blocks(A.ntiles);
-        A.xcd_split = (xcd && nb >= 8) ? 1 : 0;
+        A.opts = ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0;
         A.partial = partial.as<double2>();
-        const int mode = k == 0 ? tbk::kFirst : tbk::kRecDbl;
+        const dim3 grid(static_cast<unsigned>(wg_end[na - 1]));
+        const int c = cpl_used();
         bool ok;
-        const dim3 grid(static_cast<unsigned>(nb));
         if (precision == TBK_FP64)
-            ok = cplx ? tbk::launch_batch_f64_cplx(Rp, cpl, mode, grid, st, A)
-                      : tbk::launch_batch_f64_real(Rp, cpl, mode, grid, st, A);
+            ok = cplx ? tbk::launch_slots_f64_cplx(Rp, c, grid, st, A) : tbk::launch_slots_f64_real(Rp, c, grid, st, A);
         else
-            ok = cplx ? tbk::launch_batch_f32_cplx(Rp, cpl, mode, grid, st, A)
-                      : tbk::launch_batch_f32_real(Rp, cpl, mode, grid, st, A);
+            ok = cplx ? tbk::launch_slots_f32_cplx(Rp, c, grid, st, A) : tbk::launch_slots_f32_real(Rp, c, grid, st, A);
         if (!ok) raise(TBK_E_UNSUPPORTED, "no batched step kernel for this width");
         HIPC(hipGetLastError());
-        hipLaunchKernelGGL(batch_moments_kernel, dim3(static_cast<unsigned>(na)), dim3(tbk::kBlock), 0, st,
-                           static_cast<const double2*>(partial.p), d_tb.as<const int64_t>(), d_te.as<const int64_t>(),
-                           d_off.as<const int64_t>(), d_n.as<const int32_t>(), k, mu.as<double>());
+        hipLaunchKernelGGL(tbk::slots_moments_kernel, dim3(static_cast<unsigned>(na)), dim3(tbk::kBlock), 0, st,
+                           desc.as<const tbk::SlotDesc>(), static_cast<int32_t>(k), partial.as<const double2>());
         HIPC(hipGetLastError());
     }
 
@@ -185,6 +162,11 @@ struct tbk_batch {
             t += (mat + 2.0 * blk) + static_cast<double>(steps[i] - 1) * (mat + 3.0 * blk);
         }
         return t;
+    }
+
+    int64_t device_bytes() const {
+        return static_cast<int64_t>(rp.bytes + ci.bytes + val.bytes + mu.bytes + partial.bytes + desc.bytes +
+                                    wg_slot.bytes + buf[0].bytes + buf[1].bytes + buf[2].bytes);
     }
 };
 
@@ -221,7 +203,6 @@ void batch_create(tbk_context* ctx, int32_t np, const tbk_sparse* mats, const do
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         return (n_moments[x] + 1) / 2 > (n_moments[y] + 1) / 2;
     });
-    // moment offsets in problem order
     std::vector<int64_t> off(np);
     int64_t acc = 0;
     for (int32_t i = 0; i < np; ++i) {
@@ -230,51 +211,39 @@ void batch_create(tbk_context* ctx, int32_t np, const tbk_sparse* mats, const do
     }
     B->mu_total = acc;
 
-    // concatenated, tile-padded CSR
-    std::vector<int32_t> rp{0}, ci;
+    // concatenated CSR with local indices
+    std::vector<int32_t> rp, ci;
     std::vector<double> val;
-    std::vector<int32_t> tile_prob;
-    std::vector<double> coef;
-    const int vw = cplx ? 2 : 1;
-    int64_t row = 0, tile = 0;
+    int64_t row = 0;
     for (int32_t pos = 0; pos < np; ++pos) {
         const int32_t p = order[pos];
         std::vector<int32_t> prp, pci;
         std::vector<double> pval;
         build_csr(&mats[p], prp, pci, pval);
         const int64_t n = mats[p].rows;
-        const int64_t tiles = (n + kBatchTileRows - 1) / kBatchTileRows;
-        if (row + tiles * kBatchTileRows >= (int64_t(1) << 31) ||
-            static_cast<int64_t>(ci.size()) + static_cast<int64_t>(pci.size()) >= (int64_t(1) << 31))
-            raise(TBK_E_UNSUPPORTED, "batch exceeds int32 row or entry indexing");
-        const int32_t base = static_cast<int32_t>(ci.size());
-        for (int64_t i = 1; i <= n; ++i) rp.push_back(base + prp[i]);
-        for (int64_t i = n; i < tiles * kBatchTileRows; ++i) rp.push_back(rp.back());
-        for (int32_t c : pci) ci.push_back(c + static_cast<int32_t>(row));
-        val.insert(val.end(), pval.begin(), pval.end());
-        for (int64_t t = 0; t < tiles; ++t) tile_prob.push_back(pos);
-        coef.push_back(1.0 / a[p]);
-        coef.push_back(-b[p] / a[p]);
-        coef.push_back(2.0 / a[p]);
-        coef.push_back(-2.0 * b[p] / a[p]);
         B->prob.push_back(p);
         B->row_begin.push_back(row);
         B->rows.push_back(n);
-        B->tile_begin.push_back(tile);
-        B->tile_end.push_back(tile + tiles);
-        B->steps.push_back((n_moments[p] + 1) / 2);
+        B->rp_begin.push_back(static_cast<int64_t>(rp.size()));
+        B->nz_begin.push_back(static_cast<int64_t>(ci.size()));
         B->nnz.push_back(static_cast<int64_t>(pci.size()));
+        B->bandwidth.push_back(bandwidth_percentile(prp, pci, 0.95));
+        B->steps.push_back((n_moments[p] + 1) / 2);
         B->seed.push_back(seeds[p]);
         B->nmom.push_back(n_moments[p]);
         B->mu_off.push_back(off[p]);
-        row += tiles * kBatchTileRows;
-        tile += tiles;
+        B->coef.push_back(1.0 / a[p]);
+        B->coef.push_back(-b[p] / a[p]);
+        B->coef.push_back(2.0 / a[p]);
+        B->coef.push_back(-2.0 * b[p] / a[p]);
+        rp.insert(rp.end(), prp.begin(), prp.end());
+        ci.insert(ci.end(), pci.begin(), pci.end());
+        val.insert(val.end(), pval.begin(), pval.end());
+        row += n;
     }
     B->rows_total = row;
-    B->ntiles = tile;
     B->max_steps = B->steps[0];
 
-    // device arrays
     auto up = [&](DevBuf& d, const void* src, size_t bytes) {
         d.reserve(std::max<size_t>(bytes, 16));
         if (bytes) HIPC(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -289,24 +258,12 @@ void batch_create(tbk_context* ctx, int32_t np, const tbk_sparse* mats, const do
         for (size_t i = 0; i < val.size(); ++i) val32[i] = static_cast<float>(val[i]);
         up(B->val, val32.data(), val32.size() * sizeof(float));
     }
-    (void)vw;
-    up(B->tile_prob, tile_prob.data(), tile_prob.size() * sizeof(int32_t));
-    up(B->coef, coef.data(), coef.size() * sizeof(double));
-    up(B->d_tb, B->tile_begin.data(), B->tile_begin.size() * sizeof(int64_t));
-    up(B->d_te, B->tile_end.data(), B->tile_end.size() * sizeof(int64_t));
-    up(B->d_off, B->mu_off.data(), B->mu_off.size() * sizeof(int64_t));
-    up(B->d_n, B->nmom.data(), B->nmom.size() * sizeof(int32_t));
     B->mu.reserve(static_cast<size_t>(B->mu_total) * sizeof(double));
-    B->partial.reserve(static_cast<size_t>(4 * B->ntiles) * sizeof(double2));
     const size_t blk = static_cast<size_t>(B->rows_total) * Rp * B->elem();
-    for (auto& d : B->buf) {
-        d.reserve(blk);
-        HIPC(hipMemsetAsync(d.p, 0, blk, ctx->stream));  // padding rows and columns stay zero
-    }
+    for (auto& d : B->buf) d.reserve(blk);
     HIPC(hipStreamSynchronize(ctx->stream));
-    int64_t held = static_cast<int64_t>(B->rp.bytes + B->ci.bytes + B->val.bytes + 3 * blk + B->partial.bytes +
-                                        B->mu.bytes);
-    ctx->held_bytes += held;
+    B->build_descriptors(ctx->stream);
+    ctx->held_bytes += B->device_bytes();
     *out = B.release();
 }
 
@@ -364,10 +321,9 @@ void batch_run(tbk_batch* B, int32_t flags, double* mu_out, tbk_batch_stats* sta
         stats->graphs = graphs_launched;
         stats->bytes_total = B->bytes_total();
         stats->rows_total = B->rows_total;
-        stats->tile_rows = kBatchTileRows;
-        stats->cpl = B->cpl;
+        stats->workgroups = B->wg_end.back();
+        stats->cpl = B->cpl_used();
         stats->blocks_per_cu = B->bpc;
-        stats->xcd_split = B->xcd;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 }
@@ -380,8 +336,17 @@ bool shape_valid(int Rp, int cpl) {
     return !((cpl == 4 && Rp < 8) || (cpl == 1 && Rp > 64) || (cpl == 2 && Rp > 128));
 }
 
-// Time `steps` recursion steps over all tiles for each launch shape
-// (complex values per lane, workgroups per CU, XCD split); keep the best.
+void set_shape(tbk_batch* B, int32_t cpl, int32_t bpc) {
+    const int64_t before = B->device_bytes();
+    B->cpl = cpl;
+    B->bpc = bpc;
+    B->drop_graphs();
+    B->build_descriptors(B->ctx->stream);
+    B->ctx->held_bytes += B->device_bytes() - before;
+}
+
+// Time `steps` recursion steps with every problem active for each launch
+// shape (complex values per lane, workgroups per CU); keep the best.
 void batch_autotune(tbk_batch* B, int32_t steps, double* best_ms) {
     if (!B) raise(TBK_E_INVALID, "null argument");
     if (steps < 1) raise(TBK_E_INVALID, "steps must be positive");
@@ -391,9 +356,8 @@ void batch_autotune(tbk_batch* B, int32_t steps, double* best_ms) {
     ensure_events(ctx, 2);
     B->init_kets(st);
     B->launch_step(0, st);  // T_1, so that every buffer holds finite values
-    const int32_t keep_cpl = B->cpl, keep_bpc = B->bpc, keep_xcd = B->xcd;
     double best = -1.0;
-    int32_t bc = keep_cpl, bb = keep_bpc, bx = keep_xcd;
+    int32_t bc = B->cpl_used(), bb = B->bpc;
     // steps k = 1, 2, 3, 1, 2, ... continue the recursion in the three
     // rotating blocks (step k reads blocks k%3, (k+2)%3 and writes (k+1)%3)
     // with every problem active; the moments they write are overwritten by
@@ -403,31 +367,23 @@ void batch_autotune(tbk_batch* B, int32_t steps, double* best_ms) {
     for (int cpl : {1, 2, 4}) {
         if (!shape_valid(B->Rp, cpl)) continue;
         for (int bpc : {2, 4, 8}) {
-            for (int xcd : {1, 0}) {
-                B->cpl = cpl;
-                B->bpc = bpc;
-                B->xcd = xcd;
-                B->launch_step(next_k(), st);  // warm
-                HIPC(hipEventRecord(ctx->events[0], st));
-                for (int32_t s = 0; s < steps; ++s) B->launch_step(next_k(), st);
-                HIPC(hipEventRecord(ctx->events[1], st));
-                HIPC(hipEventSynchronize(ctx->events[1]));
-                float ms = 0.f;
-                HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
-                const double per = ms / steps;
-                if (best < 0.0 || per < best) {
-                    best = per;
-                    bc = cpl;
-                    bb = bpc;
-                    bx = xcd;
-                }
+            set_shape(B, cpl, bpc);
+            B->launch_step(next_k(), st);  // warm
+            HIPC(hipEventRecord(ctx->events[0], st));
+            for (int32_t s = 0; s < steps; ++s) B->launch_step(next_k(), st);
+            HIPC(hipEventRecord(ctx->events[1], st));
+            HIPC(hipEventSynchronize(ctx->events[1]));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+            const double per = ms / steps;
+            if (best < 0.0 || per < best) {
+                best = per;
+                bc = cpl;
+                bb = bpc;
             }
         }
     }
-    B->cpl = bc;
-    B->bpc = bb;
-    B->xcd = bx;
-    if (bc != keep_cpl || bb != keep_bpc || bx != keep_xcd) B->drop_graphs();
+    set_shape(B, bc, bb);
     if (best_ms) *best_ms = best;
 }
 
@@ -444,15 +400,13 @@ int tbk_batch_run(tbk_batch* batch, int32_t flags, double* mu_out, tbk_batch_sta
     return guarded([&] { batch_run(batch, flags, mu_out, stats); });
 }
 
-int tbk_batch_set_shape(tbk_batch* batch, int32_t cpl, int32_t blocks_per_cu, int32_t xcd_split) {
+int tbk_batch_set_shape(tbk_batch* batch, int32_t cpl, int32_t blocks_per_cu) {
     return guarded([&] {
         if (!batch) raise(TBK_E_INVALID, "null argument");
         if (!shape_valid(batch->Rp, cpl)) raise(TBK_E_UNSUPPORTED, "no batched kernel for this complex-per-lane count");
         if (blocks_per_cu < 1 || blocks_per_cu > 32) raise(TBK_E_INVALID, "blocks_per_cu must be in 1..32");
-        batch->cpl = cpl;
-        batch->bpc = blocks_per_cu;
-        batch->xcd = xcd_split ? 1 : 0;
-        batch->drop_graphs();
+        set_device(batch->ctx);
+        set_shape(batch, cpl, blocks_per_cu);
     });
 }
 
@@ -466,9 +420,7 @@ int tbk_batch_destroy(tbk_batch* batch) {
         tbk_context* ctx = batch->ctx;
         set_device(ctx);
         HIPC(hipStreamSynchronize(ctx->stream));
-        const size_t blk = static_cast<size_t>(batch->rows_total) * batch->Rp * batch->elem();
-        ctx->held_bytes -= static_cast<int64_t>(batch->rp.bytes + batch->ci.bytes + batch->val.bytes + 3 * blk +
-                                                batch->partial.bytes + batch->mu.bytes);
+        ctx->held_bytes -= batch->device_bytes();
         delete batch;
     });
 }

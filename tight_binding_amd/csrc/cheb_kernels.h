@@ -314,54 +314,6 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     }
 }
 
-// Batched step over many independent problems held as one block-diagonal
-// matrix (global row and column numbering) in which every problem starts on
-// a tile boundary; a tile is tile_rows consecutive rows (a multiple of the
-// rows one workgroup pass covers).  Tile t belongs to problem tile_prob[t],
-// whose step coefficients are coef[4p .. 4p+3] = (1/a, -b/a, 2/a, -2b/a)
-// (first step, recursion); the shift term is applied where b != 0, as in a
-// single-problem step.  Each wave writes its partial sums of a tile to
-// partial[4t + wave]: a problem's moments depend on its own tiles only, not
-// on the other problems of the batch or on which workgroup ran which tile.
-// Only tiles t < ntiles are visited (the problems still active).
-template <typename VT, bool CH, int LPR, int CPL, int MODE>
-__global__ __launch_bounds__(kBlock) void cheb_batch_kernel(
-    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-    const typename HVal<VT, CH>::type* __restrict__ hv, const typename Vec2<VT>::type* __restrict__ x,
-    const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y, int32_t rows, int32_t Rp,
-    const int32_t* __restrict__ tile_prob, const double* __restrict__ coef, int32_t tile_rows, int32_t opts,
-    int64_t ntiles, int32_t xcd_split, double2* __restrict__ partial) {
-    constexpr int RPW = kWave / LPR;
-    constexpr int TR = kWavesPerBlock * RPW;
-    static_assert(LPR >= 1 && LPR <= kWave && (kWave % LPR) == 0, "bad lanes per row");
-    static_assert(MODE != kRec, "batched steps use the bra = ket modes");
-
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int sub = lane / LPR;
-    const int q = lane % LPR;
-
-    int64_t first, end, stride;
-    tile_range(ntiles, xcd_split, first, end, stride);
-    for (int64_t t = first; t < end; t += stride) {
-        const int p = tile_prob[t];
-        const double ca = coef[4 * p + (MODE == kFirst ? 0 : 2)];
-        const double cb = coef[4 * p + (MODE == kFirst ? 1 : 3)];
-        const VT alpha = static_cast<VT>(ca), beta = static_cast<VT>(cb);
-        const int32_t o = (cb != 0.0) ? (opts | kOptShift) : (opts & ~kOptShift);
-        double p0 = 0.0, p1 = 0.0;
-        const int64_t base = t * tile_rows;
-        for (int g = 0; g < tile_rows; g += TR) {
-            int r = static_cast<int>(base + g) + wave * RPW + sub;
-            if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
-            if (r < rows) step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
-        }
-        p0 = wave_sum(p0);
-        p1 = wave_sum(p1);
-        if (lane == 0) partial[4 * t + wave] = make_double2(p0, p1);
-    }
-}
-
 // Resident problems of a serving loop ("slots"), each with its own CSR
 // arrays and three rotating vector blocks (T_k in blk[k % 3]); the slots
 // of one class (precision, value type, padded width) advance together, one

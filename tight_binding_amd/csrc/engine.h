@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <new>
 #include <stdexcept>
@@ -163,6 +164,38 @@ void matrix_fp32(tbk_matrix* m);
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial);
+
+// Work split of one problem in a cheb_slots_kernel launch, the split that
+// cheb_step_kernel uses for a single matrix: row tiles of one workgroup pass
+// (lpr lanes per row), chunks of 256 rows transposed with period P =
+// bandwidth / chunk rows, and nwg workgroups (a multiple of 8, at most
+// blocks_per_cu per CU) dealing the tiles per XCD.  Depends only on the
+// problem, so its partial sums are the same in any batch.
+inline void slot_split(int num_cu, int blocks_per_cu, int lpr, int64_t rows, int64_t bandwidth95,
+                       tbk::TileOrder& order, int32_t& nwg) {
+    const int64_t TR = static_cast<int64_t>(tbk::kWavesPerBlock) * (tbk::kWave / lpr);
+    const int64_t row_tiles = (rows + TR - 1) / TR;
+    const int64_t tpc = std::max<int64_t>(1, 256 / TR);
+    const int64_t chunks = (row_tiles + tpc - 1) / tpc;
+    int64_t P = std::max<int64_t>(1, (bandwidth95 + (tpc * TR) / 2) / (tpc * TR));
+    P = std::min<int64_t>(P, chunks);
+    const int64_t Q = (chunks + P - 1) / P;
+    order.row_tiles = static_cast<uint32_t>(row_tiles);
+    order.tpc = static_cast<uint32_t>(tpc);
+    order.P = static_cast<uint32_t>(P);
+    order.Q = static_cast<uint32_t>(Q);
+    const int64_t visits = P * Q * tpc;
+    const int64_t cap = std::max<int64_t>(8, (static_cast<int64_t>(num_cu) * blocks_per_cu / 8) * 8);
+    nwg = static_cast<int32_t>(std::min<int64_t>(cap, ((visits + 7) / 8) * 8));
+}
+
+// Complex values per lane of a slot class: 16-byte lanes (one complex128 or
+// two complex64) unless requested, within 64 lanes per row.
+inline int slot_cpl(int requested, int prec, int Rp) {
+    int cpl = requested > 0 ? requested : (prec == TBK_FP64 ? 1 : std::min(2, Rp));
+    if (Rp / cpl > tbk::kWave || (cpl == 4 && Rp < 8) || cpl > Rp) cpl = Rp > tbk::kWave ? Rp / tbk::kWave : 1;
+    return cpl;
+}
 
 // Kubo-Bastin contraction shape and launches (gamma.hip kernels).
 struct ContractionShape {

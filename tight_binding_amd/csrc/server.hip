@@ -42,52 +42,6 @@
 namespace {
 using namespace engine;
 
-constexpr int64_t kSlotChunkRows = 256;
-
-// Moments of step k0 + s of every slot (one workgroup per slot): the
-// slot's workgroup partials summed in a fixed order, then mu_0, mu_1 (step
-// 0) or the doubling identities mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
-__global__ __launch_bounds__(tbk::kBlock) void slots_moments_kernel(const tbk::SlotDesc* __restrict__ slots,
-                                                                    int32_t s, const double2* __restrict__ partial) {
-    const tbk::SlotDesc& d = slots[blockIdx.x];
-    const int64_t k = d.k0 + s;
-    if (k >= d.steps) return;  // the whole workgroup
-    const double2* p = partial + d.wg0;
-    double a0 = 0.0, a1 = 0.0;
-    for (int i = threadIdx.x; i < d.nwg; i += tbk::kBlock) {
-        a0 = a0 + p[i].x;
-        a1 = a1 + p[i].y;
-    }
-    __shared__ double r0[tbk::kBlock], r1[tbk::kBlock];
-    r0[threadIdx.x] = a0;
-    r1[threadIdx.x] = a1;
-    __syncthreads();
-    for (int w = tbk::kBlock / 2; w >= 1; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w) {
-            r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
-            r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double* m = d.mu;
-        if (k == 0) {
-            m[0] = r0[0];
-            m[1] = r1[0];
-        } else {
-            if (2 * k < d.n) m[2 * k] = 2.0 * r0[0] - m[0];
-            if (2 * k + 1 < d.n) m[2 * k + 1] = 2.0 * r1[0] - m[1];
-        }
-    }
-}
-
-// Complex values per lane of a class: 16-byte lanes (one complex128 or two
-// complex64) unless configured, within 64 lanes per row.
-int class_cpl(int cfg_cpl, int prec, int Rp) {
-    int cpl = cfg_cpl > 0 ? cfg_cpl : (prec == TBK_FP64 ? 1 : std::min(2, Rp));
-    if (Rp / cpl > tbk::kWave || (cpl == 4 && Rp < 8) || cpl > Rp) cpl = Rp > tbk::kWave ? Rp / tbk::kWave : 1;
-    return cpl;
-}
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -382,26 +336,8 @@ Slot* tbk_server::start_dos(Request* r, int32_t pin) {
     for (auto& b : s->blk) b.reserve(static_cast<size_t>(rows) * r->Rp * elem);
     s->mu.reserve(static_cast<size_t>(r->n) * 8);
     s->steps = (r->n + 1) / 2;
-    {
-        // the split of cheb_step_kernel: tiles of one workgroup pass, chunks of
-        // 256 rows transposed with period P = bandwidth / chunk rows, up to 4
-        // workgroups per CU
-        const int lpr = r->Rp / class_cpl(cfg.cpl, r->precision, r->Rp);
-        const int64_t TR = static_cast<int64_t>(tbk::kWavesPerBlock) * (tbk::kWave / lpr);
-        const int64_t row_tiles = (rows + TR - 1) / TR;
-        const int64_t tpc = std::max<int64_t>(1, kSlotChunkRows / TR);
-        const int64_t chunks = (row_tiles + tpc - 1) / tpc;
-        int64_t P = std::max<int64_t>(1, (s->M->bandwidth95 + (tpc * TR) / 2) / (tpc * TR));
-        P = std::min<int64_t>(P, chunks);
-        const int64_t Q = (chunks + P - 1) / P;
-        s->order.row_tiles = static_cast<uint32_t>(row_tiles);
-        s->order.tpc = static_cast<uint32_t>(tpc);
-        s->order.P = static_cast<uint32_t>(P);
-        s->order.Q = static_cast<uint32_t>(Q);
-        const int64_t visits = P * Q * tpc;
-        const int64_t cap = std::max<int64_t>(8, (static_cast<int64_t>(ctx->num_cu) * 4 / 8) * 8);
-        s->nwg = static_cast<int32_t>(std::min<int64_t>(cap, ((visits + 7) / 8) * 8));
-    }
+    slot_split(ctx->num_cu, 4, r->Rp / slot_cpl(cfg.cpl, r->precision, r->Rp), rows, s->M->bandwidth95, s->order,
+               s->nwg);
     const size_t total = static_cast<size_t>(rows) * r->Rp;
     const unsigned grid = static_cast<unsigned>(std::min<size_t>((total + tbk::kBlock - 1) / tbk::kBlock, 65536));
     const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
@@ -553,7 +489,7 @@ void tbk_server::enqueue_chunk() {
         A.Rp = key.Rp;
         A.opts = ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0;
         A.partial = cs.partial.as<double2>();
-        const int cpl = class_cpl(cfg.cpl, key.prec, key.Rp);
+        const int cpl = slot_cpl(cfg.cpl, key.prec, key.Rp);
         const dim3 grid(static_cast<unsigned>(nwg));
         for (int32_t s = 0; s < S; ++s) {
             A.s = s;
@@ -566,7 +502,7 @@ void tbk_server::enqueue_chunk() {
                               : tbk::launch_slots_f32_real(key.Rp, cpl, grid, st_, A);
             if (!ok) raise(TBK_E_UNSUPPORTED, "no slot kernel for this width");
             HIPC(hipGetLastError());
-            hipLaunchKernelGGL(slots_moments_kernel, dim3(static_cast<unsigned>(v.size())), dim3(tbk::kBlock), 0, st_,
+            hipLaunchKernelGGL(tbk::slots_moments_kernel, dim3(static_cast<unsigned>(v.size())), dim3(tbk::kBlock), 0, st_,
                                cs.desc.as<const tbk::SlotDesc>(), s, cs.partial.as<const double2>());
             HIPC(hipGetLastError());
         }

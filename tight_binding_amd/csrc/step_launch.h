@@ -27,77 +27,6 @@ struct StepArgs {
     double2* partial;
 };
 
-// Arguments of cheb_batch_kernel (one batched step over the active tiles).
-struct BatchArgs {
-    const int32_t* rp;
-    const int32_t* ci;
-    const void* hv;
-    const void* x;
-    const void* z;
-    void* y;
-    int32_t rows, Rp;
-    const int32_t* tile_prob;
-    const double* coef;
-    int32_t tile_rows;
-    int32_t opts;  // kOptStream
-    int64_t ntiles;
-    int32_t xcd_split;
-    double2* partial;
-};
-
-// Launch one batched step with LPR = Rp / cpl lanes per row; returns false if
-// no kernel exists for (Rp, cpl).
-bool launch_batch_f64_real(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const BatchArgs& a);
-bool launch_batch_f64_cplx(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const BatchArgs& a);
-bool launch_batch_f32_real(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const BatchArgs& a);
-bool launch_batch_f32_cplx(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const BatchArgs& a);
-
-template <typename VT, bool CH, int LPR, int CPL>
-bool launch_batch_mode(int mode, dim3 grid, hipStream_t st, const BatchArgs& a) {
-    using V2 = typename Vec2<VT>::type;
-    using H = typename HVal<VT, CH>::type;
-    if (mode == kFirst)
-        hipLaunchKernelGGL((cheb_batch_kernel<VT, CH, LPR, CPL, kFirst>), grid, dim3(kBlock), 0, st, a.rp, a.ci,
-                           static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
-                           static_cast<V2*>(a.y), a.rows, a.Rp, a.tile_prob, a.coef, a.tile_rows, a.opts, a.ntiles,
-                           a.xcd_split, a.partial);
-    else if (mode == kRecDbl)
-        hipLaunchKernelGGL((cheb_batch_kernel<VT, CH, LPR, CPL, kRecDbl>), grid, dim3(kBlock), 0, st, a.rp, a.ci,
-                           static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
-                           static_cast<V2*>(a.y), a.rows, a.Rp, a.tile_prob, a.coef, a.tile_rows, a.opts, a.ntiles,
-                           a.xcd_split, a.partial);
-    else
-        return false;
-    return true;
-}
-
-template <typename VT, bool CH>
-bool launch_batch(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const BatchArgs& a) {
-#define TBK_BATCH_CASE(RP, C)                                                 \
-    if (Rp == RP && cpl == C) return launch_batch_mode<VT, CH, RP / C, C>(mode, grid, st, a);
-    TBK_BATCH_CASE(1, 1)
-    TBK_BATCH_CASE(2, 1)
-    TBK_BATCH_CASE(2, 2)
-    TBK_BATCH_CASE(4, 1)
-    TBK_BATCH_CASE(4, 2)
-    TBK_BATCH_CASE(8, 1)
-    TBK_BATCH_CASE(8, 2)
-    TBK_BATCH_CASE(8, 4)
-    TBK_BATCH_CASE(16, 1)
-    TBK_BATCH_CASE(16, 2)
-    TBK_BATCH_CASE(16, 4)
-    TBK_BATCH_CASE(32, 1)
-    TBK_BATCH_CASE(32, 2)
-    TBK_BATCH_CASE(32, 4)
-    TBK_BATCH_CASE(64, 1)
-    TBK_BATCH_CASE(64, 2)
-    TBK_BATCH_CASE(128, 2)
-    TBK_BATCH_CASE(128, 4)
-    TBK_BATCH_CASE(256, 4)
-#undef TBK_BATCH_CASE
-    return false;
-}
-
 // Arguments of cheb_slots_kernel (one step of every slot of a class; the
 // grid is the total of the slots' workgroups).
 struct SlotArgs {

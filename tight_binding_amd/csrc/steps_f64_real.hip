@@ -5,9 +5,6 @@ namespace tbk {
 bool launch_f64_real(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
     return launch_width<double, false>(width, kch, mode, grid, st, a);
 }
-bool launch_batch_f64_real(int Rp, int cpl, int mode, dim3 grid, hipStream_t st, const BatchArgs& a) {
-    return launch_batch<double, false>(Rp, cpl, mode, grid, st, a);
-}
 bool launch_slots_f64_real(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a) {
     return launch_slots<double, false>(Rp, cpl, grid, st, a);
 }

@@ -263,8 +263,8 @@ class Batch:
         N.check_kpm(N.kpm().tbk_batch_run(self._h, N.TBK_BATCH_GRAPHS if graphs else 0, _dp(mu), C.byref(st)))
         return [mu[self.offsets[i]:self.offsets[i + 1]] for i in range(self.n_moments.size)], st.as_dict()
 
-    def set_shape(self, cpl=1, blocks_per_cu=4, xcd_split=1):
-        N.check_kpm(N.kpm().tbk_batch_set_shape(self._h, int(cpl), int(blocks_per_cu), int(xcd_split)))
+    def set_shape(self, cpl=1, blocks_per_cu=4):
+        N.check_kpm(N.kpm().tbk_batch_set_shape(self._h, int(cpl), int(blocks_per_cu)))
 
     def autotune(self, steps=8) -> float:
         ms = C.c_double()
