@@ -289,7 +289,7 @@ def make_requests(n, dos_pool, kubo_pool, qps, seed=2030):
     return out
 
 
-def serve_leg(qps, n_requests, pool_dos=24, pool_kubo=6):
+def serve_leg(qps, n_requests, pool_dos=24, pool_kubo=6, saturation_only=False):
     """Served mixed workload: continuous batching in the request server, with
     latency percentiles at a fixed Poisson arrival rate, the makespan of the
     same requests all submitted at once, and the plain per-solve GPU path on
@@ -344,6 +344,10 @@ def serve_leg(qps, n_requests, pool_dos=24, pool_kubo=6):
         srv.close()
         return results, np.asarray(lat), span, st
 
+    if saturation_only:
+        res_s, lat_s, span_s, st_s = serve(False)
+        log("serve: saturation run (all submitted at once) %.2f s" % span_s)
+        return {"served_makespan_s": span_s, "requests": len(reqs), "chunks": st_s["chunks"]}
     res_p, lat_p, span_p, st_p = serve(True)
     log("serve: Poisson run %.1f s, latency p50 %.1f ms p99 %.1f ms" % (span_p, np.percentile(lat_p, 50),
                                                                        np.percentile(lat_p, 99)))
@@ -476,7 +480,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=3, help="time this many plain block copies")
-    ap.add_argument("--extra", action="store_true", help="also time fp32 and one-moment-per-step variants")
+    ap.add_argument("--no-extra", action="store_true", help="skip the fp32 and one-moment-per-step variants")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch shape")
     ap.add_argument("--kubo-repeats", type=int, default=3, help="sigma_xy solves per precision (0: skip)")
     ap.add_argument("--kubo-cells", type=int, default=512)
@@ -488,6 +492,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=96, help="requests of the served leg (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--serve-only", action="store_true", help="run only the served leg")
+    ap.add_argument("--serve-saturation-only", action="store_true",
+                    help="served leg: only the all-at-once run (no Poisson run, no per-solve comparison)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -498,7 +504,8 @@ def main():
     from tight_binding_amd import kpm
 
     if args.serve_only:
-        print(json.dumps({"serve": serve_leg(args.serve_qps, max(1, args.serve_requests))}), flush=True)
+        print(json.dumps({"serve": serve_leg(args.serve_qps, max(1, args.serve_requests),
+                                             saturation_only=args.serve_saturation_only)}), flush=True)
         return
 
     if args.batch_only:
@@ -562,7 +569,7 @@ def main():
         raise SystemExit("bench: moments fail the mu_0 = R, |mu_n| <= mu_0 check")
 
     extra = {}
-    if args.extra:
+    if not args.no_extra:
         mu_p, stp, _ = ctx.moments(M, K, args.steps + 1, a=a, plain=True, time_launches=True)
         extra["plain_fp64"] = {
             "steps_per_s": stp["steps"] / (stp["kernel_ms"] * 1e-3),
