@@ -442,6 +442,100 @@ def serve_leg(qps, n_requests, pool_dos=24, pool_kubo=6, saturation_only=False):
     return out
 
 
+def c1_leg(ctx, repeats=5):
+    """The reference's CPU-runnable case end to end: square 100x100 periodic,
+    t = 1, on-site energies uniform in [-0.5, 0.5], a = 4.6, R = 16 random-phase
+    vectors, N = 512 moments, Jackson kernel, DoS at 1024 energies.  Device
+    path (resident H, device kets, fused recursion, host reconstruction)
+    against the oracle's restatement of the reference CPU loop (one moment per
+    step, single thread) on the same kets."""
+    import oracle
+    from oracle import lattice_oracle as O
+    from tight_binding_amd import kpm
+    from tight_binding_amd import lattice as Lm
+    from tight_binding_amd import spectral as S
+
+    cell = Lm.UnitCell.model("square")
+    cell.add_hopping_nn("A", 1, 1.0)
+    lat = Lm.PartialLattice(cell, (100, 100))
+    lat.set_disorder(1.0, 2031)
+    H = lat.get_Hamiltonian_periodic_extend_bare_sparse((1, 1))
+    a, N, R = 4.6, 512, 16
+    e = S.linspace(-1.0, 1.0, 1024)
+    g = S.jackson(N)
+    M = ctx.matrix(H)
+    K = ctx.random_block(H.n, R, seed=2032)
+    ctx.moments(M, K, N, a=a)  # warm
+    times, kms = [], []
+    for _ in range(repeats):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        mu, st, _ = ctx.moments(M, K, N, a=a)
+        dos = S.density_of_states(mu, g, e, R)
+        times.append(time.perf_counter() - t0)
+        kms.append(st["kernel_ms"])
+    kets = K.download()
+    Hs = O.rescale(O.CSC(H.n, H.colptr, H.rowidx, H.values), a, 0.0)
+    t0 = time.perf_counter()
+    mu_cpu = oracle.cheb_braket(Hs, N, kets, kets)
+    dos_cpu = -S.green_imag(mu_cpu, g, e) / (R * math.pi)
+    cpu_s = time.perf_counter() - t0
+    err = float(np.max(np.abs(mu - mu_cpu)) / np.max(np.abs(mu_cpu)))
+    K.close()
+    M.close()
+    out = {
+        "workload": "DoS, square 100x100 periodic (10^4 sites), W=1, a=4.6, R=16, N=512, Jackson, 1024 energies",
+        "gpu_solve_ms_median": 1e3 * float(np.median(times)),
+        "gpu_kernel_ms_median": float(np.median(kms)),
+        "cpu_oracle_solve_s": cpu_s, "cpu_cores": 1,
+        "speedup_vs_cpu": cpu_s / float(np.median(times)),
+        "mu_max_rel_diff_vs_cpu": err,
+        "dos_max_abs_diff_vs_cpu": float(np.max(np.abs(dos - dos_cpu))),
+    }
+    log("c1: %s" % json.dumps(out))
+    return out
+
+
+def kubo_cpu_sample(ctx, cells=64, N=128, R=32):
+    """The sigma_xy pipeline at a reduced size on the CPU (oracle stacks in C,
+    one thread; Gamma as one numpy product; zeta by the separable host form)
+    beside the device path at the same size: the baseline for the C3 solve,
+    whose CPU cost grows as N^2 x sites x R for the contraction."""
+    import oracle
+    from oracle import spectral_oracle as SO
+    from tight_binding_amd import kpm
+    from tight_binding_amd import kubo
+    from tight_binding_amd import spectral as S
+
+    H, X, Y, emax = build_c3(cells)
+    solver = kubo.KuboSolver(ctx, H, X, Y, -1.02 * emax, 1.02 * emax, volume=float(H.n), lam=6.0)
+    ef = np.linspace(-4.0, 4.0, 201)
+    solver.solve(N, R=R, seed=9, fermi_energies=ef)
+    t0 = time.perf_counter()
+    res = solver.solve(N, R=R, seed=9, fermi_energies=ef)
+    gpu_s = time.perf_counter() - t0
+    K = ctx.random_block(H.n, R, seed=9)
+    kets = K.download()
+    K.close()
+    g = S.lorentz(N, 6.0)
+    t0 = time.perf_counter()
+    TL = oracle.cheb_on_bases(solver.Hs, N, kets)
+    w = oracle.cheb_on_bases(solver.vy, 2, kets)[1]
+    TR = oracle.cheb_on_bases(solver.Hs, N, w)
+    aR = np.stack([oracle.cheb_on_bases(solver.vx, 2, TR[m])[1] for m in range(N)])
+    G = SO.kubo_gamma(TL * g[:, None, None], aR * g[:, None, None])
+    sig = S.kubo_sigma(G, (ef - solver.b) / solver.a, solver.volume)[0]
+    cpu_s = time.perf_counter() - t0
+    solver.close()
+    out = {
+        "workload": "sigma_xy, square %dx%d open, Landau flux 1/64, W=2, R=%d, N=%d" % (cells, cells, R, N),
+        "gpu_solve_s": gpu_s, "cpu_solve_s": cpu_s, "cpu_cores": "1 (stacks) + numpy BLAS threads (product)",
+        "sigma_max_rel_diff": float(np.max(np.abs(sig - res["sigma"])) / np.max(np.abs(res["sigma"]))),
+    }
+    log("kubo cpu sample: %s" % json.dumps(out))
+    return out
+
+
 def cpu_baseline(H, R, a, seconds):
     """The oracle (C restatement of the reference's CPU recursion, single
     thread) timed on a bounded number of steps of the same workload."""
@@ -596,12 +690,14 @@ def main():
         cpu = cpu_baseline(H, args.R, a, args.cpu_seconds)
         log("cpu: %.4f steps/s" % cpu["value"])
 
+    M.close()
+    K.close()
+    c1_out = c1_leg(ctx) if not args.no_cpu else None
     kubo_out = None
     if args.kubo_repeats > 0:
-        # the DoS blocks are released first: the two Kubo stacks take 137 GB
-        M.close()
-        K.close()
         kubo_out = kubo_leg(ctx, args.kubo_cells, args.kubo_N, args.kubo_R, args.kubo_repeats)
+        if not args.no_cpu:
+            kubo_out["cpu_sample"] = kubo_cpu_sample(ctx)
     batch_out = None
     if args.batch_problems > 0:
         ctx.release_work()
@@ -649,6 +745,8 @@ def main():
     }
     if extra:
         out["variants"] = extra
+    if c1_out:
+        out["c1_dos"] = c1_out
     if kubo_out:
         out["kubo_sigma_xy"] = kubo_out
     if batch_out:

@@ -38,9 +38,9 @@ class KuboSolver:
         self.a, self.b = (hi - lo) / 2, (hi + lo) / 2
         self.ctx, self.volume, self.lam = ctx, float(volume), float(lam)
         self.Hs = L.rescale(H, self.a, self.b)
-        vx, vy = velocity_operators(self.Hs, X, Y)
+        self.vx, self.vy = velocity_operators(self.Hs, X, Y)  # host CSC copies
         self.n = self.Hs.n
-        self.MH, self.MX, self.MY = ctx.matrix(self.Hs), ctx.matrix(vx), ctx.matrix(vy)
+        self.MH, self.MX, self.MY = ctx.matrix(self.Hs), ctx.matrix(self.vx), ctx.matrix(self.vy)
 
     def close(self):
         for m in (self.MH, self.MX, self.MY):
