@@ -471,14 +471,16 @@ def c1_leg(ctx, repeats=5):
         ctx.synchronize()
         t0 = time.perf_counter()
         mu, st, _ = ctx.moments(M, K, N, a=a)
-        dos = S.density_of_states(mu, g, e, R)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            dos = S.density_of_states(mu, g, e, R)
         times.append(time.perf_counter() - t0)
         kms.append(st["kernel_ms"])
     kets = K.download()
     Hs = O.rescale(O.CSC(H.n, H.colptr, H.rowidx, H.values), a, 0.0)
     t0 = time.perf_counter()
     mu_cpu = oracle.cheb_braket(Hs, N, kets, kets)
-    dos_cpu = -S.green_imag(mu_cpu, g, e) / (R * math.pi)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dos_cpu = -S.green_imag(mu_cpu, g, e) / (R * math.pi)
     cpu_s = time.perf_counter() - t0
     err = float(np.max(np.abs(mu - mu_cpu)) / np.max(np.abs(mu_cpu)))
     K.close()
@@ -490,7 +492,10 @@ def c1_leg(ctx, repeats=5):
         "cpu_oracle_solve_s": cpu_s, "cpu_cores": 1,
         "speedup_vs_cpu": cpu_s / float(np.median(times)),
         "mu_max_rel_diff_vs_cpu": err,
-        "dos_max_abs_diff_vs_cpu": float(np.max(np.abs(dos - dos_cpu))),
+        # both DoS diverge at e = +-1 (the 1/sqrt(1-e^2) factor): compare the
+        # finite values and require the same non-finite positions
+        "dos_max_abs_diff_vs_cpu": float(np.max(np.abs(dos - dos_cpu)[np.isfinite(dos_cpu) & np.isfinite(dos)])),
+        "dos_nonfinite_positions_agree": bool(np.array_equal(np.isfinite(dos), np.isfinite(dos_cpu))),
     }
     log("c1: %s" % json.dumps(out))
     return out
@@ -547,8 +552,8 @@ def cpu_baseline(H, R, a, seconds):
     # one timed probe step decides how many fit the time budget
     t0 = time.perf_counter()
     oracle.cheb_braket(Hs, 3, kets, kets)  # T1 + one recursion step
-    probe = time.perf_counter() - t0
-    steps = max(1, min(64, int(seconds / max(probe, 1e-3)) - 1))
+    per_step = (time.perf_counter() - t0) / 2
+    steps = max(1, min(64, int(seconds / max(per_step, 1e-3)) - 1))
     t0 = time.perf_counter()
     oracle.cheb_braket(Hs, steps + 2, kets, kets)
     dt = time.perf_counter() - t0

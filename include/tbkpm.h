@@ -92,6 +92,8 @@ typedef struct tbk_tuning {
     int32_t use_tiled;       /* 1: LDS-staged tiled kernel where rows have <= 16 entries; 0: row kernel */
     int32_t chunk_rows;      /* rows per chunk of the visiting order (default 256) */
     int32_t chunk_period;    /* chunk-grid period P; 0: bandwidth / chunk rows, -1: row order */
+    int32_t row_cpl;         /* row kernel: complex values per lane (1, 2, 4; 0: 16 bytes per lane, i.e.
+                                1 in fp64 and 2 in fp32; at least R/64) */
 } tbk_tuning;
 
 const char* tbk_version(void);
@@ -230,6 +232,42 @@ int tbk_stack_contract(tbk_context* ctx, const double* A, const double* B, int32
 int tbk_kubo_zeta(tbk_context* ctx, const double* gamma, int32_t N, const double* e, int32_t n_e, double volume,
                   double* zeta_out, double* kernel_ms);
 
+/* ------------------------------------------------------ spectral range */
+/*
+ * Extreme eigenvalues of a Hermitian matrix, the E_min, E_max of the
+ * rescaling.  Replaces numerical::linalg::matrix_spectral_range_sparse
+ * (numerical.h:23-46), whose Spectra solver refuses more than 5000 rows and
+ * then returns {0, 0}.  FP64 Lanczos on the device from a seeded
+ * random-phase unit vector, no reorthogonalisation; after every 16 steps the
+ * extreme Ritz values theta of the tridiagonal matrix (Sturm bisection) and
+ * their residual bounds r = beta_{k+1} |y_k| are formed on the host.  Stops
+ * when both r <= tol * max(|theta_min|, |theta_max|, theta_max - theta_min),
+ * at an invariant subspace, or after min(max_iters, rows) steps.
+ * An eigenvalue of M lies within resid_min of ritz_min (and within
+ * resid_max of ritz_max), and ritz_min >= E_min, ritz_max <= E_max; the
+ * Gershgorin interval holds the whole spectrum.  lower/upper =
+ * max(gersh_min, ritz_min - resid_min), min(gersh_max, ritz_max + resid_max).
+ */
+typedef struct tbk_bounds {
+    double ritz_min, ritz_max;    /* extreme Ritz values */
+    double resid_min, resid_max;  /* their residual bounds */
+    double gersh_min, gersh_max;  /* Gershgorin interval */
+    double lower, upper;          /* the estimate of [E_min, E_max] (see above) */
+    int32_t iters;                /* Lanczos steps taken */
+    int32_t converged;            /* 1: both residuals met tol (or an invariant subspace) */
+} tbk_bounds;
+
+int tbk_spectral_bounds(tbk_context* ctx, const tbk_matrix* M, int32_t max_iters, double tol, uint64_t seed,
+                        tbk_bounds* out);
+
+/*
+ * One-shot form with the reference's signature semantics
+ * (matrix_spectral_range_sparse, numerical.h:23-46, both overloads): the
+ * extreme Ritz values after at most 1000 steps at tol 1e-8, on the
+ * process-wide context of device 0; any size.
+ */
+int tbk_spectral_range(const tbk_sparse* M, double* e_min, double* e_max);
+
 /* ------------------------------------------- batched ragged moment problems */
 typedef struct tbk_batch tbk_batch;
 
@@ -305,7 +343,10 @@ typedef struct tbk_request {
     int32_t n_moments; /* DoS: moments (>= 2); sigma_xy: N of the N x N Gamma */
     int32_t R;         /* random-phase vectors, generated as tbk_block_random_phase(rows, R, seed) */
     uint64_t seed;
-    double a, b;       /* H~ = (H - b)/a, fused into the steps */
+    double a, b;       /* H~ = (H - b)/a, fused into the steps.  DoS: a = 0 asks the server for the
+                          bounds: Lanczos (tbk_spectral_bounds, 200 steps, tol 1e-3) gives
+                          [lower, upper], then a = (upper - lower)/1.99, b = (upper + lower)/2,
+                          the contraction of GreenFunction(N, HE, ref_HE, 0.01) (lattice.h:560-569) */
     const tbk_sparse* H;
     const tbk_sparse* vx; /* sigma_xy: velocity operators, used as given */
     const tbk_sparse* vy;
@@ -320,6 +361,7 @@ typedef struct tbk_request_info {
     double latency_ms;      /* submit to completion (so far, if not finished) */
     int64_t footprint_bytes;
     int64_t result_doubles; /* DoS: n moments; sigma_xy: 2N (zeta at the N Chebyshev nodes) + 2N^2 (Gamma) */
+    double a, b;            /* the rescaling used (estimated at admission for a DoS request with a = 0) */
 } tbk_request_info;
 
 typedef struct tbk_server_stats {

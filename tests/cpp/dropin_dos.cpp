@@ -34,6 +34,11 @@ int main(int argc, char** argv) {
     auto dos = density_of_states_seq<Complex>(X, N, E, -3.5, 3.5);
     for (size_t i = 0; i < E.size(); ++i) std::printf("dos %.17g %.17g\n", E[i], dos[i]);
 
+    // bounds from the spectrum of H itself, contracted by epsilon = 0.01
+    // (GreenFunction(N, HE, ref_HE, epsilon), lattice.h:560-569)
+    PartialLattice<2>::GreenFunction<Complex> G3(N, H, H);
+    std::printf("range %.17g %.17g %.17g %.17g\n", G3.E_min, G3.E_max, G3.a, G3.b);
+
     // n < 2 keeps the reference's behaviour: message and an empty result
     auto none = cuda::Cheb_first_kind_seq_braket(G.scaled_HE, 1, kets, kets);
     std::printf("empty %zu\n", none.size());

@@ -74,3 +74,9 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     want = SO.density_of_states(mu2, g, [e for e, _ in dos], 4)
     got = np.array([d for _, d in dos])
     assert np.max(np.abs(got - want)) <= 1e-9 * np.max(np.abs(want))
+    # GreenFunction(N, H, H): E_min, E_max = the extreme eigenvalues of H
+    # (dense eigvalsh), a = (E_max - E_min) / 1.99, b = (E_max + E_min) / 2
+    rng = [float(v) for x in lines if x.startswith("range ") for v in x.split()[1:]]
+    ev = np.linalg.eigvalsh(H.to_dense())
+    assert abs(rng[0] - ev[0]) <= 1e-9 and abs(rng[1] - ev[-1]) <= 1e-9
+    assert abs(rng[2] - (rng[1] - rng[0]) / 1.99) <= 1e-15 * rng[2] and abs(rng[3] - (rng[1] + rng[0]) / 2) <= 1e-15

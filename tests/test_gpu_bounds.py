@@ -1,0 +1,142 @@
+"""Spectral range of Hermitian lattice Hamiltonians on the GPU
+(tbk_spectral_bounds, tbk_spectral_range, and DoS requests with a = 0 on the
+request server).
+
+The reference computes the range with Spectra (matrix_spectral_range_sparse,
+numerical.h:23-46), which is absent here; what the range must be is fixed by
+the mathematics instead: the extreme eigenvalues from dense eigvalsh on
+small lattices (real and complex values, open and periodic), the Bloch band
+edges +-4|t| of the clean periodic square lattice, and the properties of the
+Lanczos process -- the computed tridiagonal matrices are nested, so the
+extreme Ritz values can only move outwards as steps are added (Cauchy
+interlacing), and they never leave the spectrum."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.test_gpu_batch import model_problem
+from tight_binding_amd import _native as Nn
+from tight_binding_amd import kpm
+from tight_binding_amd import lattice as L
+
+pytestmark = pytest.mark.gpu
+
+
+def small_cases():
+    out = []
+    H, _ = model_problem("square", 18, 2.0, 5)  # periodic, real
+    out.append(("square periodic W=2", H))
+    c = L.UnitCell.model("honeycomb")
+    c.add_hopping_nn("A", 1, -1.3)
+    lat = L.PartialLattice(c, (12, 12))
+    lat.set_gauge(L.GAUGE_LANDAU, 2 * math.pi / 12)
+    lat.set_disorder(1.0, 8)
+    out.append(("honeycomb open Landau W=1 (complex)", lat.get_Hamiltonian_sparse(True)))
+    H, _ = model_problem("kagome", 10, 0.5, 6)
+    out.append(("kagome periodic W=0.5", H))
+    H, _ = model_problem("lieb", 10, 1.5, 7)
+    out.append(("lieb periodic W=1.5", H))
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (3, 3))
+    out.append(("square open 3x3 clean (degenerate)", lat.get_Hamiltonian_sparse(False)))
+    return out
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_extremes_match_dense_eigvalsh(gpu_ctx, case):
+    name, H = small_cases()[case]
+    ev = np.linalg.eigvalsh(H.to_dense())
+    M = gpu_ctx.matrix(H)
+    b = gpu_ctx.spectral_bounds(M, max_iters=1000, tol=1e-10)
+    scale = max(abs(ev[0]), abs(ev[-1]))
+    assert b["converged"] == 1, (name, b)
+    assert abs(b["ritz_min"] - ev[0]) <= 1e-9 * scale, (name, b["ritz_min"], ev[0])
+    assert abs(b["ritz_max"] - ev[-1]) <= 1e-9 * scale, (name, b["ritz_max"], ev[-1])
+    assert b["lower"] <= ev[0] + 1e-12 * scale and b["upper"] >= ev[-1] - 1e-12 * scale
+    assert b["gersh_min"] <= ev[0] and b["gersh_max"] >= ev[-1]
+    assert b["gersh_min"] <= b["lower"] and b["upper"] <= b["gersh_max"]
+    # the one-shot form (process-wide context, its own seed) agrees
+    lo, hi = kpm.spectral_range(H)
+    assert abs(lo - ev[0]) <= 1e-9 * scale and abs(hi - ev[-1]) <= 1e-9 * scale
+    M.close()
+
+
+def test_clean_square_band_edges_and_lanczos_properties(gpu_ctx):
+    """Periodic 256 x 256 square lattice, t = 1: the spectrum is
+    2(cos kx + cos ky), kx, ky in 2 pi Z / 256, exactly [-4, 4]; every
+    Gershgorin disc is [-4, 4]."""
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    H = L.PartialLattice(c, (256, 256)).get_Hamiltonian_periodic_extend_bare_sparse((1, 1))
+    M = gpu_ctx.matrix(H)
+    runs = {k: gpu_ctx.spectral_bounds(M, max_iters=k, tol=1e-14, seed=3) for k in (32, 128, 512)}
+    for k, b in runs.items():
+        assert b["iters"] == k and b["gersh_min"] == -4.0 and b["gersh_max"] == 4.0
+        assert -4.0 - 1e-12 <= b["ritz_min"] and b["ritz_max"] <= 4.0 + 1e-12  # Ritz values stay inside
+    # nested tridiagonal matrices: the extremes only move outwards
+    assert runs[32]["ritz_max"] <= runs[128]["ritz_max"] + 1e-13 <= runs[512]["ritz_max"] + 2e-13
+    assert runs[32]["ritz_min"] >= runs[128]["ritz_min"] - 1e-13 >= runs[512]["ritz_min"] - 2e-13
+    assert abs(runs[512]["ritz_max"] - 4.0) <= 1e-4 and abs(runs[512]["ritz_min"] + 4.0) <= 1e-4
+    # deterministic: the same bits again
+    again = gpu_ctx.spectral_bounds(M, max_iters=128, tol=1e-14, seed=3)
+    assert again == runs[128]
+    M.close()
+
+
+def test_c2_sized_range(gpu_ctx):
+    """2,097,152-site honeycomb, t = -2.7, W = 0.5: the range lies in the
+    Gershgorin interval +-(3 * 2.7 + 0.25) and reaches past the clean band
+    edges +-8.1 (shifted by the mean disorder, |mean| << 1e-3)."""
+    from bench import build_c2
+
+    H, a = build_c2(1024)
+    M = gpu_ctx.matrix(H)
+    b = gpu_ctx.spectral_bounds(M, max_iters=300, tol=1e-6)
+    # discs: 3 * 2.7 + V_i, |V_i| <= 0.25 (the sampled extremes reach 0.25 to 1e-6)
+    assert 8.35 - 1e-6 <= b["gersh_max"] <= 8.35 + 1e-12 and -8.35 - 1e-12 <= b["gersh_min"] <= -8.35 + 1e-6
+    assert b["gersh_min"] < b["ritz_min"] < -8.09 and 8.09 < b["ritz_max"] < b["gersh_max"]
+    # the bench's contracted Gershgorin a covers the estimate
+    assert max(-b["lower"], b["upper"]) <= a
+    M.close()
+
+
+def test_bounds_errors(gpu_ctx):
+    H, _ = model_problem("square", 6, 0.0, 1)
+    M = gpu_ctx.matrix(H)
+    for kw in (dict(max_iters=0), dict(tol=0.0), dict(tol=1.5)):
+        with pytest.raises(Nn.NativeError) as e:
+            gpu_ctx.spectral_bounds(M, **kw)
+        assert e.value.code == Nn.TBK_E_INVALID
+    M.close()
+
+
+def test_server_estimates_bounds_for_dos_requests(gpu_ctx):
+    """DoS requests with a = 0: the server's a, b put the whole spectrum in
+    [-1 + 0.004, 1 - 0.004] (the 0.01 contraction), and the moments equal the
+    single-problem path at that a, b."""
+    probs = []
+    for i, (m, Lc, W) in enumerate([("square", 9, 1.0), ("honeycomb", 8, 0.5), ("lieb", 7, 2.0), ("kagome", 8, 0.0)]):
+        H, _ = model_problem(m, Lc, W, 60 + i)
+        probs.append((H, 64 + 32 * i, 8, 700 + i, kpm.FP64 if i % 2 == 0 else kpm.FP32))
+    with kpm.Server(0, chunk_steps=8) as srv:
+        ts = [srv.submit_dos(H, n, 0.0, 0.0, R, seed, prec) for H, n, R, seed, prec in probs]
+        res = []
+        for t in ts:
+            assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
+            res.append(srv.take(t))
+    for (H, n, R, seed, prec), (mu, info) in zip(probs, res):
+        ev = np.linalg.eigvalsh(H.to_dense())
+        a, b = info["a"], info["b"]
+        assert a > 0
+        assert (ev[-1] - b) / a <= 1 - 0.004 and (ev[0] - b) / a >= -1 + 0.004
+        # [lower, upper] widens the Ritz values by their residuals (<= 1e-3 of the scale)
+        exact = (ev[-1] - ev[0]) / 1.99
+        assert exact * (1 - 1e-12) <= a <= exact * (1 + 2e-3)
+        M = gpu_ctx.matrix(H)
+        K = gpu_ctx.random_block(H.n, R, seed=seed, precision=prec)
+        ref, _, _ = gpu_ctx.moments(M, K, n, a=a, b=b)
+        assert np.max(np.abs(mu - ref)) <= 1e-12 * np.max(np.abs(ref))
+        M.close()
+        K.close()

@@ -169,7 +169,10 @@ def test_server_errors():
             srv.submit_dos(H, 100, a)
         assert e.value.code == Nn.TBK_E_UNSUPPORTED
         with pytest.raises(Nn.NativeError) as e:
-            srv.submit_dos(H, 10, 0.0)
+            srv.submit_dos(H, 10, float("nan"))
+        assert e.value.code == Nn.TBK_E_INVALID
+        with pytest.raises(Nn.NativeError) as e:  # sigma_xy operators come rescaled: a, b required
+            srv.submit_kubo(H, H, H, 8, a=0.0)
         assert e.value.code == Nn.TBK_E_INVALID
         t = srv.submit_dos(H, 10, a)
         with pytest.raises(Nn.NativeError):

@@ -77,6 +77,7 @@ class tbk_tuning(C.Structure):
         ("use_tiled", C.c_int32),
         ("chunk_rows", C.c_int32),
         ("chunk_period", C.c_int32),
+        ("row_cpl", C.c_int32),
     ]
 
 
@@ -90,6 +91,24 @@ class tbk_kubo_stats(C.Structure):
         ("nchunks", C.c_int32),
         ("reserved", C.c_int32),
         ("recursion_bytes", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class tbk_bounds(C.Structure):
+    _fields_ = [
+        ("ritz_min", C.c_double),
+        ("ritz_max", C.c_double),
+        ("resid_min", C.c_double),
+        ("resid_max", C.c_double),
+        ("gersh_min", C.c_double),
+        ("gersh_max", C.c_double),
+        ("lower", C.c_double),
+        ("upper", C.c_double),
+        ("iters", C.c_int32),
+        ("converged", C.c_int32),
     ]
 
     def as_dict(self):
@@ -157,6 +176,8 @@ class tbk_request_info(C.Structure):
         ("latency_ms", C.c_double),
         ("footprint_bytes", C.c_int64),
         ("result_doubles", C.c_int64),
+        ("a", C.c_double),
+        ("b", C.c_double),
     ]
 
     def as_dict(self):
@@ -221,6 +242,8 @@ KPM_SIGNATURES = {
     "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
     "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
     "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),
+    "tbk_spectral_bounds": ([P, P, I32, D, U64, C.POINTER(tbk_bounds)], I32),
+    "tbk_spectral_range": ([C.POINTER(tbk_sparse), DP, DP], I32),
     "tbk_batch_create": ([P, I32, C.POINTER(tbk_sparse), DP, DP, IP, I32, I32, C.POINTER(U64), PP], I32),
     "tbk_batch_run": ([P, I32, DP, C.POINTER(tbk_batch_stats)], I32),
     "tbk_batch_set_shape": ([P, I32, I32], I32),

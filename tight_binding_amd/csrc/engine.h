@@ -110,6 +110,7 @@ struct tbk_matrix {
     bool is_complex = false;
     int32_t max_row_nnz = 0;
     int32_t bandwidth95 = 0;  // 95th percentile of |row - col| over the entries
+    double gersh_min = 0.0, gersh_max = 0.0;  // Gershgorin interval (Hermitian reading)
     engine::DevBuf rp, ci, val64, val32;
     bool have32 = false;
 };
@@ -140,6 +141,7 @@ struct KernelChoice {
     int kch;            // 0: row kernel
     int width;          // columns one sweep covers (the slab width; Rp for the row kernel)
     int rows_per_tile;
+    int cpl;            // complex values per lane
 };
 
 // ------------------------------------------------------------ the driver
@@ -154,12 +156,18 @@ struct StepPlan {
     bool cplx;
 };
 
-void launch_any(int precision, bool cplx, int width, int kch, int mode, dim3 grid, hipStream_t st,
+void launch_any(int precision, bool cplx, int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st,
                 const StepArgs& a);
 KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int Rp, int max_row_nnz,
                            int bandwidth, bool force_row);
 int bandwidth_percentile(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, double p);
 void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_t>& ci, std::vector<double>& val);
+void gershgorin(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, const std::vector<double>& val,
+                bool cplx, double& lo, double& hi);
+tbk_context* default_context();
+// FP64 Lanczos estimate of the spectral range of a Hermitian matrix (bounds.hip)
+void spectral_bounds(tbk_context* ctx, const tbk_matrix* M, int max_iters, double tol, uint64_t seed,
+                     hipStream_t st, tbk_bounds* out);
 void matrix_fp32(tbk_matrix* m);
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,

@@ -43,6 +43,13 @@ namespace {
 using namespace engine;
 
 
+// spectral range of auto-bounds requests: Lanczos steps, residual tolerance,
+// seed mix (the request's seed xor this), contraction epsilon
+constexpr int kAutoIters = 200;
+constexpr double kAutoTol = 1e-3;
+constexpr uint64_t kAutoSeedMix = 0xB0D5B0D5B0D5B0D5ull;
+constexpr double kAutoEpsilon = 0.01;
+
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -82,6 +89,7 @@ struct Request {
     int32_t kind = TBK_REQ_DOS, precision = TBK_FP64, n = 0, R = 0, Rp = 0;
     uint64_t seed = 0;
     double a = 1.0, b = 0.0;
+    bool auto_bounds = false;  // DoS submitted with a = 0: a, b from the spectral range at admission
     HostSparse H, vx, vy;
     std::vector<double> g;
     int64_t footprint = 0;
@@ -146,6 +154,7 @@ struct Chunk {
 struct tbk_server {
     tbk_context* ctx = nullptr;
     hipStream_t kstream = nullptr;
+    hipStream_t bstream = nullptr;  // spectral-range Lanczos of auto-bounds requests (prep thread)
     tbk_server_config cfg{};
     int64_t budget = 0;
 
@@ -213,7 +222,10 @@ int64_t mat_bytes(const HostSparse& m, bool fp32_copy) {
 int64_t tbk_server::dos_footprint(const Request& r) const {
     const int64_t rows = r.H.hdr.rows;
     const int64_t elem = r.precision == TBK_FP64 ? 16 : 8;
-    return mat_bytes(r.H, r.precision == TBK_FP32) + 3 * rows * r.Rp * elem + static_cast<int64_t>(r.n) * 8;
+    int64_t vec = 3 * rows * r.Rp * elem;
+    // the Lanczos vectors and partials of an auto-bounds request live before the blocks
+    if (r.auto_bounds) vec = std::max<int64_t>(vec, 3 * rows * 16 + 64 * 1024);
+    return mat_bytes(r.H, r.precision == TBK_FP32) + vec + static_cast<int64_t>(r.n) * 8;
 }
 
 int64_t tbk_server::kubo_footprint(const Request& r) const {
@@ -330,6 +342,18 @@ Slot* tbk_server::start_dos(Request* r, int32_t pin) {
     s->pin = pin;
     int rc = tbk_matrix_create(ctx, &r->H.hdr, &s->M);
     if (rc != TBK_OK) raise(rc, g_last_error);
+    if (r->auto_bounds) {
+        // E_min, E_max from the Lanczos range estimate, contracted as
+        // GreenFunction(N, HE, ref_HE, epsilon = 0.01) does (lattice.h:560-569)
+        tbk_bounds bd{};
+        spectral_bounds(ctx, s->M, kAutoIters, kAutoTol, r->seed ^ kAutoSeedMix, bstream, &bd);
+        const double a = (bd.upper - bd.lower) / (2.0 - kAutoEpsilon), b = 0.5 * (bd.upper + bd.lower);
+        if (!(a > 0.0) || !std::isfinite(a) || !std::isfinite(b))
+            raise(TBK_E_INVALID, "the spectrum is a single point: give a, b");
+        std::lock_guard<std::mutex> lk(mu);
+        r->a = a;
+        r->b = b;
+    }
     if (r->precision == TBK_FP32) matrix_fp32(s->M);
     const int64_t rows = r->H.hdr.rows;
     const size_t elem = r->precision == TBK_FP64 ? 16 : 8;
@@ -685,6 +709,8 @@ void fill_info(const Request& r, tbk_request_info* info) {
     info->latency_ms = 1e3 * ((r.t_done > 0 ? r.t_done : t) - r.t_submit);
     info->footprint_bytes = r.footprint;
     info->result_doubles = static_cast<int64_t>(r.result.size());
+    info->a = r.a;
+    info->b = r.b;
 }
 
 }  // namespace
@@ -708,6 +734,7 @@ int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server**
         if (rc != TBK_OK) raise(rc, g_last_error);
         set_device(s->ctx);
         HIPC(hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking));
+        HIPC(hipStreamCreateWithFlags(&s->bstream, hipStreamNonBlocking));
         size_t free_b = 0, total_b = 0;
         HIPC(hipMemGetInfo(&free_b, &total_b));
         s->budget = c.hbm_budget_bytes > 0 ? c.hbm_budget_bytes : static_cast<int64_t>(0.85 * static_cast<double>(free_b));
@@ -734,7 +761,8 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
         if (q->kind != TBK_REQ_DOS && q->kind != TBK_REQ_KUBO) raise(TBK_E_INVALID, "unknown request kind");
         if (!q->H) raise(TBK_E_INVALID, "request without a Hamiltonian");
         if (q->R < 1) raise(TBK_E_INVALID, "R must be positive");
-        if (!(q->a != 0.0) || !std::isfinite(q->a) || !std::isfinite(q->b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
+        if (!std::isfinite(q->a) || !std::isfinite(q->b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
+        if (q->a == 0.0 && q->kind != TBK_REQ_DOS) raise(TBK_E_INVALID, "sigma_xy requests need a, b");
         auto r = std::make_unique<Request>();
         r->kind = q->kind;
         r->precision = q->precision;
@@ -744,6 +772,7 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
         r->seed = q->seed;
         r->a = q->a;
         r->b = q->b;
+        r->auto_bounds = q->kind == TBK_REQ_DOS && q->a == 0.0;
         if (r->Rp > 256) raise(TBK_E_UNSUPPORTED, "R above 256 is not supported");
         if (q->kind == TBK_REQ_DOS) {
             if (q->precision != TBK_FP64 && q->precision != TBK_FP32) raise(TBK_E_INVALID, "unknown precision");
@@ -875,6 +904,7 @@ int tbk_server_destroy(tbk_server* s) {
         if (s->pin_mu) (void)hipHostFree(s->pin_mu);
         if (s->pin_kubo) (void)hipHostFree(s->pin_kubo);
         if (s->kstream) (void)hipStreamDestroy(s->kstream);
+        if (s->bstream) (void)hipStreamDestroy(s->bstream);
         tbk_context* ctx = s->ctx;
         delete s;
         tbk_context_destroy(ctx);

@@ -75,10 +75,10 @@ bool launch_slots(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a)
 }
 
 // Launch one step; returns false if no kernel exists for the combination.
-bool launch_f64_real(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
-bool launch_f64_cplx(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
-bool launch_f32_real(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
-bool launch_f32_cplx(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
+bool launch_f64_real(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
+bool launch_f64_cplx(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
+bool launch_f32_real(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
+bool launch_f32_cplx(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
 
 // KCH = 0: cheb_step_kernel (rows of any length); KCH in {1, 2, 4}:
 // cheb_tile_kernel for rows of at most 4*KCH entries.
@@ -125,9 +125,29 @@ bool launch_kch(int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a)
     }
 }
 
-// width: padded column count for the row kernel, slab width for the tiled one
+// width: padded column count for the row kernel, slab width for the tiled one.
+// cpl: complex values per lane of the row kernel; the default (1, or width/64
+// above 64 columns) uses the width switch below, larger values put more rows
+// in one wave (LPR = width / cpl lanes per row).
 template <typename VT, bool CH>
-bool launch_width(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+bool launch_width(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    if (kch == 0 && cpl > (width > kWave ? width / kWave : 1)) {
+        switch (width * 8 + cpl) {
+            case 2 * 8 + 2: return launch_mode<VT, CH, 1, 2, 0>(mode, grid, st, a);
+            case 4 * 8 + 2: return launch_mode<VT, CH, 2, 2, 0>(mode, grid, st, a);
+            case 4 * 8 + 4: return launch_mode<VT, CH, 1, 4, 0>(mode, grid, st, a);
+            case 8 * 8 + 2: return launch_mode<VT, CH, 4, 2, 0>(mode, grid, st, a);
+            case 8 * 8 + 4: return launch_mode<VT, CH, 2, 4, 0>(mode, grid, st, a);
+            case 16 * 8 + 2: return launch_mode<VT, CH, 8, 2, 0>(mode, grid, st, a);
+            case 16 * 8 + 4: return launch_mode<VT, CH, 4, 4, 0>(mode, grid, st, a);
+            case 32 * 8 + 2: return launch_mode<VT, CH, 16, 2, 0>(mode, grid, st, a);
+            case 32 * 8 + 4: return launch_mode<VT, CH, 8, 4, 0>(mode, grid, st, a);
+            case 64 * 8 + 2: return launch_mode<VT, CH, 32, 2, 0>(mode, grid, st, a);
+            case 64 * 8 + 4: return launch_mode<VT, CH, 16, 4, 0>(mode, grid, st, a);
+            case 128 * 8 + 4: return launch_mode<VT, CH, 32, 4, 0>(mode, grid, st, a);
+            default: return false;
+        }
+    }
     switch (width) {
         case 1: return launch_kch<VT, CH, 1, 1>(kch, mode, grid, st, a);
         case 2: return launch_kch<VT, CH, 2, 1>(kch, mode, grid, st, a);

@@ -2,8 +2,8 @@
 #include "step_launch.h"
 
 namespace tbk {
-bool launch_f32_cplx(int width, int kch, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
-    return launch_width<float, true>(width, kch, mode, grid, st, a);
+bool launch_f32_cplx(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
+    return launch_width<float, true>(width, kch, cpl, mode, grid, st, a);
 }
 bool launch_slots_f32_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a) {
     return launch_slots<float, true>(Rp, cpl, grid, st, a);

@@ -32,13 +32,15 @@ namespace engine {
 
 // --------------------------------------------------------- kernel dispatch
 
-void launch_any(int precision, bool cplx, int width, int kch, int mode, dim3 grid, hipStream_t st,
+void launch_any(int precision, bool cplx, int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st,
                 const StepArgs& a) {
     bool ok;
     if (precision == TBK_FP64)
-        ok = cplx ? tbk::launch_f64_cplx(width, kch, mode, grid, st, a) : tbk::launch_f64_real(width, kch, mode, grid, st, a);
+        ok = cplx ? tbk::launch_f64_cplx(width, kch, cpl, mode, grid, st, a)
+                  : tbk::launch_f64_real(width, kch, cpl, mode, grid, st, a);
     else
-        ok = cplx ? tbk::launch_f32_cplx(width, kch, mode, grid, st, a) : tbk::launch_f32_real(width, kch, mode, grid, st, a);
+        ok = cplx ? tbk::launch_f32_cplx(width, kch, cpl, mode, grid, st, a)
+                  : tbk::launch_f32_real(width, kch, cpl, mode, grid, st, a);
     if (!ok) raise(TBK_E_UNSUPPORTED, "no step kernel for this block width");
 }
 
@@ -65,9 +67,21 @@ KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int R
             sw = Rp;
         }
     }
+    if (kch == 0) {
+        // row kernel: cpl complex values per lane (at least Rp/64), LPR = Rp / cpl;
+        // by default 16 bytes per lane (one complex128 or two complex64): on
+        // gfx950 that is 0.61-0.83x the fp32 step time of one complex64 per
+        // lane at R = 8..64, and wider lanes lose in fp64
+        const int cmin = Rp > tbk::kWave ? Rp / tbk::kWave : 1;
+        int cpl = std::max(cmin, tu.row_cpl > 0 ? tu.row_cpl : 16 / ebytes);
+        if (cpl != 1 && cpl != 2 && cpl != 4) cpl = cmin;
+        if (cpl > Rp || Rp % cpl != 0 || (cpl > cmin && Rp * 8 + cpl > 128 * 8 + 4)) cpl = cmin;
+        const int lpr = Rp / cpl;
+        return {0, Rp, tbk::kWavesPerBlock * (tbk::kWave / lpr), cpl};
+    }
     const int lpr = std::min(sw, tbk::kWave);
-    const int tr = kch ? tbk::tile_rows(lpr, sw / lpr, kch, hbytes) : tbk::kWavesPerBlock * (tbk::kWave / lpr);
-    return {kch, sw, tr};
+    const int tr = tbk::tile_rows(lpr, sw / lpr, kch, hbytes);
+    return {kch, sw, tr, sw / lpr};
 }
 
 // ---------------------------------------------------------- matrix upload
@@ -81,6 +95,30 @@ int bandwidth_percentile(const std::vector<int32_t>& rp, const std::vector<int32
     const size_t at = std::min(d.size() - 1, static_cast<size_t>(p * static_cast<double>(d.size())));
     std::nth_element(d.begin(), d.begin() + static_cast<std::ptrdiff_t>(at), d.end());
     return d[at];
+}
+
+// Gershgorin interval of a Hermitian matrix: the union over rows of
+// [Re h_ii - r_i, Re h_ii + r_i], r_i = sum_{j != i} |h_ij|
+void gershgorin(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, const std::vector<double>& val,
+                bool cplx, double& lo, double& hi) {
+    lo = 0.0;
+    hi = 0.0;
+    bool any = false;
+    const int vw = cplx ? 2 : 1;
+    for (size_t i = 0; i + 1 < rp.size(); ++i) {
+        double c = 0.0, r = 0.0;
+        for (int32_t p = rp[i]; p < rp[i + 1]; ++p) {
+            const double re = val[static_cast<size_t>(p) * vw];
+            const double im = cplx ? val[static_cast<size_t>(p) * vw + 1] : 0.0;
+            if (ci[p] == static_cast<int32_t>(i))
+                c += re;
+            else
+                r += std::hypot(re, im);
+        }
+        lo = any ? std::min(lo, c - r) : c - r;
+        hi = any ? std::max(hi, c + r) : c + r;
+        any = true;
+    }
 }
 
 void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_t>& ci,
@@ -201,7 +239,8 @@ void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, dou
     A.y = y;
     A.bra = bra;
     A.partial = partial;
-    launch_any(pl.precision, pl.cplx, pl.kc.width, pl.kc.kch, mode, dim3(static_cast<unsigned>(pl.nb)), st, A);
+    launch_any(pl.precision, pl.cplx, pl.kc.width, pl.kc.kch, pl.kc.cpl, mode, dim3(static_cast<unsigned>(pl.nb)), st,
+               A);
     HIPC(hipGetLastError());
 }
 
@@ -554,6 +593,20 @@ tbk_context* g_default_ctx = nullptr;
 
 }  // namespace
 
+namespace engine {
+
+// the process-wide context of the one-shot entries, on device 0
+tbk_context* default_context() {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default_ctx) {
+        const int rc = tbk_context_create(0, &g_default_ctx);
+        if (rc != TBK_OK) raise(rc, g_last_error);
+    }
+    return g_default_ctx;
+}
+
+}  // namespace engine
+
 // =================================================================== C ABI
 extern "C" {
 
@@ -672,6 +725,7 @@ int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out) {
         m->is_complex = M->is_complex != 0;
         for (int32_t i = 0; i < M->rows; ++i) m->max_row_nnz = std::max(m->max_row_nnz, rp[i + 1] - rp[i]);
         m->bandwidth95 = bandwidth_percentile(rp, ci, 0.95);
+        gershgorin(rp, ci, val, m->is_complex, m->gersh_min, m->gersh_max);
         m->rp.reserve(rp.size() * 4);
         m->ci.reserve(std::max<size_t>(ci.size() * 4, 4));
         m->val64.reserve(std::max<size_t>(val.size() * 8, 8));
@@ -871,6 +925,7 @@ int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
             {&tbk_tuning::chunk_period, {0, -1}},
             {&tbk_tuning::xcd_split, {0, -1}},
             {&tbk_tuning::stream_nt, {1, -1}},
+            {&tbk_tuning::row_cpl, {1, 2, 4}},
         };
         for (int pass = 0; pass < 2; ++pass) {
             for (const auto& k : knobs) {
@@ -956,15 +1011,7 @@ int tbk_cheb_first_kind_seq_braket(const tbk_sparse* M, int32_t n, const double*
         if (!M || !bras_T || !kets || !mu_out) raise(TBK_E_INVALID, "null argument");
         if (n < 2) raise(TBK_E_INVALID, "n must be greater than or equal to 2");
         if (ket_rows != M->rows || ket_rows != M->cols) raise(TBK_E_INVALID, "kets rows do not match M");
-        tbk_context* ctx;
-        {
-            std::lock_guard<std::mutex> lk(g_default_mu);
-            if (!g_default_ctx) {
-                int rc = tbk_context_create(0, &g_default_ctx);
-                if (rc != TBK_OK) raise(rc, g_last_error);
-            }
-            ctx = g_default_ctx;
-        }
+        tbk_context* ctx = default_context();
         struct Guard {
             tbk_matrix* m = nullptr;
             tbk_block* k = nullptr;

@@ -12,6 +12,7 @@
 #pragma once
 
 #include <cmath>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -34,6 +35,14 @@ struct PartialLattice<Dim>::GreenFunction {
 
     GreenFunction(int N, const Mat& HE, double E_min, double E_max) : N(N), E_min(E_min), E_max(E_max) {
         a = (E_max - E_min) / 2;
+        b = (E_max + E_min) / 2;
+        scaled_HE = rescale(HE, a, b);
+    }
+    // bounds from the spectrum of ref_HE, contracted by epsilon
+    // (lattice.h:560-569): a = (E_max - E_min)/(2 - epsilon)
+    GreenFunction(int N, const Mat& HE, const Mat& ref_HE, double epsilon = 0.01) : N(N) {
+        std::tie(E_min, E_max) = numerical::linalg::matrix_spectral_range_sparse(ref_HE);
+        a = (E_max - E_min) / (2 - epsilon);
         b = (E_max + E_min) / 2;
         scaled_HE = rescale(HE, a, b);
     }

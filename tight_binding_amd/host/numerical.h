@@ -7,10 +7,15 @@
 #include <complex>
 #include <cstdint>
 #include <numbers>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "lattice.h"
 #include "linalg.h"
+#include "tbkpm.h"
 
 namespace tb::numerical {
 
@@ -39,6 +44,29 @@ inline MatrixXcd bases_of_uniform_ampl_with_random_phases(int n, int m) {
             k(i, r) = Complex(std::cos(th) * inv, std::sin(th) * inv);
         }
     return k;
+}
+
+// {E_min, E_max} of a Hermitian sparse matrix (numerical.h:23-46, both
+// overloads): FP64 Lanczos on the device (tbk_spectral_range), any size.
+// The reference's Spectra solver refuses more than 5000 rows and returns
+// {0, 0}; this throws std::runtime_error on a device failure instead.  A
+// template so that code which never calls it does not link libtbkpm.
+template <typename Sparse>
+std::pair<double, double> matrix_spectral_range_sparse(const Sparse& A) {
+    using Scalar = std::remove_cv_t<std::remove_pointer_t<decltype(A.valuePtr())>>;
+    tbk_sparse s{};
+    s.rows = static_cast<int32_t>(A.rows());
+    s.cols = static_cast<int32_t>(A.cols());
+    s.nnz = static_cast<int64_t>(A.nonZeros());
+    s.outer = reinterpret_cast<const int32_t*>(A.outerIndexPtr());
+    s.inner = reinterpret_cast<const int32_t*>(A.innerIndexPtr());
+    s.values = reinterpret_cast<const double*>(A.valuePtr());
+    s.is_complex = std::is_same_v<Scalar, Complex> ? 1 : 0;
+    s.layout = TBK_CSC;
+    double lo = 0.0, hi = 0.0;
+    if (tbk_spectral_range(&s, &lo, &hi) != TBK_OK)
+        throw std::runtime_error(std::string("matrix_spectral_range_sparse: ") + tbk_last_error());
+    return {lo, hi};
 }
 
 // Re tr(bras^H kets) (numerical.h:127-135)

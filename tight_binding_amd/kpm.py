@@ -125,10 +125,10 @@ class Context:
         N.check_kpm(N.kpm().tbk_context_synchronize(self._h))
 
     def set_tuning(self, slab_cols=0, blocks_per_cu=0, xcd_split=0, l2_budget_bytes=0, stream_nt=0,
-                   use_tiled=0, chunk_rows=0, chunk_period=0):
+                   use_tiled=0, chunk_rows=0, chunk_period=0, row_cpl=0):
         """Launch-shape choices (0 = default)."""
         t = N.tbk_tuning(int(slab_cols), int(blocks_per_cu), int(xcd_split), int(l2_budget_bytes),
-                         int(stream_nt), int(use_tiled), int(chunk_rows), int(chunk_period))
+                         int(stream_nt), int(use_tiled), int(chunk_rows), int(chunk_period), int(row_cpl))
         N.check_kpm(N.kpm().tbk_context_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, M: "Matrix", kets: "Block", a=1.0, b=0.0, steps=16):
@@ -147,6 +147,14 @@ class Context:
 
     def matrix(self, m: CSC) -> Matrix:
         return Matrix(self, m)
+
+    def spectral_bounds(self, M: Matrix, max_iters: int = 1000, tol: float = 1e-8, seed: int = 1) -> dict:
+        """Extreme eigenvalues of the Hermitian matrix M by FP64 Lanczos on the
+        device (tbk_spectral_bounds): ritz_min/max, their residual bounds,
+        the Gershgorin interval, lower/upper, iters, converged."""
+        b = N.tbk_bounds()
+        N.check_kpm(N.kpm().tbk_spectral_bounds(self._h, M._h, int(max_iters), float(tol), int(seed), C.byref(b)))
+        return b.as_dict()
 
     def block(self, kets: np.ndarray, precision: int = FP64) -> Block:
         kets = np.asarray(kets)
@@ -212,6 +220,23 @@ def stack_contract(ctx: Context, A: np.ndarray, B: np.ndarray, precision: int = 
     N.check_kpm(N.kpm().tbk_stack_contract(ctx._h, _dp(A.view(np.float64)), _dp(B.view(np.float64)), n, k,
                                            int(precision), _dp(out), C.byref(ms)))
     return out.view(np.complex128).reshape(n, n), ms.value
+
+
+def spectral_range(m: CSC):
+    """(E_min, E_max) of a Hermitian CSC matrix, the one-shot form of the
+    reference's matrix_spectral_range_sparse (numerical.h:23-46) without its
+    5000-row limit (tbk_spectral_range, process-wide context on device 0)."""
+    s, keep = tbk_sparse_of(m)
+    lo, hi = C.c_double(), C.c_double()
+    N.check_kpm(N.kpm().tbk_spectral_range(C.byref(s), C.byref(lo), C.byref(hi)))
+    del keep
+    return lo.value, hi.value
+
+
+def rescale_from_range(e_min: float, e_max: float, epsilon: float = 0.01):
+    """a, b of GreenFunction(N, HE, ref_HE, epsilon) (lattice.h:560-569):
+    a = (E_max - E_min) / (2 - epsilon), b = (E_max + E_min) / 2."""
+    return (e_max - e_min) / (2.0 - epsilon), (e_max + e_min) / 2.0
 
 
 def kubo_zeta(ctx: Context, gamma: np.ndarray, e, volume: float = 1.0):
@@ -305,7 +330,10 @@ class Server:
 
     def submit_dos(self, H: CSC, n: int, a: float, b: float = 0.0, R: int = 8, seed: int = 0,
                    precision: int = FP64) -> int:
-        """Moments mu_0..mu_{n-1} of (H - b)/a from R device-generated random-phase vectors."""
+        """Moments mu_0..mu_{n-1} of (H - b)/a from R device-generated random-phase vectors.
+        a = 0: the server estimates the bounds at admission (Lanczos range,
+        contracted by 0.01 as GreenFunction(N, HE, ref_HE) does); the a, b it
+        used are in info()["a"], info()["b"]."""
         sH, kH = tbk_sparse_of(H)
         req = N.tbk_request(N.TBK_REQ_DOS, int(precision), int(n), int(R), int(seed), float(a), float(b),
                             C.pointer(sH), None, None, None)
