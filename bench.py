@@ -446,7 +446,7 @@ def c1_leg(ctx, repeats=5):
     """The reference's CPU-runnable case end to end: square 100x100 periodic,
     t = 1, on-site energies uniform in [-0.5, 0.5], a = 4.6, R = 16 random-phase
     vectors, N = 512 moments, Jackson kernel, DoS at 1024 energies.  Device
-    path (resident H, device kets, fused recursion, host reconstruction)
+    path (resident H, device kets, fused recursion, device reconstruction)
     against the oracle's restatement of the reference CPU loop (one moment per
     step, single thread) on the same kets."""
     import oracle
@@ -471,8 +471,7 @@ def c1_leg(ctx, repeats=5):
         ctx.synchronize()
         t0 = time.perf_counter()
         mu, st, _ = ctx.moments(M, K, N, a=a)
-        with np.errstate(divide="ignore", invalid="ignore"):
-            dos = S.density_of_states(mu, g, e, R)
+        dos = kpm.density_of_states(ctx, mu, g, e, R)  # reconstruction on the device
         times.append(time.perf_counter() - t0)
         kms.append(st["kernel_ms"])
     kets = K.download()

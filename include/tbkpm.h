@@ -232,6 +232,26 @@ int tbk_stack_contract(tbk_context* ctx, const double* A, const double* B, int32
 int tbk_kubo_zeta(tbk_context* ctx, const double* gamma, int32_t N, const double* e, int32_t n_e, double volume,
                   double* zeta_out, double* kernel_ms);
 
+/* ------------------------------------------------ spectral reconstruction */
+/*
+ * Chebyshev reconstruction at energies e[0..n_e) in rescaled units, on the
+ * device (one workgroup per energy, fixed-order sums):
+ *   out_j = scale * [g_0 mu_0 + 2 sum_{n=1}^{N-1} g_n mu_n cos(n theta_j)] / sin(theta_j),
+ *   e_j = cos(theta_j).
+ * scale = 1: the values of GreenFunctionImag::fft() (lattice.h:625-640) -- at
+ * the nodes e_k = cos((k + 1/2) pi / N) its FFTW REDFT01 of g_n mu_n times
+ * 1/sqrt(1 - e_k^2) -- at any energies; scale = -1: GreenFunctionImag's
+ * operator() (lattice.h:617-623); scale = 1/(R pi): the DoS of
+ * density_of_states_seq (functions.h:27-30).  g may be NULL (all ones); ctx
+ * may be NULL (the process-wide context of device 0).  |e_j| >= 1 gives a
+ * non-finite value, as the reference's at e = -1.
+ * e = NULL: the n_e nodes theta_k = (k + 1/2) pi / n_e, with theta_k and
+ * sin(theta_k) formed directly (no rounding of e_k enters near the edges):
+ * fft(n) with n_e = n N, the moments zero-padded past N.
+ */
+int tbk_cheb_reconstruct(tbk_context* ctx, const double* mu, const double* g, int32_t N, const double* e,
+                         int32_t n_e, double scale, double* out, double* kernel_ms);
+
 /* ------------------------------------------------------ spectral range */
 /*
  * Extreme eigenvalues of a Hermitian matrix, the E_min, E_max of the

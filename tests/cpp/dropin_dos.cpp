@@ -28,6 +28,8 @@ int main(int argc, char** argv) {
     MatrixXcd kets = numerical::linalg::bases_of_uniform_ampl_with_random_phases(X.get_total_number(), 4);
     auto f = G.imag(kets, kets);
     for (double m : f.moments) std::printf("mu %.17g\n", m);
+    auto [En, rn] = f.fft();  // GreenFunctionImag::fft(): the expansion at the N Chebyshev nodes
+    for (size_t k = 0; k < En.size(); ++k) std::printf("fft %.17g %.17g\n", En[k], rn[k]);
 
     // the DoS driver (draws the next ket seed)
     std::vector<double> E = {-0.5, -0.1, 0.0, 0.3, 0.7};

@@ -67,6 +67,16 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     ref = oracle.cheb_braket(Hs, N, k1, k1)
     assert mu.shape == (N,)
     assert np.max(np.abs(mu - ref)) <= 1e-10 * np.max(np.abs(ref))
+    # fft(): REDFT01 of g_n mu_n (scipy's DCT-III is the same transform) over
+    # sqrt(1 - E_k^2) at E_k = cos((k + 1/2) pi / N), Lorentz(N, 4)
+    from scipy.fft import dct
+
+    fl = np.array([[float(v) for v in x.split()[1:]] for x in lines if x.startswith("fft ")])
+    gL = oracle.kernel_lorentz(N, 4.0)
+    Ek = np.cos((np.arange(N) + 0.5) * np.pi / N)
+    want = dct(gL * ref, type=3) / np.sin((np.arange(N) + 0.5) * np.pi / N)
+    assert np.max(np.abs(fl[:, 0] - Ek)) <= 1e-15  # libm and numpy cosines may differ in the last bit
+    assert np.max(np.abs(fl[:, 1] - want)) <= 1e-10 * np.max(np.abs(want))
     # density_of_states_seq draws the next seed and uses Lorentz(N, 4)
     k2 = cpp_kets(n, 4, 0x5EED + 1)
     mu2 = oracle.cheb_braket(Hs, N, k2, k2)

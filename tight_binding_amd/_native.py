@@ -242,6 +242,7 @@ KPM_SIGNATURES = {
     "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
     "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
     "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),
+    "tbk_cheb_reconstruct": ([P, DP, DP, I32, DP, I32, D, DP, DP], I32),
     "tbk_spectral_bounds": ([P, P, I32, D, U64, C.POINTER(tbk_bounds)], I32),
     "tbk_spectral_range": ([C.POINTER(tbk_sparse), DP, DP], I32),
     "tbk_batch_create": ([P, I32, C.POINTER(tbk_sparse), DP, DP, IP, I32, I32, C.POINTER(U64), PP], I32),

@@ -12,6 +12,8 @@
 #pragma once
 
 #include <cmath>
+#include <stdexcept>
+#include <string>
 #include <tuple>
 #include <utility>
 #include <vector>
@@ -66,6 +68,21 @@ struct PartialLattice<Dim>::GreenFunction {
             for (int n = 0; n < G->N && n < static_cast<int>(moments.size()); ++n)
                 sum += g[n] * alpha[n].imag() * moments[n];
             return sum;
+        }
+        // the expansion at the n N Chebyshev nodes E_k = cos((k + 1/2) pi / (n N)):
+        // REDFT01 of g_n mu_n times weight(E_k) (lattice.h:625-640), evaluated on
+        // the device at theta_k directly (tbk_cheb_reconstruct, node mode).  For
+        // n > 1 the moments are zero-padded (the reference reads past its N
+        // moments there).  Nothing is printed per node.
+        std::pair<std::vector<double>, std::vector<double>> fft(int n = 1) const {
+            if (n < 1) throw std::invalid_argument("fft: n must be >= 1");
+            const int N0 = static_cast<int>(moments.size()), Nn = n * N0;
+            std::vector<double> E(static_cast<size_t>(Nn)), r(static_cast<size_t>(Nn));
+            for (int k = 0; k < Nn; ++k) E[k] = std::cos((k + 0.5) * numerical::pi / Nn);
+            if (tbk_cheb_reconstruct(nullptr, moments.data(), g.data(), N0, nullptr, Nn, 1.0, r.data(), nullptr) !=
+                TBK_OK)
+                throw std::runtime_error(std::string("fft: ") + tbk_last_error());
+            return {E, r};
         }
     };
     GreenFunctionImag imag(const Kets& bras_T, const Kets& kets) { return GreenFunctionImag(this, bras_T, kets); }

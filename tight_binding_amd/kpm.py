@@ -222,6 +222,43 @@ def stack_contract(ctx: Context, A: np.ndarray, B: np.ndarray, precision: int = 
     return out.view(np.complex128).reshape(n, n), ms.value
 
 
+def reconstruct(ctx, mu, g, e, scale: float = 1.0):
+    """scale * [g_0 mu_0 + 2 sum_n g_n mu_n cos(n theta)] / sin(theta) at the
+    rescaled energies e = cos(theta), on the device (tbk_cheb_reconstruct);
+    e an int: at the e nodes theta_k = (k + 1/2) pi / e.  ctx None: the
+    process-wide context.  Returns (values, kernel ms)."""
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+    if gg is not None:
+        assert gg.size >= mu.size
+    if isinstance(e, int):  # node mode: theta_k = (k + 1/2) pi / e
+        n_e, ep = e, None
+    else:
+        e = np.ascontiguousarray(np.atleast_1d(e), dtype=np.float64)
+        n_e, ep = e.size, _dp(e)
+    out = np.empty(n_e, dtype=np.float64)
+    ms = C.c_double()
+    N.check_kpm(N.kpm().tbk_cheb_reconstruct(None if ctx is None else ctx._h, _dp(mu),
+                                             None if gg is None else _dp(gg), int(mu.size), ep, int(n_e),
+                                             float(scale), _dp(out), C.byref(ms)))
+    return out, ms.value
+
+
+def density_of_states(ctx, mu, g, e, R):
+    """DoS(e) = -f(e)/(R pi) of density_of_states_seq (functions.h:27-30) on
+    the device; e in rescaled units."""
+    return reconstruct(ctx, mu, g, e, 1.0 / (R * np.pi))[0]
+
+
+def green_fft(ctx, mu, g, n: int = 1):
+    """GreenFunctionImag::fft(n) (lattice.h:625-640): the nodes
+    E_k = cos((k + 1/2) pi / (n N)) and REDFT01 of g mu zero-padded to n N,
+    over sin(theta_k)."""
+    m = n * np.asarray(mu).size
+    E = np.cos((np.arange(m) + 0.5) * np.pi / m)
+    return E, reconstruct(ctx, mu, g, int(m), 1.0)[0]
+
+
 def spectral_range(m: CSC):
     """(E_min, E_max) of a Hermitian CSC matrix, the one-shot form of the
     reference's matrix_spectral_range_sparse (numerical.h:23-46) without its
