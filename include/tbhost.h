@@ -47,6 +47,27 @@ int tbh_lattice_total_number(const tbh_lattice* l, int32_t* n);
 int tbh_lattice_set_gauge(tbh_lattice* l, int32_t kind, double B);
 /* site-keyed on-site disorder V = W (u(seed, site) - 1/2) */
 int tbh_lattice_set_disorder(tbh_lattice* l, double W, uint64_t seed);
+/* what a device builder needs to know about a lattice besides its tables */
+typedef struct tbh_lattice_info {
+    int32_t dim, n_atoms, n_hops; /* n_hops: the lattice's own table (see tbh_lattice_hoppings) */
+    int32_t span[3];              /* cells per axis (1 past dim) */
+    int32_t cell_number, total_number;
+    int32_t gauge;                /* as set by tbh_lattice_set_gauge (0 none) */
+    double B;
+    int32_t has_disorder;         /* set by tbh_lattice_set_disorder */
+    double disorder_W;
+    uint64_t disorder_seed;
+    int32_t removed_sites;        /* sites removed by tbh_lattice_remove_atoms */
+    int32_t position_field;       /* 1 if a C++ caller set a position-dependent scalar field */
+} tbh_lattice_info;
+int tbh_lattice_info_get(const tbh_lattice* l, tbh_lattice_info* out);
+/* the lattice's hopping table in table order: the cell's, plus the zero
+   on-site entries a scalar field adds (lattice.h:496-505); n_hops entries */
+int tbh_lattice_hoppings(const tbh_lattice* l, int32_t* from, int32_t* to, int32_t* lattice_vector, double* t);
+/* exp(-i A(r_from).d_h) per (cell, hopping), cell_number * n_hops complex
+   (re, im), exactly as the open and periodic builders apply it
+   (lattice.h:282-285); needs a gauge field */
+int tbh_lattice_phase_table(const tbh_lattice* l, double* out);
 /* remove atom `atom` from the listed cells (n cells, dim ints each) */
 int tbh_lattice_remove_atoms(tbh_lattice* l, const char* atom, const int32_t* cells, int32_t n);
 /* builder: 0 get_Hamiltonian_sparse (open, lattice.h:262-305),

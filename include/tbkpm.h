@@ -140,6 +140,43 @@ int tbk_matrix_destroy(tbk_matrix* m);
 int tbk_matrix_info(const tbk_matrix* m, int32_t* rows, int64_t* nnz, int32_t* max_row_nnz,
                     int32_t* bandwidth95);
 
+/* copy a resident matrix back row-compressed: row pointers (rows + 1),
+   column indices (nnz, ascending within a row), values (nnz doubles, or
+   2 nnz interleaved if complex) */
+int tbk_matrix_download(const tbk_matrix* m, int32_t* rowptr, int32_t* colidx, double* values);
+
+/*
+ * A lattice Hamiltonian described by its tables, for assembly on the device.
+ * builder 0: get_Hamiltonian_sparse (open, lattice.h:262-305) without removed
+ * sites; 1: get_Hamiltonian_periodic_extend_bare_sparse(extend)
+ * (lattice.h:216-258); 2: get_Hamiltonian_bare_sparse (lattice.h:347-377,
+ * no fields).  Sites: bare index = cell . strides + atom, last lattice index
+ * fastest (lattice.h:28, 106-117).  The hopping table is the lattice's own, in
+ * table order (including the zero on-site entries a scalar field adds).
+ */
+typedef struct tbk_lattice_desc {
+    int32_t dim;              /* 1, 2 or 3 */
+    int32_t span[3];          /* cells per axis (first dim used) */
+    int32_t extend[3];        /* builder 1: wrap along axes with extend != 0 */
+    int32_t n_atoms, n_hops, builder, is_complex;
+    const int32_t* hop_from;  /* n_hops atom indices */
+    const int32_t* hop_to;
+    const int32_t* hop_lv;    /* n_hops * dim lattice vectors */
+    const double* hop_t;      /* 2 n_hops: (re, im) */
+    const double* phase;      /* NULL, or exp(-i A(r_from).d) per (cell, hopping) as the host builders
+                                 compute it, cell_number * n_hops complex (builders 0, 1) */
+    int32_t has_disorder;     /* on-site W (u(seed, site) - 1/2), the site-keyed counter generator */
+    double disorder_W;
+    uint64_t disorder_seed;
+} tbk_lattice_desc;
+
+/*
+ * Assemble on the device the matrix the host builder produces (the same
+ * entries, order and bits) and keep it resident; no host-side matrix is
+ * formed.  At most 32 hoppings may touch one atom (TBK_E_UNSUPPORTED).
+ */
+int tbk_matrix_assemble(tbk_context* ctx, const tbk_lattice_desc* d, tbk_matrix** out, double* wall_ms);
+
 /* upload rows x R column-major complex128 (NULL: zeros) as a block */
 int tbk_block_create(tbk_context* ctx, const double* colmajor, int32_t rows, int32_t R,
                      int32_t precision, tbk_block** out);
@@ -371,6 +408,8 @@ typedef struct tbk_request {
     const tbk_sparse* vx; /* sigma_xy: velocity operators, used as given */
     const tbk_sparse* vy;
     const double* g;   /* sigma_xy: kernel weights g_n (N of them) or NULL for ones */
+    const tbk_lattice_desc* lattice; /* DoS with H = NULL: the Hamiltonian as a lattice description,
+                                        assembled on the device at admission (tbk_matrix_assemble) */
 } tbk_request;
 
 typedef struct tbk_request_info {

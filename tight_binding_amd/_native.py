@@ -45,6 +45,44 @@ class tbk_sparse(C.Structure):
     ]
 
 
+class tbk_lattice_desc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("span", C.c_int32 * 3),
+        ("extend", C.c_int32 * 3),
+        ("n_atoms", C.c_int32),
+        ("n_hops", C.c_int32),
+        ("builder", C.c_int32),
+        ("is_complex", C.c_int32),
+        ("hop_from", C.POINTER(C.c_int32)),
+        ("hop_to", C.POINTER(C.c_int32)),
+        ("hop_lv", C.POINTER(C.c_int32)),
+        ("hop_t", C.POINTER(C.c_double)),
+        ("phase", C.POINTER(C.c_double)),
+        ("has_disorder", C.c_int32),
+        ("disorder_W", C.c_double),
+        ("disorder_seed", C.c_uint64),
+    ]
+
+
+class tbh_lattice_info(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("n_atoms", C.c_int32),
+        ("n_hops", C.c_int32),
+        ("span", C.c_int32 * 3),
+        ("cell_number", C.c_int32),
+        ("total_number", C.c_int32),
+        ("gauge", C.c_int32),
+        ("B", C.c_double),
+        ("has_disorder", C.c_int32),
+        ("disorder_W", C.c_double),
+        ("disorder_seed", C.c_uint64),
+        ("removed_sites", C.c_int32),
+        ("position_field", C.c_int32),
+    ]
+
+
 class tbk_stats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double),
@@ -164,6 +202,7 @@ class tbk_request(C.Structure):
         ("vx", C.POINTER(tbk_sparse)),
         ("vy", C.POINTER(tbk_sparse)),
         ("g", C.POINTER(C.c_double)),
+        ("lattice", C.POINTER(tbk_lattice_desc)),
     ]
 
 
@@ -242,6 +281,8 @@ KPM_SIGNATURES = {
     "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
     "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
     "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),
+    "tbk_matrix_assemble": ([P, C.POINTER(tbk_lattice_desc), PP, DP], I32),
+    "tbk_matrix_download": ([P, IP, IP, DP], I32),
     "tbk_cheb_reconstruct": ([P, DP, DP, I32, DP, I32, D, DP, DP], I32),
     "tbk_spectral_bounds": ([P, P, I32, D, U64, C.POINTER(tbk_bounds)], I32),
     "tbk_spectral_range": ([C.POINTER(tbk_sparse), DP, DP], I32),
@@ -275,6 +316,9 @@ HOST_SIGNATURES = {
     "tbh_lattice_set_gauge": ([P, I32, D], I32),
     "tbh_lattice_set_disorder": ([P, D, U64], I32),
     "tbh_lattice_remove_atoms": ([P, C.c_char_p, IP, I32], I32),
+    "tbh_lattice_info_get": ([P, C.POINTER(tbh_lattice_info)], I32),
+    "tbh_lattice_hoppings": ([P, IP, IP, IP, DP], I32),
+    "tbh_lattice_phase_table": ([P, DP], I32),
     "tbh_lattice_hamiltonian": ([P, I32, IP, I32, PP], I32),
     "tbh_lattice_op_position": ([P, I32, PP], I32),
     "tbh_lattice_op_velocity": ([P, I32, I32, PP], I32),

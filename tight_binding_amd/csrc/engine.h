@@ -165,6 +165,10 @@ void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_
 void gershgorin(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, const std::vector<double>& val,
                 bool cplx, double& lo, double& hi);
 tbk_context* default_context();
+// Hamiltonian assembly on the device (assemble.hip)
+int64_t lattice_rows(const tbk_lattice_desc* d);
+int64_t lattice_nnz_bound(const tbk_lattice_desc* d);
+void assemble_matrix(tbk_context* ctx, const tbk_lattice_desc* d, hipStream_t st, tbk_matrix** out, double* ms);
 // FP64 Lanczos estimate of the spectral range of a Hermitian matrix (bounds.hip)
 void spectral_bounds(tbk_context* ctx, const tbk_matrix* M, int max_iters, double tol, uint64_t seed,
                      hipStream_t st, tbk_bounds* out);

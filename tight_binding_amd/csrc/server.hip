@@ -84,13 +84,55 @@ struct HostSparse {
     }
 };
 
+// Host copy of a lattice description handed to submit (assembled on the
+// device at admission).
+struct HostLattice {
+    tbk_lattice_desc desc{};
+    std::vector<int32_t> from, to, lv;
+    std::vector<double> t, phase;
+    bool present = false;
+    void copy_from(const tbk_lattice_desc* d) {
+        if (d->dim < 1 || d->dim > 3 || d->n_atoms < 1 || d->n_hops < 0 || d->builder < 0 || d->builder > 2)
+            raise(TBK_E_INVALID, "invalid lattice description");
+        if (d->n_hops > 0 && (!d->hop_from || !d->hop_to || !d->hop_lv || !d->hop_t))
+            raise(TBK_E_INVALID, "null hopping table");
+        for (int k = 0; k < d->dim; ++k)
+            if (d->span[k] < 1) raise(TBK_E_INVALID, "span must be positive");
+        desc = *d;
+        const size_t nh = static_cast<size_t>(d->n_hops);
+        from.assign(d->hop_from, d->hop_from + nh);
+        to.assign(d->hop_to, d->hop_to + nh);
+        lv.assign(d->hop_lv, d->hop_lv + nh * d->dim);
+        t.assign(d->hop_t, d->hop_t + 2 * nh);
+        if (d->phase && d->builder != 2) {
+            const size_t pn = static_cast<size_t>(engine::lattice_rows(d) / d->n_atoms) * nh * 2;
+            phase.assign(d->phase, d->phase + pn);
+        }
+        desc.hop_from = from.data();
+        desc.hop_to = to.data();
+        desc.hop_lv = lv.data();
+        desc.hop_t = t.data();
+        desc.phase = phase.empty() ? nullptr : phase.data();
+        present = true;
+    }
+    void clear() {
+        std::vector<int32_t>().swap(from);
+        std::vector<int32_t>().swap(to);
+        std::vector<int32_t>().swap(lv);
+        std::vector<double>().swap(t);
+        std::vector<double>().swap(phase);
+        present = false;
+    }
+};
+
 struct Request {
     int64_t ticket = 0;
     int32_t kind = TBK_REQ_DOS, precision = TBK_FP64, n = 0, R = 0, Rp = 0;
     uint64_t seed = 0;
     double a = 1.0, b = 0.0;
     bool auto_bounds = false;  // DoS submitted with a = 0: a, b from the spectral range at admission
-    HostSparse H, vx, vy;
+    HostSparse H, vx, vy;      // a lattice request keeps only H.hdr's rows, is_complex and an nnz bound
+    HostLattice lat;
     std::vector<double> g;
     int64_t footprint = 0;
     int32_t state = TBK_REQ_QUEUED;
@@ -225,6 +267,9 @@ int64_t tbk_server::dos_footprint(const Request& r) const {
     int64_t vec = 3 * rows * r.Rp * elem;
     // the Lanczos vectors and partials of an auto-bounds request live before the blocks
     if (r.auto_bounds) vec = std::max<int64_t>(vec, 3 * rows * 16 + 64 * 1024);
+    // assembly scratch of a lattice request (row counts, scan, phase table) lives before them too
+    if (r.lat.present)
+        vec = std::max<int64_t>(vec, (rows + 1) * 4 + static_cast<int64_t>(r.lat.phase.size()) * 8 + (1 << 20));
     return mat_bytes(r.H, r.precision == TBK_FP32) + vec + static_cast<int64_t>(r.n) * 8;
 }
 
@@ -299,6 +344,7 @@ void tbk_server::finish(Request* r, int32_t code, const std::string& msg) {
     r->H.clear();
     r->vx.clear();
     r->vy.clear();
+    r->lat.clear();
     done_cv.notify_all();
     prep_gen++;
     cv_prep.notify_all();
@@ -340,8 +386,12 @@ Slot* tbk_server::start_dos(Request* r, int32_t pin) {
     auto s = std::make_unique<Slot>();
     s->req = r;
     s->pin = pin;
-    int rc = tbk_matrix_create(ctx, &r->H.hdr, &s->M);
-    if (rc != TBK_OK) raise(rc, g_last_error);
+    if (r->lat.present) {
+        assemble_matrix(ctx, &r->lat.desc, bstream, &s->M, nullptr);
+    } else {
+        const int rc = tbk_matrix_create(ctx, &r->H.hdr, &s->M);
+        if (rc != TBK_OK) raise(rc, g_last_error);
+    }
     if (r->auto_bounds) {
         // E_min, E_max from the Lanczos range estimate, contracted as
         // GreenFunction(N, HE, ref_HE, epsilon = 0.01) does (lattice.h:560-569)
@@ -759,7 +809,7 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
     return guarded([&] {
         if (!s || !q || !ticket) raise(TBK_E_INVALID, "null argument");
         if (q->kind != TBK_REQ_DOS && q->kind != TBK_REQ_KUBO) raise(TBK_E_INVALID, "unknown request kind");
-        if (!q->H) raise(TBK_E_INVALID, "request without a Hamiltonian");
+        if (!q->H && !(q->kind == TBK_REQ_DOS && q->lattice)) raise(TBK_E_INVALID, "request without a Hamiltonian");
         if (q->R < 1) raise(TBK_E_INVALID, "R must be positive");
         if (!std::isfinite(q->a) || !std::isfinite(q->b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
         if (q->a == 0.0 && q->kind != TBK_REQ_DOS) raise(TBK_E_INVALID, "sigma_xy requests need a, b");
@@ -786,7 +836,18 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
             if (q->n_moments > s->cfg.max_kubo_moments) raise(TBK_E_UNSUPPORTED, "more moments than the server allows");
             if (tbk::kubo_zeta_energies_per_group(q->n_moments) == 0) raise(TBK_E_UNSUPPORTED, "N too large for zeta");
         }
-        r->H.copy_from(q->H);
+        if (q->H) {
+            r->H.copy_from(q->H);
+        } else {
+            // a lattice: the device assembles it at admission; H.hdr carries
+            // the size, the value type and a bound of the stored entries
+            r->lat.copy_from(q->lattice);
+            const int64_t rows = lattice_rows(&r->lat.desc);
+            if (rows > INT32_MAX) raise(TBK_E_UNSUPPORTED, "more than 2^31 - 1 sites");
+            r->H.hdr.rows = r->H.hdr.cols = static_cast<int32_t>(rows);
+            r->H.hdr.nnz = lattice_nnz_bound(&r->lat.desc);
+            r->H.hdr.is_complex = q->lattice->is_complex;
+        }
         if (q->kind == TBK_REQ_KUBO) {
             r->vx.copy_from(q->vx);
             r->vy.copy_from(q->vy);

@@ -18,6 +18,12 @@ struct tbh_cell {
 };
 struct tbh_lattice {
     std::variant<PartialLattice<1>, PartialLattice<2>, PartialLattice<3>> v;
+    // fields set through this interface (the device builder reproduces them)
+    int32_t gauge = 0;
+    double B = 0.0;
+    bool disorder = false;
+    double W = 0.0;
+    uint64_t seed = 0;
 };
 struct tbh_csc {
     std::variant<SparseMatrix<double>, SparseMatrix<Complex>> v;
@@ -208,6 +214,8 @@ int tbh_lattice_set_gauge(tbh_lattice* l, int32_t kind, double B) {
                     lat.gauge_field = [B](Vector3d r) { return B * Vector3d{-r(1), r(0), 0} / 2; };
                 else
                     bad("unknown gauge kind");
+                l->gauge = kind;
+                l->B = B;
             },
             l->v);
     });
@@ -217,6 +225,70 @@ int tbh_lattice_set_disorder(tbh_lattice* l, double W, uint64_t seed) {
     return guarded([&] {
         if (!l) bad("invalid argument");
         std::visit([&](auto& lat) { lat.set_scalar_field_indexed(uniform_disorder(W, seed)); }, l->v);
+        l->disorder = true;
+        l->W = W;
+        l->seed = seed;
+    });
+}
+
+int tbh_lattice_info_get(const tbh_lattice* l, tbh_lattice_info* out) {
+    return guarded([&] {
+        if (!l || !out) bad("invalid argument");
+        *out = tbh_lattice_info{};
+        std::visit(
+            [&](const auto& lat) {
+                constexpr int D = std::remove_reference_t<decltype(lat)>::dim;
+                out->dim = D;
+                out->n_atoms = lat.cell().get_atom_number();
+                out->n_hops = static_cast<int32_t>(lat.cell().hopping_view().size());
+                for (int d = 0; d < 3; ++d) out->span[d] = d < D ? lat.span()(d) : 1;
+                out->cell_number = lat.get_cell_number();
+                out->total_number = lat.get_total_number();
+                out->removed_sites = lat.removed_count();
+                out->position_field = lat.has_position_field() ? 1 : 0;
+            },
+            l->v);
+        out->gauge = l->gauge;
+        out->B = l->B;
+        out->has_disorder = l->disorder ? 1 : 0;
+        out->disorder_W = l->W;
+        out->disorder_seed = l->seed;
+    });
+}
+
+int tbh_lattice_hoppings(const tbh_lattice* l, int32_t* from, int32_t* to, int32_t* lattice_vector, double* t) {
+    return guarded([&] {
+        if (!l || !from || !to || !lattice_vector || !t) bad("invalid argument");
+        std::visit(
+            [&](const auto& lat) {
+                constexpr int D = std::remove_reference_t<decltype(lat)>::dim;
+                size_t i = 0;
+                for (const auto& [hp, tv] : lat.cell().hopping_view()) {
+                    from[i] = std::get<0>(hp);
+                    to[i] = std::get<1>(hp);
+                    for (int d = 0; d < D; ++d) lattice_vector[i * D + d] = std::get<2>(hp)(d);
+                    t[2 * i] = tv.real();
+                    t[2 * i + 1] = tv.imag();
+                    ++i;
+                }
+            },
+            l->v);
+    });
+}
+
+int tbh_lattice_phase_table(const tbh_lattice* l, double* out) {
+    return guarded([&] {
+        if (!l || !out) bad("invalid argument");
+        if (l->gauge == 0) bad("no gauge field is set");
+        std::visit(
+            [&](const auto& lat) {
+                const std::vector<Complex> p = lat.phase_table();
+                for (size_t i = 0; i < p.size(); ++i) {
+                    out[2 * i] = p[i].real();
+                    out[2 * i + 1] = p[i].imag();
+                }
+            },
+            l->v);
     });
 }
 
@@ -295,7 +367,7 @@ int tbh_csc_create(int32_t n, int64_t nnz, const int32_t* colptr, const int32_t*
             M.outer().assign(colptr, colptr + n + 1);
             M.inner().assign(rowidx, rowidx + nnz);
             M.values().resize(static_cast<size_t>(nnz));
-            std::memcpy(M.values().data(), values, static_cast<size_t>(nnz) * sizeof(S));
+            std::memcpy(static_cast<void*>(M.values().data()), values, static_cast<size_t>(nnz) * sizeof(S));
         };
         if (is_complex) {
             SparseMatrix<Complex> M(n, n);

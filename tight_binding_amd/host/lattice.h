@@ -207,6 +207,29 @@ public:
     std::function<Vector3d(Vector3d)> gauge_field;
     inline static bool auto_info = false;
 
+    // exp(-i A(r_from) . d_h) for every (cell, hopping), cells in index order
+    // and hoppings in table order: the factor the open and periodic builders
+    // apply (lattice.h:282-285), computed by the same expressions.
+    std::vector<Complex> phase_table() const {
+        const auto& hops = cell_.hopping_view();
+        std::vector<Complex> out;
+        if (!gauge_field) return out;
+        out.reserve(static_cast<size_t>(cell_number) * hops.size());
+        std::vector<Vector3d> disp(hops.size());
+        for (size_t h = 0; h < hops.size(); ++h) disp[h] = cell_.displacement_of_hopping(hops[h].first);
+        for (int c = 0; c < cell_number; ++c) {
+            const LatticeVector<Dim> i_v = cell_vector(c);
+            for (size_t h = 0; h < hops.size(); ++h) {
+                Vector3d A = gauge_field(cell_.atom_position(std::get<0>(hops[h].first), i_v));
+                double theta = A.dot(disp[h]);
+                out.push_back(std::exp(Complex(0.0, -theta)));
+            }
+        }
+        return out;
+    }
+    bool has_position_field() const { return static_cast<bool>(scalar_field); }
+    int removed_count() const { return static_cast<int>(removed_atoms.size()); }
+
     template <typename Scalar>
     struct GreenFunction;
 

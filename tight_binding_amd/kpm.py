@@ -21,6 +21,10 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
 def device_count() -> int:
     n = C.c_int32()
     N.check_kpm(N.kpm().tbk_device_count(C.byref(n)))
@@ -36,21 +40,34 @@ def _as_colmajor_c128(a: np.ndarray) -> np.ndarray:
 
 
 class Matrix:
-    def __init__(self, ctx: "Context", m: CSC):
+    def __init__(self, ctx: "Context", m: CSC = None, handle=None, is_complex: bool = False):
+        """Upload a CSC matrix, or wrap a resident matrix handle (assembly on
+        the device)."""
         self.ctx = ctx
-        self.n = int(m.n)
-        self.nnz = int(np.asarray(m.rowidx).size)
-        self.is_complex = bool(np.iscomplexobj(m.values))
-        s, keep = tbk_sparse_of(m)
-        h = C.c_void_p()
-        N.check_kpm(N.kpm().tbk_matrix_create(ctx._h, C.byref(s), C.byref(h)))
-        del keep
-        self._h = h
+        if handle is None:
+            s, keep = tbk_sparse_of(m)
+            handle = C.c_void_p()
+            N.check_kpm(N.kpm().tbk_matrix_create(ctx._h, C.byref(s), C.byref(handle)))
+            del keep
+            is_complex = bool(np.iscomplexobj(m.values))
+        self._h = handle
+        self.is_complex = bool(is_complex)
+        inf = self.info()
+        self.n, self.nnz = inf["rows"], inf["nnz"]
 
     def info(self) -> dict:
         rows, nnz, mx, bw = C.c_int32(), C.c_int64(), C.c_int32(), C.c_int32()
         N.check_kpm(N.kpm().tbk_matrix_info(self._h, C.byref(rows), C.byref(nnz), C.byref(mx), C.byref(bw)))
         return {"rows": rows.value, "nnz": nnz.value, "max_row_nnz": mx.value, "bandwidth95": bw.value}
+
+    def download(self):
+        """(row pointers, column indices, values) of the resident CSR matrix."""
+        rp = np.empty(self.n + 1, dtype=np.int32)
+        ci = np.empty(max(self.nnz, 1), dtype=np.int32)
+        v = np.empty(max(self.nnz, 1) * (2 if self.is_complex else 1), dtype=np.float64)
+        N.check_kpm(N.kpm().tbk_matrix_download(self._h, _ip(rp), _ip(ci), _dp(v)))
+        vals = v[: self.nnz * (2 if self.is_complex else 1)]
+        return rp, ci[: self.nnz], (vals.view(np.complex128) if self.is_complex else vals)
 
     def close(self):
         if getattr(self, "_h", None) is not None:
@@ -148,6 +165,19 @@ class Context:
     def matrix(self, m: CSC) -> Matrix:
         return Matrix(self, m)
 
+    def assemble(self, lattice, builder: int = 1, extend=None, is_complex: bool = False) -> Matrix:
+        """The lattice's Hamiltonian assembled on the device (tbk_matrix_assemble):
+        the matrix of lattice.get_Hamiltonian_sparse (builder 0),
+        get_Hamiltonian_periodic_extend_bare_sparse(extend) (1) or
+        get_Hamiltonian_bare_sparse (2), bit for bit, never formed on the host.
+        Returns (Matrix, wall ms)."""
+        d, keep = lattice_desc_of(lattice, builder, extend, is_complex)
+        h = C.c_void_p()
+        ms = C.c_double()
+        N.check_kpm(N.kpm().tbk_matrix_assemble(self._h, C.byref(d), C.byref(h), C.byref(ms)))
+        del keep
+        return Matrix(self, handle=h, is_complex=is_complex), ms.value
+
     def spectral_bounds(self, M: Matrix, max_iters: int = 1000, tol: float = 1e-8, seed: int = 1) -> dict:
         """Extreme eigenvalues of the Hermitian matrix M by FP64 Lanczos on the
         device (tbk_spectral_bounds): ritz_min/max, their residual bounds,
@@ -220,6 +250,39 @@ def stack_contract(ctx: Context, A: np.ndarray, B: np.ndarray, precision: int = 
     N.check_kpm(N.kpm().tbk_stack_contract(ctx._h, _dp(A.view(np.float64)), _dp(B.view(np.float64)), n, k,
                                            int(precision), _dp(out), C.byref(ms)))
     return out.view(np.complex128).reshape(n, n), ms.value
+
+
+def lattice_desc_of(lattice, builder: int = 1, extend=None, is_complex: bool = False):
+    """tbk_lattice_desc of a lattice.PartialLattice (its hopping table, the
+    host's phase table when a gauge field is set, the site-keyed disorder);
+    the returned tuple keeps the arrays alive while the descriptor is used."""
+    info = N.tbh_lattice_info()
+    N.check_host(N.host().tbh_lattice_info_get(lattice._h, C.byref(info)))
+    if info.position_field:
+        raise N.NativeError(N.TBK_E_UNSUPPORTED, "a position-dependent scalar field cannot be assembled on the device")
+    if builder == 0 and info.removed_sites:
+        raise N.NativeError(N.TBK_E_UNSUPPORTED, "removed sites cannot be assembled on the device")
+    nh, dim = info.n_hops, info.dim
+    fr = np.zeros(max(nh, 1), dtype=np.int32)
+    to = np.zeros(max(nh, 1), dtype=np.int32)
+    lv = np.zeros(max(nh * dim, 1), dtype=np.int32)
+    t = np.zeros(max(2 * nh, 2), dtype=np.float64)
+    if nh:
+        N.check_host(N.host().tbh_lattice_hoppings(lattice._h, _ip(fr), _ip(to), _ip(lv), _dp(t)))
+    phase = None
+    if info.gauge != 0 and builder != 2:
+        phase = np.zeros(max(2 * info.cell_number * nh, 2), dtype=np.float64)
+        N.check_host(N.host().tbh_lattice_phase_table(lattice._h, _dp(phase)))
+    d = N.tbk_lattice_desc()
+    d.dim = dim
+    for k in range(3):
+        d.span[k] = info.span[k]
+        d.extend[k] = int(extend[k]) if (extend is not None and k < dim) else 0
+    d.n_atoms, d.n_hops, d.builder, d.is_complex = info.n_atoms, nh, int(builder), int(bool(is_complex))
+    d.hop_from, d.hop_to, d.hop_lv, d.hop_t = _ip(fr), _ip(to), _ip(lv), _dp(t)
+    d.phase = _dp(phase) if phase is not None else None
+    d.has_disorder, d.disorder_W, d.disorder_seed = info.has_disorder, info.disorder_W, info.disorder_seed
+    return d, (fr, to, lv, t, phase)
 
 
 def reconstruct(ctx, mu, g, e, scale: float = 1.0):
@@ -365,12 +428,22 @@ class Server:
         del keep
         return t.value
 
-    def submit_dos(self, H: CSC, n: int, a: float, b: float = 0.0, R: int = 8, seed: int = 0,
-                   precision: int = FP64) -> int:
+    def submit_dos(self, H, n: int, a: float, b: float = 0.0, R: int = 8, seed: int = 0,
+                   precision: int = FP64, builder: int = 1, extend=None, is_complex: bool = False) -> int:
         """Moments mu_0..mu_{n-1} of (H - b)/a from R device-generated random-phase vectors.
-        a = 0: the server estimates the bounds at admission (Lanczos range,
-        contracted by 0.01 as GreenFunction(N, HE, ref_HE) does); the a, b it
-        used are in info()["a"], info()["b"]."""
+        H: a CSC matrix, or a lattice.PartialLattice that the server assembles
+        on the device at admission (builder, extend, is_complex as for
+        Context.assemble).  a = 0: the server estimates the bounds at
+        admission (Lanczos range, contracted by 0.01 as
+        GreenFunction(N, HE, ref_HE) does); the a, b it used are in
+        info()["a"], info()["b"]."""
+        if hasattr(H, "get_total_number"):  # a lattice
+            d, keep = lattice_desc_of(H, builder, extend, is_complex)
+            req = N.tbk_request(N.TBK_REQ_DOS, int(precision), int(n), int(R), int(seed), float(a), float(b),
+                                None, None, None, None, C.pointer(d))
+            t = self._submit(req, (d, keep))
+            self._kind[t] = ("dos", int(n))
+            return t
         sH, kH = tbk_sparse_of(H)
         req = N.tbk_request(N.TBK_REQ_DOS, int(precision), int(n), int(R), int(seed), float(a), float(b),
                             C.pointer(sH), None, None, None)
