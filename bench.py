@@ -637,8 +637,14 @@ def main():
         nbytes = 2.0 * H.n * args.R * 16  # read + write
         times = [K.copy_to(K2) for _ in range(args.copy_probe)]
         copy_gbps = nbytes / (min(times) * 1e-3) / 1e9
-        log("block copy: %.3f ms best of %d, %.1f GB/s (read+write)" % (min(times), len(times), copy_gbps))
+        log("block copy (tbk_block_copy, plain stores): %.3f ms best of %d, %.1f GB/s (read+write)"
+            % (min(times), len(times), copy_gbps))
         K2.close()
+    # attainable references on buffers of one vector block: a non-temporal
+    # copy, the step's two-reads-one-write mix without gathers, a read
+    attain = ctx.bandwidth_probe(H.n * args.R * 16, repeats=5)
+    log("attainable: copy %.1f, 2 reads + 1 write %.1f, read %.1f GB/s"
+        % (attain["copy_GBps"], attain["stream_2r1w_GBps"], attain["read_GBps"]))
 
     # launch-shape autotuning for this matrix and block (outside the timed region)
     tuning = None
@@ -746,6 +752,11 @@ def main():
         },
         "cpu_baseline": cpu,
         "copy_GBps_same_block": copy_gbps,
+        "attainable_bw": {
+            "stream_2r1w_GBps": attain["stream_2r1w_GBps"], "copy_nt_GBps": attain["copy_GBps"],
+            "read_GBps": attain["read_GBps"], "buffer_bytes": attain["bytes"],
+            "step_over_stream_2r1w": achieved / attain["stream_2r1w_GBps"],
+        },
     }
     if extra:
         out["variants"] = extra

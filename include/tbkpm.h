@@ -184,6 +184,16 @@ int tbk_block_create(tbk_context* ctx, const double* colmajor, int32_t rows, int
    generated on the device */
 int tbk_block_random_phase(tbk_context* ctx, int32_t rows, int32_t R, uint64_t seed,
                            int32_t precision, tbk_block** out);
+/* Attainable-bandwidth references on buffers of `bytes` each, best of
+   `repeats`, all with non-temporal accesses: a copy (read + write), the
+   two-reads-one-write mix of a recursion step without its gathers
+   (y = a + b), and a read-only reduction.  GB/s counts every byte moved. */
+typedef struct tbk_bandwidth {
+    double copy_GBps, stream_2r1w_GBps, read_GBps;
+    double bytes; /* per buffer */
+} tbk_bandwidth;
+int tbk_bandwidth_probe(tbk_context* ctx, int64_t bytes, int32_t repeats, tbk_bandwidth* out);
+
 /* copy a block back as rows x R column-major complex128 */
 int tbk_block_download(const tbk_block* b, double* colmajor_out);
 /* device copy dst = src (same shape and precision); kernel_ms (optional)

@@ -301,3 +301,18 @@ def test_autotune_installs_a_valid_tuning(gpu_ctx):
         assert_moments_close(mu, ref, tol=1e-12)
     finally:
         gpu_ctx.set_tuning()
+
+
+def test_bandwidth_probe(gpu_ctx):
+    """The attainable-bandwidth references run and give physical figures:
+    positive, below the 8 TB/s HBM peak (the buffers exceed the caches), and
+    reading alone at least as fast as the copy."""
+    from tight_binding_amd import _native as Nn
+
+    b = gpu_ctx.bandwidth_probe(1 << 30, repeats=2)
+    assert b["bytes"] == float(1 << 30)
+    for k in ("copy_GBps", "stream_2r1w_GBps", "read_GBps"):
+        assert 100.0 < b[k] < 8000.0, (k, b)
+    assert b["read_GBps"] >= 0.9 * b["copy_GBps"]
+    with pytest.raises(Nn.NativeError):
+        gpu_ctx.bandwidth_probe(1024)

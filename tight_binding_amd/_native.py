@@ -45,6 +45,14 @@ class tbk_sparse(C.Structure):
     ]
 
 
+class tbk_bandwidth(C.Structure):
+    _fields_ = [("copy_GBps", C.c_double), ("stream_2r1w_GBps", C.c_double), ("read_GBps", C.c_double),
+                ("bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class tbk_lattice_desc(C.Structure):
     _fields_ = [
         ("dim", C.c_int32),
@@ -281,6 +289,7 @@ KPM_SIGNATURES = {
     "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
     "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
     "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),
+    "tbk_bandwidth_probe": ([P, I64, I32, C.POINTER(tbk_bandwidth)], I32),
     "tbk_matrix_assemble": ([P, C.POINTER(tbk_lattice_desc), PP, DP], I32),
     "tbk_matrix_download": ([P, IP, IP, DP], I32),
     "tbk_cheb_reconstruct": ([P, DP, DP, I32, DP, I32, D, DP, DP], I32),

@@ -91,6 +91,57 @@ __global__ __launch_bounds__(kBlock) void copy16_kernel(const uint4* __restrict_
     }
 }
 
+// Bandwidth references with non-temporal accesses (grid-stride, 4 elements
+// per thread in flight): a copy, the two-reads-one-write mix of a recursion
+// step without its gathers (y = a + b), and a read-only reduction.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(kBlock) void stream_copy_kernel(const nt_d2* __restrict__ a, nt_d2* __restrict__ y,
+                                                             int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock * 4 + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock * 4) {
+        nt_d2 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * kBlock < n) v[k] = __builtin_nontemporal_load(a + i + k * kBlock);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * kBlock < n) __builtin_nontemporal_store(v[k], y + i + k * kBlock);
+    }
+}
+__global__ __launch_bounds__(kBlock) void stream_2r1w_kernel(const nt_d2* __restrict__ a, const nt_d2* __restrict__ b,
+                                                             nt_d2* __restrict__ y, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock * 4 + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock * 4) {
+        nt_d2 u[4], w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * kBlock < n) {
+                u[k] = __builtin_nontemporal_load(a + i + k * kBlock);
+                w[k] = __builtin_nontemporal_load(b + i + k * kBlock);
+            }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * kBlock < n) __builtin_nontemporal_store(u[k] + w[k], y + i + k * kBlock);
+    }
+}
+__global__ __launch_bounds__(kBlock) void stream_read_kernel(const nt_d2* __restrict__ a, double* __restrict__ out,
+                                                             int64_t n) {
+    double s = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const nt_d2 v = __builtin_nontemporal_load(a + i);
+        s = s + (v.x + v.y);
+    }
+    __shared__ double sh[kBlock];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w >= 1; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
 // fp64 -> fp32 copy of matrix values
 __global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double* __restrict__ in,
                                                          float* __restrict__ out, int64_t n) {

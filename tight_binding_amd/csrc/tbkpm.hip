@@ -873,6 +873,58 @@ int tbk_block_copy(const tbk_block* src, tbk_block* dst, float* kernel_ms) {
     });
 }
 
+int tbk_bandwidth_probe(tbk_context* ctx, int64_t bytes, int32_t repeats, tbk_bandwidth* out) {
+    return guarded([&] {
+        if (!ctx || !out) raise(TBK_E_INVALID, "null argument");
+        if (bytes < 16 * 1024 || repeats < 1) raise(TBK_E_INVALID, "bytes must be >= 16 KiB and repeats >= 1");
+        set_device(ctx);
+        const int64_t n = bytes / 16;
+        DevBuf a, b, y, r;
+        a.reserve(static_cast<size_t>(n) * 16);
+        b.reserve(static_cast<size_t>(n) * 16);
+        y.reserve(static_cast<size_t>(n) * 16);
+        const unsigned g4 = static_cast<unsigned>(ctx->num_cu) * 4, g8 = static_cast<unsigned>(ctx->num_cu) * 8;
+        r.reserve(static_cast<size_t>(g8) * 8);
+        hipStream_t st = ctx->stream;
+        HIPC(hipMemsetAsync(a.p, 0, static_cast<size_t>(n) * 16, st));
+        HIPC(hipMemsetAsync(b.p, 0, static_cast<size_t>(n) * 16, st));
+        ensure_events(ctx, 2);
+        auto best_ms = [&](auto launch) {
+            launch();  // warm
+            float best = 1e30f;
+            for (int k = 0; k < repeats; ++k) {
+                HIPC(hipEventRecord(ctx->events[0], st));
+                launch();
+                HIPC(hipEventRecord(ctx->events[1], st));
+                HIPC(hipEventSynchronize(ctx->events[1]));
+                float ms = 0.f;
+                HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+                best = std::min(best, ms);
+            }
+            return static_cast<double>(best);
+        };
+        using tbk::nt_d2;
+        const double tc = best_ms([&] {
+            hipLaunchKernelGGL(tbk::stream_copy_kernel, dim3(g4), dim3(tbk::kBlock), 0, st, a.as<const nt_d2>(),
+                               y.as<nt_d2>(), n);
+        });
+        const double t2 = best_ms([&] {
+            hipLaunchKernelGGL(tbk::stream_2r1w_kernel, dim3(g4), dim3(tbk::kBlock), 0, st, a.as<const nt_d2>(),
+                               b.as<const nt_d2>(), y.as<nt_d2>(), n);
+        });
+        const double tr = best_ms([&] {
+            hipLaunchKernelGGL(tbk::stream_read_kernel, dim3(g8), dim3(tbk::kBlock), 0, st, a.as<const nt_d2>(),
+                               r.as<double>(), n);
+        });
+        HIPC(hipGetLastError());
+        const double B = static_cast<double>(n) * 16.0;
+        out->copy_GBps = 2.0 * B / (tc * 1e-3) / 1e9;
+        out->stream_2r1w_GBps = 3.0 * B / (t2 * 1e-3) / 1e9;
+        out->read_GBps = B / (tr * 1e-3) / 1e9;
+        out->bytes = B;
+    });
+}
+
 int tbk_block_destroy(tbk_block* b) {
     return guarded([&] {
         if (!b) return;

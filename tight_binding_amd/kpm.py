@@ -178,6 +178,13 @@ class Context:
         del keep
         return Matrix(self, handle=h, is_complex=is_complex), ms.value
 
+    def bandwidth_probe(self, nbytes: int, repeats: int = 3) -> dict:
+        """Attainable HBM references on buffers of nbytes (tbk_bandwidth_probe):
+        non-temporal copy, two-reads-one-write stream, read-only, in GB/s."""
+        b = N.tbk_bandwidth()
+        N.check_kpm(N.kpm().tbk_bandwidth_probe(self._h, int(nbytes), int(repeats), C.byref(b)))
+        return b.as_dict()
+
     def spectral_bounds(self, M: Matrix, max_iters: int = 1000, tol: float = 1e-8, seed: int = 1) -> dict:
         """Extreme eigenvalues of the Hermitian matrix M by FP64 Lanczos on the
         device (tbk_spectral_bounds): ritz_min/max, their residual bounds,
