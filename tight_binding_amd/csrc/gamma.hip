@@ -141,7 +141,7 @@ __device__ __forceinline__ bool xcd_work_item(const GammaMap& m, int& tn, int& t
 constexpr int kBigKt = 4;               // complex k per LDS stage
 constexpr int kBigStride = kBigKt + 1;  // padded LDS row stride (double2)
 
-__global__ __launch_bounds__(256) void gamma_f64_big_kernel(const double2* __restrict__ A,
+__global__ __launch_bounds__(512) void gamma_f64_big_kernel(const double2* __restrict__ A,
                                                             const double2* __restrict__ B, int N, int64_t K,
                                                             int64_t kc, GammaMap map, double2* __restrict__ P) {
     __shared__ double2 As[2][kBigTile * kBigStride];
@@ -149,51 +149,45 @@ __global__ __launch_bounds__(256) void gamma_f64_big_kernel(const double2* __res
     int tn, tm;
     int64_t c;
     if (!xcd_work_item(map, tn, tm, c)) return;  // uniform over the workgroup
+    // 8 waves: wave w owns the 64 x 32 block (rows (w >> 2) * 64, columns
+    // (w & 3) * 32) of the 128 x 128 tile, as 4 x 2 fragments of 16 x 16
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int wn = wave >> 1, wm = wave & 1;
+    const int wn = wave >> 2, wm = wave & 3;
     const int n0 = tn * kBigTile, m0 = tm * kBigTile;
     const int64_t kb = c * kc;
     const int64_t ke = (kb + kc < K) ? kb + kc : K;
 
-    // staging slots: thread t loads rows (t + 256 i) / 4, column (t % 4)
-    const double2* ga[2];
-    const double2* gb[2];
-    int soff[2];
-    const int cc = tid & 3;
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int r = (tid + 256 * i) >> 2;
-        const int rn = (n0 + r < N) ? n0 + r : N - 1;
-        const int rm = (m0 + r < N) ? m0 + r : N - 1;
-        ga[i] = A + static_cast<int64_t>(rn) * K + cc;
-        gb[i] = B + static_cast<int64_t>(rm) * K + cc;
-        soff[i] = r * kBigStride + cc;
-    }
-    double2 ra[2], rb[2];
+    // staging: thread t loads row t / 4, column t % 4 of each operand
+    const int cc = tid & 3, r = tid >> 2;
+    const int rn = (n0 + r < N) ? n0 + r : N - 1;
+    const int rm = (m0 + r < N) ? m0 + r : N - 1;
+    const double2* ga = A + static_cast<int64_t>(rn) * K + cc;
+    const double2* gb = B + static_cast<int64_t>(rm) * K + cc;
+    const int soff = r * kBigStride + cc;
+    double2 ra, rb;
     auto gload = [&](int64_t k0) {
         const bool ok = k0 + cc < ke;
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            ra[i] = ld_or_zero(ga[i] + k0, ok);
-            rb[i] = ld_or_zero(gb[i] + k0, ok);
-        }
+        ra = ld_or_zero(ga + k0, ok);
+        rb = ld_or_zero(gb + k0, ok);
     };
     auto sstore = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            As[buf][soff[i]] = ra[i];
-            Bs[buf][soff[i]] = rb[i];
-        }
+        As[buf][soff] = ra;
+        Bs[buf][soff] = rb;
     };
 
-    v4d re[4][4], im[4][4];
+    // Three real products per complex one (Gauss):
+    //   t1 = Ar.Br, t2 = Ai.Bi, t3 = (Ar - Ai).(Br + Bi),
+    //   Re = t1 + t2, Im = t3 - t1 + t2 = Ar.Bi - Ai.Br,
+    // 24 MFMAs per stage and wave where the four-product form needs 32.
+    v4d t1[4][2], t2[4][2], t3[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            re[i][j] = v4d{0, 0, 0, 0};
-            im[i][j] = v4d{0, 0, 0, 0};
+        for (int j = 0; j < 2; ++j) {
+            t1[i][j] = v4d{0, 0, 0, 0};
+            t2[i][j] = v4d{0, 0, 0, 0};
+            t3[i][j] = v4d{0, 0, 0, 0};
         }
 
     gload(kb);
@@ -203,20 +197,23 @@ __global__ __launch_bounds__(256) void gamma_f64_big_kernel(const double2* __res
     for (int64_t k0 = kb; k0 < ke; k0 += kBigKt) {
         const bool more = k0 + kBigKt < ke;
         if (more) gload(k0 + kBigKt);
-        double2 a[4], b[4];
+        double2 a[4], b[2];
+        double bs[2];
 #pragma unroll
         for (int i = 0; i < 4; ++i) a[i] = As[buf][(wn * 64 + 16 * i + li) * kBigStride + lk];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = Bs[buf][(wm * 64 + 16 * j + li) * kBigStride + lk];
+        for (int j = 0; j < 2; ++j) {
+            b[j] = Bs[buf][(wm * 32 + 16 * j + li) * kBigStride + lk];
+            bs[j] = b[j].x + b[j].y;
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const double ay_neg = -a[i].y;
+            const double ad = a[i].x - a[i].y;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, re[i][j], 0, 0, 0);
-                im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].y, im[i][j], 0, 0, 0);
-                re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, re[i][j], 0, 0, 0);
-                im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ay_neg, b[j].x, im[i][j], 0, 0, 0);
+            for (int j = 0; j < 2; ++j) {
+                t1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, t1[i][j], 0, 0, 0);
+                t2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, t2[i][j], 0, 0, 0);
+                t3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bs[j], t3[i][j], 0, 0, 0);
             }
         }
         if (more) sstore(buf ^ 1);
@@ -228,12 +225,14 @@ __global__ __launch_bounds__(256) void gamma_f64_big_kernel(const double2* __res
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int row = n0 + wn * 64 + 16 * i + 4 * t + lk;
-                const int col = m0 + wm * 64 + 16 * j + li;
-                if (row < N && col < N) Pc[static_cast<int64_t>(row) * N + col] = make_double2(re[i][j][t], im[i][j][t]);
+                const int col = m0 + wm * 32 + 16 * j + li;
+                const double p1 = t1[i][j][t], p2 = t2[i][j][t];
+                if (row < N && col < N)
+                    Pc[static_cast<int64_t>(row) * N + col] = make_double2(p1 + p2, (t3[i][j][t] - p1) + p2);
             }
 }
 
@@ -616,7 +615,7 @@ void launch_gamma_reduce(const double2* P, int nchunks, int64_t NN, double2* out
 
 void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
                           double2* P, hipStream_t st) {
-    hipLaunchKernelGGL(gamma_f64_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(256), 0, st, A, B, N,
+    hipLaunchKernelGGL(gamma_f64_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0, st, A, B, N,
                        K, kc, map, P);
 }
 
