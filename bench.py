@@ -80,6 +80,9 @@ def kubo_leg(ctx, cells, N, R, repeats):
     log("kubo: assembled and uploaded n=%d nnz=%d a=%.4f in %.1f s" % (H.n, H.nnz, solver.a, time.perf_counter() - t0))
     ef = np.linspace(-4.0, 4.0, 201)
     solver.solve(16, R=R, seed=1, fermi_energies=ef)  # loads the kernels
+    saved = ctx.get_tuning()
+    tuning, tms = solver.tune(R)  # launch shape of the stack recursion (restored after the leg)
+    log("kubo: recursion tuning %s, %.3f ms per step" % (json.dumps(tuning), tms))
     out = {
         "workload": "Kubo-Bastin sigma_xy, square %dx%d open (%d sites), Landau flux 1/64 per plaquette, W=2, "
                     "R=%d, N=%d, Lorentz lambda=6, 201 Fermi energies in [-4, 4]" % (cells, cells, H.n, R, N),
@@ -120,6 +123,8 @@ def kubo_leg(ctx, cells, N, R, repeats):
     s64, sbf = sigmas["fp64"], sigmas["bf16x3"]
     out["bf16x3"]["sigma_rel_err_vs_fp64"] = float(np.max(np.abs(sbf - s64)) / np.max(np.abs(s64)))
     out["sigma_fp64_sample"] = [float(x) for x in s64[::25]]
+    out["launch_tuning"] = tuning
+    ctx.set_tuning(**saved)
     solver.close()
     return out
 

@@ -161,3 +161,28 @@ def test_sigma_full_basis_equals_exact_diagonalisation(gpu_ctx):
     assert scale > 0
     assert np.max(np.abs(I - I_ref)) <= 1e-9 * scale
     assert np.max(np.abs(sig - sig_ref)) <= 1e-9 * scale
+
+
+def test_tuned_solver_gives_the_same_gamma(gpu_ctx):
+    """KuboSolver.tune changes only the launch shape of the stack recursion:
+    every stored T_n is formed by the same per-row sums, so Gamma is the same
+    bits before and after."""
+    from tight_binding_amd import kubo
+
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (24, 24))
+    lat.set_gauge(L.GAUGE_LANDAU, 2 * np.pi / 24)
+    lat.set_disorder(1.0, 4)
+    H = lat.get_Hamiltonian_sparse(True)
+    saved = gpu_ctx.get_tuning()
+    s = kubo.KuboSolver(gpu_ctx, H, lat.op_position(0), lat.op_position(1), -5.1, 5.1, volume=float(H.n))
+    try:
+        g0 = s.solve(64, R=8, seed=3)["gamma"]
+        tuning, ms = s.tune(R=8, steps=4)
+        assert ms > 0 and tuning["row_cpl"] in (0, 1, 2, 4)
+        g1 = s.solve(64, R=8, seed=3)["gamma"]
+        assert np.array_equal(g0, g1)
+    finally:
+        s.close()
+        gpu_ctx.set_tuning(**saved)

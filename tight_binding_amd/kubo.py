@@ -46,6 +46,16 @@ class KuboSolver:
         for m in (self.MH, self.MX, self.MY):
             m.close()
 
+    def tune(self, R=4, steps=16):
+        """Autotune the context's launch shape for this Hamiltonian's
+        recursion steps at R vectors (tbk_autotune); the tuning stays
+        installed on the context.  Returns (tuning, ms per step)."""
+        K = self.ctx.random_block(self.n, R, seed=1)
+        try:
+            return self.ctx.autotune(self.MH, K, steps=steps)
+        finally:
+            K.close()
+
     def zeta(self, gamma, e, volume):
         return kpm.kubo_zeta(self.ctx, gamma, e, volume)[0]
 
