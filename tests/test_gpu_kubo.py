@@ -186,3 +186,41 @@ def test_tuned_solver_gives_the_same_gamma(gpu_ctx):
     finally:
         s.close()
         gpu_ctx.set_tuning(**saved)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 40])
+def test_fused_velocity_stacks_are_the_same_bits(gpu_ctx, N):
+    """When v_x has H~'s pattern (v_x = [X, H~]) the right stack forms
+    v_x T_{m-1} inside the step that makes T_m; when v_x is uploaded with a
+    different pattern (an explicit zero dropped), the separate launches run.
+    Both give the same Gamma bits, and both match the oracle's stacks."""
+    import scipy.sparse as sp
+
+    from tight_binding_amd import sparse as SP
+
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (14, 11))
+    lat.set_gauge(L.GAUGE_LANDAU, 2 * np.pi / 14)
+    lat.set_disorder(1.5, 6)
+    H = lat.get_Hamiltonian_sparse(True)
+    Hs = L.rescale(H, 5.0, 0.1)
+    vx = L.commutator_diag(lat.op_position(0), Hs)
+    vy = L.commutator_diag(lat.op_position(1), Hs)
+    # the same operator without its stored zeros: another pattern
+    csc = sp.csc_matrix((vx.values.copy(), vx.rowidx.copy(), vx.colptr.copy()), shape=(vx.n, vx.n))
+    csc.eliminate_zeros()
+    vx2 = SP.CSC(vx.n, csc.indptr.astype(np.int32), csc.indices.astype(np.int32), csc.data.astype(np.complex128))
+    assert vx2.values.size < vx.values.size
+    K = gpu_ctx.random_block(Hs.n, 4, seed=21)
+    g1, st1 = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy), K, N)
+    g2, st2 = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx2), gpu_ctx.matrix(vy), K, N)
+    assert np.array_equal(g1, g2)
+    kets = K.download()
+    TL = oracle.cheb_on_bases(Hs, N, kets)
+    w = oracle.cheb_on_bases(vy, 2, kets)[1]
+    TR = oracle.cheb_on_bases(Hs, N, w)
+    aR = np.stack([oracle.cheb_on_bases(vx, 2, TR[m])[1] for m in range(N)])
+    ref = SO.kubo_gamma(TL, aR)
+    assert np.max(np.abs(g1 - ref)) <= 1e-12 * np.max(np.abs(ref))
+    K.close()

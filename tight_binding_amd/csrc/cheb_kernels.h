@@ -31,6 +31,8 @@ enum StepMode : int {
     kFirst = 0,   // T1 = H~ T0;           partials (Re<bra|T0>, Re<bra|T1>)
     kRec = 1,     // T_{k+1} = 2H~T_k - T_{k-1}; partial Re<bra|T_{k+1}>
     kRecDbl = 2,  // same update;          partials (Re<T_k|T_k>, Re<T_{k+1}|T_k>)
+    kDual = 3,    // kRecDbl, and y2 = H2 T_k for a second matrix H2 with the same
+                  // pattern (values hv2), formed from the same gathered rows
 };
 
 template <typename VT>
@@ -169,16 +171,20 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
                                          const typename Vec2<VT>::type* __restrict__ x,
                                          const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y,
                                          const typename Vec2<VT>::type* __restrict__ bra, int r, int q, int32_t Rp,
-                                         VT alpha, VT beta, int32_t opts, double& p0, double& p1) {
+                                         VT alpha, VT beta, int32_t opts, double& p0, double& p1,
+                                         const typename HVal<VT, CH>::type* __restrict__ hv2 = nullptr,
+                                         typename Vec2<VT>::type* y2 = nullptr) {
     using V2 = typename Vec2<VT>::type;
     using H = typename HVal<VT, CH>::type;
     const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * CPL;
 
-    V2 zv[CPL], bv[CPL], acc[CPL], xs[CPL];
+    V2 zv[CPL], bv[CPL], acc[CPL], acc2[CPL], xs[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
         acc[c].x = 0;
         acc[c].y = 0;
+        acc2[c].x = 0;
+        acc2[c].y = 0;
         xs[c].x = 0;
         xs[c].y = 0;
     }
@@ -186,7 +192,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
 #pragma unroll
         for (int c = 0; c < CPL; ++c) zv[c] = ld_stream(z + own + c, opts);
     }
-    if (MODE != kRecDbl) {
+    if (MODE == kFirst || MODE == kRec) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) bv[c] = ld_stream(bra + own + c, opts);
     }
@@ -195,12 +201,13 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
     int p = s;
     for (; p + 4 <= e; p += 4) {
         int cc[4];
-        H hh[4];
+        H hh[4], hh2[4];
         V2 xv[4][CPL];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             cc[k] = ci[p + k];
             hh[k] = hv[p + k];
+            if (MODE == kDual) hh2[k] = hv2[p + k];
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -212,6 +219,10 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         for (int k = 0; k < 4; ++k) {
 #pragma unroll
             for (int c = 0; c < CPL; ++c) madd(acc[c], hh[k], xv[k][c]);
+            if (MODE == kDual) {
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) madd(acc2[c], hh2[k], xv[k][c]);
+            }
             if (cc[k] == r) {
                 have_self = true;
 #pragma unroll
@@ -228,6 +239,11 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         for (int c = 0; c < CPL; ++c) xv[c] = xr[c];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) madd(acc[c], hh, xv[c]);
+        if (MODE == kDual) {
+            const H hh2 = hv2[p];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) madd(acc2[c], hh2, xv[c]);
+        }
         if (cc == r) {
             have_self = true;
 #pragma unroll
@@ -253,6 +269,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             v.y = v.y - zv[c].y;
         }
         st_stream(y + own + c, v, opts);
+        if (MODE == kDual) st_stream(y2 + own + c, acc2[c], opts);
         if (MODE == kFirst) {
             p0 = p0 + redot(bv[c], xs[c]);
             p1 = p1 + redot(bv[c], v);
@@ -271,7 +288,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const typename HVal<VT, CH>::type* __restrict__ hv, const typename Vec2<VT>::type* __restrict__ x,
     const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y,
     const typename Vec2<VT>::type* __restrict__ bra, int32_t rows, int32_t Rp, VT alpha, VT beta,
-    int32_t opts, int64_t ntiles, TileOrder order, int32_t xcd_split, double2* __restrict__ partial) {
+    int32_t opts, int64_t ntiles, TileOrder order, int32_t xcd_split, double2* __restrict__ partial,
+    const typename HVal<VT, CH>::type* __restrict__ hv2, typename Vec2<VT>::type* y2) {
     constexpr int RPW = kWave / LPR;
     constexpr int TR = kWavesPerBlock * RPW;
     static_assert(LPR >= 1 && LPR <= kWave && (kWave % LPR) == 0, "bad lanes per row");
@@ -292,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
         int r = static_cast<int>(rt * TR) + wave * RPW + sub;
         if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
         if (r >= rows) continue;
-        step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q, Rp, alpha, beta, opts, p0, p1);
+        step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q, Rp, alpha, beta, opts, p0, p1, hv2, y2);
     }
 
     // deterministic block reduction: butterfly in the wave, then waves in order

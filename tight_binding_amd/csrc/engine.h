@@ -111,6 +111,7 @@ struct tbk_matrix {
     int32_t max_row_nnz = 0;
     int32_t bandwidth95 = 0;  // 95th percentile of |row - col| over the entries
     double gersh_min = 0.0, gersh_max = 0.0;  // Gershgorin interval (Hermitian reading)
+    uint64_t pattern = 0;  // fingerprint of (rows, row pointers, columns); 0: not known
     engine::DevBuf rp, ci, val64, val32;
     bool have32 = false;
 };
@@ -175,6 +176,8 @@ void spectral_bounds(tbk_context* ctx, const tbk_matrix* M, int max_iters, doubl
 void matrix_fp32(tbk_matrix* m);
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
+                 const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2);
+void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial);
 
 // Work split of one problem in a cheb_slots_kernel launch, the split that
@@ -219,10 +222,12 @@ struct ContractionShape {
 ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int prec);
 void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int prec,
                          double2* P, double2* out, hipStream_t st);
+// vx_values: v_x's values when v_x has H's pattern (fused_velocity), else null
 void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
                          const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int N,
-                         double2* part, hipStream_t st);
+                         double2* part, hipStream_t st, const void* vx_values);
+bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH);
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
-                            int64_t N);
+                            int64_t N, bool fused);
 
 }  // namespace engine

@@ -25,6 +25,8 @@ struct StepArgs {
     TileOrder order;
     int32_t xcd_split;
     double2* partial;
+    const void* hv2 = nullptr;  // kDual: values of the second matrix (same pattern)
+    void* y2 = nullptr;         // kDual: its product with x
 };
 
 // Arguments of cheb_slots_kernel (one step of every slot of a class; the
@@ -91,8 +93,10 @@ bool launch_step(dim3 grid, hipStream_t st, const StepArgs& a) {
                            static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
                            static_cast<V2*>(a.y), static_cast<const V2*>(a.bra), a.rows, a.Rp,
                            static_cast<VT>(a.alpha), static_cast<VT>(a.beta), a.opts, a.ntiles, a.order, a.xcd_split,
-                           a.partial);
+                           a.partial, static_cast<const H*>(a.hv2), static_cast<V2*>(a.y2));
         return true;
+    } else if constexpr (MODE == kDual) {
+        return false;  // the dual product exists for the row kernel only
     } else if constexpr (tile_fits(LPR, CPL, KCH, sizeof(H))) {
         hipLaunchKernelGGL((cheb_tile_kernel<VT, CH, LPR, CPL, MODE, KCH>), grid, dim3(kBlock), 0, st, a.rp, a.ci,
                            static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
@@ -110,6 +114,10 @@ bool launch_mode(int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
     switch (mode) {
         case kFirst: return launch_step<VT, CH, LPR, CPL, kFirst, KCH>(grid, st, a);
         case kRec: return launch_step<VT, CH, LPR, CPL, kRec, KCH>(grid, st, a);
+        case kDual:
+            // instantiated for the fp64 complex row kernel (the Kubo stacks) only
+            if constexpr (KCH == 0 && CH && sizeof(VT) == 8) return launch_step<VT, CH, LPR, CPL, kDual, 0>(grid, st, a);
+            return false;
         default: return launch_step<VT, CH, LPR, CPL, kRecDbl, KCH>(grid, st, a);
     }
 }

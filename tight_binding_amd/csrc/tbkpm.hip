@@ -97,6 +97,22 @@ int bandwidth_percentile(const std::vector<int32_t>& rp, const std::vector<int32
     return d[at];
 }
 
+// 64-bit fingerprint of a CSR pattern (rows, row pointers, column indices):
+// a splitmix64 chain, never 0
+uint64_t pattern_fingerprint(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci) {
+    auto mix = [](uint64_t x) {
+        x += 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    };
+    uint64_t h = mix(static_cast<uint64_t>(rp.size()));
+    for (int32_t v : rp) h = mix(h ^ static_cast<uint32_t>(v));
+    h = mix(h ^ 0xC0FFEEull);
+    for (int32_t v : ci) h = mix(h ^ static_cast<uint32_t>(v));
+    return h ? h : 1;
+}
+
 // Gershgorin interval of a Hermitian matrix: the union over rows of
 // [Re h_ii - r_i, Re h_ii + r_i], r_i = sum_{j != i} |h_ij|
 void gershgorin(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, const std::vector<double>& val,
@@ -231,6 +247,11 @@ StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int 
 // the mode into partial[0 .. plan.nb).
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial) {
+    launch_plan(pl, st, mode, alpha, beta, x, z, y, bra, partial, nullptr, nullptr);
+}
+
+void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
+                 const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2) {
     StepArgs A = pl.args;
     A.alpha = alpha;
     A.beta = beta;
@@ -239,6 +260,8 @@ void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, dou
     A.y = y;
     A.bra = bra;
     A.partial = partial;
+    A.hv2 = hv2;
+    A.y2 = y2;
     launch_any(pl.precision, pl.cplx, pl.kc.width, pl.kc.kch, pl.kc.cpl, mode, dim3(static_cast<unsigned>(pl.nb)), st,
                A);
     HIPC(hipGetLastError());
@@ -449,9 +472,19 @@ void enqueue_contraction(const ContractionShape& cs, const double2* A, const dou
 
 // Both Kubo stacks: L[n] = T_n(H~)|r>, R[m] = v_x T_m(H~) v_y|r> (n, m < N),
 // blocks of blk_bytes each; T_m of the right side rotates through W0, W1.
+// v_x = [X, H~] stores H~'s pattern; when the uploaded arrays agree (same
+// fingerprint) and H~ uses the row kernel, v_x T_{m-1} is formed in the step
+// that gathers T_{m-1} for T_m (kDual), which saves a read of every right
+// block and a launch per step.  Each product is the same sum in the same
+// order as a separate v_x launch, so the stack is the same bits.
+bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH) {
+    return H->is_complex && vx->is_complex && H->pattern != 0 && H->pattern == vx->pattern &&
+           H->rows == vx->rows && H->nnz == vx->nnz && pH.kc.kch == 0;
+}
+
 void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
                          const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int N,
-                         double2* part, hipStream_t st) {
+                         double2* part, hipStream_t st, const void* vx_values) {
     auto Lk = [&](int64_t k) { return static_cast<char*>(L) + static_cast<size_t>(k) * blk_bytes; };
     auto Rk = [&](int64_t k) { return static_cast<char*>(Rs) + static_cast<size_t>(k) * blk_bytes; };
     auto W = [&](int64_t k) { return (k & 1) ? W1 : W0; };
@@ -463,16 +496,22 @@ void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan&
     launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(0), nullptr, Rk(0), W(0), part);
     if (N > 1) {
         launch_plan(pH, st, tbk::kFirst, 1.0 / a, -b / a, W(0), nullptr, W(1), W(0), part);
-        launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(1), nullptr, Rk(1), W(1), part);
+        if (!vx_values) launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(1), nullptr, Rk(1), W(1), part);
     }
     for (int64_t m = 2; m < N; ++m) {
-        launch_plan(pH, st, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part);
-        launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(m), nullptr, Rk(m), W(m), part);
+        if (vx_values) {  // T_m and v_x T_{m-1} from one pass over T_{m-1}
+            launch_plan(pH, st, tbk::kDual, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part, vx_values,
+                        Rk(m - 1));
+        } else {
+            launch_plan(pH, st, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part);
+            launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(m), nullptr, Rk(m), W(m), part);
+        }
     }
+    if (vx_values && N > 1) launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(N - 1), nullptr, Rk(N - 1), W(N - 1), part);
 }
 
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
-                            int64_t N) {
+                            int64_t N, bool fused) {
     // matrix arrays once per launch; vector blocks: x (+ z) read and y
     // written per step, x read and y written per velocity application
     auto mat = [&](const tbk_matrix* m) {
@@ -484,7 +523,15 @@ double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk
     double bytes = 2.0 * blk;                              // |r> into the left stack
     bytes += mat(vy) + 2.0 * blk + mat(vx) + 2.0 * blk;    // v_x v_y |r>
     if (N > 1) bytes += 2.0 * (mat(H) + 2.0 * blk) + mat(vx) + 2.0 * blk;
-    bytes += Nm2 * (2.0 * (mat(H) + 3.0 * blk) + mat(vx) + 2.0 * blk);
+    if (fused && N > 1) {
+        // each fused step also reads v_x's values and writes one right block;
+        // the last v_x product is a separate launch
+        const double vals = static_cast<double>(vx->nnz) * 16.0;
+        bytes -= mat(vx) + 2.0 * blk;  // the v_x launch for T_1 is folded into the first fused step
+        bytes += Nm2 * ((mat(H) + 3.0 * blk) + (mat(H) + vals + 4.0 * blk)) + mat(vx) + 2.0 * blk;
+    } else {
+        bytes += Nm2 * (2.0 * (mat(H) + 3.0 * blk) + mat(vx) + 2.0 * blk);
+    }
     return bytes;
 }
 
@@ -557,8 +604,9 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     double2* part = ctx->partial.as<double2>();
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+    const bool fuse = fused_velocity(H, vx, pH);
     enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, N, part,
-                        ctx->stream);
+                        ctx->stream, fuse ? vx->val64.p : nullptr);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
     float rec_ms = 0.f;
@@ -581,7 +629,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         stats->flops = 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
         stats->stack_bytes = 2.0 * static_cast<double>(stack_bytes);
         stats->nchunks = nchunks;
-        const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N);
+        const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N, fuse);
         stats->recursion_bytes = bytes;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -726,6 +774,7 @@ int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out) {
         for (int32_t i = 0; i < M->rows; ++i) m->max_row_nnz = std::max(m->max_row_nnz, rp[i + 1] - rp[i]);
         m->bandwidth95 = bandwidth_percentile(rp, ci, 0.95);
         gershgorin(rp, ci, val, m->is_complex, m->gersh_min, m->gersh_max);
+        m->pattern = pattern_fingerprint(rp, ci);
         m->rp.reserve(rp.size() * 4);
         m->ci.reserve(std::max<size_t>(ci.size() * 4, 4));
         m->val64.reserve(std::max<size_t>(val.size() * 8, 8));
