@@ -92,8 +92,8 @@ typedef struct tbk_tuning {
     int32_t use_tiled;       /* 1: LDS-staged tiled kernel where rows have <= 16 entries; 0: row kernel */
     int32_t chunk_rows;      /* rows per chunk of the visiting order (default 256) */
     int32_t chunk_period;    /* chunk-grid period P; 0: bandwidth / chunk rows, -1: row order */
-    int32_t row_cpl;         /* row kernel: complex values per lane (1, 2, 4; 0: 16 bytes per lane, i.e.
-                                1 in fp64 and 2 in fp32; at least R/64) */
+    int32_t row_lane_bytes;  /* row kernel: vector bytes per lane, 8, 16 or 32 (0: 16), i.e. 1 or 2
+                                complex128 / 1, 2 or 4 complex64 (at least R/64 complex) */
 } tbk_tuning;
 
 const char* tbk_version(void);

@@ -123,7 +123,7 @@ class tbk_tuning(C.Structure):
         ("use_tiled", C.c_int32),
         ("chunk_rows", C.c_int32),
         ("chunk_period", C.c_int32),
-        ("row_cpl", C.c_int32),
+        ("row_lane_bytes", C.c_int32),
     ]
 
 

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void assemble_fill_kernel(LatticeArgs A, co
             if (c[x].col == i)
                 center += re;
             else
-                radius += A.is_complex ? hypot(re, im) : fabs(re);
+                radius += A.is_complex ? sqrt(re * re + im * im) : fabs(re);
             ++p;
             x = y;
         }
@@ -452,6 +452,7 @@ void assemble_matrix(tbk_context* ctx, const tbk_lattice_desc* d, hipStream_t st
     }
     m->gersh_min = std::isfinite(lo) ? lo : 0.0;
     m->gersh_max = std::isfinite(hi) ? hi : 0.0;
+    m->have_gersh = true;
     ctx->held_bytes += static_cast<int64_t>(m->rp.bytes + m->ci.bytes + m->val64.bytes);
     if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = m.release();

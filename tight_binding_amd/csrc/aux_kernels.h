@@ -142,6 +142,72 @@ __global__ __launch_bounds__(kBlock) void stream_read_kernel(const nt_d2* __rest
     if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
+// Gershgorin discs of a Hermitian CSR matrix: per workgroup the min of
+// Re h_ii - r_i and the max of Re h_ii + r_i, r_i = sum_{j != i} |h_ij| (in
+// column order, |h| = sqrt(re^2 + im^2))
+__global__ __launch_bounds__(kBlock) void gershgorin_kernel(const int32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ ci,
+                                                            const double* __restrict__ val, int cplx, int32_t n,
+                                                            double2* __restrict__ out) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t ii = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; ii < n;
+         ii += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const int32_t i = static_cast<int32_t>(ii);
+        double c = 0.0, r = 0.0;
+        for (int32_t p = rp[i]; p < rp[i + 1]; ++p) {
+            const double re = cplx ? val[2 * static_cast<int64_t>(p)] : val[p];
+            const double im = cplx ? val[2 * static_cast<int64_t>(p) + 1] : 0.0;
+            if (ci[p] == i)
+                c = c + re;
+            else
+                r = r + (cplx ? sqrt(re * re + im * im) : fabs(re));
+        }
+        lo = fmin(lo, c - r);
+        hi = fmax(hi, c + r);
+    }
+    __shared__ double slo[kBlock], shi[kBlock];
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = kBlock / 2; s >= 1; s >>= 1) {
+        if (static_cast<int>(threadIdx.x) < s) {
+            slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + s]);
+            shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = make_double2(slo[0], shi[0]);
+}
+
+// Position-keyed fingerprint of a CSR pattern: per workgroup the sum mod 2^64
+// of mix(position, value) over the row pointers and the column indices
+// (order-free, so the result does not depend on the launch shape)
+__device__ __forceinline__ uint64_t fp_mix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__global__ __launch_bounds__(kBlock) void pattern_hash_kernel(const int32_t* __restrict__ rp,
+                                                              const int32_t* __restrict__ ci, int32_t n,
+                                                              int64_t nnz, unsigned long long* __restrict__ out) {
+    unsigned long long h = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i <= n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock)
+        h += fp_mix((static_cast<uint64_t>(i) << 32) | static_cast<uint32_t>(rp[i]));
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; p < nnz;
+         p += static_cast<int64_t>(gridDim.x) * kBlock)
+        h += fp_mix(((static_cast<uint64_t>(p) << 32) | static_cast<uint32_t>(ci[p])) ^ 0xC01C01C01C01C01Cull);
+    __shared__ unsigned long long sh[kBlock];
+    sh[threadIdx.x] = h;
+    __syncthreads();
+    for (int s = kBlock / 2; s >= 1; s >>= 1) {
+        if (static_cast<int>(threadIdx.x) < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
 // fp64 -> fp32 copy of matrix values
 __global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double* __restrict__ in,
                                                          float* __restrict__ out, int64_t n) {

@@ -308,6 +308,7 @@ void spectral_bounds(tbk_context* ctx, const tbk_matrix* M, int max_iters, doubl
     out->ritz_max = e.tmax;
     out->resid_min = e.rmin;
     out->resid_max = e.rmax;
+    matrix_gershgorin(M, st);
     out->gersh_min = M->gersh_min;
     out->gersh_max = M->gersh_max;
     out->lower = std::max(M->gersh_min, e.tmin - e.rmin);

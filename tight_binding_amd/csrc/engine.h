@@ -110,8 +110,10 @@ struct tbk_matrix {
     bool is_complex = false;
     int32_t max_row_nnz = 0;
     int32_t bandwidth95 = 0;  // 95th percentile of |row - col| over the entries
+    // formed on the device on first use (matrix_gershgorin, matrix_pattern)
+    bool have_gersh = false;
     double gersh_min = 0.0, gersh_max = 0.0;  // Gershgorin interval (Hermitian reading)
-    uint64_t pattern = 0;  // fingerprint of (rows, row pointers, columns); 0: not known
+    uint64_t pattern = 0;                     // fingerprint of (rows, row pointers, columns); 0: not formed
     engine::DevBuf rp, ci, val64, val32;
     bool have32 = false;
 };
@@ -163,8 +165,8 @@ KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int R
                            int bandwidth, bool force_row);
 int bandwidth_percentile(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, double p);
 void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_t>& ci, std::vector<double>& val);
-void gershgorin(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, const std::vector<double>& val,
-                bool cplx, double& lo, double& hi);
+void matrix_gershgorin(const tbk_matrix* M, hipStream_t st);
+uint64_t matrix_pattern(const tbk_matrix* M, hipStream_t st);
 tbk_context* default_context();
 // Hamiltonian assembly on the device (assemble.hip)
 int64_t lattice_rows(const tbk_lattice_desc* d);
@@ -226,7 +228,7 @@ void enqueue_contraction(const ContractionShape& cs, const double2* A, const dou
 void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
                          const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int N,
                          double2* part, hipStream_t st, const void* vx_values);
-bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH);
+bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH, hipStream_t st);
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
                             int64_t N, bool fused);
 

@@ -468,7 +468,7 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
                        j->kets.as<double2>(), rows, r->R, Rp, r->seed, inv);
     HIPC(hipGetLastError());
     enqueue_kubo_stacks(pH, pX, pY, r->a, r->b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, N,
-                        j->part.as<double2>(), st_, fused_velocity(j->H, j->vx, pH) ? j->vx->val64.p : nullptr);
+                        j->part.as<double2>(), st_, fused_velocity(j->H, j->vx, pH, bstream) ? j->vx->val64.p : nullptr);
     enqueue_contraction(cs, j->L.as<const double2>(), j->Rs.as<const double2>(), N, K, r->precision,
                         j->P.as<double2>(), j->G.as<double2>(), st_);
     tbk::launch_gamma_weight(j->G.as<double2>(), j->g.as<const double>(), N, st_);

@@ -69,11 +69,14 @@ KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int R
     }
     if (kch == 0) {
         // row kernel: cpl complex values per lane (at least Rp/64), LPR = Rp / cpl;
-        // by default 16 bytes per lane (one complex128 or two complex64): on
-        // gfx950 that is 0.61-0.83x the fp32 step time of one complex64 per
-        // lane at R = 8..64, and wider lanes lose in fp64
+        // lanes of row_lane_bytes, 16 by default (one complex128 or two
+        // complex64): on gfx950 that is 0.61-0.83x the fp32 step time of one
+        // complex64 per lane at R = 8..64, and wider lanes lose in fp64.  A
+        // width in bytes means the same in both precisions, so a tuning found
+        // on one carries over to the other.
         const int cmin = Rp > tbk::kWave ? Rp / tbk::kWave : 1;
-        int cpl = std::max(cmin, tu.row_cpl > 0 ? tu.row_cpl : 16 / ebytes);
+        const int lane_bytes = tu.row_lane_bytes > 0 ? tu.row_lane_bytes : 16;
+        int cpl = std::max({cmin, 1, lane_bytes / ebytes});
         if (cpl != 1 && cpl != 2 && cpl != 4) cpl = cmin;
         if (cpl > Rp || Rp % cpl != 0 || (cpl > cmin && Rp * 8 + cpl > 128 * 8 + 4)) cpl = cmin;
         const int lpr = Rp / cpl;
@@ -97,44 +100,51 @@ int bandwidth_percentile(const std::vector<int32_t>& rp, const std::vector<int32
     return d[at];
 }
 
-// 64-bit fingerprint of a CSR pattern (rows, row pointers, column indices):
-// a splitmix64 chain, never 0
-uint64_t pattern_fingerprint(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci) {
-    auto mix = [](uint64_t x) {
-        x += 0x9E3779B97F4A7C15ull;
-        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-        return x ^ (x >> 31);
-    };
-    uint64_t h = mix(static_cast<uint64_t>(rp.size()));
-    for (int32_t v : rp) h = mix(h ^ static_cast<uint32_t>(v));
-    h = mix(h ^ 0xC0FFEEull);
-    for (int32_t v : ci) h = mix(h ^ static_cast<uint32_t>(v));
-    return h ? h : 1;
+// Gershgorin interval of a resident matrix, formed on the device on first use
+// (the union over rows of [Re h_ii - r_i, Re h_ii + r_i], r_i = sum_{j != i} |h_ij|)
+void matrix_gershgorin(const tbk_matrix* M, hipStream_t st) {
+    auto* m = const_cast<tbk_matrix*>(M);
+    if (m->have_gersh) return;
+    const unsigned grid =
+        static_cast<unsigned>(std::min<int64_t>((static_cast<int64_t>(m->rows) + tbk::kBlock - 1) / tbk::kBlock, 1024));
+    DevBuf part;
+    part.reserve(static_cast<size_t>(grid) * 16);
+    hipLaunchKernelGGL(tbk::gershgorin_kernel, dim3(grid), dim3(tbk::kBlock), 0, st, m->rp.as<const int32_t>(),
+                       m->ci.as<const int32_t>(), m->val64.as<const double>(), m->is_complex ? 1 : 0, m->rows,
+                       part.as<double2>());
+    HIPC(hipGetLastError());
+    std::vector<double2> h(grid);
+    HIPC(hipMemcpyAsync(h.data(), part.p, static_cast<size_t>(grid) * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    double lo = INFINITY, hi = -INFINITY;
+    for (const double2& g : h) {
+        lo = std::min(lo, g.x);
+        hi = std::max(hi, g.y);
+    }
+    m->gersh_min = std::isfinite(lo) ? lo : 0.0;
+    m->gersh_max = std::isfinite(hi) ? hi : 0.0;
+    m->have_gersh = true;
 }
 
-// Gershgorin interval of a Hermitian matrix: the union over rows of
-// [Re h_ii - r_i, Re h_ii + r_i], r_i = sum_{j != i} |h_ij|
-void gershgorin(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, const std::vector<double>& val,
-                bool cplx, double& lo, double& hi) {
-    lo = 0.0;
-    hi = 0.0;
-    bool any = false;
-    const int vw = cplx ? 2 : 1;
-    for (size_t i = 0; i + 1 < rp.size(); ++i) {
-        double c = 0.0, r = 0.0;
-        for (int32_t p = rp[i]; p < rp[i + 1]; ++p) {
-            const double re = val[static_cast<size_t>(p) * vw];
-            const double im = cplx ? val[static_cast<size_t>(p) * vw + 1] : 0.0;
-            if (ci[p] == static_cast<int32_t>(i))
-                c += re;
-            else
-                r += std::hypot(re, im);
-        }
-        lo = any ? std::min(lo, c - r) : c - r;
-        hi = any ? std::max(hi, c + r) : c + r;
-        any = true;
-    }
+// 64-bit fingerprint of a resident matrix's pattern (rows, row pointers,
+// columns), formed on the device on first use; never 0
+uint64_t matrix_pattern(const tbk_matrix* M, hipStream_t st) {
+    auto* m = const_cast<tbk_matrix*>(M);
+    if (m->pattern) return m->pattern;
+    const int64_t work = std::max<int64_t>(static_cast<int64_t>(m->rows) + 1, m->nnz);
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((work + tbk::kBlock - 1) / tbk::kBlock, 1024));
+    DevBuf part;
+    part.reserve(static_cast<size_t>(grid) * 8);
+    hipLaunchKernelGGL(tbk::pattern_hash_kernel, dim3(grid), dim3(tbk::kBlock), 0, st, m->rp.as<const int32_t>(),
+                       m->ci.as<const int32_t>(), m->rows, m->nnz, part.as<unsigned long long>());
+    HIPC(hipGetLastError());
+    std::vector<unsigned long long> h(grid);
+    HIPC(hipMemcpyAsync(h.data(), part.p, static_cast<size_t>(grid) * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    uint64_t v = static_cast<uint64_t>(m->rows) * 0x9E3779B97F4A7C15ull ^ static_cast<uint64_t>(m->nnz);
+    for (unsigned long long x : h) v += x;
+    m->pattern = v ? v : 1;
+    return m->pattern;
 }
 
 void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_t>& ci,
@@ -477,9 +487,9 @@ void enqueue_contraction(const ContractionShape& cs, const double2* A, const dou
 // that gathers T_{m-1} for T_m (kDual), which saves a read of every right
 // block and a launch per step.  Each product is the same sum in the same
 // order as a separate v_x launch, so the stack is the same bits.
-bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH) {
-    return H->is_complex && vx->is_complex && H->pattern != 0 && H->pattern == vx->pattern &&
-           H->rows == vx->rows && H->nnz == vx->nnz && pH.kc.kch == 0;
+bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH, hipStream_t st) {
+    if (!H->is_complex || !vx->is_complex || H->rows != vx->rows || H->nnz != vx->nnz || pH.kc.kch != 0) return false;
+    return matrix_pattern(H, st) == matrix_pattern(vx, st);
 }
 
 void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
@@ -604,7 +614,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     double2* part = ctx->partial.as<double2>();
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    const bool fuse = fused_velocity(H, vx, pH);
+    const bool fuse = fused_velocity(H, vx, pH, ctx->stream);
     enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, N, part,
                         ctx->stream, fuse ? vx->val64.p : nullptr);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
@@ -773,8 +783,6 @@ int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out) {
         m->is_complex = M->is_complex != 0;
         for (int32_t i = 0; i < M->rows; ++i) m->max_row_nnz = std::max(m->max_row_nnz, rp[i + 1] - rp[i]);
         m->bandwidth95 = bandwidth_percentile(rp, ci, 0.95);
-        gershgorin(rp, ci, val, m->is_complex, m->gersh_min, m->gersh_max);
-        m->pattern = pattern_fingerprint(rp, ci);
         m->rp.reserve(rp.size() * 4);
         m->ci.reserve(std::max<size_t>(ci.size() * 4, 4));
         m->val64.reserve(std::max<size_t>(val.size() * 8, 8));
@@ -1026,7 +1034,7 @@ int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
             {&tbk_tuning::chunk_period, {0, -1}},
             {&tbk_tuning::xcd_split, {0, -1}},
             {&tbk_tuning::stream_nt, {1, -1}},
-            {&tbk_tuning::row_cpl, {1, 2, 4}},
+            {&tbk_tuning::row_lane_bytes, {8, 16, 32}},
         };
         for (int pass = 0; pass < 2; ++pass) {
             for (const auto& k : knobs) {
