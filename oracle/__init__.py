@@ -34,6 +34,8 @@ def lib():
         L.oracle_cheb_braket.restype = C.c_int
         L.oracle_cheb_steps.argtypes = [C.c_int, I32, IP, IP, DP, I32, DP, I32, DP]
         L.oracle_cheb_steps.restype = C.c_int
+        L.oracle_cheb_steps_threads.argtypes = [C.c_int, I32, IP, IP, DP, I32, DP, I32, DP, I32]
+        L.oracle_cheb_steps_threads.restype = C.c_int
         L.oracle_cheb_on_bases.argtypes = [C.c_int, I32, IP, IP, DP, I32, DP, I32, DP]
         L.oracle_cheb_on_bases.restype = C.c_int
         L.oracle_trace_braket.argtypes = [DP, DP, I32, I32]
@@ -88,6 +90,21 @@ def cheb_braket(M, n, bras_T, kets, last_index=None):
     if rc != 0:
         raise ValueError("oracle_cheb_braket rejected its arguments")
     return (mu, last) if last is not None else mu
+
+
+def cheb_steps_threads(M, steps, kets, nthreads=0):
+    """mu[0 .. steps+1] of bras = kets by the same recursion with each step
+    split over the block columns across OpenMP threads (the all-cores timing
+    variant; nthreads <= 0: the OpenMP default)."""
+    cp, ri, vv, cplx = _csc_arrays(M.colptr, M.rowidx, M.values)
+    k = _colmajor(kets)
+    rows, R = k.shape
+    mu = np.zeros(int(steps) + 2)
+    rc = lib().oracle_cheb_steps_threads(int(cplx), rows, _ip(cp), _ip(ri), _dp(vv), int(steps),
+                                         _dp(k.ravel(order="F").view(np.float64)), R, _dp(mu), int(nthreads))
+    if rc != 0:
+        raise ValueError("oracle_cheb_steps_threads rejected its arguments")
+    return mu
 
 
 def cheb_on_bases(M, n, V):

@@ -15,6 +15,11 @@
  *   oracle_trace_braket    numerical.h:127-135  Re tr(bras^H kets), rows
  *                          outer, columns inner, products rounded then added
  *   oracle_cheb_on_bases   numerical.h:334-344  stored blocks T_n(M) V
+ *   oracle_cheb_steps_threads  the same recursion with each step's product
+ *                          and trace split over the block columns across
+ *                          OpenMP threads (all-cores timing variant; the
+ *                          moments differ from the single-thread loop only
+ *                          in the summation order of the trace)
  *   oracle_kernel_*        numerical.h:393-426  Jackson / Lorentz / Fejer
  *
  * Arithmetic is written out in real operations in the order the reference's
@@ -24,6 +29,7 @@
  * Blocks are column-major rows x R, complex128 interleaved (re, im).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -135,6 +141,74 @@ int oracle_cheb_braket(int is_complex, int32_t rows, const int32_t* colptr, cons
 int oracle_cheb_steps(int is_complex, int32_t rows, const int32_t* colptr, const int32_t* rowidx,
                       const double* val, int32_t steps, const double* kets, int32_t R, double* mu) {
     return oracle_cheb_braket(is_complex, rows, colptr, rowidx, val, steps + 2, kets, kets, R, mu, NULL, 0);
+}
+
+/*
+ * All-cores timing variant of oracle_cheb_steps: every step's product,
+ * subtraction and trace run over the block columns in parallel (one column's
+ * work is the single-thread loop's for that column); the per-column traces
+ * are added in column order.  nthreads <= 0: the OpenMP default.
+ */
+int oracle_cheb_steps_threads(int is_complex, int32_t rows, const int32_t* colptr, const int32_t* rowidx,
+                              const double* val, int32_t steps, const double* kets, int32_t R, double* mu,
+                              int32_t nthreads) {
+    const int32_t n = steps + 2;
+    if (steps < 1 || rows <= 0 || R <= 0) return -1;
+    const size_t blk = (size_t)rows * R;
+    cplx* T0 = (cplx*)malloc(sizeof(cplx) * blk);
+    cplx* T1 = (cplx*)malloc(sizeof(cplx) * blk);
+    cplx* T2 = (cplx*)malloc(sizeof(cplx) * blk);
+    double* part = (double*)malloc(sizeof(double) * (size_t)R);
+    if (!T0 || !T1 || !T2 || !part) {
+        free(T0);
+        free(T1);
+        free(T2);
+        free(part);
+        return -1;
+    }
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    memcpy(T0, kets, sizeof(cplx) * blk);
+    cplx* Tn_2 = T0;
+    cplx* Tn_1 = T1;
+    cplx* Tn = T2;
+    const cplx* k = (const cplx*)kets;
+    for (int32_t i = 1; i < n; ++i) {
+        const double scale = i == 1 ? 1.0 : 2.0;
+        const cplx* src = i == 1 ? Tn_2 : Tn_1;
+        cplx* dst = i == 1 ? Tn_1 : Tn;
+#pragma omp parallel for schedule(static)
+        for (int32_t c = 0; c < R; ++c) {
+            oracle_spmm_csc(is_complex, rows, colptr, rowidx, val, scale, src + (size_t)c * rows, 1,
+                            dst + (size_t)c * rows);
+            cplx* d = dst + (size_t)c * rows;
+            if (i > 1) {
+                const cplx* z = Tn_2 + (size_t)c * rows;
+                for (int32_t r = 0; r < rows; ++r) {
+                    d[r].re = d[r].re - z[r].re;
+                    d[r].im = d[r].im - z[r].im;
+                }
+            }
+            double t = 0.0;
+            const cplx* kc = k + (size_t)c * rows;
+            for (int32_t r = 0; r < rows; ++r) t += kc[r].re * d[r].re + kc[r].im * d[r].im;
+            part[c] = t;
+        }
+        double s = 0.0;
+        for (int32_t c = 0; c < R; ++c) s += part[c];
+        mu[i] = s;
+        if (i > 1) {
+            cplx* t = Tn_2;
+            Tn_2 = Tn_1;
+            Tn_1 = Tn;
+            Tn = t;
+        }
+    }
+    mu[0] = oracle_trace_braket(kets, kets, rows, R);
+    free(T0);
+    free(T1);
+    free(T2);
+    free(part);
+    return 0;
 }
 
 /* Stored blocks T_0..T_{n-1} of M applied to V (numerical.h:334-344); out is

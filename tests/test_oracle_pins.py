@@ -326,3 +326,23 @@ def test_lorentz_kernel_identities(N, lam):
 def test_fejer_kernel():
     g = oracle.kernel_fejer(10)
     assert g[0] == 1.0 and np.allclose(np.diff(g), -0.1)
+
+
+@pytest.mark.parametrize("cplx,threads", [(False, 1), (False, 3), (True, 4)])
+def test_all_cores_timing_variant_equals_the_oracle_loop(cplx, threads):
+    """The OpenMP timing variant of the CPU baseline runs the oracle's
+    recursion per block column: the same T_n, the moments differ only in the
+    summation order of the trace."""
+    cell = O.model("honeycomb")
+    cell.add_hopping_nn("A", 1, 1.0)
+    lat = O.PartialLattice(cell, (9, 7))
+    lat.set_scalar_field_indexed(O.uniform_disorder(0.8, 5))
+    if cplx:
+        lat.gauge_field = O.symmetric_gauge(0.05)
+    Hs = O.rescale(lat.get_Hamiltonian_periodic_extend_bare_sparse((1, 1), cplx), 3.5, 0.1)
+    kets = O.random_phase_kets(Hs.n, 5, 12)
+    steps = 30
+    ref = oracle.cheb_braket(Hs, steps + 2, kets, kets)
+    got = oracle.cheb_steps_threads(Hs, steps, kets, threads)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
