@@ -370,7 +370,7 @@ __device__ __forceinline__ uint32_t swap_neg(uint32_t w) { return ((w >> 16) | (
 
 __device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
 
-__global__ __launch_bounds__(256) void gamma_bf16x3_big_kernel(const double2* __restrict__ A,
+__global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const double2* __restrict__ A,
                                                                const double2* __restrict__ B, int N, int64_t K,
                                                                int64_t kc, GammaMap map, double2* __restrict__ P) {
     __shared__ __align__(16) uint32_t As[2][kBigTile * kSplitRowWords];
@@ -378,39 +378,42 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_big_kernel(const double2* __
     int tn, tm;
     int64_t c;
     if (!xcd_work_item(map, tn, tm, c)) return;  // uniform over the workgroup
+    // 8 waves: wave w owns the 64 x 32 block (rows (w >> 2) * 64, columns
+    // (w & 3) * 32) of the 128 x 128 tile; two waves per SIMD, so one wave's
+    // staging and splitting overlaps another's matrix-core work
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int wn = wave >> 1, wm = wave & 1;
+    const int wn = wave >> 2, wm = wave & 3;
     const int n0 = tn * kBigTile, m0 = tm * kBigTile;
     const int64_t kb = c * kc;
     const int64_t ke = (kb + kc < K) ? kb + kc : K;
 
-    // staging: thread t takes complex column t % 16 of rows t / 16 + 16 i, i < 8
+    // staging: thread t takes complex column t % 16 of rows t / 16 + 32 i, i < 4
     const int col = tid & 15;
     const int r0 = tid >> 4;
-    const double2* ga[8];
-    const double2* gb[8];
+    const double2* ga[4];
+    const double2* gb[4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int r = r0 + 16 * i;
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + 32 * i;
         const int rn = (n0 + r < N) ? n0 + r : N - 1;
         const int rm = (m0 + r < N) ? m0 + r : N - 1;
         ga[i] = A + static_cast<int64_t>(rn) * K + col;
         gb[i] = B + static_cast<int64_t>(rm) * K + col;
     }
-    double2 ra[8], rb[8];
+    double2 ra[4], rb[4];
     auto gload = [&](int64_t k0) {
         const bool ok = k0 + col < ke;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < 4; ++i) {
             ra[i] = ld_or_zero(ga[i] + k0, ok);
             rb[i] = ld_or_zero(gb[i] + k0, ok);
         }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int o = (r0 + 16 * i) * kSplitRowWords + col;
+        for (int i = 0; i < 4; ++i) {
+            const int o = (r0 + 32 * i) * kSplitRowWords + col;
             const uint2 wa = split_word(ra[i]);
             const uint2 wb = split_word(rb[i]);
             As[buf][o] = wa.x;
@@ -420,11 +423,11 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_big_kernel(const double2* __
         }
     };
 
-    v4f re[4][4], im[4][4];
+    v4f re[4][2], im[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 2; ++j) {
             re[i][j] = v4f{0, 0, 0, 0};
             im[i][j] = v4f{0, 0, 0, 0};
         }
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_big_kernel(const double2* __
     for (int64_t k0 = kb; k0 < ke; k0 += kSplitKt) {
         const bool more = k0 + kSplitKt < ke;
         if (more) gload(k0 + kSplitKt);
-        v8bf ah[4], al[4], bh[4], bl[4], bih[4], bil[4];
+        v8bf ah[4], al[4], bh[2], bl[2], bih[2], bil[2];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const uint32_t* row = &As[buf][(wn * 64 + 16 * i + li) * kSplitRowWords + 4 * lk];
@@ -444,8 +447,8 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_big_kernel(const double2* __
             al[i] = as_v8bf(*reinterpret_cast<const uint4*>(row + 16));
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t* row = &Bs[buf][(wm * 64 + 16 * j + li) * kSplitRowWords + 4 * lk];
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t* row = &Bs[buf][(wm * 32 + 16 * j + li) * kSplitRowWords + 4 * lk];
             const uint4 h = *reinterpret_cast<const uint4*>(row);
             const uint4 l = *reinterpret_cast<const uint4*>(row + 16);
             bh[j] = as_v8bf(h);
@@ -456,7 +459,7 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_big_kernel(const double2* __
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < 2; ++j) {
                 re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], re[i][j], 0, 0, 0);
                 im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bih[j], im[i][j], 0, 0, 0);
                 re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], re[i][j], 0, 0, 0);
@@ -473,11 +476,11 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_big_kernel(const double2* __
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int row = n0 + wn * 64 + 16 * i + 4 * lk + t;
-                const int cl = m0 + wm * 64 + 16 * j + li;
+                const int cl = m0 + wm * 32 + 16 * j + li;
                 if (row < N && cl < N)
                     Pc[static_cast<int64_t>(row) * N + cl] =
                         make_double2(static_cast<double>(re[i][j][t]), static_cast<double>(im[i][j][t]));
@@ -621,7 +624,7 @@ void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, 
 
 void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
                              double2* P, hipStream_t st) {
-    hipLaunchKernelGGL(gamma_bf16x3_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(256), 0, st, A, B,
+    hipLaunchKernelGGL(gamma_bf16x3_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0, st, A, B,
                        N, K, kc, map, P);
 }
 
