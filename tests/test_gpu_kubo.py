@@ -224,3 +224,45 @@ def test_fused_velocity_stacks_are_the_same_bits(gpu_ctx, N):
     ref = SO.kubo_gamma(TL, aR)
     assert np.max(np.abs(g1 - ref)) <= 1e-12 * np.max(np.abs(ref))
     K.close()
+
+
+def test_presplit_stacks_give_the_bf16x3_gamma_of_fp64_stacks(gpu_ctx):
+    """Split-BF16x3 solves with N >= 256 on complex operators write their
+    stacks pre-split (the recursion stores the hi/lo bf16 words the
+    contraction would form).  Gamma must be the same bits as the BF16x3
+    contraction of the fp64 stacks -- here the oracle's, which the device's
+    equal bit for bit at a = 1, b = 0 -- with v_x fused or not."""
+    import scipy.sparse as sp
+
+    from tight_binding_amd import sparse as SP
+
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (12, 10))
+    lat.set_gauge(L.GAUGE_LANDAU, 2 * np.pi / 12)
+    lat.set_disorder(1.0, 9)
+    Hs = L.rescale(lat.get_Hamiltonian_sparse(True), 5.0, 0.0)
+    vx = L.commutator_diag(lat.op_position(0), Hs)
+    vy = L.commutator_diag(lat.op_position(1), Hs)
+    N, R = 256, 2
+    K = gpu_ctx.random_block(Hs.n, R, seed=17)
+    kets = K.download()
+    g_ps, st = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy), K, N,
+                              precision=kpm.GAMMA_BF16X3)
+    assert st["stack_bytes"] == 2 * N * Hs.n * R * 8  # half-size entries
+    TL = oracle.cheb_on_bases(Hs, N, kets)
+    w = oracle.cheb_on_bases(vy, 2, kets)[1]
+    TR = oracle.cheb_on_bases(Hs, N, w)
+    aR = np.stack([oracle.cheb_on_bases(vx, 2, TR[m])[1] for m in range(N)])
+    ref, _ = kpm.stack_contract(gpu_ctx, TL.reshape(N, -1), aR.reshape(N, -1), kpm.GAMMA_BF16X3)
+    assert np.array_equal(g_ps, ref)
+    g64, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy), K, N)
+    assert np.max(np.abs(g_ps - g64)) <= 1e-4 * np.max(np.abs(g64))
+    # v_x without its stored zeros: the unfused pre-split path, the same bits
+    csc = sp.csc_matrix((vx.values.copy(), vx.rowidx.copy(), vx.colptr.copy()), shape=(vx.n, vx.n))
+    csc.eliminate_zeros()
+    vx2 = SP.CSC(vx.n, csc.indptr.astype(np.int32), csc.indices.astype(np.int32), csc.data.astype(np.complex128))
+    g_ps2, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx2), gpu_ctx.matrix(vy), K, N,
+                              precision=kpm.GAMMA_BF16X3)
+    assert np.array_equal(g_ps2, ref)
+    K.close()

@@ -180,3 +180,17 @@ def test_server_errors():
         srv.wait(t)
         mu, info = srv.take(t)
         assert info["state"] == Nn.TBK_REQ_DONE and mu.size == 10
+
+
+def test_presplit_kubo_request_matches_direct_path(gpu_ctx):
+    """A split-BF16x3 request with N >= 256 runs on pre-split stacks in the
+    server as in the direct path: the same Gamma bits."""
+    Hs, vx, vy, g = landau_kubo(L0=10, N=256, R=2)
+    N = g.size
+    ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
+                            gpu_ctx.random_block(Hs.n, 2, seed=31), N, precision=kpm.GAMMA_BF16X3, g=g)
+    with kpm.Server(0) as srv:
+        t = srv.submit_kubo(Hs, vx, vy, N, R=2, seed=31, precision=kpm.GAMMA_BF16X3, g=g)
+        assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
+        (z, G), info = srv.take(t)
+    assert np.array_equal(G, ref)

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "split_bf16.h"
+
 namespace tbk {
 
 constexpr int kWave = 64;
@@ -33,6 +35,11 @@ enum StepMode : int {
     kRecDbl = 2,  // same update;          partials (Re<T_k|T_k>, Re<T_{k+1}|T_k>)
     kDual = 3,    // kRecDbl, and y2 = H2 T_k for a second matrix H2 with the same
                   // pattern (values hv2), formed from the same gathered rows
+    // pre-split stacks (fp64 complex only): the output also, or instead,
+    // stored as split-bf16 words (split_word) at y2 viewed as uint2
+    kFirstSplit = 4,   // kFirst, and y's split words at y2
+    kRecDblSplit = 5,  // kRecDbl, and y's split words at y2
+    kDualSplit = 6,    // kDual with H2 T_k stored only as split words at y2
 };
 
 template <typename VT>
@@ -116,6 +123,19 @@ __device__ __forceinline__ void st_stream(V2* p, V2 v, int opts) {
     }
 }
 
+// split-bf16 words of an fp64 complex value, stored non-temporally (the
+// stack entries are read only by the contraction)
+__device__ __forceinline__ void st_split(uint2* p, double2 v) {
+    typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+    const uint2 w = split_word(v);
+    u2v u;
+    u.x = w.x;
+    u.y = w.y;
+    __builtin_nontemporal_store(u, reinterpret_cast<u2v*>(p));
+}
+template <typename V2>
+__device__ __forceinline__ void st_split(uint2*, V2) {}  // fp32 vectors: never instantiated by a split mode
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, kWave);
@@ -176,6 +196,10 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
                                          typename Vec2<VT>::type* y2 = nullptr) {
     using V2 = typename Vec2<VT>::type;
     using H = typename HVal<VT, CH>::type;
+    // the update and moment form of the mode, and its extra outputs
+    constexpr bool FIRST = MODE == kFirst || MODE == kFirstSplit;
+    constexpr bool DUAL = MODE == kDual || MODE == kDualSplit;
+    constexpr bool SPLIT_Y = MODE == kFirstSplit || MODE == kRecDblSplit;
     const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * CPL;
 
     V2 zv[CPL], bv[CPL], acc[CPL], acc2[CPL], xs[CPL];
@@ -188,11 +212,11 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         xs[c].x = 0;
         xs[c].y = 0;
     }
-    if (MODE != kFirst) {
+    if (!FIRST) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) zv[c] = ld_stream(z + own + c, opts);
     }
-    if (MODE == kFirst || MODE == kRec) {
+    if (FIRST || MODE == kRec) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) bv[c] = ld_stream(bra + own + c, opts);
     }
@@ -207,7 +231,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         for (int k = 0; k < 4; ++k) {
             cc[k] = ci[p + k];
             hh[k] = hv[p + k];
-            if (MODE == kDual) hh2[k] = hv2[p + k];
+            if (DUAL) hh2[k] = hv2[p + k];
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -219,7 +243,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         for (int k = 0; k < 4; ++k) {
 #pragma unroll
             for (int c = 0; c < CPL; ++c) madd(acc[c], hh[k], xv[k][c]);
-            if (MODE == kDual) {
+            if (DUAL) {
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) madd(acc2[c], hh2[k], xv[k][c]);
             }
@@ -239,7 +263,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         for (int c = 0; c < CPL; ++c) xv[c] = xr[c];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) madd(acc[c], hh, xv[c]);
-        if (MODE == kDual) {
+        if (DUAL) {
             const H hh2 = hv2[p];
 #pragma unroll
             for (int c = 0; c < CPL; ++c) madd(acc2[c], hh2, xv[c]);
@@ -264,13 +288,15 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             v.x = v.x + beta * xs[c].x;
             v.y = v.y + beta * xs[c].y;
         }
-        if (MODE != kFirst) {
+        if (!FIRST) {
             v.x = v.x - zv[c].x;
             v.y = v.y - zv[c].y;
         }
         st_stream(y + own + c, v, opts);
-        if (MODE == kDual) st_stream(y2 + own + c, acc2[c], opts);
-        if (MODE == kFirst) {
+        if constexpr (MODE == kDual) st_stream(y2 + own + c, acc2[c], opts);
+        if constexpr (SPLIT_Y) st_split(reinterpret_cast<uint2*>(y2) + own + c, v);
+        if constexpr (MODE == kDualSplit) st_split(reinterpret_cast<uint2*>(y2) + own + c, acc2[c]);
+        if (FIRST) {
             p0 = p0 + redot(bv[c], xs[c]);
             p1 = p1 + redot(bv[c], v);
         } else if (MODE == kRec) {

@@ -97,6 +97,7 @@ struct tbk_context {
     engine::DevBuf partial;
     engine::DevBuf sums;
     engine::DevBuf stack[2];                 // Kubo left/right vector stacks, kept between calls
+    engine::DevBuf kwork[3];                 // rolling fp64 blocks of pre-split Kubo stacks
     engine::DevBuf gamma_part, gamma_dev;    // split-K Gamma partials and the reduced Gamma
     std::vector<hipEvent_t> events;
     int64_t held_bytes = 0;  // matrices + blocks owned through this context
@@ -229,7 +230,13 @@ void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan&
                          const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int N,
                          double2* part, hipStream_t st, const void* vx_values);
 bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH, hipStream_t st);
+bool kubo_presplit(int prec, int N, bool h_complex, bool vx_complex);
+void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
+                                  const void* kets, uint2* Ls, uint2* Rs, void* B0, void* B1, void* S, void* W0,
+                                  void* W1, int64_t K, int N, double2* part, hipStream_t st, const void* vx_values);
+void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
+                                  double2* P, double2* out, hipStream_t st);
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
-                            int64_t N, bool fused);
+                            int64_t N, bool fused, bool presplit);
 
 }  // namespace engine

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gamma_launch.h"
+#include "split_bf16.h"
 
 namespace tbk {
 
@@ -348,31 +349,24 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_kernel(const double2* __rest
 constexpr int kSplitKt = 16;        // complex k per stage = one bf16 MFMA k-step
 constexpr int kSplitRowWords = 36;  // 16 hi + 16 lo + 4 pad
 
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-    return static_cast<uint32_t>(__builtin_bit_cast(unsigned short, static_cast<__bf16>(f)));
-}
-
-// (hi, lo) words of one complex value
-__device__ __forceinline__ uint2 split_word(double2 v) {
-    const float hr = static_cast<float>(v.x), hi_ = static_cast<float>(v.y);
-    const __bf16 br = static_cast<__bf16>(hr), bi = static_cast<__bf16>(hi_);
-    const float lr = static_cast<float>(v.x - static_cast<double>(static_cast<float>(br)));
-    const float li = static_cast<float>(v.y - static_cast<double>(static_cast<float>(bi)));
-    uint2 w;
-    w.x = static_cast<uint32_t>(__builtin_bit_cast(unsigned short, br)) |
-          (static_cast<uint32_t>(__builtin_bit_cast(unsigned short, bi)) << 16);
-    w.y = bf16_bits(lr) | (bf16_bits(li) << 16);
-    return w;
-}
 
 // (re, im) -> (im, -re) on a packed bf16 pair
 __device__ __forceinline__ uint32_t swap_neg(uint32_t w) { return ((w >> 16) | (w << 16)) ^ 0x80000000u; }
 
 __device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
 
-__global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const double2* __restrict__ A,
-                                                               const double2* __restrict__ B, int N, int64_t K,
-                                                               int64_t kc, GammaMap map, double2* __restrict__ P) {
+// the (hi, lo) words of an operand element: fp64 values are split here,
+// pre-split stack entries are used as stored
+__device__ __forceinline__ uint2 operand_words(double2 v) { return split_word(v); }
+__device__ __forceinline__ uint2 operand_words(uint2 w) { return w; }
+__device__ __forceinline__ uint2 ld_or_zero(const uint2* p, bool ok) { return ok ? *p : make_uint2(0u, 0u); }
+
+// T = double2: fp64 stacks, split at staging; T = uint2: stacks written
+// pre-split by the recursion (the same words), half the bytes to stage
+template <typename T>
+__global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restrict__ A, const T* __restrict__ B,
+                                                               int N, int64_t K, int64_t kc, GammaMap map,
+                                                               double2* __restrict__ P) {
     __shared__ __align__(16) uint32_t As[2][kBigTile * kSplitRowWords];
     __shared__ __align__(16) uint32_t Bs[2][kBigTile * kSplitRowWords];
     int tn, tm;
@@ -391,8 +385,8 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const double2* __
     // staging: thread t takes complex column t % 16 of rows t / 16 + 32 i, i < 4
     const int col = tid & 15;
     const int r0 = tid >> 4;
-    const double2* ga[4];
-    const double2* gb[4];
+    const T* ga[4];
+    const T* gb[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int r = r0 + 32 * i;
@@ -401,7 +395,7 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const double2* __
         ga[i] = A + static_cast<int64_t>(rn) * K + col;
         gb[i] = B + static_cast<int64_t>(rm) * K + col;
     }
-    double2 ra[4], rb[4];
+    T ra[4], rb[4];
     auto gload = [&](int64_t k0) {
         const bool ok = k0 + col < ke;
 #pragma unroll
@@ -414,8 +408,8 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const double2* __
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int o = (r0 + 32 * i) * kSplitRowWords + col;
-            const uint2 wa = split_word(ra[i]);
-            const uint2 wb = split_word(rb[i]);
+            const uint2 wa = operand_words(ra[i]);
+            const uint2 wb = operand_words(rb[i]);
             As[buf][o] = wa.x;
             As[buf][o + 16] = wa.y;
             Bs[buf][o] = wb.x;
@@ -624,8 +618,28 @@ void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, 
 
 void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
                              double2* P, hipStream_t st) {
-    hipLaunchKernelGGL(gamma_bf16x3_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0, st, A, B,
-                       N, K, kc, map, P);
+    hipLaunchKernelGGL(gamma_bf16x3_big_kernel<double2>, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0,
+                       st, A, B, N, K, kc, map, P);
+}
+
+void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t kc,
+                                      const GammaMap& map, double2* P, hipStream_t st) {
+    hipLaunchKernelGGL(gamma_bf16x3_big_kernel<uint2>, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0,
+                       st, A, B, N, K, kc, map, P);
+}
+
+namespace {
+__global__ __launch_bounds__(256) void split_block_kernel(const double2* __restrict__ in, uint2* __restrict__ out,
+                                                          int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * 256)
+        out[i] = split_word(in[i]);
+}
+}  // namespace
+
+void launch_split_block(const double2* in, uint2* out, int64_t n, hipStream_t st) {
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 65536));
+    hipLaunchKernelGGL(split_block_kernel, dim3(grid), dim3(256), 0, st, in, out, n);
 }
 
 void launch_gamma_weight(double2* G, const double* g, int N, hipStream_t st) {

@@ -158,7 +158,7 @@ struct Slot {
 struct KuboJob {
     Request* req = nullptr;
     tbk_matrix *H = nullptr, *vx = nullptr, *vy = nullptr;
-    DevBuf kets, L, Rs, W0, W1, part, P, G, g, e, z;
+    DevBuf kets, L, Rs, W0, W1, B0, B1, S, part, P, G, g, e, z;
     hipEvent_t done = nullptr;
     int32_t pin = -1;
     ~KuboJob() {
@@ -278,8 +278,11 @@ int64_t tbk_server::kubo_footprint(const Request& r) const {
     const int64_t blk = rows * r.Rp * 16;
     const int64_t N = r.n;
     const ContractionShape cs = contraction_plan(ctx, r.n, rows * r.Rp, r.precision);
+    // pre-split stacks: half-size entries and three more rolling blocks
+    const bool ps = kubo_presplit(r.precision, r.n, r.H.hdr.is_complex != 0, r.vx.hdr.is_complex != 0);
+    const int64_t stacks = ps ? N * blk : 2 * N * blk, rolling = ps ? 5 * blk : 2 * blk;
     return mat_bytes(r.H, false) + mat_bytes(r.vx, false) + mat_bytes(r.vy, false) + blk /* kets */ +
-           2 * N * blk + 2 * blk + static_cast<int64_t>(ctx->num_cu) * 32 * 16 /* step partials */ +
+           stacks + rolling + static_cast<int64_t>(ctx->num_cu) * 32 * 16 /* step partials */ +
            static_cast<int64_t>(cs.nchunks) * N * N * 16 + N * N * 16 + N * 8 * 2 + N * 16;
 }
 
@@ -436,14 +439,24 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     const int rows = r->H.hdr.rows, Rp = r->Rp, N = r->n;
     const int64_t K = static_cast<int64_t>(rows) * Rp;
     const size_t blk = static_cast<size_t>(K) * sizeof(double2);
-    j->kets.reserve(blk);
-    j->L.reserve(static_cast<size_t>(N) * blk);
-    j->Rs.reserve(static_cast<size_t>(N) * blk);
-    j->W0.reserve(blk);
-    j->W1.reserve(blk);
     const StepPlan pH = make_plan(ctx, j->H, TBK_FP64, Rp, rows, r->b, false);
     const StepPlan pX = make_plan(ctx, j->vx, TBK_FP64, Rp, rows, 0.0, false);
     const StepPlan pY = make_plan(ctx, j->vy, TBK_FP64, Rp, rows, 0.0, false);
+    // the footprint counted the pre-split form whenever kubo_presplit holds;
+    // the server's default tuning always takes the row kernel it needs
+    const bool presplit = kubo_presplit(r->precision, N, j->H->is_complex, j->vx->is_complex);
+    if (presplit && (pH.kc.kch != 0 || pX.kc.kch != 0)) raise(TBK_E_UNSUPPORTED, "pre-split stacks need the row kernel");
+    const size_t entry = presplit ? blk / 2 : blk;
+    j->kets.reserve(blk);
+    j->L.reserve(static_cast<size_t>(N) * entry);
+    j->Rs.reserve(static_cast<size_t>(N) * entry);
+    j->W0.reserve(blk);
+    j->W1.reserve(blk);
+    if (presplit) {
+        j->B0.reserve(blk);
+        j->B1.reserve(blk);
+        j->S.reserve(blk);
+    }
     j->part.reserve(static_cast<size_t>(std::max(pH.nb, std::max(pX.nb, pY.nb))) * sizeof(double2));
     const ContractionShape cs = contraction_plan(ctx, N, K, r->precision);
     const int64_t NN = static_cast<int64_t>(N) * N;
@@ -467,10 +480,18 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st_,
                        j->kets.as<double2>(), rows, r->R, Rp, r->seed, inv);
     HIPC(hipGetLastError());
-    enqueue_kubo_stacks(pH, pX, pY, r->a, r->b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, N,
-                        j->part.as<double2>(), st_, fused_velocity(j->H, j->vx, pH, bstream) ? j->vx->val64.p : nullptr);
-    enqueue_contraction(cs, j->L.as<const double2>(), j->Rs.as<const double2>(), N, K, r->precision,
-                        j->P.as<double2>(), j->G.as<double2>(), st_);
+    const void* vxv = fused_velocity(j->H, j->vx, pH, bstream) ? j->vx->val64.p : nullptr;
+    if (presplit) {
+        enqueue_kubo_stacks_presplit(pH, pX, pY, r->a, r->b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->B0.p,
+                                     j->B1.p, j->S.p, j->W0.p, j->W1.p, K, N, j->part.as<double2>(), st_, vxv);
+        enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, j->P.as<double2>(),
+                                     j->G.as<double2>(), st_);
+    } else {
+        enqueue_kubo_stacks(pH, pX, pY, r->a, r->b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, N,
+                            j->part.as<double2>(), st_, vxv);
+        enqueue_contraction(cs, j->L.as<const double2>(), j->Rs.as<const double2>(), N, K, r->precision,
+                            j->P.as<double2>(), j->G.as<double2>(), st_);
+    }
     tbk::launch_gamma_weight(j->G.as<double2>(), j->g.as<const double>(), N, st_);
     HIPC(hipGetLastError());
     tbk::launch_kubo_zeta(j->G.as<const double2>(), N, j->e.as<const double>(), N, 1.0, j->z.as<double2>(), st_);

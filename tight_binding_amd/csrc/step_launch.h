@@ -95,8 +95,8 @@ bool launch_step(dim3 grid, hipStream_t st, const StepArgs& a) {
                            static_cast<VT>(a.alpha), static_cast<VT>(a.beta), a.opts, a.ntiles, a.order, a.xcd_split,
                            a.partial, static_cast<const H*>(a.hv2), static_cast<V2*>(a.y2));
         return true;
-    } else if constexpr (MODE == kDual) {
-        return false;  // the dual product exists for the row kernel only
+    } else if constexpr (MODE >= kDual) {
+        return false;  // the dual product and split outputs exist for the row kernel only
     } else if constexpr (tile_fits(LPR, CPL, KCH, sizeof(H))) {
         hipLaunchKernelGGL((cheb_tile_kernel<VT, CH, LPR, CPL, MODE, KCH>), grid, dim3(kBlock), 0, st, a.rp, a.ci,
                            static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
@@ -114,9 +114,22 @@ bool launch_mode(int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
     switch (mode) {
         case kFirst: return launch_step<VT, CH, LPR, CPL, kFirst, KCH>(grid, st, a);
         case kRec: return launch_step<VT, CH, LPR, CPL, kRec, KCH>(grid, st, a);
+        // the dual product and the pre-split stack outputs: fp64 complex row
+        // kernel (the Kubo stacks) only
         case kDual:
-            // instantiated for the fp64 complex row kernel (the Kubo stacks) only
             if constexpr (KCH == 0 && CH && sizeof(VT) == 8) return launch_step<VT, CH, LPR, CPL, kDual, 0>(grid, st, a);
+            return false;
+        case kFirstSplit:
+            if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
+                return launch_step<VT, CH, LPR, CPL, kFirstSplit, 0>(grid, st, a);
+            return false;
+        case kRecDblSplit:
+            if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
+                return launch_step<VT, CH, LPR, CPL, kRecDblSplit, 0>(grid, st, a);
+            return false;
+        case kDualSplit:
+            if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
+                return launch_step<VT, CH, LPR, CPL, kDualSplit, 0>(grid, st, a);
             return false;
         default: return launch_step<VT, CH, LPR, CPL, kRecDbl, KCH>(grid, st, a);
     }

@@ -520,27 +520,92 @@ void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan&
     if (vx_values && N > 1) launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(N - 1), nullptr, Rk(N - 1), W(N - 1), part);
 }
 
+// Split-BF16x3 solves with N >= 256 on complex operators keep their stacks
+// pre-split: the recursion stores every stack entry as the (hi, lo) bf16
+// words the contraction would form from it (split_bf16.h), 8 bytes per
+// complex, and the contraction stages those words directly.  Gamma is the
+// same bits as from fp64 stacks.
+bool kubo_presplit(int prec, int N, bool h_complex, bool vx_complex) {
+    return prec == TBK_GAMMA_BF16X3 && N >= 2 * tbk::kBigTile && h_complex && vx_complex;
+}
+
+// The left chain runs in two rolling fp64 blocks B0, B1 (B0 = a copy of the
+// kets) and stores each T_n's words at Ls[n]; the right chain is
+// enqueue_kubo_stacks's with R_m stored only as words at Rs[m] (the fp64
+// results of separate v_x launches go to the scratch block S).  K: complex
+// elements per block.
+void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
+                                  const void* kets, uint2* Ls, uint2* Rs, void* B0, void* B1, void* S, void* W0,
+                                  void* W1, int64_t K, int N, double2* part, hipStream_t st, const void* vx_values) {
+    if (pH.kc.kch != 0 || pX.kc.kch != 0 || !pH.cplx || !pX.cplx)
+        raise(TBK_E_UNSUPPORTED, "pre-split stacks need the complex row kernel");
+    auto Lk = [&](int64_t k) { return Ls + k * K; };
+    auto Rk = [&](int64_t k) { return Rs + k * K; };
+    auto B = [&](int64_t k) { return (k & 1) ? B1 : B0; };
+    auto W = [&](int64_t k) { return (k & 1) ? W1 : W0; };
+    HIPC(hipMemcpyAsync(B0, kets, static_cast<size_t>(K) * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    tbk::launch_split_block(static_cast<const double2*>(kets), Lk(0), K, st);
+    HIPC(hipGetLastError());
+    if (N > 1) launch_plan(pH, st, tbk::kFirstSplit, 1.0 / a, -b / a, B(0), nullptr, B(1), B(0), part, nullptr, Lk(1));
+    for (int64_t n = 1; n + 1 < N; ++n)  // T_{n+1} over T_{n-1} in place
+        launch_plan(pH, st, tbk::kRecDblSplit, 2.0 / a, -2.0 * b / a, B(n), B(n + 1), B(n + 1), nullptr, part, nullptr,
+                    Lk(n + 1));
+    launch_plan(pY, st, tbk::kFirst, 1.0, 0.0, kets, nullptr, W(0), kets, part);
+    launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(0), nullptr, S, W(0), part, nullptr, Rk(0));
+    if (N > 1) {
+        launch_plan(pH, st, tbk::kFirst, 1.0 / a, -b / a, W(0), nullptr, W(1), W(0), part);
+        if (!vx_values) launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(1), nullptr, S, W(1), part, nullptr, Rk(1));
+    }
+    for (int64_t m = 2; m < N; ++m) {
+        if (vx_values) {
+            launch_plan(pH, st, tbk::kDualSplit, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part, vx_values,
+                        Rk(m - 1));
+        } else {
+            launch_plan(pH, st, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part);
+            launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(m), nullptr, S, W(m), part, nullptr, Rk(m));
+        }
+    }
+    if (vx_values && N > 1)
+        launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(N - 1), nullptr, S, W(N - 1), part, nullptr, Rk(N - 1));
+}
+
+void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
+                                  double2* P, double2* out, hipStream_t st) {
+    if (!cs.big) raise(TBK_E_UNSUPPORTED, "pre-split contraction needs N >= 256");
+    tbk::launch_gamma_bf16x3_big_presplit(A, B, N, K, cs.kc, cs.map, P, st);
+    HIPC(hipGetLastError());
+    tbk::launch_gamma_reduce(P, cs.nchunks, static_cast<int64_t>(N) * N, out, st);
+    HIPC(hipGetLastError());
+}
+
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
-                            int64_t N, bool fused) {
-    // matrix arrays once per launch; vector blocks: x (+ z) read and y
-    // written per step, x read and y written per velocity application
+                            int64_t N, bool fused, bool presplit) {
+    // per launch: the matrix arrays once; vector blocks x (+ z) read and y
+    // written; split words are half a block
     auto mat = [&](const tbk_matrix* m) {
         return static_cast<double>(m->nnz) * ((m->is_complex ? 16.0 : 8.0) + 4.0) +
                (static_cast<double>(rows) + 1.0) * 4.0;
     };
-    const double blk = static_cast<double>(rows) * R * 16.0;
+    const double blk = static_cast<double>(rows) * R * 16.0, half = 0.5 * blk;
+    const double split = presplit ? half : 0.0;
+    const double vals = static_cast<double>(vx->nnz) * 16.0;
     const double Nm2 = static_cast<double>(std::max<int64_t>(0, N - 2));
-    double bytes = 2.0 * blk;                              // |r> into the left stack
-    bytes += mat(vy) + 2.0 * blk + mat(vx) + 2.0 * blk;    // v_x v_y |r>
-    if (N > 1) bytes += 2.0 * (mat(H) + 2.0 * blk) + mat(vx) + 2.0 * blk;
+    double bytes = 2.0 * blk;  // the kets copied (into the stack, or the rolling block)
+    // left chain
+    if (presplit) bytes += blk + half;  // the kets' words
+    if (N > 1) bytes += mat(H) + 2.0 * blk + split;
+    bytes += Nm2 * (mat(H) + 3.0 * blk + split);
+    // right chain; a separate v_x launch writes fp64 (and the words)
+    const double rout = presplit ? blk + half : blk;
+    bytes += mat(vy) + 2.0 * blk;        // v_y |r>
+    bytes += mat(vx) + blk + rout;       // R_0
+    if (N > 1) bytes += mat(H) + 2.0 * blk;
     if (fused && N > 1) {
-        // each fused step also reads v_x's values and writes one right block;
-        // the last v_x product is a separate launch
-        const double vals = static_cast<double>(vx->nnz) * 16.0;
-        bytes -= mat(vx) + 2.0 * blk;  // the v_x launch for T_1 is folded into the first fused step
-        bytes += Nm2 * ((mat(H) + 3.0 * blk) + (mat(H) + vals + 4.0 * blk)) + mat(vx) + 2.0 * blk;
+        bytes += Nm2 * (mat(H) + vals + 3.0 * blk + (presplit ? half : blk));  // T_m and R_{m-1}
+        bytes += mat(vx) + blk + rout;                                        // R_{N-1}
     } else {
-        bytes += Nm2 * (2.0 * (mat(H) + 3.0 * blk) + mat(vx) + 2.0 * blk);
+        if (N > 1) bytes += mat(vx) + blk + rout;  // R_1
+        bytes += Nm2 * (mat(H) + 3.0 * blk + mat(vx) + blk + rout);
     }
     return bytes;
 }
@@ -571,6 +636,28 @@ double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int
     return ms;
 }
 
+// gamma = A^H B over pre-split stacks (N x K words each), into host gamma_out.
+double contract_stacks_presplit(tbk_context* ctx, const uint2* A, const uint2* B, int N, int64_t K,
+                                double* gamma_out, int* nchunks_out) {
+    const ContractionShape cs = contraction_plan(ctx, N, K, TBK_GAMMA_BF16X3);
+    const int64_t NN = static_cast<int64_t>(N) * N;
+    DevBuf& P = ctx->gamma_part;
+    DevBuf& out = ctx->gamma_dev;
+    P.reserve(static_cast<size_t>(cs.nchunks) * NN * sizeof(double2));
+    out.reserve(static_cast<size_t>(NN) * sizeof(double2));
+    ensure_events(ctx, 2);
+    HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+    enqueue_contraction_presplit(cs, A, B, N, K, P.as<double2>(), out.as<double2>(), ctx->stream);
+    HIPC(hipEventRecord(ctx->events[1], ctx->stream));
+    HIPC(hipMemcpyAsync(gamma_out, out.p, static_cast<size_t>(NN) * sizeof(double2), hipMemcpyDeviceToHost,
+                        ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+    if (nchunks_out) *nchunks_out = cs.nchunks;
+    return ms;
+}
+
 void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const tbk_matrix* vx, const tbk_matrix* vy,
               const tbk_block* kets, int32_t N, int32_t prec, const double* g, double* gamma_out,
               tbk_kubo_stats* stats) {
@@ -587,7 +674,13 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     const int rows = kets->rows, Rp = kets->Rp;
     const int64_t K = static_cast<int64_t>(rows) * Rp;  // complex columns of one stacked block
     const size_t blk_bytes = static_cast<size_t>(K) * sizeof(double2);
-    const size_t stack_bytes = static_cast<size_t>(N) * blk_bytes;
+    const StepPlan pH = make_plan(ctx, H, TBK_FP64, Rp, rows, b, false);
+    const StepPlan pX = make_plan(ctx, vx, TBK_FP64, Rp, rows, 0.0, false);
+    const StepPlan pY = make_plan(ctx, vy, TBK_FP64, Rp, rows, 0.0, false);
+    const bool presplit = kubo_presplit(prec, N, H->is_complex, vx->is_complex) && pH.kc.kch == 0 && pX.kc.kch == 0;
+    const size_t entry_bytes = presplit ? blk_bytes / 2 : blk_bytes;  // one stack entry
+    const size_t stack_bytes = static_cast<size_t>(N) * entry_bytes;
+    const int rolling = presplit ? 5 : 2;  // fp64 work blocks
     DevBuf& L = ctx->stack[0];
     DevBuf& Rs = ctx->stack[1];
     if (L.bytes < stack_bytes || Rs.bytes < stack_bytes) {
@@ -598,7 +691,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         Rs.release();
         size_t free_b = 0, total_b = 0;
         HIPC(hipMemGetInfo(&free_b, &total_b));
-        const double need = 2.0 * static_cast<double>(stack_bytes) + 2.0 * blk_bytes;
+        const double need = 2.0 * static_cast<double>(stack_bytes) + rolling * static_cast<double>(blk_bytes);
         if (need > 0.95 * static_cast<double>(free_b))
             raise(TBK_E_OOM, "Kubo stacks need " + std::to_string(need / 1e9) + " GB, " +
                                  std::to_string(static_cast<double>(free_b) / 1e9) + " GB free");
@@ -607,24 +700,30 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     }
     ctx->work[0].reserve(blk_bytes);
     ctx->work[1].reserve(blk_bytes);
-    const StepPlan pH = make_plan(ctx, H, TBK_FP64, Rp, rows, b, false);
-    const StepPlan pX = make_plan(ctx, vx, TBK_FP64, Rp, rows, 0.0, false);
-    const StepPlan pY = make_plan(ctx, vy, TBK_FP64, Rp, rows, 0.0, false);
+    if (presplit)
+        for (DevBuf& w : ctx->kwork) w.reserve(blk_bytes);
     ctx->partial.reserve(static_cast<size_t>(std::max(pH.nb, std::max(pX.nb, pY.nb))) * sizeof(double2));
     double2* part = ctx->partial.as<double2>();
+    const bool fuse = fused_velocity(H, vx, pH, ctx->stream);
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    const bool fuse = fused_velocity(H, vx, pH, ctx->stream);
-    enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, N, part,
-                        ctx->stream, fuse ? vx->val64.p : nullptr);
+    if (presplit)
+        enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
+                                     ctx->kwork[1].p, ctx->kwork[2].p, ctx->work[0].p, ctx->work[1].p, K, N, part,
+                                     ctx->stream, fuse ? vx->val64.p : nullptr);
+    else
+        enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, N,
+                            part, ctx->stream, fuse ? vx->val64.p : nullptr);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
     float rec_ms = 0.f;
     HIPC(hipEventElapsedTime(&rec_ms, ctx->events[0], ctx->events[1]));
 
     int nchunks = 0;
-    const double con_ms = contract_stacks(ctx, static_cast<const double2*>(L.p), static_cast<const double2*>(Rs.p),
-                                          N, K, prec, gamma_out, &nchunks);
+    const double con_ms = presplit ? contract_stacks_presplit(ctx, L.as<const uint2>(), Rs.as<const uint2>(), N, K,
+                                                              gamma_out, &nchunks)
+                                   : contract_stacks(ctx, static_cast<const double2*>(L.p),
+                                                     static_cast<const double2*>(Rs.p), N, K, prec, gamma_out, &nchunks);
     if (g) {
         for (int64_t n = 0; n < N; ++n)
             for (int64_t m = 0; m < N; ++m) {
@@ -639,7 +738,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         stats->flops = 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
         stats->stack_bytes = 2.0 * static_cast<double>(stack_bytes);
         stats->nchunks = nchunks;
-        const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N, fuse);
+        const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N, fuse, presplit);
         stats->recursion_bytes = bytes;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -760,7 +859,7 @@ int tbk_context_release_work(tbk_context* ctx) {
         set_device(ctx);
         HIPC(hipStreamSynchronize(ctx->stream));
         for (DevBuf* b : {&ctx->work[0], &ctx->work[1], &ctx->partial, &ctx->sums, &ctx->stack[0], &ctx->stack[1],
-                          &ctx->gamma_part, &ctx->gamma_dev})
+                          &ctx->kwork[0], &ctx->kwork[1], &ctx->kwork[2], &ctx->gamma_part, &ctx->gamma_dev})
             b->release();
     });
 }
