@@ -133,8 +133,21 @@ __device__ __forceinline__ void st_split(uint2* p, double2 v) {
     u.y = w.y;
     __builtin_nontemporal_store(u, reinterpret_cast<u2v*>(p));
 }
+// two consecutive values' words as one 16-byte store (p 16-byte aligned)
+__device__ __forceinline__ void st_split2(uint2* p, double2 v0, double2 v1) {
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    const uint2 w0 = split_word(v0), w1 = split_word(v1);
+    u4v u;
+    u.x = w0.x;
+    u.y = w0.y;
+    u.z = w1.x;
+    u.w = w1.y;
+    __builtin_nontemporal_store(u, reinterpret_cast<u4v*>(p));
+}
 template <typename V2>
 __device__ __forceinline__ void st_split(uint2*, V2) {}  // fp32 vectors: never instantiated by a split mode
+template <typename V2>
+__device__ __forceinline__ void st_split2(uint2*, V2, V2) {}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -202,7 +215,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
     constexpr bool SPLIT_Y = MODE == kFirstSplit || MODE == kRecDblSplit;
     const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * CPL;
 
-    V2 zv[CPL], bv[CPL], acc[CPL], acc2[CPL], xs[CPL];
+    V2 zv[CPL], bv[CPL], acc[CPL], acc2[CPL], xs[CPL], sv[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
         acc[c].x = 0;
@@ -294,8 +307,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         }
         st_stream(y + own + c, v, opts);
         if constexpr (MODE == kDual) st_stream(y2 + own + c, acc2[c], opts);
-        if constexpr (SPLIT_Y) st_split(reinterpret_cast<uint2*>(y2) + own + c, v);
-        if constexpr (MODE == kDualSplit) st_split(reinterpret_cast<uint2*>(y2) + own + c, acc2[c]);
+        if constexpr (SPLIT_Y) sv[c] = v;
         if (FIRST) {
             p0 = p0 + redot(bv[c], xs[c]);
             p1 = p1 + redot(bv[c], v);
@@ -304,6 +316,22 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         } else {
             p0 = p0 + redot(xs[c], xs[c]);
             p1 = p1 + redot(v, xs[c]);
+        }
+    }
+    // split words of the pre-split stacks: pairs of complex values as one
+    // 16-byte store (own is even when CPL is)
+    if constexpr (SPLIT_Y || MODE == kDualSplit) {
+        uint2* ws = reinterpret_cast<uint2*>(y2) + own;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            if (CPL % 2 == 0 && (c & 1)) continue;
+            const V2& a0 = SPLIT_Y ? sv[c] : acc2[c];
+            if (CPL % 2 == 0) {
+                const V2& a1 = SPLIT_Y ? sv[c + 1] : acc2[c + 1];
+                st_split2(ws + c, a0, a1);
+            } else {
+                st_split(ws + c, a0);
+            }
         }
     }
 }
