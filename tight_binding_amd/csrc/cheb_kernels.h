@@ -123,6 +123,67 @@ __device__ __forceinline__ void st_stream(V2* p, V2 v, int opts) {
     }
 }
 
+// The CPL consecutive values of one lane.  8-byte elements (complex64) go in
+// pairs through 16-byte accesses: p is 16-byte aligned whenever CPL is even,
+// since lane offsets are multiples of CPL and rows multiples of Rp.
+template <int CPL, typename V2>
+__device__ __forceinline__ void ld_lane(const V2* p, V2* out) {
+    if constexpr (sizeof(V2) == 8 && CPL % 2 == 0) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int c = 0; c < CPL; c += 2) {
+            const f4 w = *reinterpret_cast<const f4*>(p + c);
+            out[c].x = w.x;
+            out[c].y = w.y;
+            out[c + 1].x = w.z;
+            out[c + 1].y = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) out[c] = p[c];
+    }
+}
+template <int CPL, typename V2>
+__device__ __forceinline__ void ld_lane_stream(const V2* p, V2* out, int opts) {
+    if constexpr (sizeof(V2) == 8 && CPL % 2 == 0) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int c = 0; c < CPL; c += 2) {
+            const f4* q = reinterpret_cast<const f4*>(p + c);
+            const f4 w = (opts & kOptStream) ? __builtin_nontemporal_load(q) : *q;
+            out[c].x = w.x;
+            out[c].y = w.y;
+            out[c + 1].x = w.z;
+            out[c + 1].y = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) out[c] = ld_stream(p + c, opts);
+    }
+}
+template <int CPL, typename V2>
+__device__ __forceinline__ void st_lane_stream(V2* p, const V2* v, int opts) {
+    if constexpr (sizeof(V2) == 8 && CPL % 2 == 0) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int c = 0; c < CPL; c += 2) {
+            f4 w;
+            w.x = v[c].x;
+            w.y = v[c].y;
+            w.z = v[c + 1].x;
+            w.w = v[c + 1].y;
+            f4* q = reinterpret_cast<f4*>(p + c);
+            if (opts & kOptStream)
+                __builtin_nontemporal_store(w, q);
+            else
+                *q = w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) st_stream(p + c, v[c], opts);
+    }
+}
+
 // split-bf16 words of an fp64 complex value, stored non-temporally (the
 // stack entries are read only by the contraction)
 __device__ __forceinline__ void st_split(uint2* p, double2 v) {
@@ -225,14 +286,8 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         xs[c].x = 0;
         xs[c].y = 0;
     }
-    if (!FIRST) {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) zv[c] = ld_stream(z + own + c, opts);
-    }
-    if (FIRST || MODE == kRec) {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) bv[c] = ld_stream(bra + own + c, opts);
-    }
+    if (!FIRST) ld_lane_stream<CPL>(z + own, zv, opts);
+    if (FIRST || MODE == kRec) ld_lane_stream<CPL>(bra + own, bv, opts);
     const int s = rp[r], e = rp[r + 1];
     bool have_self = false;
     int p = s;
@@ -248,9 +303,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const V2* xr = x + static_cast<size_t>(cc[k]) * Rp + static_cast<size_t>(q) * CPL;
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) xv[k][c] = xr[c];
+            ld_lane<CPL>(x + static_cast<size_t>(cc[k]) * Rp + static_cast<size_t>(q) * CPL, xv[k]);
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -270,10 +323,8 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
     for (; p < e; ++p) {
         const int cc = ci[p];
         const H hh = hv[p];
-        const V2* xr = x + static_cast<size_t>(cc) * Rp + static_cast<size_t>(q) * CPL;
         V2 xv[CPL];
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) xv[c] = xr[c];
+        ld_lane<CPL>(x + static_cast<size_t>(cc) * Rp + static_cast<size_t>(q) * CPL, xv);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) madd(acc[c], hh, xv[c]);
         if (DUAL) {
@@ -287,10 +338,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             for (int c = 0; c < CPL; ++c) xs[c] = xv[c];
         }
     }
-    if (!have_self) {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) xs[c] = x[own + c];
-    }
+    if (!have_self) ld_lane<CPL>(x + own, xs);
 
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
@@ -305,9 +353,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             v.x = v.x - zv[c].x;
             v.y = v.y - zv[c].y;
         }
-        st_stream(y + own + c, v, opts);
-        if constexpr (MODE == kDual) st_stream(y2 + own + c, acc2[c], opts);
-        if constexpr (SPLIT_Y) sv[c] = v;
+        sv[c] = v;
         if (FIRST) {
             p0 = p0 + redot(bv[c], xs[c]);
             p1 = p1 + redot(bv[c], v);
@@ -318,6 +364,8 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             p1 = p1 + redot(v, xs[c]);
         }
     }
+    st_lane_stream<CPL>(y + own, sv, opts);
+    if constexpr (MODE == kDual) st_lane_stream<CPL>(y2 + own, acc2, opts);
     // split words of the pre-split stacks: pairs of complex values as one
     // 16-byte store (own is even when CPL is)
     if constexpr (SPLIT_Y || MODE == kDualSplit) {

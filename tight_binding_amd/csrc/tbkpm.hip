@@ -1133,7 +1133,11 @@ int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
             {&tbk_tuning::chunk_period, {0, -1}},
             {&tbk_tuning::xcd_split, {0, -1}},
             {&tbk_tuning::stream_nt, {1, -1}},
-            {&tbk_tuning::row_lane_bytes, {8, 16, 32}},
+            // lanes narrower than one element are the same kernel as 16 bytes in
+            // fp64; offering them only lets noise pick a width that would slow
+            // fp32 work sharing the context
+            {&tbk_tuning::row_lane_bytes,
+             kets->precision == TBK_FP64 ? std::vector<int32_t>{16, 32} : std::vector<int32_t>{8, 16, 32}},
         };
         for (int pass = 0; pass < 2; ++pass) {
             for (const auto& k : knobs) {
@@ -1142,7 +1146,7 @@ int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
                     tbk_tuning t = cur;
                     t.*(k.field) = v;
                     const double ms = trial(t);
-                    if (ms < cur_ms) {
+                    if (ms < 0.99 * cur_ms) {  // a change must win by more than 1%
                         cur_ms = ms;
                         cur = t;
                     }
