@@ -116,6 +116,19 @@ int tbk_cheb_first_kind_seq_braket(const tbk_sparse* M, int32_t n, const double*
                                    const double* kets, int32_t ket_rows, int32_t ket_cols,
                                    double* mu_out);
 
+/*
+ * Host-buffer drop-in for Chebyshev::Cheb_first_kind_seq_on_bases
+ * (numerical.h:334-344) with a sparse M (used as given, already rescaled):
+ *   out block k = T_k(M) v, k = 0 .. n-1,
+ * n consecutive v_rows x v_cols column-major complex128 blocks.  FP64 on the
+ * device in two rolling blocks, each T_k copied to out once formed, so device
+ * memory stays at three blocks whatever n is (tbk_cheb_seq_on_bases keeps
+ * every block resident instead).  n >= 1; n = 1 gives v.  Uses the
+ * process-wide context on device 0.
+ */
+int tbk_cheb_first_kind_seq_on_bases(const tbk_sparse* M, int32_t n, const double* v, int32_t v_rows,
+                                     int32_t v_cols, double* out);
+
 /* ------------------------------------------------ device-resident objects */
 typedef struct tbk_context tbk_context;
 typedef struct tbk_matrix tbk_matrix;
@@ -215,6 +228,19 @@ int tbk_block_destroy(tbk_block* b);
 int tbk_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b,
                 const tbk_block* bras, const tbk_block* kets, int32_t n, int32_t flags,
                 double* mu_out, double* last_out, tbk_stats* stats);
+
+/*
+ * Stored Chebyshev blocks, device-resident: replaces
+ * Chebyshev::Cheb_first_kind_seq_on_bases (numerical.h:334-344) for sparse M.
+ *   out[0] = v,  out[1] = H~ v,  out[k] = 2 H~ out[k-1] - out[k-2],  k < n,
+ * H~ = (M - b I)/a.  out holds n blocks made by the caller with v's rows, R
+ * and precision, pairwise distinct; out[0] may be v itself (no copy), no
+ * other may.  The blocks have the bits of the vectors tbk_moments forms.
+ * kernel_ms (optional): device time of the n - 1 launches.  n >= 1 (the
+ * reference's loop writes past its vector for n < 2).
+ */
+int tbk_cheb_seq_on_bases(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* v,
+                          int32_t n, tbk_block* const* out, float* kernel_ms);
 
 /*
  * Autotuner: times `steps` recursion steps of tbk_moments for M and kets under

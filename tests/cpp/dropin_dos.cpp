@@ -1,7 +1,9 @@
 // A caller written against the reference's API (models, UnitCell,
 // PartialLattice, GreenFunction, density_of_states_seq -- the flow of
 // snippets.h:743-771 / functions.h:16-33), built on this package's headers.
-// Prints the N moments of GreenFunctionImag, then DoS at a few energies.
+// Prints the N moments of GreenFunctionImag, the expansion at the Chebyshev
+// nodes, stored blocks T_k(H~) kets, DoS at a few energies and the spectral
+// range.
 //
 // usage: dropin_dos <L> <N>
 #include <cstdio>
@@ -40,6 +42,14 @@ int main(int argc, char** argv) {
     // (GreenFunction(N, HE, ref_HE, epsilon), lattice.h:560-569)
     PartialLattice<2>::GreenFunction<Complex> G3(N, H, H);
     std::printf("range %.17g %.17g %.17g %.17g\n", G3.E_min, G3.E_max, G3.a, G3.b);
+
+    // stored blocks T_k(scaled_HE) kets, k < 6 (Cheb_first_kind_seq_on_bases):
+    // column 1 of each
+    auto Tn = numerical::Chebyshev::Cheb_first_kind_seq_on_bases(G.scaled_HE, 6, kets);
+    for (size_t k = 0; k < Tn.size(); ++k)
+        for (int64_t i = 0; i < Tn[k].rows(); ++i)
+            std::printf("tn %zu %lld %.17g %.17g\n", k, static_cast<long long>(i), Tn[k](i, 1).real(),
+                        Tn[k](i, 1).imag());
 
     // n < 2 keeps the reference's behaviour: message and an empty result
     auto none = cuda::Cheb_first_kind_seq_braket(G.scaled_HE, 1, kets, kets);

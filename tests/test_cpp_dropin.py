@@ -77,6 +77,13 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     want = dct(gL * ref, type=3) / np.sin((np.arange(N) + 0.5) * np.pi / N)
     assert np.max(np.abs(fl[:, 0] - Ek)) <= 1e-15  # libm and numpy cosines may differ in the last bit
     assert np.max(np.abs(fl[:, 1] - want)) <= 1e-10 * np.max(np.abs(want))
+    # Cheb_first_kind_seq_on_bases(scaled_HE, 6, kets): column 1 of each block
+    tn = np.array([[float(v) for v in x.split()[1:]] for x in lines if x.startswith("tn ")])
+    assert tn.shape == (6 * n, 4)
+    got = (tn[:, 2] + 1j * tn[:, 3]).reshape(6, n)
+    want = oracle.cheb_on_bases(Hs, 6, k1)[:, :, 1]
+    assert np.array_equal(got[0], k1[:, 1])
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
     # density_of_states_seq draws the next seed and uses Lorentz(N, 4)
     k2 = cpp_kets(n, 4, 0x5EED + 1)
     mu2 = oracle.cheb_braket(Hs, N, k2, k2)

@@ -293,6 +293,32 @@ def test_oracle_chebyshev_product_identity_and_stored_blocks():
     assert np.all(np.abs(mu) <= mu[0] + 1e-12)
 
 
+def test_oracle_stored_blocks_match_eigendecomposition():
+    """T_k(H) V = U cos(k arccos Lambda) U^H V for Hermitian H with spectrum
+    in [-1, 1] (complex Landau-gauge hoppings, so a transposed or unconjugated
+    operand fails); n = 1 is V alone."""
+    cell = O.model("square")
+    cell.add_hopping_nn("A", 1, 1.0)
+    lat = O.PartialLattice(cell, (6, 5))
+    lat.set_scalar_field_indexed(O.uniform_disorder(1.0, 2))
+    lat.gauge_field = O.landau_gauge(2 * math.pi / 7)
+    H = lat.get_Hamiltonian_sparse(True)
+    Hs = O.rescale(H, 5.1, 0.1)
+    D = Hs.to_dense()
+    assert np.abs(D - D.conj().T).max() == 0 and np.abs(D.imag).max() > 0.01
+    lam, U = np.linalg.eigh(D)
+    assert np.max(np.abs(lam)) < 1
+    rng = np.random.default_rng(3)
+    V = rng.standard_normal((Hs.n, 3)) + 1j * rng.standard_normal((Hs.n, 3))
+    n = 40
+    T = oracle.cheb_on_bases(Hs, n, V)
+    W = U.conj().T @ V
+    for k in range(n):
+        want = U @ (np.cos(k * np.arccos(lam))[:, None] * W)
+        assert np.max(np.abs(T[k] - want)) <= 1e-12 * np.max(np.abs(V)) * (k + 1), k
+    assert np.array_equal(oracle.cheb_on_bases(Hs, 1, V)[0], V)
+
+
 def test_trace_braket_matches_numpy():
     rng = np.random.default_rng(1)
     B = rng.standard_normal((50, 7)) + 1j * rng.standard_normal((50, 7))

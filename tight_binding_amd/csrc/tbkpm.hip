@@ -418,6 +418,74 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
 }
 
 
+// ------------------------------------------------- stored Chebyshev blocks
+// Device block (rows x Rp, row-major) to column-major complex128 on the host.
+void block_to_colmajor(const void* src, int prec, int rows, int R, int Rp, double* out) {
+    const size_t elems = static_cast<size_t>(rows) * Rp;
+    auto put = [&](const auto& h) {
+        for (int r = 0; r < R; ++r)
+            for (int i = 0; i < rows; ++i) {
+                const size_t s = 2 * (static_cast<size_t>(r) * rows + i);
+                out[s] = h[static_cast<size_t>(i) * Rp + r].x;
+                out[s + 1] = h[static_cast<size_t>(i) * Rp + r].y;
+            }
+    };
+    if (prec == TBK_FP64) {
+        std::vector<double2> h(elems);
+        HIPC(hipMemcpy(h.data(), src, elems * sizeof(double2), hipMemcpyDeviceToHost));
+        put(h);
+    } else {
+        std::vector<float2> h(elems);
+        HIPC(hipMemcpy(h.data(), src, elems * sizeof(float2), hipMemcpyDeviceToHost));
+        put(h);
+    }
+}
+
+void check_on_bases_args(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* v,
+                         int32_t n) {
+    if (!ctx || !M || !v) raise(TBK_E_INVALID, "null argument");
+    if (n < 1) raise(TBK_E_INVALID, "n must be at least 1");
+    if (!(a != 0.0) || !std::isfinite(a) || !std::isfinite(b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
+    if (M->ctx != ctx || v->ctx != ctx) raise(TBK_E_INVALID, "objects belong to another context");
+    if (v->rows != M->rows) raise(TBK_E_INVALID, "block rows do not match the matrix");
+}
+
+// Stored blocks T_k(M~) v, M~ = (M - b)/a, k = 0..n-1
+// (Cheb_first_kind_seq_on_bases, numerical.h:334-344): dst(k) is the device
+// memory of T_k, laid out as v.  dst(0) gets a copy of v (skipped when it is
+// v's own memory); each k >= 1 is one launch, T_1 = M~ T_0 and
+// T_k = 2 M~ T_{k-1} - T_{k-2}, the arithmetic of run_moments's steps.
+// after(k) runs on the host once T_k is enqueued (the host-buffer drop-in
+// downloads there).  Returns the milliseconds between the first and the
+// last launch's events.
+template <typename Dst, typename After>
+double enqueue_on_bases(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* v, int32_t n,
+                        Dst dst, After after) {
+    set_device(ctx);
+    const StepPlan plan = make_plan(ctx, M, v->precision, v->Rp, v->rows, b, false);
+    ctx->partial.reserve(static_cast<size_t>(plan.nb) * sizeof(double2));
+    double2* part = ctx->partial.as<double2>();
+    const size_t block_bytes = static_cast<size_t>(v->rows) * v->Rp * v->elem_bytes();
+    ensure_events(ctx, 2);
+    if (dst(0) != v->data.p)
+        HIPC(hipMemcpyAsync(dst(0), v->data.p, block_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    after(0);
+    HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+    for (int64_t k = 1; k < n; ++k) {
+        if (k == 1)
+            launch_plan(plan, ctx->stream, tbk::kFirst, 1.0 / a, -b / a, dst(0), nullptr, dst(1), dst(0), part);
+        else
+            launch_plan(plan, ctx->stream, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, dst(k - 1), dst(k - 2), dst(k),
+                        nullptr, part);
+        after(k);
+    }
+    HIPC(hipEventRecord(ctx->events[1], ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+    return ms;
+}
+
 // ------------------------------------------------------ Kubo-Bastin Gamma
 // Split-K shape of the contraction: enough workgroups to fill the device,
 // chunks a multiple of 16 complex columns.
@@ -1097,6 +1165,31 @@ int tbk_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const
     return guarded([&] { run_moments(ctx, M, a, b, bras, kets, n, flags, mu_out, last_out, stats); });
 }
 
+int tbk_cheb_seq_on_bases(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* v,
+                          int32_t n, tbk_block* const* out, float* kernel_ms) {
+    return guarded([&] {
+        check_on_bases_args(ctx, M, a, b, v, n);
+        if (!out) raise(TBK_E_INVALID, "null argument");
+        std::vector<const tbk_block*> seen;
+        seen.reserve(static_cast<size_t>(n));
+        for (int32_t k = 0; k < n; ++k) {
+            const tbk_block* o = out[k];
+            if (!o) raise(TBK_E_INVALID, "null output block");
+            if (o->ctx != ctx) raise(TBK_E_INVALID, "objects belong to another context");
+            if (o->rows != v->rows || o->R != v->R || o->Rp != v->Rp || o->precision != v->precision)
+                raise(TBK_E_INVALID, "output block shape or precision does not match v");
+            if (k > 0 && o == v) raise(TBK_E_INVALID, "only out[0] may be v itself");
+            seen.push_back(o);
+        }
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+            raise(TBK_E_INVALID, "output blocks must be distinct");
+        const double ms = enqueue_on_bases(
+            ctx, M, a, b, v, n, [&](int64_t k) -> void* { return out[k]->data.p; }, [](int64_t) {});
+        if (kernel_ms) *kernel_ms = static_cast<float>(ms);
+    });
+}
+
 int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
                  int32_t steps, tbk_tuning* best, double* best_ms_per_step) {
     return guarded([&] {
@@ -1243,6 +1336,41 @@ int tbk_cheb_first_kind_seq_braket(const tbk_sparse* M, int32_t n, const double*
             if (rc != TBK_OK) raise(rc, g_last_error);
         }
         run_moments(ctx, g.m, 1.0, 0.0, g.b, g.k, n, 0, mu_out, nullptr, nullptr);
+    });
+}
+
+int tbk_cheb_first_kind_seq_on_bases(const tbk_sparse* M, int32_t n, const double* v, int32_t v_rows,
+                                     int32_t v_cols, double* out) {
+    return guarded([&] {
+        if (!M || !v || !out) raise(TBK_E_INVALID, "null argument");
+        if (n < 1) raise(TBK_E_INVALID, "n must be at least 1");
+        if (v_rows != M->rows || v_rows != M->cols) raise(TBK_E_INVALID, "v rows do not match M");
+        tbk_context* ctx = default_context();
+        struct Guard {
+            tbk_matrix* m = nullptr;
+            tbk_block* k = nullptr;
+            ~Guard() {
+                if (k) tbk_block_destroy(k);
+                if (m) tbk_matrix_destroy(m);
+            }
+        } g;
+        int rc = tbk_matrix_create(ctx, M, &g.m);
+        if (rc != TBK_OK) raise(rc, g_last_error);
+        rc = tbk_block_create(ctx, v, v_rows, v_cols, TBK_FP64, &g.k);
+        if (rc != TBK_OK) raise(rc, g_last_error);
+        const tbk_block* vb = g.k;
+        const size_t block_bytes = static_cast<size_t>(vb->rows) * vb->Rp * vb->elem_bytes();
+        ctx->work[0].reserve(block_bytes);
+        ctx->work[1].reserve(block_bytes);
+        // T_k over T_{k-2} in two rolling blocks, each downloaded once formed
+        auto dst = [&](int64_t k) -> void* { return k == 0 ? vb->data.p : ctx->work[k & 1].p; };
+        const size_t blk = 2 * static_cast<size_t>(v_rows) * v_cols;
+        std::copy(v, v + blk, out);
+        enqueue_on_bases(ctx, g.m, 1.0, 0.0, vb, n, dst, [&](int64_t k) {
+            if (k == 0) return;
+            HIPC(hipStreamSynchronize(ctx->stream));
+            block_to_colmajor(dst(k), TBK_FP64, vb->rows, vb->R, vb->Rp, out + static_cast<size_t>(k) * blk);
+        });
     });
 }
 

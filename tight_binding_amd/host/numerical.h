@@ -97,6 +97,42 @@ inline std::vector<Complex> expn_coef_alpha_seq(Complex z, int n) {
     return g;
 }
 
+// [T_0(x) v, ..., T_{n-1}(x) v] for a sparse x (already rescaled) and a
+// rows x R complex block v (numerical.h:334-344), formed in FP64 on the
+// device (tbk_cheb_first_kind_seq_on_bases).  n = 1 gives {v}; n < 1 and
+// device failures throw (the reference's loop writes past its vector for
+// n < 2).  A template so that code which never calls it does not link
+// libtbkpm.
+template <typename Sparse>
+std::vector<MatrixXcd> Cheb_first_kind_seq_on_bases(const Sparse& x, int n, const MatrixXcd& v) {
+    using Scalar = std::remove_cv_t<std::remove_pointer_t<decltype(x.valuePtr())>>;
+    tbk_sparse s{};
+    s.rows = static_cast<int32_t>(x.rows());
+    s.cols = static_cast<int32_t>(x.cols());
+    s.nnz = static_cast<int64_t>(x.nonZeros());
+    s.outer = reinterpret_cast<const int32_t*>(x.outerIndexPtr());
+    s.inner = reinterpret_cast<const int32_t*>(x.innerIndexPtr());
+    s.values = reinterpret_cast<const double*>(x.valuePtr());
+    s.is_complex = std::is_same_v<Scalar, Complex> ? 1 : 0;
+    s.layout = TBK_CSC;
+    if (n < 1) throw std::invalid_argument("Cheb_first_kind_seq_on_bases: n must be at least 1");
+    const size_t blk = static_cast<size_t>(v.rows() * v.cols());
+    std::vector<Complex> all(static_cast<size_t>(n) * blk);
+    if (tbk_cheb_first_kind_seq_on_bases(&s, n, reinterpret_cast<const double*>(v.data()),
+                                         static_cast<int32_t>(v.rows()), static_cast<int32_t>(v.cols()),
+                                         reinterpret_cast<double*>(all.data())) != TBK_OK)
+        throw std::runtime_error(std::string("Cheb_first_kind_seq_on_bases: ") + tbk_last_error());
+    std::vector<MatrixXcd> Tn;
+    Tn.reserve(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) {
+        MatrixXcd t(v.rows(), v.cols());
+        std::copy(all.begin() + static_cast<std::ptrdiff_t>(k * blk),
+                  all.begin() + static_cast<std::ptrdiff_t>((k + 1) * blk), t.data());
+        Tn.push_back(std::move(t));
+    }
+    return Tn;
+}
+
 }  // namespace Chebyshev
 
 namespace kernels {

@@ -203,10 +203,32 @@ class Context:
         N.check_kpm(N.kpm().tbk_block_create(self._h, _dp(buf), rows, R, int(precision), C.byref(h)))
         return Block(self, h, rows, R, precision)
 
+    def zeros_block(self, rows: int, R: int, precision: int = FP64) -> Block:
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_block_create(self._h, None, int(rows), int(R), int(precision), C.byref(h)))
+        return Block(self, h, rows, R, precision)
+
     def random_block(self, rows: int, R: int, seed: int, precision: int = FP64) -> Block:
         h = C.c_void_p()
         N.check_kpm(N.kpm().tbk_block_random_phase(self._h, rows, R, int(seed), int(precision), C.byref(h)))
         return Block(self, h, rows, R, precision)
+
+    def cheb_on_bases(self, M: Matrix, v: Block, n: int, a: float = 1.0, b: float = 0.0,
+                      out: list | None = None):
+        """Stored blocks [T_0(H~) v, ..., T_{n-1}(H~) v], H~ = (M - b)/a, all
+        resident on the device (tbk_cheb_seq_on_bases; Cheb_first_kind_seq_on_bases,
+        numerical.h:334-344).  out: n blocks of v's shape to fill (made here
+        when None; out[0] may be v).  Returns (blocks, kernel ms)."""
+        n = int(n)
+        if out is None:
+            out = [self.zeros_block(v.rows, v.R, v.precision) for _ in range(n)]
+        if len(out) != n:
+            raise ValueError("out must hold n blocks")
+        arr = (C.c_void_p * max(n, 1))(*[o._h for o in out])
+        ms = C.c_float()
+        N.check_kpm(N.kpm().tbk_cheb_seq_on_bases(self._h, M._h, float(a), float(b), v._h, n,
+                                                  C.cast(arr, N.PP), C.byref(ms)))
+        return out, ms.value
 
     def moments(self, M: Matrix, kets: Block, n: int, bras: Block | None = None, a: float = 1.0,
                 b: float = 0.0, plain: bool = False, time_launches: bool = False, want_last: bool = False,
@@ -515,6 +537,19 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def cheb_first_kind_seq_on_bases(M: CSC, n: int, v: np.ndarray) -> np.ndarray:
+    """Drop-in of Chebyshev::Cheb_first_kind_seq_on_bases for a sparse M (already
+    rescaled, CSC): [T_0(M) v, ..., T_{n-1}(M) v] as an array (n, rows, R)
+    complex, formed in FP64 on the device."""
+    s, keep = tbk_sparse_of(M)
+    vb = _as_colmajor_c128(v)
+    rows, R = (v.shape[0], 1) if np.ndim(v) == 1 else v.shape
+    out = np.zeros(2 * max(int(n), 0) * rows * R, dtype=np.float64)
+    N.check_kpm(N.kpm().tbk_cheb_first_kind_seq_on_bases(C.byref(s), int(n), _dp(vb), rows, R, _dp(out)))
+    del keep
+    return out.view(np.complex128).reshape(int(n), R, rows).transpose(0, 2, 1)
 
 
 def cheb_first_kind_seq_braket(M: CSC, n: int, bras_T: np.ndarray, kets: np.ndarray) -> np.ndarray:
