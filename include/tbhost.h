@@ -68,6 +68,10 @@ int tbh_lattice_hoppings(const tbh_lattice* l, int32_t* from, int32_t* to, int32
    (re, im), exactly as the open and periodic builders apply it
    (lattice.h:282-285); needs a gauge field */
 int tbh_lattice_phase_table(const tbh_lattice* l, double* out);
+/* the cell's geometry, as op_position uses it (lattice.h:314-333):
+   bases[3 d + k] = component k of lattice vector d (d < dim; the rest 0),
+   atom_pos[3 a + k] = component k of atom a's position (n_atoms atoms) */
+int tbh_lattice_geometry(const tbh_lattice* l, double* bases, double* atom_pos);
 /* remove atom `atom` from the listed cells (n cells, dim ints each) */
 int tbh_lattice_remove_atoms(tbh_lattice* l, const char* atom, const int32_t* cells, int32_t n);
 /* builder: 0 get_Hamiltonian_sparse (open, lattice.h:262-305),

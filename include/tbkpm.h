@@ -181,6 +181,12 @@ typedef struct tbk_lattice_desc {
     int32_t has_disorder;     /* on-site W (u(seed, site) - 1/2), the site-keyed counter generator */
     double disorder_W;
     uint64_t disorder_seed;
+    /* geometry, for position and velocity operators (tbk_matrix_velocity;
+       tbh_lattice_geometry gives both): bases[3 d + k] = component k of
+       lattice vector d; atom_pos[3 a + k] = component k of atom a's
+       position, or NULL where no position operator is formed */
+    double bases[9];
+    const double* atom_pos;
 } tbk_lattice_desc;
 
 /*
@@ -189,6 +195,28 @@ typedef struct tbk_lattice_desc {
  * formed.  At most 32 hoppings may touch one atom (TBK_E_UNSUPPORTED).
  */
 int tbk_matrix_assemble(tbk_context* ctx, const tbk_lattice_desc* d, tbk_matrix** out, double* wall_ms);
+
+/*
+ * H~ = (M - b I)/a as a new resident matrix over the union pattern of M and
+ * I, every diagonal entry explicit: rescale / GreenFunction::scale
+ * (lattice.h:550-559).  The entries and bits of the host rescale of the same
+ * matrix: (m_ij - b)/a on the diagonal, (m_ij - 0)/a elsewhere, (0 - b)/a
+ * for a diagonal M lacks.
+ */
+int tbk_matrix_rescale(tbk_context* ctx, const tbk_matrix* M, double a, double b, tbk_matrix** out);
+
+/*
+ * The velocity operator v = X M - M X without the factor i
+ * (commutator_with_diagonal, lattice.h:338-345; snippets.h:316-317) for the
+ * position operator X along `axis` of the lattice's sites (op_position,
+ * lattice.h:314-333):
+ *   x_s = atom_pos[a][axis] + (sum_d bases[d][axis] i_d)  for site s of
+ *   cell i, atom a (the builders' bare index; no removed sites).
+ * M's pattern; values x_i m_ij - m_ij x_j, the host's bits.  M's rows must
+ * equal the lattice's sites and d->atom_pos must be set.
+ */
+int tbk_matrix_velocity(tbk_context* ctx, const tbk_matrix* M, const tbk_lattice_desc* d, int32_t axis,
+                        tbk_matrix** out);
 
 /* upload rows x R column-major complex128 (NULL: zeros) as a block */
 int tbk_block_create(tbk_context* ctx, const double* colmajor, int32_t rows, int32_t R,
@@ -444,8 +472,11 @@ typedef struct tbk_request {
     const tbk_sparse* vx; /* sigma_xy: velocity operators, used as given */
     const tbk_sparse* vy;
     const double* g;   /* sigma_xy: kernel weights g_n (N of them) or NULL for ones */
-    const tbk_lattice_desc* lattice; /* DoS with H = NULL: the Hamiltonian as a lattice description,
-                                        assembled on the device at admission (tbk_matrix_assemble) */
+    const tbk_lattice_desc* lattice; /* with H = NULL: the Hamiltonian as a lattice description,
+                                        assembled on the device at admission (tbk_matrix_assemble).
+                                        sigma_xy: vx = vy = NULL; the server forms
+                                        H~ = (H - b)/a (tbk_matrix_rescale) and v_x, v_y of H~
+                                        along axes 0 and 1 (tbk_matrix_velocity; atom_pos set) */
 } tbk_request;
 
 typedef struct tbk_request_info {

@@ -12,9 +12,8 @@ from tight_binding_amd import lattice as L
 GAUGES = {None: (0, None), "landau": (1, O.landau_gauge), "symmetric": (2, O.symmetric_gauge)}
 
 
-def build_pair(model, span, nn, builder="periodic", extend=None, is_complex=False, W=0.0, seed=1,
-               gauge=None, B=0.0, removed=None, explicit=None):
-    """Return (product CSC, oracle CSC) of the same Hamiltonian.
+def lattice_pair(model, span, nn, W=0.0, seed=1, gauge=None, B=0.0, removed=None, explicit=None):
+    """(product PartialLattice, oracle PartialLattice) of the same lattice.
     nn: [(atom, k, t)] shells; explicit: [(from, to, dR, t)] hoppings."""
     c = L.UnitCell.model(model)
     oc = O.model(model)
@@ -37,6 +36,14 @@ def build_pair(model, span, nn, builder="periodic", extend=None, is_complex=Fals
         atom, cells = removed
         lat.remove_atoms(atom, cells)
         ol.remove_atoms(atom, cells)
+    return lat, ol
+
+
+def build_pair(model, span, nn, builder="periodic", extend=None, is_complex=False, W=0.0, seed=1,
+               gauge=None, B=0.0, removed=None, explicit=None):
+    """Return (product CSC, oracle CSC) of the same Hamiltonian (lattice_pair's
+    arguments, then the builder)."""
+    lat, ol = lattice_pair(model, span, nn, W, seed, gauge, B, removed, explicit)
     if builder == "periodic":
         ext = extend or [1] * len(span)
         return (lat.get_Hamiltonian_periodic_extend_bare_sparse(ext, is_complex),

@@ -17,19 +17,20 @@ from oracle import spectral_oracle as SO
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def build(tmp_path):
-    exe = os.path.join(str(tmp_path), "dropin_dos")
+def build(tmp_path, name="dropin_dos"):
+    exe = os.path.join(str(tmp_path), name)
     subprocess.check_call([
         "g++", "-std=c++20", "-O2", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"),
         "-I" + os.path.join(ROOT, "tight_binding_amd", "host"), "-o", exe,
-        os.path.join(ROOT, "tests", "cpp", "dropin_dos.cpp"), "-L" + os.path.join(ROOT, "tight_binding_amd"),
+        os.path.join(ROOT, "tests", "cpp", name + ".cpp"), "-L" + os.path.join(ROOT, "tight_binding_amd"),
         "-ltbkpm", "-Wl,-rpath," + os.path.join(ROOT, "tight_binding_amd"),
     ])
     return exe
 
 
-def test_cpp_dropin_compiles_and_links(tmp_path):
-    assert os.path.exists(build(tmp_path))
+@pytest.mark.parametrize("name", ["dropin_dos", "kubo_xy_entry"])
+def test_cpp_dropin_compiles_and_links(tmp_path, name):
+    assert os.path.exists(build(tmp_path, name))
 
 
 def cpp_kets(n, m, seed):
@@ -97,3 +98,46 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     ev = np.linalg.eigvalsh(H.to_dense())
     assert abs(rng[0] - ev[0]) <= 1e-9 and abs(rng[1] - ev[-1]) <= 1e-9
     assert abs(rng[2] - (rng[1] - rng[0]) / 1.99) <= 1e-15 * rng[2] and abs(rng[3] - (rng[1] + rng[0]) / 2) <= 1e-15
+
+
+def kubo_xy_lattice(L):
+    """test_conductivity_Kubo_xy's lattice (snippets.h:745-766)."""
+    cell = O.model("honeycomb")
+    cell.add_hopping_nn("A", 1, 1.0)
+    t = 0.0
+    for atom, sgn in (("A", 1.0), ("B", -1.0)):
+        for dr in ((0, 1), (1, -1), (-1, 0)):
+            cell.add_hopping(atom, atom, dr, complex(-0.0 if sgn < 0 else 0.0, sgn) * t)
+    lat = O.PartialLattice(cell, (L, L))
+    lat.gauge_field = O.symmetric_gauge(300.0 / 205102.0 * 2 * math.pi)
+    return lat
+
+
+@pytest.mark.gpu
+def test_named_entry_kubo_xy_at_full_size(tmp_path):
+    """The reference's entry test_conductivity_Kubo_xy at its own sizes
+    (graphene 128 x 128, N = 1024, 2000 energies): the DoS printed by the C++
+    program against the oracle's moments and reconstruction on the same
+    lattice and kets, 1e-9 relative; not finite at e = -1, where
+    1/sqrt(1 - e^2) is infinite; the open Hamiltonian's size."""
+    L, N, nE = 128, 1024, 2000
+    out = subprocess.run([build(tmp_path, "kubo_xy_entry"), str(L), str(N), str(nE)], capture_output=True, text=True,
+                         check=True, timeout=300)
+    lines = out.stdout.split("\n")
+    dos = np.array([[float(v) for v in x.split()[1:]] for x in lines if x.startswith("dos ")])
+    assert dos.shape == (nE, 2)
+    E = -1.0 + np.arange(nE) * (2.0 / nE)
+    assert np.array_equal(dos[:, 0], E)
+    lat = kubo_xy_lattice(L)
+    H = lat.get_Hamiltonian_periodic_extend_bare_sparse((1, 1), True)
+    Emin, Emax = -6 - 1e-8, 6 + 1e-8
+    Hs = O.rescale(H, (Emax - Emin) / 2, (Emax + Emin) / 2)
+    k = cpp_kets(Hs.n, 4, 0x5EED)
+    mu = oracle.cheb_braket(Hs, N, k, k)
+    # the oracle's complex division raises at e = -1; the C++ result is not finite there
+    want = SO.density_of_states(mu, oracle.kernel_lorentz(N, 4.0), E[1:], 4)
+    assert not np.isfinite(dos[0, 1]) and np.isfinite(dos[1:, 1]).all()
+    assert np.max(np.abs(dos[1:, 1] - want)) <= 1e-9 * np.max(np.abs(want))
+    Ho = lat.get_Hamiltonian_sparse(True)
+    opn = [x.split()[1:] for x in lines if x.startswith("open ")][0]
+    assert (int(opn[0]), int(opn[1])) == (Ho.n, int(Ho.colptr[-1]))

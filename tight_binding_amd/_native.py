@@ -70,6 +70,8 @@ class tbk_lattice_desc(C.Structure):
         ("has_disorder", C.c_int32),
         ("disorder_W", C.c_double),
         ("disorder_seed", C.c_uint64),
+        ("bases", C.c_double * 9),
+        ("atom_pos", C.POINTER(C.c_double)),
     ]
 
 
@@ -293,6 +295,8 @@ KPM_SIGNATURES = {
     "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),
     "tbk_bandwidth_probe": ([P, I64, I32, C.POINTER(tbk_bandwidth)], I32),
     "tbk_matrix_assemble": ([P, C.POINTER(tbk_lattice_desc), PP, DP], I32),
+    "tbk_matrix_rescale": ([P, P, D, D, PP], I32),
+    "tbk_matrix_velocity": ([P, P, C.POINTER(tbk_lattice_desc), I32, PP], I32),
     "tbk_matrix_download": ([P, IP, IP, DP], I32),
     "tbk_cheb_reconstruct": ([P, DP, DP, I32, DP, I32, D, DP, DP], I32),
     "tbk_spectral_bounds": ([P, P, I32, D, U64, C.POINTER(tbk_bounds)], I32),
@@ -330,6 +334,7 @@ HOST_SIGNATURES = {
     "tbh_lattice_info_get": ([P, C.POINTER(tbh_lattice_info)], I32),
     "tbh_lattice_hoppings": ([P, IP, IP, IP, DP], I32),
     "tbh_lattice_phase_table": ([P, DP], I32),
+    "tbh_lattice_geometry": ([P, DP, DP], I32),
     "tbh_lattice_hamiltonian": ([P, I32, IP, I32, PP], I32),
     "tbh_lattice_op_position": ([P, I32, PP], I32),
     "tbh_lattice_op_velocity": ([P, I32, I32, PP], I32),

@@ -294,6 +294,37 @@ int64_t lattice_nnz_bound(const tbk_lattice_desc* d) {
     return (lattice_rows(d) / std::max(d->n_atoms, 1)) * 2 * static_cast<int64_t>(std::max(d->n_hops, 0));
 }
 
+// The exact 95th percentile of |row - col| over the entries of a resident
+// CSR pattern (the entry nth_element would pick at rank 0.95 nnz): a
+// three-pass radix select over 11-bit digits.
+int32_t distance_percentile95(const int32_t* rp, const int32_t* ci, int32_t n, int64_t nnz, hipStream_t st) {
+    if (nnz <= 0) return 0;
+    const unsigned grid = static_cast<unsigned>(
+        std::min<int64_t>((static_cast<int64_t>(n) + tbk::kBlock - 1) / tbk::kBlock, 8192));
+    uint64_t rank = std::min<uint64_t>(static_cast<uint64_t>(nnz) - 1,
+                                       static_cast<uint64_t>(0.95 * static_cast<double>(nnz)));
+    uint64_t prefix = 0;
+    DevBuf dhist;
+    dhist.reserve(tbk::kSelBins * 4);
+    std::vector<uint32_t> hh(tbk::kSelBins);
+    for (int shift = 2 * tbk::kSelBits; shift >= 0; shift -= tbk::kSelBits) {
+        HIPC(hipMemsetAsync(dhist.p, 0, tbk::kSelBins * 4, st));
+        hipLaunchKernelGGL(tbk::distance_digit_kernel, dim3(grid), dim3(tbk::kBlock), 0, st, rp, ci, n, shift, prefix,
+                           dhist.as<uint32_t>());
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(hh.data(), dhist.p, tbk::kSelBins * 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        int b = 0;
+        for (; b < tbk::kSelBins; ++b) {
+            if (rank < hh[b]) break;
+            rank -= hh[b];
+        }
+        if (b == tbk::kSelBins) raise(TBK_E_DEVICE, "distance select lost its rank");
+        prefix = (prefix << tbk::kSelBits) | static_cast<uint64_t>(b);
+    }
+    return static_cast<int32_t>(prefix);
+}
+
 void assemble_matrix(tbk_context* ctx, const tbk_lattice_desc* d, hipStream_t st, tbk_matrix** out, double* ms) {
     const auto t0 = std::chrono::steady_clock::now();
     if (d->dim < 1 || d->dim > 3) raise(TBK_E_INVALID, "dim must be 1, 2 or 3");
@@ -346,7 +377,7 @@ void assemble_matrix(tbk_context* ctx, const tbk_lattice_desc* d, hipStream_t st
     const size_t o_fo = put(fwd_off), o_f = put(fwd), o_bo = put(bwd_off), o_b = put(bwd), o_lv = put(lv3);
     const size_t o_from = put(std::vector<int32_t>(d->hop_from, d->hop_from + d->n_hops));
     const size_t o_to = put(std::vector<int32_t>(d->hop_to, d->hop_to + d->n_hops));
-    DevBuf dtab, dhop, dphase, dcnt, dmax, dtemp, dgersh, dhist;
+    DevBuf dtab, dhop, dphase, dcnt, dmax, dtemp, dgersh;
     dtab.reserve(itab.size() * 4);
     HIPC(hipMemcpyAsync(dtab.p, itab.data(), itab.size() * 4, hipMemcpyHostToDevice, st));
     dhop.reserve(static_cast<size_t>(std::max(d->n_hops, 1)) * 16);
@@ -417,34 +448,8 @@ void assemble_matrix(tbk_context* ctx, const tbk_lattice_desc* d, hipStream_t st
     HIPC(hipGetLastError());
     std::vector<double2> gp(grid);
     HIPC(hipMemcpyAsync(gp.data(), dgersh.p, static_cast<size_t>(grid) * 16, hipMemcpyDeviceToHost, st));
-    // exact 95th percentile of |row - col|, the entry nth_element would pick
-    int32_t bw = 0;
-    if (nnz > 0) {
-        uint64_t rank = std::min<uint64_t>(static_cast<uint64_t>(nnz) - 1,
-                                           static_cast<uint64_t>(0.95 * static_cast<double>(nnz)));
-        uint64_t prefix = 0;
-        dhist.reserve(tbk::kSelBins * 4);
-        std::vector<uint32_t> hh(tbk::kSelBins);
-        for (int shift = 2 * tbk::kSelBits; shift >= 0; shift -= tbk::kSelBits) {
-            HIPC(hipMemsetAsync(dhist.p, 0, tbk::kSelBins * 4, st));
-            hipLaunchKernelGGL(tbk::distance_digit_kernel, dim3(grid), dim3(tbk::kBlock), 0, st,
-                               m->rp.as<const int32_t>(), m->ci.as<const int32_t>(), n, shift, prefix,
-                               dhist.as<uint32_t>());
-            HIPC(hipGetLastError());
-            HIPC(hipMemcpyAsync(hh.data(), dhist.p, tbk::kSelBins * 4, hipMemcpyDeviceToHost, st));
-            HIPC(hipStreamSynchronize(st));
-            int b = 0;
-            for (; b < tbk::kSelBins; ++b) {
-                if (rank < hh[b]) break;
-                rank -= hh[b];
-            }
-            if (b == tbk::kSelBins) raise(TBK_E_DEVICE, "distance select lost its rank");
-            prefix = (prefix << tbk::kSelBits) | static_cast<uint64_t>(b);
-        }
-        bw = static_cast<int32_t>(prefix);
-    }
     HIPC(hipStreamSynchronize(st));
-    m->bandwidth95 = bw;
+    m->bandwidth95 = distance_percentile95(m->rp.as<const int32_t>(), m->ci.as<const int32_t>(), n, nnz, st);
     double lo = INFINITY, hi = -INFINITY;
     for (const double2& g : gp) {
         lo = std::min(lo, g.x);

@@ -173,6 +173,11 @@ tbk_context* default_context();
 int64_t lattice_rows(const tbk_lattice_desc* d);
 int64_t lattice_nnz_bound(const tbk_lattice_desc* d);
 void assemble_matrix(tbk_context* ctx, const tbk_lattice_desc* d, hipStream_t st, tbk_matrix** out, double* ms);
+int32_t distance_percentile95(const int32_t* rp, const int32_t* ci, int32_t n, int64_t nnz, hipStream_t st);
+// Operators formed from resident matrices (operators.hip)
+void rescale_matrix(tbk_context* ctx, const tbk_matrix* M, double a, double b, hipStream_t st, tbk_matrix** out);
+void velocity_matrix(tbk_context* ctx, const tbk_matrix* M, const tbk_lattice_desc* d, int axis, hipStream_t st,
+                     tbk_matrix** out);
 // FP64 Lanczos estimate of the spectral range of a Hermitian matrix (bounds.hip)
 void spectral_bounds(tbk_context* ctx, const tbk_matrix* M, int max_iters, double tol, uint64_t seed,
                      hipStream_t st, tbk_bounds* out);

@@ -89,7 +89,7 @@ struct HostSparse {
 struct HostLattice {
     tbk_lattice_desc desc{};
     std::vector<int32_t> from, to, lv;
-    std::vector<double> t, phase;
+    std::vector<double> t, phase, apos;
     bool present = false;
     void copy_from(const tbk_lattice_desc* d) {
         if (d->dim < 1 || d->dim > 3 || d->n_atoms < 1 || d->n_hops < 0 || d->builder < 0 || d->builder > 2)
@@ -113,6 +113,8 @@ struct HostLattice {
         desc.hop_lv = lv.data();
         desc.hop_t = t.data();
         desc.phase = phase.empty() ? nullptr : phase.data();
+        if (d->atom_pos) apos.assign(d->atom_pos, d->atom_pos + 3 * static_cast<size_t>(d->n_atoms));
+        desc.atom_pos = apos.empty() ? nullptr : apos.data();
         present = true;
     }
     void clear() {
@@ -121,6 +123,7 @@ struct HostLattice {
         std::vector<int32_t>().swap(lv);
         std::vector<double>().swap(t);
         std::vector<double>().swap(phase);
+        std::vector<double>().swap(apos);
         present = false;
     }
 };
@@ -281,7 +284,11 @@ int64_t tbk_server::kubo_footprint(const Request& r) const {
     // pre-split stacks: half-size entries and three more rolling blocks
     const bool ps = kubo_presplit(r.precision, r.n, r.H.hdr.is_complex != 0, r.vx.hdr.is_complex != 0);
     const int64_t stacks = ps ? N * blk : 2 * N * blk, rolling = ps ? 5 * blk : 2 * blk;
-    return mat_bytes(r.H, false) + mat_bytes(r.vx, false) + mat_bytes(r.vy, false) + blk /* kets */ +
+    // a lattice request also holds the assembled H while H~ is formed
+    // (H~ has the size of a velocity operator: one more entry per row)
+    const int64_t mats = r.lat.present ? mat_bytes(r.H, false) + 3 * mat_bytes(r.vx, false)
+                                       : mat_bytes(r.H, false) + mat_bytes(r.vx, false) + mat_bytes(r.vy, false);
+    return mats + blk /* kets */ +
            stacks + rolling + static_cast<int64_t>(ctx->num_cu) * 32 * 16 /* step partials */ +
            static_cast<int64_t>(cs.nchunks) * N * N * 16 + N * N * 16 + N * 8 * 2 + N * 16;
 }
@@ -432,14 +439,28 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     auto j = std::make_unique<KuboJob>();
     j->req = r;
     j->pin = pin;
-    int rc = tbk_matrix_create(ctx, &r->H.hdr, &j->H);
-    if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vx.hdr, &j->vx);
-    if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vy.hdr, &j->vy);
-    if (rc != TBK_OK) raise(rc, g_last_error);
+    double a = r->a, b = r->b;
+    if (r->lat.present) {
+        // assemble H, form H~ (the steps then run it as given) and the
+        // velocity operators of H~ along x and y
+        tbk_matrix* Hraw = nullptr;
+        assemble_matrix(ctx, &r->lat.desc, bstream, &Hraw, nullptr);
+        std::unique_ptr<tbk_matrix, int (*)(tbk_matrix*)> hold(Hraw, tbk_matrix_destroy);
+        rescale_matrix(ctx, Hraw, a, b, bstream, &j->H);
+        velocity_matrix(ctx, j->H, &r->lat.desc, 0, bstream, &j->vx);
+        velocity_matrix(ctx, j->H, &r->lat.desc, 1, bstream, &j->vy);
+        a = 1.0;
+        b = 0.0;
+    } else {
+        int rc = tbk_matrix_create(ctx, &r->H.hdr, &j->H);
+        if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vx.hdr, &j->vx);
+        if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vy.hdr, &j->vy);
+        if (rc != TBK_OK) raise(rc, g_last_error);
+    }
     const int rows = r->H.hdr.rows, Rp = r->Rp, N = r->n;
     const int64_t K = static_cast<int64_t>(rows) * Rp;
     const size_t blk = static_cast<size_t>(K) * sizeof(double2);
-    const StepPlan pH = make_plan(ctx, j->H, TBK_FP64, Rp, rows, r->b, false);
+    const StepPlan pH = make_plan(ctx, j->H, TBK_FP64, Rp, rows, b, false);
     const StepPlan pX = make_plan(ctx, j->vx, TBK_FP64, Rp, rows, 0.0, false);
     const StepPlan pY = make_plan(ctx, j->vy, TBK_FP64, Rp, rows, 0.0, false);
     // the footprint counted the pre-split form whenever kubo_presplit holds;
@@ -482,12 +503,12 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     HIPC(hipGetLastError());
     const void* vxv = fused_velocity(j->H, j->vx, pH, bstream) ? j->vx->val64.p : nullptr;
     if (presplit) {
-        enqueue_kubo_stacks_presplit(pH, pX, pY, r->a, r->b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->B0.p,
+        enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->B0.p,
                                      j->B1.p, j->S.p, j->W0.p, j->W1.p, K, N, j->part.as<double2>(), st_, vxv);
         enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, j->P.as<double2>(),
                                      j->G.as<double2>(), st_);
     } else {
-        enqueue_kubo_stacks(pH, pX, pY, r->a, r->b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, N,
+        enqueue_kubo_stacks(pH, pX, pY, a, b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, N,
                             j->part.as<double2>(), st_, vxv);
         enqueue_contraction(cs, j->L.as<const double2>(), j->Rs.as<const double2>(), N, K, r->precision,
                             j->P.as<double2>(), j->G.as<double2>(), st_);
@@ -830,7 +851,7 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
     return guarded([&] {
         if (!s || !q || !ticket) raise(TBK_E_INVALID, "null argument");
         if (q->kind != TBK_REQ_DOS && q->kind != TBK_REQ_KUBO) raise(TBK_E_INVALID, "unknown request kind");
-        if (!q->H && !(q->kind == TBK_REQ_DOS && q->lattice)) raise(TBK_E_INVALID, "request without a Hamiltonian");
+        if (!q->H && !q->lattice) raise(TBK_E_INVALID, "request without a Hamiltonian");
         if (q->R < 1) raise(TBK_E_INVALID, "R must be positive");
         if (!std::isfinite(q->a) || !std::isfinite(q->b)) raise(TBK_E_INVALID, "invalid rescaling a, b");
         if (q->a == 0.0 && q->kind != TBK_REQ_DOS) raise(TBK_E_INVALID, "sigma_xy requests need a, b");
@@ -852,7 +873,11 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
         } else {
             if (q->precision != TBK_GAMMA_FP64 && q->precision != TBK_GAMMA_BF16X3)
                 raise(TBK_E_INVALID, "unknown contraction precision");
-            if (!q->vx || !q->vy) raise(TBK_E_INVALID, "sigma_xy request without velocity operators");
+            if (q->H && (!q->vx || !q->vy)) raise(TBK_E_INVALID, "sigma_xy request without velocity operators");
+            if (!q->H && (q->vx || q->vy))
+                raise(TBK_E_INVALID, "a lattice sigma_xy request forms its velocity operators itself");
+            if (!q->H && (!q->lattice->atom_pos || q->lattice->dim < 2))
+                raise(TBK_E_INVALID, "a lattice sigma_xy request needs atom positions and two axes");
             if (q->n_moments < 1) raise(TBK_E_INVALID, "n_moments must be at least 1");
             if (q->n_moments > s->cfg.max_kubo_moments) raise(TBK_E_UNSUPPORTED, "more moments than the server allows");
             if (tbk::kubo_zeta_energies_per_group(q->n_moments) == 0) raise(TBK_E_UNSUPPORTED, "N too large for zeta");
@@ -869,7 +894,14 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
             r->H.hdr.nnz = lattice_nnz_bound(&r->lat.desc);
             r->H.hdr.is_complex = q->lattice->is_complex;
         }
-        if (q->kind == TBK_REQ_KUBO) {
+        if (q->kind == TBK_REQ_KUBO && !q->H) {
+            // H~ = (H - b)/a over the union pattern and v_x, v_y on its
+            // pattern, all formed on the device at admission
+            r->vx.hdr = r->vy.hdr = r->H.hdr;
+            r->vx.hdr.nnz = r->vy.hdr.nnz = r->H.hdr.nnz + r->H.hdr.rows;
+            if (q->g) r->g.assign(q->g, q->g + q->n_moments);
+            r->footprint = s->kubo_footprint(*r);
+        } else if (q->kind == TBK_REQ_KUBO) {
             r->vx.copy_from(q->vx);
             r->vy.copy_from(q->vy);
             if (r->vx.hdr.rows != r->H.hdr.rows || r->vy.hdr.rows != r->H.hdr.rows)

@@ -276,6 +276,23 @@ int tbh_lattice_hoppings(const tbh_lattice* l, int32_t* from, int32_t* to, int32
     });
 }
 
+int tbh_lattice_geometry(const tbh_lattice* l, double* bases, double* atom_pos) {
+    return guarded([&] {
+        if (!l || !bases || !atom_pos) bad("invalid argument");
+        std::visit(
+            [&](const auto& lat) {
+                constexpr int D = std::remove_reference_t<decltype(lat)>::dim;
+                const auto& c = lat.cell();
+                for (int k = 0; k < 9; ++k) bases[k] = 0.0;
+                for (int d = 0; d < D; ++d)
+                    for (int k = 0; k < 3; ++k) bases[3 * d + k] = c.basis(d)(k);
+                for (int a = 0; a < c.get_atom_number(); ++a)
+                    for (int k = 0; k < 3; ++k) atom_pos[3 * a + k] = c.get_atom_pos(a)(k);
+            },
+            l->v);
+    });
+}
+
 int tbh_lattice_phase_table(const tbh_lattice* l, double* out) {
     return guarded([&] {
         if (!l || !out) bad("invalid argument");

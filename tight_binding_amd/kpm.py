@@ -178,6 +178,27 @@ class Context:
         del keep
         return Matrix(self, handle=h, is_complex=is_complex), ms.value
 
+    def rescale(self, M: Matrix, a: float, b: float) -> Matrix:
+        """H~ = (M - b)/a over the union pattern of M and I as a new resident
+        matrix (tbk_matrix_rescale): the host rescale's entries and bits."""
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_matrix_rescale(self._h, M._h, float(a), float(b), C.byref(h)))
+        return Matrix(self, handle=h, is_complex=M.is_complex)
+
+    def velocity(self, M: Matrix, lattice, axis: int) -> Matrix:
+        """v = X M - M X (no factor i) for the lattice's position operator
+        along axis, on M's pattern (tbk_matrix_velocity): the bits of
+        lattice.commutator_diag(lattice.op_position(axis), M)."""
+        info = N.tbh_lattice_info()
+        N.check_host(N.host().tbh_lattice_info_get(lattice._h, C.byref(info)))
+        if info.removed_sites:
+            raise N.NativeError(N.TBK_E_UNSUPPORTED, "removed sites: positions use compressed indices")
+        d, keep = lattice_desc_of(lattice, 2, None, M.is_complex)
+        h = C.c_void_p()
+        N.check_kpm(N.kpm().tbk_matrix_velocity(self._h, M._h, C.byref(d), int(axis), C.byref(h)))
+        del keep
+        return Matrix(self, handle=h, is_complex=M.is_complex)
+
     def bandwidth_probe(self, nbytes: int, repeats: int = 3) -> dict:
         """Attainable HBM references on buffers of nbytes (tbk_bandwidth_probe):
         non-temporal copy, two-reads-one-write stream, read-only, in GB/s."""
@@ -311,7 +332,13 @@ def lattice_desc_of(lattice, builder: int = 1, extend=None, is_complex: bool = F
     d.hop_from, d.hop_to, d.hop_lv, d.hop_t = _ip(fr), _ip(to), _ip(lv), _dp(t)
     d.phase = _dp(phase) if phase is not None else None
     d.has_disorder, d.disorder_W, d.disorder_seed = info.has_disorder, info.disorder_W, info.disorder_seed
-    return d, (fr, to, lv, t, phase)
+    bases = np.zeros(9, dtype=np.float64)
+    apos = np.zeros(3 * max(info.n_atoms, 1), dtype=np.float64)
+    N.check_host(N.host().tbh_lattice_geometry(lattice._h, _dp(bases), _dp(apos)))
+    for k in range(9):
+        d.bases[k] = bases[k]
+    d.atom_pos = _dp(apos)
+    return d, (fr, to, lv, t, phase, apos)
 
 
 def reconstruct(ctx, mu, g, e, scale: float = 1.0):
@@ -490,6 +517,21 @@ class Server:
         req = N.tbk_request(N.TBK_REQ_KUBO, int(precision), int(N_), int(R), int(seed), float(a), float(b),
                             C.pointer(sH), C.pointer(sx), C.pointer(sy), _dp(gg) if gg is not None else None)
         t = self._submit(req, (sH, kH, sx, kx, sy, ky, gg))
+        self._kind[t] = ("kubo", int(N_))
+        return t
+
+    def submit_kubo_lattice(self, lattice, N_: int, a: float, b: float = 0.0, R: int = 8, seed: int = 0,
+                            precision: int = N.TBK_GAMMA_FP64, g=None, builder: int = 0, extend=None,
+                            is_complex: bool = True) -> int:
+        """sigma_xy of a lattice.PartialLattice formed entirely on the device at
+        admission: H assembled (builder, extend, is_complex as for
+        Context.assemble), H~ = (H - b)/a, v_x, v_y = [X, H~], [Y, H~]; the
+        result as submit_kubo's for those three matrices with a = 1, b = 0."""
+        d, keep = lattice_desc_of(lattice, builder, extend, is_complex)
+        gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        req = N.tbk_request(N.TBK_REQ_KUBO, int(precision), int(N_), int(R), int(seed), float(a), float(b),
+                            None, None, None, _dp(gg) if gg is not None else None, C.pointer(d))
+        t = self._submit(req, (d, keep, gg))
         self._kind[t] = ("kubo", int(N_))
         return t
 
