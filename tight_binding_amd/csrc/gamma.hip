@@ -338,20 +338,20 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_kernel(const double2* __rest
 // ---------------------------------------------------------------------------
 // Large-N split-bf16 x3 contraction, same 128 x 128 tiling and XCD placement
 // as gamma_f64_big_kernel.  Each double is split once, when its tile is
-// staged: hi = bf16(x), lo = bf16(x - hi), a complex value becoming one
-// 32-bit word of (re, im) bf16 pairs per part.  An LDS row holds the 16
-// complex columns of one stage as 16 hi words, 16 lo words and 4 words of
-// padding (144 bytes: the 16 rows of a fragment read fall on distinct bank
-// groups).  Lane l of a fragment reads 4 consecutive words = 8 bf16 = the k
-// range 8*(l/16) .. +7 of its row.  The imaginary part uses B words turned
-// into (im, -re) in registers.  Chunks are short (about 32K complex columns)
-// so fp32 accumulation never runs long; chunk partials are summed in fp64.
+// staged (hi = bf16(x), lo = bf16(x - hi)); pre-split stacks arrive as those
+// words already.  In LDS the real and imaginary parts are kept apart: an LDS
+// word holds one part of two consecutive complex columns, and a row of a
+// 16-column stage is, per hi / lo part, [re 8 words | im 8 words] (virtual
+// k: 16 real parts, then 16 imaginary parts), plus 4 words of padding (rows
+// of 36 words: the 16 rows of a fragment read fall on distinct banks).  Lane
+// group g of a fragment reads words 4g .. 4g + 3.  B read that way gives
+// (re, im): A.B = sum ar br + ai bi = Re conj(a) b.  Groups 0, 1 reading the
+// im words and groups 2, 3 the re words with the sign bits flipped give
+// (im, -re): A.B' = sum ar bi - ai br = Im conj(a) b.  Chunks are short
+// (about 32K complex columns) so fp32 accumulation never runs long; chunk
+// partials are summed in fp64.
 constexpr int kSplitKt = 16;        // complex k per stage = one bf16 MFMA k-step
-constexpr int kSplitRowWords = 36;  // 16 hi + 16 lo + 4 pad
-
-
-// (re, im) -> (im, -re) on a packed bf16 pair
-__device__ __forceinline__ uint32_t swap_neg(uint32_t w) { return ((w >> 16) | (w << 16)) ^ 0x80000000u; }
+constexpr int kSplitRowWords = 36;  // hi re, hi im, lo re, lo im (8 words each), 4 pad
 
 __device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
 
@@ -360,6 +360,27 @@ __device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8b
 __device__ __forceinline__ uint2 operand_words(double2 v) { return split_word(v); }
 __device__ __forceinline__ uint2 operand_words(uint2 w) { return w; }
 __device__ __forceinline__ uint2 ld_or_zero(const uint2* p, bool ok) { return ok ? *p : make_uint2(0u, 0u); }
+
+// two consecutive operand elements e, e + 1 of a row.  rows16: K even, so
+// every pair (2m, 2m + 1) is one 16-byte aligned load of pre-split words.
+__device__ __forceinline__ void ld_pair(const uint2* p, bool rows16, uint2& x0, uint2& x1) {
+    if (rows16) {
+        const uint4 u = *reinterpret_cast<const uint4*>(p);
+        x0 = make_uint2(u.x, u.y);
+        x1 = make_uint2(u.z, u.w);
+    } else {
+        x0 = p[0];
+        x1 = p[1];
+    }
+}
+__device__ __forceinline__ void ld_pair(const double2* p, bool, double2& x0, double2& x1) {
+    x0 = p[0];
+    x1 = p[1];
+}
+
+// real parts of two (re, im) bf16 words, and their imaginary parts
+__device__ __forceinline__ uint32_t re_pair(uint32_t w0, uint32_t w1) { return (w0 & 0xffffu) | (w1 << 16); }
+__device__ __forceinline__ uint32_t im_pair(uint32_t w0, uint32_t w1) { return (w0 >> 16) | (w1 & 0xffff0000u); }
 
 // T = double2: fp64 stacks, split at staging; T = uint2: stacks written
 // pre-split by the recursion (the same words), half the bytes to stage
@@ -374,7 +395,7 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
     if (!xcd_work_item(map, tn, tm, c)) return;  // uniform over the workgroup
     // 8 waves: wave w owns the 64 x 32 block (rows (w >> 2) * 64, columns
     // (w & 3) * 32) of the 128 x 128 tile; two waves per SIMD, so one wave's
-    // staging and splitting overlaps another's matrix-core work
+    // staging overlaps another's matrix-core work
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int wn = wave >> 2, wm = wave & 3;
@@ -382,38 +403,59 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
     const int64_t kb = c * kc;
     const int64_t ke = (kb + kc < K) ? kb + kc : K;
 
-    // staging: thread t takes complex column t % 16 of rows t / 16 + 32 i, i < 4
-    const int col = tid & 15;
-    const int r0 = tid >> 4;
-    const T* ga[4];
-    const T* gb[4];
+    // staging: thread t takes complex columns 2p, 2p + 1 (p = t % 8) of rows
+    // q and q + 64, q = 8 (t / 64) + 2 ((t / 8) % 4) + (t / 32) % 2: the 32
+    // lanes of one LDS write pass cover rows r, r + 2, r + 4, r + 6, whose
+    // 8-word groups fall on distinct banks
+    const int p = tid & 7;
+    const int q8 = (tid >> 3) & 7;
+    const int r0 = 8 * wave + 2 * (q8 & 3) + (q8 >> 2);
+    const T* ga[2];
+    const T* gb[2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int r = r0 + 32 * i;
+    for (int i = 0; i < 2; ++i) {
+        const int r = r0 + 64 * i;
         const int rn = (n0 + r < N) ? n0 + r : N - 1;
         const int rm = (m0 + r < N) ? m0 + r : N - 1;
-        ga[i] = A + static_cast<int64_t>(rn) * K + col;
-        gb[i] = B + static_cast<int64_t>(rm) * K + col;
+        ga[i] = A + static_cast<int64_t>(rn) * K;
+        gb[i] = B + static_cast<int64_t>(rm) * K;
     }
-    T ra[4], rb[4];
+    T ra[2][2], rb[2][2];
+    const bool rows16 = (K & 1) == 0;  // row starts, and so column pairs, 16-byte aligned
     auto gload = [&](int64_t k0) {
-        const bool ok = k0 + col < ke;
+        const int64_t e = k0 + 2 * p;
+        if (k0 + kSplitKt <= ke) {  // a full stage (all but a row's last, short one): plain loads
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            ra[i] = ld_or_zero(ga[i] + k0, ok);
-            rb[i] = ld_or_zero(gb[i] + k0, ok);
+            for (int i = 0; i < 2; ++i) {
+                ld_pair(ga[i] + e, rows16, ra[i][0], ra[i][1]);
+                ld_pair(gb[i] + e, rows16, rb[i][0], rb[i][1]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ra[i][0] = ld_or_zero(ga[i] + e, e < ke);
+                ra[i][1] = ld_or_zero(ga[i] + e + 1, e + 1 < ke);
+                rb[i][0] = ld_or_zero(gb[i] + e, e < ke);
+                rb[i][1] = ld_or_zero(gb[i] + e + 1, e + 1 < ke);
+            }
         }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int o = (r0 + 32 * i) * kSplitRowWords + col;
-            const uint2 wa = operand_words(ra[i]);
-            const uint2 wb = operand_words(rb[i]);
-            As[buf][o] = wa.x;
-            As[buf][o + 16] = wa.y;
-            Bs[buf][o] = wb.x;
-            Bs[buf][o + 16] = wb.y;
+        for (int i = 0; i < 2; ++i) {
+            const int r = r0 + 64 * i;
+            const uint2 a0 = operand_words(ra[i][0]), a1 = operand_words(ra[i][1]);
+            const uint2 b0 = operand_words(rb[i][0]), b1 = operand_words(rb[i][1]);
+            uint32_t* ar = &As[buf][r * kSplitRowWords + p];
+            ar[0] = re_pair(a0.x, a1.x);
+            ar[8] = im_pair(a0.x, a1.x);
+            ar[16] = re_pair(a0.y, a1.y);
+            ar[24] = im_pair(a0.y, a1.y);
+            uint32_t* br = &Bs[buf][r * kSplitRowWords + p];
+            br[0] = re_pair(b0.x, b1.x);
+            br[8] = im_pair(b0.x, b1.x);
+            br[16] = re_pair(b0.y, b1.y);
+            br[24] = im_pair(b0.y, b1.y);
         }
     };
 
@@ -426,44 +468,77 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
             im[i][j] = v4f{0, 0, 0, 0};
         }
 
-    gload(kb);
-    sstore(0);
-    __syncthreads();
-    int buf = 0;
-    for (int64_t k0 = kb; k0 < ke; k0 += kSplitKt) {
-        const bool more = k0 + kSplitKt < ke;
-        if (more) gload(k0 + kSplitKt);
-        v8bf ah[4], al[4], bh[2], bl[2], bih[2], bil[2];
+    // Fragments of one part (hi or lo) of a stage: A of the wave's 4 row
+    // blocks; B and B' = (im, -re) of its 2 column blocks.  B' lanes: groups
+    // 0, 1 read the im words, 2, 3 the re words with the sign bits flipped.
+    struct FragB {
+        v8bf b[2], bp[2];
+    };
+    const int prime_off = lk < 2 ? 8 + 4 * lk : 4 * lk - 8;
+    const uint32_t prime_neg = lk < 2 ? 0u : 0x80008000u;
+    auto read_a = [&](v8bf (&a)[4], int buf, int part) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t* row = &As[buf][(wn * 64 + 16 * i + li) * kSplitRowWords + 4 * lk];
-            ah[i] = as_v8bf(*reinterpret_cast<const uint4*>(row));
-            al[i] = as_v8bf(*reinterpret_cast<const uint4*>(row + 16));
-        }
+        for (int i = 0; i < 4; ++i)
+            a[i] = as_v8bf(*reinterpret_cast<const uint4*>(
+                &As[buf][(wn * 64 + 16 * i + li) * kSplitRowWords + 16 * part + 4 * lk]));
+    };
+    auto read_b = [&](FragB& f, int buf, int part) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const uint32_t* row = &Bs[buf][(wm * 32 + 16 * j + li) * kSplitRowWords + 4 * lk];
-            const uint4 h = *reinterpret_cast<const uint4*>(row);
-            const uint4 l = *reinterpret_cast<const uint4*>(row + 16);
-            bh[j] = as_v8bf(h);
-            bl[j] = as_v8bf(l);
-            bih[j] = as_v8bf(make_uint4(swap_neg(h.x), swap_neg(h.y), swap_neg(h.z), swap_neg(h.w)));
-            bil[j] = as_v8bf(make_uint4(swap_neg(l.x), swap_neg(l.y), swap_neg(l.z), swap_neg(l.w)));
+            const uint32_t* row = &Bs[buf][(wm * 32 + 16 * j + li) * kSplitRowWords + 16 * part];
+            f.b[j] = as_v8bf(*reinterpret_cast<const uint4*>(row + 4 * lk));
+            const uint4 u = *reinterpret_cast<const uint4*>(row + prime_off);
+            f.bp[j] = as_v8bf(make_uint4(u.x ^ prime_neg, u.y ^ prime_neg, u.z ^ prime_neg, u.w ^ prime_neg));
         }
+    };
+    // 16 products with independent accumulators
+    auto phase = [&](const v8bf (&a)[4], const FragB& f) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], re[i][j], 0, 0, 0);
-                im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bih[j], im[i][j], 0, 0, 0);
-                re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], re[i][j], 0, 0, 0);
-                im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bil[j], im[i][j], 0, 0, 0);
-                re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], re[i][j], 0, 0, 0);
-                im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bih[j], im[i][j], 0, 0, 0);
+                re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], f.b[j], re[i][j], 0, 0, 0);
+                im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], f.bp[j], im[i][j], 0, 0, 0);
             }
-        if (more) sstore(buf ^ 1);
+    };
+
+    // Stage s sits in LDS buffer s % 2 and is multiplied in three phases,
+    // hi.hi, hi.lo, lo.hi, each reading the next phase's fragments while its
+    // products run; phase 3 reads stage s + 1's hi fragments.  The one
+    // barrier per stage comes after the last read of stage s's buffer, which
+    // then takes stage s + 2 (its global loads were issued at the start of
+    // the stage).
+    const int64_t stages = (ke - kb + kSplitKt - 1) / kSplitKt;
+    v8bf ah[4], al[4];
+    FragB bhi, blo;
+    gload(kb);
+    sstore(0);
+    __syncthreads();
+    read_a(ah, 0, 0);
+    read_b(bhi, 0, 0);
+    if (stages > 1) {
+        gload(kb + kSplitKt);
+        sstore(1);
+    }
+    for (int64_t st = 0; st < stages; ++st) {
+        const int a = static_cast<int>(st & 1), b = a ^ 1;
+        if (st + 2 < stages) gload(kb + (st + 2) * kSplitKt);
+        read_b(blo, a, 1);
+        phase(ah, bhi);
+        read_a(al, a, 1);
+        phase(ah, blo);
         __syncthreads();
-        buf ^= 1;
+        if (st + 2 < stages) sstore(a);
+        v8bf ah_n[4];
+        FragB bhi_n;
+        if (st + 1 < stages) {
+            read_a(ah_n, b, 0);
+            read_b(bhi_n, b, 0);
+        }
+        phase(al, bhi);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ah[i] = ah_n[i];
+        bhi = bhi_n;
     }
 
     double2* Pc = P + c * static_cast<int64_t>(N) * N;
