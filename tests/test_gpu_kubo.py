@@ -226,12 +226,15 @@ def test_fused_velocity_stacks_are_the_same_bits(gpu_ctx, N):
     K.close()
 
 
-def test_presplit_stacks_give_the_bf16x3_gamma_of_fp64_stacks(gpu_ctx):
+@pytest.mark.parametrize("R,lane_bytes", [(2, 0), (4, 0), (8, 32), (32, 32)])
+def test_presplit_stacks_give_the_bf16x3_gamma_of_fp64_stacks(gpu_ctx, R, lane_bytes):
     """Split-BF16x3 solves with N >= 256 on complex operators write their
     stacks pre-split (the recursion stores the hi/lo bf16 words the
     contraction would form).  Gamma must be the same bits as the BF16x3
     contraction of the fp64 stacks -- here the oracle's, which the device's
-    equal bit for bit at a = 1, b = 0 -- with v_x fused or not."""
+    equal bit for bit at a = 1, b = 0 -- with v_x fused (both chains then run
+    as one block of 2 Rp columns) or not, at one or two complex values per
+    lane."""
     import scipy.sparse as sp
 
     from tight_binding_amd import sparse as SP
@@ -244,12 +247,15 @@ def test_presplit_stacks_give_the_bf16x3_gamma_of_fp64_stacks(gpu_ctx):
     Hs = L.rescale(lat.get_Hamiltonian_sparse(True), 5.0, 0.0)
     vx = L.commutator_diag(lat.op_position(0), Hs)
     vy = L.commutator_diag(lat.op_position(1), Hs)
-    N, R = 256, 2
+    N = 256
+    saved = gpu_ctx.get_tuning()
+    gpu_ctx.set_tuning(row_lane_bytes=lane_bytes)
     K = gpu_ctx.random_block(Hs.n, R, seed=17)
     kets = K.download()
     g_ps, st = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy), K, N,
                               precision=kpm.GAMMA_BF16X3)
-    assert st["stack_bytes"] == 2 * N * Hs.n * R * 8  # half-size entries
+    Rp = 1 << (R - 1).bit_length()
+    assert st["stack_bytes"] == 2 * N * Hs.n * Rp * 8  # half-size entries
     TL = oracle.cheb_on_bases(Hs, N, kets)
     w = oracle.cheb_on_bases(vy, 2, kets)[1]
     TR = oracle.cheb_on_bases(Hs, N, w)
@@ -265,4 +271,5 @@ def test_presplit_stacks_give_the_bf16x3_gamma_of_fp64_stacks(gpu_ctx):
     g_ps2, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx2), gpu_ctx.matrix(vy), K, N,
                               precision=kpm.GAMMA_BF16X3)
     assert np.array_equal(g_ps2, ref)
+    gpu_ctx.set_tuning(**saved)
     K.close()

@@ -40,6 +40,12 @@ enum StepMode : int {
     kFirstSplit = 4,   // kFirst, and y's split words at y2
     kRecDblSplit = 5,  // kRecDbl, and y's split words at y2
     kDualSplit = 6,    // kDual with H2 T_k stored only as split words at y2
+    // paired chains (fp64 complex only): columns [0, Rp/2) and [Rp/2, Rp) of
+    // the block are two recursions on the same matrix; the left half's y is
+    // stored as split words at y2, the right half's H2 x as split words at
+    // y3, both in blocks of Rp/2 columns
+    kPairFirst = 7,  // kFirst form
+    kPairRec = 8,    // kRecDbl form
 };
 
 template <typename VT>
@@ -205,6 +211,29 @@ __device__ __forceinline__ void st_split2(uint2* p, double2 v0, double2 v1) {
     u.w = w1.y;
     __builtin_nontemporal_store(u, reinterpret_cast<u4v*>(p));
 }
+// one value per lane: the words of this lane's value and of its row
+// neighbour's (lane ^ 1, by a quad permutation) as one 16-byte store from
+// the even lane (p 16-byte aligned there).  Both lanes of a pair must be
+// active: they hold one row.
+__device__ __forceinline__ void st_split_lanes(uint2* p, double2 v, bool even) {
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    const uint2 w = split_word(v);
+    constexpr int kSwapNeighbour = 0xB1;  // quad_perm(1, 0, 3, 2)
+    const uint32_t nx = static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(w.x), kSwapNeighbour, 0xF, 0xF, false));
+    const uint32_t ny = static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(w.y), kSwapNeighbour, 0xF, 0xF, false));
+    if (even) {
+        u4v u;
+        u.x = w.x;
+        u.y = w.y;
+        u.z = nx;
+        u.w = ny;
+        __builtin_nontemporal_store(u, reinterpret_cast<u4v*>(p));
+    }
+}
+template <typename V2>
+__device__ __forceinline__ void st_split_lanes(uint2*, V2, bool) {}
 template <typename V2>
 __device__ __forceinline__ void st_split(uint2*, V2) {}  // fp32 vectors: never instantiated by a split mode
 template <typename V2>
@@ -267,12 +296,14 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
                                          const typename Vec2<VT>::type* __restrict__ bra, int r, int q, int32_t Rp,
                                          VT alpha, VT beta, int32_t opts, double& p0, double& p1,
                                          const typename HVal<VT, CH>::type* __restrict__ hv2 = nullptr,
-                                         typename Vec2<VT>::type* y2 = nullptr) {
+                                         typename Vec2<VT>::type* y2 = nullptr,
+                                         typename Vec2<VT>::type* y3 = nullptr) {
     using V2 = typename Vec2<VT>::type;
     using H = typename HVal<VT, CH>::type;
     // the update and moment form of the mode, and its extra outputs
-    constexpr bool FIRST = MODE == kFirst || MODE == kFirstSplit;
-    constexpr bool DUAL = MODE == kDual || MODE == kDualSplit;
+    constexpr bool FIRST = MODE == kFirst || MODE == kFirstSplit || MODE == kPairFirst;
+    constexpr bool PAIR = MODE == kPairFirst || MODE == kPairRec;
+    constexpr bool DUAL = MODE == kDual || MODE == kDualSplit || PAIR;
     constexpr bool SPLIT_Y = MODE == kFirstSplit || MODE == kRecDblSplit;
     const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * CPL;
 
@@ -370,12 +401,44 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
     // 16-byte store (own is even when CPL is)
     if constexpr (SPLIT_Y || MODE == kDualSplit) {
         uint2* ws = reinterpret_cast<uint2*>(y2) + own;
+        if constexpr (CPL == 1 && LPR >= 2) {
+            // one value per lane: row neighbours' words in one 16-byte store
+            // (own is even for even q: Rp = LPR is even)
+            st_split_lanes(ws, SPLIT_Y ? sv[0] : acc2[0], (q & 1) == 0);
+        } else {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                if (CPL % 2 == 0 && (c & 1)) continue;
+                const V2& a0 = SPLIT_Y ? sv[c] : acc2[c];
+                if (CPL % 2 == 0) {
+                    const V2& a1 = SPLIT_Y ? sv[c + 1] : acc2[c + 1];
+                    st_split2(ws + c, a0, a1);
+                } else {
+                    st_split(ws + c, a0);
+                }
+            }
+        }
+    }
+    // paired chains: a lane's columns lie in one half (Rp/2 is a multiple
+    // of CPL); the left half stores y, the right half H2 x
+    if constexpr (PAIR) {
+        const int half = Rp >> 1;
+        const int col = q * CPL;
+        const bool left = col < half;
+        uint2* ws = left ? reinterpret_cast<uint2*>(y2) + static_cast<size_t>(r) * half + col
+                         : reinterpret_cast<uint2*>(y3) + static_cast<size_t>(r) * half + (col - half);
+        if constexpr (CPL == 1) {
+            if ((half & 1) == 0) {  // even halves: neighbour lanes' words in one 16-byte store
+                st_split_lanes(ws, left ? sv[0] : acc2[0], (q & 1) == 0);
+                return;
+            }
+        }
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
             if (CPL % 2 == 0 && (c & 1)) continue;
-            const V2& a0 = SPLIT_Y ? sv[c] : acc2[c];
+            const V2 a0 = left ? sv[c] : acc2[c];
             if (CPL % 2 == 0) {
-                const V2& a1 = SPLIT_Y ? sv[c + 1] : acc2[c + 1];
+                const V2 a1 = left ? sv[c + 1] : acc2[c + 1];
                 st_split2(ws + c, a0, a1);
             } else {
                 st_split(ws + c, a0);
@@ -391,7 +454,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y,
     const typename Vec2<VT>::type* __restrict__ bra, int32_t rows, int32_t Rp, VT alpha, VT beta,
     int32_t opts, int64_t ntiles, TileOrder order, int32_t xcd_split, double2* __restrict__ partial,
-    const typename HVal<VT, CH>::type* __restrict__ hv2, typename Vec2<VT>::type* y2) {
+    const typename HVal<VT, CH>::type* __restrict__ hv2, typename Vec2<VT>::type* y2,
+    typename Vec2<VT>::type* y3) {
     constexpr int RPW = kWave / LPR;
     constexpr int TR = kWavesPerBlock * RPW;
     static_assert(LPR >= 1 && LPR <= kWave && (kWave % LPR) == 0, "bad lanes per row");
@@ -412,7 +476,8 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
         int r = static_cast<int>(rt * TR) + wave * RPW + sub;
         if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
         if (r >= rows) continue;
-        step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q, Rp, alpha, beta, opts, p0, p1, hv2, y2);
+        step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q, Rp, alpha, beta, opts, p0, p1, hv2, y2,
+                                         y3);
     }
 
     // deterministic block reduction: butterfly in the wave, then waves in order

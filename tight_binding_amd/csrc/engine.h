@@ -183,8 +183,12 @@ void spectral_bounds(tbk_context* ctx, const tbk_matrix* M, int max_iters, doubl
                      hipStream_t st, tbk_bounds* out);
 void matrix_fp32(tbk_matrix* m);
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row);
+StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row,
+                   const tbk_tuning& tu);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2);
+void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
+                 const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2, void* y3);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial);
 
@@ -241,7 +245,13 @@ void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const 
                                   void* W1, int64_t K, int N, double2* part, hipStream_t st, const void* vx_values);
 void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
                                   double2* P, double2* out, hipStream_t st);
+// pre-split stacks with both chains in one block of 2 Rp columns (fused v_x
+// only; pP: H's plan at 2 Rp columns, row kernel); X0, X1: 2 blocks each, S:
+// one block of scratch
+void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a, double b, const void* kets,
+                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int N,
+                                double2* part, hipStream_t st, const void* vx_values);
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
-                            int64_t N, bool fused, bool presplit);
+                            int64_t N, bool fused, bool presplit, bool paired = false);
 
 }  // namespace engine

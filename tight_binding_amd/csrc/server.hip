@@ -467,18 +467,30 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     // the server's default tuning always takes the row kernel it needs
     const bool presplit = kubo_presplit(r->precision, N, j->H->is_complex, j->vx->is_complex);
     if (presplit && (pH.kc.kch != 0 || pX.kc.kch != 0)) raise(TBK_E_UNSUPPORTED, "pre-split stacks need the row kernel");
+    const void* vxv = fused_velocity(j->H, j->vx, pH, bstream) ? j->vx->val64.p : nullptr;
+    // pre-split with a fused v_x: both chains as one block of 2 Rp columns in
+    // W0, W1 (two blocks each) and S -- the five blocks the footprint counts
+    StepPlan pP{};
+    bool paired = false;
+    if (presplit && vxv && 2 * Rp <= 256) {
+        pP = make_plan(ctx, j->H, TBK_FP64, 2 * Rp, rows, b, true, tbk_tuning{});
+        paired = pP.kc.kch == 0;
+    }
     const size_t entry = presplit ? blk / 2 : blk;
     j->kets.reserve(blk);
     j->L.reserve(static_cast<size_t>(N) * entry);
     j->Rs.reserve(static_cast<size_t>(N) * entry);
-    j->W0.reserve(blk);
-    j->W1.reserve(blk);
+    j->W0.reserve(paired ? 2 * blk : blk);
+    j->W1.reserve(paired ? 2 * blk : blk);
     if (presplit) {
-        j->B0.reserve(blk);
-        j->B1.reserve(blk);
+        if (!paired) {
+            j->B0.reserve(blk);
+            j->B1.reserve(blk);
+        }
         j->S.reserve(blk);
     }
-    j->part.reserve(static_cast<size_t>(std::max(pH.nb, std::max(pX.nb, pY.nb))) * sizeof(double2));
+    j->part.reserve(static_cast<size_t>(std::max({pH.nb, pX.nb, pY.nb, paired ? pP.nb : int64_t{0}})) *
+                    sizeof(double2));
     const ContractionShape cs = contraction_plan(ctx, N, K, r->precision);
     const int64_t NN = static_cast<int64_t>(N) * N;
     j->P.reserve(static_cast<size_t>(cs.nchunks) * NN * sizeof(double2));
@@ -501,8 +513,12 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st_,
                        j->kets.as<double2>(), rows, r->R, Rp, r->seed, inv);
     HIPC(hipGetLastError());
-    const void* vxv = fused_velocity(j->H, j->vx, pH, bstream) ? j->vx->val64.p : nullptr;
-    if (presplit) {
+    if (paired) {
+        enqueue_kubo_stacks_paired(pP, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->W0.p, j->W1.p,
+                                   j->S.p, rows, Rp, N, j->part.as<double2>(), st_, vxv);
+        enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, j->P.as<double2>(),
+                                     j->G.as<double2>(), st_);
+    } else if (presplit) {
         enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->B0.p,
                                      j->B1.p, j->S.p, j->W0.p, j->W1.p, K, N, j->part.as<double2>(), st_, vxv);
         enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, j->P.as<double2>(),

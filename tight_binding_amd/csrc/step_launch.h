@@ -27,6 +27,7 @@ struct StepArgs {
     double2* partial;
     const void* hv2 = nullptr;  // kDual: values of the second matrix (same pattern)
     void* y2 = nullptr;         // kDual: its product with x
+    void* y3 = nullptr;         // kPairFirst, kPairRec: the right half's split words
 };
 
 // Arguments of cheb_slots_kernel (one step of every slot of a class; the
@@ -93,7 +94,7 @@ bool launch_step(dim3 grid, hipStream_t st, const StepArgs& a) {
                            static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
                            static_cast<V2*>(a.y), static_cast<const V2*>(a.bra), a.rows, a.Rp,
                            static_cast<VT>(a.alpha), static_cast<VT>(a.beta), a.opts, a.ntiles, a.order, a.xcd_split,
-                           a.partial, static_cast<const H*>(a.hv2), static_cast<V2*>(a.y2));
+                           a.partial, static_cast<const H*>(a.hv2), static_cast<V2*>(a.y2), static_cast<V2*>(a.y3));
         return true;
     } else if constexpr (MODE >= kDual) {
         return false;  // the dual product and split outputs exist for the row kernel only
@@ -130,6 +131,14 @@ bool launch_mode(int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
         case kDualSplit:
             if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
                 return launch_step<VT, CH, LPR, CPL, kDualSplit, 0>(grid, st, a);
+            return false;
+        case kPairFirst:
+            if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
+                return launch_step<VT, CH, LPR, CPL, kPairFirst, 0>(grid, st, a);
+            return false;
+        case kPairRec:
+            if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
+                return launch_step<VT, CH, LPR, CPL, kPairRec, 0>(grid, st, a);
             return false;
         default: return launch_step<VT, CH, LPR, CPL, kRecDbl, KCH>(grid, st, a);
     }

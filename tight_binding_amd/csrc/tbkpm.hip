@@ -200,13 +200,14 @@ void matrix_fp32(tbk_matrix* m) {
     m->have32 = true;
 }
 
-StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row) {
+StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row,
+                   const tbk_tuning& tu) {
     StepPlan pl{};
     pl.precision = prec;
     pl.cplx = M->is_complex;
     if (prec == TBK_FP32) matrix_fp32(const_cast<tbk_matrix*>(M));
-    pl.kc = choose_kernel(ctx->tuning, prec, M->is_complex, Rp, M->max_row_nnz, M->bandwidth95,
-                          force_row || ctx->tuning.use_tiled <= 0);
+    pl.kc = choose_kernel(tu, prec, M->is_complex, Rp, M->max_row_nnz, M->bandwidth95,
+                          force_row || tu.use_tiled <= 0);
     const KernelChoice& kc = pl.kc;
     const int TR = kc.rows_per_tile;
     const int64_t row_tiles = (static_cast<int64_t>(rows) + TR - 1) / TR;
@@ -214,12 +215,12 @@ StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int 
     // period P = (matrix bandwidth) / (chunk rows) when that exceeds 1
     tbk::TileOrder order{};
     {
-        const int64_t chunk_rows = ctx->tuning.chunk_rows > 0 ? ctx->tuning.chunk_rows : 256;
+        const int64_t chunk_rows = tu.chunk_rows > 0 ? tu.chunk_rows : 256;
         const int64_t tpc = std::max<int64_t>(1, chunk_rows / TR);
         int64_t P = 1;
-        if (ctx->tuning.chunk_period > 0)
-            P = ctx->tuning.chunk_period;
-        else if (ctx->tuning.chunk_period == 0)
+        if (tu.chunk_period > 0)
+            P = tu.chunk_period;
+        else if (tu.chunk_period == 0)
             P = std::max<int64_t>(1, (M->bandwidth95 + (tpc * TR) / 2) / (tpc * TR));
         const int64_t nchunks = (row_tiles + tpc - 1) / tpc;
         P = std::min<int64_t>(P, nchunks);
@@ -231,11 +232,11 @@ StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int 
         order.Q = static_cast<uint32_t>(Q);
     }
     const int64_t ntiles = static_cast<int64_t>(order.P) * order.Q * order.tpc * (Rp / kc.width);
-    const int bpc = ctx->tuning.blocks_per_cu > 0 ? ctx->tuning.blocks_per_cu : 4;
+    const int bpc = tu.blocks_per_cu > 0 ? tu.blocks_per_cu : 4;
     int64_t cap = static_cast<int64_t>(ctx->num_cu) * bpc;
     int64_t nb = std::min<int64_t>(ntiles, cap);
     int xcd_split = 0;
-    if ((nb >= 64 && ctx->tuning.xcd_split >= 0) || (ctx->tuning.xcd_split > 0 && nb >= 8)) {
+    if ((nb >= 64 && tu.xcd_split >= 0) || (tu.xcd_split > 0 && nb >= 8)) {
         nb = (nb / 8) * 8;
         xcd_split = 1;
     }
@@ -246,11 +247,15 @@ StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int 
     A.hv = prec == TBK_FP64 ? M->val64.p : M->val32.p;
     A.rows = rows;
     A.Rp = Rp;
-    A.opts = ((b != 0.0) ? tbk::kOptShift : 0) | (ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0);
+    A.opts = ((b != 0.0) ? tbk::kOptShift : 0) | (tu.stream_nt >= 0 ? tbk::kOptStream : 0);
     A.ntiles = ntiles;
     A.order = order;
     A.xcd_split = xcd_split;
     return pl;
+}
+
+StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row) {
+    return make_plan(ctx, M, prec, Rp, rows, b, force_row, ctx->tuning);
 }
 
 // One step: y = alpha*(M x) + beta*x - z (mode kFirst: no z), partial sums of
@@ -262,6 +267,11 @@ void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, dou
 
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2) {
+    launch_plan(pl, st, mode, alpha, beta, x, z, y, bra, partial, hv2, y2, nullptr);
+}
+
+void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
+                 const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2, void* y3) {
     StepArgs A = pl.args;
     A.alpha = alpha;
     A.beta = beta;
@@ -272,6 +282,7 @@ void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, dou
     A.partial = partial;
     A.hv2 = hv2;
     A.y2 = y2;
+    A.y3 = y3;
     launch_any(pl.precision, pl.cplx, pl.kc.width, pl.kc.kch, pl.kc.cpl, mode, dim3(static_cast<unsigned>(pl.nb)), st,
                A);
     HIPC(hipGetLastError());
@@ -637,6 +648,39 @@ void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const 
         launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(N - 1), nullptr, S, W(N - 1), part, nullptr, Rk(N - 1));
 }
 
+// Both chains of the pre-split stacks as one block of 2 Rp columns, the left
+// chain in columns [0, Rp) and the right one in [Rp, 2 Rp): X_0 = [|r> |
+// v_y |r>], X_1 = H~ X_0, X_{k+1} = 2 H~ X_k - X_{k-1} in two rolling blocks.
+// Each step stores the left half's X_{k+1} as words at Ls[k + 1] and the
+// right half's v_x X_k at Rs[k] (kPairFirst, kPairRec): the matrix is read
+// once per index for both chains, one launch instead of two.  The words are
+// enqueue_kubo_stacks_presplit's with the fused v_x product, bit for bit
+// (the same sums per lane).  The last step's left output is not a stack
+// entry and goes to S.
+void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a, double b, const void* kets,
+                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int N,
+                                double2* part, hipStream_t st, const void* vx_values) {
+    if (pP.kc.kch != 0 || !pP.cplx || !vx_values) raise(TBK_E_UNSUPPORTED, "paired stacks need the fused row kernel");
+    const int64_t K = rows * Rp;
+    auto Lk = [&](int64_t k) { return Ls + k * K; };
+    auto Rk = [&](int64_t k) { return Rs + k * K; };
+    auto X = [&](int64_t k) { return (k & 1) ? X1 : X0; };
+    tbk::launch_split_block(static_cast<const double2*>(kets), Lk(0), K, st);
+    HIPC(hipGetLastError());
+    launch_plan(pY, st, tbk::kFirst, 1.0, 0.0, kets, nullptr, S, kets, part);  // v_y |r>
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((2 * K + tbk::kBlock - 1) / tbk::kBlock, 65536));
+    hipLaunchKernelGGL(tbk::pair_blocks_kernel, dim3(grid), dim3(tbk::kBlock), 0, st,
+                       static_cast<const double2*>(kets), static_cast<const double2*>(S), static_cast<double2*>(X0),
+                       rows, Rp);
+    HIPC(hipGetLastError());
+    uint2* spare = static_cast<uint2*>(S);
+    launch_plan(pP, st, tbk::kPairFirst, 1.0 / a, -b / a, X(0), nullptr, X(1), X(0), part, vx_values,
+                N > 1 ? Lk(1) : spare, Rk(0));
+    for (int64_t k = 1; k < N; ++k)  // X_{k+1} over X_{k-1}
+        launch_plan(pP, st, tbk::kPairRec, 2.0 / a, -2.0 * b / a, X(k), X(k - 1), X(k + 1), nullptr, part, vx_values,
+                    k + 1 < N ? Lk(k + 1) : spare, Rk(k));
+}
+
 void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
                                   double2* P, double2* out, hipStream_t st) {
     if (!cs.big) raise(TBK_E_UNSUPPORTED, "pre-split contraction needs N >= 256");
@@ -647,7 +691,7 @@ void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, co
 }
 
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
-                            int64_t N, bool fused, bool presplit) {
+                            int64_t N, bool fused, bool presplit, bool paired) {
     // per launch: the matrix arrays once; vector blocks x (+ z) read and y
     // written; split words are half a block
     auto mat = [&](const tbk_matrix* m) {
@@ -658,6 +702,15 @@ double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk
     const double split = presplit ? half : 0.0;
     const double vals = static_cast<double>(vx->nnz) * 16.0;
     const double Nm2 = static_cast<double>(std::max<int64_t>(0, N - 2));
+    if (paired) {
+        // kets' words; v_y |r>; the paired block X_0; N launches over 2 R
+        // columns, each reading the matrix, H2's values, x (and z) and
+        // writing y and two half-blocks of words
+        double bytes = blk + half + mat(vy) + 2.0 * blk + 4.0 * blk;
+        bytes += mat(H) + vals + 4.0 * blk + blk;
+        bytes += static_cast<double>(N - 1) * (mat(H) + vals + 6.0 * blk + blk);
+        return bytes;
+    }
     double bytes = 2.0 * blk;  // the kets copied (into the stack, or the rolling block)
     // left chain
     if (presplit) bytes += blk + half;  // the kets' words
@@ -746,9 +799,20 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     const StepPlan pX = make_plan(ctx, vx, TBK_FP64, Rp, rows, 0.0, false);
     const StepPlan pY = make_plan(ctx, vy, TBK_FP64, Rp, rows, 0.0, false);
     const bool presplit = kubo_presplit(prec, N, H->is_complex, vx->is_complex) && pH.kc.kch == 0 && pX.kc.kch == 0;
+    const bool fuse = fused_velocity(H, vx, pH, ctx->stream);
+    // pre-split stacks with a fused v_x: both chains as one block of 2 Rp columns
+    StepPlan pP{};
+    bool paired = false;
+    if (presplit && fuse && 2 * Rp <= 256) {
+        // the default launch shape: the context's tuning is chosen for Rp
+        // columns, and the paired step is fastest at the defaults (16-byte
+        // lanes, 4 workgroups per CU) across measured shapes
+        pP = make_plan(ctx, H, TBK_FP64, 2 * Rp, rows, b, true, tbk_tuning{});
+        paired = pP.kc.kch == 0;
+    }
     const size_t entry_bytes = presplit ? blk_bytes / 2 : blk_bytes;  // one stack entry
     const size_t stack_bytes = static_cast<size_t>(N) * entry_bytes;
-    const int rolling = presplit ? 5 : 2;  // fp64 work blocks
+    const int rolling = paired ? 7 : presplit ? 5 : 2;  // fp64 work blocks
     DevBuf& L = ctx->stack[0];
     DevBuf& Rs = ctx->stack[1];
     if (L.bytes < stack_bytes || Rs.bytes < stack_bytes) {
@@ -768,14 +832,20 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     }
     ctx->work[0].reserve(blk_bytes);
     ctx->work[1].reserve(blk_bytes);
-    if (presplit)
-        for (DevBuf& w : ctx->kwork) w.reserve(blk_bytes);
-    ctx->partial.reserve(static_cast<size_t>(std::max(pH.nb, std::max(pX.nb, pY.nb))) * sizeof(double2));
+    if (presplit) {
+        ctx->kwork[0].reserve(paired ? 2 * blk_bytes : blk_bytes);
+        ctx->kwork[1].reserve(paired ? 2 * blk_bytes : blk_bytes);
+        ctx->kwork[2].reserve(blk_bytes);
+    }
+    ctx->partial.reserve(static_cast<size_t>(std::max({pH.nb, pX.nb, pY.nb, paired ? pP.nb : int64_t{0}})) *
+                         sizeof(double2));
     double2* part = ctx->partial.as<double2>();
-    const bool fuse = fused_velocity(H, vx, pH, ctx->stream);
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    if (presplit)
+    if (paired)
+        enqueue_kubo_stacks_paired(pP, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
+                                   ctx->kwork[1].p, ctx->kwork[2].p, rows, Rp, N, part, ctx->stream, vx->val64.p);
+    else if (presplit)
         enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
                                      ctx->kwork[1].p, ctx->kwork[2].p, ctx->work[0].p, ctx->work[1].p, K, N, part,
                                      ctx->stream, fuse ? vx->val64.p : nullptr);
@@ -806,7 +876,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         stats->flops = 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
         stats->stack_bytes = 2.0 * static_cast<double>(stack_bytes);
         stats->nchunks = nchunks;
-        const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N, fuse, presplit);
+        const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N, fuse, presplit, paired);
         stats->recursion_bytes = bytes;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
