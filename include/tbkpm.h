@@ -292,7 +292,7 @@ typedef struct tbk_kubo_stats {
     double contraction_ms; /* device time of the Gamma contraction and its chunk reduction */
     double wall_ms;
     double flops;          /* 8 N^2 K: real flops of the complex contraction, K = sites x columns */
-    double stack_bytes;    /* bytes of the two stacks held during the call */
+    double stack_bytes;    /* bytes of the two stacks' entries (each entry padded to a multiple of 64 elements) */
     int32_t nchunks;       /* split-K chunks of the contraction */
     int32_t reserved;
     double recursion_bytes; /* algorithmic HBM bytes of the stack-building launches and copies */

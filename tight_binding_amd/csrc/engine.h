@@ -232,25 +232,29 @@ struct ContractionShape {
     tbk::GammaMap map;
 };
 ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int prec);
-void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int prec,
+void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int64_t ldk,
+                         int prec,
                          double2* P, double2* out, hipStream_t st);
 // vx_values: v_x's values when v_x has H's pattern (fused_velocity), else null
+// stack entry n of L and Rs at n * ldk elements (stack_stride)
+int64_t stack_stride(int64_t K);
 void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
-                         const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int N,
-                         double2* part, hipStream_t st, const void* vx_values);
+                         const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int64_t ldk,
+                         int N, double2* part, hipStream_t st, const void* vx_values);
 bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH, hipStream_t st);
 bool kubo_presplit(int prec, int N, bool h_complex, bool vx_complex);
 void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
                                   const void* kets, uint2* Ls, uint2* Rs, void* B0, void* B1, void* S, void* W0,
-                                  void* W1, int64_t K, int N, double2* part, hipStream_t st, const void* vx_values);
+                                  void* W1, int64_t K, int64_t ldk, int N, double2* part, hipStream_t st,
+                                  const void* vx_values);
 void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
-                                  double2* P, double2* out, hipStream_t st);
+                                  int64_t ldk, double2* P, double2* out, hipStream_t st);
 // pre-split stacks with both chains in one block of 2 Rp columns (fused v_x
 // only; pP: H's plan at 2 Rp columns, row kernel); X0, X1: 2 blocks each, S:
 // one block of scratch
 void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a, double b, const void* kets,
-                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int N,
-                                double2* part, hipStream_t st, const void* vx_values);
+                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int64_t ldk,
+                                int N, double2* part, hipStream_t st, const void* vx_values);
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
                             int64_t N, bool fused, bool presplit, bool paired = false);
 

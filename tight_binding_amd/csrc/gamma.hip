@@ -39,7 +39,8 @@ __device__ __forceinline__ double2 ld_or_zero(const double2* p, bool ok) {
 }
 
 __global__ __launch_bounds__(256) void gamma_f64_kernel(const double2* __restrict__ A, const double2* __restrict__ B,
-                                                        int N, int64_t K, int64_t kc, double2* __restrict__ P) {
+                                                        int N, int64_t K, int64_t ldk, int64_t kc,
+                                                        double2* __restrict__ P) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int n0 = blockIdx.x * kGammaTile + (wave >> 1) * 32;
@@ -52,8 +53,8 @@ __global__ __launch_bounds__(256) void gamma_f64_kernel(const double2* __restric
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int rn = n0 + 16 * i + li, rm = m0 + 16 * i + li;
-        a[i] = A + static_cast<int64_t>(rn < N ? rn : N - 1) * K;
-        b[i] = B + static_cast<int64_t>(rm < N ? rm : N - 1) * K;
+        a[i] = A + static_cast<int64_t>(rn < N ? rn : N - 1) * ldk;
+        b[i] = B + static_cast<int64_t>(rm < N ? rm : N - 1) * ldk;
     }
     v4d re[2][2], im[2][2];
 #pragma unroll
@@ -115,11 +116,14 @@ __global__ __launch_bounds__(256) void gamma_f64_kernel(const double2* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Large-N fp64 contraction: a 128 x 128 Gamma tile per workgroup, each wave a
-// 64 x 64 sub-tile (4 x 4 blocks of 16 x 16).  Tiles of A and B (128 rows x 4
-// complex columns) are staged through LDS, double-buffered, rows padded to 5
-// double2 to spread the fragment reads over the banks; the next tile's global
-// loads are in flight while the matrix cores work on the current one.
+// Large-N fp64 contraction: a 128 x 128 Gamma tile per workgroup.  Tiles of A
+// and B (128 rows x 8 complex columns, two MFMA k-steps) are staged through
+// LDS, double-buffered, rows padded to 9 double2 to spread the fragment reads
+// over the banks; the next tile's global loads are in flight while the matrix
+// cores work on the current one.  A wave's staging load covers 8 rows x 8
+// columns: every stack row is its own page, and the address translations a
+// load needs (one per row it touches) bound the kernel at C3 sizes, so wider
+// row pieces per load mean fewer translations per byte.
 //
 // Work item w = tile + tiles * chunk.  Workgroup b runs on XCD b % 8; the
 // items are dealt so that XCD x takes w in [x * per_xcd, (x + 1) * per_xcd):
@@ -139,12 +143,13 @@ __device__ __forceinline__ bool xcd_work_item(const GammaMap& m, int& tn, int& t
     return true;
 }
 
-constexpr int kBigKt = 4;               // complex k per LDS stage
+constexpr int kBigKt = 8;               // complex k per LDS stage (two MFMA k-steps)
 constexpr int kBigStride = kBigKt + 1;  // padded LDS row stride (double2)
 
 __global__ __launch_bounds__(512) void gamma_f64_big_kernel(const double2* __restrict__ A,
                                                             const double2* __restrict__ B, int N, int64_t K,
-                                                            int64_t kc, GammaMap map, double2* __restrict__ P) {
+                                                            int64_t ldk, int64_t kc, GammaMap map,
+                                                            double2* __restrict__ P) {
     __shared__ double2 As[2][kBigTile * kBigStride];
     __shared__ double2 Bs[2][kBigTile * kBigStride];
     int tn, tm;
@@ -159,22 +164,41 @@ __global__ __launch_bounds__(512) void gamma_f64_big_kernel(const double2* __res
     const int64_t kb = c * kc;
     const int64_t ke = (kb + kc < K) ? kb + kc : K;
 
-    // staging: thread t loads row t / 4, column t % 4 of each operand
-    const int cc = tid & 3, r = tid >> 2;
-    const int rn = (n0 + r < N) ? n0 + r : N - 1;
-    const int rm = (m0 + r < N) ? m0 + r : N - 1;
-    const double2* ga = A + static_cast<int64_t>(rn) * K + cc;
-    const double2* gb = B + static_cast<int64_t>(rm) * K + cc;
-    const int soff = r * kBigStride + cc;
-    double2 ra, rb;
+    // staging: thread t loads column t % 8 of rows t / 8 and t / 8 + 64 of
+    // each operand
+    const int cc = tid & 7, r = tid >> 3;
+    const double2* ga[2];
+    const double2* gb[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int rn = (n0 + r + 64 * i < N) ? n0 + r + 64 * i : N - 1;
+        const int rm = (m0 + r + 64 * i < N) ? m0 + r + 64 * i : N - 1;
+        ga[i] = A + static_cast<int64_t>(rn) * ldk + cc;
+        gb[i] = B + static_cast<int64_t>(rm) * ldk + cc;
+    }
+    double2 ra[2], rb[2];
     auto gload = [&](int64_t k0) {
-        const bool ok = k0 + cc < ke;
-        ra = ld_or_zero(ga + k0, ok);
-        rb = ld_or_zero(gb + k0, ok);
+        if (k0 + kBigKt <= ke) {  // a full stage: plain loads
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ra[i] = ga[i][k0];
+                rb[i] = gb[i][k0];
+            }
+        } else {
+            const bool ok = k0 + cc < ke;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ra[i] = ld_or_zero(ga[i] + k0, ok);
+                rb[i] = ld_or_zero(gb[i] + k0, ok);
+            }
+        }
     };
     auto sstore = [&](int buf) {
-        As[buf][soff] = ra;
-        Bs[buf][soff] = rb;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            As[buf][(r + 64 * i) * kBigStride + cc] = ra[i];
+            Bs[buf][(r + 64 * i) * kBigStride + cc] = rb[i];
+        }
     };
 
     // Three real products per complex one (Gauss):
@@ -198,23 +222,26 @@ __global__ __launch_bounds__(512) void gamma_f64_big_kernel(const double2* __res
     for (int64_t k0 = kb; k0 < ke; k0 += kBigKt) {
         const bool more = k0 + kBigKt < ke;
         if (more) gload(k0 + kBigKt);
-        double2 a[4], b[2];
-        double bs[2];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = As[buf][(wn * 64 + 16 * i + li) * kBigStride + lk];
+        for (int ks = 0; ks < kBigKt / 4; ++ks) {  // the stage's MFMA k-steps of 4
+            double2 a[4], b[2];
+            double bs[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            b[j] = Bs[buf][(wm * 32 + 16 * j + li) * kBigStride + lk];
-            bs[j] = b[j].x + b[j].y;
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const double ad = a[i].x - a[i].y;
+            for (int i = 0; i < 4; ++i) a[i] = As[buf][(wn * 64 + 16 * i + li) * kBigStride + 4 * ks + lk];
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                t1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, t1[i][j], 0, 0, 0);
-                t2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, t2[i][j], 0, 0, 0);
-                t3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bs[j], t3[i][j], 0, 0, 0);
+                b[j] = Bs[buf][(wm * 32 + 16 * j + li) * kBigStride + 4 * ks + lk];
+                bs[j] = b[j].x + b[j].y;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double ad = a[i].x - a[i].y;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    t1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, t1[i][j], 0, 0, 0);
+                    t2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, t2[i][j], 0, 0, 0);
+                    t3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, bs[j], t3[i][j], 0, 0, 0);
+                }
             }
         }
         if (more) sstore(buf ^ 1);
@@ -265,7 +292,7 @@ __device__ __forceinline__ Split split4(const double2* v, bool neg_swap) {
 
 __global__ __launch_bounds__(256) void gamma_bf16x3_kernel(const double2* __restrict__ A,
                                                            const double2* __restrict__ B, int N, int64_t K,
-                                                           int64_t kc, double2* __restrict__ P) {
+                                                           int64_t ldk, int64_t kc, double2* __restrict__ P) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int n0 = blockIdx.x * kGammaTile + (wave >> 1) * 32;
@@ -278,8 +305,8 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_kernel(const double2* __rest
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int rn = n0 + 16 * i + li, rm = m0 + 16 * i + li;
-        a[i] = A + static_cast<int64_t>(rn < N ? rn : N - 1) * K;
-        b[i] = B + static_cast<int64_t>(rm < N ? rm : N - 1) * K;
+        a[i] = A + static_cast<int64_t>(rn < N ? rn : N - 1) * ldk;
+        b[i] = B + static_cast<int64_t>(rm < N ? rm : N - 1) * ldk;
     }
     v4f re[2][2], im[2][2];
 #pragma unroll
@@ -386,7 +413,7 @@ __device__ __forceinline__ uint32_t im_pair(uint32_t w0, uint32_t w1) { return (
 // pre-split by the recursion (the same words), half the bytes to stage
 template <typename T>
 __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restrict__ A, const T* __restrict__ B,
-                                                               int N, int64_t K, int64_t kc, GammaMap map,
+                                                               int N, int64_t K, int64_t ldk, int64_t kc, GammaMap map,
                                                                double2* __restrict__ P) {
     __shared__ __align__(16) uint32_t As[2][kBigTile * kSplitRowWords];
     __shared__ __align__(16) uint32_t Bs[2][kBigTile * kSplitRowWords];
@@ -417,11 +444,11 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
         const int r = r0 + 64 * i;
         const int rn = (n0 + r < N) ? n0 + r : N - 1;
         const int rm = (m0 + r < N) ? m0 + r : N - 1;
-        ga[i] = A + static_cast<int64_t>(rn) * K;
-        gb[i] = B + static_cast<int64_t>(rm) * K;
+        ga[i] = A + static_cast<int64_t>(rn) * ldk;
+        gb[i] = B + static_cast<int64_t>(rm) * ldk;
     }
     T ra[2][2], rb[2][2];
-    const bool rows16 = (K & 1) == 0;  // row starts, and so column pairs, 16-byte aligned
+    const bool rows16 = (ldk & 1) == 0;  // row starts, and so column pairs, 16-byte aligned
     auto gload = [&](int64_t k0) {
         const int64_t e = k0 + 2 * p;
         if (k0 + kSplitKt <= ke) {  // a full stage (all but a row's last, short one): plain loads
@@ -668,14 +695,14 @@ dim3 gamma_grid(int N, int nchunks) {
 
 }  // namespace
 
-void launch_gamma_f64(const double2* A, const double2* B, int N, int64_t K, int64_t kc, int nchunks, double2* P,
-                      hipStream_t st) {
-    hipLaunchKernelGGL(gamma_f64_kernel, gamma_grid(N, nchunks), dim3(256), 0, st, A, B, N, K, kc, P);
+void launch_gamma_f64(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc, int nchunks,
+                      double2* P, hipStream_t st) {
+    hipLaunchKernelGGL(gamma_f64_kernel, gamma_grid(N, nchunks), dim3(256), 0, st, A, B, N, K, ldk, kc, P);
 }
 
-void launch_gamma_bf16x3(const double2* A, const double2* B, int N, int64_t K, int64_t kc, int nchunks, double2* P,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(gamma_bf16x3_kernel, gamma_grid(N, nchunks), dim3(256), 0, st, A, B, N, K, kc, P);
+void launch_gamma_bf16x3(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc, int nchunks,
+                         double2* P, hipStream_t st) {
+    hipLaunchKernelGGL(gamma_bf16x3_kernel, gamma_grid(N, nchunks), dim3(256), 0, st, A, B, N, K, ldk, kc, P);
 }
 
 void launch_gamma_reduce(const double2* P, int nchunks, int64_t NN, double2* out, hipStream_t st) {
@@ -685,22 +712,22 @@ void launch_gamma_reduce(const double2* P, int nchunks, int64_t NN, double2* out
 }
 
 
-void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
-                          double2* P, hipStream_t st) {
+void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc,
+                          const GammaMap& map, double2* P, hipStream_t st) {
     hipLaunchKernelGGL(gamma_f64_big_kernel, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0, st, A, B, N,
-                       K, kc, map, P);
+                       K, ldk, kc, map, P);
 }
 
-void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
-                             double2* P, hipStream_t st) {
+void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc,
+                             const GammaMap& map, double2* P, hipStream_t st) {
     hipLaunchKernelGGL(gamma_bf16x3_big_kernel<double2>, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0,
-                       st, A, B, N, K, kc, map, P);
+                       st, A, B, N, K, ldk, kc, map, P);
 }
 
-void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t kc,
+void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t ldk, int64_t kc,
                                       const GammaMap& map, double2* P, hipStream_t st) {
     hipLaunchKernelGGL(gamma_bf16x3_big_kernel<uint2>, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0,
-                       st, A, B, N, K, kc, map, P);
+                       st, A, B, N, K, ldk, kc, map, P);
 }
 
 namespace {

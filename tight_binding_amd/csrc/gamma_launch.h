@@ -21,31 +21,32 @@ struct GammaMap {
 };
 
 // Split-K partial sums P[c][n][m] for k in chunk c = [c*kc, min(K, (c+1)*kc)),
-// fp64 matrix cores.  A, B: N rows of K complex128 each.
-void launch_gamma_f64(const double2* A, const double2* B, int N, int64_t K, int64_t kc, int nchunks, double2* P,
-                      hipStream_t st);
+// fp64 matrix cores.  A, B: N rows of K complex128 each, row n at n * ldk
+// (ldk >= K; every launcher below takes the same row stride).
+void launch_gamma_f64(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc, int nchunks,
+                      double2* P, hipStream_t st);
 
 // Same partial sums with 128 x 128 tiles staged through LDS and the chunks
 // placed per XCD (for N >= 256).
-void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
-                          double2* P, hipStream_t st);
+void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc,
+                          const GammaMap& map, double2* P, hipStream_t st);
 
 // Split-BF16 three-term variant: each fp64 operand value x is split as
 // hi = bf16(x), lo = bf16(x - hi); products hi*hi + hi*lo + lo*hi are
 // accumulated in fp32 on matrix cores within a chunk and summed in fp64
 // across chunks.
-void launch_gamma_bf16x3(const double2* A, const double2* B, int N, int64_t K, int64_t kc, int nchunks, double2* P,
-                         hipStream_t st);
+void launch_gamma_bf16x3(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc, int nchunks,
+                         double2* P, hipStream_t st);
 
 // Split-bf16 x3 partial sums with 128 x 128 tiles (operands split once, at
 // LDS staging) and the chunks placed per XCD (for N >= 256).
-void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t kc, const GammaMap& map,
-                             double2* P, hipStream_t st);
+void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc,
+                             const GammaMap& map, double2* P, hipStream_t st);
 
 // The same contraction over stacks written pre-split by the recursion: each
 // complex element is its (hi, lo) words (split_bf16.h), 8 bytes instead of 16,
 // so the partial sums equal launch_gamma_bf16x3_big's on the fp64 stacks.
-void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t kc,
+void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t ldk, int64_t kc,
                                       const GammaMap& map, double2* P, hipStream_t st);
 
 // out[i] = split words of in[i] (n complex values)

@@ -544,23 +544,24 @@ ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int 
 }
 
 // split-K partial tiles into P (nchunks x N x N), then their ordered sum into out
-void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int prec,
-                         double2* P, double2* out, hipStream_t st) {
+void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int64_t ldk,
+                         int prec, double2* P, double2* out, hipStream_t st) {
     if (prec == TBK_GAMMA_FP64 && cs.big)
-        tbk::launch_gamma_f64_big(A, B, N, K, cs.kc, cs.map, P, st);
+        tbk::launch_gamma_f64_big(A, B, N, K, ldk, cs.kc, cs.map, P, st);
     else if (prec == TBK_GAMMA_FP64)
-        tbk::launch_gamma_f64(A, B, N, K, cs.kc, cs.nchunks, P, st);
+        tbk::launch_gamma_f64(A, B, N, K, ldk, cs.kc, cs.nchunks, P, st);
     else if (cs.big)
-        tbk::launch_gamma_bf16x3_big(A, B, N, K, cs.kc, cs.map, P, st);
+        tbk::launch_gamma_bf16x3_big(A, B, N, K, ldk, cs.kc, cs.map, P, st);
     else
-        tbk::launch_gamma_bf16x3(A, B, N, K, cs.kc, cs.nchunks, P, st);
+        tbk::launch_gamma_bf16x3(A, B, N, K, ldk, cs.kc, cs.nchunks, P, st);
     HIPC(hipGetLastError());
     tbk::launch_gamma_reduce(P, cs.nchunks, static_cast<int64_t>(N) * N, out, st);
     HIPC(hipGetLastError());
 }
 
 // Both Kubo stacks: L[n] = T_n(H~)|r>, R[m] = v_x T_m(H~) v_y|r> (n, m < N),
-// blocks of blk_bytes each; T_m of the right side rotates through W0, W1.
+// blocks of blk_bytes each, entry n at n * ldk elements (stack_stride); T_m
+// of the right side rotates through W0, W1.
 // v_x = [X, H~] stores H~'s pattern; when the uploaded arrays agree (same
 // fingerprint) and H~ uses the row kernel, v_x T_{m-1} is formed in the step
 // that gathers T_{m-1} for T_m (kDual), which saves a read of every right
@@ -571,11 +572,18 @@ bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& p
     return matrix_pattern(H, st) == matrix_pattern(vx, st);
 }
 
+// Row stride of the stacks: K rounded up to 64 elements, so every stack
+// entry starts 1 KB aligned and the contraction's 16-byte column-pair loads
+// apply whatever K is (K = sites x columns is odd for one column and an odd
+// site count).
+int64_t stack_stride(int64_t K) { return ((K + 63) / 64) * 64; }
+
 void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
-                         const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int N,
-                         double2* part, hipStream_t st, const void* vx_values) {
-    auto Lk = [&](int64_t k) { return static_cast<char*>(L) + static_cast<size_t>(k) * blk_bytes; };
-    auto Rk = [&](int64_t k) { return static_cast<char*>(Rs) + static_cast<size_t>(k) * blk_bytes; };
+                         const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int64_t ldk,
+                         int N, double2* part, hipStream_t st, const void* vx_values) {
+    const size_t entry = static_cast<size_t>(ldk) * sizeof(double2);
+    auto Lk = [&](int64_t k) { return static_cast<char*>(L) + static_cast<size_t>(k) * entry; };
+    auto Rk = [&](int64_t k) { return static_cast<char*>(Rs) + static_cast<size_t>(k) * entry; };
     auto W = [&](int64_t k) { return (k & 1) ? W1 : W0; };
     HIPC(hipMemcpyAsync(Lk(0), kets, blk_bytes, hipMemcpyDeviceToDevice, st));
     if (N > 1) launch_plan(pH, st, tbk::kFirst, 1.0 / a, -b / a, Lk(0), nullptr, Lk(1), Lk(0), part);
@@ -615,11 +623,12 @@ bool kubo_presplit(int prec, int N, bool h_complex, bool vx_complex) {
 // elements per block.
 void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
                                   const void* kets, uint2* Ls, uint2* Rs, void* B0, void* B1, void* S, void* W0,
-                                  void* W1, int64_t K, int N, double2* part, hipStream_t st, const void* vx_values) {
+                                  void* W1, int64_t K, int64_t ldk, int N, double2* part, hipStream_t st,
+                                  const void* vx_values) {
     if (pH.kc.kch != 0 || pX.kc.kch != 0 || !pH.cplx || !pX.cplx)
         raise(TBK_E_UNSUPPORTED, "pre-split stacks need the complex row kernel");
-    auto Lk = [&](int64_t k) { return Ls + k * K; };
-    auto Rk = [&](int64_t k) { return Rs + k * K; };
+    auto Lk = [&](int64_t k) { return Ls + k * ldk; };
+    auto Rk = [&](int64_t k) { return Rs + k * ldk; };
     auto B = [&](int64_t k) { return (k & 1) ? B1 : B0; };
     auto W = [&](int64_t k) { return (k & 1) ? W1 : W0; };
     HIPC(hipMemcpyAsync(B0, kets, static_cast<size_t>(K) * sizeof(double2), hipMemcpyDeviceToDevice, st));
@@ -658,12 +667,12 @@ void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const 
 // (the same sums per lane).  The last step's left output is not a stack
 // entry and goes to S.
 void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a, double b, const void* kets,
-                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int N,
-                                double2* part, hipStream_t st, const void* vx_values) {
+                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int64_t ldk,
+                                int N, double2* part, hipStream_t st, const void* vx_values) {
     if (pP.kc.kch != 0 || !pP.cplx || !vx_values) raise(TBK_E_UNSUPPORTED, "paired stacks need the fused row kernel");
     const int64_t K = rows * Rp;
-    auto Lk = [&](int64_t k) { return Ls + k * K; };
-    auto Rk = [&](int64_t k) { return Rs + k * K; };
+    auto Lk = [&](int64_t k) { return Ls + k * ldk; };
+    auto Rk = [&](int64_t k) { return Rs + k * ldk; };
     auto X = [&](int64_t k) { return (k & 1) ? X1 : X0; };
     tbk::launch_split_block(static_cast<const double2*>(kets), Lk(0), K, st);
     HIPC(hipGetLastError());
@@ -682,9 +691,9 @@ void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a
 }
 
 void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
-                                  double2* P, double2* out, hipStream_t st) {
+                                  int64_t ldk, double2* P, double2* out, hipStream_t st) {
     if (!cs.big) raise(TBK_E_UNSUPPORTED, "pre-split contraction needs N >= 256");
-    tbk::launch_gamma_bf16x3_big_presplit(A, B, N, K, cs.kc, cs.map, P, st);
+    tbk::launch_gamma_bf16x3_big_presplit(A, B, N, K, ldk, cs.kc, cs.map, P, st);
     HIPC(hipGetLastError());
     tbk::launch_gamma_reduce(P, cs.nchunks, static_cast<int64_t>(N) * N, out, st);
     HIPC(hipGetLastError());
@@ -736,8 +745,8 @@ double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk
 namespace {
 
 // gamma = A^H B on the device (A, B: N x K complex128), into host gamma_out.
-double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int N, int64_t K, int prec,
-                       double* gamma_out, int* nchunks_out) {
+double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int N, int64_t K, int64_t ldk,
+                       int prec, double* gamma_out, int* nchunks_out) {
     const ContractionShape cs = contraction_plan(ctx, N, K, prec);
     const int64_t NN = static_cast<int64_t>(N) * N;
     DevBuf& P = ctx->gamma_part;
@@ -746,7 +755,7 @@ double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int
     out.reserve(static_cast<size_t>(NN) * sizeof(double2));
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    enqueue_contraction(cs, A, B, N, K, prec, P.as<double2>(), out.as<double2>(), ctx->stream);
+    enqueue_contraction(cs, A, B, N, K, ldk, prec, P.as<double2>(), out.as<double2>(), ctx->stream);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipMemcpyAsync(gamma_out, out.p, static_cast<size_t>(NN) * sizeof(double2), hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -758,7 +767,7 @@ double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int
 }
 
 // gamma = A^H B over pre-split stacks (N x K words each), into host gamma_out.
-double contract_stacks_presplit(tbk_context* ctx, const uint2* A, const uint2* B, int N, int64_t K,
+double contract_stacks_presplit(tbk_context* ctx, const uint2* A, const uint2* B, int N, int64_t K, int64_t ldk,
                                 double* gamma_out, int* nchunks_out) {
     const ContractionShape cs = contraction_plan(ctx, N, K, TBK_GAMMA_BF16X3);
     const int64_t NN = static_cast<int64_t>(N) * N;
@@ -768,7 +777,7 @@ double contract_stacks_presplit(tbk_context* ctx, const uint2* A, const uint2* B
     out.reserve(static_cast<size_t>(NN) * sizeof(double2));
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    enqueue_contraction_presplit(cs, A, B, N, K, P.as<double2>(), out.as<double2>(), ctx->stream);
+    enqueue_contraction_presplit(cs, A, B, N, K, ldk, P.as<double2>(), out.as<double2>(), ctx->stream);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipMemcpyAsync(gamma_out, out.p, static_cast<size_t>(NN) * sizeof(double2), hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -810,7 +819,8 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         pP = make_plan(ctx, H, TBK_FP64, 2 * Rp, rows, b, true, tbk_tuning{});
         paired = pP.kc.kch == 0;
     }
-    const size_t entry_bytes = presplit ? blk_bytes / 2 : blk_bytes;  // one stack entry
+    const int64_t ldk = stack_stride(K);
+    const size_t entry_bytes = static_cast<size_t>(ldk) * (presplit ? sizeof(uint2) : sizeof(double2));
     const size_t stack_bytes = static_cast<size_t>(N) * entry_bytes;
     const int rolling = paired ? 7 : presplit ? 5 : 2;  // fp64 work blocks
     DevBuf& L = ctx->stack[0];
@@ -844,14 +854,15 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
     if (paired)
         enqueue_kubo_stacks_paired(pP, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
-                                   ctx->kwork[1].p, ctx->kwork[2].p, rows, Rp, N, part, ctx->stream, vx->val64.p);
+                                   ctx->kwork[1].p, ctx->kwork[2].p, rows, Rp, ldk, N, part, ctx->stream,
+                                   vx->val64.p);
     else if (presplit)
         enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
-                                     ctx->kwork[1].p, ctx->kwork[2].p, ctx->work[0].p, ctx->work[1].p, K, N, part,
-                                     ctx->stream, fuse ? vx->val64.p : nullptr);
+                                     ctx->kwork[1].p, ctx->kwork[2].p, ctx->work[0].p, ctx->work[1].p, K, ldk, N,
+                                     part, ctx->stream, fuse ? vx->val64.p : nullptr);
     else
-        enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, N,
-                            part, ctx->stream, fuse ? vx->val64.p : nullptr);
+        enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, ldk,
+                            N, part, ctx->stream, fuse ? vx->val64.p : nullptr);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
     float rec_ms = 0.f;
@@ -859,9 +870,10 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
 
     int nchunks = 0;
     const double con_ms = presplit ? contract_stacks_presplit(ctx, L.as<const uint2>(), Rs.as<const uint2>(), N, K,
-                                                              gamma_out, &nchunks)
+                                                              ldk, gamma_out, &nchunks)
                                    : contract_stacks(ctx, static_cast<const double2*>(L.p),
-                                                     static_cast<const double2*>(Rs.p), N, K, prec, gamma_out, &nchunks);
+                                                     static_cast<const double2*>(Rs.p), N, K, ldk, prec, gamma_out,
+                                                     &nchunks);
     if (g) {
         for (int64_t n = 0; n < N; ++n)
             for (int64_t m = 0; m < N; ++m) {
@@ -874,7 +886,9 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         stats->recursion_ms = rec_ms;
         stats->contraction_ms = con_ms;
         stats->flops = 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
-        stats->stack_bytes = 2.0 * static_cast<double>(stack_bytes);
+        // the stacks' entries, without the stride pad
+        stats->stack_bytes = 2.0 * static_cast<double>(N) * static_cast<double>(K) *
+                             static_cast<double>(presplit ? sizeof(uint2) : sizeof(double2));
         stats->nchunks = nchunks;
         const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N, fuse, presplit, paired);
         stats->recursion_bytes = bytes;
@@ -1341,7 +1355,7 @@ int tbk_stack_contract(tbk_context* ctx, const double* A, const double* B, int32
         dB.reserve(bytes);
         HIPC(hipMemcpy(dA.p, A, bytes, hipMemcpyHostToDevice));
         HIPC(hipMemcpy(dB.p, B, bytes, hipMemcpyHostToDevice));
-        const double ms = contract_stacks(ctx, dA.as<const double2>(), dB.as<const double2>(), N, K,
+        const double ms = contract_stacks(ctx, dA.as<const double2>(), dB.as<const double2>(), N, K, K,
                                           gamma_precision, gamma_out, nullptr);
         if (contraction_ms) *contraction_ms = ms;
     });
