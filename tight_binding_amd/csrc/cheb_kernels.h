@@ -297,7 +297,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
                                          VT alpha, VT beta, int32_t opts, double& p0, double& p1,
                                          const typename HVal<VT, CH>::type* __restrict__ hv2 = nullptr,
                                          typename Vec2<VT>::type* y2 = nullptr,
-                                         typename Vec2<VT>::type* y3 = nullptr) {
+                                         typename Vec2<VT>::type* y3 = nullptr, int64_t wbs = 0) {
     using V2 = typename Vec2<VT>::type;
     using H = typename HVal<VT, CH>::type;
     // the update and moment form of the mode, and its extra outputs
@@ -400,7 +400,8 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
     // split words of the pre-split stacks: pairs of complex values as one
     // 16-byte store (own is even when CPL is)
     if constexpr (SPLIT_Y || MODE == kDualSplit) {
-        uint2* ws = reinterpret_cast<uint2*>(y2) + own;
+        // own is even when CPL is, and pairs never straddle a word block
+        uint2* ws = reinterpret_cast<uint2*>(y2) + word_offset(static_cast<int64_t>(own), wbs);
         if constexpr (CPL == 1 && LPR >= 2) {
             // one value per lane: row neighbours' words in one 16-byte store
             // (own is even for even q: Rp = LPR is even)
@@ -425,8 +426,10 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         const int half = Rp >> 1;
         const int col = q * CPL;
         const bool left = col < half;
-        uint2* ws = left ? reinterpret_cast<uint2*>(y2) + static_cast<size_t>(r) * half + col
-                         : reinterpret_cast<uint2*>(y3) + static_cast<size_t>(r) * half + (col - half);
+        // the last step's left output is no stack entry: y2 null, not stored
+        if (left && y2 == nullptr) return;
+        const int64_t k = static_cast<int64_t>(r) * half + (left ? col : col - half);
+        uint2* ws = (left ? reinterpret_cast<uint2*>(y2) : reinterpret_cast<uint2*>(y3)) + word_offset(k, wbs);
         if constexpr (CPL == 1) {
             if ((half & 1) == 0) {  // even halves: neighbour lanes' words in one 16-byte store
                 st_split_lanes(ws, left ? sv[0] : acc2[0], (q & 1) == 0);
@@ -455,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     const typename Vec2<VT>::type* __restrict__ bra, int32_t rows, int32_t Rp, VT alpha, VT beta,
     int32_t opts, int64_t ntiles, TileOrder order, int32_t xcd_split, double2* __restrict__ partial,
     const typename HVal<VT, CH>::type* __restrict__ hv2, typename Vec2<VT>::type* y2,
-    typename Vec2<VT>::type* y3) {
+    typename Vec2<VT>::type* y3, int64_t wbs) {
     constexpr int RPW = kWave / LPR;
     constexpr int TR = kWavesPerBlock * RPW;
     static_assert(LPR >= 1 && LPR <= kWave && (kWave % LPR) == 0, "bad lanes per row");
@@ -477,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
         if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
         if (r >= rows) continue;
         step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q, Rp, alpha, beta, opts, p0, p1, hv2, y2,
-                                         y3);
+                                         y3, wbs);
     }
 
     // deterministic block reduction: butterfly in the wave, then waves in order

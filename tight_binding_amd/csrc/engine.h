@@ -187,8 +187,11 @@ StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int 
                    const tbk_tuning& tu);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2);
+// y3: the paired modes' right-half words; wbs: block stride of k-blocked
+// word stacks (word_offset)
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
-                 const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2, void* y3);
+                 const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2, void* y3,
+                 int64_t wbs);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial);
 
@@ -245,16 +248,16 @@ bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& p
 bool kubo_presplit(int prec, int N, bool h_complex, bool vx_complex);
 void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
                                   const void* kets, uint2* Ls, uint2* Rs, void* B0, void* B1, void* S, void* W0,
-                                  void* W1, int64_t K, int64_t ldk, int N, double2* part, hipStream_t st,
-                                  const void* vx_values);
+                                  void* W1, int64_t K, int N, double2* part, hipStream_t st, const void* vx_values);
+// A, B: k-blocked word stacks of N entries (word_offset, block stride N * kWordBlock)
 void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
-                                  int64_t ldk, double2* P, double2* out, hipStream_t st);
+                                  double2* P, double2* out, hipStream_t st);
 // pre-split stacks with both chains in one block of 2 Rp columns (fused v_x
 // only; pP: H's plan at 2 Rp columns, row kernel); X0, X1: 2 blocks each, S:
 // one block of scratch
 void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a, double b, const void* kets,
-                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int64_t ldk,
-                                int N, double2* part, hipStream_t st, const void* vx_values);
+                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int N,
+                                double2* part, hipStream_t st, const void* vx_values);
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
                             int64_t N, bool fused, bool presplit, bool paired = false);
 

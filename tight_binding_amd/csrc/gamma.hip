@@ -409,12 +409,14 @@ __device__ __forceinline__ void ld_pair(const double2* p, bool, double2& x0, dou
 __device__ __forceinline__ uint32_t re_pair(uint32_t w0, uint32_t w1) { return (w0 & 0xffffu) | (w1 << 16); }
 __device__ __forceinline__ uint32_t im_pair(uint32_t w0, uint32_t w1) { return (w0 >> 16) | (w1 & 0xffff0000u); }
 
-// T = double2: fp64 stacks, split at staging; T = uint2: stacks written
-// pre-split by the recursion (the same words), half the bytes to stage
+// T = double2: fp64 stacks, split at staging, row n at n * ldk; T = uint2:
+// stacks written pre-split by the recursion (the same words), half the bytes
+// to stage, k-blocked with block stride wbs (word_offset; wbs = 0: rows at
+// n * ldk)
 template <typename T>
 __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restrict__ A, const T* __restrict__ B,
-                                                               int N, int64_t K, int64_t ldk, int64_t kc, GammaMap map,
-                                                               double2* __restrict__ P) {
+                                                               int N, int64_t K, int64_t ldk, int64_t wbs, int64_t kc,
+                                                               GammaMap map, double2* __restrict__ P) {
     __shared__ __align__(16) uint32_t As[2][kBigTile * kSplitRowWords];
     __shared__ __align__(16) uint32_t Bs[2][kBigTile * kSplitRowWords];
     int tn, tm;
@@ -444,26 +446,30 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
         const int r = r0 + 64 * i;
         const int rn = (n0 + r < N) ? n0 + r : N - 1;
         const int rm = (m0 + r < N) ? m0 + r : N - 1;
-        ga[i] = A + static_cast<int64_t>(rn) * ldk;
-        gb[i] = B + static_cast<int64_t>(rm) * ldk;
+        const int64_t row_step = wbs ? kWordBlock : ldk;
+        ga[i] = A + static_cast<int64_t>(rn) * row_step;
+        gb[i] = B + static_cast<int64_t>(rm) * row_step;
     }
     T ra[2][2], rb[2][2];
-    const bool rows16 = (ldk & 1) == 0;  // row starts, and so column pairs, 16-byte aligned
+    // row starts, and so column pairs, 16-byte aligned (blocked: always)
+    const bool rows16 = wbs != 0 || (ldk & 1) == 0;
     auto gload = [&](int64_t k0) {
         const int64_t e = k0 + 2 * p;
+        // a stage of 16 columns lies within one word block
+        const int64_t o = word_offset(k0, wbs) + 2 * p;
         if (k0 + kSplitKt <= ke) {  // a full stage (all but a row's last, short one): plain loads
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                ld_pair(ga[i] + e, rows16, ra[i][0], ra[i][1]);
-                ld_pair(gb[i] + e, rows16, rb[i][0], rb[i][1]);
+                ld_pair(ga[i] + o, rows16, ra[i][0], ra[i][1]);
+                ld_pair(gb[i] + o, rows16, rb[i][0], rb[i][1]);
             }
         } else {
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                ra[i][0] = ld_or_zero(ga[i] + e, e < ke);
-                ra[i][1] = ld_or_zero(ga[i] + e + 1, e + 1 < ke);
-                rb[i][0] = ld_or_zero(gb[i] + e, e < ke);
-                rb[i][1] = ld_or_zero(gb[i] + e + 1, e + 1 < ke);
+                ra[i][0] = ld_or_zero(ga[i] + o, e < ke);
+                ra[i][1] = ld_or_zero(ga[i] + o + 1, e + 1 < ke);
+                rb[i][0] = ld_or_zero(gb[i] + o, e < ke);
+                rb[i][1] = ld_or_zero(gb[i] + o + 1, e + 1 < ke);
             }
         }
     };
@@ -721,27 +727,27 @@ void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, 
 void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc,
                              const GammaMap& map, double2* P, hipStream_t st) {
     hipLaunchKernelGGL(gamma_bf16x3_big_kernel<double2>, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0,
-                       st, A, B, N, K, ldk, kc, map, P);
+                       st, A, B, N, K, ldk, int64_t{0}, kc, map, P);
 }
 
-void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t ldk, int64_t kc,
+void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t wbs, int64_t kc,
                                       const GammaMap& map, double2* P, hipStream_t st) {
     hipLaunchKernelGGL(gamma_bf16x3_big_kernel<uint2>, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0,
-                       st, A, B, N, K, ldk, kc, map, P);
+                       st, A, B, N, K, K, wbs, kc, map, P);
 }
 
 namespace {
 __global__ __launch_bounds__(256) void split_block_kernel(const double2* __restrict__ in, uint2* __restrict__ out,
-                                                          int64_t n) {
+                                                          int64_t n, int64_t bs) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * 256)
-        out[i] = split_word(in[i]);
+        out[word_offset(i, bs)] = split_word(in[i]);
 }
 }  // namespace
 
-void launch_split_block(const double2* in, uint2* out, int64_t n, hipStream_t st) {
+void launch_split_block(const double2* in, uint2* out, int64_t n, int64_t bs, hipStream_t st) {
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 65536));
-    hipLaunchKernelGGL(split_block_kernel, dim3(grid), dim3(256), 0, st, in, out, n);
+    hipLaunchKernelGGL(split_block_kernel, dim3(grid), dim3(256), 0, st, in, out, n, bs);
 }
 
 void launch_gamma_weight(double2* G, const double* g, int N, hipStream_t st) {

@@ -45,12 +45,13 @@ void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t 
 
 // The same contraction over stacks written pre-split by the recursion: each
 // complex element is its (hi, lo) words (split_bf16.h), 8 bytes instead of 16,
-// so the partial sums equal launch_gamma_bf16x3_big's on the fp64 stacks.
-void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t ldk, int64_t kc,
+// so the partial sums equal launch_gamma_bf16x3_big's on the fp64 stacks.  The
+// stacks are k-blocked with block stride wbs = N * kWordBlock (word_offset).
+void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t wbs, int64_t kc,
                                       const GammaMap& map, double2* P, hipStream_t st);
 
-// out[i] = split words of in[i] (n complex values)
-void launch_split_block(const double2* in, uint2* out, int64_t n, hipStream_t st);
+// out[word_offset(i, bs)] = split words of in[i] (n complex values)
+void launch_split_block(const double2* in, uint2* out, int64_t n, int64_t bs, hipStream_t st);
 
 // out[i] = sum_{c < nchunks} P[c*NN + i], chunks in order.
 void launch_gamma_reduce(const double2* P, int nchunks, int64_t NN, double2* out, hipStream_t st);

@@ -283,7 +283,9 @@ int64_t tbk_server::kubo_footprint(const Request& r) const {
     const ContractionShape cs = contraction_plan(ctx, r.n, rows * r.Rp, r.precision);
     // pre-split stacks: half-size entries and three more rolling blocks
     const bool ps = kubo_presplit(r.precision, r.n, r.H.hdr.is_complex != 0, r.vx.hdr.is_complex != 0);
-    const int64_t entry = stack_stride(rows * r.Rp) * (ps ? 8 : 16);  // one padded stack entry
+    // one stack entry: fp64 at the stack stride, words in whole word blocks
+    const int64_t K = rows * r.Rp;
+    const int64_t entry = ps ? ((K + tbk::kWordBlock - 1) / tbk::kWordBlock) * tbk::kWordBlock * 8 : stack_stride(K) * 16;
     const int64_t stacks = 2 * N * entry, rolling = ps ? 5 * blk : 2 * blk;
     // a lattice request also holds the assembled H while H~ is formed
     // (H~ has the size of a velocity operator: one more entry per row)
@@ -477,8 +479,10 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
         pP = make_plan(ctx, j->H, TBK_FP64, 2 * Rp, rows, b, true, tbk_tuning{});
         paired = pP.kc.kch == 0;
     }
+    // fp64 stacks at a stride of ldk elements; word stacks k-blocked
     const int64_t ldk = stack_stride(K);
-    const size_t entry = static_cast<size_t>(ldk) * (presplit ? sizeof(uint2) : sizeof(double2));
+    const int64_t kw = ((K + tbk::kWordBlock - 1) / tbk::kWordBlock) * tbk::kWordBlock;
+    const size_t entry = presplit ? static_cast<size_t>(kw) * sizeof(uint2) : static_cast<size_t>(ldk) * sizeof(double2);
     j->kets.reserve(blk);
     j->L.reserve(static_cast<size_t>(N) * entry);
     j->Rs.reserve(static_cast<size_t>(N) * entry);
@@ -517,14 +521,14 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     HIPC(hipGetLastError());
     if (paired) {
         enqueue_kubo_stacks_paired(pP, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->W0.p, j->W1.p,
-                                   j->S.p, rows, Rp, ldk, N, j->part.as<double2>(), st_, vxv);
-        enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, ldk,
-                                     j->P.as<double2>(), j->G.as<double2>(), st_);
+                                   j->S.p, rows, Rp, N, j->part.as<double2>(), st_, vxv);
+        enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, j->P.as<double2>(),
+                                     j->G.as<double2>(), st_);
     } else if (presplit) {
         enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->B0.p,
-                                     j->B1.p, j->S.p, j->W0.p, j->W1.p, K, ldk, N, j->part.as<double2>(), st_, vxv);
-        enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, ldk,
-                                     j->P.as<double2>(), j->G.as<double2>(), st_);
+                                     j->B1.p, j->S.p, j->W0.p, j->W1.p, K, N, j->part.as<double2>(), st_, vxv);
+        enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, j->P.as<double2>(),
+                                     j->G.as<double2>(), st_);
     } else {
         enqueue_kubo_stacks(pH, pX, pY, a, b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, ldk, N,
                             j->part.as<double2>(), st_, vxv);

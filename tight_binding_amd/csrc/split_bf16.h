@@ -27,4 +27,16 @@ __device__ __forceinline__ uint2 split_word(double2 v) {
     return w;
 }
 
+// Layout of a stack of split words (N entries of K elements): k-blocked,
+// the kWordBlock-element pieces of all entries at one k side by side,
+//   word (n, k) at (k / kWordBlock) * bs + n * kWordBlock + k % kWordBlock,
+// bs = N * kWordBlock (the block stride).  The contraction reads 16 columns
+// of hundreds of entries at a time; blocked, those lie within a few pages
+// instead of one page per entry.  An entry's own words start at
+// n * kWordBlock; bs = 0 means a plain contiguous entry.
+constexpr int kWordBlock = 256;
+__host__ __device__ __forceinline__ int64_t word_offset(int64_t k, int64_t bs) {
+    return bs ? (k / kWordBlock) * bs + (k % kWordBlock) : k;
+}
+
 }  // namespace tbk

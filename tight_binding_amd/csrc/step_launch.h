@@ -28,6 +28,7 @@ struct StepArgs {
     const void* hv2 = nullptr;  // kDual: values of the second matrix (same pattern)
     void* y2 = nullptr;         // kDual: its product with x
     void* y3 = nullptr;         // kPairFirst, kPairRec: the right half's split words
+    int64_t wbs = 0;            // split words: block stride of a k-blocked stack (word_offset), 0 contiguous
 };
 
 // Arguments of cheb_slots_kernel (one step of every slot of a class; the
@@ -94,7 +95,8 @@ bool launch_step(dim3 grid, hipStream_t st, const StepArgs& a) {
                            static_cast<const H*>(a.hv), static_cast<const V2*>(a.x), static_cast<const V2*>(a.z),
                            static_cast<V2*>(a.y), static_cast<const V2*>(a.bra), a.rows, a.Rp,
                            static_cast<VT>(a.alpha), static_cast<VT>(a.beta), a.opts, a.ntiles, a.order, a.xcd_split,
-                           a.partial, static_cast<const H*>(a.hv2), static_cast<V2*>(a.y2), static_cast<V2*>(a.y3));
+                           a.partial, static_cast<const H*>(a.hv2), static_cast<V2*>(a.y2), static_cast<V2*>(a.y3),
+                           a.wbs);
         return true;
     } else if constexpr (MODE >= kDual) {
         return false;  // the dual product and split outputs exist for the row kernel only

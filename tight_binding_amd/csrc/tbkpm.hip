@@ -267,11 +267,12 @@ void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, dou
 
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2) {
-    launch_plan(pl, st, mode, alpha, beta, x, z, y, bra, partial, hv2, y2, nullptr);
+    launch_plan(pl, st, mode, alpha, beta, x, z, y, bra, partial, hv2, y2, nullptr, 0);
 }
 
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
-                 const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2, void* y3) {
+                 const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2, void* y3,
+                 int64_t wbs) {
     StepArgs A = pl.args;
     A.alpha = alpha;
     A.beta = beta;
@@ -283,6 +284,7 @@ void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, dou
     A.hv2 = hv2;
     A.y2 = y2;
     A.y3 = y3;
+    A.wbs = wbs;
     launch_any(pl.precision, pl.cplx, pl.kc.width, pl.kc.kch, pl.kc.cpl, mode, dim3(static_cast<unsigned>(pl.nb)), st,
                A);
     HIPC(hipGetLastError());
@@ -623,38 +625,43 @@ bool kubo_presplit(int prec, int N, bool h_complex, bool vx_complex) {
 // elements per block.
 void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
                                   const void* kets, uint2* Ls, uint2* Rs, void* B0, void* B1, void* S, void* W0,
-                                  void* W1, int64_t K, int64_t ldk, int N, double2* part, hipStream_t st,
-                                  const void* vx_values) {
+                                  void* W1, int64_t K, int N, double2* part, hipStream_t st, const void* vx_values) {
     if (pH.kc.kch != 0 || pX.kc.kch != 0 || !pH.cplx || !pX.cplx)
         raise(TBK_E_UNSUPPORTED, "pre-split stacks need the complex row kernel");
-    auto Lk = [&](int64_t k) { return Ls + k * ldk; };
-    auto Rk = [&](int64_t k) { return Rs + k * ldk; };
+    // k-blocked word stacks (word_offset): entry k's words start at k * kWordBlock
+    const int64_t bs = static_cast<int64_t>(N) * tbk::kWordBlock;
+    auto Lk = [&](int64_t k) { return Ls + k * tbk::kWordBlock; };
+    auto Rk = [&](int64_t k) { return Rs + k * tbk::kWordBlock; };
     auto B = [&](int64_t k) { return (k & 1) ? B1 : B0; };
     auto W = [&](int64_t k) { return (k & 1) ? W1 : W0; };
     HIPC(hipMemcpyAsync(B0, kets, static_cast<size_t>(K) * sizeof(double2), hipMemcpyDeviceToDevice, st));
-    tbk::launch_split_block(static_cast<const double2*>(kets), Lk(0), K, st);
+    tbk::launch_split_block(static_cast<const double2*>(kets), Lk(0), K, bs, st);
     HIPC(hipGetLastError());
-    if (N > 1) launch_plan(pH, st, tbk::kFirstSplit, 1.0 / a, -b / a, B(0), nullptr, B(1), B(0), part, nullptr, Lk(1));
+    if (N > 1)
+        launch_plan(pH, st, tbk::kFirstSplit, 1.0 / a, -b / a, B(0), nullptr, B(1), B(0), part, nullptr, Lk(1),
+                    nullptr, bs);
     for (int64_t n = 1; n + 1 < N; ++n)  // T_{n+1} over T_{n-1} in place
         launch_plan(pH, st, tbk::kRecDblSplit, 2.0 / a, -2.0 * b / a, B(n), B(n + 1), B(n + 1), nullptr, part, nullptr,
-                    Lk(n + 1));
+                    Lk(n + 1), nullptr, bs);
     launch_plan(pY, st, tbk::kFirst, 1.0, 0.0, kets, nullptr, W(0), kets, part);
-    launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(0), nullptr, S, W(0), part, nullptr, Rk(0));
+    launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(0), nullptr, S, W(0), part, nullptr, Rk(0), nullptr, bs);
     if (N > 1) {
         launch_plan(pH, st, tbk::kFirst, 1.0 / a, -b / a, W(0), nullptr, W(1), W(0), part);
-        if (!vx_values) launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(1), nullptr, S, W(1), part, nullptr, Rk(1));
+        if (!vx_values)
+            launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(1), nullptr, S, W(1), part, nullptr, Rk(1), nullptr, bs);
     }
     for (int64_t m = 2; m < N; ++m) {
         if (vx_values) {
             launch_plan(pH, st, tbk::kDualSplit, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part, vx_values,
-                        Rk(m - 1));
+                        Rk(m - 1), nullptr, bs);
         } else {
             launch_plan(pH, st, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part);
-            launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(m), nullptr, S, W(m), part, nullptr, Rk(m));
+            launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(m), nullptr, S, W(m), part, nullptr, Rk(m), nullptr, bs);
         }
     }
     if (vx_values && N > 1)
-        launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(N - 1), nullptr, S, W(N - 1), part, nullptr, Rk(N - 1));
+        launch_plan(pX, st, tbk::kFirstSplit, 1.0, 0.0, W(N - 1), nullptr, S, W(N - 1), part, nullptr, Rk(N - 1),
+                    nullptr, bs);
 }
 
 // Both chains of the pre-split stacks as one block of 2 Rp columns, the left
@@ -665,16 +672,18 @@ void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const 
 // once per index for both chains, one launch instead of two.  The words are
 // enqueue_kubo_stacks_presplit's with the fused v_x product, bit for bit
 // (the same sums per lane).  The last step's left output is not a stack
-// entry and goes to S.
+// entry and is not stored (null target).
 void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a, double b, const void* kets,
-                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int64_t ldk,
-                                int N, double2* part, hipStream_t st, const void* vx_values) {
+                                uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int N,
+                                double2* part, hipStream_t st, const void* vx_values) {
     if (pP.kc.kch != 0 || !pP.cplx || !vx_values) raise(TBK_E_UNSUPPORTED, "paired stacks need the fused row kernel");
     const int64_t K = rows * Rp;
-    auto Lk = [&](int64_t k) { return Ls + k * ldk; };
-    auto Rk = [&](int64_t k) { return Rs + k * ldk; };
+    // k-blocked word stacks (word_offset): entry k's words start at k * kWordBlock
+    const int64_t bs = static_cast<int64_t>(N) * tbk::kWordBlock;
+    auto Lk = [&](int64_t k) { return Ls + k * tbk::kWordBlock; };
+    auto Rk = [&](int64_t k) { return Rs + k * tbk::kWordBlock; };
     auto X = [&](int64_t k) { return (k & 1) ? X1 : X0; };
-    tbk::launch_split_block(static_cast<const double2*>(kets), Lk(0), K, st);
+    tbk::launch_split_block(static_cast<const double2*>(kets), Lk(0), K, bs, st);
     HIPC(hipGetLastError());
     launch_plan(pY, st, tbk::kFirst, 1.0, 0.0, kets, nullptr, S, kets, part);  // v_y |r>
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>((2 * K + tbk::kBlock - 1) / tbk::kBlock, 65536));
@@ -682,18 +691,18 @@ void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a
                        static_cast<const double2*>(kets), static_cast<const double2*>(S), static_cast<double2*>(X0),
                        rows, Rp);
     HIPC(hipGetLastError());
-    uint2* spare = static_cast<uint2*>(S);
     launch_plan(pP, st, tbk::kPairFirst, 1.0 / a, -b / a, X(0), nullptr, X(1), X(0), part, vx_values,
-                N > 1 ? Lk(1) : spare, Rk(0));
+                N > 1 ? Lk(1) : nullptr, Rk(0), bs);
     for (int64_t k = 1; k < N; ++k)  // X_{k+1} over X_{k-1}
         launch_plan(pP, st, tbk::kPairRec, 2.0 / a, -2.0 * b / a, X(k), X(k - 1), X(k + 1), nullptr, part, vx_values,
-                    k + 1 < N ? Lk(k + 1) : spare, Rk(k));
+                    k + 1 < N ? Lk(k + 1) : nullptr, Rk(k), bs);
 }
 
 void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
-                                  int64_t ldk, double2* P, double2* out, hipStream_t st) {
+                                  double2* P, double2* out, hipStream_t st) {
     if (!cs.big) raise(TBK_E_UNSUPPORTED, "pre-split contraction needs N >= 256");
-    tbk::launch_gamma_bf16x3_big_presplit(A, B, N, K, ldk, cs.kc, cs.map, P, st);
+    tbk::launch_gamma_bf16x3_big_presplit(A, B, N, K, static_cast<int64_t>(N) * tbk::kWordBlock, cs.kc, cs.map, P,
+                                          st);
     HIPC(hipGetLastError());
     tbk::launch_gamma_reduce(P, cs.nchunks, static_cast<int64_t>(N) * N, out, st);
     HIPC(hipGetLastError());
@@ -767,7 +776,7 @@ double contract_stacks(tbk_context* ctx, const double2* A, const double2* B, int
 }
 
 // gamma = A^H B over pre-split stacks (N x K words each), into host gamma_out.
-double contract_stacks_presplit(tbk_context* ctx, const uint2* A, const uint2* B, int N, int64_t K, int64_t ldk,
+double contract_stacks_presplit(tbk_context* ctx, const uint2* A, const uint2* B, int N, int64_t K,
                                 double* gamma_out, int* nchunks_out) {
     const ContractionShape cs = contraction_plan(ctx, N, K, TBK_GAMMA_BF16X3);
     const int64_t NN = static_cast<int64_t>(N) * N;
@@ -777,7 +786,7 @@ double contract_stacks_presplit(tbk_context* ctx, const uint2* A, const uint2* B
     out.reserve(static_cast<size_t>(NN) * sizeof(double2));
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    enqueue_contraction_presplit(cs, A, B, N, K, ldk, P.as<double2>(), out.as<double2>(), ctx->stream);
+    enqueue_contraction_presplit(cs, A, B, N, K, P.as<double2>(), out.as<double2>(), ctx->stream);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipMemcpyAsync(gamma_out, out.p, static_cast<size_t>(NN) * sizeof(double2), hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -819,8 +828,11 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         pP = make_plan(ctx, H, TBK_FP64, 2 * Rp, rows, b, true, tbk_tuning{});
         paired = pP.kc.kch == 0;
     }
+    // fp64 stacks: entries at a stride of ldk elements; word stacks: k-blocked
+    // (word_offset), each entry's share rounded up to whole word blocks
     const int64_t ldk = stack_stride(K);
-    const size_t entry_bytes = static_cast<size_t>(ldk) * (presplit ? sizeof(uint2) : sizeof(double2));
+    const int64_t kw = ((K + tbk::kWordBlock - 1) / tbk::kWordBlock) * tbk::kWordBlock;
+    const size_t entry_bytes = presplit ? static_cast<size_t>(kw) * sizeof(uint2) : static_cast<size_t>(ldk) * sizeof(double2);
     const size_t stack_bytes = static_cast<size_t>(N) * entry_bytes;
     const int rolling = paired ? 7 : presplit ? 5 : 2;  // fp64 work blocks
     DevBuf& L = ctx->stack[0];
@@ -854,12 +866,11 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
     if (paired)
         enqueue_kubo_stacks_paired(pP, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
-                                   ctx->kwork[1].p, ctx->kwork[2].p, rows, Rp, ldk, N, part, ctx->stream,
-                                   vx->val64.p);
+                                   ctx->kwork[1].p, ctx->kwork[2].p, rows, Rp, N, part, ctx->stream, vx->val64.p);
     else if (presplit)
         enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
-                                     ctx->kwork[1].p, ctx->kwork[2].p, ctx->work[0].p, ctx->work[1].p, K, ldk, N,
-                                     part, ctx->stream, fuse ? vx->val64.p : nullptr);
+                                     ctx->kwork[1].p, ctx->kwork[2].p, ctx->work[0].p, ctx->work[1].p, K, N, part,
+                                     ctx->stream, fuse ? vx->val64.p : nullptr);
     else
         enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, ldk,
                             N, part, ctx->stream, fuse ? vx->val64.p : nullptr);
@@ -870,7 +881,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
 
     int nchunks = 0;
     const double con_ms = presplit ? contract_stacks_presplit(ctx, L.as<const uint2>(), Rs.as<const uint2>(), N, K,
-                                                              ldk, gamma_out, &nchunks)
+                                                              gamma_out, &nchunks)
                                    : contract_stacks(ctx, static_cast<const double2*>(L.p),
                                                      static_cast<const double2*>(Rs.p), N, K, ldk, prec, gamma_out,
                                                      &nchunks);
