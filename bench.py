@@ -30,6 +30,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per step of the default workload (1024 x 1024 cells, R = 64),
+# measured with rocprofv3 --pmc over the 16 timed steps of a short run of this
+# script: TCC_EA0_RDREQ_DRAM_32B and TCC_EA0_WRREQ_WRITE_DRAM_32B count
+# 32-byte sectors and read 1.000x the traffic of stream kernels of known size
+# (profiles/cheb_step_fp64_honeycomb2M_R64_pmc_counters.md).  The figure is
+# for the launch shape the tuner picked in that run; other shapes read
+# 4.4-7.3 GB per step.
+STEP_TRAFFIC_BYTES = {(1024, 64): 7.5821e9}
+
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
@@ -787,7 +796,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": STEP_TRAFFIC_BYTES.get((args.cells, args.R)),
+            "traffic_source": ("rocprofv3 --pmc DRAM sector counters, median of 16 timed steps of the same workload"
+                               if (args.cells, args.R) in STEP_TRAFFIC_BYTES else None),
             "bytes_per_step": st["bytes_per_step"],
         },
         "cpu_baseline": cpu,
