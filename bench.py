@@ -642,8 +642,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(sharding over R vectors is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (a multi-GPU run "
+              "would give each GPU R/G of the vectors and all-reduce the moments)", file=sys.stderr)
         sys.exit(2)
 
     from tight_binding_amd import kpm
