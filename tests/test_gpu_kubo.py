@@ -226,15 +226,15 @@ def test_fused_velocity_stacks_are_the_same_bits(gpu_ctx, N):
     K.close()
 
 
-@pytest.mark.parametrize("R,lane_bytes", [(2, 0), (4, 0), (8, 32), (32, 32)])
+@pytest.mark.parametrize("R,lane_bytes", [(2, 16), (2, 0), (4, 32), (8, 16), (8, 0), (32, 16), (32, 32)])
 def test_presplit_stacks_give_the_bf16x3_gamma_of_fp64_stacks(gpu_ctx, R, lane_bytes):
     """Split-BF16x3 solves with N >= 256 on complex operators write their
     stacks pre-split (the recursion stores the hi/lo bf16 words the
     contraction would form).  Gamma must be the same bits as the BF16x3
     contraction of the fp64 stacks -- here the oracle's, which the device's
     equal bit for bit at a = 1, b = 0 -- with v_x fused (both chains then run
-    as one block of 2 Rp columns) or not, at one or two complex values per
-    lane."""
+    as one block of 2 Rp columns) or not, at one (16-byte lanes) or two
+    complex values per lane (32-byte lanes, and the paired block's default)."""
     import scipy.sparse as sp
 
     from tight_binding_amd import sparse as SP

@@ -5,8 +5,9 @@
 //     acc  = sum_{p in row r, ascending column} H[p] * x[col[p]]
 //     y_r  = alpha*acc + beta*x_r - z_r          (three-term update)
 //     partial moments from y_r, x_r and (optionally) a bra row
-// A row is handled by LPR lanes (each lane owns CPL consecutive columns), so
-// the gather of a neighbour row is one coalesced LPR*CPL*16-byte access.
+// A row is handled by LPR lanes, each owning CPL columns in 16-byte units
+// interleaved across the lanes (lane_unit_elems), so the gather of a
+// neighbour row is one coalesced access per unit.
 // Workgroups take tiles of consecutive rows; with xcd_split the tiles are
 // dealt so that each XCD (blocks b with equal b % 8) walks one contiguous
 // eighth of the rows, keeping a lattice neighbourhood inside one L2.
@@ -285,6 +286,42 @@ __device__ __forceinline__ void tile_range(int64_t ntiles, int xcd_split, int64_
     }
 }
 
+// Columns of a row held by a lane.  A lane's CPL values form NU units of UE
+// consecutive values, one 16-byte access each (a single unit when the lane
+// holds at most 16 bytes); unit u of lane q starts at column
+// (q + u * LPR) * UE.  Each access instruction of a wave then covers whole
+// 16-byte-aligned runs of the row.  With a lane's units side by side instead,
+// every store instruction would write half of each 32-byte sector, and the
+// memory side sees about 1.5x the bytes.
+template <typename V2, int CPL>
+constexpr int lane_unit_elems() {
+    return CPL * static_cast<int>(sizeof(V2)) > 16 ? 16 / static_cast<int>(sizeof(V2)) : CPL;
+}
+// Offset of unit u from the lane's first unit.
+template <typename V2, int LPR, int CPL>
+constexpr int unit_step(int u) {
+    return u * LPR * lane_unit_elems<V2, CPL>();
+}
+// p: the lane's first unit in the row (row offset + q * UE)
+template <int LPR, int CPL, typename V2>
+__device__ __forceinline__ void ld_row(const V2* p, V2* out) {
+    constexpr int UE = lane_unit_elems<V2, CPL>();
+#pragma unroll
+    for (int u = 0; u < CPL / UE; ++u) ld_lane<UE>(p + unit_step<V2, LPR, CPL>(u), out + u * UE);
+}
+template <int LPR, int CPL, typename V2>
+__device__ __forceinline__ void ld_row_stream(const V2* p, V2* out, int opts) {
+    constexpr int UE = lane_unit_elems<V2, CPL>();
+#pragma unroll
+    for (int u = 0; u < CPL / UE; ++u) ld_lane_stream<UE>(p + unit_step<V2, LPR, CPL>(u), out + u * UE, opts);
+}
+template <int LPR, int CPL, typename V2>
+__device__ __forceinline__ void st_row_stream(V2* p, const V2* v, int opts) {
+    constexpr int UE = lane_unit_elems<V2, CPL>();
+#pragma unroll
+    for (int u = 0; u < CPL / UE; ++u) st_lane_stream<UE>(p + unit_step<V2, LPR, CPL>(u), v + u * UE, opts);
+}
+
 // One site row r of a step (q: this lane's column group): the neighbour
 // gathers, y_r = alpha*acc + beta*x_r - z_r, and the moment partials of the
 // mode added to p0, p1.
@@ -305,7 +342,9 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
     constexpr bool PAIR = MODE == kPairFirst || MODE == kPairRec;
     constexpr bool DUAL = MODE == kDual || MODE == kDualSplit || PAIR;
     constexpr bool SPLIT_Y = MODE == kFirstSplit || MODE == kRecDblSplit;
-    const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * CPL;
+    constexpr int UE = lane_unit_elems<V2, CPL>();
+    // this lane's first unit in row r
+    const size_t own = static_cast<size_t>(r) * Rp + static_cast<size_t>(q) * UE;
 
     V2 zv[CPL], bv[CPL], acc[CPL], acc2[CPL], xs[CPL], sv[CPL];
 #pragma unroll
@@ -317,8 +356,8 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         xs[c].x = 0;
         xs[c].y = 0;
     }
-    if (!FIRST) ld_lane_stream<CPL>(z + own, zv, opts);
-    if (FIRST || MODE == kRec) ld_lane_stream<CPL>(bra + own, bv, opts);
+    if (!FIRST) ld_row_stream<LPR, CPL>(z + own, zv, opts);
+    if (FIRST || MODE == kRec) ld_row_stream<LPR, CPL>(bra + own, bv, opts);
     const int s = rp[r], e = rp[r + 1];
     bool have_self = false;
     int p = s;
@@ -334,7 +373,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            ld_lane<CPL>(x + static_cast<size_t>(cc[k]) * Rp + static_cast<size_t>(q) * CPL, xv[k]);
+            ld_row<LPR, CPL>(x + static_cast<size_t>(cc[k]) * Rp + static_cast<size_t>(q) * UE, xv[k]);
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -355,7 +394,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         const int cc = ci[p];
         const H hh = hv[p];
         V2 xv[CPL];
-        ld_lane<CPL>(x + static_cast<size_t>(cc) * Rp + static_cast<size_t>(q) * CPL, xv);
+        ld_row<LPR, CPL>(x + static_cast<size_t>(cc) * Rp + static_cast<size_t>(q) * UE, xv);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) madd(acc[c], hh, xv[c]);
         if (DUAL) {
@@ -369,7 +408,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             for (int c = 0; c < CPL; ++c) xs[c] = xv[c];
         }
     }
-    if (!have_self) ld_lane<CPL>(x + own, xs);
+    if (!have_self) ld_row<LPR, CPL>(x + own, xs);
 
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
@@ -395,18 +434,25 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             p1 = p1 + redot(v, xs[c]);
         }
     }
-    st_lane_stream<CPL>(y + own, sv, opts);
-    if constexpr (MODE == kDual) st_lane_stream<CPL>(y2 + own, acc2, opts);
-    // split words of the pre-split stacks: pairs of complex values as one
-    // 16-byte store (own is even when CPL is)
+    st_row_stream<LPR, CPL>(y + own, sv, opts);
+    if constexpr (MODE == kDual) st_row_stream<LPR, CPL>(y2 + own, acc2, opts);
+    // split words of the pre-split stacks (fp64: one value per unit).  With
+    // two or more lanes per row, lanes q and q ^ 1 hold adjacent columns of
+    // every unit, and the even lane stores both values' words as one 16-byte
+    // store: rows and units start at even columns, so a pair never straddles
+    // a 256-word block.
     if constexpr (SPLIT_Y || MODE == kDualSplit) {
-        // own is even when CPL is, and pairs never straddle a word block
-        uint2* ws = reinterpret_cast<uint2*>(y2) + word_offset(static_cast<int64_t>(own), wbs);
-        if constexpr (CPL == 1 && LPR >= 2) {
-            // one value per lane: row neighbours' words in one 16-byte store
-            // (own is even for even q: Rp = LPR is even)
-            st_split_lanes(ws, SPLIT_Y ? sv[0] : acc2[0], (q & 1) == 0);
+        uint2* ws = reinterpret_cast<uint2*>(y2);
+        if constexpr (LPR >= 2) {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const int64_t w = static_cast<int64_t>(own) + unit_step<V2, LPR, CPL>(c);
+                st_split_lanes(ws + word_offset(w, wbs), SPLIT_Y ? sv[c] : acc2[c], (q & 1) == 0);
+            }
         } else {
+            // one lane per row: its values are the row's consecutive columns,
+            // and a row (CPL values, CPL dividing 256) lies in one word block
+            ws += word_offset(static_cast<int64_t>(own), wbs);
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 if (CPL % 2 == 0 && (c & 1)) continue;
@@ -420,31 +466,37 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             }
         }
     }
-    // paired chains: a lane's columns lie in one half (Rp/2 is a multiple
-    // of CPL); the left half stores y, the right half H2 x
+    // paired chains: the left half of the columns stores y, the right half
+    // H2 x, as words in blocks of Rp / 2 columns
     if constexpr (PAIR) {
         const int half = Rp >> 1;
-        const int col = q * CPL;
-        const bool left = col < half;
-        // the last step's left output is no stack entry: y2 null, not stored
-        if (left && y2 == nullptr) return;
-        const int64_t k = static_cast<int64_t>(r) * half + (left ? col : col - half);
-        uint2* ws = (left ? reinterpret_cast<uint2*>(y2) : reinterpret_cast<uint2*>(y3)) + word_offset(k, wbs);
         if constexpr (CPL == 1) {
+            const bool left = q < half;
+            // the last step's left output is no stack entry: y2 null, not stored
+            if (left && y2 == nullptr) return;
+            const int64_t k = static_cast<int64_t>(r) * half + (left ? q : q - half);
+            uint2* ws = (left ? reinterpret_cast<uint2*>(y2) : reinterpret_cast<uint2*>(y3)) + word_offset(k, wbs);
             if ((half & 1) == 0) {  // even halves: neighbour lanes' words in one 16-byte store
                 st_split_lanes(ws, left ? sv[0] : acc2[0], (q & 1) == 0);
-                return;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            if (CPL % 2 == 0 && (c & 1)) continue;
-            const V2 a0 = left ? sv[c] : acc2[c];
-            if (CPL % 2 == 0) {
-                const V2 a1 = left ? sv[c + 1] : acc2[c + 1];
-                st_split2(ws + c, a0, a1);
             } else {
-                st_split(ws + c, a0);
+                st_split(ws, left ? sv[0] : acc2[0]);
+            }
+        } else {
+            // unit c of a lane sits at column q + c * LPR: the first CPL / 2
+            // units of every lane are left-half columns, the others right
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const bool left = c < CPL / 2;
+                if (left && y2 == nullptr) continue;
+                const int col = q + unit_step<V2, LPR, CPL>(c);
+                const int64_t k = static_cast<int64_t>(r) * half + (left ? col : col - half);
+                uint2* ws =
+                    (left ? reinterpret_cast<uint2*>(y2) : reinterpret_cast<uint2*>(y3)) + word_offset(k, wbs);
+                if constexpr (LPR >= 2) {
+                    st_split_lanes(ws, left ? sv[c] : acc2[c], (q & 1) == 0);
+                } else {
+                    st_split(ws, left ? sv[c] : acc2[c]);
+                }
             }
         }
     }
