@@ -146,6 +146,7 @@ def test_block_widths(gpu_ctx, R):
     ("fcc", [("A", 1, 1.0)], 32, "fp64"),
     ("cubic", [("A", 1, 1.0), ("A", 2, 0.3), ("A", 3, 0.1)], 8, "fp64"),  # 27 entries/row: row kernel
     ("honeycomb", [("A", 1, -2.7)], 64, "fp32"),
+    ("square", [("A", 1, 1.0)], 128, "fp64"),  # two values per lane: row kernel, and tiled at a 128-column slab
 ])
 def test_tiled_and_row_kernels_and_slabs_agree(gpu_ctx, model, nn, R, prec):
     """Every kernel family and slab width produces the oracle's T_n bit for
@@ -163,7 +164,7 @@ def test_tiled_and_row_kernels_and_slabs_agree(gpu_ctx, model, nn, R, prec):
     seen = set()
     try:
         for slab, chunk_rows, period, nt in [(0, 0, 0, 0), (8, 64, 3, -1), (16, 0, -1, 0), (32, 256, 0, -1),
-                                             (64, 32, 0, 0), (64, 64, 7, 1)]:
+                                             (64, 32, 0, 0), (64, 64, 7, 1), (128, 0, 0, 0)]:
             for row in (False, True):
                 gpu_ctx.set_tuning(slab_cols=slab, chunk_rows=chunk_rows, chunk_period=period, stream_nt=nt,
                                    use_tiled=0 if row else 1)

@@ -717,7 +717,9 @@ __global__ __launch_bounds__(kBlock) void cheb_tile_kernel(
         const int64_t rt = visit_row_tile(order, static_cast<uint32_t>(t), slab);
         if (rt < 0) continue;  // uniform over the workgroup
         const int64_t r0 = rt * TR;
-        const size_t col0 = static_cast<size_t>(slab) * SW + static_cast<size_t>(q) * CPL;
+        // value c of lane q at column col0 + c * LPR: each access instruction
+        // of a wave covers whole runs of the row (no partial sectors)
+        const size_t col0 = static_cast<size_t>(slab) * SW + static_cast<size_t>(q);
         for (int i = threadIdx.x; i <= TR; i += kBlock) {
             const int64_t rr = r0 + i;
             s_rp[i] = rp[rr < rows ? rr : rows];
@@ -751,11 +753,11 @@ __global__ __launch_bounds__(kBlock) void cheb_tile_kernel(
         for (int k = 0; k < RW; ++k) {
             if (MODE != kFirst) {
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) zv[k][c] = ld_stream(z + own[k] + c, opts);
+                for (int c = 0; c < CPL; ++c) zv[k][c] = ld_stream(z + own[k] + c * LPR, opts);
             }
             if (MODE != kRecDbl) {
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) bv[k][c] = ld_stream(bra + own[k] + c, opts);
+                for (int c = 0; c < CPL; ++c) bv[k][c] = ld_stream(bra + own[k] + c * LPR, opts);
             }
         }
 #pragma unroll
@@ -770,7 +772,7 @@ __global__ __launch_bounds__(kBlock) void cheb_tile_kernel(
                 hh[k][j] = ok ? s_hv[idx] : H{};
                 const V2* xr = x + static_cast<size_t>(cc[k][j]) * Rp + col0;
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) xv[k][j][c] = xr[c];
+                for (int c = 0; c < CPL; ++c) xv[k][j][c] = xr[c * LPR];
             }
         }
 
@@ -799,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void cheb_tile_kernel(
             }
             if (!have_self) {
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) xs[c] = x[own[k] + c];
+                for (int c = 0; c < CPL; ++c) xs[c] = x[own[k] + c * LPR];
             }
             if (!live[k]) continue;
 #pragma unroll
@@ -815,7 +817,7 @@ __global__ __launch_bounds__(kBlock) void cheb_tile_kernel(
                     v.x = v.x - zv[k][c].x;
                     v.y = v.y - zv[k][c].y;
                 }
-                st_stream(y + own[k] + c, v, opts);
+                st_stream(y + own[k] + c * LPR, v, opts);
                 if (MODE == kFirst) {
                     p0 = p0 + redot(bv[k][c], xs[c]);
                     p1 = p1 + redot(bv[k][c], v);
