@@ -93,7 +93,9 @@ typedef struct tbk_tuning {
     int32_t chunk_rows;      /* rows per chunk of the visiting order (default 256) */
     int32_t chunk_period;    /* chunk-grid period P; 0: bandwidth / chunk rows, -1: row order */
     int32_t row_lane_bytes;  /* row kernel: vector bytes per lane, 8, 16 or 32 (0: 16), i.e. 1 or 2
-                                complex128 / 1, 2 or 4 complex64 (at least R/64 complex) */
+                                complex128 / 1, 2 or 4 complex64 (at least R/64 complex); the
+                                paired sigma_xy block (both chains, 2R columns) uses this width,
+                                or 32 when it is 0 */
 } tbk_tuning;
 
 const char* tbk_version(void);
