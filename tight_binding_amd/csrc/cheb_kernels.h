@@ -130,9 +130,10 @@ __device__ __forceinline__ void st_stream(V2* p, V2 v, int opts) {
     }
 }
 
-// The CPL consecutive values of one lane.  8-byte elements (complex64) go in
-// pairs through 16-byte accesses: p is 16-byte aligned whenever CPL is even,
-// since lane offsets are multiples of CPL and rows multiples of Rp.
+// CPL consecutive values at p (the row kernel passes one 16-byte unit of a
+// lane, lane_unit_elems values).  8-byte elements (complex64) go in pairs
+// through 16-byte accesses: p is 16-byte aligned whenever CPL is even, since
+// unit offsets are multiples of CPL and rows multiples of Rp.
 template <int CPL, typename V2>
 __device__ __forceinline__ void ld_lane(const V2* p, V2* out) {
     if constexpr (sizeof(V2) == 8 && CPL % 2 == 0) {
