@@ -684,7 +684,7 @@ def main():
         nbytes = 2.0 * H.n * args.R * 16  # read + write
         times = [K.copy_to(K2) for _ in range(args.copy_probe)]
         copy_gbps = nbytes / (min(times) * 1e-3) / 1e9
-        log("block copy (tbk_block_copy, plain stores): %.3f ms best of %d, %.1f GB/s (read+write)"
+        log("block copy (tbk_block_copy): %.3f ms best of %d, %.1f GB/s (read+write)"
             % (min(times), len(times), copy_gbps))
         K2.close()
     # attainable references on buffers of one vector block: a non-temporal

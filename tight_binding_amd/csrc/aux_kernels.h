@@ -71,23 +71,26 @@ __global__ __launch_bounds__(kBlock) void random_phase_kernel(V2* __restrict__ o
     }
 }
 
-// dst = src, 16-byte elements (a vector block copy; also the reference
-// kernel of known traffic for the counter calibration)
-// Each thread moves kCopyUnroll elements, all loads issued before the stores.
+// dst = src, 16-byte elements (a vector block copy), non-temporal: a block
+// is far larger than the caches.  Each thread moves kCopyUnroll elements, all
+// loads issued before the stores.
 constexpr int kCopyUnroll = 8;
 __global__ __launch_bounds__(kBlock) void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                         int64_t n) {
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    const u4v* s = reinterpret_cast<const u4v*>(src);
+    u4v* d = reinterpret_cast<u4v*>(dst);
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kBlock * kCopyUnroll + threadIdx.x;
-    uint4 v[kCopyUnroll];
+    u4v v[kCopyUnroll];
 #pragma unroll
     for (int k = 0; k < kCopyUnroll; ++k) {
         const int64_t i = base + static_cast<int64_t>(k) * kBlock;
-        if (i < n) v[k] = src[i];
+        if (i < n) v[k] = __builtin_nontemporal_load(s + i);
     }
 #pragma unroll
     for (int k = 0; k < kCopyUnroll; ++k) {
         const int64_t i = base + static_cast<int64_t>(k) * kBlock;
-        if (i < n) dst[i] = v[k];
+        if (i < n) __builtin_nontemporal_store(v[k], d + i);
     }
 }
 

@@ -1179,7 +1179,9 @@ int tbk_block_copy(const tbk_block* src, tbk_block* dst, float* kernel_ms) {
             raise(TBK_E_INVALID, "block shapes differ");
         tbk_context* ctx = src->ctx;
         set_device(ctx);
-        const int64_t n16 = static_cast<int64_t>(static_cast<size_t>(src->rows) * src->Rp * src->elem_bytes() / 16);
+        const size_t nbytes = static_cast<size_t>(src->rows) * src->Rp * src->elem_bytes();
+        const int64_t n16 = static_cast<int64_t>(nbytes / 16);
+        const size_t tail = nbytes % 16;  // one complex64 when rows * Rp is odd
         const int64_t per_block = static_cast<int64_t>(tbk::kBlock) * tbk::kCopyUnroll;
         if ((n16 + per_block - 1) / per_block > INT32_MAX) raise(TBK_E_UNSUPPORTED, "block too large to copy");
         const int grid = static_cast<int>((n16 + per_block - 1) / per_block);
@@ -1189,6 +1191,10 @@ int tbk_block_copy(const tbk_block* src, tbk_block* dst, float* kernel_ms) {
             hipLaunchKernelGGL(tbk::copy16_kernel, dim3(std::max(grid, 1)), dim3(tbk::kBlock), 0, ctx->stream,
                                src->data.as<const uint4>(), dst->data.as<uint4>(), n16);
         HIPC(hipGetLastError());
+        if (tail)
+            HIPC(hipMemcpyAsync(static_cast<char*>(dst->data.p) + nbytes - tail,
+                                static_cast<const char*>(src->data.p) + nbytes - tail, tail,
+                                hipMemcpyDeviceToDevice, ctx->stream));
         HIPC(hipEventRecord(ctx->events[1], ctx->stream));
         HIPC(hipStreamSynchronize(ctx->stream));
         if (kernel_ms) HIPC(hipEventElapsedTime(kernel_ms, ctx->events[0], ctx->events[1]));
