@@ -64,7 +64,7 @@ int tbh_lattice_info_get(const tbh_lattice* l, tbh_lattice_info* out);
 /* the lattice's hopping table in table order: the cell's, plus the zero
    on-site entries a scalar field adds (lattice.h:496-505); n_hops entries */
 int tbh_lattice_hoppings(const tbh_lattice* l, int32_t* from, int32_t* to, int32_t* lattice_vector, double* t);
-/* exp(-i A(r_from).d_h) per (cell, hopping), cell_number * n_hops complex
+/* exp(+i A(r_from).d_h) per (cell, hopping), cell_number * n_hops complex
    (re, im), exactly as the open and periodic builders apply it
    (lattice.h:282-285); needs a gauge field */
 int tbh_lattice_phase_table(const tbh_lattice* l, double* out);

@@ -110,9 +110,11 @@ int tbk_device_count(int32_t* count);
  *   T_0 = kets, T_1 = M kets, T_i = 2 M T_{i-1} - T_{i-2}, i < n.
  * Not divided by the number of columns.  M is used as given (already
  * rescaled).  n < 2 fails with TBK_E_INVALID, as numerics_cuda.cu:194-197.
- * bras_T and kets: ket_rows x ket_cols column-major complex128.  bras_T may
- * equal kets (the same pointer), which selects the two-moments-per-step path.
- * Uses a process-wide context on device 0.
+ * bras_T and kets: ket_rows x ket_cols column-major complex128, both of that
+ * size.  When bras_T is the same pointer as kets and M is exactly Hermitian
+ * (checked on the host: M^H equals M value for value), two moments are formed
+ * per step by the doubling identities; any other input takes one moment per
+ * step against the bra block.  Uses a process-wide context on device 0.
  */
 int tbk_cheb_first_kind_seq_braket(const tbk_sparse* M, int32_t n, const double* bras_T,
                                    const double* kets, int32_t ket_rows, int32_t ket_cols,
@@ -178,7 +180,7 @@ typedef struct tbk_lattice_desc {
     const int32_t* hop_to;
     const int32_t* hop_lv;    /* n_hops * dim lattice vectors */
     const double* hop_t;      /* 2 n_hops: (re, im) */
-    const double* phase;      /* NULL, or exp(-i A(r_from).d) per (cell, hopping) as the host builders
+    const double* phase;      /* NULL, or exp(+i A(r_from).d) per (cell, hopping) as the host builders
                                  compute it, cell_number * n_hops complex (builders 0, 1) */
     int32_t has_disorder;     /* on-site W (u(seed, site) - 1/2), the site-keyed counter generator */
     double disorder_W;

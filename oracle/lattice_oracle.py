@@ -499,7 +499,7 @@ class PartialLattice:
                 d = self.cell.displacement_of_hopping(hp)
                 A = self.gauge_field(self._positions(ivk, frm))
                 theta = (A[:, 0] * d[0] + A[:, 1] * d[1]) + A[:, 2] * d[2]
-                cs, sn = _cos_sin_exact(-theta)
+                cs, sn = _cos_sin_exact(theta)
                 tre, tim = tre * cs - tim * sn, tre * sn + tim * cs
             if fields:
                 diag = row == col

@@ -8,6 +8,7 @@
 // usage: dropin_dos <L> <N>
 #include <cstdio>
 #include <cstdlib>
+#include <stdexcept>
 #include <vector>
 
 #include "green.h"
@@ -54,5 +55,13 @@ int main(int argc, char** argv) {
     // n < 2 keeps the reference's behaviour: message and an empty result
     auto none = cuda::Cheb_first_kind_seq_braket(G.scaled_HE, 1, kets, kets);
     std::printf("empty %zu\n", none.size());
+    // bras_T must have the kets' shape
+    try {
+        MatrixXcd narrow = numerical::linalg::bases_of_uniform_ampl_with_random_phases(X.get_total_number(), 2);
+        cuda::Cheb_first_kind_seq_braket(G.scaled_HE, N, narrow, kets);
+        std::printf("shape_error none\n");
+    } catch (const std::invalid_argument&) {
+        std::printf("shape_error thrown\n");
+    }
     return 0;
 }

@@ -113,3 +113,19 @@ def test_rescale_bit_exact_and_csr_view():
     # values; conj flips the sign of the zero imaginary parts on the diagonal)
     assert np.array_equal(rp, ps.colptr) and np.array_equal(ci, ps.rowidx)
     assert np.array_equal(v, np.conj(ps.values))
+
+
+def test_product_peierls_sign_by_hand():
+    """Open 2x2 square lattice, t = 1, Landau gauge A = (0, B x, 0), B = 0.4.
+    Sites are x*2 + y.  The bond (1,0)-(1,1) has d = (0, +-1) and
+    A(r_from).d = +-B from either end, so lattice.h:284 (t exp(+i A.d))
+    gives H[2,3] = exp(+0.4 i) whichever orientation the table stores; the
+    x = 0 bond carries no phase and x-bonds none either (A_x = 0)."""
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (2, 2))
+    lat.set_gauge(L.GAUGE_LANDAU, 0.4)
+    H = lat.get_Hamiltonian_sparse(True).to_dense()
+    e = complex(0.9210609940028851, 0.3894183423086505)  # exp(0.4 i)
+    assert abs(H[2, 3] - e) < 1e-15 and abs(H[3, 2] - e.conjugate()) < 1e-15
+    assert H[0, 1] == 1.0 and H[0, 2] == 1.0 and H[1, 3] == 1.0

@@ -53,6 +53,7 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     mu = np.array([float(x.split()[1]) for x in lines if x.startswith("mu ")])
     dos = [(float(x.split()[1]), float(x.split()[2])) for x in lines if x.startswith("dos ")]
     assert any(x == "empty 0" for x in lines)
+    assert any(x == "shape_error thrown" for x in lines)
     assert "n must be greater than or equal to 2" in out.stderr
 
     cell = O.model("honeycomb")

@@ -48,6 +48,31 @@ def test_dropin_c1_doubling_and_general_match_oracle():
 
 
 @pytest.mark.parametrize("cplx", [False, True])
+def test_dropin_non_hermitian_same_kets_takes_direct_path(cplx):
+    """bras_T is kets but M is not Hermitian: the doubling identities do not
+    hold, so the drop-in must form every moment directly, as
+    numerical.h:363-381 does."""
+    if cplx:
+        p, _ = build_pair("square", (16, 16), [("A", 1, 1.0)], builder="open", is_complex=True, W=1.0, seed=4,
+                          gauge="landau", B=2 * math.pi / 16)
+    else:
+        p, _ = build_pair("square", (16, 16), [("A", 1, 1.0)], W=1.0, seed=4)
+    Hs = O.rescale(p, 4.8, 0.0)
+    cols = np.repeat(np.arange(Hs.n), np.diff(Hs.colptr))
+    v = Hs.values.copy()
+    v[Hs.rowidx < cols] *= 0.9  # upper triangle scaled: M != M^H; moments stay bounded (max |mu| = R)
+    M = O.CSC(Hs.n, Hs.colptr, Hs.rowidx, v)
+    kets = O.random_phase_kets(M.n, 8, 21)
+    N = 96
+    ref = oracle.cheb_braket(M, N, kets, kets)
+    mu = kpm.cheb_first_kind_seq_braket(M, N, kets, kets)
+    assert_moments_close(mu, ref)
+    # the doubling identities would give a different answer on this M
+    herm = kpm.cheb_first_kind_seq_braket(Hs, N, kets, kets)
+    assert np.max(np.abs(herm - mu)) > 1e-3 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("R", [8, 64])
 def test_recursion_vectors_bit_exact(gpu_ctx, cplx, R):
     if cplx:

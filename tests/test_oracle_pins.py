@@ -194,10 +194,27 @@ def test_periodic_spectra_equal_bloch_closed_forms():
     assert np.allclose(np.sort(np.linalg.eigvalsh(H)), np.sort(honeycomb_bloch_eigs(5, 7, -2.7)), atol=1e-11)
 
 
+def test_peierls_phase_two_site_chain_by_hand():
+    """Two sites of an open chain, t = -1.3, constant A = (0.7, 0, 0).  The
+    stored hopping is A->A over dR = +1 or -1; either way lattice.h:284 gives
+    H[0,1] = t exp(+i A.d) with d = (1, 0, 0), i.e. -1.3 (cos 0.7 + i sin 0.7),
+    and H[1,0] its conjugate."""
+    cell = O.model("chain")
+    cell.add_hopping_nn("A", 1, -1.3)
+    lat = O.PartialLattice(cell, (2,))
+    lat.gauge_field = lambda pos: np.tile(np.array([0.7, 0.0, 0.0]), (pos.shape[0], 1))
+    H = lat.get_Hamiltonian_sparse(True).to_dense()
+    h01 = complex(-1.3 * 0.7648421872844885, -1.3 * 0.644217687237691)
+    assert abs(H[0, 1] - h01) < 1e-15
+    assert abs(H[1, 0] - h01.conjugate()) < 1e-15
+    assert H[0, 0] == 0 and H[1, 1] == 0
+
+
 @pytest.mark.parametrize("gauge", ["landau", "symmetric"])
 def test_peierls_flux_per_plaquette(gauge):
-    """t*exp(-i A(r_from).d): the product of the four hoppings around a unit
-    plaquette has phase -B (gauge invariant); a flipped sign gives +B."""
+    """t*exp(+i A(r_from).d) (lattice.h:284: `1.0i * A.dot(d)` with A and d
+    real): the product H[a,b] H[b,c] H[c,d] H[d,a] around a counter-clockwise
+    unit plaquette has phase +B (gauge invariant); a reversed sign gives -B."""
     B = 2 * math.pi / 7
     cell = O.model("square")
     cell.add_hopping_nn("A", 1, 1.0)
@@ -210,7 +227,7 @@ def test_peierls_flux_per_plaquette(gauge):
         for y in range(L - 1):
             loop = (H[idx(x, y), idx(x + 1, y)] * H[idx(x + 1, y), idx(x + 1, y + 1)]
                     * H[idx(x + 1, y + 1), idx(x, y + 1)] * H[idx(x, y + 1), idx(x, y)])
-            assert abs(np.angle(loop) - (-B)) < 1e-12
+            assert abs(np.angle(loop) - B) < 1e-12
     assert np.allclose(H, H.conj().T)
 
 

@@ -17,7 +17,7 @@
 // sums as the host, without a global sort.  Two passes (count, scan, fill).
 //
 // Values: t from the table; times the host's own phase table
-// exp(-i A(r_from).d) per (cell, hopping) when a gauge field is set (the
+// exp(+i A(r_from).d) per (cell, hopping) when a gauge field is set (the
 // complex product written out as the host's: re = a c - b d, im = a d + b c);
 // plus the site-keyed disorder W (u(seed, site) - 1/2) on the diagonal pushes
 // (the same splitmix64 counter, so the same bits).  Compiled without

@@ -186,6 +186,32 @@ void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_
         }
 }
 
+// M == M^H exactly (values compared bit for bit after conjugation), from the
+// CSR arrays: the counting transpose of (rp, ci, conj val), rows visited in
+// order, equals (rp, ci, val).  A row whose columns are not ascending makes
+// the two orders differ and reads as not Hermitian, which only costs the
+// faster path.
+bool csr_is_hermitian(int32_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                      const std::vector<double>& val, bool cplx) {
+    const int vw = cplx ? 2 : 1;
+    const size_t nnz = ci.size();
+    std::vector<int32_t> tp(static_cast<size_t>(n) + 1, 0);
+    for (size_t p = 0; p < nnz; ++p) tp[ci[p] + 1]++;
+    for (int32_t i = 0; i < n; ++i) tp[i + 1] += tp[i];
+    if (tp != rp) return false;
+    std::vector<int32_t> fill(tp.begin(), tp.end() - 1);
+    for (int32_t r = 0; r < n; ++r)
+        for (int32_t p = rp[r]; p < rp[r + 1]; ++p) {
+            const int32_t q = fill[ci[p]]++;
+            if (ci[q] != r) return false;
+            const double* a = &val[static_cast<size_t>(p) * vw];
+            const double* b = &val[static_cast<size_t>(q) * vw];
+            if (a[0] != b[0]) return false;
+            if (cplx && a[1] != -b[1]) return false;
+        }
+    return true;
+}
+
 void matrix_fp32(tbk_matrix* m) {
     if (m->have32) return;
     const int64_t n = m->nnz * (m->is_complex ? 2 : 1);
@@ -1435,7 +1461,17 @@ int tbk_cheb_first_kind_seq_braket(const tbk_sparse* M, int32_t n, const double*
         if (rc != TBK_OK) raise(rc, g_last_error);
         rc = tbk_block_create(ctx, kets, ket_rows, ket_cols, TBK_FP64, &g.k);
         if (rc != TBK_OK) raise(rc, g_last_error);
-        if (bras_T != kets) {
+        // Two moments per step needs bra = ket and a Hermitian M (T_k(M)^H
+        // T_k(M) = T_k(M)^2 = (T_2k + T_0)/2); otherwise one moment per step
+        // against the bra block, as numerical.h:363-381 computes it.
+        bool doubling = bras_T == kets;
+        if (doubling) {
+            std::vector<int32_t> rp, ci;
+            std::vector<double> val;
+            build_csr(M, rp, ci, val);
+            doubling = csr_is_hermitian(M->rows, rp, ci, val, M->is_complex != 0);
+        }
+        if (!doubling) {
             rc = tbk_block_create(ctx, bras_T, ket_rows, ket_cols, TBK_FP64, &g.b);
             if (rc != TBK_OK) raise(rc, g_last_error);
         }

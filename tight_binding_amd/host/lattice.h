@@ -7,7 +7,9 @@
 // order and, inside a cell, hoppings in table order (lattice.h:466-493).
 //
 // Value pipeline per (cell, hopping) (lattice.h:274-298):
-//   t  <- t * exp(-i A(r_from).d)          if a gauge field is set
+//   t  <- t * exp(+i A(r_from).d)          if a gauge field is set
+//        (A and d are real vectors: `1.0i * A.dot(d)` in lattice.h:284 is
+//        i times the real scalar A.d)
 //   t  <- t + V(r_from)                     if row == col and a scalar field
 //                                           (or a site-keyed field) is set
 //   push (row, col, t) and, if row != col, (col, row, conj t); a real Scalar
@@ -207,7 +209,7 @@ public:
     std::function<Vector3d(Vector3d)> gauge_field;
     inline static bool auto_info = false;
 
-    // exp(-i A(r_from) . d_h) for every (cell, hopping), cells in index order
+    // exp(+i A(r_from) . d_h) for every (cell, hopping), cells in index order
     // and hoppings in table order: the factor the open and periodic builders
     // apply (lattice.h:282-285), computed by the same expressions.
     std::vector<Complex> phase_table() const {
@@ -222,7 +224,7 @@ public:
             for (size_t h = 0; h < hops.size(); ++h) {
                 Vector3d A = gauge_field(cell_.atom_position(std::get<0>(hops[h].first), i_v));
                 double theta = A.dot(disp[h]);
-                out.push_back(std::exp(Complex(0.0, -theta)));
+                out.push_back(std::exp(Complex(0.0, theta)));
             }
         }
         return out;
@@ -317,7 +319,7 @@ private:
                     if (gauge_field) {
                         Vector3d A = gauge_field(cell_.atom_position(from, i_v));
                         double theta = A.dot(disp[h]);
-                        t *= std::exp(Complex(0.0, -theta));
+                        t *= std::exp(Complex(0.0, theta));
                     }
                     if (row == col) {
                         if (scalar_field) t += scalar_field(cell_.atom_position(from, i_v));

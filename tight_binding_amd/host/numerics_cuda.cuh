@@ -6,10 +6,14 @@
 //   T_0 = kets, T_1 = M kets, T_i = 2 M T_{i-1} - T_{i-2}, i < n,
 //
 // M is the already rescaled matrix (GreenFunction::scaled_HE), stored
-// column-major compressed; bras_T and kets are rows x R complex blocks.
-// n < 2 prints the reference's message and returns an empty vector
-// (numerics_cuda.cu:194-197); a device failure throws std::runtime_error
-// (the reference ignores CUDA status codes).  Nothing is printed per step.
+// column-major compressed (an uncompressed Eigen matrix is rejected, as the
+// reference's comments at numerics_cuda.cu:189,266 require); bras_T and kets
+// are rows x R complex blocks of the same shape.  The two-moments-per-step
+// path runs only for bras_T == kets (same storage) and an exactly Hermitian
+// M; anything else takes one moment per step.  n < 2 prints the
+// reference's message and returns an empty vector (numerics_cuda.cu:194-197);
+// a device failure throws std::runtime_error (the reference ignores CUDA
+// status codes).  Nothing is printed per step.
 //
 // Accepted types: tb::SparseMatrix / tb::MatrixXcd of this package, and any
 // type with Eigen's accessors (rows(), cols(), nonZeros(), outerIndexPtr(),
@@ -33,6 +37,17 @@ using Complex = std::complex<double>;
 
 namespace detail {
 
+// false only for a sparse type that reports itself uncompressed (Eigen's
+// isCompressed()); types without the accessor are compressed by construction
+template <typename Sparse>
+auto is_compressed(const Sparse& M, int) -> decltype(M.isCompressed(), bool()) {
+    return M.isCompressed();
+}
+template <typename Sparse>
+bool is_compressed(const Sparse&, long) {
+    return true;
+}
+
 template <typename Sparse, typename Dense>
 std::vector<double> cheb_braket(const Sparse& M, int n, const Dense& bras_T, const Dense& kets) {
     using Scalar = std::remove_cv_t<std::remove_pointer_t<decltype(M.valuePtr())>>;
@@ -42,6 +57,10 @@ std::vector<double> cheb_braket(const Sparse& M, int n, const Dense& bras_T, con
         std::cerr << "n must be greater than or equal to 2" << std::endl;
         return {};
     }
+    if (bras_T.rows() != kets.rows() || bras_T.cols() != kets.cols())
+        throw std::invalid_argument("Cheb_first_kind_seq_braket: bras_T and kets differ in shape");
+    if (!is_compressed(M, 0))
+        throw std::invalid_argument("Cheb_first_kind_seq_braket: M must be compressed (makeCompressed())");
     tbk_sparse s{};
     s.rows = static_cast<int32_t>(M.rows());
     s.cols = static_cast<int32_t>(M.cols());
