@@ -8,6 +8,11 @@ conductivity_Kubo_Bastin_test (snippets.h:306-456).
   zeta(e) = sum Gamma_nm g_mn(e) / (1 - e^2)^2 / Volume          (snippets.h:343-349, 400-405)
   sigma(mu) = integral of Re zeta from e = 0 to mu               (snippets.h:420-449)
 
+With the sample area given, solve() also reports sigma_xy in units of e^2/h
+(spectral.kubo_sigma_physical): -(2 Volume n/(R area)) times the integral of
+Re zeta from the band bottom, the reference's zeta converted to the Bastin
+formula with physical velocities i[H, r].
+
 Gamma is computed on the GPU (two vector stacks in HBM and a complex
 contraction on matrix cores, fp64 or split-bf16 x3); zeta at the Chebyshev
 nodes is evaluated on the GPU from the N x N Gamma, and the integral over
@@ -59,10 +64,12 @@ class KuboSolver:
     def zeta(self, gamma, e, volume):
         return kpm.kubo_zeta(self.ctx, gamma, e, volume)[0]
 
-    def solve(self, N, R=4, seed=1, precision=kpm.GAMMA_FP64, fermi_energies=None, kets=None):
+    def solve(self, N, R=4, seed=1, precision=kpm.GAMMA_FP64, fermi_energies=None, kets=None, area=None):
         """Returns dict(gamma, a, b, sigma, sigma_grid, zeta_e, zeta,
         fermi_energies, stats, timings); fermi_energies in the units of H
-        (None skips sigma)."""
+        (None skips sigma).  With area (the sample's area in lattice length
+        units) also sigma_xy_e2h and sigma_xy_e2h_grid: sigma_xy in e^2/h,
+        integrated from the band bottom."""
         t0 = time.perf_counter()
         g = spectral.lorentz(N, self.lam)
         K = self.ctx.block(kets) if kets is not None else self.ctx.random_block(self.n, R, seed=seed)
@@ -74,19 +81,25 @@ class KuboSolver:
         out = dict(gamma=gamma, a=self.a, b=self.b, stats=st)
         if fermi_energies is not None:
             ef = (np.asarray(fermi_energies, dtype=np.float64) - self.b) / self.a
-            sigma, e, zr, integral = spectral.kubo_sigma(gamma, ef, self.volume, zeta=self.zeta)
+            z_nodes = self.zeta(gamma, spectral.kubo_nodes(N), self.volume)
+            sigma, e, zr, integral = spectral.kubo_sigma(gamma, ef, self.volume, zeta=lambda *_: z_nodes)
             out.update(fermi_energies=np.asarray(fermi_energies), sigma=sigma, zeta_e=e, zeta=zr,
                        sigma_grid=integral)
+            if area is not None:
+                r_cols = kets.shape[1] if kets is not None else R
+                sp, _, grid = spectral.kubo_sigma_physical(gamma, ef, self.n, r_cols, area, self.volume,
+                                                           zeta=lambda *_: z_nodes)
+                out.update(sigma_xy_e2h=sp, sigma_xy_e2h_grid=grid)
         t2 = time.perf_counter()
         out["timings"] = {"gamma_s": t1 - t0, "sigma_s": t2 - t1, "total_s": t2 - t0}
         return out
 
 
 def conductivity_kubo_bastin(ctx, H, X, Y, N, E_min, E_max, volume=1.0, R=4, lam=6.0, seed=1,
-                             precision=kpm.GAMMA_FP64, fermi_energies=None, kets=None):
+                             precision=kpm.GAMMA_FP64, fermi_energies=None, kets=None, area=None):
     """One-shot form of KuboSolver.solve; returns the same dict."""
     s = KuboSolver(ctx, H, X, Y, E_min, E_max, volume, lam)
     try:
-        return s.solve(N, R, seed, precision, fermi_energies, kets)
+        return s.solve(N, R, seed, precision, fermi_energies, kets, area)
     finally:
         s.close()

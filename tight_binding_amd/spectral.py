@@ -10,6 +10,10 @@ Host-side mirror of the reference's front end:
                           states per energy)
   dct_reconstruction      lattice.h:625-640: REDFT01 of g*mu at the Chebyshev
                           nodes e_k = cos((k + 1/2) pi / N), times 1/sqrt(1-e^2)
+  kubo_zeta / kubo_sigma   snippets.h:343-449 (the reference's unnormalised
+                          zeta and its integral anchored at e = 0)
+  kubo_sigma_physical     sigma_xy in units of e^2/h (not in the reference;
+                          the conversion is derived below)
 For real e in (-1, 1), alpha_n(e) = -2i e^{-i n theta}/sin(theta) (n >= 1),
 alpha_0 = -i/sin(theta), e = cos(theta); green_imag evaluates that closed
 form (the reference multiplies out powers of e^{-i theta}, equal up to
@@ -196,3 +200,63 @@ def kubo_sigma(gamma, e_fermi, volume=1.0, zeta=None):
     integral = integrate_from_zero(E, zr)
     ef = np.atleast_1d(np.asarray(e_fermi, dtype=np.float64))
     return np.interp(ef, E, integral), E, zr, integral
+
+
+# ------------------------------------------- sigma_xy in physical units
+def sigma_xy_e2h_factor(n_sites, R, area, volume=1.0):
+    """c in sigma_xy [e^2/h] = c * integral of Re zeta(e) de over rescaled e.
+
+    The reference's zeta (snippets.h:343-405) is built from v = [r, H~]
+    (no factor i, scaled H~), stochastic kets of norm 1 (elements
+    e^{i phi}/sqrt(n)), no 1/R and no e^2/h.  With hbar = 1 the Bastin
+    formula is
+
+      sigma_xy = (i e^2 / Omega) int de f(e) Tr[v_x d(e-H) v_y G+'(e)
+                                                - v_x G-'(e) v_y d(e-H)]
+
+    with physical velocities v = i[H, r] = -i a [r, H~].  Expanding d and G+-
+    in Chebyshev polynomials turns the reference's double sum into
+    zeta(e) Volume = i pi x (that trace, in rescaled units and with v = [r, H~]);
+    the a's of v^2 (-a^2), of d (1/a), of G' (1/a^2) and of de (a) cancel.
+    The stochastic trace estimates sum_r <r|X|r> = (R/n) Tr X.  So
+
+      sigma_xy / (e^2/h) = -(2 Volume n / (R Omega)) int_{-1}^{mu} Re zeta(e) de,
+
+    Omega the sample area (2D; a volume in 3D, where sigma is per length) in
+    the length unit of the lattice vectors.  The imaginary part of zeta
+    integrates to zero.  Checked against the exact TKNN sum of a magnetic
+    torus, where it gives the integer Landau-level plateaus."""
+    return -2.0 * float(volume) * float(n_sites) / (float(R) * float(area))
+
+
+def integrate_from_bottom(E, z):
+    """Cumulative trapezoid integral of z over the ascending grid E, zero at
+    E[0]."""
+    E = np.asarray(E, dtype=np.float64)
+    z = np.asarray(z, dtype=np.float64)
+    out = np.zeros(E.size)
+    if E.size > 1:
+        out[1:] = np.cumsum((z[1:] + z[:-1]) / 2 * np.diff(E))
+    return out
+
+
+def kubo_sigma_physical(gamma, e_fermi, n_sites, R, area, volume=1.0, zeta=None):
+    """sigma_xy in units of e^2/h at rescaled Fermi energies.
+
+    zeta at the N Chebyshev nodes as the reference evaluates it, N/24 nodes
+    cut at each end (kubo_sigma), Re zeta integrated from the lowest kept
+    node upward (the band bottom, where the occupied states start, instead
+    of the reference's anchor at e = 0), times sigma_xy_e2h_factor.  The
+    cut nodes must hold no states: keep the spectrum inside
+    |e| < cos(pi (N/24 + 1/2)/N) (about 0.99) by the padding of [E_min, E_max].
+    Returns (sigma at e_fermi, grid E ascending, sigma on E)."""
+    gamma = np.asarray(gamma)
+    N = gamma.shape[0]
+    eps = kubo_nodes(N)
+    z = (zeta or kubo_zeta)(gamma, eps, volume)
+    cut = N // 24
+    E = eps[cut:N - cut][::-1]
+    zr = np.real(z)[cut:N - cut][::-1]
+    grid = sigma_xy_e2h_factor(n_sites, R, area, volume) * integrate_from_bottom(E, zr)
+    ef = np.atleast_1d(np.asarray(e_fermi, dtype=np.float64))
+    return np.interp(ef, E, grid), E, grid
