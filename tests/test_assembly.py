@@ -18,6 +18,8 @@ CASES = [
     ("honeycomb_32", "honeycomb", (32, 32), [("A", 1, -2.7)], "periodic", False, 0.5, None, 0.0),
     ("c3_landau_32", "square", (32, 32), [("A", 1, 1.0)], "open", True, 2.0, "landau", 2 * math.pi / 64),
     ("symmetric_honeycomb", "honeycomb", (12, 10), [("A", 1, 1.0)], "open", True, 0.3, "symmetric", 0.07),
+    # theta = -2.4183991523122903 occurs here: cos and sincos round it apart
+    ("landau_honeycomb", "honeycomb", (9, 7), [("A", 1, 1.0)], "open", True, 0.5, "landau", 2 * math.pi / 9),
     ("lieb", "lieb", (9, 7), [("A", 1, 1.0), ("B", 1, 1.0), ("C", 1, 1.0)], "periodic", False, 3.0, None, 0.0),
     ("kagome", "kagome", (8, 8), [("A", 1, 1.0), ("B", 1, 1.0), ("C", 1, 1.0)], "periodic", False, 1.5, None, 0.0),
     ("cubic_nnn", "cubic", (5, 4, 3), [("A", 1, 1.0), ("A", 2, 0.25)], "periodic", False, 1.0, None, 0.0),

@@ -13,6 +13,7 @@ import time
 import numpy as np
 import pytest
 
+from oracle import lattice_oracle as O
 from tight_binding_amd import _native as Nn
 from tight_binding_amd import kpm
 from tight_binding_amd import lattice as L
@@ -32,6 +33,26 @@ def lattice(model, span, shells=(1,), t=1.0, W=0.0, seed=1, gauge=0, B=0.0, all_
     if W:
         lat.set_disorder(W, seed)
     return lat
+
+
+def oracle_csr(model, span, shells, t, W, seed, gauge, B, builder, extend, is_complex):
+    """The same Hamiltonian from the oracle's restatement of the builders, as
+    CSR arrays (lattice.h:216-305)."""
+    c = O.model(model)
+    for k in shells:
+        c.add_hopping_nn("A", k, t * (1.0 if k == 1 else 0.37))
+    lat = O.PartialLattice(c, span)
+    if gauge:
+        lat.gauge_field = (O.landau_gauge if gauge == L.GAUGE_LANDAU else O.symmetric_gauge)(B)
+    if W:
+        lat.set_scalar_field_indexed(O.uniform_disorder(W, seed))
+    if builder == L.BUILD_OPEN:
+        H = lat.get_Hamiltonian_sparse(is_complex)
+    elif builder == L.BUILD_PERIODIC:
+        H = lat.get_Hamiltonian_periodic_extend_bare_sparse(extend, is_complex)
+    else:
+        H = lat.get_Hamiltonian_bare_sparse(is_complex)
+    return H.to_csr()
 
 
 def host_csr(lat, builder, extend, is_complex):
@@ -76,6 +97,11 @@ def test_device_assembly_is_bit_identical(gpu_ctx, case):
     assert np.array_equal(drp, rp), name
     assert np.array_equal(dci, ci), name
     assert np.array_equal(bits(dval), bits(val)), name
+    # and directly against the oracle, independently of the host builder
+    orp, oci, oval = oracle_csr(model, span, shells, 1.0, W, 7 + case, gauge, B, builder, ext, cplx)
+    assert np.array_equal(drp, orp), name
+    assert np.array_equal(dci, oci), name
+    assert np.array_equal(bits(dval), bits(np.ascontiguousarray(oval))), name
     # the engine's statistics: as for the uploaded host matrix
     Mh = gpu_ctx.matrix(H)
     assert M.info() == Mh.info(), name

@@ -22,6 +22,7 @@
 // independent of iteration order); uniform_disorder(W, seed) is that field.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <functional>
 #include <iostream>
@@ -53,6 +54,15 @@ inline double counter_uniform(uint64_t seed, uint64_t counter) {
 inline std::function<double(int)> uniform_disorder(double W, uint64_t seed) {
     return [W, seed](int site) { return W * (counter_uniform(seed, static_cast<uint64_t>(site)) - 0.5); };
 }
+
+// exp(+i theta) as (cos theta, sin theta) from two separate libm calls.
+// std::exp of an imaginary argument goes through sincos, whose rounding
+// differs from cos/sin in near-tie cases (theta = -2.4183991523122903 is
+// one); the separate, non-inlined calls keep the compiler from fusing them
+// into sincos, so every phase has the bits of libm's cos and sin.
+__attribute__((noinline)) inline double phase_cos(double theta) { return std::cos(theta); }
+__attribute__((noinline)) inline double phase_sin(double theta) { return std::sin(theta); }
+inline Complex peierls_phase(double theta) { return Complex(phase_cos(theta), phase_sin(theta)); }
 
 template <int Dim>
 class PartialLattice {
@@ -224,7 +234,7 @@ public:
             for (size_t h = 0; h < hops.size(); ++h) {
                 Vector3d A = gauge_field(cell_.atom_position(std::get<0>(hops[h].first), i_v));
                 double theta = A.dot(disp[h]);
-                out.push_back(std::exp(Complex(0.0, theta)));
+                out.push_back(peierls_phase(theta));
             }
         }
         return out;
@@ -319,7 +329,7 @@ private:
                     if (gauge_field) {
                         Vector3d A = gauge_field(cell_.atom_position(from, i_v));
                         double theta = A.dot(disp[h]);
-                        t *= std::exp(Complex(0.0, theta));
+                        t *= peierls_phase(theta);
                     }
                     if (row == col) {
                         if (scalar_field) t += scalar_field(cell_.atom_position(from, i_v));
