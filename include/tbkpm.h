@@ -262,6 +262,35 @@ int tbk_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b,
                 double* mu_out, double* last_out, tbk_stats* stats);
 
 /*
+ * tbk_moments without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.
+ *
+ * Same moments as tbk_moments (the same bits for the same context tuning),
+ * written to mu_dev, a device array of n doubles.  The two rolling vector
+ * blocks are the caller's: work0 and work1, device buffers of at least
+ * kets' rows x Rp elements each (Rp = R rounded up to a power of two;
+ * elements complex128 in TBK_FP64, complex64 in TBK_FP32; see
+ * tbk_block_device_ptr), distinct from each other and from kets and bras.  On
+ * return T_steps is in work1 when steps is odd and in work0 when even
+ * (steps = (n+1)/2 with doubling, n-1 otherwise), T_{steps-1} in the other.
+ * The per-step moment partials live in a fixed 2.5 MiB device array, one
+ * per GPU, kept until the process ends: at most 32768 steps and 4096
+ * workgroups per step launch (TBK_E_UNSUPPORTED beyond).  Every call on a
+ * GPU shares that array, so calls that overlap on one GPU (two streams, two
+ * threads) race; calls ordered on one stream do not.  Work is enqueued on
+ * `stream` (a hipStream_t; NULL: the context's stream) and the call returns
+ * without synchronising.  Nothing is allocated: in TBK_FP32 the matrix's
+ * fp32 values must already exist (formed by its first fp32 tbk_moments).
+ * TBK_FLAG_TIME_LAUNCHES is rejected (timing needs events).
+ */
+int tbk_moments_noalloc(tbk_context* ctx, const tbk_matrix* M, double a, double b,
+                        const tbk_block* bras, const tbk_block* kets, int32_t n, int32_t flags,
+                        void* work0, void* work1, double* mu_dev, void* stream);
+/* the device address of a block's storage: rows x Rp elements, row-major */
+int tbk_block_device_ptr(const tbk_block* b, void** out);
+
+/*
  * Stored Chebyshev blocks, device-resident: replaces
  * Chebyshev::Cheb_first_kind_seq_on_bases (numerical.h:334-344) for sparse M.
  *   out[0] = v,  out[1] = H~ v,  out[k] = 2 H~ out[k-1] - out[k-2],  k < n,

@@ -289,6 +289,8 @@ KPM_SIGNATURES = {
     "tbk_block_copy": ([P, P, C.POINTER(C.c_float)], I32),
     "tbk_block_destroy": ([P], I32),
     "tbk_moments": ([P, P, D, D, P, P, I32, I32, DP, DP, C.POINTER(tbk_stats)], I32),
+    "tbk_moments_noalloc": ([P, P, D, D, P, P, I32, I32, P, P, P, P], I32),
+    "tbk_block_device_ptr": ([P, C.POINTER(C.c_void_p)], I32),
     "tbk_autotune": ([P, P, D, D, P, I32, C.POINTER(tbk_tuning), DP], I32),
     "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
     "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
