@@ -96,6 +96,9 @@ typedef struct tbk_tuning {
                                 complex128 / 1, 2 or 4 complex64 (at least R/64 complex); the
                                 paired sigma_xy block (both chains, 2R columns) uses this width,
                                 or 32 when it is 0 */
+    int32_t row_slab_cols;   /* moment recursions with the row kernel: sweep the rows this many
+                                vector columns at a time (a power of two below the padded R; 0:
+                                whole rows), so that the gathered rows of a sweep take less L2 */
 } tbk_tuning;
 
 const char* tbk_version(void);

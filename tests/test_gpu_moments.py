@@ -342,3 +342,32 @@ def test_bandwidth_probe(gpu_ctx):
     assert b["read_GBps"] >= 0.9 * b["copy_GBps"]
     with pytest.raises(Nn.NativeError):
         gpu_ctx.bandwidth_probe(1024)
+
+
+@pytest.mark.parametrize("prec,R,slab,cplx", [(kpm.FP64, 64, 16, False), (kpm.FP64, 64, 32, True),
+                                              (kpm.FP32, 32, 8, False), (kpm.FP64, 128, 64, False)])
+def test_row_slabs_give_the_same_vectors(gpu_ctx, prec, R, slab, cplx):
+    """row_slab_cols sweeps the rows a few vector columns at a time: every
+    T_n element is formed by the same per-row sum, so the last block is the
+    same bits and the moments agree to rounding (partials grouped by sweep)."""
+    if cplx:
+        p, _ = build_pair("square", (40, 40), [("A", 1, 1.0)], builder="open", is_complex=True, W=1.0, seed=6,
+                          gauge="landau", B=2 * math.pi / 20)
+    else:
+        p, _ = build_pair("honeycomb", (40, 40), [("A", 1, -2.7)], W=0.5, seed=6)
+    M = gpu_ctx.matrix(p)
+    K = gpu_ctx.random_block(p.n, R, seed=12, precision=prec)
+    saved = gpu_ctx.get_tuning()
+    try:
+        for plain in (False, True):
+            mu0, st0, last0 = gpu_ctx.moments(M, K, 65, a=8.6, b=0.1, plain=plain, want_last=True)
+            gpu_ctx.set_tuning(**dict(saved, row_slab_cols=slab))
+            mu1, st1, last1 = gpu_ctx.moments(M, K, 65, a=8.6, b=0.1, plain=plain, want_last=True)
+            gpu_ctx.set_tuning(**saved)
+            assert st1["slab_cols"] == slab and st0["slab_cols"] == st0["padded_R"]
+            assert np.array_equal(np.ascontiguousarray(last0).view(np.uint64), np.ascontiguousarray(last1).view(np.uint64))
+            assert np.max(np.abs(mu1 - mu0)) <= 1e-12 * np.max(np.abs(mu0))
+    finally:
+        gpu_ctx.set_tuning(**saved)
+    M.close()
+    K.close()

@@ -126,6 +126,7 @@ class tbk_tuning(C.Structure):
         ("chunk_rows", C.c_int32),
         ("chunk_period", C.c_int32),
         ("row_lane_bytes", C.c_int32),
+        ("row_slab_cols", C.c_int32),
     ]
 
 

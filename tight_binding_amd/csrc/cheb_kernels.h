@@ -524,16 +524,22 @@ __global__ __launch_bounds__(kBlock) void cheb_step_kernel(
     int64_t first, end, stride;
     tile_range(ntiles, xcd_split, first, end, stride);
 
+    // a sweep covers LPR * CPL columns; with Rp wider, slab s of the visiting
+    // order is columns [s LPR CPL, (s + 1) LPR CPL), i.e. lanes shifted by
+    // s LPR CPL / UE units of UE values
+    using V2 = typename Vec2<VT>::type;
+    constexpr int UE = lane_unit_elems<V2, CPL>();
+    constexpr int kSlabUnits = LPR * CPL / UE;
     double p0 = 0.0, p1 = 0.0;
     for (int64_t t = first; t < end; t += stride) {
-        uint32_t slab_unused;
-        const int64_t rt = visit_row_tile(order, static_cast<uint32_t>(t), slab_unused);
+        uint32_t slab;
+        const int64_t rt = visit_row_tile(order, static_cast<uint32_t>(t), slab);
         if (rt < 0) continue;
         int r = static_cast<int>(rt * TR) + wave * RPW + sub;
         if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
         if (r >= rows) continue;
-        step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q, Rp, alpha, beta, opts, p0, p1, hv2, y2,
-                                         y3, wbs);
+        step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q + static_cast<int>(slab) * kSlabUnits, Rp,
+                                         alpha, beta, opts, p0, p1, hv2, y2, y3, wbs);
     }
 
     // deterministic block reduction: butterfly in the wave, then waves in order

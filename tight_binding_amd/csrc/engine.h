@@ -163,7 +163,7 @@ struct StepPlan {
 void launch_any(int precision, bool cplx, int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st,
                 const StepArgs& a);
 KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int Rp, int max_row_nnz,
-                           int bandwidth, bool force_row);
+                           int bandwidth, bool force_row, bool row_slabs = false);
 int bandwidth_percentile(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, double p);
 void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_t>& ci, std::vector<double>& val);
 void matrix_gershgorin(const tbk_matrix* M, hipStream_t st);
@@ -185,6 +185,10 @@ void matrix_fp32(tbk_matrix* m);
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row);
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row,
                    const tbk_tuning& tu);
+// row_slabs: the row kernel may sweep tu.row_slab_cols columns at a time (the
+// moment recursions; the Kubo stack modes store whole rows)
+StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row,
+                   const tbk_tuning& tu, bool row_slabs);
 void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, double beta, const void* x,
                  const void* z, void* y, const void* bra, double2* partial, const void* hv2, void* y2);
 // y3: the paired modes' right-half words; wbs: block stride of k-blocked

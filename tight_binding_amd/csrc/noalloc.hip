@@ -103,7 +103,7 @@ extern "C" int tbk_moments_noalloc(tbk_context* ctx, const tbk_matrix* M, double
             raise(TBK_E_UNSUPPORTED, "more than 32768 recursion steps (n > 65536 with doubling, n > 32769 plain)");
         set_device(ctx);
         const StepPlan plan = make_plan(ctx, M, kets->precision, kets->Rp, kets->rows, b,
-                                        (flags & TBK_FLAG_ROW_KERNEL) != 0);
+                                        (flags & TBK_FLAG_ROW_KERNEL) != 0, ctx->tuning, true);
         const int64_t nb = plan.nb;
         if (nb > kScratchMaxWorkgroups) raise(TBK_E_UNSUPPORTED, "step launch wider than 4096 workgroups");
         double2* ring = scratch_on(ctx->device);

@@ -45,7 +45,7 @@ void launch_any(int precision, bool cplx, int width, int kch, int cpl, int mode,
 }
 
 KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int Rp, int max_row_nnz,
-                           int bandwidth, bool force_row) {
+                           int bandwidth, bool force_row, bool row_slabs) {
     const int hbytes = (precision == TBK_FP64 ? 8 : 4) * (cplx ? 2 : 1);
     const int ebytes = precision == TBK_FP64 ? 16 : 8;
     int kch = max_row_nnz <= 4 ? 1 : max_row_nnz <= 8 ? 2 : max_row_nnz <= 16 ? 4 : 0;
@@ -74,13 +74,19 @@ KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int R
         // complex64 per lane at R = 8..64, and wider lanes lose in fp64.  A
         // width in bytes means the same in both precisions, so a tuning found
         // on one carries over to the other.
-        const int cmin = Rp > tbk::kWave ? Rp / tbk::kWave : 1;
+        // A sweep covers W columns: Rp, or row_slab_cols for the moment
+        // recursions, whose grid then visits every row once per sweep.
+        int W = Rp;
+        if (row_slabs && tu.row_slab_cols >= 2 && tu.row_slab_cols < Rp &&
+            (tu.row_slab_cols & (tu.row_slab_cols - 1)) == 0)
+            W = tu.row_slab_cols;
+        const int cmin = W > tbk::kWave ? W / tbk::kWave : 1;
         const int lane_bytes = tu.row_lane_bytes > 0 ? tu.row_lane_bytes : 16;
         int cpl = std::max({cmin, 1, lane_bytes / ebytes});
         if (cpl != 1 && cpl != 2 && cpl != 4) cpl = cmin;
-        if (cpl > Rp || Rp % cpl != 0 || (cpl > cmin && Rp * 8 + cpl > 128 * 8 + 4)) cpl = cmin;
-        const int lpr = Rp / cpl;
-        return {0, Rp, tbk::kWavesPerBlock * (tbk::kWave / lpr), cpl};
+        if (cpl > W || W % cpl != 0 || (cpl > cmin && W * 8 + cpl > 128 * 8 + 4)) cpl = cmin;
+        const int lpr = W / cpl;
+        return {0, W, tbk::kWavesPerBlock * (tbk::kWave / lpr), cpl};
     }
     const int lpr = std::min(sw, tbk::kWave);
     const int tr = tbk::tile_rows(lpr, sw / lpr, kch, hbytes);
@@ -227,13 +233,13 @@ void matrix_fp32(tbk_matrix* m) {
 }
 
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row,
-                   const tbk_tuning& tu) {
+                   const tbk_tuning& tu, bool row_slabs) {
     StepPlan pl{};
     pl.precision = prec;
     pl.cplx = M->is_complex;
     if (prec == TBK_FP32) matrix_fp32(const_cast<tbk_matrix*>(M));
     pl.kc = choose_kernel(tu, prec, M->is_complex, Rp, M->max_row_nnz, M->bandwidth95,
-                          force_row || tu.use_tiled <= 0);
+                          force_row || tu.use_tiled <= 0, row_slabs);
     const KernelChoice& kc = pl.kc;
     const int TR = kc.rows_per_tile;
     const int64_t row_tiles = (static_cast<int64_t>(rows) + TR - 1) / TR;
@@ -281,7 +287,12 @@ StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int 
 }
 
 StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row) {
-    return make_plan(ctx, M, prec, Rp, rows, b, force_row, ctx->tuning);
+    return make_plan(ctx, M, prec, Rp, rows, b, force_row, ctx->tuning, false);
+}
+
+StepPlan make_plan(tbk_context* ctx, const tbk_matrix* M, int prec, int Rp, int rows, double b, bool force_row,
+                   const tbk_tuning& tu) {
+    return make_plan(ctx, M, prec, Rp, rows, b, force_row, tu, false);
 }
 
 // One step: y = alpha*(M x) + beta*x - z (mode kFirst: no z), partial sums of
@@ -347,7 +358,7 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
     const tbk_block* bra = bras ? bras : kets;
     const int64_t steps = doubling ? (n + 1) / 2 : (n - 1);
 
-    const StepPlan plan = make_plan(ctx, M, prec, Rp, rows, b, (flags & TBK_FLAG_ROW_KERNEL) != 0);
+    const StepPlan plan = make_plan(ctx, M, prec, Rp, rows, b, (flags & TBK_FLAG_ROW_KERNEL) != 0, ctx->tuning, true);
     const KernelChoice& kc = plan.kc;
     const int TR = kc.rows_per_tile;
     const tbk::TileOrder order = plan.args.order;
@@ -1361,11 +1372,15 @@ int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
             // fp32 work sharing the context
             {&tbk_tuning::row_lane_bytes,
              kets->precision == TBK_FP64 ? std::vector<int32_t>{16, 32} : std::vector<int32_t>{8, 16, 32}},
+            // column sweeps of the row kernel: a sweep's gather window shrinks
+            // with its width (widths below the padded R only)
+            {&tbk_tuning::row_slab_cols, {0, 16, 32, 64}},
         };
         for (int pass = 0; pass < 2; ++pass) {
             for (const auto& k : knobs) {
                 for (int32_t v : k.values) {
                     if (cur.*(k.field) == v) continue;
+                    if (k.field == &tbk_tuning::row_slab_cols && v >= kets->Rp) continue;
                     tbk_tuning t = cur;
                     t.*(k.field) = v;
                     const double ms = trial(t);
