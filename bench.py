@@ -30,14 +30,31 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per step of the default workload (1024 x 1024 cells, R = 64),
-# measured with rocprofv3 --pmc over the 16 timed steps of a short run of this
-# script: TCC_EA0_RDREQ_DRAM_32B and TCC_EA0_WRREQ_WRITE_DRAM_32B count
-# 32-byte sectors and read 1.000x the traffic of stream kernels of known size
-# (profiles/cheb_step_fp64_honeycomb2M_R64_pmc_counters.md).  The figure is
-# for the launch shape the tuner picked in that run; other shapes read
-# 4.4-7.3 GB per step.
-STEP_TRAFFIC_BYTES = {(1024, 64): 7.5821e9}
+# DRAM bytes per step of the default workload (1024 x 1024 cells, R = 64)
+# from rocprofv3 --pmc: TCC_EA0_RDREQ_DRAM_32B and TCC_EA0_WRREQ_WRITE_DRAM_32B
+# count 32-byte sectors and read 1.000x the traffic of stream kernels of known
+# size.  The figure holds for the launch shape of the profiled run only (other
+# shapes read 4.4-7.3 GB per step), so it is reported only when the tuner
+# installs that shape: (bytes, profiled shape, profile summary).
+STEP_TRAFFIC = {
+    (1024, 64): (7.5821e9,
+                 {"blocks_per_cu": (0, 4), "chunk_rows": (0, 256), "chunk_period": (0,), "xcd_split": (0,),
+                  "stream_nt": (0, 1), "use_tiled": (0,), "row_lane_bytes": (0, 16), "row_slab_cols": (0,)},
+                 "profiles/cheb_step_fp64_honeycomb2M_R64_pmc_counters.md"),
+}
+
+
+def step_traffic(cells, R, tuning):
+    """(bytes, source) of the profiled DRAM traffic when `tuning` is the
+    profiled launch shape, else (None, reason)."""
+    entry = STEP_TRAFFIC.get((cells, R))
+    if entry is None:
+        return None, None
+    nbytes, shape, source = entry
+    t = tuning or {}
+    if all(t.get(k, 0) in allowed for k, allowed in shape.items()):
+        return nbytes, "rocprofv3 --pmc DRAM sector counters of this workload at this launch shape (%s)" % source
+    return None, "not profiled at this launch shape (%s holds the default shape)" % source
 
 
 def log(msg):
@@ -704,11 +721,14 @@ def main():
     ctx.moments(M, K, 2 * args.warmup, a=a)
     ctx.synchronize()
 
-    # timed: exactly K steps (2K moments, doubling path)
+    # timed: exactly K steps (2K moments, doubling path).  One HIP event pair
+    # on the library's stream brackets the K step launches, so kernel_ms / K
+    # is the mean launch-to-launch time of a step (per-launch event pairs
+    # would add each launch's dispatch delay to its time).
     n_mom = 2 * args.steps
     ctx.synchronize()
     t0 = time.perf_counter()
-    mu, st, _ = ctx.moments(M, K, n_mom, a=a, time_launches=True)
+    mu, st, _ = ctx.moments(M, K, n_mom, a=a)
     ctx.synchronize()
     wall = time.perf_counter() - t0
     assert st["steps"] == args.steps
@@ -721,17 +741,29 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        mu_p, stp, _ = ctx.moments(M, K, args.steps + 1, a=a, plain=True, time_launches=True)
+        # one moment per step (distinct bras in general, k = 4 blocks per
+        # step), with a launch shape tuned for that mode
+        saved = ctx.get_tuning()
+        tuning_plain = None
+        if not args.no_autotune:
+            tuning_plain, _ = ctx.autotune(M, K, a=a, steps=16, plain=True)
+        ctx.moments(M, K, 8, a=a, plain=True)
+        mu_p, stp, _ = ctx.moments(M, K, args.steps + 1, a=a, plain=True)
+        ctx.set_tuning(**saved)
+        gbs_p = stp["bytes_total"] / (stp["kernel_ms"] * 1e-3) / 1e9
         extra["plain_fp64"] = {
             "steps_per_s": stp["steps"] / (stp["kernel_ms"] * 1e-3),
-            "GBps": stp["bytes_total"] / (stp["kernel_ms"] * 1e-3) / 1e9,
+            "GBps": gbs_p,
+            "frac_hbm": gbs_p / HBM_PEAK_GBS,
             "bytes_per_step": stp["bytes_per_step"],
+            "launch_tuning": tuning_plain,
         }
         K32 = ctx.random_block(H.n, args.R, seed=7, precision=kpm.FP32)
         saved = ctx.get_tuning()
         if not args.no_autotune:
             ctx.autotune(M, K32, a=a, steps=16)
-        mu32, st32, _ = ctx.moments(M, K32, n_mom, a=a, time_launches=True)
+        ctx.moments(M, K32, 8, a=a)
+        mu32, st32, _ = ctx.moments(M, K32, n_mom, a=a)
         ctx.set_tuning(**saved)
         extra["fp32_recursion"] = {
             "steps_per_s": st32["steps"] / (st32["kernel_ms"] * 1e-3),
@@ -796,9 +828,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": STEP_TRAFFIC_BYTES.get((args.cells, args.R)),
-            "traffic_source": ("rocprofv3 --pmc DRAM sector counters, median of 16 timed steps of the same workload"
-                               if (args.cells, args.R) in STEP_TRAFFIC_BYTES else None),
+            "traffic": step_traffic(args.cells, args.R, tuning)[0],
+            "traffic_source": step_traffic(args.cells, args.R, tuning)[1],
             "bytes_per_step": st["bytes_per_step"],
         },
         "cpu_baseline": cpu,

@@ -316,6 +316,24 @@ int tbk_cheb_seq_on_bases(tbk_context* ctx, const tbk_matrix* M, double a, doubl
  */
 int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
                  int32_t steps, tbk_tuning* best, double* best_ms_per_step);
+/* flags of tbk_autotune_flags, besides TBK_FLAG_PLAIN (tune the
+   one-moment-per-step recursion instead of the two-moments-per-step one) */
+enum { TBK_AUTOTUNE_REFRESH = 8 /* run the trials even when the cache holds this problem's shape */ };
+/*
+ * tbk_autotune for one recursion mode, with a process-wide cache: the shape
+ * found for a problem is kept under the matrix's rows, entries, longest row
+ * and 95th-percentile bandwidth, the padded R, precision, value type, mode
+ * and the device's CU count, and a later call with the same key installs it
+ * without trials (best_ms_per_step then is the time measured when it was
+ * found).  tbk_autotune is this call with flags 0.
+ */
+int tbk_autotune_flags(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
+                       int32_t steps, int32_t flags, tbk_tuning* best, double* best_ms_per_step);
+/* the tuning cache as a text file (one line per problem shape), read back by
+   a later process; entries (optional) receives the number of lines read */
+int tbk_tuning_cache_save(const char* path);
+int tbk_tuning_cache_load(const char* path, int32_t* entries);
+int tbk_tuning_cache_clear(void);
 
 /* ----------------------------------------------- Kubo-Bastin Gamma_nm */
 enum {

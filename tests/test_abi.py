@@ -51,3 +51,35 @@ def test_host_errors_are_codes_not_exceptions():
     c = L.UnitCell.model("square")
     with pytest.raises(N.NativeError):
         c.add_hopping("A", "Z", (0, 1), 1.0)
+
+
+def test_tuning_cache_file_round_trip(tmp_path):
+    """The autotuner cache's text form: load, save, load again (host only)."""
+    from tight_binding_amd import _native as N
+    from tight_binding_amd import kpm
+
+    nf = len(N.tbk_tuning._fields_)
+    src = tmp_path / "tuning.txt"
+    lines = ["tbkpm-tuning 1 %d" % nf,
+             # num_cu rows max_row_nnz bandwidth95 Rp precision complex plain nnz, tuning, ms
+             "256 2097152 4 2048 64 0 0 0 8388608 " + " ".join(["0"] * nf) + " 1.165",
+             "256 262144 5 512 32 0 1 0 1308672 " + " ".join(str(i % 3) for i in range(nf)) + " 0.0381"]
+    src.write_text("\n".join(lines) + "\n")
+    kpm.tuning_cache_clear()
+    assert kpm.tuning_cache_load(str(src)) == 2
+    out = tmp_path / "saved.txt"
+    kpm.tuning_cache_save(str(out))
+    kpm.tuning_cache_clear()
+    assert kpm.tuning_cache_load(str(out)) == 2
+    again = tmp_path / "again.txt"
+    kpm.tuning_cache_save(str(again))
+    assert out.read_text() == again.read_text()
+    kpm.tuning_cache_clear()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("tbkpm-tuning 1 %d\n1 2 3\n" % nf)
+    with pytest.raises(N.NativeError):
+        kpm.tuning_cache_load(str(bad))
+    bad.write_text("tbkpm-tuning 1 %d\n" % (nf + 1))
+    with pytest.raises(N.NativeError):
+        kpm.tuning_cache_load(str(bad))
+    kpm.tuning_cache_clear()

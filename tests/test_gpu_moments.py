@@ -371,3 +371,45 @@ def test_row_slabs_give_the_same_vectors(gpu_ctx, prec, R, slab, cplx):
         gpu_ctx.set_tuning(**saved)
     M.close()
     K.close()
+
+
+def test_autotune_cache_and_plain_mode(gpu_ctx, tmp_path):
+    """A second autotune of the same problem shape is served from the cache
+    (same tuning, no trials); refresh runs the trials again; the
+    one-moment-per-step mode has its own entry; a saved cache read back into
+    an empty one gives the same entries."""
+    import time
+
+    p, _ = build_pair("honeycomb", (64, 64), [("A", 1, -2.7)], W=0.5, seed=1)
+    M = gpu_ctx.matrix(p)
+    K = gpu_ctx.random_block(p.n, 16, seed=3)
+    saved = gpu_ctx.get_tuning()
+    try:
+        kpm.tuning_cache_clear()
+        t0 = time.perf_counter()
+        best, ms = gpu_ctx.autotune(M, K, a=8.5, steps=4)
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        best2, ms2 = gpu_ctx.autotune(M, K, a=8.5, steps=4)
+        second = time.perf_counter() - t0
+        assert best2 == best and ms2 == ms and second < first
+        best_p, ms_p = gpu_ctx.autotune(M, K, a=8.5, steps=4, plain=True)
+        assert ms_p > 0
+        _, ms3 = gpu_ctx.autotune(M, K, a=8.5, steps=4, refresh=True)
+        assert ms3 > 0
+        path = tmp_path / "cache.txt"
+        kpm.tuning_cache_save(str(path))
+        kpm.tuning_cache_clear()
+        assert kpm.tuning_cache_load(str(path)) == 2
+        best4, _ = gpu_ctx.autotune(M, K, a=8.5, steps=4, plain=True)
+        assert best4 == best_p
+        # the installed tuning gives the moments of the default shape
+        ref, _, _ = gpu_ctx.moments(M, K, 64, a=8.5, plain=True)
+        gpu_ctx.set_tuning(**saved)
+        mu, _, _ = gpu_ctx.moments(M, K, 64, a=8.5, plain=True)
+        assert_moments_close(mu, ref, tol=1e-12)
+    finally:
+        gpu_ctx.set_tuning(**saved)
+        kpm.tuning_cache_clear()
+    M.close()
+    K.close()

@@ -22,6 +22,7 @@ TBK_FP32 = 1
 TBK_CSC = 0
 TBK_CSR = 1
 TBK_FLAG_PLAIN = 1
+TBK_AUTOTUNE_REFRESH = 8
 TBK_FLAG_TIME_LAUNCHES = 2
 TBK_FLAG_ROW_KERNEL = 4
 
@@ -293,6 +294,10 @@ KPM_SIGNATURES = {
     "tbk_moments_noalloc": ([P, P, D, D, P, P, I32, I32, P, P, P, P], I32),
     "tbk_block_device_ptr": ([P, C.POINTER(C.c_void_p)], I32),
     "tbk_autotune": ([P, P, D, D, P, I32, C.POINTER(tbk_tuning), DP], I32),
+    "tbk_autotune_flags": ([P, P, D, D, P, I32, I32, C.POINTER(tbk_tuning), DP], I32),
+    "tbk_tuning_cache_save": ([C.c_char_p], I32),
+    "tbk_tuning_cache_load": ([C.c_char_p, C.POINTER(C.c_int32)], I32),
+    "tbk_tuning_cache_clear": ([], I32),
     "tbk_kubo_gamma": ([P, P, D, D, P, P, P, I32, I32, DP, DP, C.POINTER(tbk_kubo_stats)], I32),
     "tbk_stack_contract": ([P, DP, DP, I32, I64, I32, DP, DP], I32),
     "tbk_kubo_zeta": ([P, DP, I32, DP, I32, D, DP, DP], I32),

@@ -358,7 +358,10 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         xs[c].y = 0;
     }
     if (!FIRST) ld_row_stream<LPR, CPL>(z + own, zv, opts);
-    if (FIRST || MODE == kRec) ld_row_stream<LPR, CPL>(bra + own, bv, opts);
+    // the first step with bra = kets (two moments per step) takes the bra row
+    // from the gathered own row instead of reading the block a second time
+    const bool bra_is_x = FIRST && bra == x;
+    if ((FIRST || MODE == kRec) && !bra_is_x) ld_row_stream<LPR, CPL>(bra + own, bv, opts);
     const int s = rp[r], e = rp[r + 1];
     bool have_self = false;
     int p = s;
@@ -410,6 +413,10 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         }
     }
     if (!have_self) ld_row<LPR, CPL>(x + own, xs);
+    if (bra_is_x) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) bv[c] = xs[c];
+    }
 
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {

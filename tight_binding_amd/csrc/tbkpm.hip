@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -332,6 +333,106 @@ void launch_plan(const StepPlan& pl, hipStream_t st, int mode, double alpha, dou
 namespace {
 
 using namespace engine;
+
+// ------------------------------------------------------- tuning cache
+// Tuned launch shapes by problem statistics, process-wide, optionally saved
+// to and loaded from a text file (one entry per line: the key's fields, then
+// the tuning's fields, then the time per step in ms).
+struct TuningKey {
+    int32_t num_cu, rows, max_row_nnz, bandwidth95, Rp, precision, cplx, plain;
+    int64_t nnz;
+    bool operator==(const TuningKey& o) const {
+        return num_cu == o.num_cu && rows == o.rows && max_row_nnz == o.max_row_nnz &&
+               bandwidth95 == o.bandwidth95 && Rp == o.Rp && precision == o.precision && cplx == o.cplx &&
+               plain == o.plain && nnz == o.nnz;
+    }
+};
+struct TuningEntry {
+    tbk_tuning tuning;
+    double ms_per_step;
+};
+std::mutex g_tuning_mutex;
+std::vector<std::pair<TuningKey, TuningEntry>> g_tuning_cache;
+
+TuningKey tuning_key(const tbk_context* ctx, const tbk_matrix* M, const tbk_block* kets, bool plain) {
+    return TuningKey{ctx->num_cu, M->rows, M->max_row_nnz, M->bandwidth95, kets->Rp, kets->precision,
+                     M->is_complex ? 1 : 0, plain ? 1 : 0, M->nnz};
+}
+
+bool tuning_cache_find(const TuningKey& k, TuningEntry& out) {
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    for (const auto& e : g_tuning_cache)
+        if (e.first == k) {
+            out = e.second;
+            return true;
+        }
+    return false;
+}
+
+void tuning_cache_store(const TuningKey& k, const TuningEntry& v) {
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    for (auto& e : g_tuning_cache)
+        if (e.first == k) {
+            e.second = v;
+            return;
+        }
+    g_tuning_cache.emplace_back(k, v);
+}
+
+void tuning_cache_clear() {
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    g_tuning_cache.clear();
+}
+
+constexpr int kTuningFields = static_cast<int>(sizeof(tbk_tuning) / sizeof(int32_t));
+
+void tuning_cache_save(const char* path) {
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    FILE* f = std::fopen(path, "w");
+    if (!f) raise(TBK_E_INVALID, std::string("cannot write ") + path);
+    std::fprintf(f, "tbkpm-tuning 1 %d\n", kTuningFields);
+    for (const auto& e : g_tuning_cache) {
+        const TuningKey& k = e.first;
+        std::fprintf(f, "%d %d %d %d %d %d %d %d %lld", k.num_cu, k.rows, k.max_row_nnz, k.bandwidth95, k.Rp,
+                     k.precision, k.cplx, k.plain, static_cast<long long>(k.nnz));
+        const int32_t* t = reinterpret_cast<const int32_t*>(&e.second.tuning);
+        for (int i = 0; i < kTuningFields; ++i) std::fprintf(f, " %d", t[i]);
+        std::fprintf(f, " %.9g\n", e.second.ms_per_step);
+    }
+    const bool ok = std::fflush(f) == 0;
+    std::fclose(f);
+    if (!ok) raise(TBK_E_INVALID, std::string("cannot write ") + path);
+}
+
+int32_t tuning_cache_load(const char* path) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) raise(TBK_E_INVALID, std::string("cannot read ") + path);
+    struct Closer {
+        FILE* f;
+        ~Closer() { std::fclose(f); }
+    } closer{f};
+    int version = 0, fields = 0;
+    if (std::fscanf(f, "tbkpm-tuning %d %d", &version, &fields) != 2 || version != 1 || fields != kTuningFields)
+        raise(TBK_E_INVALID, std::string(path) + ": not a tuning cache of this library version");
+    int32_t count = 0;
+    for (;;) {
+        TuningKey k{};
+        long long nnz = 0;
+        const int got = std::fscanf(f, "%d %d %d %d %d %d %d %d %lld", &k.num_cu, &k.rows, &k.max_row_nnz,
+                                    &k.bandwidth95, &k.Rp, &k.precision, &k.cplx, &k.plain, &nnz);
+        if (got == EOF) break;
+        if (got != 9) raise(TBK_E_INVALID, std::string(path) + ": malformed entry");
+        k.nnz = nnz;
+        TuningEntry v{};
+        int32_t* t = reinterpret_cast<int32_t*>(&v.tuning);
+        for (int i = 0; i < kTuningFields; ++i)
+            if (std::fscanf(f, "%d", &t[i]) != 1) raise(TBK_E_INVALID, std::string(path) + ": malformed entry");
+        if (std::fscanf(f, "%lf", &v.ms_per_step) != 1) raise(TBK_E_INVALID, std::string(path) + ": malformed entry");
+        tuning_cache_store(k, v);
+        ++count;
+    }
+    return count;
+}
 
 void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* bras,
                  const tbk_block* kets, int32_t n, int32_t flags, double* mu_out, double* last_out,
@@ -1331,12 +1432,28 @@ int tbk_cheb_seq_on_bases(tbk_context* ctx, const tbk_matrix* M, double a, doubl
     });
 }
 
-int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
-                 int32_t steps, tbk_tuning* best, double* best_ms_per_step) {
+int tbk_autotune_flags(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
+                       int32_t steps, int32_t flags, tbk_tuning* best, double* best_ms_per_step) {
     return guarded([&] {
         if (!ctx || !M || !kets) raise(TBK_E_INVALID, "null argument");
         if (steps < 2) raise(TBK_E_INVALID, "autotune needs at least 2 steps per trial");
-        const int32_t n = 2 * steps;  // two moments per step (bras = kets)
+        if (flags & ~(TBK_FLAG_PLAIN | TBK_AUTOTUNE_REFRESH)) raise(TBK_E_INVALID, "unknown autotune flag");
+        const bool plain = (flags & TBK_FLAG_PLAIN) != 0;
+        // a shape tuned before for the same matrix statistics, width,
+        // precision and mode on this device is installed without trials
+        const TuningKey key = tuning_key(ctx, M, kets, plain);
+        if (!(flags & TBK_AUTOTUNE_REFRESH)) {
+            TuningEntry hit;
+            if (tuning_cache_find(key, hit)) {
+                ctx->tuning = hit.tuning;
+                if (best) *best = hit.tuning;
+                if (best_ms_per_step) *best_ms_per_step = hit.ms_per_step;
+                return;
+            }
+        }
+        // two moments per step (bras = kets), or one with TBK_FLAG_PLAIN
+        const int32_t n = plain ? steps + 1 : 2 * steps;
+        const int32_t mode = plain ? TBK_FLAG_PLAIN : 0;
         std::vector<double> mu(static_cast<size_t>(n));
         struct Restore {  // a failed trial leaves the caller's tuning in place
             tbk_context* c;
@@ -1349,8 +1466,8 @@ int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         auto trial = [&](const tbk_tuning& t) -> double {
             ctx->tuning = t;
             tbk_stats st{};
-            run_moments(ctx, M, a, b, nullptr, kets, 4, 0, mu.data(), nullptr, nullptr);  // warm
-            run_moments(ctx, M, a, b, nullptr, kets, n, TBK_FLAG_TIME_LAUNCHES, mu.data(), nullptr, &st);
+            run_moments(ctx, M, a, b, nullptr, kets, 4, mode, mu.data(), nullptr, nullptr);  // warm
+            run_moments(ctx, M, a, b, nullptr, kets, n, mode | TBK_FLAG_TIME_LAUNCHES, mu.data(), nullptr, &st);
             return st.kernel_ms / static_cast<double>(st.steps);
         };
         tbk_tuning cur = ctx->tuning;
@@ -1393,9 +1510,34 @@ int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         }
         ctx->tuning = cur;
         restore.done = true;
+        tuning_cache_store(key, TuningEntry{cur, cur_ms});
         if (best) *best = cur;
         if (best_ms_per_step) *best_ms_per_step = cur_ms;
     });
+}
+
+int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
+                 int32_t steps, tbk_tuning* best, double* best_ms_per_step) {
+    return tbk_autotune_flags(ctx, M, a, b, kets, steps, 0, best, best_ms_per_step);
+}
+
+int tbk_tuning_cache_save(const char* path) {
+    return guarded([&] {
+        if (!path) raise(TBK_E_INVALID, "null argument");
+        tuning_cache_save(path);
+    });
+}
+
+int tbk_tuning_cache_load(const char* path, int32_t* entries) {
+    return guarded([&] {
+        if (!path) raise(TBK_E_INVALID, "null argument");
+        const int32_t k = tuning_cache_load(path);
+        if (entries) *entries = k;
+    });
+}
+
+int tbk_tuning_cache_clear(void) {
+    return guarded([&] { tuning_cache_clear(); });
 }
 
 int tbk_kubo_gamma(tbk_context* ctx, const tbk_matrix* H, double a, double b, const tbk_matrix* vx,

@@ -149,13 +149,16 @@ class Context:
                          int(row_slab_cols))
         N.check_kpm(N.kpm().tbk_context_set_tuning(self._h, C.byref(t)))
 
-    def autotune(self, M: "Matrix", kets: "Block", a=1.0, b=0.0, steps=16):
-        """Search launch shapes for (M, kets); installs and returns the best
-        tuning and its time per step in ms."""
+    def autotune(self, M: "Matrix", kets: "Block", a=1.0, b=0.0, steps=16, plain=False, refresh=False):
+        """Search launch shapes for (M, kets) -- the one-moment-per-step
+        recursion with plain=True -- or take them from the process's tuning
+        cache (unless refresh); installs and returns the best tuning and its
+        time per step in ms."""
         t = N.tbk_tuning()
         ms = C.c_double()
-        N.check_kpm(N.kpm().tbk_autotune(self._h, M._h, float(a), float(b), kets._h, int(steps), C.byref(t),
-                                         C.byref(ms)))
+        flags = (N.TBK_FLAG_PLAIN if plain else 0) | (N.TBK_AUTOTUNE_REFRESH if refresh else 0)
+        N.check_kpm(N.kpm().tbk_autotune_flags(self._h, M._h, float(a), float(b), kets._h, int(steps), flags,
+                                               C.byref(t), C.byref(ms)))
         return {k: getattr(t, k) for k, _ in t._fields_}, ms.value
 
     def get_tuning(self) -> dict:
@@ -274,6 +277,22 @@ class Context:
 
 GAMMA_FP64 = N.TBK_GAMMA_FP64
 GAMMA_BF16X3 = N.TBK_GAMMA_BF16X3
+
+
+def tuning_cache_save(path: str):
+    """Write the process's autotuner cache (tbk_tuning_cache_save)."""
+    N.check_kpm(N.kpm().tbk_tuning_cache_save(str(path).encode()))
+
+
+def tuning_cache_load(path: str) -> int:
+    """Add a saved autotuner cache to the process's; returns its entry count."""
+    k = C.c_int32()
+    N.check_kpm(N.kpm().tbk_tuning_cache_load(str(path).encode(), C.byref(k)))
+    return k.value
+
+
+def tuning_cache_clear():
+    N.check_kpm(N.kpm().tbk_tuning_cache_clear())
 
 
 def kubo_gamma(ctx: Context, H: Matrix, vx: Matrix, vy: Matrix, kets: Block, n: int, a: float = 1.0,
