@@ -194,3 +194,38 @@ def test_presplit_kubo_request_matches_direct_path(gpu_ctx):
         assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
         (z, G), info = srv.take(t)
     assert np.array_equal(G, ref)
+
+
+@pytest.mark.parametrize("prec,tol", [(kpm.GAMMA_FP64, 1e-12), (kpm.GAMMA_BF16X3, 1e-5)])
+def test_kubo_request_over_budget_runs_in_vector_groups(gpu_ctx, prec, tol):
+    """A sigma_xy request whose stacks exceed the server's budget runs its
+    random vectors in groups (Gamma is a sum over vectors) instead of being
+    refused: its footprint drops below the budget, and Gamma and zeta equal
+    the whole-block result to rounding (the groups' Gammas are added in
+    group order).  R = 12 is not a multiple of the group width: the last
+    group is partial."""
+    Hs, vx, vy, g = landau_kubo(40, 64, 12)
+    N = g.size
+    with kpm.Server(0) as srv:
+        t = srv.submit_kubo(Hs, vx, vy, N, R=12, seed=31, precision=prec, g=g)
+        assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
+        (z_full, G_full), info_full = srv.take(t)
+    budget = int(0.6 * info_full["footprint_bytes"])
+    with kpm.Server(0, hbm_budget_bytes=budget) as srv:
+        t = srv.submit_kubo(Hs, vx, vy, N, R=12, seed=31, precision=prec, g=g)
+        info = srv.wait(t, 60.0)
+        assert info["state"] == Nn.TBK_REQ_DONE, info
+        assert info["footprint_bytes"] <= budget < info_full["footprint_bytes"]
+        (z, G), _ = srv.take(t)
+    assert np.max(np.abs(G - G_full)) <= tol * np.max(np.abs(G_full))
+    # zeta weights Gamma by up to (1 - e^2)^-2 at the outermost nodes; sigma
+    # uses the nodes inside the N/24 cut at each end (snippets.h:420-424)
+    cut = N // 24
+    zi, zf = z[cut:N - cut], z_full[cut:N - cut]
+    ztol = 1e-10 if prec == kpm.GAMMA_FP64 else 1e-4
+    assert np.max(np.abs(zi - zf)) <= ztol * np.max(np.abs(zf))
+    # the whole-block server result is the direct path's
+    if prec == kpm.GAMMA_FP64:
+        ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
+                                gpu_ctx.random_block(Hs.n, 12, seed=31), N, precision=prec, g=g)
+        assert np.array_equal(G_full, ref)

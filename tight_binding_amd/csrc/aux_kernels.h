@@ -44,11 +44,14 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
-// e^{i 2 pi u(seed, site*R + col)} / sqrt(rows); padding columns zero
+// Columns [col0, col0 + R) of the random-phase block of width Rtot:
+// e^{i 2 pi u(seed, site*Rtot + col0 + col)} / sqrt(rows), stored R wide in a
+// block padded to Rp columns (padding zero).  col0 = 0, Rtot = R is the
+// whole block; a narrower call gives exactly its columns of that block.
 template <typename V2>
 __global__ __launch_bounds__(kBlock) void random_phase_kernel(V2* __restrict__ out, int32_t rows,
                                                               int32_t R, int32_t Rp, uint64_t seed,
-                                                              double inv_sqrt_n) {
+                                                              double inv_sqrt_n, int32_t Rtot, int32_t col0) {
     const int64_t total = static_cast<int64_t>(rows) * Rp;
     const uint64_t key = splitmix64(seed);
     for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; idx < total;
@@ -57,7 +60,7 @@ __global__ __launch_bounds__(kBlock) void random_phase_kernel(V2* __restrict__ o
         const int col = static_cast<int>(idx - i * Rp);
         V2 v;
         if (col < R) {
-            const uint64_t h = splitmix64(key ^ static_cast<uint64_t>(i * R + col));
+            const uint64_t h = splitmix64(key ^ static_cast<uint64_t>(i * Rtot + col0 + col));
             const double u = static_cast<double>(h >> 11) * 0x1.0p-53;
             double sn, cs;
             sincos(2.0 * 3.141592653589793 * u, &sn, &cs);

@@ -143,10 +143,10 @@ struct tbk_batch {
             const double inv = 1.0 / std::sqrt(static_cast<double>(rows[i]));
             if (precision == TBK_FP64)
                 hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st,
-                                   reinterpret_cast<double2*>(dst), static_cast<int32_t>(rows[i]), R, Rp, seed[i], inv);
+                                   reinterpret_cast<double2*>(dst), static_cast<int32_t>(rows[i]), R, Rp, seed[i], inv, R, 0);
             else
                 hipLaunchKernelGGL(tbk::random_phase_kernel<float2>, dim3(grid), dim3(tbk::kBlock), 0, st,
-                                   reinterpret_cast<float2*>(dst), static_cast<int32_t>(rows[i]), R, Rp, seed[i], inv);
+                                   reinterpret_cast<float2*>(dst), static_cast<int32_t>(rows[i]), R, Rp, seed[i], inv, R, 0);
             HIPC(hipGetLastError());
         }
     }

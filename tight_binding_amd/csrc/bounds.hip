@@ -248,7 +248,7 @@ void spectral_bounds(tbk_context* ctx, const tbk_matrix* M, int max_iters, doubl
     {
         const int grid = static_cast<int>(std::min<int64_t>((static_cast<int64_t>(n) + tbk::kBlock - 1) / tbk::kBlock, 65536));
         hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st,
-                           vec[0].as<double2>(), n, 1, 1, seed, 1.0 / std::sqrt(static_cast<double>(n)));
+                           vec[0].as<double2>(), n, 1, 1, seed, 1.0 / std::sqrt(static_cast<double>(n)), 1, 0);
         HIPC(hipGetLastError());
     }
     const int32_t* rp = M->rp.as<const int32_t>();

@@ -137,6 +137,7 @@ struct Request {
     HostSparse H, vx, vy;      // a lattice request keeps only H.hdr's rows, is_complex and an nnz bound
     HostLattice lat;
     std::vector<double> g;
+    int32_t Gw = 0;            // sigma_xy: random vectors per group (a power of two <= Rp; see kubo_footprint)
     int64_t footprint = 0;
     int32_t state = TBK_REQ_QUEUED;
     int32_t error_code = 0;
@@ -161,7 +162,7 @@ struct Slot {
 struct KuboJob {
     Request* req = nullptr;
     tbk_matrix *H = nullptr, *vx = nullptr, *vy = nullptr;
-    DevBuf kets, L, Rs, W0, W1, B0, B1, S, part, P, G, g, e, z;
+    DevBuf kets, L, Rs, W0, W1, B0, B1, S, part, P, G, Gacc, g, e, z;
     hipEvent_t done = nullptr;
     int32_t pin = -1;
     ~KuboJob() {
@@ -237,6 +238,7 @@ struct tbk_server {
 
     int64_t dos_footprint(const Request& r) const;
     int64_t kubo_footprint(const Request& r) const;
+    void size_kubo(Request& r) const;
     void run();
     void run_prep();
     std::vector<std::pair<Request*, int32_t>> pick_admissions();
@@ -254,6 +256,24 @@ struct tbk_server {
 };
 
 namespace {
+
+// acc = g (first) or acc + g, over n complex values
+__global__ __launch_bounds__(256) void gamma_accumulate_kernel(double2* __restrict__ acc,
+                                                               const double2* __restrict__ g, int64_t n,
+                                                               int32_t first) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * 256) {
+        const double2 v = g[i];
+        if (first) {
+            acc[i] = v;
+        } else {
+            double2 a = acc[i];
+            a.x = a.x + v.x;
+            a.y = a.y + v.y;
+            acc[i] = a;
+        }
+    }
+}
 
 int64_t mat_bytes(const HostSparse& m, bool fp32_copy) {
     const int64_t vw = m.hdr.is_complex ? 2 : 1;
@@ -276,15 +296,18 @@ int64_t tbk_server::dos_footprint(const Request& r) const {
     return mat_bytes(r.H, r.precision == TBK_FP32) + vec + static_cast<int64_t>(r.n) * 8;
 }
 
+// Device memory of a sigma_xy request whose random vectors run in groups of
+// r.Gw: Gamma is a sum over vectors, so the two N-entry stacks only ever hold
+// one group's columns, and the groups' Gammas are added up.
 int64_t tbk_server::kubo_footprint(const Request& r) const {
     const int64_t rows = r.H.hdr.rows;
-    const int64_t blk = rows * r.Rp * 16;
+    const int64_t blk = rows * r.Gw * 16;
     const int64_t N = r.n;
-    const ContractionShape cs = contraction_plan(ctx, r.n, rows * r.Rp, r.precision);
+    const ContractionShape cs = contraction_plan(ctx, r.n, rows * r.Gw, r.precision);
     // pre-split stacks: half-size entries and three more rolling blocks
     const bool ps = kubo_presplit(r.precision, r.n, r.H.hdr.is_complex != 0, r.vx.hdr.is_complex != 0);
     // one stack entry: fp64 at the stack stride, words in whole word blocks
-    const int64_t K = rows * r.Rp;
+    const int64_t K = rows * r.Gw;
     const int64_t entry = ps ? ((K + tbk::kWordBlock - 1) / tbk::kWordBlock) * tbk::kWordBlock * 8 : stack_stride(K) * 16;
     const int64_t stacks = 2 * N * entry, rolling = ps ? 5 * blk : 2 * blk;
     // a lattice request also holds the assembled H while H~ is formed
@@ -293,7 +316,20 @@ int64_t tbk_server::kubo_footprint(const Request& r) const {
                                        : mat_bytes(r.H, false) + mat_bytes(r.vx, false) + mat_bytes(r.vy, false);
     return mats + blk /* kets */ +
            stacks + rolling + static_cast<int64_t>(ctx->num_cu) * 32 * 16 /* step partials */ +
-           static_cast<int64_t>(cs.nchunks) * N * N * 16 + N * N * 16 + N * 8 * 2 + N * 16;
+           static_cast<int64_t>(cs.nchunks) * N * N * 16 + 2 * N * N * 16 /* Gamma, its sum */ + N * 8 * 2 + N * 16;
+}
+
+// The widest group of random vectors (a power of two, at most the padded R)
+// whose footprint fits the budget, set with its footprint; the choice
+// depends on the request and the budget only, so a request's groups and
+// therefore its result are the same whatever else the server runs.
+void tbk_server::size_kubo(Request& r) const {
+    r.Gw = r.Rp;
+    r.footprint = kubo_footprint(r);
+    while (r.footprint > budget && r.Gw > 1) {
+        r.Gw /= 2;
+        r.footprint = kubo_footprint(r);
+    }
 }
 
 hipEvent_t tbk_server::get_event() {
@@ -430,10 +466,10 @@ Slot* tbk_server::start_dos(Request* r, int32_t pin) {
     const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
     if (r->precision == TBK_FP64)
         hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
-                           s->blk[0].as<double2>(), static_cast<int32_t>(rows), r->R, r->Rp, r->seed, inv);
+                           s->blk[0].as<double2>(), static_cast<int32_t>(rows), r->R, r->Rp, r->seed, inv, r->R, 0);
     else
         hipLaunchKernelGGL(tbk::random_phase_kernel<float2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
-                           s->blk[0].as<float2>(), static_cast<int32_t>(rows), r->R, r->Rp, r->seed, inv);
+                           s->blk[0].as<float2>(), static_cast<int32_t>(rows), r->R, r->Rp, r->seed, inv, r->R, 0);
     HIPC(hipGetLastError());
     return s.release();
 }
@@ -460,7 +496,10 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
         if (rc == TBK_OK) rc = tbk_matrix_create(ctx, &r->vy.hdr, &j->vy);
         if (rc != TBK_OK) raise(rc, g_last_error);
     }
-    const int rows = r->H.hdr.rows, Rp = r->Rp, N = r->n;
+    // the R random vectors run in groups of Gw columns (the whole block when
+    // it fits): Rp below is a group's padded width
+    const int rows = r->H.hdr.rows, Rp = r->Gw, N = r->n;
+    const int ngroups = (r->R + Rp - 1) / Rp;
     const int64_t K = static_cast<int64_t>(rows) * Rp;
     const size_t blk = static_cast<size_t>(K) * sizeof(double2);
     const StepPlan pH = make_plan(ctx, j->H, TBK_FP64, Rp, rows, b, false);
@@ -501,6 +540,7 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     const int64_t NN = static_cast<int64_t>(N) * N;
     j->P.reserve(static_cast<size_t>(cs.nchunks) * NN * sizeof(double2));
     j->G.reserve(static_cast<size_t>(NN) * sizeof(double2));
+    if (ngroups > 1) j->Gacc.reserve(static_cast<size_t>(NN) * sizeof(double2));
     j->g.reserve(static_cast<size_t>(N) * sizeof(double));
     j->e.reserve(static_cast<size_t>(N) * sizeof(double));
     j->z.reserve(static_cast<size_t>(N) * sizeof(double2));
@@ -516,31 +556,43 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     HIPC(hipMemcpyAsync(j->e.p, pg + N, static_cast<size_t>(N) * 8, hipMemcpyHostToDevice, st_));
     const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>((K + tbk::kBlock - 1) / tbk::kBlock, 65536));
-    hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st_,
-                       j->kets.as<double2>(), rows, r->R, Rp, r->seed, inv);
-    HIPC(hipGetLastError());
-    if (paired) {
-        enqueue_kubo_stacks_paired(pP, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->W0.p, j->W1.p,
-                                   j->S.p, rows, Rp, N, j->part.as<double2>(), st_, vxv);
-        enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, j->P.as<double2>(),
-                                     j->G.as<double2>(), st_);
-    } else if (presplit) {
-        enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->B0.p,
-                                     j->B1.p, j->S.p, j->W0.p, j->W1.p, K, N, j->part.as<double2>(), st_, vxv);
-        enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K, j->P.as<double2>(),
-                                     j->G.as<double2>(), st_);
-    } else {
-        enqueue_kubo_stacks(pH, pX, pY, a, b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, ldk, N,
-                            j->part.as<double2>(), st_, vxv);
-        enqueue_contraction(cs, j->L.as<const double2>(), j->Rs.as<const double2>(), N, K, ldk, r->precision,
-                            j->P.as<double2>(), j->G.as<double2>(), st_);
+    for (int grp = 0; grp < ngroups; ++grp) {
+        // columns [grp Gw, grp Gw + cols) of the request's R-wide random block
+        const int col0 = grp * Rp, cols = std::min(Rp, r->R - col0);
+        hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st_,
+                           j->kets.as<double2>(), rows, cols, Rp, r->seed, inv, r->R, col0);
+        HIPC(hipGetLastError());
+        if (paired) {
+            enqueue_kubo_stacks_paired(pP, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->W0.p,
+                                       j->W1.p, j->S.p, rows, Rp, N, j->part.as<double2>(), st_, vxv);
+            enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K,
+                                         j->P.as<double2>(), j->G.as<double2>(), st_);
+        } else if (presplit) {
+            enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->B0.p,
+                                         j->B1.p, j->S.p, j->W0.p, j->W1.p, K, N, j->part.as<double2>(), st_, vxv);
+            enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K,
+                                         j->P.as<double2>(), j->G.as<double2>(), st_);
+        } else {
+            enqueue_kubo_stacks(pH, pX, pY, a, b, j->kets.p, j->L.p, j->Rs.p, j->W0.p, j->W1.p, blk, ldk, N,
+                                j->part.as<double2>(), st_, vxv);
+            enqueue_contraction(cs, j->L.as<const double2>(), j->Rs.as<const double2>(), N, K, ldk, r->precision,
+                                j->P.as<double2>(), j->G.as<double2>(), st_);
+        }
+        if (ngroups > 1) {
+            // the groups' Gammas added in group order
+            const unsigned ag = static_cast<unsigned>(std::min<int64_t>((NN + 255) / 256, 4096));
+            hipLaunchKernelGGL(gamma_accumulate_kernel, dim3(ag), dim3(256), 0, st_, j->Gacc.as<double2>(),
+                               j->G.as<const double2>(), NN, grp == 0 ? 1 : 0);
+            HIPC(hipGetLastError());
+        }
     }
-    tbk::launch_gamma_weight(j->G.as<double2>(), j->g.as<const double>(), N, st_);
+    double2* Gsum = ngroups > 1 ? j->Gacc.as<double2>() : j->G.as<double2>();
+    tbk::launch_gamma_weight(Gsum, j->g.as<const double>(), N, st_);
     HIPC(hipGetLastError());
-    tbk::launch_kubo_zeta(j->G.as<const double2>(), N, j->e.as<const double>(), N, 1.0, j->z.as<double2>(), st_);
+    tbk::launch_kubo_zeta(Gsum, N, j->e.as<const double>(), N, 1.0, j->z.as<double2>(), st_);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(pz, j->z.p, static_cast<size_t>(N) * 16, hipMemcpyDeviceToHost, st_));
-    HIPC(hipMemcpyAsync(pg, j->G.p, static_cast<size_t>(NN) * 16, hipMemcpyDeviceToHost, st_));
+    HIPC(hipMemcpyAsync(pg, Gsum, static_cast<size_t>(NN) * 16, hipMemcpyDeviceToHost, st_));
     HIPC(hipEventCreateWithFlags(&j->done, hipEventDisableTiming));
     HIPC(hipEventRecord(j->done, st_));
     kjobs.push_back(j.release());
@@ -922,14 +974,14 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
             r->vx.hdr = r->vy.hdr = r->H.hdr;
             r->vx.hdr.nnz = r->vy.hdr.nnz = r->H.hdr.nnz + r->H.hdr.rows;
             if (q->g) r->g.assign(q->g, q->g + q->n_moments);
-            r->footprint = s->kubo_footprint(*r);
+            s->size_kubo(*r);
         } else if (q->kind == TBK_REQ_KUBO) {
             r->vx.copy_from(q->vx);
             r->vy.copy_from(q->vy);
             if (r->vx.hdr.rows != r->H.hdr.rows || r->vy.hdr.rows != r->H.hdr.rows)
                 raise(TBK_E_INVALID, "velocity operators and H differ in size");
             if (q->g) r->g.assign(q->g, q->g + q->n_moments);
-            r->footprint = s->kubo_footprint(*r);
+            s->size_kubo(*r);
         } else {
             r->footprint = s->dos_footprint(*r);
         }

@@ -1270,10 +1270,10 @@ int tbk_block_random_phase(tbk_context* ctx, int32_t rows, int32_t R, uint64_t s
         const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
         if (precision == TBK_FP64)
             hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
-                               b->data.as<double2>(), rows, R, b->Rp, seed, inv);
+                               b->data.as<double2>(), rows, R, b->Rp, seed, inv, R, 0);
         else
             hipLaunchKernelGGL(tbk::random_phase_kernel<float2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
-                               b->data.as<float2>(), rows, R, b->Rp, seed, inv);
+                               b->data.as<float2>(), rows, R, b->Rp, seed, inv, R, 0);
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(ctx->stream));
         ctx->held_bytes += static_cast<int64_t>(b->data.bytes);
