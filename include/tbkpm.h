@@ -99,6 +99,9 @@ typedef struct tbk_tuning {
     int32_t row_slab_cols;   /* moment recursions with the row kernel: sweep the rows this many
                                 vector columns at a time (a power of two below the padded R; 0:
                                 whole rows), so that the gathered rows of a sweep take less L2 */
+    int32_t kubo_stack_mib;  /* tbk_kubo_gamma: the most device memory (MiB) its vector stacks and
+                                work blocks may take; beyond it the random vectors run in groups
+                                (0: 95% of the free memory) */
 } tbk_tuning;
 
 const char* tbk_version(void);
@@ -348,7 +351,8 @@ typedef struct tbk_kubo_stats {
     double flops;          /* 8 N^2 K: real flops of the complex contraction, K = sites x columns */
     double stack_bytes;    /* bytes of the two stacks' entries (each entry padded to a multiple of 64 elements) */
     int32_t nchunks;       /* split-K chunks of the contraction */
-    int32_t reserved;
+    int32_t vector_groups; /* groups the random vectors ran in (1: the whole block; more when the
+                              whole block's stacks exceed the free device memory) */
     double recursion_bytes; /* algorithmic HBM bytes of the stack-building launches and copies */
 } tbk_kubo_stats;
 

@@ -331,3 +331,34 @@ def test_sigma_xy_e2h_device_path_matches_tknn_on_magnetic_torus(gpu_ctx):
                                       zeta=lambda G, e, v: kpm.kubo_zeta(gpu_ctx, G, e, v)[0])
     print("torus 16x16 flux 1/8: TKNN", exact, "KPM N=512", sig)
     assert np.all(np.abs(sig - exact) <= 0.03 * np.abs(exact)), (sig, exact)
+
+
+@pytest.mark.parametrize("prec,tol", [(kpm.GAMMA_FP64, 1e-12), (kpm.GAMMA_BF16X3, 1e-5)])
+def test_kubo_gamma_in_vector_groups_under_a_stack_cap(gpu_ctx, prec, tol):
+    """tbk_kubo_gamma with the stacks capped (kubo_stack_mib) below what the
+    whole 12-vector block needs: the vectors run in groups (the last one
+    partial) and Gamma equals the whole block's to rounding."""
+    Hs, vx, vy = landau_problem(48, 2 * math.pi / 16, 1.0, 5, 5.2)
+    kets = O.random_phase_kets(Hs.n, 12, 17)
+    N = 96
+    g = S.lorentz(N, 6.0)
+    M = [gpu_ctx.matrix(x) for x in (Hs, vx, vy)]
+    K = gpu_ctx.block(kets)
+    whole, st = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=prec, g=g)
+    assert st["vector_groups"] == 1
+    saved = gpu_ctx.get_tuning()
+    try:
+        # room for about a third of the whole block's stacks
+        gpu_ctx.set_tuning(**dict(saved, kubo_stack_mib=max(1, int(st["stack_bytes"] / 3 / 2**20))))
+        gpu_ctx.release_work()
+        grouped, stg = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=prec, g=g)
+    finally:
+        gpu_ctx.set_tuning(**saved)
+        gpu_ctx.release_work()
+    assert stg["vector_groups"] >= 3
+    assert np.max(np.abs(grouped - whole)) <= tol * np.max(np.abs(whole))
+    if prec == kpm.GAMMA_FP64:
+        ref = oracle_gamma(Hs, vx, vy, kets, N, g)
+        assert np.max(np.abs(grouped - ref)) <= 1e-10 * np.max(np.abs(ref))
+    for x in M + [K]:
+        x.close()

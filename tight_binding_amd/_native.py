@@ -128,6 +128,7 @@ class tbk_tuning(C.Structure):
         ("chunk_period", C.c_int32),
         ("row_lane_bytes", C.c_int32),
         ("row_slab_cols", C.c_int32),
+        ("kubo_stack_mib", C.c_int32),
     ]
 
 
@@ -139,7 +140,7 @@ class tbk_kubo_stats(C.Structure):
         ("flops", C.c_double),
         ("stack_bytes", C.c_double),
         ("nchunks", C.c_int32),
-        ("reserved", C.c_int32),
+        ("vector_groups", C.c_int32),
         ("recursion_bytes", C.c_double),
     ]
 

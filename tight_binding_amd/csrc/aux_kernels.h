@@ -74,6 +74,20 @@ __global__ __launch_bounds__(kBlock) void random_phase_kernel(V2* __restrict__ o
     }
 }
 
+// Columns [col0, col0 + cols) of a row-major rows x Rp complex128 block into a
+// rows x W block (padding columns zero).
+__global__ __launch_bounds__(kBlock) void column_group_kernel(const double2* __restrict__ in, int32_t rows,
+                                                              int32_t Rp, int32_t col0, int32_t cols, int32_t W,
+                                                              double2* __restrict__ out) {
+    const int64_t total = static_cast<int64_t>(rows) * W;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; idx < total;
+         idx += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const int64_t i = idx / W;
+        const int c = static_cast<int>(idx - i * W);
+        out[idx] = c < cols ? in[i * Rp + col0 + c] : make_double2(0.0, 0.0);
+    }
+}
+
 // dst = src, 16-byte elements (a vector block copy), non-temporal: a block
 // is far larger than the caches.  Each thread moves kCopyUnroll elements, all
 // loads issued before the stores.

@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -948,7 +949,66 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     if (H->rows != kets->rows || vx->rows != kets->rows || vy->rows != kets->rows)
         raise(TBK_E_INVALID, "matrix and kets rows differ");
     set_device(ctx);
-    const int rows = kets->rows, Rp = kets->Rp;
+    const int rows = kets->rows, Rp0 = kets->Rp;
+    // Device bytes of the stacks and work blocks for W columns (rolling: the
+    // most fp64 work blocks any stack form uses).
+    auto stack_entry = [&](int W, bool split) -> size_t {
+        const int64_t Kw = static_cast<int64_t>(rows) * W;
+        const int64_t kw = ((Kw + tbk::kWordBlock - 1) / tbk::kWordBlock) * tbk::kWordBlock;
+        return split ? static_cast<size_t>(kw) * sizeof(uint2) : static_cast<size_t>(stack_stride(Kw)) * sizeof(double2);
+    };
+    auto need_bytes = [&](int W) -> double {
+        const bool split = kubo_presplit(prec, N, H->is_complex, vx->is_complex);
+        return 2.0 * static_cast<double>(N) * static_cast<double>(stack_entry(W, split)) +
+               7.0 * static_cast<double>(rows) * W * sizeof(double2);
+    };
+    // Gamma is a sum over the random vectors: when the stacks of the whole
+    // block do not fit, the vectors run in groups of Gw columns (the widest
+    // power of two that fits) and the groups' Gammas are added in order.
+    DevBuf& L = ctx->stack[0];
+    DevBuf& Rs = ctx->stack[1];
+    int Gw = Rp0;
+    const size_t whole_stack =
+        static_cast<size_t>(N) * stack_entry(Rp0, kubo_presplit(prec, N, H->is_complex, vx->is_complex));
+    const double cap = ctx->tuning.kubo_stack_mib > 0 ? static_cast<double>(ctx->tuning.kubo_stack_mib) * 1048576.0
+                                                      : std::numeric_limits<double>::infinity();
+    if (L.bytes < whole_stack || Rs.bytes < whole_stack || need_bytes(Rp0) > cap) {
+        HIPC(hipStreamSynchronize(ctx->stream));
+        L.release();
+        Rs.release();
+        size_t free_b = 0, total_b = 0;
+        HIPC(hipMemGetInfo(&free_b, &total_b));
+        const double avail = std::min(0.95 * static_cast<double>(free_b), cap);
+        while (Gw > 1 && need_bytes(Gw) > avail) Gw /= 2;
+        if (need_bytes(Gw) > avail)
+            raise(TBK_E_OOM, "Kubo stacks of one random vector need " + std::to_string(need_bytes(1) / 1e9) +
+                                 " GB, " + std::to_string(avail / 1e9) + " GB available");
+    }
+    const int ngroups = (kets->R + Gw - 1) / Gw;
+    DevBuf gkets;
+    std::vector<double> gsum;
+    if (ngroups > 1) {
+        gkets.reserve(static_cast<size_t>(rows) * Gw * sizeof(double2));
+        gsum.assign(static_cast<size_t>(2) * N * N, 0.0);
+    }
+    double rec_total = 0.0, con_total = 0.0, flops_total = 0.0, rbytes_total = 0.0, stack_b = 0.0;
+    int nchunks = 0;
+    for (int grp = 0; grp < ngroups; ++grp) {
+    const int Rp = Gw;
+    const void* kp = kets->data.p;
+    int Rreal = kets->R;
+    if (ngroups > 1) {
+        // columns [grp Gw, grp Gw + cols) of the caller's kets, Gw wide
+        const int col0 = grp * Gw;
+        Rreal = std::min(Gw, kets->R - col0);
+        const int64_t total = static_cast<int64_t>(rows) * Gw;
+        const unsigned cg = static_cast<unsigned>(std::min<int64_t>((total + tbk::kBlock - 1) / tbk::kBlock, 65536));
+        hipLaunchKernelGGL(tbk::column_group_kernel, dim3(cg), dim3(tbk::kBlock), 0, ctx->stream,
+                           kets->data.as<const double2>(), rows, Rp0, col0, Rreal, Gw, gkets.as<double2>());
+        HIPC(hipGetLastError());
+        kp = gkets.p;
+    }
+    double* gout = ngroups > 1 ? gsum.data() : gamma_out;
     const int64_t K = static_cast<int64_t>(rows) * Rp;  // complex columns of one stacked block
     const size_t blk_bytes = static_cast<size_t>(K) * sizeof(double2);
     const StepPlan pH = make_plan(ctx, H, TBK_FP64, Rp, rows, b, false);
@@ -976,8 +1036,6 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     const size_t entry_bytes = presplit ? static_cast<size_t>(kw) * sizeof(uint2) : static_cast<size_t>(ldk) * sizeof(double2);
     const size_t stack_bytes = static_cast<size_t>(N) * entry_bytes;
     const int rolling = paired ? 7 : presplit ? 5 : 2;  // fp64 work blocks
-    DevBuf& L = ctx->stack[0];
-    DevBuf& Rs = ctx->stack[1];
     if (L.bytes < stack_bytes || Rs.bytes < stack_bytes) {
         // the stacks are reallocated at the new size; what the context already
         // holds for them is freed first and counts as available
@@ -1006,26 +1064,37 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     ensure_events(ctx, 2);
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
     if (paired)
-        enqueue_kubo_stacks_paired(pP, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
+        enqueue_kubo_stacks_paired(pP, pY, a, b, kp, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
                                    ctx->kwork[1].p, ctx->kwork[2].p, rows, Rp, N, part, ctx->stream, vx->val64.p);
     else if (presplit)
-        enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, kets->data.p, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
+        enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, kp, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
                                      ctx->kwork[1].p, ctx->kwork[2].p, ctx->work[0].p, ctx->work[1].p, K, N, part,
                                      ctx->stream, fuse ? vx->val64.p : nullptr);
     else
-        enqueue_kubo_stacks(pH, pX, pY, a, b, kets->data.p, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, ldk,
+        enqueue_kubo_stacks(pH, pX, pY, a, b, kp, L.p, Rs.p, ctx->work[0].p, ctx->work[1].p, blk_bytes, ldk,
                             N, part, ctx->stream, fuse ? vx->val64.p : nullptr);
     HIPC(hipEventRecord(ctx->events[1], ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
     float rec_ms = 0.f;
     HIPC(hipEventElapsedTime(&rec_ms, ctx->events[0], ctx->events[1]));
 
-    int nchunks = 0;
     const double con_ms = presplit ? contract_stacks_presplit(ctx, L.as<const uint2>(), Rs.as<const uint2>(), N, K,
-                                                              gamma_out, &nchunks)
+                                                              gout, &nchunks)
                                    : contract_stacks(ctx, static_cast<const double2*>(L.p),
-                                                     static_cast<const double2*>(Rs.p), N, K, ldk, prec, gamma_out,
+                                                     static_cast<const double2*>(Rs.p), N, K, ldk, prec, gout,
                                                      &nchunks);
+    if (ngroups > 1) {
+        const size_t nn2 = static_cast<size_t>(2) * N * N;
+        if (grp == 0) std::copy(gout, gout + nn2, gamma_out);
+        else for (size_t i = 0; i < nn2; ++i) gamma_out[i] = gamma_out[i] + gout[i];
+    }
+    rec_total += rec_ms;
+    con_total += con_ms;
+    flops_total += 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
+    stack_b = 2.0 * static_cast<double>(N) * static_cast<double>(K) *
+              static_cast<double>(presplit ? sizeof(uint2) : sizeof(double2));
+    rbytes_total += kubo_recursion_bytes(H, vx, vy, rows, Rreal, N, fuse, presplit, paired);
+    }  // groups
     if (g) {
         for (int64_t n = 0; n < N; ++n)
             for (int64_t m = 0; m < N; ++m) {
@@ -1035,15 +1104,14 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
             }
     }
     if (stats) {
-        stats->recursion_ms = rec_ms;
-        stats->contraction_ms = con_ms;
-        stats->flops = 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
-        // the stacks' entries, without the stride pad
-        stats->stack_bytes = 2.0 * static_cast<double>(N) * static_cast<double>(K) *
-                             static_cast<double>(presplit ? sizeof(uint2) : sizeof(double2));
+        stats->recursion_ms = rec_total;
+        stats->contraction_ms = con_total;
+        stats->flops = flops_total;
+        // the stacks' entries of one group, without the stride pad
+        stats->stack_bytes = stack_b;
         stats->nchunks = nchunks;
-        const double bytes = kubo_recursion_bytes(H, vx, vy, rows, kets->R, N, fuse, presplit, paired);
-        stats->recursion_bytes = bytes;
+        stats->vector_groups = ngroups;
+        stats->recursion_bytes = rbytes_total;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
