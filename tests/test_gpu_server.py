@@ -229,3 +229,29 @@ def test_kubo_request_over_budget_runs_in_vector_groups(gpu_ctx, prec, tol):
         ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
                                 gpu_ctx.random_block(Hs.n, 12, seed=31), N, precision=prec, g=g)
         assert np.array_equal(G_full, ref)
+
+
+def test_kubo_fp64_request_over_budget_runs_in_row_ranges(gpu_ctx):
+    """One random vector, fp64 stacks larger than the budget: the stacks hold
+    a range of rows of every entry at a time, the chains run once per range,
+    and the ranges' Gammas add up to the whole-stack Gamma (1e-12), which is
+    the direct path's."""
+    Hs, vx, vy, g = landau_kubo(40, 48, 1)
+    N = g.size
+    with kpm.Server(0) as srv:
+        t = srv.submit_kubo(Hs, vx, vy, N, R=1, seed=9, g=g)
+        assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
+        (z_full, G_full), info_full = srv.take(t)
+    budget = int(0.6 * info_full["footprint_bytes"])
+    with kpm.Server(0, hbm_budget_bytes=budget) as srv:
+        t = srv.submit_kubo(Hs, vx, vy, N, R=1, seed=9, g=g)
+        info = srv.wait(t, 60.0)
+        assert info["state"] == Nn.TBK_REQ_DONE, info
+        assert info["footprint_bytes"] <= budget
+        (z, G), _ = srv.take(t)
+    assert np.max(np.abs(G - G_full)) <= 1e-12 * np.max(np.abs(G_full))
+    cut = N // 24
+    assert np.max(np.abs(z[cut:N - cut] - z_full[cut:N - cut])) <= 1e-10 * np.max(np.abs(z_full[cut:N - cut]))
+    ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
+                            gpu_ctx.random_block(Hs.n, 1, seed=9), N, g=g)
+    assert np.array_equal(G_full, ref)

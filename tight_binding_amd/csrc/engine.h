@@ -249,6 +249,11 @@ void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan&
                          const void* kets, void* L, void* Rs, void* W0, void* W1, size_t blk_bytes, int64_t ldk,
                          int N, double2* part, hipStream_t st, const void* vx_values);
 bool fused_velocity(const tbk_matrix* H, const tbk_matrix* vx, const StepPlan& pH, hipStream_t st);
+// fp64 stacks of rows [r0, r1) of every entry, the chains in full rolling
+// blocks B0, B1, W0, W1 and the scratch block S
+void enqueue_kubo_stacks_rows(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
+                              const void* kets, void* L, void* Rs, void* B0, void* B1, void* W0, void* W1, void* S,
+                              int rows, int Rp, int r0, int r1, int64_t ldk, int N, double2* part, hipStream_t st);
 bool kubo_presplit(int prec, int N, bool h_complex, bool vx_complex);
 void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
                                   const void* kets, uint2* Ls, uint2* Rs, void* B0, void* B1, void* S, void* W0,

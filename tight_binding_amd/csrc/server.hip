@@ -138,6 +138,7 @@ struct Request {
     HostLattice lat;
     std::vector<double> g;
     int32_t Gw = 0;            // sigma_xy: random vectors per group (a power of two <= Rp; see kubo_footprint)
+    int32_t row_chunks = 1;    // sigma_xy, fp64 stacks: row ranges the stacks hold one at a time
     int64_t footprint = 0;
     int32_t state = TBK_REQ_QUEUED;
     int32_t error_code = 0;
@@ -303,13 +304,15 @@ int64_t tbk_server::kubo_footprint(const Request& r) const {
     const int64_t rows = r.H.hdr.rows;
     const int64_t blk = rows * r.Gw * 16;
     const int64_t N = r.n;
-    const ContractionShape cs = contraction_plan(ctx, r.n, rows * r.Gw, r.precision);
+    // fp64 stacks may hold one of row_chunks row ranges at a time
+    const int64_t crows = (rows + r.row_chunks - 1) / r.row_chunks;
+    const ContractionShape cs = contraction_plan(ctx, r.n, crows * r.Gw, r.precision);
     // pre-split stacks: half-size entries and three more rolling blocks
     const bool ps = kubo_presplit(r.precision, r.n, r.H.hdr.is_complex != 0, r.vx.hdr.is_complex != 0);
     // one stack entry: fp64 at the stack stride, words in whole word blocks
-    const int64_t K = rows * r.Gw;
+    const int64_t K = crows * r.Gw;
     const int64_t entry = ps ? ((K + tbk::kWordBlock - 1) / tbk::kWordBlock) * tbk::kWordBlock * 8 : stack_stride(K) * 16;
-    const int64_t stacks = 2 * N * entry, rolling = ps ? 5 * blk : 2 * blk;
+    const int64_t stacks = 2 * N * entry, rolling = ps ? 5 * blk : r.row_chunks > 1 ? 5 * blk : 2 * blk;
     // a lattice request also holds the assembled H while H~ is formed
     // (H~ has the size of a velocity operator: one more entry per row)
     const int64_t mats = r.lat.present ? mat_bytes(r.H, false) + 3 * mat_bytes(r.vx, false)
@@ -325,9 +328,17 @@ int64_t tbk_server::kubo_footprint(const Request& r) const {
 // therefore its result are the same whatever else the server runs.
 void tbk_server::size_kubo(Request& r) const {
     r.Gw = r.Rp;
+    r.row_chunks = 1;
     r.footprint = kubo_footprint(r);
     while (r.footprint > budget && r.Gw > 1) {
         r.Gw /= 2;
+        r.footprint = kubo_footprint(r);
+    }
+    // fp64 stacks of one vector still too large: hold them a row range at a
+    // time (the chains then run once per range)
+    const bool ps = kubo_presplit(r.precision, r.n, r.H.hdr.is_complex != 0, r.vx.hdr.is_complex != 0);
+    while (r.footprint > budget && !ps && r.row_chunks < 64) {
+        ++r.row_chunks;
         r.footprint = kubo_footprint(r);
     }
 }
@@ -500,8 +511,11 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     // it fits): Rp below is a group's padded width
     const int rows = r->H.hdr.rows, Rp = r->Gw, N = r->n;
     const int ngroups = (r->R + Rp - 1) / Rp;
-    const int64_t K = static_cast<int64_t>(rows) * Rp;
-    const size_t blk = static_cast<size_t>(K) * sizeof(double2);
+    // row ranges of the fp64 stacks (one unless the stacks of a group exceed the budget)
+    const int nrc = r->row_chunks;
+    const int crows = (rows + nrc - 1) / nrc;
+    const int64_t K = static_cast<int64_t>(nrc > 1 ? crows : rows) * Rp;  // stack entry: complex elements
+    const size_t blk = static_cast<size_t>(rows) * Rp * sizeof(double2);  // a full vector block
     const StepPlan pH = make_plan(ctx, j->H, TBK_FP64, Rp, rows, b, false);
     const StepPlan pX = make_plan(ctx, j->vx, TBK_FP64, Rp, rows, 0.0, false);
     const StepPlan pY = make_plan(ctx, j->vy, TBK_FP64, Rp, rows, 0.0, false);
@@ -509,6 +523,7 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     // the server's default tuning always takes the row kernel it needs
     const bool presplit = kubo_presplit(r->precision, N, j->H->is_complex, j->vx->is_complex);
     if (presplit && (pH.kc.kch != 0 || pX.kc.kch != 0)) raise(TBK_E_UNSUPPORTED, "pre-split stacks need the row kernel");
+    if (presplit && nrc > 1) raise(TBK_E_UNSUPPORTED, "row ranges of pre-split stacks");
     const void* vxv = fused_velocity(j->H, j->vx, pH, bstream) ? j->vx->val64.p : nullptr;
     // pre-split with a fused v_x: both chains as one block of 2 Rp columns in
     // W0, W1 (two blocks each) and S -- the five blocks the footprint counts
@@ -527,6 +542,11 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     j->Rs.reserve(static_cast<size_t>(N) * entry);
     j->W0.reserve(paired ? 2 * blk : blk);
     j->W1.reserve(paired ? 2 * blk : blk);
+    if (nrc > 1) {
+        j->B0.reserve(blk);
+        j->B1.reserve(blk);
+        j->S.reserve(blk);
+    }
     if (presplit) {
         if (!paired) {
             j->B0.reserve(blk);
@@ -540,7 +560,7 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     const int64_t NN = static_cast<int64_t>(N) * N;
     j->P.reserve(static_cast<size_t>(cs.nchunks) * NN * sizeof(double2));
     j->G.reserve(static_cast<size_t>(NN) * sizeof(double2));
-    if (ngroups > 1) j->Gacc.reserve(static_cast<size_t>(NN) * sizeof(double2));
+    if (ngroups * nrc > 1) j->Gacc.reserve(static_cast<size_t>(NN) * sizeof(double2));
     j->g.reserve(static_cast<size_t>(N) * sizeof(double));
     j->e.reserve(static_cast<size_t>(N) * sizeof(double));
     j->z.reserve(static_cast<size_t>(N) * sizeof(double2));
@@ -555,14 +575,28 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     HIPC(hipMemcpyAsync(j->g.p, pg, static_cast<size_t>(N) * 8, hipMemcpyHostToDevice, st_));
     HIPC(hipMemcpyAsync(j->e.p, pg + N, static_cast<size_t>(N) * 8, hipMemcpyHostToDevice, st_));
     const double inv = 1.0 / std::sqrt(static_cast<double>(rows));
-    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((K + tbk::kBlock - 1) / tbk::kBlock, 65536));
-    for (int grp = 0; grp < ngroups; ++grp) {
-        // columns [grp Gw, grp Gw + cols) of the request's R-wide random block
-        const int col0 = grp * Rp, cols = std::min(Rp, r->R - col0);
-        hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st_,
-                           j->kets.as<double2>(), rows, cols, Rp, r->seed, inv, r->R, col0);
-        HIPC(hipGetLastError());
-        if (paired) {
+    const int64_t kets_elems = static_cast<int64_t>(rows) * Rp;
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((kets_elems + tbk::kBlock - 1) / tbk::kBlock, 65536));
+    const int nparts = ngroups * nrc;
+    for (int part = 0; part < nparts; ++part) {
+        const int grp = part / nrc, rc = part % nrc;
+        if (rc == 0) {
+            // columns [grp Gw, grp Gw + cols) of the request's R-wide random block
+            const int col0 = grp * Rp, cols = std::min(Rp, r->R - col0);
+            hipLaunchKernelGGL(tbk::random_phase_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st_,
+                               j->kets.as<double2>(), rows, cols, Rp, r->seed, inv, r->R, col0);
+            HIPC(hipGetLastError());
+        }
+        if (nrc > 1) {
+            // rows [r0, r1) of every stack entry; the last range may be shorter
+            const int r0 = rc * crows, r1 = std::min(rows, r0 + crows);
+            const int64_t Kc = static_cast<int64_t>(r1 - r0) * Rp;
+            enqueue_kubo_stacks_rows(pH, pX, pY, a, b, j->kets.p, j->L.p, j->Rs.p, j->B0.p, j->B1.p, j->W0.p,
+                                     j->W1.p, j->S.p, rows, Rp, r0, r1, ldk, N, j->part.as<double2>(), st_);
+            const ContractionShape csr = contraction_plan(ctx, N, Kc, r->precision);
+            enqueue_contraction(csr, j->L.as<const double2>(), j->Rs.as<const double2>(), N, Kc, ldk, r->precision,
+                                j->P.as<double2>(), j->G.as<double2>(), st_);
+        } else if (paired) {
             enqueue_kubo_stacks_paired(pP, pY, a, b, j->kets.p, j->L.as<uint2>(), j->Rs.as<uint2>(), j->W0.p,
                                        j->W1.p, j->S.p, rows, Rp, N, j->part.as<double2>(), st_, vxv);
             enqueue_contraction_presplit(cs, j->L.as<const uint2>(), j->Rs.as<const uint2>(), N, K,
@@ -578,15 +612,15 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
             enqueue_contraction(cs, j->L.as<const double2>(), j->Rs.as<const double2>(), N, K, ldk, r->precision,
                                 j->P.as<double2>(), j->G.as<double2>(), st_);
         }
-        if (ngroups > 1) {
-            // the groups' Gammas added in group order
+        if (nparts > 1) {
+            // the parts' Gammas (vector groups x row ranges) added in order
             const unsigned ag = static_cast<unsigned>(std::min<int64_t>((NN + 255) / 256, 4096));
             hipLaunchKernelGGL(gamma_accumulate_kernel, dim3(ag), dim3(256), 0, st_, j->Gacc.as<double2>(),
-                               j->G.as<const double2>(), NN, grp == 0 ? 1 : 0);
+                               j->G.as<const double2>(), NN, part == 0 ? 1 : 0);
             HIPC(hipGetLastError());
         }
     }
-    double2* Gsum = ngroups > 1 ? j->Gacc.as<double2>() : j->G.as<double2>();
+    double2* Gsum = nparts > 1 ? j->Gacc.as<double2>() : j->G.as<double2>();
     tbk::launch_gamma_weight(Gsum, j->g.as<const double>(), N, st_);
     HIPC(hipGetLastError());
     tbk::launch_kubo_zeta(Gsum, N, j->e.as<const double>(), N, 1.0, j->z.as<double2>(), st_);

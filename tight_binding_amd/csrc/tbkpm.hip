@@ -748,6 +748,51 @@ void enqueue_kubo_stacks(const StepPlan& pH, const StepPlan& pX, const StepPlan&
     if (vx_values && N > 1) launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(N - 1), nullptr, Rk(N - 1), W(N - 1), part);
 }
 
+// enqueue_kubo_stacks with the stacks holding rows [r0, r1) of every entry
+// only: both chains run in full rolling blocks (B0, B1 for T_n; W0, W1 for
+// T_m v_y|r>; S for v_x T_m v_y|r>) and each entry's rows, contiguous in the
+// row-major block, are copied into its stack slot (entries ldk elements
+// apart).  Gamma over all rows is the sum of the row ranges' Gammas, so a
+// problem whose stacks exceed the device runs the chains once per range.
+void enqueue_kubo_stacks_rows(const StepPlan& pH, const StepPlan& pX, const StepPlan& pY, double a, double b,
+                              const void* kets, void* L, void* Rs, void* B0, void* B1, void* W0, void* W1, void* S,
+                              int rows, int Rp, int r0, int r1, int64_t ldk, int N, double2* part, hipStream_t st) {
+    const size_t row_bytes = static_cast<size_t>(Rp) * sizeof(double2);
+    const size_t blk = static_cast<size_t>(rows) * row_bytes;
+    const size_t off = static_cast<size_t>(r0) * row_bytes, len = static_cast<size_t>(r1 - r0) * row_bytes;
+    const size_t entry = static_cast<size_t>(ldk) * sizeof(double2);
+    auto Lk = [&](int64_t k) { return static_cast<char*>(L) + static_cast<size_t>(k) * entry; };
+    auto Rk = [&](int64_t k) { return static_cast<char*>(Rs) + static_cast<size_t>(k) * entry; };
+    auto B = [&](int64_t k) { return (k & 1) ? B1 : B0; };
+    auto W = [&](int64_t k) { return (k & 1) ? W1 : W0; };
+    auto keep = [&](void* dst, const void* src) {
+        HIPC(hipMemcpyAsync(dst, static_cast<const char*>(src) + off, len, hipMemcpyDeviceToDevice, st));
+    };
+    HIPC(hipMemcpyAsync(B0, kets, blk, hipMemcpyDeviceToDevice, st));
+    keep(Lk(0), B0);
+    if (N > 1) {
+        launch_plan(pH, st, tbk::kFirst, 1.0 / a, -b / a, B(0), nullptr, B(1), B(0), part);
+        keep(Lk(1), B(1));
+    }
+    for (int64_t n = 1; n + 1 < N; ++n) {  // T_{n+1} over T_{n-1} in place
+        launch_plan(pH, st, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, B(n), B(n - 1), B(n + 1), nullptr, part);
+        keep(Lk(n + 1), B(n + 1));
+    }
+    launch_plan(pY, st, tbk::kFirst, 1.0, 0.0, kets, nullptr, W(0), kets, part);
+    launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(0), nullptr, S, W(0), part);
+    keep(Rk(0), S);
+    if (N > 1) {
+        launch_plan(pH, st, tbk::kFirst, 1.0 / a, -b / a, W(0), nullptr, W(1), W(0), part);
+        launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(1), nullptr, S, W(1), part);
+        keep(Rk(1), S);
+    }
+    for (int64_t m = 2; m < N; ++m) {
+        launch_plan(pH, st, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, W(m - 1), W(m), W(m), nullptr, part);
+        launch_plan(pX, st, tbk::kFirst, 1.0, 0.0, W(m), nullptr, S, W(m), part);
+        keep(Rk(m), S);
+    }
+}
+
 // Split-BF16x3 solves with N >= 256 on complex operators keep their stacks
 // pre-split: the recursion stores every stack entry as the (hi, lo) bf16
 // words the contraction would form from it (split_bf16.h), 8 bytes per
