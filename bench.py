@@ -256,102 +256,124 @@ SERVE_MODELS = {"square": (2, 4), "honeycomb": (2, 3), "lieb": (2, 4), "kagome":
                 "fcc": (3, 12)}
 
 
-def build_serve_pool(n_dos, n_kubo, seed=2029):
-    """Hamiltonians the served requests draw from.  DoS: model uniform over
-    square, honeycomb, Lieb, Kagome, simple cubic and FCC (nearest neighbours,
-    t = 1, periodic), sites log-uniform in [1e3, 4e6], W ~ U[0, 2].  sigma_xy:
-    square or honeycomb, open, Landau field of flux 1/16 per plaquette (square)
-    or per cell (honeycomb), sites log-uniform in [1e3, 6.5e4], W ~ U[0, 2],
-    with v_x, v_y = [X, H~], [Y, H~] of the rescaled H~."""
-    from tight_binding_amd import kubo as Kb
-    from tight_binding_amd import lattice as Lm
+SERVE_ATOMS = {"honeycomb": 2, "lieb": 3, "kagome": 3}
+SERVE_BANDS = ((1e3, 1e4), (1e4, 1e5), (1e5, 1e6), (1e6, 4.2e6))
+SERVE_SIGMA_FLOP_CAP = 4e14
 
+
+def make_requests(n, qps, seed=2030, sigma_flop_cap=SERVE_SIGMA_FLOP_CAP):
+    """n served requests, each carrying its own lattice: Poisson arrivals at
+    `qps`; 70% DoS, 30% sigma_xy; model uniform over square, honeycomb,
+    Lieb, Kagome, simple cubic and FCC (nearest neighbours, t = 1); sites
+    log-uniform in [1e3, 4e6] (L = round((sites/atoms)^(1/dim)) cells a side);
+    W ~ U[0, 2]; N ~ U{256..2048}; R in {8, 16, 32}; half fp64, half mixed
+    precision (fp32 recursion / split-bf16 Gamma).  DoS lattices are
+    periodic; sigma_xy lattices are open with a Landau gauge A = (0, B x, 0),
+    B = 2 pi / 16.  A sigma_xy draw whose contraction 8 N^2 n R exceeds
+    sigma_flop_cap is drawn again (the returned count says how often): the
+    largest sigma_xy of the distribution, 4e6 sites at N = 2048 and R = 32,
+    is 4.3e18 flop, over an hour at the bf16 MFMA peak."""
     rng = np.random.default_rng(seed)
-    dos, kub = [], []
-    for i in range(n_dos):
-        model = list(SERVE_MODELS)[int(rng.integers(0, len(SERVE_MODELS)))]
-        dim, z = SERVE_MODELS[model]
-        c = Lm.UnitCell.model(model)
-        c.add_hopping_nn("A", 1, 1.0)
-        if model == "kagome":
-            c.add_hopping_nn("B", 1, 1.0)
-        atoms = {"honeycomb": 2, "lieb": 3, "kagome": 3}.get(model, 1)
-        sites = math.exp(rng.uniform(math.log(1e3), math.log(4e6)))
-        Lc = max(2, int(round((sites / atoms) ** (1.0 / dim))))
-        W = float(rng.uniform(0.0, 2.0))
-        lat = Lm.PartialLattice(c, (Lc,) * dim)
-        lat.set_disorder(W, seed + i)
-        H = lat.get_Hamiltonian_periodic_extend_bare_sparse((1,) * dim)
-        dos.append(dict(H=H, a=1.02 * (z + W / 2), model=model, sites=H.n))
-    for i in range(n_kubo):
-        model = ["square", "honeycomb"][i % 2]
-        c = Lm.UnitCell.model(model)
-        c.add_hopping_nn("A", 1, 1.0)
-        atoms = 2 if model == "honeycomb" else 1
-        sites = math.exp(rng.uniform(math.log(1e3), math.log(6.5e4)))
-        Lc = max(4, int(round(math.sqrt(sites / atoms))))
-        W = float(rng.uniform(0.0, 2.0))
-        lat = Lm.PartialLattice(c, (Lc, Lc))
-        lat.set_gauge(Lm.GAUGE_LANDAU, 2 * math.pi / 16)
-        lat.set_disorder(W, seed + 1000 + i)
-        H = lat.get_Hamiltonian_sparse(True)
-        emax = 1.02 * ((4 if model == "square" else 3) + W / 2)
-        lo, hi = -emax - 1e-8, emax + 1e-8
-        a = (hi - lo) / 2
-        Hs = Lm.rescale(H, a, (hi + lo) / 2)
-        vx, vy = Kb.velocity_operators(Hs, lat.op_position(0), lat.op_position(1))
-        kub.append(dict(H=Hs, vx=vx, vy=vy, model=model, sites=Hs.n))
-    return dos, kub
-
-
-def make_requests(n, dos_pool, kubo_pool, qps, seed=2030):
-    """n requests: Poisson arrivals at `qps`, 70% DoS (N ~ U{256..2048}) and
-    30% sigma_xy (N ~ U{256..1024}), R in {8, 16, 32}, half of each kind in
-    fp64 and half mixed precision (fp32 recursion / split-bf16 Gamma)."""
-    rng = np.random.default_rng(seed)
+    models = list(SERVE_MODELS)
     t = 0.0
-    out = []
+    out, redrawn = [], 0
     for i in range(n):
         t += float(rng.exponential(1.0 / qps))
-        dos = rng.uniform() < 0.7
+        dos = bool(rng.uniform() < 0.7)
         mixed = bool(rng.uniform() < 0.5)
-        if dos:
-            out.append(dict(kind="dos", t=t, p=int(rng.integers(0, len(dos_pool))), N=int(rng.integers(256, 2049)),
-                            R=int(rng.choice([8, 16, 32])), mixed=mixed, seed=7000 + i))
-        else:
-            out.append(dict(kind="kubo", t=t, p=int(rng.integers(0, len(kubo_pool))), N=int(rng.integers(256, 1025)),
-                            R=int(rng.choice([8, 16, 32])), mixed=mixed, seed=7000 + i))
+        while True:
+            model = models[int(rng.integers(0, len(models)))]
+            dim, z = SERVE_MODELS[model]
+            atoms = SERVE_ATOMS.get(model, 1)
+            target = math.exp(rng.uniform(math.log(1e3), math.log(4e6)))
+            Lc = max(2, int(round((target / atoms) ** (1.0 / dim))))
+            sites = atoms * Lc ** dim
+            W = float(rng.uniform(0.0, 2.0))
+            N = int(rng.integers(256, 2049))
+            R = int(rng.choice([8, 16, 32]))
+            if dos or 8.0 * N * N * sites * R <= sigma_flop_cap:
+                break
+            redrawn += 1
+        out.append(dict(kind="dos" if dos else "kubo", t=t, model=model, dim=dim, L=Lc, sites=sites, W=W, N=N, R=R,
+                        mixed=mixed, seed=7000 + i, a=1.02 * (z + W / 2)))
+    return out, redrawn
+
+
+def serve_lattice(r):
+    """The lattice request r carries (disorder keyed by the request's seed)."""
+    from tight_binding_amd import lattice as Lm
+
+    c = Lm.UnitCell.model(r["model"])
+    c.add_hopping_nn("A", 1, 1.0)
+    if r["model"] == "kagome":
+        c.add_hopping_nn("B", 1, 1.0)
+    lat = Lm.PartialLattice(c, (r["L"],) * r["dim"])
+    if r["kind"] == "kubo":
+        lat.set_gauge(Lm.GAUGE_LANDAU, 2 * math.pi / 16)
+    lat.set_disorder(r["W"], r["seed"] + 100000)
+    return lat
+
+
+def percentiles_by(lat, reqs, key):
+    """p50 / p99 latency (ms) and count of the requests in each group."""
+    out = {}
+    for name in sorted({key(r) for r in reqs}):
+        sel = np.array([key(r) == name for r in reqs])
+        if sel.any():
+            x = lat[sel]
+            out[name] = {"n": int(sel.sum()), "p50_ms": float(np.percentile(x, 50)),
+                         "p99_ms": float(np.percentile(x, 99))}
     return out
 
 
-def serve_leg(qps, n_requests, pool_dos=24, pool_kubo=6, saturation_only=False):
-    """Served mixed workload: continuous batching in the request server, with
-    latency percentiles at a fixed Poisson arrival rate, the makespan of the
-    same requests all submitted at once, and the plain per-solve GPU path on
-    the same requests (one call per request) for comparison."""
+def size_band(sites):
+    for lo, hi in SERVE_BANDS:
+        if lo <= sites < hi:
+            return "%.0e-%.0e" % (lo, hi)
+    return "other"
+
+
+def serve_leg(qps, n_requests, saturation_only=False):
+    """Served mixed workload: continuous batching in the request server, the
+    requests carrying lattices that the server assembles on the device at
+    admission (sigma_xy: H, H~ = H/a and v_x, v_y formed there too).  Latency
+    percentiles at a fixed Poisson arrival rate (overall, per kind, per size
+    band), the makespan of the same requests all submitted at once, and the
+    plain per-solve GPU path on the same requests (device assembly, then one
+    moments or Gamma call per request) for comparison."""
     import threading
 
     from tight_binding_amd import kpm
+    from tight_binding_amd import lattice as Lm
     from tight_binding_amd import spectral as S
 
-    t0 = time.perf_counter()
-    dpool, kpool = build_serve_pool(pool_dos, pool_kubo)
-    reqs = make_requests(n_requests, dpool, kpool, qps)
-    log("serve: pool of %d DoS + %d sigma_xy Hamiltonians (%.1fM sites) assembled in %.1f s; %d requests at %.1f QPS"
-        % (len(dpool), len(kpool), sum(d["sites"] for d in dpool) / 1e6, time.perf_counter() - t0, len(reqs), qps))
+    reqs, redrawn = make_requests(n_requests, qps)
+    lats = [serve_lattice(r) for r in reqs]
+    nd = sum(r["kind"] == "dos" for r in reqs)
+    log("serve: %d requests at %.1f QPS (%d DoS, %d sigma_xy, %.1fM sites in all, largest %d DoS / %d sigma_xy; "
+        "%d sigma_xy draws above %.0e flop redrawn)"
+        % (len(reqs), qps, nd, len(reqs) - nd, sum(r["sites"] for r in reqs) / 1e6,
+           max([r["sites"] for r in reqs if r["kind"] == "dos"] or [0]),
+           max([r["sites"] for r in reqs if r["kind"] == "kubo"] or [0]), redrawn, SERVE_SIGMA_FLOP_CAP))
     g_of = {}
 
-    def submit(srv, r):
-        if r["kind"] == "dos":
-            d = dpool[r["p"]]
-            return srv.submit_dos(d["H"], r["N"], d["a"], 0.0, r["R"], r["seed"], kpm.FP32 if r["mixed"] else kpm.FP64)
-        k = kpool[r["p"]]
-        g = g_of.setdefault(r["N"], S.lorentz(r["N"], 6.0))
-        return srv.submit_kubo(k["H"], k["vx"], k["vy"], r["N"], R=r["R"], seed=r["seed"],
-                               precision=kpm.GAMMA_BF16X3 if r["mixed"] else kpm.GAMMA_FP64, g=g)
+    def gweights(N):
+        return g_of.setdefault(N, S.lorentz(N, 6.0))
 
-    def serve(arrivals):
-        srv = kpm.Server(0, max_kubo=2)
+    def submit(srv, r, lat):
+        if r["kind"] == "dos":
+            return srv.submit_dos(lat, r["N"], r["a"], 0.0, r["R"], r["seed"], kpm.FP32 if r["mixed"] else kpm.FP64,
+                                  builder=Lm.BUILD_PERIODIC, extend=(1,) * r["dim"])
+        return srv.submit_kubo_lattice(lat, r["N"], r["a"], 0.0, R=r["R"], seed=r["seed"],
+                                       precision=kpm.GAMMA_BF16X3 if r["mixed"] else kpm.GAMMA_FP64,
+                                       g=gweights(r["N"]))
+
+    created = {}
+
+    def serve(arrivals, label):
+        t_c = time.perf_counter()
+        srv = kpm.Server(0, max_kubo=2, max_kubo_moments=2048)
+        created[label] = time.perf_counter() - t_c
         tickets = [None] * len(reqs)
         start = time.perf_counter()
 
@@ -361,107 +383,120 @@ def serve_leg(qps, n_requests, pool_dos=24, pool_kubo=6, saturation_only=False):
                     dt = r["t"] - (time.perf_counter() - start)
                     if dt > 0:
                         time.sleep(dt)
-                tickets[i] = submit(srv, r)
+                tickets[i] = submit(srv, r, lats[i])
 
         th = threading.Thread(target=feeder)
         th.start()
         th.join()
-        results, lat = [], []
+        results, lat, infos = [], [], []
         for i, t in enumerate(tickets):
-            info = srv.wait(t, 300.0)
+            info = srv.wait(t, 600.0)
             if info["state"] != 2:
                 raise SystemExit("bench: served request %d did not finish: %s" % (i, info))
             res, info = srv.take(t)
             results.append(res)
+            infos.append(info)
             lat.append(info["latency_ms"])
+            if (i + 1) % 16 == 0:
+                log("serve: %s run, %d of %d requests done, %.1f s" % (label, i + 1, len(reqs),
+                                                                        time.perf_counter() - start))
         span = time.perf_counter() - start
         st = srv.stats()
         srv.close()
-        return results, np.asarray(lat), span, st
+        return results, np.asarray(lat), span, st, infos
 
     if saturation_only:
-        res_s, lat_s, span_s, st_s = serve(False)
+        res_s, lat_s, span_s, st_s, _ = serve(False, "saturation")
         log("serve: saturation run (all submitted at once) %.2f s" % span_s)
         return {"served_makespan_s": span_s, "requests": len(reqs), "chunks": st_s["chunks"]}
-    res_p, lat_p, span_p, st_p = serve(True)
+    res_p, lat_p, span_p, st_p, info_p = serve(True, "Poisson")
     log("serve: Poisson run %.1f s, latency p50 %.1f ms p99 %.1f ms" % (span_p, np.percentile(lat_p, 50),
                                                                        np.percentile(lat_p, 99)))
-    res_s, lat_s, span_s, st_s = serve(False)
+    res_s, lat_s, span_s, st_s, _ = serve(False, "saturation")
     log("serve: saturation run (all submitted at once) %.2f s" % span_s)
 
-    # plain per-solve GPU path: one call per request, fp64 and mixed as requested,
-    # plus the fp64 answer of every mixed-precision request for its error
+    # plain per-solve GPU path: device assembly and one call per request, in
+    # the precision asked for
     ctx = kpm.Context(0)
+
+    def plain_dos(r, lat, prec):
+        M, _ = ctx.assemble(lat, Lm.BUILD_PERIODIC, (1,) * r["dim"], False)
+        K = ctx.random_block(r["sites"], r["R"], seed=r["seed"], precision=prec)
+        mu, _, _ = ctx.moments(M, K, r["N"], a=r["a"])
+        K.close()
+        M.close()
+        return mu
+
+    def plain_kubo(r, lat, prec):
+        M, _ = ctx.assemble(lat, Lm.BUILD_OPEN, None, True)
+        Hs = ctx.rescale(M, r["a"], 0.0)
+        M.close()
+        vx, vy = ctx.velocity(Hs, lat, 0), ctx.velocity(Hs, lat, 1)
+        K = ctx.random_block(r["sites"], r["R"], seed=r["seed"])
+        G, st = kpm.kubo_gamma(ctx, Hs, vx, vy, K, r["N"], precision=prec, g=gweights(r["N"]))
+        z, _ = kpm.kubo_zeta(ctx, G, S.kubo_nodes(r["N"]))
+        for m in (K, Hs, vx, vy):
+            m.close()
+        return z, G, st
+
     t1 = time.perf_counter()
     plain = []
-    for r in reqs:
+    for i, (r, lat) in enumerate(zip(reqs, lats)):
         if r["kind"] == "dos":
-            d = dpool[r["p"]]
-            M = ctx.matrix(d["H"])
-            K = ctx.random_block(d["H"].n, r["R"], seed=r["seed"], precision=kpm.FP32 if r["mixed"] else kpm.FP64)
-            mu, _, _ = ctx.moments(M, K, r["N"], a=d["a"])
-            plain.append(mu)
-            K.close()
-            M.close()
+            plain.append(plain_dos(r, lat, kpm.FP32 if r["mixed"] else kpm.FP64))
         else:
-            k = kpool[r["p"]]
-            Ms = [ctx.matrix(k[x]) for x in ("H", "vx", "vy")]
-            K = ctx.random_block(k["H"].n, r["R"], seed=r["seed"])
-            G, _ = kpm.kubo_gamma(ctx, *Ms, K, r["N"], precision=kpm.GAMMA_BF16X3 if r["mixed"] else kpm.GAMMA_FP64,
-                                  g=g_of[r["N"]])
-            z, _ = kpm.kubo_zeta(ctx, G, S.kubo_nodes(r["N"]))
-            plain.append((z, G))
-            K.close()
-            for m in Ms:
-                m.close()
+            plain.append(plain_kubo(r, lat, kpm.GAMMA_BF16X3 if r["mixed"] else kpm.GAMMA_FP64))
+        if (i + 1) % 16 == 0:
+            log("serve: plain path, %d of %d requests, %.1f s" % (i + 1, len(reqs), time.perf_counter() - t1))
     span_plain = time.perf_counter() - t1
     ctx.release_work()
     log("serve: plain per-solve path %.2f s" % span_plain)
 
-    # checks: served == plain (same requests, same precision); mixed vs fp64
-    err_dos, err_sig = [], []
+    # checks: served == plain (same requests, same precision; Gamma to the
+    # rounding of its summation order, which vector groups change)
     for r, a, b in zip(reqs, res_s, plain):
         if r["kind"] == "dos":
-            assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
+            assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b)), r
         else:
-            assert np.max(np.abs(a[1] - b[1])) <= 1e-12 * np.max(np.abs(b[1]))
-    for r, a in zip(reqs, res_s):
+            tol = 1e-5 if r["mixed"] else 1e-12
+            assert np.max(np.abs(a[1] - b[1])) <= tol * np.max(np.abs(b[1])), r
+    # mixed precision against fp64 on the same requests
+    err_dos, err_sig = [], []
+    for r, lat, a in zip(reqs, lats, res_s):
         if not r["mixed"]:
             continue
         if r["kind"] == "dos":
-            d = dpool[r["p"]]
-            M = ctx.matrix(d["H"])
-            K = ctx.random_block(d["H"].n, r["R"], seed=r["seed"])
-            mu64, _, _ = ctx.moments(M, K, r["N"], a=d["a"])
+            mu64 = plain_dos(r, lat, kpm.FP64)
             err_dos.append(float(np.max(np.abs(a - mu64)) / np.max(np.abs(mu64))))
-            K.close()
-            M.close()
         elif len(err_sig) < 4:
-            k = kpool[r["p"]]
-            Ms = [ctx.matrix(k[x]) for x in ("H", "vx", "vy")]
-            K = ctx.random_block(k["H"].n, r["R"], seed=r["seed"])
-            G64, _ = kpm.kubo_gamma(ctx, *Ms, K, r["N"], g=g_of[r["N"]])
+            G64 = plain_kubo(r, lat, kpm.GAMMA_FP64)[1]
             ef = np.linspace(-0.7, 0.7, 201)
             s64 = S.kubo_sigma(G64, ef)[0]
             smx = S.kubo_sigma(a[1], ef)[0]
             err_sig.append(float(np.max(np.abs(smx - s64)) / np.max(np.abs(s64))))
-            K.close()
-            for m in Ms:
-                m.close()
     ctx.close()
     kinds = np.array([r["kind"] for r in reqs])
     moments = sum(r["N"] for r in reqs if r["kind"] == "dos")
+    split = [dict(sites=r["sites"], N=r["N"], R=r["R"], vector_groups=inf["vector_groups"],
+                  row_ranges=inf["row_ranges"])
+             for r, inf in zip(reqs, info_p) if r["kind"] == "kubo" and inf["vector_groups"] * inf["row_ranges"] > 1]
     out = {
-        "workload": "%d requests, Poisson arrivals at %.1f QPS, 70%% DoS (6 models, 1e3-4e6 sites, N 256-2048) / 30%% "
-                    "sigma_xy (1e3-6.5e4 sites, N 256-1024), R in {8,16,32}, half fp64 and half mixed precision; "
-                    "Hamiltonians drawn from a pool of %d + %d pre-assembled lattices" % (len(reqs), qps, len(dpool),
-                                                                                        len(kpool)),
+        "workload": "%d requests, each carrying its lattice (assembled on the device at admission), Poisson arrivals "
+                    "at %.1f QPS, 70%% DoS / 30%% sigma_xy, models square/honeycomb/Lieb/Kagome/cubic/FCC, sites "
+                    "log-uniform in [1e3, 4e6], W ~ U[0,2], N ~ U{256..2048}, R in {8,16,32}, half fp64 and half "
+                    "mixed precision; sigma_xy draws with 8 N^2 n R > %.0e flop redrawn (%d times)"
+                    % (len(reqs), qps, SERVE_SIGMA_FLOP_CAP, redrawn),
         "latency_ms_p50": float(np.percentile(lat_p, 50)),
         "latency_ms_p99": float(np.percentile(lat_p, 99)),
         "latency_ms_p50_dos": float(np.percentile(lat_p[kinds == "dos"], 50)),
         "latency_ms_p99_dos": float(np.percentile(lat_p[kinds == "dos"], 99)),
         "latency_ms_p50_sigma": float(np.percentile(lat_p[kinds == "kubo"], 50)) if (kinds == "kubo").any() else None,
         "latency_ms_p99_sigma": float(np.percentile(lat_p[kinds == "kubo"], 99)) if (kinds == "kubo").any() else None,
+        "latency_by_size_band": percentiles_by(lat_p, reqs, lambda r: size_band(r["sites"])),
+        "latency_by_kind_and_band": percentiles_by(lat_p, reqs,
+                                                   lambda r: "%s %s" % (r["kind"], size_band(r["sites"]))),
+        "largest_sigma_sites": max([r["sites"] for r in reqs if r["kind"] == "kubo"] or [0]),
+        "sigma_requests_split_to_fit": split,
         "poisson_run_s": span_p,
         "served_makespan_s": span_s,
         "served_requests_per_s": len(reqs) / span_s,
@@ -470,6 +505,7 @@ def serve_leg(qps, n_requests, pool_dos=24, pool_kubo=6, saturation_only=False):
         "plain_requests_per_s": len(reqs) / span_plain,
         "throughput_ratio_served_over_plain": span_plain / span_s,
         "chunks": st_s["chunks"],
+        "server_create_s": created,
         "mixed_dos_max_rel_err_vs_fp64": max(err_dos) if err_dos else None,
         "mixed_sigma_max_rel_err_vs_fp64": max(err_sig) if err_sig else None,
     }
@@ -652,7 +688,7 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=256, help="problems of the batched leg (0: skip)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
     ap.add_argument("--serve-requests", type=int, default=96, help="requests of the served leg (0: skip)")
-    ap.add_argument("--serve-qps", type=float, default=8.0)
+    ap.add_argument("--serve-qps", type=float, default=2.0)
     ap.add_argument("--serve-only", action="store_true", help="run only the served leg")
     ap.add_argument("--serve-saturation-only", action="store_true",
                     help="served leg: only the all-at-once run (no Poisson run, no per-solve comparison)")

@@ -354,6 +354,9 @@ typedef struct tbk_kubo_stats {
     int32_t vector_groups; /* groups the random vectors ran in (1: the whole block; more when the
                               whole block's stacks exceed the free device memory) */
     double recursion_bytes; /* algorithmic HBM bytes of the stack-building launches and copies */
+    int32_t row_ranges;     /* row ranges the fp64 stacks held one at a time (1: whole entries; more
+                               when the stacks of one random vector exceed the memory available) */
+    int32_t reserved;
 } tbk_kubo_stats;
 
 /*
@@ -506,7 +509,9 @@ enum { TBK_REQ_DOS = 0, TBK_REQ_KUBO = 1 };
 enum { TBK_REQ_QUEUED = 0, TBK_REQ_RUNNING = 1, TBK_REQ_DONE = 2, TBK_REQ_FAILED = 3 };
 
 typedef struct tbk_server_config {
-    int64_t hbm_budget_bytes;  /* device memory for requests (0: 85% of the free memory at creation) */
+    int64_t hbm_budget_bytes;  /* device memory for requests (0: 85% of the free memory at creation): the
+                                  arena that every admitted request's matrices, vector blocks, stacks and
+                                  Gamma buffers are carved from (see lazy_arena) */
     int32_t max_slots;         /* DoS requests on the device at once (0: 512) */
     int32_t max_kubo;          /* sigma_xy requests on the device at once (0: 1) */
     int32_t chunk_steps;       /* steps per batched chunk (0: 32) */
@@ -514,6 +519,11 @@ typedef struct tbk_server_config {
     int32_t max_moments;       /* largest DoS moment count accepted (0: 4096) */
     int32_t max_kubo_moments;  /* largest sigma_xy N accepted (0: 1024) */
     double max_wait_bypass_ms; /* a request queued longer than this is not overtaken (0: 1000) */
+    int32_t lazy_arena;        /* 0: the whole budget is allocated at creation (one allocation; serving then
+                                  allocates nothing); 1: slabs are allocated as requests need them (the
+                                  first of 4 GiB, each later one at least as large as all before), kept,
+                                  and freed only to make room for a larger one */
+    int32_t reserved;
 } tbk_server_config;
 
 typedef struct tbk_request {
@@ -546,12 +556,15 @@ typedef struct tbk_request_info {
     int64_t footprint_bytes;
     int64_t result_doubles; /* DoS: n moments; sigma_xy: 2N (zeta at the N Chebyshev nodes) + 2N^2 (Gamma) */
     double a, b;            /* the rescaling used (estimated at admission for a DoS request with a = 0) */
+    int32_t vector_groups;  /* sigma_xy: groups the random vectors run in (1: the whole block at once) */
+    int32_t row_ranges;     /* sigma_xy, fp64 stacks: row ranges the stacks hold one at a time (1: whole entries) */
 } tbk_request_info;
 
 typedef struct tbk_server_stats {
     int64_t submitted, completed, failed, queued;
     int64_t running_dos, running_kubo, kubo_done, chunks;
     int64_t used_bytes, budget_bytes;
+    int64_t arena_bytes, arena_slabs; /* device memory the arena holds, in how many slabs */
 } tbk_server_stats;
 
 int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server** out);

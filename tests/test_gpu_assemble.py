@@ -171,7 +171,7 @@ def test_server_assembles_lattice_requests(gpu_ctx):
         (lattice("square", (25, 20), W=1.0, seed=5, gauge=L.GAUGE_LANDAU, B=0.2), L.BUILD_OPEN, None, True,
          kpm.FP32, 4.8),
     ]
-    with kpm.Server(0, chunk_steps=8) as srv:
+    with kpm.Server(0, lazy_arena=True, chunk_steps=8) as srv:
         tl = [srv.submit_dos(lat, 96, a, 0.0, 8, 40 + i, prec, builder=b, extend=e, is_complex=c)
               for i, (lat, b, e, c, prec, a) in enumerate(specs)]
         tc = [srv.submit_dos(host_csr(lat, b, e, c)[0], 96, a, 0.0, 8, 40 + i, prec)
@@ -187,3 +187,48 @@ def test_server_assembles_lattice_requests(gpu_ctx):
     a, b = info["a"], info["b"]
     assert (ev[-1] - b) / a <= 1 - 0.004 and (ev[0] - b) / a >= -1 + 0.004
     assert np.max(np.abs(mu)) <= mu[0] * (1 + 1e-12)
+
+
+@pytest.mark.parametrize("model,span,z", [("square", (20, 18), 4), ("cubic", (7, 6, 5), 6)])
+def test_server_assembles_sigma_xy_lattice_requests(gpu_ctx, model, span, z):
+    """sigma_xy requests that carry a lattice (H, H~ = H/a and v_x, v_y all
+    formed on the device at admission) give the Gamma of the same request
+    carrying the host-built H~, v_x, v_y, bit for bit, and the Gamma of the
+    oracle's lattice, rescale and commutators (lattice.h:314-345) with the
+    same kets within 1e-10."""
+    import oracle
+    from oracle import spectral_oracle as SO
+    from tight_binding_amd import kubo as Kb
+    from tight_binding_amd import spectral as S
+
+    B, W, seed, N, R, kseed = 2 * math.pi / 16, 1.0, 61, 48, 4, 91
+    a = 1.02 * (z + W / 2)
+    lat = lattice(model, span, W=W, seed=seed, gauge=L.GAUGE_LANDAU, B=B)
+    Hs = L.rescale(lat.get_Hamiltonian_sparse(True), a, 0.0)
+    vx, vy = Kb.velocity_operators(Hs, lat.op_position(0), lat.op_position(1))
+    g = S.lorentz(N, 4.0)
+    with kpm.Server(0, lazy_arena=True) as srv:
+        tl = srv.submit_kubo_lattice(lat, N, a, 0.0, R=R, seed=kseed, g=g)
+        th = srv.submit_kubo(Hs, vx, vy, N, R=R, seed=kseed, g=g)
+        for t in (tl, th):
+            assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
+        (zl, Gl), info = srv.take(tl)
+        (zh, Gh), _ = srv.take(th)
+    assert info["vector_groups"] == 1 and info["row_ranges"] == 1
+    assert np.array_equal(Gl, Gh) and np.array_equal(zl, zh)
+    # the oracle's Hamiltonian, rescale and velocities, the server's kets
+    ocell = O.model(model)
+    ocell.add_hopping_nn("A", 1, 1.0)
+    olat = O.PartialLattice(ocell, span)
+    olat.gauge_field = O.landau_gauge(B)
+    olat.set_scalar_field_indexed(O.uniform_disorder(W, seed))
+    oHs = O.rescale(olat.get_Hamiltonian_sparse(True), a, 0.0)
+    ovx = O.commutator_with_diagonal(olat.op_position(0), oHs)
+    ovy = O.commutator_with_diagonal(olat.op_position(1), oHs)
+    kets = gpu_ctx.random_block(oHs.n, R, seed=kseed).download()
+    TL = oracle.cheb_on_bases(oHs, N, kets)
+    w = oracle.cheb_on_bases(ovy, 2, kets)[1]
+    TR = oracle.cheb_on_bases(oHs, N, w)
+    aR = np.stack([oracle.cheb_on_bases(ovx, 2, TR[m])[1] for m in range(N)])
+    ref = SO.kubo_gamma(TL * g[:, None, None], aR * g[:, None, None])
+    assert np.max(np.abs(Gl - ref)) <= 1e-10 * np.max(np.abs(ref))

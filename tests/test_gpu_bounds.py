@@ -120,7 +120,7 @@ def test_server_estimates_bounds_for_dos_requests(gpu_ctx):
     for i, (m, Lc, W) in enumerate([("square", 9, 1.0), ("honeycomb", 8, 0.5), ("lieb", 7, 2.0), ("kagome", 8, 0.0)]):
         H, _ = model_problem(m, Lc, W, 60 + i)
         probs.append((H, 64 + 32 * i, 8, 700 + i, kpm.FP64 if i % 2 == 0 else kpm.FP32))
-    with kpm.Server(0, chunk_steps=8) as srv:
+    with kpm.Server(0, lazy_arena=True, chunk_steps=8) as srv:
         ts = [srv.submit_dos(H, n, 0.0, 0.0, R, seed, prec) for H, n, R, seed, prec in probs]
         res = []
         for t in ts:

@@ -362,3 +362,34 @@ def test_kubo_gamma_in_vector_groups_under_a_stack_cap(gpu_ctx, prec, tol):
         assert np.max(np.abs(grouped - ref)) <= 1e-10 * np.max(np.abs(ref))
     for x in M + [K]:
         x.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(kpm.GAMMA_FP64, 1e-12), (kpm.GAMMA_BF16X3, 1e-5)])
+def test_kubo_gamma_in_row_ranges_when_one_vector_exceeds_the_cap(gpu_ctx, prec, tol):
+    """tbk_kubo_gamma with a 1 MiB stack cap, below the 7 MB the stacks of one
+    random vector need: every group is one vector and its stacks hold a range
+    of rows of every entry at a time (chains rerun per range); Gamma equals
+    the whole block's to rounding and the oracle's at 1e-10 (fp64)."""
+    Hs, vx, vy = landau_problem(48, 2 * math.pi / 16, 1.0, 5, 5.2)
+    kets = O.random_phase_kets(Hs.n, 3, 23)
+    N = 96
+    g = S.lorentz(N, 6.0)
+    M = [gpu_ctx.matrix(x) for x in (Hs, vx, vy)]
+    K = gpu_ctx.block(kets)
+    whole, st = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=prec, g=g)
+    assert st["row_ranges"] == 1
+    saved = gpu_ctx.get_tuning()
+    try:
+        gpu_ctx.set_tuning(**dict(saved, kubo_stack_mib=1))
+        gpu_ctx.release_work()
+        ranged, str_ = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=prec, g=g)
+    finally:
+        gpu_ctx.set_tuning(**saved)
+        gpu_ctx.release_work()
+    assert str_["vector_groups"] == 3 and str_["row_ranges"] >= 8, str_
+    assert np.max(np.abs(ranged - whole)) <= tol * np.max(np.abs(whole))
+    if prec == kpm.GAMMA_FP64:
+        ref = oracle_gamma(Hs, vx, vy, kets, N, g)
+        assert np.max(np.abs(ranged - ref)) <= 1e-10 * np.max(np.abs(ref))
+    for x in M + [K]:
+        x.close()

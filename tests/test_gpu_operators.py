@@ -125,7 +125,7 @@ def test_lattice_kubo_request_matches_host_operators(gpu_ctx, prec):
     N = g.size
     ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
                             gpu_ctx.random_block(Hs.n, 4, seed=9), N, precision=prec, g=g)
-    with kpm.Server(0) as srv:
+    with kpm.Server(0, lazy_arena=True) as srv:
         t = srv.submit_kubo_lattice(lat, N, a, b, R=4, seed=9, precision=prec, g=g)
         t2 = srv.submit_kubo(Hs, vx, vy, N, R=4, seed=9, precision=prec, g=g)
         assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
@@ -138,7 +138,7 @@ def test_lattice_kubo_request_matches_host_operators(gpu_ctx, prec):
 
 def test_lattice_kubo_request_errors():
     lat, _ = lattice_pair("square", (6, 6), [("A", 1, 1.0)])
-    with kpm.Server(0) as srv:
+    with kpm.Server(0, lazy_arena=True) as srv:
         with pytest.raises(Nn.NativeError) as e:
             srv.submit_kubo_lattice(lat, 16, 0.0)  # sigma_xy needs a, b
         assert e.value.code == Nn.TBK_E_INVALID

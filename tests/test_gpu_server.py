@@ -47,7 +47,7 @@ def run_all(srv, reqs, stagger=0.0):
 
 def test_dos_requests_match_single_path(gpu_ctx):
     reqs = dos_requests()
-    with kpm.Server(0, chunk_steps=8) as srv:
+    with kpm.Server(0, lazy_arena=True, chunk_steps=8) as srv:
         res = run_all(srv, reqs)
         st = srv.stats()
     assert st["completed"] == len(reqs) and st["failed"] == 0 and st["used_bytes"] == 0
@@ -62,17 +62,17 @@ def test_dos_requests_match_single_path(gpu_ctx):
 def test_alone_and_in_mix_are_bit_identical():
     reqs = dos_requests(10, seed=8)
     target = 6
-    with kpm.Server(0, chunk_steps=4) as srv:
+    with kpm.Server(0, lazy_arena=True, chunk_steps=4) as srv:
         alone = run_all(srv, [reqs[target]])[0]
     # the same request amid others, submitted while they are running
-    with kpm.Server(0, chunk_steps=4) as srv:
+    with kpm.Server(0, lazy_arena=True, chunk_steps=4) as srv:
         mixed = run_all(srv, reqs, stagger=0.002)
     assert np.array_equal(alone, mixed[target])
     # and with another chunk length and slot width
-    with kpm.Server(0, chunk_steps=16, cpl=1) as srv:
+    with kpm.Server(0, lazy_arena=True, chunk_steps=16, cpl=1) as srv:
         again = run_all(srv, reqs[::-1])
     for i, mu in enumerate(again[::-1]):
-        with kpm.Server(0, chunk_steps=16, cpl=1) as s2:
+        with kpm.Server(0, lazy_arena=True, chunk_steps=16, cpl=1) as s2:
             solo = run_all(s2, [reqs[i]])[0]
         assert np.array_equal(solo, mu)
         if i >= 2:
@@ -98,7 +98,7 @@ def test_kubo_request_matches_direct_path(gpu_ctx, prec):
     N = g.size
     ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
                             gpu_ctx.random_block(Hs.n, 4, seed=77), N, precision=prec, g=g)
-    with kpm.Server(0) as srv:
+    with kpm.Server(0, lazy_arena=True) as srv:
         t = srv.submit_kubo(Hs, vx, vy, N, R=4, seed=77, precision=prec, g=g)
         assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
         (z, G), info = srv.take(t)
@@ -111,12 +111,12 @@ def test_kubo_request_matches_direct_path(gpu_ctx, prec):
 def test_mixed_dos_and_kubo_overlap_and_results_unchanged():
     reqs = dos_requests(6, seed=13)
     Hs, vx, vy, g = landau_kubo(10, 32, 4)
-    with kpm.Server(0, chunk_steps=4) as srv:
+    with kpm.Server(0, lazy_arena=True, chunk_steps=4) as srv:
         alone = run_all(srv, reqs)
         tk = srv.submit_kubo(Hs, vx, vy, g.size, R=4, seed=5, g=g)
         assert srv.wait(tk, 60.0)["state"] == Nn.TBK_REQ_DONE
         (z_alone, G_alone), _ = srv.take(tk)
-    with kpm.Server(0, chunk_steps=4, max_kubo=2) as srv:
+    with kpm.Server(0, lazy_arena=True, chunk_steps=4, max_kubo=2) as srv:
         tk = srv.submit_kubo(Hs, vx, vy, g.size, R=4, seed=5, g=g)
         mixed = run_all(srv, reqs)
         assert srv.wait(tk, 60.0)["state"] == Nn.TBK_REQ_DONE
@@ -131,7 +131,7 @@ def test_mixed_dos_and_kubo_overlap_and_results_unchanged():
 def test_admission_respects_the_memory_budget():
     reqs = dos_requests(8, seed=21)
     # footprint of the largest request, from a probe server
-    with kpm.Server(0) as srv:
+    with kpm.Server(0, lazy_arena=True) as srv:
         fp = []
         for r in reqs:
             t = srv.submit_dos(r["H"], r["n"], r["a"], r["b"], r["R"], r["seed"], r["precision"])
@@ -161,7 +161,7 @@ def test_admission_respects_the_memory_budget():
 
 def test_server_errors():
     H, a = model_problem("square", 6, 0.0, 1)
-    with kpm.Server(0, max_moments=64) as srv:
+    with kpm.Server(0, lazy_arena=True, max_moments=64) as srv:
         with pytest.raises(Nn.NativeError) as e:
             srv.submit_dos(H, 1, a)
         assert e.value.code == Nn.TBK_E_INVALID
@@ -189,7 +189,7 @@ def test_presplit_kubo_request_matches_direct_path(gpu_ctx):
     N = g.size
     ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
                             gpu_ctx.random_block(Hs.n, 2, seed=31), N, precision=kpm.GAMMA_BF16X3, g=g)
-    with kpm.Server(0) as srv:
+    with kpm.Server(0, lazy_arena=True) as srv:
         t = srv.submit_kubo(Hs, vx, vy, N, R=2, seed=31, precision=kpm.GAMMA_BF16X3, g=g)
         assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
         (z, G), info = srv.take(t)
@@ -206,7 +206,7 @@ def test_kubo_request_over_budget_runs_in_vector_groups(gpu_ctx, prec, tol):
     group is partial."""
     Hs, vx, vy, g = landau_kubo(40, 64, 12)
     N = g.size
-    with kpm.Server(0) as srv:
+    with kpm.Server(0, lazy_arena=True) as srv:
         t = srv.submit_kubo(Hs, vx, vy, N, R=12, seed=31, precision=prec, g=g)
         assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
         (z_full, G_full), info_full = srv.take(t)
@@ -238,7 +238,7 @@ def test_kubo_fp64_request_over_budget_runs_in_row_ranges(gpu_ctx):
     the direct path's."""
     Hs, vx, vy, g = landau_kubo(40, 48, 1)
     N = g.size
-    with kpm.Server(0) as srv:
+    with kpm.Server(0, lazy_arena=True) as srv:
         t = srv.submit_kubo(Hs, vx, vy, N, R=1, seed=9, g=g)
         assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
         (z_full, G_full), info_full = srv.take(t)
@@ -255,3 +255,56 @@ def test_kubo_fp64_request_over_budget_runs_in_row_ranges(gpu_ctx):
     ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
                             gpu_ctx.random_block(Hs.n, 1, seed=9), N, g=g)
     assert np.array_equal(G_full, ref)
+
+
+def _square_lattice(Lc, W, seed):
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (Lc, Lc))
+    lat.set_disorder(W, seed)
+    return lat
+
+
+def _served_vs_single(gpu_ctx, srv, lat, n, R, seed, a):
+    t = srv.submit_dos(lat, n, a, 0.0, R, seed, kpm.FP64, builder=L.BUILD_PERIODIC, extend=(1, 1))
+    info = srv.wait(t, 120.0)
+    assert info["state"] == Nn.TBK_REQ_DONE, info
+    mu, info = srv.take(t)
+    M, _ = gpu_ctx.assemble(lat, L.BUILD_PERIODIC, (1, 1), False)
+    K = gpu_ctx.random_block(M.info()["rows"], R, seed=seed)
+    ref, _, _ = gpu_ctx.moments(M, K, n, a=a)
+    K.close()
+    M.close()
+    assert np.max(np.abs(mu - ref)) <= 1e-12 * np.max(np.abs(ref))
+    return info
+
+
+def test_lazy_arena_grows_and_retires_idle_slabs(gpu_ctx):
+    """lazy_arena with a 6 GiB budget: a small request makes the first 4 GiB
+    slab; a 4.6 GB request that fits neither that slab nor the room left
+    beside it retires the idle slab and gets one of its own; both results
+    equal the single path."""
+    gib = 1 << 30
+    with kpm.Server(0, hbm_budget_bytes=6 * gib, lazy_arena=True) as srv:
+        st = srv.stats()
+        assert st["arena_bytes"] == 0 and st["arena_slabs"] == 0
+        _served_vs_single(gpu_ctx, srv, _square_lattice(128, 1.0, 5), 16, 8, 31, 4.6)
+        st = srv.stats()
+        assert st["arena_slabs"] == 1 and st["arena_bytes"] == 4 * gib
+        info = _served_vs_single(gpu_ctx, srv, _square_lattice(1750, 1.0, 6), 16, 32, 32, 4.6)
+        assert info["footprint_bytes"] > 4 * gib
+        st = srv.stats()
+        assert st["arena_slabs"] == 1 and info["footprint_bytes"] <= st["arena_bytes"] <= 6 * gib
+        assert st["used_bytes"] == 0
+
+
+def test_preallocated_arena_holds_the_budget(gpu_ctx):
+    """The default arena is the whole budget, allocated at creation; requests
+    are carved from it and their results equal the single path."""
+    budget = 3 << 30
+    with kpm.Server(0, hbm_budget_bytes=budget) as srv:
+        st = srv.stats()
+        assert st["arena_slabs"] == 1 and budget - (1 << 16) < st["arena_bytes"] <= budget
+        for i in range(3):
+            _served_vs_single(gpu_ctx, srv, _square_lattice(200 + 50 * i, 0.5, 9 + i), 24, 16, 40 + i, 4.3)
+        assert srv.stats()["arena_bytes"] == st["arena_bytes"]

@@ -142,6 +142,8 @@ class tbk_kubo_stats(C.Structure):
         ("nchunks", C.c_int32),
         ("vector_groups", C.c_int32),
         ("recursion_bytes", C.c_double),
+        ("row_ranges", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     def as_dict(self):
@@ -199,6 +201,8 @@ class tbk_server_config(C.Structure):
         ("max_moments", C.c_int32),
         ("max_kubo_moments", C.c_int32),
         ("max_wait_bypass_ms", C.c_double),
+        ("lazy_arena", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -230,6 +234,8 @@ class tbk_request_info(C.Structure):
         ("result_doubles", C.c_int64),
         ("a", C.c_double),
         ("b", C.c_double),
+        ("vector_groups", C.c_int32),
+        ("row_ranges", C.c_int32),
     ]
 
     def as_dict(self):
@@ -238,7 +244,8 @@ class tbk_request_info(C.Structure):
 
 class tbk_server_stats(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "queued", "running_dos",
-                                          "running_kubo", "kubo_done", "chunks", "used_bytes", "budget_bytes")]
+                                          "running_kubo", "kubo_done", "chunks", "used_bytes", "budget_bytes",
+                                          "arena_bytes", "arena_slabs")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

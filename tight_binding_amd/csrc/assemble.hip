@@ -420,7 +420,7 @@ void assemble_matrix(tbk_context* ctx, const tbk_lattice_desc* d, hipStream_t st
     m->ctx = ctx;
     m->rows = n;
     m->is_complex = d->is_complex != 0;
-    m->rp.reserve((static_cast<size_t>(n) + 1) * 4);
+    m->rp.reserve_placed((static_cast<size_t>(n) + 1) * 4);
     dcnt.reserve((static_cast<size_t>(n) + 1) * 4);
     dmax.reserve(4);
     HIPC(hipMemsetAsync(dmax.p, 0, 4, st));
@@ -440,8 +440,8 @@ void assemble_matrix(tbk_context* ctx, const tbk_lattice_desc* d, hipStream_t st
     m->nnz = nnz;
     m->max_row_nnz = max_row;
     const int vw = m->is_complex ? 2 : 1;
-    m->ci.reserve(std::max<size_t>(static_cast<size_t>(nnz) * 4, 4));
-    m->val64.reserve(std::max<size_t>(static_cast<size_t>(nnz) * vw * 8, 8));
+    m->ci.reserve_placed(std::max<size_t>(static_cast<size_t>(nnz) * 4, 4));
+    m->val64.reserve_placed(std::max<size_t>(static_cast<size_t>(nnz) * vw * 8, 8));
     dgersh.reserve(static_cast<size_t>(grid) * 16);
     hipLaunchKernelGGL(tbk::assemble_fill_kernel, dim3(grid), dim3(tbk::kBlock), 0, st, A, m->rp.as<const int32_t>(),
                        m->ci.as<int32_t>(), m->val64.as<double>(), dgersh.as<double2>());

@@ -55,26 +55,64 @@ inline int guarded(F&& f) {
     }
 }
 
-// Grow-only device buffer.
+// A span of device memory that buffers are carved from front to back
+// (256-byte aligned); the owner of the span frees it, the buffers never do.
+struct Region {
+    char* base = nullptr;
+    size_t cap = 0, off = 0;
+    void* take(size_t n) {
+        const size_t o = (off + 255) & ~static_cast<size_t>(255);
+        if (!base || o > cap || n > cap - o) return nullptr;
+        off = o + n;
+        return base + o;
+    }
+};
+
+// The region that matrix arrays are placed in on this thread (null: their
+// own allocations); set by RegionScope.
+inline thread_local Region* tl_matrix_region = nullptr;
+
+struct RegionScope {
+    Region* prev;
+    explicit RegionScope(Region* r) : prev(tl_matrix_region) { tl_matrix_region = r; }
+    ~RegionScope() { tl_matrix_region = prev; }
+    RegionScope(const RegionScope&) = delete;
+    RegionScope& operator=(const RegionScope&) = delete;
+};
+
+// Grow-only device buffer: its own allocation, or a piece of a Region when
+// one is given and has room.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = true;
     }
-    void reserve(size_t n) {
+    void reserve(size_t n, Region* rg = nullptr) {
         if (n <= bytes) return;
         release();
         if (n == 0) return;
+        if (rg) {
+            if (void* q = rg->take(n)) {
+                p = q;
+                bytes = n;
+                owned = false;
+                return;
+            }
+        }
         HIPC(hipMalloc(&p, n));
         bytes = n;
     }
+    // a matrix array: in the thread's matrix region when one is set
+    void reserve_placed(size_t n) { reserve(n, tl_matrix_region); }
     template <typename T>
     T* as() const {
         return static_cast<T*>(p);

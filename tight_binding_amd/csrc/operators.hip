@@ -158,7 +158,7 @@ void rescale_matrix(tbk_context* ctx, const tbk_matrix* M, double a, double b, h
     m->ctx = ctx;
     m->rows = n;
     m->is_complex = M->is_complex;
-    m->rp.reserve((static_cast<size_t>(n) + 1) * 4);
+    m->rp.reserve_placed((static_cast<size_t>(n) + 1) * 4);
     DevBuf dcnt, dmax, dtemp;
     dcnt.reserve((static_cast<size_t>(n) + 1) * 4);
     dmax.reserve(4);
@@ -180,8 +180,8 @@ void rescale_matrix(tbk_context* ctx, const tbk_matrix* M, double a, double b, h
     m->nnz = nnz;
     m->max_row_nnz = max_row;
     const int vw = m->is_complex ? 2 : 1;
-    m->ci.reserve(std::max<size_t>(static_cast<size_t>(nnz) * 4, 4));
-    m->val64.reserve(std::max<size_t>(static_cast<size_t>(nnz) * vw * 8, 8));
+    m->ci.reserve_placed(std::max<size_t>(static_cast<size_t>(nnz) * 4, 4));
+    m->val64.reserve_placed(std::max<size_t>(static_cast<size_t>(nnz) * vw * 8, 8));
     if (m->is_complex)
         hipLaunchKernelGGL(tbk::rescale_fill_kernel<true>, dim3(grid), dim3(tbk::kBlock), 0, st,
                            M->rp.as<const int32_t>(), M->ci.as<const int32_t>(), M->val64.as<const double>(), n, a, b,
@@ -235,9 +235,9 @@ void velocity_matrix(tbk_context* ctx, const tbk_matrix* M, const tbk_lattice_de
     m->bandwidth95 = M->bandwidth95;
     m->pattern = M->pattern;  // the same rows, row pointers and columns
     const int vw = m->is_complex ? 2 : 1;
-    m->rp.reserve((static_cast<size_t>(n) + 1) * 4);
-    m->ci.reserve(std::max<size_t>(static_cast<size_t>(M->nnz) * 4, 4));
-    m->val64.reserve(std::max<size_t>(static_cast<size_t>(M->nnz) * vw * 8, 8));
+    m->rp.reserve_placed((static_cast<size_t>(n) + 1) * 4);
+    m->ci.reserve_placed(std::max<size_t>(static_cast<size_t>(M->nnz) * 4, 4));
+    m->val64.reserve_placed(std::max<size_t>(static_cast<size_t>(M->nnz) * vw * 8, 8));
     HIPC(hipMemcpyAsync(m->rp.p, M->rp.p, (static_cast<size_t>(n) + 1) * 4, hipMemcpyDeviceToDevice, st));
     if (M->nnz > 0)
         HIPC(hipMemcpyAsync(m->ci.p, M->ci.p, static_cast<size_t>(M->nnz) * 4, hipMemcpyDeviceToDevice, st));

@@ -1,9 +1,16 @@
 // Request server: DoS and sigma_xy requests of mixed models and sizes,
 // continuously batched on one GPU.
 //
-// A worker thread owns the device work.  Requests wait in a queue until
-// their device-memory footprint fits the budget (and a slot is free); then
-// they are admitted while other requests are running:
+// A worker thread owns the device work.  The server's device memory is an
+// arena: its whole budget allocated at creation (or slabs grown within the
+// budget as requests need them, and kept): every admitted request gets one
+// contiguous span of it (its footprint), and its matrices, vector blocks,
+// stacks and Gamma buffers are carved from that span, so serving allocates
+// and frees no device memory for requests (freeing and reallocating
+// hundreds of GB costs seconds per request).  Requests wait in
+// a queue until their footprint fits the budget and a span of the arena
+// (and a slot is free); then they are admitted while other requests are
+// running:
 //
 //  * DoS request -> a slot: its matrix, three rotating vector blocks and a
 //    moment buffer stay resident until it finishes.  All slots of one class
@@ -128,6 +135,8 @@ struct HostLattice {
     }
 };
 
+struct Slab;
+
 struct Request {
     int64_t ticket = 0;
     int32_t kind = TBK_REQ_DOS, precision = TBK_FP64, n = 0, R = 0, Rp = 0;
@@ -140,6 +149,9 @@ struct Request {
     int32_t Gw = 0;            // sigma_xy: random vectors per group (a power of two <= Rp; see kubo_footprint)
     int32_t row_chunks = 1;    // sigma_xy, fp64 stacks: row ranges the stacks hold one at a time
     int64_t footprint = 0;
+    Slab* slab = nullptr;      // its span of the arena while admitted (null: a new slab is being made for it)
+    size_t arena_off = 0;
+    size_t new_slab = 0;       // bytes of that new slab
     int32_t state = TBK_REQ_QUEUED;
     int32_t error_code = 0;
     std::string error;
@@ -147,7 +159,128 @@ struct Request {
     std::vector<double> result;
 };
 
+}  // namespace
+struct tbk_server;
+namespace {
+
+// One slab of the arena: a device allocation carved into spans.
+struct Slab {
+    char* base = nullptr;
+    size_t cap = 0;
+    std::map<size_t, size_t> holes;  // offset -> bytes
+    int32_t live = 0;                // spans handed out and not returned
+};
+
+// The arena span a request's device buffers are carved from; handed back to
+// the server's arena when the slot or job holding it is destroyed, after
+// every buffer in it has been released (it is their first member).
+struct Lease {
+    tbk_server* srv = nullptr;
+    Slab* slab = nullptr;
+    size_t off = 0;
+    Region region;
+    Lease() = default;
+    Lease(const Lease&) = delete;
+    Lease& operator=(const Lease&) = delete;
+    ~Lease();
+};
+
+// The server's device memory: the whole budget as one slab allocated at
+// creation, or (lazy_arena) slabs allocated as requests need them (the
+// first of 4 GiB, each later one at least as large as all before it, all
+// within the budget) and kept, so a server that has run a request of some
+// size holds the memory for the next one.  A request gets one span of one
+// slab: the smallest hole that fits, in address order.  Slabs with no span
+// handed out are freed only to make room for a larger slab.  Under the
+// server's mutex; the device allocations themselves happen outside it.
+struct Arena {
+    static constexpr size_t kGrain = 64 * 1024;
+    static constexpr size_t kFirstSlab = size_t{4} << 30;
+    std::vector<std::unique_ptr<Slab>> slabs;
+    size_t held = 0;     // bytes of the slabs
+    size_t pending = 0;  // bytes of slabs being allocated
+    static size_t round(size_t n) { return (n + kGrain - 1) / kGrain * kGrain; }
+    static void cut(Slab& sl, std::map<size_t, size_t>::iterator it, size_t n, size_t& off) {
+        off = it->first;
+        const size_t rest = it->second - n;
+        sl.holes.erase(it);
+        if (rest) sl.holes[off + n] = rest;
+        sl.live++;
+    }
+    // best fit over the holes of every slab
+    bool take(size_t n, Slab*& where, size_t& off) {
+        n = round(n);
+        Slab* best = nullptr;
+        std::map<size_t, size_t>::iterator bit;
+        for (auto& sl : slabs)
+            for (auto it = sl->holes.begin(); it != sl->holes.end(); ++it)
+                if (it->second >= n && (!best || it->second < bit->second)) {
+                    best = sl.get();
+                    bit = it;
+                }
+        if (!best) return false;
+        cut(*best, bit, n, off);
+        where = best;
+        return true;
+    }
+    // size of a new slab for a span of n bytes within `limit`, retiring idle
+    // slabs into `retired` when that makes the room; 0: none fits now
+    size_t plan_slab(size_t n, size_t limit, std::vector<Slab*>& retired) {
+        n = round(n);
+        auto size_for = [&](size_t room) -> size_t {
+            if (room < n) return 0;
+            const size_t want = std::max({n, kFirstSlab, held + pending});
+            return std::min(room, want) / kGrain * kGrain;
+        };
+        const size_t room = limit > held + pending ? limit - held - pending : 0;
+        if (size_t sz = size_for(room)) return sz;
+        size_t idle = 0;
+        for (auto& sl : slabs)
+            if (sl->live == 0) idle += sl->cap;
+        if (room + idle < n) return 0;
+        // retire idle slabs, smallest first, until the span fits
+        std::vector<Slab*> cand;
+        for (auto& sl : slabs)
+            if (sl->live == 0) cand.push_back(sl.get());
+        std::sort(cand.begin(), cand.end(), [](const Slab* a, const Slab* b) { return a->cap < b->cap; });
+        size_t freed = 0;
+        for (Slab* c : cand) {
+            if (room + freed >= n) break;
+            freed += c->cap;
+            retired.push_back(c);
+        }
+        for (Slab* c : retired) {
+            held -= c->cap;
+            for (auto& sl : slabs)
+                if (sl.get() == c) sl.release();
+        }
+        slabs.erase(std::remove(slabs.begin(), slabs.end(), nullptr), slabs.end());
+        return size_for(room + freed);
+    }
+    void give(Slab* sl, size_t off, size_t n) {
+        n = round(n);
+        sl->live--;
+        auto next = sl->holes.lower_bound(off);
+        if (next != sl->holes.end() && off + n == next->first) {
+            n += next->second;
+            next = sl->holes.erase(next);
+        }
+        if (next != sl->holes.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += n;
+                return;
+            }
+        }
+        sl->holes[off] = n;
+    }
+};
+
+// bytes lost to the 256-byte alignment of the buffers carved from a span
+constexpr int64_t kCarveSlack = 64 * 256;
+
 struct Slot {
+    Lease lease;
     Request* req = nullptr;
     tbk_matrix* M = nullptr;
     DevBuf blk[3], mu;
@@ -161,6 +294,7 @@ struct Slot {
 };
 
 struct KuboJob {
+    Lease lease;
     Request* req = nullptr;
     tbk_matrix *H = nullptr, *vx = nullptr, *vy = nullptr;
     DevBuf kets, L, Rs, W0, W1, B0, B1, S, part, P, G, Gacc, g, e, z;
@@ -204,6 +338,8 @@ struct tbk_server {
     hipStream_t bstream = nullptr;  // spectral-range Lanczos of auto-bounds requests (prep thread)
     tbk_server_config cfg{};
     int64_t budget = 0;
+    Arena arena;                     // the budget's device memory (under mu)
+    std::vector<Slab*> retired_slabs;  // idle slabs to free (by the preparation thread)
 
     // shared state, under mu
     std::mutex mu;
@@ -252,6 +388,9 @@ struct tbk_server {
     void fail_all(const std::string& msg);
     hipEvent_t get_event();
     void grow(DevBuf& b, size_t bytes);
+    void give_back(Slab* sl, size_t off, size_t bytes);
+    void make_slabs(std::vector<std::pair<Request*, int32_t>>& adm);
+    void lease_to(Lease& l, const Request* r);
     template <typename T>
     T* stage(T*& p, size_t& cap, size_t n);
 };
@@ -294,7 +433,7 @@ int64_t tbk_server::dos_footprint(const Request& r) const {
     // assembly scratch of a lattice request (row counts, scan, phase table) lives before them too
     if (r.lat.present)
         vec = std::max<int64_t>(vec, (rows + 1) * 4 + static_cast<int64_t>(r.lat.phase.size()) * 8 + (1 << 20));
-    return mat_bytes(r.H, r.precision == TBK_FP32) + vec + static_cast<int64_t>(r.n) * 8;
+    return mat_bytes(r.H, r.precision == TBK_FP32) + vec + static_cast<int64_t>(r.n) * 8 + kCarveSlack;
 }
 
 // Device memory of a sigma_xy request whose random vectors run in groups of
@@ -319,7 +458,8 @@ int64_t tbk_server::kubo_footprint(const Request& r) const {
                                        : mat_bytes(r.H, false) + mat_bytes(r.vx, false) + mat_bytes(r.vy, false);
     return mats + blk /* kets */ +
            stacks + rolling + static_cast<int64_t>(ctx->num_cu) * 32 * 16 /* step partials */ +
-           static_cast<int64_t>(cs.nchunks) * N * N * 16 + 2 * N * N * 16 /* Gamma, its sum */ + N * 8 * 2 + N * 16;
+           static_cast<int64_t>(cs.nchunks) * N * N * 16 + 2 * N * N * 16 /* Gamma, its sum */ + N * 8 * 2 + N * 16 +
+           kCarveSlack;
 }
 
 // The widest group of random vectors (a power of two, at most the padded R)
@@ -358,7 +498,7 @@ hipEvent_t tbk_server::get_event() {
 // in-flight chunk, so it is released when the newest in-flight chunk completes
 void tbk_server::grow(DevBuf& b, size_t bytes) {
     if (bytes <= b.bytes) return;
-    if (b.p) {
+    if (b.p && b.owned) {
         if (!inflight.empty()) {
             inflight.back().graveyard.push_back(b.p);
         } else {
@@ -371,6 +511,24 @@ void tbk_server::grow(DevBuf& b, size_t bytes) {
         b.bytes = 0;
     }
     b.reserve(std::max(bytes, static_cast<size_t>(4096)));
+}
+
+// a span back to the arena (not under mu); queued requests may now fit
+void tbk_server::give_back(Slab* sl, size_t off, size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu);
+    arena.give(sl, off, bytes);
+    prep_gen++;
+    cv_prep.notify_all();
+}
+
+void tbk_server::lease_to(Lease& l, const Request* r) {
+    if (!r->slab) raise(TBK_E_OOM, "the device refused a slab of " + std::to_string(r->new_slab / 1e9) + " GB");
+    l.srv = this;
+    l.slab = r->slab;
+    l.off = r->arena_off;
+    l.region.base = r->slab->base + r->arena_off;
+    l.region.cap = static_cast<size_t>(r->footprint);
+    l.region.off = 0;
 }
 
 template <typename T>
@@ -420,7 +578,17 @@ std::vector<std::pair<Request*, int32_t>> tbk_server::pick_admissions() {
         Request* r = *it;
         const bool is_dos = r->kind == TBK_REQ_DOS;
         std::vector<int32_t>& pins = is_dos ? free_pin : free_kpin;
-        if (!pins.empty() && used + r->footprint <= budget) {
+        size_t off = 0, nsl = 0;
+        Slab* sl = nullptr;
+        const bool fits = !pins.empty() && used + r->footprint <= budget &&
+                          (arena.take(static_cast<size_t>(r->footprint), sl, off) ||
+                           (nsl = arena.plan_slab(static_cast<size_t>(r->footprint), static_cast<size_t>(budget),
+                                                  retired_slabs)) > 0);
+        if (fits) {
+            r->slab = sl;
+            r->arena_off = off;
+            r->new_slab = nsl;
+            arena.pending += nsl;
             used += r->footprint;
             r->state = TBK_REQ_RUNNING;
             r->t_admit = t;
@@ -442,10 +610,51 @@ std::vector<std::pair<Request*, int32_t>> tbk_server::pick_admissions() {
     return out;
 }
 
+// Outside mu: free the retired slabs, then allocate the new slab of every
+// admission that needs one and place its span at the slab's start.  An
+// allocation the device refuses fails its request.
+void tbk_server::make_slabs(std::vector<std::pair<Request*, int32_t>>& adm) {
+    std::vector<Slab*> dead;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        dead.swap(retired_slabs);
+    }
+    for (Slab* sl : dead) {
+        (void)hipFree(sl->base);
+        delete sl;
+    }
+    for (auto& pr : adm) {
+        Request* r = pr.first;
+        if (r->slab) continue;
+        auto sl = std::make_unique<Slab>();
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, r->new_slab);
+        std::lock_guard<std::mutex> lk(mu);
+        arena.pending -= r->new_slab;
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            r->slab = nullptr;
+            continue;  // start_* fails it
+        }
+        sl->base = static_cast<char*>(p);
+        sl->cap = r->new_slab;
+        sl->holes[0] = sl->cap;
+        size_t off = 0;
+        Arena::cut(*sl, sl->holes.begin(), Arena::round(static_cast<size_t>(r->footprint)), off);
+        r->slab = sl.get();
+        r->arena_off = off;
+        arena.held += sl->cap;
+        arena.slabs.push_back(std::move(sl));
+    }
+}
+
 Slot* tbk_server::start_dos(Request* r, int32_t pin) {
     auto s = std::make_unique<Slot>();
+    lease_to(s->lease, r);
     s->req = r;
     s->pin = pin;
+    Region* rg = &s->lease.region;
+    RegionScope placed(rg);  // the matrix arrays too
     if (r->lat.present) {
         assemble_matrix(ctx, &r->lat.desc, bstream, &s->M, nullptr);
     } else {
@@ -467,8 +676,8 @@ Slot* tbk_server::start_dos(Request* r, int32_t pin) {
     if (r->precision == TBK_FP32) matrix_fp32(s->M);
     const int64_t rows = r->H.hdr.rows;
     const size_t elem = r->precision == TBK_FP64 ? 16 : 8;
-    for (auto& b : s->blk) b.reserve(static_cast<size_t>(rows) * r->Rp * elem);
-    s->mu.reserve(static_cast<size_t>(r->n) * 8);
+    for (auto& b : s->blk) b.reserve(static_cast<size_t>(rows) * r->Rp * elem, rg);
+    s->mu.reserve(static_cast<size_t>(r->n) * 8, rg);
     s->steps = (r->n + 1) / 2;
     slot_split(ctx->num_cu, 4, r->Rp / slot_cpl(cfg.cpl, r->precision, r->Rp), rows, s->M->bandwidth95, s->order,
                s->nwg);
@@ -487,8 +696,11 @@ Slot* tbk_server::start_dos(Request* r, int32_t pin) {
 
 void tbk_server::start_kubo(Request* r, int32_t pin) {
     auto j = std::make_unique<KuboJob>();
+    lease_to(j->lease, r);
     j->req = r;
     j->pin = pin;
+    Region* rg = &j->lease.region;
+    RegionScope placed(rg);  // the matrix arrays too
     double a = r->a, b = r->b;
     if (r->lat.present) {
         // assemble H, form H~ (the steps then run it as given) and the
@@ -537,33 +749,33 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     const int64_t ldk = stack_stride(K);
     const int64_t kw = ((K + tbk::kWordBlock - 1) / tbk::kWordBlock) * tbk::kWordBlock;
     const size_t entry = presplit ? static_cast<size_t>(kw) * sizeof(uint2) : static_cast<size_t>(ldk) * sizeof(double2);
-    j->kets.reserve(blk);
-    j->L.reserve(static_cast<size_t>(N) * entry);
-    j->Rs.reserve(static_cast<size_t>(N) * entry);
-    j->W0.reserve(paired ? 2 * blk : blk);
-    j->W1.reserve(paired ? 2 * blk : blk);
+    j->kets.reserve(blk, rg);
+    j->L.reserve(static_cast<size_t>(N) * entry, rg);
+    j->Rs.reserve(static_cast<size_t>(N) * entry, rg);
+    j->W0.reserve(paired ? 2 * blk : blk, rg);
+    j->W1.reserve(paired ? 2 * blk : blk, rg);
     if (nrc > 1) {
-        j->B0.reserve(blk);
-        j->B1.reserve(blk);
-        j->S.reserve(blk);
+        j->B0.reserve(blk, rg);
+        j->B1.reserve(blk, rg);
+        j->S.reserve(blk, rg);
     }
     if (presplit) {
         if (!paired) {
-            j->B0.reserve(blk);
-            j->B1.reserve(blk);
+            j->B0.reserve(blk, rg);
+            j->B1.reserve(blk, rg);
         }
-        j->S.reserve(blk);
+        j->S.reserve(blk, rg);
     }
     j->part.reserve(static_cast<size_t>(std::max({pH.nb, pX.nb, pY.nb, paired ? pP.nb : int64_t{0}})) *
-                    sizeof(double2));
+                    sizeof(double2), rg);
     const ContractionShape cs = contraction_plan(ctx, N, K, r->precision);
     const int64_t NN = static_cast<int64_t>(N) * N;
-    j->P.reserve(static_cast<size_t>(cs.nchunks) * NN * sizeof(double2));
-    j->G.reserve(static_cast<size_t>(NN) * sizeof(double2));
-    if (ngroups * nrc > 1) j->Gacc.reserve(static_cast<size_t>(NN) * sizeof(double2));
-    j->g.reserve(static_cast<size_t>(N) * sizeof(double));
-    j->e.reserve(static_cast<size_t>(N) * sizeof(double));
-    j->z.reserve(static_cast<size_t>(N) * sizeof(double2));
+    j->P.reserve(static_cast<size_t>(cs.nchunks) * NN * sizeof(double2), rg);
+    j->G.reserve(static_cast<size_t>(NN) * sizeof(double2), rg);
+    if (ngroups * nrc > 1) j->Gacc.reserve(static_cast<size_t>(NN) * sizeof(double2), rg);
+    j->g.reserve(static_cast<size_t>(N) * sizeof(double), rg);
+    j->e.reserve(static_cast<size_t>(N) * sizeof(double), rg);
+    j->z.reserve(static_cast<size_t>(N) * sizeof(double2), rg);
     const int64_t Ncap = cfg.max_kubo_moments;
     double* pz = pin_kubo + static_cast<size_t>(j->pin) * static_cast<size_t>(2 * Ncap + 2 * Ncap * Ncap);
     double* pg = pz + 2 * N;  // Gamma region; the weights and nodes are staged there first
@@ -863,6 +1075,7 @@ void tbk_server::run_prep() {
             }
             for (Slot* x : dead) delete x;
             for (void* p : dead_raw) (void)hipFree(p);
+            make_slabs(adm);
             for (auto& pr : adm) {
                 Request* r = pr.first;
                 try {
@@ -899,6 +1112,10 @@ void tbk_server::run_prep() {
 
 namespace {
 
+Lease::~Lease() {
+    if (srv) srv->give_back(slab, off, region.cap);
+}
+
 void fill_info(const Request& r, tbk_request_info* info) {
     if (!info) return;
     info->state = r.state;
@@ -911,6 +1128,9 @@ void fill_info(const Request& r, tbk_request_info* info) {
     info->result_doubles = static_cast<int64_t>(r.result.size());
     info->a = r.a;
     info->b = r.b;
+    const bool kubo = r.kind == TBK_REQ_KUBO && r.Gw > 0;
+    info->vector_groups = kubo ? (r.R + r.Gw - 1) / r.Gw : 1;
+    info->row_ranges = kubo ? r.row_chunks : 1;
 }
 
 }  // namespace
@@ -938,6 +1158,23 @@ int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server**
         size_t free_b = 0, total_b = 0;
         HIPC(hipMemGetInfo(&free_b, &total_b));
         s->budget = c.hbm_budget_bytes > 0 ? c.hbm_budget_bytes : static_cast<int64_t>(0.85 * static_cast<double>(free_b));
+        if (c.lazy_arena != 0 && c.lazy_arena != 1) raise(TBK_E_INVALID, "lazy_arena must be 0 or 1");
+        if (!c.lazy_arena) {
+            // the whole budget as one slab, now
+            auto sl = std::make_unique<Slab>();
+            sl->cap = static_cast<size_t>(s->budget) / Arena::kGrain * Arena::kGrain;
+            void* ap = nullptr;
+            if (hipMalloc(&ap, sl->cap) != hipSuccess) {
+                (void)hipGetLastError();
+                raise(TBK_E_OOM, "could not allocate the server's budget of " + std::to_string(s->budget / 1e9) +
+                                     " GB (" + std::to_string(static_cast<double>(free_b) / 1e9) + " GB free)");
+            }
+            sl->base = static_cast<char*>(ap);
+            sl->holes[0] = sl->cap;
+            s->arena.held = sl->cap;
+            s->arena.slabs.push_back(std::move(sl));
+        }
+
         void* p = nullptr;
         HIPC(hipHostMalloc(&p, static_cast<size_t>(c.max_slots) * c.max_moments * sizeof(double), hipHostMallocDefault));
         s->pin_mu = static_cast<double*>(p);
@@ -1087,6 +1324,8 @@ int tbk_server_get_stats(tbk_server* s, tbk_server_stats* out) {
         if (!s || !out) raise(TBK_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(s->mu);
         *out = s->st;
+        out->arena_bytes = static_cast<int64_t>(s->arena.held);
+        out->arena_slabs = static_cast<int64_t>(s->arena.slabs.size());
         out->used_bytes = s->used;
         out->queued = static_cast<int64_t>(s->queue.size());
     });
@@ -1125,6 +1364,11 @@ int tbk_server_destroy(tbk_server* s) {
         for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
         if (s->pin_mu) (void)hipHostFree(s->pin_mu);
         if (s->pin_kubo) (void)hipHostFree(s->pin_kubo);
+        for (auto& sl : s->arena.slabs) (void)hipFree(sl->base);
+        for (Slab* sl : s->retired_slabs) {
+            (void)hipFree(sl->base);
+            delete sl;
+        }
         if (s->kstream) (void)hipStreamDestroy(s->kstream);
         if (s->bstream) (void)hipStreamDestroy(s->bstream);
         tbk_context* ctx = s->ctx;

@@ -223,7 +223,7 @@ bool csr_is_hermitian(int32_t n, const std::vector<int32_t>& rp, const std::vect
 void matrix_fp32(tbk_matrix* m) {
     if (m->have32) return;
     const int64_t n = m->nnz * (m->is_complex ? 2 : 1);
-    m->val32.reserve(std::max<size_t>(static_cast<size_t>(n) * 4, 4));
+    m->val32.reserve_placed(std::max<size_t>(static_cast<size_t>(n) * 4, 4));
     if (n > 0) {
         const int grid = static_cast<int>(std::min<int64_t>((n + tbk::kBlock - 1) / tbk::kBlock, 65536));
         hipLaunchKernelGGL(tbk::to_fp32_kernel, dim3(grid), dim3(tbk::kBlock), 0, m->ctx->stream,
@@ -1013,6 +1013,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     DevBuf& L = ctx->stack[0];
     DevBuf& Rs = ctx->stack[1];
     int Gw = Rp0;
+    int nrc = 1;  // row ranges of the fp64 stacks
     const size_t whole_stack =
         static_cast<size_t>(N) * stack_entry(Rp0, kubo_presplit(prec, N, H->is_complex, vx->is_complex));
     const double cap = ctx->tuning.kubo_stack_mib > 0 ? static_cast<double>(ctx->tuning.kubo_stack_mib) * 1048576.0
@@ -1025,7 +1026,17 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         HIPC(hipMemGetInfo(&free_b, &total_b));
         const double avail = std::min(0.95 * static_cast<double>(free_b), cap);
         while (Gw > 1 && need_bytes(Gw) > avail) Gw /= 2;
-        if (need_bytes(Gw) > avail)
+        // fp64 stacks of one vector still too large: they hold a range of
+        // rows of every entry at a time and the chains run once per range
+        const bool split = kubo_presplit(prec, N, H->is_complex, vx->is_complex);
+        auto need_rows = [&](int nrc) {
+            const int crows = (rows + nrc - 1) / nrc;
+            return 2.0 * static_cast<double>(N) * static_cast<double>(stack_stride(crows)) * sizeof(double2) +
+                   7.0 * static_cast<double>(rows) * sizeof(double2);
+        };
+        if (need_bytes(Gw) > avail && !split)
+            while (nrc < 64 && need_rows(nrc) > avail) ++nrc;
+        if (nrc == 1 ? need_bytes(Gw) > avail : need_rows(nrc) > avail)
             raise(TBK_E_OOM, "Kubo stacks of one random vector need " + std::to_string(need_bytes(1) / 1e9) +
                                  " GB, " + std::to_string(avail / 1e9) + " GB available");
     }
@@ -1054,6 +1065,51 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         kp = gkets.p;
     }
     double* gout = ngroups > 1 ? gsum.data() : gamma_out;
+    if (nrc > 1) {
+        // one vector per group (Rp = 1), rows [r0, r1) of every stack entry
+        const int crows = (rows + nrc - 1) / nrc;
+        const int64_t ldk_c = stack_stride(crows);
+        const size_t blk1 = static_cast<size_t>(rows) * sizeof(double2);
+        L.reserve(static_cast<size_t>(N) * ldk_c * sizeof(double2));
+        Rs.reserve(static_cast<size_t>(N) * ldk_c * sizeof(double2));
+        ctx->work[0].reserve(blk1);
+        ctx->work[1].reserve(blk1);
+        for (DevBuf& w : ctx->kwork) w.reserve(blk1);
+        const StepPlan qH = make_plan(ctx, H, TBK_FP64, 1, rows, b, false);
+        const StepPlan qX = make_plan(ctx, vx, TBK_FP64, 1, rows, 0.0, false);
+        const StepPlan qY = make_plan(ctx, vy, TBK_FP64, 1, rows, 0.0, false);
+        ctx->partial.reserve(static_cast<size_t>(std::max({qH.nb, qX.nb, qY.nb})) * sizeof(double2));
+        std::vector<double> gpart(static_cast<size_t>(2) * N * N);
+        for (int rc = 0; rc < nrc; ++rc) {
+            const int r0 = rc * crows, r1 = std::min(rows, r0 + crows);
+            ensure_events(ctx, 2);
+            HIPC(hipEventRecord(ctx->events[0], ctx->stream));
+            enqueue_kubo_stacks_rows(qH, qX, qY, a, b, kp, L.p, Rs.p, ctx->kwork[0].p, ctx->kwork[1].p,
+                                     ctx->work[0].p, ctx->work[1].p, ctx->kwork[2].p, rows, 1, r0, r1, ldk_c, N,
+                                     ctx->partial.as<double2>(), ctx->stream);
+            HIPC(hipEventRecord(ctx->events[1], ctx->stream));
+            HIPC(hipStreamSynchronize(ctx->stream));
+            float rms = 0.f;
+            HIPC(hipEventElapsedTime(&rms, ctx->events[0], ctx->events[1]));
+            const int64_t Kc = r1 - r0;
+            const double cms = contract_stacks(ctx, L.as<const double2>(), Rs.as<const double2>(), N, Kc, ldk_c, prec,
+                                               gpart.data(), &nchunks);
+            double* dst = gout;
+            if (rc == 0) std::copy(gpart.begin(), gpart.end(), dst);
+            else for (size_t i = 0; i < gpart.size(); ++i) dst[i] = dst[i] + gpart[i];
+            rec_total += rms;
+            con_total += cms;
+            flops_total += 8.0 * static_cast<double>(N) * N * static_cast<double>(Kc);
+        }
+        stack_b = 2.0 * static_cast<double>(N) * static_cast<double>(crows) * sizeof(double2);
+        rbytes_total += static_cast<double>(nrc) * kubo_recursion_bytes(H, vx, vy, rows, Rreal, N, false, false, false);
+        if (ngroups > 1) {
+            const size_t nn2 = static_cast<size_t>(2) * N * N;
+            if (grp == 0) std::copy(gout, gout + nn2, gamma_out);
+            else for (size_t i = 0; i < nn2; ++i) gamma_out[i] = gamma_out[i] + gout[i];
+        }
+        continue;
+    }
     const int64_t K = static_cast<int64_t>(rows) * Rp;  // complex columns of one stacked block
     const size_t blk_bytes = static_cast<size_t>(K) * sizeof(double2);
     const StepPlan pH = make_plan(ctx, H, TBK_FP64, Rp, rows, b, false);
@@ -1156,6 +1212,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         stats->stack_bytes = stack_b;
         stats->nchunks = nchunks;
         stats->vector_groups = ngroups;
+        stats->row_ranges = nrc;
         stats->recursion_bytes = rbytes_total;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1299,9 +1356,9 @@ int tbk_matrix_create(tbk_context* ctx, const tbk_sparse* M, tbk_matrix** out) {
         m->is_complex = M->is_complex != 0;
         for (int32_t i = 0; i < M->rows; ++i) m->max_row_nnz = std::max(m->max_row_nnz, rp[i + 1] - rp[i]);
         m->bandwidth95 = bandwidth_percentile(rp, ci, 0.95);
-        m->rp.reserve(rp.size() * 4);
-        m->ci.reserve(std::max<size_t>(ci.size() * 4, 4));
-        m->val64.reserve(std::max<size_t>(val.size() * 8, 8));
+        m->rp.reserve_placed(rp.size() * 4);
+        m->ci.reserve_placed(std::max<size_t>(ci.size() * 4, 4));
+        m->val64.reserve_placed(std::max<size_t>(val.size() * 8, 8));
         HIPC(hipMemcpy(m->rp.p, rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
         if (!ci.empty()) {
             HIPC(hipMemcpy(m->ci.p, ci.data(), ci.size() * 4, hipMemcpyHostToDevice));

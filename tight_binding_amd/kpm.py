@@ -490,9 +490,10 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_slots: int = 0, max_kubo: int = 0,
                  chunk_steps: int = 0, cpl: int = 0, max_moments: int = 0, max_kubo_moments: int = 0,
-                 max_wait_bypass_ms: float = 0.0):
+                 max_wait_bypass_ms: float = 0.0, lazy_arena: bool = False):
         cfg = N.tbk_server_config(int(hbm_budget_bytes), int(max_slots), int(max_kubo), int(chunk_steps), int(cpl),
-                                  int(max_moments), int(max_kubo_moments), float(max_wait_bypass_ms))
+                                  int(max_moments), int(max_kubo_moments), float(max_wait_bypass_ms),
+                                  int(bool(lazy_arena)), 0)
         h = C.c_void_p()
         N.check_kpm(N.kpm().tbk_server_create(int(device), C.byref(cfg), C.byref(h)))
         self._h = h
