@@ -21,6 +21,8 @@
 // chunks in order, so results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "gamma_launch.h"
 #include "split_bf16.h"
 
@@ -364,67 +366,86 @@ __global__ __launch_bounds__(256) void gamma_bf16x3_kernel(const double2* __rest
 
 // ---------------------------------------------------------------------------
 // Large-N split-bf16 x3 contraction, same 128 x 128 tiling and XCD placement
-// as gamma_f64_big_kernel.  Each double is split once, when its tile is
-// staged (hi = bf16(x), lo = bf16(x - hi)); pre-split stacks arrive as those
-// words already.  In LDS the real and imaginary parts are kept apart: an LDS
-// word holds one part of two consecutive complex columns, and a row of a
-// 16-column stage is, per hi / lo part, [re 8 words | im 8 words] (virtual
-// k: 16 real parts, then 16 imaginary parts), plus 4 words of padding (rows
-// of 36 words: the 16 rows of a fragment read fall on distinct banks).  Lane
-// group g of a fragment reads words 4g .. 4g + 3.  B read that way gives
-// (re, im): A.B = sum ar br + ai bi = Re conj(a) b.  Groups 0, 1 reading the
-// im words and groups 2, 3 the re words with the sign bits flipped give
-// (im, -re): A.B' = sum ar bi - ai br = Im conj(a) b.  Chunks are short
-// (about 32K complex columns) so fp32 accumulation never runs long; chunk
-// partials are summed in fp64.
-constexpr int kSplitKt = 16;        // complex k per stage = one bf16 MFMA k-step
-constexpr int kSplitRowWords = 36;  // hi re, hi im, lo re, lo im (8 words each), 4 pad
+// as gamma_f64_big_kernel.  An operand value x is split as hi = bf16(x),
+// lo = bf16(x - hi) per real part, and a column's (hi, lo) words are
+// (re, im) bf16 pairs (split_bf16.h); pre-split stacks arrive as those words
+// already, fp64 stacks are split when staged.  Chunks are short (about 32K
+// complex columns) so fp32 accumulation never runs long; chunk partials are
+// summed in fp64.
+//
+// A stage row (16 complex columns of one entry) is kept as stored: (hi, lo)
+// per column, 32 words, 128 bytes.  The k order of the matrix-core products
+// is a free choice as long as both operands use it, so lane group g of a
+// fragment takes columns 4g .. 4g + 3: its hi fragment is their hi words and
+// its lo fragment their lo words, both out of one 32-byte read.  A word is
+// (re, im), so A.B = sum ar br + ai bi = Re conj(a) b; B' = (im, -re) per
+// column, formed from B in registers (a 16-bit rotate and a sign flip),
+// gives A.B' = sum ar bi - ai br = Im conj(a) b.  Products hi.hi, hi.lo and
+// lo.hi carry ~16 significant bits of each operand.
+//
+// LDS: two stage buffers of a 128-row A tile and a 128-row B tile, rows of
+// 128 bytes = 8 chunks of 16 bytes, chunk c of row r at position
+// c ^ ((r / 2) % 8): the 16 rows of a fragment read fall on distinct banks.
+// Pre-split stacks are copied global -> LDS by the load unit
+// (global_load_lds_dwordx4: lane L of a wave writes 16 bytes at the wave's
+// base + 16 L, so lane L loads the chunk that belongs there), with no
+// registers, VALU or LDS write instructions; fp64 stacks are loaded, split
+// and written by the lanes in the same layout.  Stage s + 1 is fetched
+// while stage s is multiplied, one barrier per stage.
+//
+// Measured at 512 x 512 x 2^23 (C3): staging through registers with the
+// real and imaginary parts repacked (16 LDS writes and ~60 VALU per lane per
+// stage) ran 48.4 ms in the solve; this form 43.7 ms.  Without any staging
+// (LDS contents left as they are) the same loop runs at 99% of the matrix
+// cores' peak, so staging is what remains.  Fetching two or three stages
+// ahead (three or four buffers) was no faster (44.7, 45.6 ms).
+constexpr int kSplitKt = 16;  // complex k per stage = one bf16 MFMA k-step
 
 __device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
-
-// the (hi, lo) words of an operand element: fp64 values are split here,
-// pre-split stack entries are used as stored
-__device__ __forceinline__ uint2 operand_words(double2 v) { return split_word(v); }
-__device__ __forceinline__ uint2 operand_words(uint2 w) { return w; }
 __device__ __forceinline__ uint2 ld_or_zero(const uint2* p, bool ok) { return ok ? *p : make_uint2(0u, 0u); }
 
-// two consecutive operand elements e, e + 1 of a row.  rows16: K even, so
-// every pair (2m, 2m + 1) is one 16-byte aligned load of pre-split words.
-__device__ __forceinline__ void ld_pair(const uint2* p, bool rows16, uint2& x0, uint2& x1) {
-    if (rows16) {
-        const uint4 u = *reinterpret_cast<const uint4*>(p);
-        x0 = make_uint2(u.x, u.y);
-        x1 = make_uint2(u.z, u.w);
-    } else {
-        x0 = p[0];
-        x1 = p[1];
-    }
-}
-__device__ __forceinline__ void ld_pair(const double2* p, bool, double2& x0, double2& x1) {
-    x0 = p[0];
-    x1 = p[1];
+constexpr int kRawRowBytes = 128;
+constexpr int kRawTileBytes = kBigTile * kRawRowBytes;  // 16 KB
+
+__device__ __forceinline__ uint32_t prime_word(uint32_t w) {
+    return __builtin_amdgcn_alignbit(w, w, 16) ^ 0x80000000u;  // (re, im) -> (im, -re)
 }
 
-// real parts of two (re, im) bf16 words, and their imaginary parts
-__device__ __forceinline__ uint32_t re_pair(uint32_t w0, uint32_t w1) { return (w0 & 0xffffu) | (w1 << 16); }
-__device__ __forceinline__ uint32_t im_pair(uint32_t w0, uint32_t w1) { return (w0 >> 16) | (w1 & 0xffff0000u); }
+// Register staging of one 16-byte chunk (columns e, e + 1 of a row's stage;
+// zero past ke): fp64 stacks always, pre-split stacks for a short last stage.
+struct RawChunk {
+    uint2 w0, w1;
+    double2 v0, v1;
+};
+__device__ __forceinline__ void raw_fetch(RawChunk& ch, const uint2* src, bool full, int64_t e, int64_t ke) {
+    ch.w0 = ld_or_zero(src, full || e < ke);
+    ch.w1 = ld_or_zero(src + 1, full || e + 1 < ke);
+}
+__device__ __forceinline__ void raw_fetch(RawChunk& ch, const double2* src, bool full, int64_t e, int64_t ke) {
+    ch.v0 = ld_or_zero(src, full || e < ke);
+    ch.v1 = ld_or_zero(src + 1, full || e + 1 < ke);
+}
+__device__ __forceinline__ uint4 raw_words(const RawChunk& ch, const uint2*) {
+    return make_uint4(ch.w0.x, ch.w0.y, ch.w1.x, ch.w1.y);
+}
+__device__ __forceinline__ uint4 raw_words(const RawChunk& ch, const double2*) {
+    const uint2 w0 = split_word(ch.v0), w1 = split_word(ch.v1);
+    return make_uint4(w0.x, w0.y, w1.x, w1.y);
+}
 
-// T = double2: fp64 stacks, split at staging, row n at n * ldk; T = uint2:
-// stacks written pre-split by the recursion (the same words), half the bytes
-// to stage, k-blocked with block stride wbs (word_offset; wbs = 0: rows at
-// n * ldk)
 template <typename T>
 __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restrict__ A, const T* __restrict__ B,
                                                                int N, int64_t K, int64_t ldk, int64_t wbs, int64_t kc,
                                                                GammaMap map, double2* __restrict__ P) {
-    __shared__ __align__(16) uint32_t As[2][kBigTile * kSplitRowWords];
-    __shared__ __align__(16) uint32_t Bs[2][kBigTile * kSplitRowWords];
+    // two stage buffers (A tile, then B tile), as separate objects, each use
+    // site naming its own, so the compiler can tell the copies into one from
+    // the reads of the other
+    constexpr int kStageBytes = 2 * kRawTileBytes;
+    __shared__ __align__(1024) uint8_t S0[kStageBytes];
+    __shared__ __align__(1024) uint8_t S1[kStageBytes];
     int tn, tm;
     int64_t c;
     if (!xcd_work_item(map, tn, tm, c)) return;  // uniform over the workgroup
-    // 8 waves: wave w owns the 64 x 32 block (rows (w >> 2) * 64, columns
-    // (w & 3) * 32) of the 128 x 128 tile; two waves per SIMD, so one wave's
-    // staging overlaps another's matrix-core work
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int wn = wave >> 2, wm = wave & 3;
@@ -432,64 +453,58 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
     const int64_t kb = c * kc;
     const int64_t ke = (kb + kc < K) ? kb + kc : K;
 
-    // staging: thread t takes complex columns 2p, 2p + 1 (p = t % 8) of rows
-    // q and q + 64, q = 8 (t / 64) + 2 ((t / 8) % 4) + (t / 32) % 2: the 32
-    // lanes of one LDS write pass cover rows r, r + 2, r + 4, r + 6, whose
-    // 8-word groups fall on distinct banks
-    const int p = tid & 7;
-    const int q8 = (tid >> 3) & 7;
-    const int r0 = 8 * wave + 2 * (q8 & 3) + (q8 >> 2);
-    const T* ga[2];
-    const T* gb[2];
+    // copies: wave w moves 8-row groups 2w, 2w + 1 of A (i = 0, 1) and of B
+    // (i = 2, 3); lane L takes row r = 8 g + L / 8 and the logical chunk that
+    // belongs at byte 16 L of the group, (L % 8) ^ ((r / 2) % 8)
+    constexpr bool kPre = std::is_same<T, uint2>::value;  // pre-split, k-blocked (wbs != 0)
+    const int lr = lane >> 3;
+    int64_t roff[4];  // elements from a sub-stage's start to the lane's chunk
+    int lcol[4];      // the chunk's first column within the sub-stage
+    int dst_off[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int r = r0 + 64 * i;
-        const int rn = (n0 + r < N) ? n0 + r : N - 1;
-        const int rm = (m0 + r < N) ? m0 + r : N - 1;
-        const int64_t row_step = wbs ? kWordBlock : ldk;
-        ga[i] = A + static_cast<int64_t>(rn) * row_step;
-        gb[i] = B + static_cast<int64_t>(rm) * row_step;
+    for (int i = 0; i < 4; ++i) {
+        const int g = 2 * wave + (i & 1);
+        const int r = 8 * g + lr;
+        const int lci = (lane & 7) ^ ((r >> 1) & 7);
+        const bool isb = i >= 2;
+        const int row = isb ? ((m0 + r < N) ? m0 + r : N - 1) : ((n0 + r < N) ? n0 + r : N - 1);
+        const int64_t row_step = kPre ? kWordBlock : ldk;
+        roff[i] = static_cast<int64_t>(row) * row_step + 2 * lci;
+        lcol[i] = 2 * lci;
+        dst_off[i] = (isb ? kRawTileBytes : 0) + 8 * g * kRawRowBytes;
     }
-    T ra[2][2], rb[2][2];
-    // row starts, and so column pairs, 16-byte aligned (blocked: always)
-    const bool rows16 = wbs != 0 || (ldk & 1) == 0;
-    auto gload = [&](int64_t k0) {
-        const int64_t e = k0 + 2 * p;
-        // a stage of 16 columns lies within one word block
-        const int64_t o = word_offset(k0, wbs) + 2 * p;
-        if (k0 + kSplitKt <= ke) {  // a full stage (all but a row's last, short one): plain loads
+    RawChunk ch[4];
+    bool dma = false;  // the newest fetched stage went by the load unit
+    // stage s into buffer buf: a full stage of pre-split words by the load
+    // unit (a uniform base plus the lane's 32-bit offset), anything else
+    // through registers (put writes them)
+    auto fetch = [&](int64_t s, uint8_t* buf) {
+        const int64_t k0 = kb + s * kSplitKt;
+        const bool full = k0 + kSplitKt <= ke;
+        const int64_t o = word_offset(k0, wbs);
+        dma = kPre && full;
+        if (dma) {
+            const T* pa = A + o;
+            const T* pb = B + o;
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                ld_pair(ga[i] + o, rows16, ra[i][0], ra[i][1]);
-                ld_pair(gb[i] + o, rows16, rb[i][0], rb[i][1]);
+            for (int i = 0; i < 4; ++i) {
+                // The host pass of this kernel never runs it and cannot form
+                // an LDS address: the call exists in the device pass.
+#if defined(__HIP_DEVICE_COMPILE__)
+                __builtin_amdgcn_global_load_lds((i >= 2 ? pb : pa) + static_cast<uint32_t>(roff[i]),
+                                                 (__attribute__((address_space(3))) void*)(buf + dst_off[i]), 16, 0,
+                                                 0);
+#endif
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                ra[i][0] = ld_or_zero(ga[i] + o, e < ke);
-                ra[i][1] = ld_or_zero(ga[i] + o + 1, e + 1 < ke);
-                rb[i][0] = ld_or_zero(gb[i] + o, e < ke);
-                rb[i][1] = ld_or_zero(gb[i] + o + 1, e + 1 < ke);
-            }
+            for (int i = 0; i < 4; ++i) raw_fetch(ch[i], (i >= 2 ? B : A) + o + roff[i], full, k0 + lcol[i], ke);
         }
     };
-    auto sstore = [&](int buf) {
+    auto put = [&](uint8_t* buf) {
+        if (dma) return;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int r = r0 + 64 * i;
-            const uint2 a0 = operand_words(ra[i][0]), a1 = operand_words(ra[i][1]);
-            const uint2 b0 = operand_words(rb[i][0]), b1 = operand_words(rb[i][1]);
-            uint32_t* ar = &As[buf][r * kSplitRowWords + p];
-            ar[0] = re_pair(a0.x, a1.x);
-            ar[8] = im_pair(a0.x, a1.x);
-            ar[16] = re_pair(a0.y, a1.y);
-            ar[24] = im_pair(a0.y, a1.y);
-            uint32_t* br = &Bs[buf][r * kSplitRowWords + p];
-            br[0] = re_pair(b0.x, b1.x);
-            br[8] = im_pair(b0.x, b1.x);
-            br[16] = re_pair(b0.y, b1.y);
-            br[24] = im_pair(b0.y, b1.y);
-        }
+        for (int i = 0; i < 4; ++i) *reinterpret_cast<uint4*>(buf + dst_off[i] + 16 * lane) = raw_words(ch[i], A);
     };
 
     v4f re[4][2], im[4][2];
@@ -500,78 +515,78 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
             re[i][j] = v4f{0, 0, 0, 0};
             im[i][j] = v4f{0, 0, 0, 0};
         }
-
-    // Fragments of one part (hi or lo) of a stage: A of the wave's 4 row
-    // blocks; B and B' = (im, -re) of its 2 column blocks.  B' lanes: groups
-    // 0, 1 read the im words, 2, 3 the re words with the sign bits flipped.
-    struct FragB {
-        v8bf b[2], bp[2];
+    // the 32 bytes (chunks 2 lk, 2 lk + 1) of tile row r: hi words of columns
+    // 4 lk .. 4 lk + 3 in x, z of both, lo words in y, w
+    auto rd = [&](const uint8_t* tile, int r, uint4& x0, uint4& x1) {
+        const uint8_t* row = tile + r * kRawRowBytes;
+        const int sw = (r >> 1) & 7;
+        x0 = *reinterpret_cast<const uint4*>(row + (((2 * lk) ^ sw) << 4));
+        x1 = *reinterpret_cast<const uint4*>(row + (((2 * lk + 1) ^ sw) << 4));
     };
-    const int prime_off = lk < 2 ? 8 + 4 * lk : 4 * lk - 8;
-    const uint32_t prime_neg = lk < 2 ? 0u : 0x80008000u;
-    auto read_a = [&](v8bf (&a)[4], int buf, int part) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            a[i] = as_v8bf(*reinterpret_cast<const uint4*>(
-                &As[buf][(wn * 64 + 16 * i + li) * kSplitRowWords + 16 * part + 4 * lk]));
+    // fragments of a sub-stage: B, B' = (im, -re) and A, hi and lo parts
+    struct Frags {
+        v8bf bh[2], bl[2], bph[2], bpl[2], ah[4], al[4];
     };
-    auto read_b = [&](FragB& f, int buf, int part) {
+    auto read_sub = [&](const uint8_t* buf, Frags& f) {
+        const uint8_t* At = buf;
+        const uint8_t* Bt = buf + kRawTileBytes;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const uint32_t* row = &Bs[buf][(wm * 32 + 16 * j + li) * kSplitRowWords + 16 * part];
-            f.b[j] = as_v8bf(*reinterpret_cast<const uint4*>(row + 4 * lk));
-            const uint4 u = *reinterpret_cast<const uint4*>(row + prime_off);
-            f.bp[j] = as_v8bf(make_uint4(u.x ^ prime_neg, u.y ^ prime_neg, u.z ^ prime_neg, u.w ^ prime_neg));
+            uint4 x0, x1;
+            rd(Bt, wm * 32 + 16 * j + li, x0, x1);
+            f.bh[j] = as_v8bf(make_uint4(x0.x, x0.z, x1.x, x1.z));
+            f.bl[j] = as_v8bf(make_uint4(x0.y, x0.w, x1.y, x1.w));
+            f.bph[j] = as_v8bf(make_uint4(prime_word(x0.x), prime_word(x0.z), prime_word(x1.x), prime_word(x1.z)));
+            f.bpl[j] = as_v8bf(make_uint4(prime_word(x0.y), prime_word(x0.w), prime_word(x1.y), prime_word(x1.w)));
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint4 x0, x1;
+            rd(At, wn * 64 + 16 * i + li, x0, x1);
+            f.ah[i] = as_v8bf(make_uint4(x0.x, x0.z, x1.x, x1.z));
+            f.al[i] = as_v8bf(make_uint4(x0.y, x0.w, x1.y, x1.w));
         }
     };
-    // 16 products with independent accumulators
-    auto phase = [&](const v8bf (&a)[4], const FragB& f) {
+    // products of a sub-stage in three phases, hi.hi, hi.lo, lo.hi, each over
+    // the 16 accumulators, so consecutive products into one accumulator are
+    // 16 instructions apart
+    auto mul_sub = [&](const Frags& f) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int ph = 0; ph < 3; ++ph)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], f.b[j], re[i][j], 0, 0, 0);
-                im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], f.bp[j], im[i][j], 0, 0, 0);
-            }
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const v8bf& a = ph == 2 ? f.al[i] : f.ah[i];
+                    const v8bf& b = ph == 1 ? f.bl[j] : f.bh[j];
+                    const v8bf& bp = ph == 1 ? f.bpl[j] : f.bph[j];
+                    re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, re[i][j], 0, 0, 0);
+                    im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bp, im[i][j], 0, 0, 0);
+                }
     };
 
-    // Stage s sits in LDS buffer s % 2 and is multiplied in three phases,
-    // hi.hi, hi.lo, lo.hi, each reading the next phase's fragments while its
-    // products run; phase 3 reads stage s + 1's hi fragments.  The one
-    // barrier per stage comes after the last read of stage s's buffer, which
-    // then takes stage s + 2 (its global loads were issued at the start of
-    // the stage).
     const int64_t stages = (ke - kb + kSplitKt - 1) / kSplitKt;
-    v8bf ah[4], al[4];
-    FragB bhi, blo;
-    gload(kb);
-    sstore(0);
+    fetch(0, S0);
+    put(S0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    read_a(ah, 0, 0);
-    read_b(bhi, 0, 0);
-    if (stages > 1) {
-        gload(kb + kSplitKt);
-        sstore(1);
-    }
-    for (int64_t st = 0; st < stages; ++st) {
-        const int a = static_cast<int>(st & 1), b = a ^ 1;
-        if (st + 2 < stages) gload(kb + (st + 2) * kSplitKt);
-        read_b(blo, a, 1);
-        phase(ah, bhi);
-        read_a(al, a, 1);
-        phase(ah, blo);
+    // Stage st: its fragments are read from cur, stage st + 1 is fetched
+    // into nxt (after the reads: the compiler makes LDS reads that follow a
+    // copy into LDS wait for it), then the products.  Before the barrier this
+    // wave's copies have landed and its LDS writes are done; the barrier
+    // publishes stage st + 1 and frees cur.
+    auto step = [&](int64_t st, const uint8_t* cur, uint8_t* nxt) {
+        Frags f;
+        read_sub(cur, f);
+        const bool ahead = st + 1 < stages;
+        if (ahead) fetch(st + 1, nxt);
+        mul_sub(f);
+        if (ahead) put(nxt);
         __syncthreads();
-        if (st + 2 < stages) sstore(a);
-        v8bf ah_n[4];
-        FragB bhi_n;
-        if (st + 1 < stages) {
-            read_a(ah_n, b, 0);
-            read_b(bhi_n, b, 0);
-        }
-        phase(al, bhi);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) ah[i] = ah_n[i];
-        bhi = bhi_n;
+    };
+    for (int64_t st = 0; st < stages; st += 2) {
+        step(st, S0, S1);
+        if (st + 1 < stages) step(st + 1, S1, S0);
     }
 
     double2* Pc = P + c * static_cast<int64_t>(N) * N;
@@ -726,14 +741,14 @@ void launch_gamma_f64_big(const double2* A, const double2* B, int N, int64_t K, 
 
 void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t K, int64_t ldk, int64_t kc,
                              const GammaMap& map, double2* P, hipStream_t st) {
-    hipLaunchKernelGGL(gamma_bf16x3_big_kernel<double2>, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0,
-                       st, A, B, N, K, ldk, int64_t{0}, kc, map, P);
+    gamma_bf16x3_big_kernel<double2><<<dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0, st>>>(
+        A, B, N, K, ldk, int64_t{0}, kc, map, P);
 }
 
 void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t wbs, int64_t kc,
                                       const GammaMap& map, double2* P, hipStream_t st) {
-    hipLaunchKernelGGL(gamma_bf16x3_big_kernel<uint2>, dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0,
-                       st, A, B, N, K, K, wbs, kc, map, P);
+    gamma_bf16x3_big_kernel<uint2><<<dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0, st>>>(
+        A, B, N, K, K, wbs, kc, map, P);
 }
 
 namespace {
