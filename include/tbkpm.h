@@ -35,7 +35,8 @@ enum {
     TBK_E_INVALID = -1,     /* bad argument: n < 2, R < 1, null pointer, shape mismatch */
     TBK_E_DEVICE = -2,      /* HIP runtime or launch failure, or no device */
     TBK_E_OOM = -3,         /* device allocation failed */
-    TBK_E_UNSUPPORTED = -4  /* valid request outside what this build supports */
+    TBK_E_UNSUPPORTED = -4, /* valid request outside what this build supports */
+    TBK_E_CANCELLED = -5    /* a server request cancelled by tbk_server_cancel */
 };
 
 /* precision of a vector block and of the recursion run on it */
@@ -565,11 +566,21 @@ typedef struct tbk_server_stats {
     int64_t running_dos, running_kubo, kubo_done, chunks;
     int64_t used_bytes, budget_bytes;
     int64_t arena_bytes, arena_slabs; /* device memory the arena holds, in how many slabs */
+    int64_t cancelled;                /* requests ended by tbk_server_cancel (counted in failed too) */
 } tbk_server_stats;
 
 int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server** out);
-/* copy the request's inputs and queue it; TBK_E_OOM if its footprint exceeds the budget */
+/* copy the request's inputs and queue it; TBK_E_OOM if its footprint exceeds the budget.
+   Uploaded matrices are checked: every value finite, and H (DoS and sigma_xy) Hermitian to
+   1e-12 of its largest magnitude, same pattern (TBK_E_INVALID otherwise; the moments' doubling
+   identities and the spectral range need a Hermitian H).  A lattice's hopping table, phase
+   table and disorder must be finite; its Hermiticity is the lattice's own. */
 int tbk_server_submit(tbk_server* s, const tbk_request* r, int64_t* ticket);
+/* Cancel a queued or running request: it ends TBK_REQ_FAILED with TBK_E_CANCELLED.  A queued
+   request leaves the queue at once, a running DoS request at the end of the chunk in flight,
+   a running sigma_xy request when its enqueued work ends (its result is dropped).  TBK_E_INVALID
+   for an unknown ticket or a request already finished. */
+int tbk_server_cancel(tbk_server* s, int64_t ticket);
 int tbk_server_info(tbk_server* s, int64_t ticket, tbk_request_info* info);
 /* block until the request is done or failed, or timeout_s passes (< 0: no limit) */
 int tbk_server_wait(tbk_server* s, int64_t ticket, double timeout_s, tbk_request_info* info);

@@ -28,7 +28,7 @@ def build(tmp_path, name="dropin_dos"):
     return exe
 
 
-@pytest.mark.parametrize("name", ["dropin_dos", "kubo_xy_entry"])
+@pytest.mark.parametrize("name", ["dropin_dos", "kubo_xy_entry", "kubo_bastin"])
 def test_cpp_dropin_compiles_and_links(tmp_path, name):
     assert os.path.exists(build(tmp_path, name))
 
@@ -142,3 +142,72 @@ def test_named_entry_kubo_xy_at_full_size(tmp_path):
     Ho = lat.get_Hamiltonian_sparse(True)
     opn = [x.split()[1:] for x in lines if x.startswith("open ")][0]
     assert (int(opn[0]), int(opn[1])) == (Ho.n, int(Ho.colptr[-1]))
+
+
+@pytest.mark.gpu
+def test_cpp_conductivity_kubo_bastin_matches_oracle(tmp_path):
+    """conductivity_Kubo_Bastin_test (host/kubo.h; snippets.h:306-456) on a
+    12 x 12 open square lattice with a Landau gauge (B = 2 pi / 8), site
+    disorder and Volume = 2, against the oracle: Gamma from the stored
+    blocks, zeta by the reference's live DCT-III / FFT route, the kept nodes
+    and the integral of Re zeta by the reference's rule (kubo_sigma_reference)
+    and of Im zeta by the same loops; trace_braket_complex of two seeded
+    blocks; density_of_states_function = -f(E/a)/(4 pi) with Lorentz(N, 4)."""
+    L, N, vol = 12, 64, 2.0
+    out = subprocess.run([build(tmp_path, "kubo_bastin"), str(L), str(N)], capture_output=True, text=True,
+                         check=True, timeout=120)
+    lines = out.stdout.split("\n")
+    kub = np.array([[float(v) for v in x.split()[1:]] for x in lines if x.startswith("kubo ")])
+    cell = O.model("square")
+    cell.add_hopping_nn("A", 1, 1.0)
+    lat = O.PartialLattice(cell, (L, L))
+    lat.set_scalar_field_indexed(O.uniform_disorder(1.0, 31))
+    lat.gauge_field = O.landau_gauge(2 * math.pi / 8)
+    H = lat.get_Hamiltonian_sparse(True)
+    Emin, Emax = -5.0 - 1e-8, 5.0 + 1e-8
+    Hs = O.rescale(H, (Emax - Emin) / 2, (Emax + Emin) / 2)
+    vx = O.commutator_with_diagonal(lat.op_position(0), Hs)
+    vy = O.commutator_with_diagonal(lat.op_position(1), Hs)
+    n = Hs.n
+    kets = cpp_kets(n, 4, 0x5EED)
+    g = oracle.kernel_lorentz(N, 6.0)
+    TL = oracle.cheb_on_bases(Hs, N, kets) * g[:, None, None]
+    w = oracle.cheb_on_bases(vy, 2, kets)[1]
+    TR = oracle.cheb_on_bases(Hs, N, w)
+    aR = np.stack([oracle.cheb_on_bases(vx, 2, TR[m])[1] for m in range(N)]) * g[:, None, None]
+    eps, zeta = SO.kubo_zeta_fft(TL.reshape(N, -1), aR.reshape(N, -1), vol)
+    E, re_int = SO.kubo_sigma_reference(eps, zeta)
+    cut = N // 24
+    zr = np.real(zeta[cut:N - cut])[::-1]
+    zi = np.imag(zeta[cut:N - cut])[::-1]
+    assert kub.shape == (N - 2 * cut, 5)
+    assert np.max(np.abs(kub[:, 0] - E)) <= 1e-15
+    scale = np.max(np.abs(zeta[cut:N - cut]))
+    assert np.max(np.abs(kub[:, 1] - zr)) <= 1e-9 * scale
+    assert np.max(np.abs(kub[:, 2] - zi)) <= 1e-9 * scale
+    assert np.max(np.abs(kub[:, 3] - re_int)) <= 1e-9 * np.max(np.abs(re_int))
+    # Im zeta: the same outward loops, without the seeded pair (snippets.h:440-449)
+    e0 = int(np.searchsorted(E, 0.0, side="right"))
+    im_int = np.zeros_like(E)
+    for i in range(e0 + 1, len(E)):
+        im_int[i] = im_int[i - 1] + (zi[i] + zi[i - 1]) / 2 * (E[i] - E[i - 1])
+    for i in range(e0 - 2, -1, -1):
+        im_int[i] = im_int[i + 1] - (zi[i] + zi[i + 1]) / 2 * (E[i + 1] - E[i])
+    assert np.max(np.abs(kub[:, 4] - im_int)) <= 1e-9 * max(np.max(np.abs(im_int)), scale * 1e-3)
+    # trace_braket_complex(a, b) = sum conj(a) b over seeded blocks 77 and 78
+    tr = [complex(float(x.split()[1]), float(x.split()[2])) for x in lines if x.startswith("trace ")][0]
+    a, b = cpp_kets(n, 3, 77), cpp_kets(n, 3, 78)
+    assert abs(tr - np.sum(np.conj(a) * b)) <= 1e-14
+    # density_of_states_function on the periodic H, GreenFunction(N, Hp, -5, 5): seed 79
+    dosf = np.array([[float(v) for v in x.split()[1:]] for x in lines if x.startswith("dosf ")])
+    Hp = lat.get_Hamiltonian_periodic_extend_bare_sparse((1, 1), True)
+    Hps = O.rescale(Hp, 5.0, 0.0)
+    k3 = cpp_kets(n, 4, 79)
+    mu = oracle.cheb_braket(Hps, N, k3, k3)
+    gl = oracle.kernel_lorentz(N, 4.0)
+    want = []
+    for e in dosf[:, 0]:
+        al = SO.expn_coef_alpha_seq(complex(e / 5.0, 0.0), N)
+        want.append(-1.0 / 4 / math.pi * float(np.sum(gl * np.imag(al) * mu)))
+    want = np.array(want)
+    assert np.max(np.abs(dosf[:, 1] - want)) <= 1e-9 * np.max(np.abs(want))

@@ -308,3 +308,100 @@ def test_preallocated_arena_holds_the_budget(gpu_ctx):
         for i in range(3):
             _served_vs_single(gpu_ctx, srv, _square_lattice(200 + 50 * i, 0.5, 9 + i), 24, 16, 40 + i, 4.3)
         assert srv.stats()["arena_bytes"] == st["arena_bytes"]
+
+
+def _with_values(H, values):
+    from tight_binding_amd.sparse import CSC
+    return CSC(H.n, H.colptr.copy(), H.rowidx.copy(), np.asarray(values).copy())
+
+
+def test_server_cancel():
+    """tbk_server_cancel: a queued request leaves the queue, a running DoS
+    request at the next chunk, a sigma_xy request when its enqueued work
+    ends; each ends failed with TBK_E_CANCELLED (take raises it); unknown and
+    finished tickets are refused; the server keeps serving."""
+    H, a = model_problem("square", 256, 0.5, 5)
+    with kpm.Server(0, lazy_arena=True, max_slots=1, chunk_steps=1) as srv:
+        t1 = srv.submit_dos(H, 4096, a, 0.0, 16, 1)   # takes the only slot: 2048 one-step chunks
+        t2 = srv.submit_dos(H, 64, a, 0.0, 8, 2)      # waits for the slot
+        assert srv.info(t2)["state"] == Nn.TBK_REQ_QUEUED
+        srv.cancel(t2)
+        info2 = srv.info(t2)
+        assert info2["state"] == Nn.TBK_REQ_FAILED and info2["error_code"] == Nn.TBK_E_CANCELLED
+        t0 = time.time()
+        while srv.info(t1)["state"] != Nn.TBK_REQ_RUNNING and time.time() - t0 < 10:
+            time.sleep(0.001)
+        assert srv.info(t1)["state"] == Nn.TBK_REQ_RUNNING
+        srv.cancel(t1)
+        info1 = srv.wait(t1, 60.0)
+        assert info1["state"] == Nn.TBK_REQ_FAILED and info1["error_code"] == Nn.TBK_E_CANCELLED
+        for t in (t1, t2):
+            with pytest.raises(Nn.NativeError) as e:
+                srv.take(t)
+            assert e.value.code == Nn.TBK_E_CANCELLED
+        with pytest.raises(Nn.NativeError) as e:
+            srv.cancel(123456)
+        assert e.value.code == Nn.TBK_E_INVALID
+        # still serving: a new request equals the single path
+        t3 = srv.submit_dos(H, 64, a, 0.0, 8, 3)
+        assert srv.wait(t3, 60.0)["state"] == Nn.TBK_REQ_DONE
+        with pytest.raises(Nn.NativeError) as e:
+            srv.cancel(t3)
+        assert e.value.code == Nn.TBK_E_INVALID
+        mu, _ = srv.take(t3)
+        st = srv.stats()
+    ctx = kpm.Context(0)
+    ref, _, _ = ctx.moments(ctx.matrix(H), ctx.random_block(H.n, 8, seed=3), 64, a=a)
+    ctx.close()
+    assert np.max(np.abs(mu - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert st["cancelled"] == 2 and st["completed"] == 1
+
+    # sigma_xy: one running, one queued behind it (max_kubo = 1); the first
+    # may already have ended when its cancel arrives, then cancel is refused
+    Hs, vx, vy, g = landau_kubo(L0=64, N=512, R=16)
+    with kpm.Server(0, lazy_arena=True, max_kubo=1, max_kubo_moments=512) as srv:
+        k1 = srv.submit_kubo(Hs, vx, vy, g.size, R=16, seed=4, g=g)
+        k2 = srv.submit_kubo(Hs, vx, vy, g.size, R=16, seed=5, g=g)
+        srv.cancel(k2)
+        info = srv.wait(k2, 60.0)
+        assert info["state"] == Nn.TBK_REQ_FAILED and info["error_code"] == Nn.TBK_E_CANCELLED
+        try:
+            srv.cancel(k1)
+            info = srv.wait(k1, 60.0)
+            assert info["state"] == Nn.TBK_REQ_FAILED and info["error_code"] == Nn.TBK_E_CANCELLED
+        except Nn.NativeError as e:
+            assert e.code == Nn.TBK_E_INVALID and srv.info(k1)["state"] == Nn.TBK_REQ_DONE
+        assert srv.stats()["cancelled"] >= 1
+
+
+def test_server_rejects_non_finite_and_non_hermitian_matrices():
+    """Submission checks: a NaN or infinite value in H, v_x or v_y, and an H
+    that is not Hermitian, are refused with TBK_E_INVALID; an H that is
+    Hermitian to rounding (1e-14 relative asymmetry) is accepted."""
+    H, a = model_problem("honeycomb", 8, 1.0, 6)
+    Hs, vx, vy, g = landau_kubo()
+    with kpm.Server(0, lazy_arena=True) as srv:
+        for bad in (np.nan, np.inf):
+            v = H.values.copy()
+            v[3] = bad
+            with pytest.raises(Nn.NativeError) as e:
+                srv.submit_dos(_with_values(H, v), 32, a, 0.0, 8, 1)
+            assert e.value.code == Nn.TBK_E_INVALID and "non-finite" in str(e.value)
+        # one off-diagonal entry changed: no longer Hermitian
+        off = next(p for j in range(H.n) for p in range(H.colptr[j], H.colptr[j + 1]) if H.rowidx[p] != j)
+        v = H.values.copy()
+        v[off] += 0.25
+        with pytest.raises(Nn.NativeError) as e:
+            srv.submit_dos(_with_values(H, v), 32, a, 0.0, 8, 1)
+        assert e.value.code == Nn.TBK_E_INVALID and "Hermitian" in str(e.value)
+        v = H.values.copy()
+        v[off] *= 1 + 1e-14
+        t = srv.submit_dos(_with_values(H, v), 32, a, 0.0, 8, 1)
+        assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
+        srv.take(t)
+        # sigma_xy: a non-finite velocity value
+        w = vx.values.copy()
+        w[5] = np.nan
+        with pytest.raises(Nn.NativeError) as e:
+            srv.submit_kubo(Hs, _with_values(vx, w), vy, g.size, R=4, seed=1, g=g)
+        assert e.value.code == Nn.TBK_E_INVALID

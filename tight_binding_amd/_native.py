@@ -17,6 +17,7 @@ TBK_E_INVALID = -1
 TBK_E_DEVICE = -2
 TBK_E_OOM = -3
 TBK_E_UNSUPPORTED = -4
+TBK_E_CANCELLED = -5
 TBK_FP64 = 0
 TBK_FP32 = 1
 TBK_CSC = 0
@@ -245,7 +246,7 @@ class tbk_request_info(C.Structure):
 class tbk_server_stats(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "failed", "queued", "running_dos",
                                           "running_kubo", "kubo_done", "chunks", "used_bytes", "budget_bytes",
-                                          "arena_bytes", "arena_slabs")]
+                                          "arena_bytes", "arena_slabs", "cancelled")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -329,6 +330,7 @@ KPM_SIGNATURES = {
     "tbk_server_take": ([P, I64, DP, I64, C.POINTER(tbk_request_info)], I32),
     "tbk_server_get_stats": ([P, C.POINTER(tbk_server_stats)], I32),
     "tbk_server_destroy": ([P], I32),
+    "tbk_server_cancel": ([P, I64], I32),
 }
 
 HOST_SIGNATURES = {

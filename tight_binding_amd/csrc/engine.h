@@ -204,6 +204,8 @@ KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int R
                            int bandwidth, bool force_row, bool row_slabs = false);
 int bandwidth_percentile(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, double p);
 void build_csr(const tbk_sparse* M, std::vector<int32_t>& rp, std::vector<int32_t>& ci, std::vector<double>& val);
+bool csr_hermitian_within(int32_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                          const std::vector<double>& val, bool cplx, double tol);
 void matrix_gershgorin(const tbk_matrix* M, hipStream_t st);
 uint64_t matrix_pattern(const tbk_matrix* M, hipStream_t st);
 tbk_context* default_context();

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -149,6 +150,7 @@ struct Request {
     int32_t Gw = 0;            // sigma_xy: random vectors per group (a power of two <= Rp; see kubo_footprint)
     int32_t row_chunks = 1;    // sigma_xy, fp64 stacks: row ranges the stacks hold one at a time
     int64_t footprint = 0;
+    std::atomic<bool> cancel{false};  // tbk_server_cancel on a running request
     Slab* slab = nullptr;      // its span of the arena while admitted (null: a new slab is being made for it)
     size_t arena_off = 0;
     size_t new_slab = 0;       // bytes of that new slab
@@ -288,6 +290,7 @@ struct Slot {
     tbk::TileOrder order{};  // row tiles of one workgroup pass, chunks of 256 rows, transposed
     int32_t nwg = 8;         // workgroups of the slot in a launch (multiple of 8)
     int32_t pin = -1;        // row of the pinned moment staging
+    bool cancelled = false;  // leaves without a result
     ~Slot() {
         if (M) tbk_matrix_destroy(M);
     }
@@ -557,6 +560,7 @@ void tbk_server::finish(Request* r, int32_t code, const std::string& msg) {
         st.completed++;
     else
         st.failed++;
+    if (code == TBK_E_CANCELLED) st.cancelled++;
     if (r->kind == TBK_REQ_DOS) st.running_dos--;
     else st.running_kubo--;
     r->H.clear();
@@ -856,7 +860,8 @@ void tbk_server::poll_kubo(bool block) {
         Request* r = j->req;
         const int64_t N = r->n, Ncap = cfg.max_kubo_moments;
         const double* pz = pin_kubo + static_cast<size_t>(j->pin) * static_cast<size_t>(2 * Ncap + 2 * Ncap * Ncap);
-        r->result.assign(pz, pz + 2 * N + 2 * N * N);
+        const bool dropped = r->cancel.load();
+        if (!dropped) r->result.assign(pz, pz + 2 * N + 2 * N * N);
         const int32_t pin = j->pin;
         kjobs.erase(kjobs.begin() + static_cast<std::ptrdiff_t>(i));
         delete j;
@@ -865,7 +870,10 @@ void tbk_server::poll_kubo(bool block) {
             free_kpin.push_back(pin);
             st.kubo_done++;
         }
-        finish(r, TBK_OK, "");
+        if (dropped)
+            finish(r, TBK_E_CANCELLED, "cancelled");
+        else
+            finish(r, TBK_OK, "");
     }
 }
 
@@ -873,6 +881,17 @@ void tbk_server::enqueue_chunk() {
     const int32_t S = cfg.chunk_steps;
     Chunk c;
     c.parity = static_cast<int>(chunk_seq & 1);
+    // cancelled slots leave without running: freed once this chunk (and so
+    // the one before it, which may still use them) has completed
+    for (size_t i = 0; i < slots.size();) {
+        if (slots[i]->req->cancel.load()) {
+            slots[i]->cancelled = true;
+            c.finishing.push_back(slots[i]);
+            slots.erase(slots.begin() + static_cast<std::ptrdiff_t>(i));
+        } else {
+            ++i;
+        }
+    }
     hipStream_t st_ = ctx->stream;
     // group the slots by class
     std::map<ClassKey, std::vector<Slot*>> groups;
@@ -969,6 +988,10 @@ void tbk_server::complete_chunk(Chunk& c) {
     free_events.push_back(c.ev);
     for (Slot* s : c.finishing) {
         Request* r = s->req;
+        if (s->cancelled) {
+            finish(r, TBK_E_CANCELLED, "cancelled");
+            continue;
+        }
         const double* src = pin_mu + static_cast<size_t>(s->pin) * static_cast<size_t>(cfg.max_moments);
         r->result.assign(src, src + r->n);
         finish(r, TBK_OK, "");
@@ -1112,6 +1135,23 @@ void tbk_server::run_prep() {
 
 namespace {
 
+void check_values(const HostSparse& m, const char* name) {
+    for (double v : m.values)
+        if (!std::isfinite(v)) raise(TBK_E_INVALID, std::string("non-finite value in ") + name);
+}
+
+void check_lattice(const HostLattice& l) {
+    auto finite = [](const std::vector<double>& v) {
+        for (double x : v)
+            if (!std::isfinite(x)) return false;
+        return true;
+    };
+    if (!finite(l.t) || !finite(l.phase) || !finite(l.apos) || !std::isfinite(l.desc.disorder_W))
+        raise(TBK_E_INVALID, "non-finite hopping, phase, position or disorder value in the lattice");
+    for (double b : l.desc.bases)
+        if (!std::isfinite(b)) raise(TBK_E_INVALID, "non-finite lattice basis");
+}
+
 Lease::~Lease() {
     if (srv) srv->give_back(slab, off, region.cap);
 }
@@ -1229,10 +1269,17 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
         }
         if (q->H) {
             r->H.copy_from(q->H);
+            check_values(r->H, "H");
+            std::vector<int32_t> rp, ci;
+            std::vector<double> val;
+            build_csr(&r->H.hdr, rp, ci, val);
+            if (!csr_hermitian_within(r->H.hdr.rows, rp, ci, val, r->H.hdr.is_complex != 0, 1e-12))
+                raise(TBK_E_INVALID, "H is not Hermitian (M^H differs from M beyond 1e-12 of its largest value)");
         } else {
             // a lattice: the device assembles it at admission; H.hdr carries
             // the size, the value type and a bound of the stored entries
             r->lat.copy_from(q->lattice);
+            check_lattice(r->lat);
             const int64_t rows = lattice_rows(&r->lat.desc);
             if (rows > INT32_MAX) raise(TBK_E_UNSUPPORTED, "more than 2^31 - 1 sites");
             r->H.hdr.rows = r->H.hdr.cols = static_cast<int32_t>(rows);
@@ -1249,6 +1296,8 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
         } else if (q->kind == TBK_REQ_KUBO) {
             r->vx.copy_from(q->vx);
             r->vy.copy_from(q->vy);
+            check_values(r->vx, "v_x");
+            check_values(r->vy, "v_y");
             if (r->vx.hdr.rows != r->H.hdr.rows || r->vy.hdr.rows != r->H.hdr.rows)
                 raise(TBK_E_INVALID, "velocity operators and H differ in size");
             if (q->g) r->g.assign(q->g, q->g + q->n_moments);
@@ -1316,6 +1365,40 @@ int tbk_server_take(tbk_server* s, int64_t ticket, double* out, int64_t capacity
         if (static_cast<int64_t>(r->result.size()) > capacity) raise(TBK_E_INVALID, "output buffer too small");
         if (out && !r->result.empty()) std::memcpy(out, r->result.data(), r->result.size() * sizeof(double));
         s->reqs.erase(it);
+    });
+}
+
+int tbk_server_cancel(tbk_server* s, int64_t ticket) {
+    return guarded([&] {
+        if (!s) raise(TBK_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) raise(TBK_E_INVALID, "unknown ticket");
+        Request* r = it->second.get();
+        if (r->state == TBK_REQ_QUEUED) {
+            auto q = std::find(s->queue.begin(), s->queue.end(), r);
+            if (q != s->queue.end()) s->queue.erase(q);
+            r->state = TBK_REQ_FAILED;
+            r->error_code = TBK_E_CANCELLED;
+            r->error = "cancelled";
+            r->t_done = now_s();
+            r->H.clear();
+            r->vx.clear();
+            r->vy.clear();
+            r->lat.clear();
+            s->st.failed++;
+            s->st.cancelled++;
+            s->st.queued = static_cast<int64_t>(s->queue.size());
+            s->done_cv.notify_all();
+            s->prep_gen++;
+            s->cv_prep.notify_all();
+            return;
+        }
+        if (r->state == TBK_REQ_RUNNING) {
+            r->cancel.store(true);  // the worker or the preparation thread ends it
+            return;
+        }
+        raise(TBK_E_INVALID, "request already finished");
     });
 }
 

@@ -220,6 +220,49 @@ bool csr_is_hermitian(int32_t n, const std::vector<int32_t>& rp, const std::vect
     return true;
 }
 
+// M^H = M to within tol of the largest magnitude, pattern included (a
+// structural mismatch or an asymmetric pair beyond tol: false).  Rows are
+// put in column order first, so any order of a row's entries is accepted.
+bool csr_hermitian_within(int32_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci_in,
+                          const std::vector<double>& val_in, bool cplx, double tol) {
+    const int vw = cplx ? 2 : 1;
+    const size_t nnz = ci_in.size();
+    std::vector<int32_t> ci(ci_in);
+    std::vector<double> val(val_in);
+    {
+        std::vector<std::pair<int32_t, int64_t>> row;
+        for (int32_t r = 0; r < n; ++r) {
+            row.clear();
+            for (int64_t p = rp[r]; p < rp[r + 1]; ++p) row.emplace_back(ci_in[p], p);
+            std::stable_sort(row.begin(), row.end(),
+                             [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (size_t k = 0; k < row.size(); ++k) {
+                const int64_t dst = rp[r] + static_cast<int64_t>(k), src = row[k].second;
+                ci[dst] = row[k].first;
+                for (int c = 0; c < vw; ++c) val[dst * vw + c] = val_in[src * vw + c];
+            }
+        }
+    }
+    double amax = 0.0;
+    for (double v : val) amax = std::max(amax, std::fabs(v));
+    const double lim = tol * amax;
+    std::vector<int32_t> tp(static_cast<size_t>(n) + 1, 0);
+    for (size_t p = 0; p < nnz; ++p) tp[ci[p] + 1]++;
+    for (int32_t i = 0; i < n; ++i) tp[i + 1] += tp[i];
+    if (tp != rp) return false;
+    std::vector<int32_t> fill(tp.begin(), tp.end() - 1);
+    for (int32_t r = 0; r < n; ++r)
+        for (int32_t p = rp[r]; p < rp[r + 1]; ++p) {
+            const int32_t q = fill[ci[p]]++;
+            if (ci[q] != r) return false;
+            const double* a = &val[static_cast<size_t>(p) * vw];
+            const double* b = &val[static_cast<size_t>(q) * vw];
+            if (!(std::fabs(a[0] - b[0]) <= lim)) return false;
+            if (cplx && !(std::fabs(a[1] + b[1]) <= lim)) return false;
+        }
+    return true;
+}
+
 void matrix_fp32(tbk_matrix* m) {
     if (m->have32) return;
     const int64_t n = m->nnz * (m->is_complex ? 2 : 1);

@@ -49,6 +49,8 @@ struct PartialLattice<Dim>::GreenFunction {
         scaled_HE = rescale(HE, a, b);
     }
     double scale(double x) const { return (x - b) / a; }
+    // (x - b id) / a, the reference's two-argument form (lattice.h:550)
+    Complex scale(Complex x, Complex id) const { return (x - b * id) / a; }
     std::pair<double, double> spectral_range() const { return {E_min, E_max}; }
 
     struct GreenFunctionImag {
@@ -87,6 +89,21 @@ struct PartialLattice<Dim>::GreenFunction {
     };
     GreenFunctionImag imag(const Kets& bras_T, const Kets& kets) { return GreenFunctionImag(this, bras_T, kets); }
 };
+
+// A functor E -> DoS(E) of a GreenFunction (functions.h:5-13): 4 random-phase
+// vectors, f = G.imag(kets, kets) (moments on the device), and
+// DoS(E) = -f(omega)/(4 pi) at omega = G.scale(E, i) = (E - i b)/a, the
+// reference's argument (its imaginary part is -b/a; zero for symmetric
+// bounds).  The functor refers to G, which must outlive it.
+template <typename GF>
+auto density_of_states_function(GF& G) {
+    MatrixXcd kets = numerical::linalg::bases_of_uniform_ampl_with_random_phases(G.scaled_HE.rows(), 4);
+    auto f = G.imag(kets, kets);
+    return [f, &G](double E) -> double {
+        const Complex omega_t = G.scale(Complex(E, 0.0), Complex(0.0, 1.0));
+        return -1.0 / 4 / numerical::pi * f(omega_t);
+    };
+}
 
 // DoS(e) = -f(e)/(R pi) at rescaled energies e (functions.h:16-33): periodic
 // H (extend 1 along every axis), bounds padded by 1e-8, R = 4 random-phase

@@ -78,6 +78,14 @@ inline double trace_braket(const MatrixXcd& bras_T, const MatrixXcd& kets) {
     return t;
 }
 
+// tr(bras^H kets) = sum conj(bras_T(i, j)) kets(i, j)  (numerical.h:137-144)
+inline Complex trace_braket_complex(const MatrixXcd& bras_T, const MatrixXcd& kets) {
+    Complex t = 0.0;
+    for (int64_t i = 0; i < bras_T.rows(); ++i)
+        for (int64_t j = 0; j < bras_T.cols(); ++j) t += std::conj(bras_T(i, j)) * kets(i, j);
+    return t;
+}
+
 }  // namespace linalg
 
 namespace Chebyshev {

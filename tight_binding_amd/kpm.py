@@ -579,6 +579,12 @@ class Server:
         G = out[2 * n:].view(np.complex128).reshape(n, n)
         return (z, G), inf.as_dict()
 
+    def cancel(self, ticket: int) -> None:
+        """Cancel a queued or running request (tbk_server_cancel): it ends
+        failed with TBK_E_CANCELLED (take raises); a running sigma_xy
+        request's enqueued work still runs."""
+        N.check_kpm(N.kpm().tbk_server_cancel(self._h, int(ticket)))
+
     def stats(self) -> dict:
         st = N.tbk_server_stats()
         N.check_kpm(N.kpm().tbk_server_get_stats(self._h, C.byref(st)))
