@@ -326,6 +326,14 @@ __device__ __forceinline__ void st_row_stream(V2* p, const V2* v, int opts) {
 // One site row r of a step (q: this lane's column group): the neighbour
 // gathers, y_r = alpha*acc + beta*x_r - z_r, and the moment partials of the
 // mode added to p0, p1.
+// Units whose second product (the fused operator hv2) is used: all of them,
+// except in the paired modes with two or more units per lane, whose first
+// CPL / 2 units are left-half columns that store only the first product.
+template <bool PAIR, int CPL>
+__device__ __forceinline__ constexpr bool dual_unit(int c) {
+    return !PAIR || CPL < 2 || c >= CPL / 2;
+}
+
 template <typename VT, bool CH, int LPR, int CPL, int MODE>
 __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                          const typename HVal<VT, CH>::type* __restrict__ hv,
@@ -385,7 +393,8 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             for (int c = 0; c < CPL; ++c) madd(acc[c], hh[k], xv[k][c]);
             if (DUAL) {
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) madd(acc2[c], hh2[k], xv[k][c]);
+                for (int c = 0; c < CPL; ++c)
+                    if (dual_unit<PAIR, CPL>(c)) madd(acc2[c], hh2[k], xv[k][c]);
             }
             if (cc[k] == r) {
                 have_self = true;
@@ -404,7 +413,8 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
         if (DUAL) {
             const H hh2 = hv2[p];
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) madd(acc2[c], hh2, xv[c]);
+            for (int c = 0; c < CPL; ++c)
+                if (dual_unit<PAIR, CPL>(c)) madd(acc2[c], hh2, xv[c]);
         }
         if (cc == r) {
             have_self = true;
