@@ -79,8 +79,6 @@ typedef struct tbk_stats {
     int32_t slab_cols;      /* vector columns one sweep over the rows covers */
     int32_t chunk_rows;     /* rows per chunk of the visiting order */
     int32_t chunk_period;   /* period P of the transposed chunk order (1: row order) */
-    int32_t pair_launches;  /* launches that advanced two steps (tbk_tuning.pair_steps) */
-    int32_t pair_tiles;     /* work items per front of those launches (0: none) */
 } tbk_stats;
 
 /* Launch-shape choices of a context (tbk_autotune sets these; zero fields
@@ -105,10 +103,6 @@ typedef struct tbk_tuning {
     int32_t kubo_stack_mib;  /* tbk_kubo_gamma: the most device memory (MiB) its vector stacks and
                                 work blocks may take; beyond it the random vectors run in groups
                                 (0: 95% of the free memory) */
-    int32_t pair_steps;      /* moment recursions with the row kernel over whole rows: 1 advances two
-                                steps per launch (T_{k+1} is gathered again while still in cache),
-                                0 or -1 one step per launch */
-    int32_t pair_tile_rows;  /* rows per work item of the two-step launch (0: 64) */
 } tbk_tuning;
 
 const char* tbk_version(void);

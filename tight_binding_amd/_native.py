@@ -111,8 +111,6 @@ class tbk_stats(C.Structure):
         ("slab_cols", C.c_int32),
         ("chunk_rows", C.c_int32),
         ("chunk_period", C.c_int32),
-        ("pair_launches", C.c_int32),
-        ("pair_tiles", C.c_int32),
     ]
 
     def as_dict(self):
@@ -132,8 +130,6 @@ class tbk_tuning(C.Structure):
         ("row_lane_bytes", C.c_int32),
         ("row_slab_cols", C.c_int32),
         ("kubo_stack_mib", C.c_int32),
-        ("pair_steps", C.c_int32),
-        ("pair_tile_rows", C.c_int32),
     ]
 
 

@@ -37,23 +37,6 @@ __global__ __launch_bounds__(kBlock) void reduce_partials_kernel(const double2* 
     if (threadIdx.x == 0) out[s] = make_double2(r0[0], r1[0]);
 }
 
-// After the last two-step launch of a run: every queue was served.
-__global__ __launch_bounds__(kBlock) void pair_check_kernel(PairSched s) {
-    if (static_cast<int>(threadIdx.x) < s.nq) pair_check_served(s, threadIdx.x);
-}
-
-// XCC_ID of each workgroup of the launch (the XCD it ran on).
-__global__ __launch_bounds__(kBlock) void xcc_probe_kernel(int32_t* __restrict__ out) {
-    if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
-}
-
-// p[0 .. n) = 0 (16-byte words)
-__global__ __launch_bounds__(kBlock) void zero16_kernel(double2* __restrict__ p, int64_t n) {
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
-        p[i] = make_double2(0.0, 0.0);
-}
-
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -874,192 +874,4 @@ __global__ __launch_bounds__(kBlock) void cheb_tile_kernel(
     }
 }
 
-// ------------------------------------------------------ two steps per launch
-// cheb_pair_kernel advances the recursion by two steps on blocks P (T_{k-1})
-// and Q (T_k), in place: front 1 of tile i stores T_{k+1} rows over P[i],
-// front 2 of tile j stores T_{k+2} rows over Q[j].  The tiles, slabs, queues
-// and the tile ranges a front-2 tile waits on come from the host schedule
-// (build_pair_schedule); cheb_pair_prologue_kernel checks that the previous
-// launch served every queue, resets the queues and flags, and computes
-// front 1 of the band tiles, the ones other slabs wait on.
-//
-// Queue q is served by the workgroups running on XCD q (the XCC_ID hardware
-// register, mapped to a queue by xcc_queue), so a slab's rows, its flags and
-// every producer and consumer of them share one L2.  A workgroup draws
-// tickets from its queue's counter in order, one ahead of the item it works
-// on.  Front-1 items never wait; a front-2 item waits for the done flags of
-// its ranges, all of them front-1 items with lower tickets of the same queue
-// or band tiles finished before the launch, so every wait ends.  A wait is
-// still bounded: past spin_limit polls, or once another wait has given up, it
-// sets *error and goes on, and the host discards the launch's results.
-//
-// Ordering within one XCD: each wave of a front-1 item waits until L2 has
-// acknowledged its row stores, the workgroup meets at a barrier, and thread 0
-// stores the tile's flag (an L2 atomic); a front-2 item polls the flags with
-// L2 loads and invalidates its CU's L1 (agent-scope acquire) before the
-// barrier that releases its waves to gather.  No L2 write-back is needed:
-// the data never leaves the L2 that every reader of it uses.  (An
-// agent-scope release, which writes back the whole L2 at every tile, made
-// the launch two to three times slower.)
-struct PairSched {
-    const int32_t* items;      // queue items: t >= 0 front-1 tile t, ~t front-2 tile t
-    const int32_t* qoff;       // queue q's items at [qoff[q], qoff[q + 1])
-    const int32_t* dep_ptr;    // front 2 of tile j waits on the tile ranges [dep_ptr[j], dep_ptr[j + 1])
-    const int32_t* dep_rng;    // 2 per range: first and last tile
-    const int32_t* band;       // band tiles (front 1 in the prologue)
-    const int32_t* done_init;  // per tile: 1 for band tiles, else 0
-    int32_t* counters;         // nq ticket counters
-    int32_t* done;             // per tile: 1 once T_{k+1} rows of the tile are stored
-    int32_t* error;            // set by a wait that gave up or by an unserved queue
-    int32_t nq, nband, ntiles, tt;
-    int32_t spin_limit;
-    int32_t check_prev;        // prologue: the counters hold a previous launch's draws
-    int32_t xcc_queue[16];     // queue of each XCC_ID value, -1: none
-};
-
-// XCC_ID hardware register (bits 3:0): the XCD the wave runs on
-__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
-
-// Every queue of the previous launch drew all its tickets (a workgroup that
-// draws a ticket works on it), else *error.
-__device__ __forceinline__ void pair_check_served(const PairSched& s, int q) {
-    if (s.counters[q] < s.qoff[q + 1] - s.qoff[q])
-        __hip_atomic_store(s.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Front rows of tile t: rows [t*tt, t*tt + tt), TR rows per pass of the
-// workgroup, the lanes' columns shifted by q0 units (a sweep of LPR*CPL
-// columns of a wider block); y = alpha*(H x) + beta*x - z over the tile,
-// partials of MODE reduced over the workgroup and stored by thread 0 at *part.
-template <typename VT, bool CH, int LPR, int CPL, int MODE>
-__device__ __forceinline__ void pair_tile(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                          const typename HVal<VT, CH>::type* __restrict__ hv,
-                                          const typename Vec2<VT>::type* __restrict__ x,
-                                          const typename Vec2<VT>::type* z, typename Vec2<VT>::type* y,
-                                          const typename Vec2<VT>::type* __restrict__ bra, int32_t rows,
-                                          int32_t Rp, int32_t q0, VT alpha, VT beta, int32_t opts, int64_t t,
-                                          int32_t tt, double2* part, double (*red)[kWavesPerBlock]) {
-    constexpr int RPW = kWave / LPR;
-    constexpr int TR = kWavesPerBlock * RPW;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int sub = lane / LPR;
-    const int q = lane % LPR;
-    double p0 = 0.0, p1 = 0.0;
-    const int64_t r0 = t * tt;
-    for (int it = 0; it < tt; it += TR) {
-        int64_t rr = r0 + it + wave * RPW + sub;
-        if (rr >= rows) break;
-        int r = static_cast<int>(rr);
-        if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
-        step_row<VT, CH, LPR, CPL, MODE>(rp, ci, hv, x, z, y, bra, r, q + q0, Rp, alpha, beta, opts, p0, p1);
-    }
-    p0 = wave_sum(p0);
-    p1 = wave_sum(p1);
-    if (lane == 0) {
-        red[0][wave] = p0;
-        red[1][wave] = p1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s0 = red[0][0], s1 = red[1][0];
-        for (int w = 1; w < kWavesPerBlock; ++w) {
-            s0 = s0 + red[0][w];
-            s1 = s1 + red[1][w];
-        }
-        *part = make_double2(s0, s1);
-    }
-}
-
-template <typename VT, bool CH, int LPR, int CPL, int MODE>
-__global__ __launch_bounds__(kBlock) void cheb_pair_prologue_kernel(
-    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-    const typename HVal<VT, CH>::type* __restrict__ hv, typename Vec2<VT>::type* P,
-    const typename Vec2<VT>::type* __restrict__ Q, const typename Vec2<VT>::type* __restrict__ bra, int32_t rows,
-    int32_t Rp, int32_t q0, VT alpha, VT beta, int32_t opts, PairSched s, double2* __restrict__ part1) {
-    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    const int64_t gsz = static_cast<int64_t>(gridDim.x) * kBlock;
-    for (int64_t i = gid; i < s.ntiles; i += gsz) s.done[i] = s.done_init[i];
-    if (gid < s.nq) {
-        if (s.check_prev) pair_check_served(s, static_cast<int>(gid));
-        s.counters[gid] = 0;
-    }
-    __shared__ double red[2][kWavesPerBlock];
-    for (int32_t b = blockIdx.x; b < s.nband; b += gridDim.x) {
-        const int32_t t = s.band[b];
-        pair_tile<VT, CH, LPR, CPL, MODE>(rp, ci, hv, Q, P, P, bra, rows, Rp, q0, alpha, beta, opts, t, s.tt,
-                                          part1 + t, red);
-        __syncthreads();  // red is reused by the next tile
-    }
-}
-
-template <typename VT, bool CH, int LPR, int CPL, int MODE>
-__global__ __launch_bounds__(kBlock) void cheb_pair_kernel(
-    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-    const typename HVal<VT, CH>::type* __restrict__ hv, typename Vec2<VT>::type* P, typename Vec2<VT>::type* Q,
-    const typename Vec2<VT>::type* __restrict__ bra, int32_t rows, int32_t Rp, int32_t q0, VT alpha, VT beta,
-    int32_t opts, PairSched s, double2* __restrict__ part1, double2* __restrict__ part2) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int qid = s.xcc_queue[xcc_id()];
-    if (qid < 0) return;  // an XCD without a queue (the whole workgroup, before any barrier)
-    const int32_t* items = s.items + s.qoff[qid];
-    const int32_t len = s.qoff[qid + 1] - s.qoff[qid];
-    int32_t* counter = s.counters + qid;
-
-    __shared__ double red[2][kWavesPerBlock];
-    __shared__ int32_t ticket[2];
-    if (threadIdx.x == 0)
-        ticket[0] = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    int32_t cur = ticket[0];
-    while (cur < len) {
-        // the next ticket is drawn now and read after this item's barriers
-        if (threadIdx.x == 0)
-            ticket[1] = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int32_t item = items[cur];
-        if (item >= 0) {
-            pair_tile<VT, CH, LPR, CPL, MODE>(rp, ci, hv, Q, P, P, bra, rows, Rp, q0, alpha, beta, opts, item, s.tt,
-                                              part1 + item, red);
-            // every wave waits until L2 has acknowledged its row stores; the
-            // flag then goes to the same L2
-            __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            __syncthreads();
-            if (threadIdx.x == 0) __hip_atomic_store(s.done + item, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const int32_t j = ~item;
-            if (wave == 0) {
-                bool ok = true;
-                int32_t spins = 0;
-                for (int32_t g = s.dep_ptr[j]; g < s.dep_ptr[j + 1] && ok; ++g) {
-                    const int32_t lo = s.dep_rng[2 * g], hi = s.dep_rng[2 * g + 1];
-                    for (int32_t base = lo; base <= hi && ok; base += kWave) {
-                        const int32_t f = base + lane;
-                        for (;;) {
-                            const int32_t v = f <= hi ? __hip_atomic_load(s.done + f, __ATOMIC_RELAXED,
-                                                                          __HIP_MEMORY_SCOPE_AGENT)
-                                                      : 1;
-                            if (__all(v != 0)) break;
-                            if (++spins > s.spin_limit ||
-                                __hip_atomic_load(s.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                                ok = false;
-                                break;
-                            }
-                            __builtin_amdgcn_s_sleep(2);
-                        }
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                if (lane == 0 && !ok) __hip_atomic_store(s.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            __syncthreads();
-            pair_tile<VT, CH, LPR, CPL, MODE>(rp, ci, hv, P, Q, Q, bra, rows, Rp, q0, alpha, beta, opts, j, s.tt,
-                                              part2 + j, red);
-        }
-        __syncthreads();  // ticket[1] stored; red free again
-        cur = ticket[1];
-        __syncthreads();  // everyone has read ticket[1] before thread 0 overwrites it
-    }
-}
-
 }  // namespace tbk

@@ -7,7 +7,6 @@
 
 #include <algorithm>
 #include <cstdint>
-#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -140,22 +139,8 @@ struct tbk_context {
     engine::DevBuf gamma_part, gamma_dev;    // split-K Gamma partials and the reduced Gamma
     std::vector<hipEvent_t> events;
     int64_t held_bytes = 0;  // matrices + blocks owned through this context
-    // XCDs seen by a probe launch (two-step launches): 0 not probed, 1 the
-    // eight XCC_IDs 0..7 and no other, -1 otherwise
-    int32_t xcc_state = 0;
     tbk_tuning tuning{};     // zeros: automatic choices
 };
-
-namespace engine {
-// Device copy of a two-step schedule (pair_plan.h) for one matrix and tile
-// size, formed on first use and kept with the matrix (one per tile size).
-struct PairDev {
-    int32_t tt = 0, nq = 0, slack = 0;
-    bool ok = false;  // false: the matrix does not suit two-step launches
-    int32_t ntiles = 0, nband = 0;
-    DevBuf items, qoff, dep_ptr, dep_rng, band, done_init, counters, done, error;
-};
-}  // namespace engine
 
 struct tbk_matrix {
     tbk_context* ctx = nullptr;
@@ -170,7 +155,6 @@ struct tbk_matrix {
     uint64_t pattern = 0;                     // fingerprint of (rows, row pointers, columns); 0: not formed
     engine::DevBuf rp, ci, val64, val32;
     bool have32 = false;
-    mutable std::vector<std::unique_ptr<engine::PairDev>> pairs;  // two-step schedules by tile size
 };
 
 struct tbk_block {

@@ -78,76 +78,6 @@ bool launch_slots(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a)
     return false;
 }
 
-// Arguments of the two-step launch (cheb_pair_kernel and its prologue).
-struct PairArgs {
-    const int32_t* rp;
-    const int32_t* ci;
-    const void* hv;
-    void* P;  // T_{k-1} in, T_{k+1} out
-    void* Q;  // T_k in, T_{k+2} out
-    const void* bra;
-    int32_t rows, Rp;
-    int32_t q0;  // column sweep: lane units shifted by q0
-    double alpha, beta;
-    int32_t opts;
-    PairSched sched;
-    double2* part1;  // per tile: partials of step k (front 1)
-    double2* part2;  // per tile: partials of step k + 1 (front 2)
-};
-
-bool launch_pair_f64_real(int width, int cpl, int mode, bool prologue, dim3 grid, hipStream_t st, const PairArgs& a);
-bool launch_pair_f64_cplx(int width, int cpl, int mode, bool prologue, dim3 grid, hipStream_t st, const PairArgs& a);
-bool launch_pair_f32_real(int width, int cpl, int mode, bool prologue, dim3 grid, hipStream_t st, const PairArgs& a);
-bool launch_pair_f32_cplx(int width, int cpl, int mode, bool prologue, dim3 grid, hipStream_t st, const PairArgs& a);
-
-template <typename VT, bool CH, int LPR, int CPL, int MODE>
-bool launch_pair_kernel(bool prologue, dim3 grid, hipStream_t st, const PairArgs& a) {
-    using V2 = typename Vec2<VT>::type;
-    using H = typename HVal<VT, CH>::type;
-    if (prologue)
-        hipLaunchKernelGGL((cheb_pair_prologue_kernel<VT, CH, LPR, CPL, MODE>), grid, dim3(kBlock), 0, st, a.rp, a.ci,
-                           static_cast<const H*>(a.hv), static_cast<V2*>(a.P), static_cast<const V2*>(a.Q),
-                           static_cast<const V2*>(a.bra), a.rows, a.Rp, a.q0, static_cast<VT>(a.alpha),
-                           static_cast<VT>(a.beta), a.opts, a.sched, a.part1);
-    else
-        hipLaunchKernelGGL((cheb_pair_kernel<VT, CH, LPR, CPL, MODE>), grid, dim3(kBlock), 0, st, a.rp, a.ci,
-                           static_cast<const H*>(a.hv), static_cast<V2*>(a.P), static_cast<V2*>(a.Q),
-                           static_cast<const V2*>(a.bra), a.rows, a.Rp, a.q0, static_cast<VT>(a.alpha),
-                           static_cast<VT>(a.beta), a.opts, a.sched, a.part1, a.part2);
-    return true;
-}
-
-template <typename VT, bool CH, int LPR, int CPL>
-bool launch_pair_mode(int mode, bool prologue, dim3 grid, hipStream_t st, const PairArgs& a) {
-    if (mode == kRecDbl) return launch_pair_kernel<VT, CH, LPR, CPL, kRecDbl>(prologue, grid, st, a);
-    if (mode == kRec) return launch_pair_kernel<VT, CH, LPR, CPL, kRec>(prologue, grid, st, a);
-    return false;
-}
-
-// Row-kernel shapes of the two-step launch: a sweep of width columns (the
-// padded row, or a slab of it), at least 8.
-template <typename VT, bool CH>
-bool launch_pair_width(int width, int cpl, int mode, bool prologue, dim3 grid, hipStream_t st, const PairArgs& a) {
-    switch (width * 8 + cpl) {
-        case 8 * 8 + 1: return launch_pair_mode<VT, CH, 8, 1>(mode, prologue, grid, st, a);
-        case 8 * 8 + 2: return launch_pair_mode<VT, CH, 4, 2>(mode, prologue, grid, st, a);
-        case 8 * 8 + 4: return launch_pair_mode<VT, CH, 2, 4>(mode, prologue, grid, st, a);
-        case 16 * 8 + 1: return launch_pair_mode<VT, CH, 16, 1>(mode, prologue, grid, st, a);
-        case 16 * 8 + 2: return launch_pair_mode<VT, CH, 8, 2>(mode, prologue, grid, st, a);
-        case 16 * 8 + 4: return launch_pair_mode<VT, CH, 4, 4>(mode, prologue, grid, st, a);
-        case 32 * 8 + 1: return launch_pair_mode<VT, CH, 32, 1>(mode, prologue, grid, st, a);
-        case 32 * 8 + 2: return launch_pair_mode<VT, CH, 16, 2>(mode, prologue, grid, st, a);
-        case 32 * 8 + 4: return launch_pair_mode<VT, CH, 8, 4>(mode, prologue, grid, st, a);
-        case 64 * 8 + 1: return launch_pair_mode<VT, CH, 64, 1>(mode, prologue, grid, st, a);
-        case 64 * 8 + 2: return launch_pair_mode<VT, CH, 32, 2>(mode, prologue, grid, st, a);
-        case 64 * 8 + 4: return launch_pair_mode<VT, CH, 16, 4>(mode, prologue, grid, st, a);
-        case 128 * 8 + 2: return launch_pair_mode<VT, CH, 64, 2>(mode, prologue, grid, st, a);
-        case 128 * 8 + 4: return launch_pair_mode<VT, CH, 32, 4>(mode, prologue, grid, st, a);
-        case 256 * 8 + 4: return launch_pair_mode<VT, CH, 64, 4>(mode, prologue, grid, st, a);
-        default: return false;
-    }
-}
-
 // Launch one step; returns false if no kernel exists for the combination.
 bool launch_f64_real(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
 bool launch_f64_cplx(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a);

@@ -8,7 +8,4 @@ bool launch_f64_cplx(int width, int kch, int cpl, int mode, dim3 grid, hipStream
 bool launch_slots_f64_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a) {
     return launch_slots<double, true>(Rp, cpl, grid, st, a);
 }
-bool launch_pair_f64_cplx(int width, int cpl, int mode, bool prologue, dim3 grid, hipStream_t st, const PairArgs& a) {
-    return launch_pair_width<double, true>(width, cpl, mode, prologue, grid, st, a);
-}
 }  // namespace tbk

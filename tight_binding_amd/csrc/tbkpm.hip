@@ -27,7 +27,6 @@
 #include "cheb_kernels.h"
 #include "gamma_launch.h"
 #include "engine.h"
-#include "pair_plan.h"
 #include "step_launch.h"
 #include "tbkpm.h"
 
@@ -45,18 +44,6 @@ void launch_any(int precision, bool cplx, int width, int kch, int cpl, int mode,
         ok = cplx ? tbk::launch_f32_cplx(width, kch, cpl, mode, grid, st, a)
                   : tbk::launch_f32_real(width, kch, cpl, mode, grid, st, a);
     if (!ok) raise(TBK_E_UNSUPPORTED, "no step kernel for this block width");
-}
-
-void launch_pair_any(int precision, bool cplx, int width, int cpl, int mode, bool prologue, dim3 grid,
-                     hipStream_t st, const tbk::PairArgs& a) {
-    bool ok;
-    if (precision == TBK_FP64)
-        ok = cplx ? tbk::launch_pair_f64_cplx(width, cpl, mode, prologue, grid, st, a)
-                  : tbk::launch_pair_f64_real(width, cpl, mode, prologue, grid, st, a);
-    else
-        ok = cplx ? tbk::launch_pair_f32_cplx(width, cpl, mode, prologue, grid, st, a)
-                  : tbk::launch_pair_f32_real(width, cpl, mode, prologue, grid, st, a);
-    if (!ok) raise(TBK_E_UNSUPPORTED, "no two-step kernel for this block width");
 }
 
 KernelChoice choose_kernel(const tbk_tuning& tu, int precision, bool cplx, int Rp, int max_row_nnz,
@@ -491,69 +478,6 @@ int32_t tuning_cache_load(const char* path) {
     return count;
 }
 
-// Two-step schedule of M for work items of tt rows (pair_plan.h), built on
-// the host from the matrix's pattern and kept with the matrix.
-constexpr int32_t kPairQueues = 8;           // one per XCD
-constexpr double kPairMaxBand = 0.125;       // largest share of tiles the prologue may take
-constexpr int32_t kPairSpinLimit = 1 << 20;  // polls of one wait (about a second) before it gives up
-
-// Whether a launch of `grid` workgroups lands on eight XCDs numbered 0..7
-// (the two-step launch keeps one queue per XCD and needs every queue
-// served; the launches check that again each time).
-bool pair_xcds_ready(tbk_context* ctx, int64_t grid) {
-    if (ctx->xcc_state == 0) {
-        DevBuf ids;
-        ids.reserve(static_cast<size_t>(grid) * sizeof(int32_t));
-        hipLaunchKernelGGL(tbk::xcc_probe_kernel, dim3(static_cast<unsigned>(grid)), dim3(tbk::kBlock), 0,
-                           ctx->stream, ids.as<int32_t>());
-        HIPC(hipGetLastError());
-        std::vector<int32_t> h(static_cast<size_t>(grid));
-        HIPC(hipMemcpyAsync(h.data(), ids.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-        HIPC(hipStreamSynchronize(ctx->stream));
-        int64_t count[16] = {};
-        for (int32_t x : h) count[x & 15]++;
-        bool ok = true;
-        for (int x = 0; x < 16; ++x) ok = ok && (x < kPairQueues ? count[x] > 0 : count[x] == 0);
-        ctx->xcc_state = ok ? 1 : -1;
-    }
-    return ctx->xcc_state > 0;
-}
-
-engine::PairDev* pair_schedule(tbk_context* ctx, const tbk_matrix* M, int32_t tt, int32_t slack) {
-    for (const auto& p : M->pairs)
-        if (p->tt == tt && p->slack == slack) return p.get();
-    auto pd = std::make_unique<PairDev>();
-    pd->tt = tt;
-    pd->slack = slack;
-    pd->nq = kPairQueues;
-    const int32_t rows = M->rows;
-    std::vector<int32_t> rp(static_cast<size_t>(rows) + 1), ci(static_cast<size_t>(M->nnz));
-    HIPC(hipStreamSynchronize(ctx->stream));
-    HIPC(hipMemcpy(rp.data(), M->rp.p, rp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (!ci.empty()) HIPC(hipMemcpy(ci.data(), M->ci.p, ci.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    const tbk::PairSchedule sc = tbk::build_pair_schedule(rp, ci, rows, tt, kPairQueues, slack, kPairMaxBand);
-    pd->ntiles = sc.ntiles;
-    if (sc.ok) {
-        auto up = [&](DevBuf& d, const std::vector<int32_t>& v) {
-            d.reserve(std::max<size_t>(v.size(), 1) * sizeof(int32_t));
-            if (!v.empty()) HIPC(hipMemcpy(d.p, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        };
-        up(pd->items, sc.items);
-        up(pd->qoff, sc.qoff);
-        up(pd->dep_ptr, sc.dep_ptr);
-        up(pd->dep_rng, sc.dep_rng);
-        up(pd->band, sc.band);
-        up(pd->done_init, sc.done_init);
-        pd->counters.reserve(kPairQueues * sizeof(int32_t));
-        pd->done.reserve(static_cast<size_t>(sc.ntiles) * sizeof(int32_t));
-        pd->error.reserve(sizeof(double2));  // cleared as one 16-byte word
-        pd->nband = static_cast<int32_t>(sc.band.size());
-        pd->ok = true;
-    }
-    M->pairs.push_back(std::move(pd));
-    return M->pairs.back().get();
-}
-
 void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* bras,
                  const tbk_block* kets, int32_t n, int32_t flags, double* mu_out, double* last_out,
                  tbk_stats* stats) {
@@ -584,127 +508,37 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
     const int TR = kc.rows_per_tile;
     const tbk::TileOrder order = plan.args.order;
     const int64_t nb = plan.nb;
-    const int mode = doubling ? tbk::kRecDbl : tbk::kRec;
-
-    // two steps per launch: row kernel (whole rows, or sweeps of kc.width
-    // columns, each its own launch), T_{k-1} of the first pair a work block
-    // (k - 1 >= 1), and a schedule the matrix admits
-    PairDev* pd = nullptr;
-    const int64_t sweeps = Rp / kc.width;
-    // workgroups of a two-step launch: the step's count, whole rounds of the queues
-    const int64_t pair_nb = std::max<int64_t>(kPairQueues, nb / kPairQueues * kPairQueues);
-    if (ctx->tuning.pair_steps > 0 && kc.kch == 0 && kc.width >= 8 && steps >= 4 && pair_xcds_ready(ctx, pair_nb)) {
-        const int32_t want = ctx->tuning.pair_tile_rows > 0 ? ctx->tuning.pair_tile_rows : 64;
-        const int32_t tt = std::max<int32_t>(TR, (want + TR - 1) / TR * TR);
-        // a front-2 item is listed one round of the queue's workgroups after
-        // the last front-1 item it waits on, so that item is usually done
-        pd = pair_schedule(ctx, M, tt, static_cast<int32_t>(pair_nb / kPairQueues));
-        if (!pd->ok) pd = nullptr;
-    }
-    // launches: (first step, is a two-step launch)
-    std::vector<std::pair<int64_t, bool>> launches;
-    for (int64_t k = 0; k < steps;) {
-        const bool two = pd && k >= 2 && k + 1 < steps;
-        launches.emplace_back(k, two);
-        k += two ? 2 : 1;
-    }
-    // per-step partials: nb per single-step launch, one per tile, sweep and
-    // front in a two-step launch; unused entries stay zero
-    const int64_t NP = pd ? std::max<int64_t>(nb, pd->ntiles * sweeps) : nb;
-    ctx->partial.reserve(static_cast<size_t>(steps) * NP * sizeof(double2));
+    ctx->partial.reserve(static_cast<size_t>(steps) * nb * sizeof(double2));
     ctx->sums.reserve(static_cast<size_t>(steps) * sizeof(double2));
 
     const bool time_each = (flags & TBK_FLAG_TIME_LAUNCHES) != 0;
-    ensure_events(ctx, time_each ? 2 * launches.size() : 2);
+    ensure_events(ctx, time_each ? static_cast<size_t>(2 * steps) : 2);
     double2* partial = ctx->partial.as<double2>();
-    if (pd) {
-        const int64_t words = static_cast<int64_t>(steps) * NP;
-        hipLaunchKernelGGL(tbk::zero16_kernel, dim3(static_cast<unsigned>(std::min<int64_t>((words + 255) / 256, 4096))),
-                           dim3(tbk::kBlock), 0, ctx->stream, partial, words);
-        HIPC(hipGetLastError());
-        hipLaunchKernelGGL(tbk::zero16_kernel, dim3(1), dim3(tbk::kBlock), 0, ctx->stream, pd->error.as<double2>(),
-                           int64_t{1});
-        HIPC(hipGetLastError());
-    }
 
     auto buf = [&](int64_t k) -> void* {  // storage of T_k
         if (k == 0) return kets->data.p;
         return ctx->work[(k & 1) ? 1 : 0].p;
     };
-    tbk::PairArgs PA{};
-    if (pd) {
-        PA.rp = plan.args.rp;
-        PA.ci = plan.args.ci;
-        PA.hv = plan.args.hv;
-        PA.bra = bra->data.p;
-        PA.rows = rows;
-        PA.Rp = Rp;
-        PA.alpha = 2.0 / a;
-        PA.beta = -2.0 * b / a;
-        PA.opts = plan.args.opts;
-        PA.sched = tbk::PairSched{pd->items.as<const int32_t>(), pd->qoff.as<const int32_t>(),
-                                  pd->dep_ptr.as<const int32_t>(), pd->dep_rng.as<const int32_t>(),
-                                  pd->band.as<const int32_t>(),
-                                  pd->done_init.as<const int32_t>(), pd->counters.as<int32_t>(),
-                                  pd->done.as<int32_t>(), pd->error.as<int32_t>(), pd->nq, pd->nband, pd->ntiles,
-                                  pd->tt, kPairSpinLimit, 0, {}};
-        for (int x = 0; x < 16; ++x) PA.sched.xcc_queue[x] = x < kPairQueues ? x : -1;
-    }
-    // values per 16-byte lane unit (lane_unit_elems): a sweep's column offset in units
-    const int64_t ebytes = static_cast<int64_t>(kets->elem_bytes());
-    const int64_t unit_elems = kc.cpl * ebytes > 16 ? 16 / ebytes : kc.cpl;
-    const int64_t pro_nb = pd ? std::max<int64_t>(1, std::min<int64_t>(pd->nband, nb)) : 1;
     if (!time_each) HIPC(hipEventRecord(ctx->events[0], ctx->stream));
-    for (size_t li = 0; li < launches.size(); ++li) {
-        const int64_t k = launches[li].first;
-        if (time_each) HIPC(hipEventRecord(ctx->events[2 * li], ctx->stream));
-        if (k == 0) {
+    for (int64_t k = 0; k < steps; ++k) {
+        if (time_each) HIPC(hipEventRecord(ctx->events[2 * k], ctx->stream));
+        if (k == 0)
             launch_plan(plan, ctx->stream, tbk::kFirst, 1.0 / a, -b / a, buf(0), nullptr, buf(1), bra->data.p,
                         partial);
-        } else if (!launches[li].second) {
-            launch_plan(plan, ctx->stream, mode, 2.0 / a, -2.0 * b / a, buf(k), buf(k - 1), buf(k + 1), bra->data.p,
-                        partial + static_cast<size_t>(k) * NP);
-        } else {
-            PA.P = buf(k - 1);
-            PA.Q = buf(k);
-            for (int64_t sw = 0; sw < sweeps; ++sw) {
-                PA.q0 = static_cast<int32_t>(sw * kc.width / unit_elems);
-                PA.part1 = partial + static_cast<size_t>(k) * NP + sw * pd->ntiles;
-                PA.part2 = partial + static_cast<size_t>(k + 1) * NP + sw * pd->ntiles;
-                launch_pair_any(prec, M->is_complex, kc.width, kc.cpl, mode, true,
-                                dim3(static_cast<unsigned>(pro_nb)), ctx->stream, PA);
-                HIPC(hipGetLastError());
-                launch_pair_any(prec, M->is_complex, kc.width, kc.cpl, mode, false,
-                                dim3(static_cast<unsigned>(pair_nb)), ctx->stream, PA);
-                HIPC(hipGetLastError());
-                PA.sched.check_prev = 1;
-            }
-        }
-        if (time_each) HIPC(hipEventRecord(ctx->events[2 * li + 1], ctx->stream));
+        else
+            launch_plan(plan, ctx->stream, doubling ? tbk::kRecDbl : tbk::kRec, 2.0 / a, -2.0 * b / a, buf(k),
+                        buf(k - 1), buf(k + 1), bra->data.p, partial + static_cast<size_t>(k) * nb);
+        if (time_each) HIPC(hipEventRecord(ctx->events[2 * k + 1], ctx->stream));
     }
     if (!time_each) HIPC(hipEventRecord(ctx->events[1], ctx->stream));
-    if (PA.sched.check_prev) {
-        hipLaunchKernelGGL(tbk::pair_check_kernel, dim3(1), dim3(tbk::kBlock), 0, ctx->stream, PA.sched);
-        HIPC(hipGetLastError());
-    }
     hipLaunchKernelGGL(tbk::reduce_partials_kernel, dim3(static_cast<unsigned>(steps)), dim3(tbk::kBlock), 0,
-                       ctx->stream, static_cast<const double2*>(partial), static_cast<int32_t>(NP),
+                       ctx->stream, static_cast<const double2*>(partial), static_cast<int32_t>(nb),
                        ctx->sums.as<double2>());
     HIPC(hipGetLastError());
     std::vector<double2> sums(static_cast<size_t>(steps));
     HIPC(hipMemcpyAsync(sums.data(), ctx->sums.p, sums.size() * sizeof(double2), hipMemcpyDeviceToHost,
                         ctx->stream));
-    int32_t pair_error = 0;
-    if (pd)
-        HIPC(hipMemcpyAsync(&pair_error, pd->error.p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
-    if (pair_error) {
-        // a wait gave up or a queue went unserved: the two-step results are
-        // void; this matrix and tile size take one step per launch from now on
-        pd->ok = false;
-        run_moments(ctx, M, a, b, bras, kets, n, flags, mu_out, last_out, stats);
-        return;
-    }
 
     // moments
     const double mu0 = sums[0].x, mu1 = sums[0].y;
@@ -746,8 +580,8 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         float ms = 0.f;
         double kernel_ms = 0.0;
         if (time_each) {
-            for (size_t li = 0; li < launches.size(); ++li) {
-                HIPC(hipEventElapsedTime(&ms, ctx->events[2 * li], ctx->events[2 * li + 1]));
+            for (int64_t k = 0; k < steps; ++k) {
+                HIPC(hipEventElapsedTime(&ms, ctx->events[2 * k], ctx->events[2 * k + 1]));
                 kernel_ms += ms;
             }
         } else {
@@ -773,9 +607,6 @@ void run_moments(tbk_context* ctx, const tbk_matrix* M, double a, double b, cons
         stats->slab_cols = kc.width;
         stats->chunk_period = static_cast<int32_t>(order.P);
         stats->chunk_rows = static_cast<int32_t>(order.tpc) * TR;
-        stats->pair_launches = static_cast<int32_t>(
-            std::count_if(launches.begin(), launches.end(), [](const auto& l) { return l.second; }));
-        stats->pair_tiles = pd ? pd->ntiles : 0;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -1874,16 +1705,12 @@ int tbk_autotune_flags(tbk_context* ctx, const tbk_matrix* M, double a, double b
             // column sweeps of the row kernel: a sweep's gather window shrinks
             // with its width (widths below the padded R only)
             {&tbk_tuning::row_slab_cols, {0, 16, 32, 64}},
-            // two steps per launch, and its work-item rows
-            {&tbk_tuning::pair_steps, {0, 1}},
-            {&tbk_tuning::pair_tile_rows, {32, 64, 128}},
         };
         for (int pass = 0; pass < 2; ++pass) {
             for (const auto& k : knobs) {
                 for (int32_t v : k.values) {
                     if (cur.*(k.field) == v) continue;
                     if (k.field == &tbk_tuning::row_slab_cols && v >= kets->Rp) continue;
-                    if (k.field == &tbk_tuning::pair_tile_rows && cur.pair_steps <= 0) continue;
                     tbk_tuning t = cur;
                     t.*(k.field) = v;
                     const double ms = trial(t);
