@@ -115,7 +115,8 @@ def kubo_leg(ctx, cells, N, R, repeats):
         "sites": H.n, "R": R, "N": N, "solves_per_precision": repeats,
     }
     sigmas = {}
-    for prec, name, unit in ((kpm.GAMMA_FP64, "fp64", "fp64"), (kpm.GAMMA_BF16X3, "bf16x3", "bf16")):
+    for prec, name, unit in ((kpm.GAMMA_FP64, "fp64", "fp64"), (kpm.GAMMA_BF16X3, "bf16x3", "bf16"),
+                             (kpm.GAMMA_BF16X3_FP32, "bf16x3_fp32rec", "bf16")):
         lat, rec_ms, con_ms, sig_s = [], [], [], []
         for r in range(repeats):
             res = solver.solve(N, R=R, seed=100 + r, precision=prec, fermi_energies=ef)
@@ -128,7 +129,7 @@ def kubo_leg(ctx, cells, N, R, repeats):
         # st["flops"] = 8 N^2 K, the complex GEMM.  Issued MFMA flops: split-bf16
         # makes three products of every real product; fp64 forms each complex
         # product from three real ones (Gauss), 6 N^2 K
-        flops = st["flops"] * (3.0 if name == "bf16x3" else 0.75)
+        flops = st["flops"] * (3.0 if unit == "bf16" else 0.75)
         c = float(np.median(con_ms))
         r_ms = float(np.median(rec_ms))
         out[name] = {
@@ -144,10 +145,12 @@ def kubo_leg(ctx, cells, N, R, repeats):
             "mfma_frac": flops / (c * 1e-3) / 1e12 / MFMA_PEAK_TFLOPS[unit],
             "zeta_and_integral_ms": 1e3 * float(np.median(sig_s)),
             "stack_GB": st["stack_bytes"] / 1e9,
+            "recursion": "fp32" if st["recursion_fp32"] else "fp64",
         }
         log("kubo %s: %s" % (name, json.dumps(out[name])))
-    s64, sbf = sigmas["fp64"], sigmas["bf16x3"]
-    out["bf16x3"]["sigma_rel_err_vs_fp64"] = float(np.max(np.abs(sbf - s64)) / np.max(np.abs(s64)))
+    s64 = sigmas["fp64"]
+    for name in ("bf16x3", "bf16x3_fp32rec"):
+        out[name]["sigma_rel_err_vs_fp64"] = float(np.max(np.abs(sigmas[name] - s64)) / np.max(np.abs(s64)))
     out["sigma_fp64_sample"] = [float(x) for x in s64[::25]]
     out["launch_tuning"] = tuning
     ctx.set_tuning(**saved)

@@ -341,8 +341,16 @@ int tbk_tuning_cache_clear(void);
 
 /* ----------------------------------------------- Kubo-Bastin Gamma_nm */
 enum {
-    TBK_GAMMA_FP64 = 0,  /* fp64 matrix-core contraction */
-    TBK_GAMMA_BF16X3 = 1 /* split-bf16 three-term products, fp32 accumulation per chunk, fp64 across */
+    TBK_GAMMA_FP64 = 0,   /* fp64 matrix-core contraction */
+    TBK_GAMMA_BF16X3 = 1, /* split-bf16 three-term products, fp32 accumulation per chunk, fp64 across */
+    /* TBK_GAMMA_BF16X3's contraction over stacks from an fp32 recursion
+     * (complex64 vectors, fp32 H and v_x values, words split from the fp32
+     * values) where the paired stacks apply: complex H, v_x with H's pattern,
+     * N >= 256 and 2 <= R <= 128.  Other problems take TBK_GAMMA_BF16X3's
+     * fp64 recursion; tbk_kubo_stats.recursion_fp32 says which ran.  The
+     * split words carry 16 significant bits, so the fp32 recursion's rounding
+     * stays below the contraction's own. */
+    TBK_GAMMA_BF16X3_FP32 = 2
 };
 
 typedef struct tbk_kubo_stats {
@@ -357,7 +365,7 @@ typedef struct tbk_kubo_stats {
     double recursion_bytes; /* algorithmic HBM bytes of the stack-building launches and copies */
     int32_t row_ranges;     /* row ranges the fp64 stacks held one at a time (1: whole entries; more
                                when the stacks of one random vector exceed the memory available) */
-    int32_t reserved;
+    int32_t recursion_fp32; /* 1 when the stacks came from the fp32 recursion (TBK_GAMMA_BF16X3_FP32) */
 } tbk_kubo_stats;
 
 /*

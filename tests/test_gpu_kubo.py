@@ -393,3 +393,135 @@ def test_kubo_gamma_in_row_ranges_when_one_vector_exceeds_the_cap(gpu_ctx, prec,
         assert np.max(np.abs(ranged - ref)) <= 1e-10 * np.max(np.abs(ref))
     for x in M + [K]:
         x.close()
+
+
+def _f32_rows(M):
+    """CSR of M padded to its longest row: column indices, fp32 real and
+    imaginary parts (the device's rounding of the fp64 values) and a mask."""
+    rp, ci, val = M.to_csr()
+    n = M.n
+    cnt = np.diff(rp)
+    m = int(cnt.max())
+    cols = np.zeros((n, m), dtype=np.int64)
+    hr = np.zeros((n, m), dtype=np.float32)
+    hi = np.zeros((n, m), dtype=np.float32)
+    mask = np.zeros((n, m), dtype=bool)
+    for j in range(m):
+        rows = np.nonzero(cnt > j)[0]
+        p = rp[rows] + j
+        cols[rows, j] = ci[p]
+        hr[rows, j] = np.real(val[p]).astype(np.float32)
+        hi[rows, j] = np.imag(val[p]).astype(np.float32)
+        mask[rows, j] = True
+    return cols, hr, hi, mask
+
+
+def _spmm_f32(rows, xr, xi):
+    """sum over a row's entries in ascending column order of h * x, each
+    complex product formed as the step kernel forms it (products rounded,
+    then added), all in fp32."""
+    cols, hr, hi, mask = rows
+    ar = np.zeros_like(xr)
+    ai = np.zeros_like(xi)
+    for j in range(cols.shape[1]):
+        gr, gi = xr[cols[:, j]], xi[cols[:, j]]
+        h_r, h_i = hr[:, j, None], hi[:, j, None]
+        re = h_r * gr - h_i * gi
+        im = h_r * gi + h_i * gr
+        mk = mask[:, j, None]
+        ar = np.where(mk, ar + re, ar)
+        ai = np.where(mk, ai + im, ai)
+    return ar, ai
+
+
+def _fp32_paired_stacks(Hs, vx, vy, kets, N):
+    """The stacks of the fp32 paired recursion restated on the CPU in numpy
+    fp32: X_0 = [kets | v_y kets] rounded to complex64 (v_y kets in fp64, the
+    oracle's, equal to the device's), X_1 = H X_0, X_{k+1} = 2 H X_k - X_{k-1};
+    L_0 = kets, L_n = left half of X_n, R_m = v_x (right half of X_m), as
+    complex128 (exact) for the contraction."""
+    n, R = kets.shape
+    w = oracle.cheb_on_bases(vy, 2, kets)[1]
+    X0 = np.concatenate([kets, w], axis=1).astype(np.complex64)
+    rows_h, rows_x = _f32_rows(Hs), _f32_rows(vx)
+    xr, xi = np.ascontiguousarray(X0.real), np.ascontiguousarray(X0.imag)
+    zr = zi = None
+    Ls = [kets.astype(np.complex128)]
+    Rs = []
+    two = np.float32(2.0)
+    for k in range(N):
+        ar, ai = _spmm_f32(rows_h, xr, xi)
+        br, bi = _spmm_f32(rows_x, xr[:, R:], xi[:, R:])
+        Rs.append((br + 1j * bi.astype(np.float64)).astype(np.complex128))
+        if k == 0:
+            yr, yi = ar, ai
+        else:
+            yr, yi = two * ar - zr, two * ai - zi
+        if k + 1 < N:
+            Ls.append((yr[:, :R].astype(np.float64) + 1j * yi[:, :R].astype(np.float64)))
+        zr, zi, xr, xi = xr, xi, yr, yi
+    return np.stack(Ls), np.stack(Rs)
+
+
+@pytest.mark.parametrize("R,lane_bytes", [(2, 0), (4, 16), (8, 32), (16, 8), (32, 0)])
+def test_fp32_recursion_stacks_match_an_fp32_restatement(gpu_ctx, R, lane_bytes):
+    """TBK_GAMMA_BF16X3_FP32 builds the paired stacks with an fp32 recursion
+    (complex64 vectors, fp32 H and v_x values) and splits the fp32 values
+    into bf16 words.  Gamma must be the same bits as the split-bf16x3
+    contraction of the same recursion restated in numpy fp32 (an fp32 value
+    splits to the same words through the fp64 split), and within 1e-4 of the
+    fp64 Gamma."""
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (12, 10))
+    lat.set_gauge(L.GAUGE_LANDAU, 2 * np.pi / 12)
+    lat.set_disorder(1.0, 9)
+    Hs = L.rescale(lat.get_Hamiltonian_sparse(True), 5.0, 0.0)
+    vx = L.commutator_diag(lat.op_position(0), Hs)
+    vy = L.commutator_diag(lat.op_position(1), Hs)
+    N = 256
+    saved = gpu_ctx.get_tuning()
+    gpu_ctx.set_tuning(row_lane_bytes=lane_bytes)
+    try:
+        K = gpu_ctx.random_block(Hs.n, R, seed=23)
+        kets = K.download()
+        M = [gpu_ctx.matrix(m) for m in (Hs, vx, vy)]
+        g32, st = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=kpm.GAMMA_BF16X3_FP32)
+        assert st["recursion_fp32"] == 1
+        Ls, Rs = _fp32_paired_stacks(Hs, vx, vy, kets, N)
+        ref, _ = kpm.stack_contract(gpu_ctx, Ls.reshape(N, -1), Rs.reshape(N, -1), kpm.GAMMA_BF16X3)
+        assert np.array_equal(g32, ref)
+        g64, _ = kpm.kubo_gamma(gpu_ctx, *M, K, N)
+        err = np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))
+        print("fp32 recursion + bf16x3 vs fp64: Gamma %.2e" % err)
+        assert err <= 1e-4
+        K.close()
+    finally:
+        gpu_ctx.set_tuning(**saved)
+
+
+def test_fp32_recursion_sigma_error_against_fp64(gpu_ctx):
+    """sigma_xy from the fp32-recursion stacks within 1e-4 of fp64 on a
+    Landau lattice at N = 256; problems outside the paired path (one vector
+    per chain) take the fp64 recursion and say so."""
+    Hs, vx, vy = landau_problem(40, 2 * math.pi / 16, 1.0, 7, 5.2)
+    N = 256
+    g = S.lorentz(N, 6.0)
+    M = [gpu_ctx.matrix(m) for m in (Hs, vx, vy)]
+    K = gpu_ctx.random_block(Hs.n, 8, seed=5)
+    g64, _ = kpm.kubo_gamma(gpu_ctx, *M, K, N, g=g)
+    g32, st = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=kpm.GAMMA_BF16X3_FP32, g=g)
+    assert st["recursion_fp32"] == 1
+    ef = np.linspace(-0.6, 0.6, 41)
+    s64 = S.kubo_sigma(g64, ef)[0]
+    s32 = S.kubo_sigma(g32, ef)[0]
+    gerr = np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))
+    serr = np.max(np.abs(s32 - s64)) / np.max(np.abs(s64))
+    print("fp32 recursion + bf16x3 vs fp64: Gamma %.2e, sigma %.2e" % (gerr, serr))
+    assert gerr < 1e-4 and serr < 1e-4
+    K.close()
+    K1 = gpu_ctx.random_block(Hs.n, 1, seed=5)
+    g1, st1 = kpm.kubo_gamma(gpu_ctx, *M, K1, N, precision=kpm.GAMMA_BF16X3_FP32, g=g)
+    gb, _ = kpm.kubo_gamma(gpu_ctx, *M, K1, N, precision=kpm.GAMMA_BF16X3, g=g)
+    assert st1["recursion_fp32"] == 0 and np.array_equal(g1, gb)
+    K1.close()

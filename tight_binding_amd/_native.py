@@ -144,7 +144,7 @@ class tbk_kubo_stats(C.Structure):
         ("vector_groups", C.c_int32),
         ("recursion_bytes", C.c_double),
         ("row_ranges", C.c_int32),
-        ("reserved", C.c_int32),
+        ("recursion_fp32", C.c_int32),
     ]
 
     def as_dict(self):
@@ -253,6 +253,7 @@ class tbk_server_stats(C.Structure):
 
 TBK_GAMMA_FP64 = 0
 TBK_GAMMA_BF16X3 = 1
+TBK_GAMMA_BF16X3_FP32 = 2
 
 
 def _load(name):

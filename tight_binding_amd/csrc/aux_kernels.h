@@ -274,16 +274,21 @@ __global__ __launch_bounds__(kBlock) void slots_moments_kernel(const SlotDesc* _
 }
 
 // out = [a | b]: two blocks of half columns per row side by side in one of
-// 2 half columns
+// 2 half columns (complex128 or rounded to complex64)
+template <typename V2>
 __global__ __launch_bounds__(kBlock) void pair_blocks_kernel(const double2* __restrict__ a,
                                                              const double2* __restrict__ b,
-                                                             double2* __restrict__ out, int64_t rows, int32_t half) {
+                                                             V2* __restrict__ out, int64_t rows, int32_t half) {
     const int64_t w = 2 * static_cast<int64_t>(half), n = rows * w;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kBlock) {
         const int64_t r = i / w;
         const int64_t c = i - r * w;
-        out[i] = c < half ? a[r * half + c] : b[r * half + (c - half)];
+        const double2 v = c < half ? a[r * half + c] : b[r * half + (c - half)];
+        V2 o;
+        o.x = static_cast<decltype(o.x)>(v.x);
+        o.y = static_cast<decltype(o.y)>(v.y);
+        out[i] = o;
     }
 }
 

@@ -41,10 +41,10 @@ enum StepMode : int {
     kFirstSplit = 4,   // kFirst, and y's split words at y2
     kRecDblSplit = 5,  // kRecDbl, and y's split words at y2
     kDualSplit = 6,    // kDual with H2 T_k stored only as split words at y2
-    // paired chains (fp64 complex only): columns [0, Rp/2) and [Rp/2, Rp) of
-    // the block are two recursions on the same matrix; the left half's y is
-    // stored as split words at y2, the right half's H2 x as split words at
-    // y3, both in blocks of Rp/2 columns
+    // paired chains (complex matrices; fp64 or fp32 vectors): columns
+    // [0, Rp/2) and [Rp/2, Rp) of the block are two recursions on the same
+    // matrix; the left half's y is stored as split words at y2, the right
+    // half's H2 x as split words at y3, both in blocks of Rp/2 columns
     kPairFirst = 7,  // kFirst form
     kPairRec = 8,    // kRecDbl form
 };
@@ -192,9 +192,10 @@ __device__ __forceinline__ void st_lane_stream(V2* p, const V2* v, int opts) {
     }
 }
 
-// split-bf16 words of an fp64 complex value, stored non-temporally (the
-// stack entries are read only by the contraction)
-__device__ __forceinline__ void st_split(uint2* p, double2 v) {
+// split-bf16 words of a complex value (fp64 or fp32 vectors), stored
+// non-temporally (the stack entries are read only by the contraction)
+template <typename V2>
+__device__ __forceinline__ void st_split(uint2* p, V2 v) {
     typedef uint32_t u2v __attribute__((ext_vector_type(2)));
     const uint2 w = split_word(v);
     u2v u;
@@ -203,7 +204,8 @@ __device__ __forceinline__ void st_split(uint2* p, double2 v) {
     __builtin_nontemporal_store(u, reinterpret_cast<u2v*>(p));
 }
 // two consecutive values' words as one 16-byte store (p 16-byte aligned)
-__device__ __forceinline__ void st_split2(uint2* p, double2 v0, double2 v1) {
+template <typename V2>
+__device__ __forceinline__ void st_split2(uint2* p, V2 v0, V2 v1) {
     typedef uint32_t u4v __attribute__((ext_vector_type(4)));
     const uint2 w0 = split_word(v0), w1 = split_word(v1);
     u4v u;
@@ -217,7 +219,8 @@ __device__ __forceinline__ void st_split2(uint2* p, double2 v0, double2 v1) {
 // neighbour's (lane ^ 1, by a quad permutation) as one 16-byte store from
 // the even lane (p 16-byte aligned there).  Both lanes of a pair must be
 // active: they hold one row.
-__device__ __forceinline__ void st_split_lanes(uint2* p, double2 v, bool even) {
+template <typename V2>
+__device__ __forceinline__ void st_split_lanes(uint2* p, V2 v, bool even) {
     typedef uint32_t u4v __attribute__((ext_vector_type(4)));
     const uint2 w = split_word(v);
     constexpr int kSwapNeighbour = 0xB1;  // quad_perm(1, 0, 3, 2)
@@ -234,12 +237,6 @@ __device__ __forceinline__ void st_split_lanes(uint2* p, double2 v, bool even) {
         __builtin_nontemporal_store(u, reinterpret_cast<u4v*>(p));
     }
 }
-template <typename V2>
-__device__ __forceinline__ void st_split_lanes(uint2*, V2, bool) {}
-template <typename V2>
-__device__ __forceinline__ void st_split(uint2*, V2) {}  // fp32 vectors: never instantiated by a split mode
-template <typename V2>
-__device__ __forceinline__ void st_split2(uint2*, V2, V2) {}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -329,9 +326,11 @@ __device__ __forceinline__ void st_row_stream(V2* p, const V2* v, int opts) {
 // Units whose second product (the fused operator hv2) is used: all of them,
 // except in the paired modes with two or more units per lane, whose first
 // CPL / 2 units are left-half columns that store only the first product.
-template <bool PAIR, int CPL>
+// Value c lies in unit c / UE of the lane's CPL / UE units; with one unit
+// per lane a lane's values are all on one side and all form both products.
+template <bool PAIR, int CPL, int UE>
 __device__ __forceinline__ constexpr bool dual_unit(int c) {
-    return !PAIR || CPL < 2 || c >= CPL / 2;
+    return !PAIR || CPL / UE < 2 || c / UE >= (CPL / UE) / 2;
 }
 
 template <typename VT, bool CH, int LPR, int CPL, int MODE>
@@ -394,7 +393,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             if (DUAL) {
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
-                    if (dual_unit<PAIR, CPL>(c)) madd(acc2[c], hh2[k], xv[k][c]);
+                    if (dual_unit<PAIR, CPL, UE>(c)) madd(acc2[c], hh2[k], xv[k][c]);
             }
             if (cc[k] == r) {
                 have_self = true;
@@ -414,7 +413,7 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
             const H hh2 = hv2[p];
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
-                if (dual_unit<PAIR, CPL>(c)) madd(acc2[c], hh2, xv[c]);
+                if (dual_unit<PAIR, CPL, UE>(c)) madd(acc2[c], hh2, xv[c]);
         }
         if (cc == r) {
             have_self = true;
@@ -498,6 +497,23 @@ __device__ __forceinline__ void step_row(const int32_t* __restrict__ rp, const i
                 st_split_lanes(ws, left ? sv[0] : acc2[0], (q & 1) == 0);
             } else {
                 st_split(ws, left ? sv[0] : acc2[0]);
+            }
+        } else if constexpr (UE >= 2) {
+            // complex64 pairs: unit u of a lane holds the adjacent columns
+            // q UE + u LPR UE (+1), on one side of the block since the half
+            // width is even; both values' words go out as one 16-byte store
+            // (k even, so the pair lies in one word block)
+            static_assert(UE == 2, "split words of 16-byte units of two values");
+#pragma unroll
+            for (int u = 0; u < CPL / UE; ++u) {
+                const int col = q * UE + unit_step<V2, LPR, CPL>(u);
+                const bool left = col < half;
+                if (left && y2 == nullptr) continue;
+                const int64_t k = static_cast<int64_t>(r) * half + (left ? col : col - half);
+                uint2* ws =
+                    (left ? reinterpret_cast<uint2*>(y2) : reinterpret_cast<uint2*>(y3)) + word_offset(k, wbs);
+                const V2* src = (left ? sv : acc2) + u * UE;
+                st_split2(ws, src[0], src[1]);
             }
         } else {
             // unit c of a lane sits at column q + c * LPR: the first CPL / 2

@@ -302,12 +302,16 @@ void enqueue_kubo_stacks_presplit(const StepPlan& pH, const StepPlan& pX, const 
 void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
                                   double2* P, double2* out, hipStream_t st);
 // pre-split stacks with both chains in one block of 2 Rp columns (fused v_x
-// only; pP: H's plan at 2 Rp columns, row kernel); X0, X1: 2 blocks each, S:
-// one block of scratch
+// only; pP: H's plan at 2 Rp columns, row kernel, fp64 or fp32 -- then
+// vx_values are v_x's fp32 values); X0, X1: 2 fp64 blocks each (one in
+// fp32), S: one block of scratch
 void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a, double b, const void* kets,
                                 uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int N,
                                 double2* part, hipStream_t st, const void* vx_values);
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
-                            int64_t N, bool fused, bool presplit, bool paired = false);
+                            int64_t N, bool fused, bool presplit, bool paired = false, bool paired_f32 = false);
+// the contraction's kernel family of a gamma precision (TBK_GAMMA_BF16X3_FP32
+// contracts as TBK_GAMMA_BF16X3)
+int contraction_precision(int prec);
 
 }  // namespace engine

@@ -134,13 +134,12 @@ bool launch_mode(int mode, dim3 grid, hipStream_t st, const StepArgs& a) {
             if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
                 return launch_step<VT, CH, LPR, CPL, kDualSplit, 0>(grid, st, a);
             return false;
+        // the paired chains: complex row kernel, fp64 or fp32 vectors
         case kPairFirst:
-            if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
-                return launch_step<VT, CH, LPR, CPL, kPairFirst, 0>(grid, st, a);
+            if constexpr (KCH == 0 && CH) return launch_step<VT, CH, LPR, CPL, kPairFirst, 0>(grid, st, a);
             return false;
         case kPairRec:
-            if constexpr (KCH == 0 && CH && sizeof(VT) == 8)
-                return launch_step<VT, CH, LPR, CPL, kPairRec, 0>(grid, st, a);
+            if constexpr (KCH == 0 && CH) return launch_step<VT, CH, LPR, CPL, kPairRec, 0>(grid, st, a);
             return false;
         default: return launch_step<VT, CH, LPR, CPL, kRecDbl, KCH>(grid, st, a);
     }

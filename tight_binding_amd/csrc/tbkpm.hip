@@ -714,8 +714,14 @@ void contraction_shape_big(const tbk_context* ctx, int N, int64_t K, int64_t max
 
 namespace engine {
 
+int contraction_precision(int prec) {
+    if (prec == TBK_GAMMA_FP64) return TBK_GAMMA_FP64;
+    if (prec == TBK_GAMMA_BF16X3 || prec == TBK_GAMMA_BF16X3_FP32) return TBK_GAMMA_BF16X3;
+    raise(TBK_E_INVALID, "unknown contraction precision");
+}
+
 ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int prec) {
-    if (prec != TBK_GAMMA_FP64 && prec != TBK_GAMMA_BF16X3) raise(TBK_E_INVALID, "unknown contraction precision");
+    prec = contraction_precision(prec);
     ContractionShape cs{};
     cs.big = N >= 2 * tbk::kBigTile;
     if (cs.big) {
@@ -730,6 +736,7 @@ ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int 
 // split-K partial tiles into P (nchunks x N x N), then their ordered sum into out
 void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int64_t ldk,
                          int prec, double2* P, double2* out, hipStream_t st) {
+    prec = contraction_precision(prec);
     if (prec == TBK_GAMMA_FP64 && cs.big)
         tbk::launch_gamma_f64_big(A, B, N, K, ldk, cs.kc, cs.map, P, st);
     else if (prec == TBK_GAMMA_FP64)
@@ -842,7 +849,8 @@ void enqueue_kubo_stacks_rows(const StepPlan& pH, const StepPlan& pX, const Step
 // complex, and the contraction stages those words directly.  Gamma is the
 // same bits as from fp64 stacks.
 bool kubo_presplit(int prec, int N, bool h_complex, bool vx_complex) {
-    return prec == TBK_GAMMA_BF16X3 && N >= 2 * tbk::kBigTile && h_complex && vx_complex;
+    return (prec == TBK_GAMMA_BF16X3 || prec == TBK_GAMMA_BF16X3_FP32) && N >= 2 * tbk::kBigTile && h_complex &&
+           vx_complex;
 }
 
 // The left chain runs in two rolling fp64 blocks B0, B1 (B0 = a copy of the
@@ -904,6 +912,11 @@ void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a
                                 uint2* Ls, uint2* Rs, void* X0, void* X1, void* S, int64_t rows, int Rp, int N,
                                 double2* part, hipStream_t st, const void* vx_values) {
     if (pP.kc.kch != 0 || !pP.cplx || !vx_values) raise(TBK_E_UNSUPPORTED, "paired stacks need the fused row kernel");
+    // fp32 plan: X blocks of complex64, H's and v_x's fp32 values (vx_values
+    // must be v_x's val32 then); the words of X_{k+1} and v_x X_k are split
+    // from the fp32 values
+    const bool f32 = pP.precision == TBK_FP32;
+    if (f32 && Rp < 2) raise(TBK_E_UNSUPPORTED, "fp32 paired stacks need at least two columns per chain");
     const int64_t K = rows * Rp;
     // k-blocked word stacks (word_offset): entry k's words start at k * kWordBlock
     const int64_t bs = static_cast<int64_t>(N) * tbk::kWordBlock;
@@ -914,9 +927,14 @@ void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a
     HIPC(hipGetLastError());
     launch_plan(pY, st, tbk::kFirst, 1.0, 0.0, kets, nullptr, S, kets, part);  // v_y |r>
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>((2 * K + tbk::kBlock - 1) / tbk::kBlock, 65536));
-    hipLaunchKernelGGL(tbk::pair_blocks_kernel, dim3(grid), dim3(tbk::kBlock), 0, st,
-                       static_cast<const double2*>(kets), static_cast<const double2*>(S), static_cast<double2*>(X0),
-                       rows, Rp);
+    if (f32)
+        hipLaunchKernelGGL(tbk::pair_blocks_kernel<float2>, dim3(grid), dim3(tbk::kBlock), 0, st,
+                           static_cast<const double2*>(kets), static_cast<const double2*>(S), static_cast<float2*>(X0),
+                           rows, Rp);
+    else
+        hipLaunchKernelGGL(tbk::pair_blocks_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, st,
+                           static_cast<const double2*>(kets), static_cast<const double2*>(S),
+                           static_cast<double2*>(X0), rows, Rp);
     HIPC(hipGetLastError());
     launch_plan(pP, st, tbk::kPairFirst, 1.0 / a, -b / a, X(0), nullptr, X(1), X(0), part, vx_values,
                 N > 1 ? Lk(1) : nullptr, Rk(0), bs);
@@ -936,7 +954,7 @@ void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, co
 }
 
 double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk_matrix* vy, int rows, int R,
-                            int64_t N, bool fused, bool presplit, bool paired) {
+                            int64_t N, bool fused, bool presplit, bool paired, bool paired_f32) {
     // per launch: the matrix arrays once; vector blocks x (+ z) read and y
     // written; split words are half a block
     auto mat = [&](const tbk_matrix* m) {
@@ -947,6 +965,16 @@ double kubo_recursion_bytes(const tbk_matrix* H, const tbk_matrix* vx, const tbk
     const double split = presplit ? half : 0.0;
     const double vals = static_cast<double>(vx->nnz) * 16.0;
     const double Nm2 = static_cast<double>(std::max<int64_t>(0, N - 2));
+    if (paired && paired_f32) {
+        // as below with the paired block in complex64 (2 R columns of 8 bytes:
+        // one blk) and the matrix values in fp32
+        const double mat32 = static_cast<double>(H->nnz) * (8.0 + 4.0) + (static_cast<double>(rows) + 1.0) * 4.0;
+        const double vals32 = static_cast<double>(vx->nnz) * 8.0;
+        double bytes = blk + half + mat(vy) + 2.0 * blk + 3.0 * blk;
+        bytes += mat32 + vals32 + 2.0 * blk + blk;
+        bytes += static_cast<double>(N - 1) * (mat32 + vals32 + 3.0 * blk + blk);
+        return bytes;
+    }
     if (paired) {
         // kets' words; v_y |r>; the paired block X_0; N launches over 2 R
         // columns, each reading the matrix, H2's values, x (and z) and
@@ -1092,6 +1120,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     }
     double rec_total = 0.0, con_total = 0.0, flops_total = 0.0, rbytes_total = 0.0, stack_b = 0.0;
     int nchunks = 0;
+    bool any_fp32 = false;
     for (int grp = 0; grp < ngroups; ++grp) {
     const int Rp = Gw;
     const void* kp = kets->data.p;
@@ -1162,16 +1191,21 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     const bool fuse = fused_velocity(H, vx, pH, ctx->stream);
     // pre-split stacks with a fused v_x: both chains as one block of 2 Rp columns
     StepPlan pP{};
-    bool paired = false;
+    bool paired = false, paired32 = false;
     if (presplit && fuse && 2 * Rp <= 256) {
         // the default launch shape (the context's tuning is chosen for Rp
         // columns) with the context's lane width, else 32-byte lanes: two
-        // complex values per lane take the paired step on the 512 x 512
+        // complex values per lane take the fp64 paired step on the 512 x 512
         // square lattice at R = 32 from 229 us to 211 us
         tbk_tuning pt{};
         pt.row_lane_bytes = ctx->tuning.row_lane_bytes > 0 ? ctx->tuning.row_lane_bytes : 32;
-        pP = make_plan(ctx, H, TBK_FP64, 2 * Rp, rows, b, true, pt);
+        // the fp32 recursion needs a unit's two complex64 on one side of the
+        // block: at least two columns per chain
+        paired32 = prec == TBK_GAMMA_BF16X3_FP32 && Rp >= 2;
+        if (paired32) matrix_fp32(const_cast<tbk_matrix*>(vx));
+        pP = make_plan(ctx, H, paired32 ? TBK_FP32 : TBK_FP64, 2 * Rp, rows, b, true, pt);
         paired = pP.kc.kch == 0;
+        paired32 = paired32 && paired;
     }
     // fp64 stacks: entries at a stride of ldk elements; word stacks: k-blocked
     // (word_offset), each entry's share rounded up to whole word blocks
@@ -1209,7 +1243,8 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     HIPC(hipEventRecord(ctx->events[0], ctx->stream));
     if (paired)
         enqueue_kubo_stacks_paired(pP, pY, a, b, kp, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
-                                   ctx->kwork[1].p, ctx->kwork[2].p, rows, Rp, N, part, ctx->stream, vx->val64.p);
+                                   ctx->kwork[1].p, ctx->kwork[2].p, rows, Rp, N, part, ctx->stream,
+                                   paired32 ? vx->val32.p : vx->val64.p);
     else if (presplit)
         enqueue_kubo_stacks_presplit(pH, pX, pY, a, b, kp, L.as<uint2>(), Rs.as<uint2>(), ctx->kwork[0].p,
                                      ctx->kwork[1].p, ctx->kwork[2].p, ctx->work[0].p, ctx->work[1].p, K, N, part,
@@ -1237,7 +1272,8 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
     flops_total += 8.0 * static_cast<double>(N) * N * static_cast<double>(K);
     stack_b = 2.0 * static_cast<double>(N) * static_cast<double>(K) *
               static_cast<double>(presplit ? sizeof(uint2) : sizeof(double2));
-    rbytes_total += kubo_recursion_bytes(H, vx, vy, rows, Rreal, N, fuse, presplit, paired);
+    rbytes_total += kubo_recursion_bytes(H, vx, vy, rows, Rreal, N, fuse, presplit, paired, paired32);
+    any_fp32 = any_fp32 || paired32;
     }  // groups
     if (g) {
         for (int64_t n = 0; n < N; ++n)
@@ -1256,6 +1292,7 @@ void run_kubo(tbk_context* ctx, const tbk_matrix* H, double a, double b, const t
         stats->nchunks = nchunks;
         stats->vector_groups = ngroups;
         stats->row_ranges = nrc;
+        stats->recursion_fp32 = any_fp32 ? 1 : 0;
         stats->recursion_bytes = rbytes_total;
         stats->wall_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

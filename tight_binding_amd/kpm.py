@@ -277,6 +277,9 @@ class Context:
 
 GAMMA_FP64 = N.TBK_GAMMA_FP64
 GAMMA_BF16X3 = N.TBK_GAMMA_BF16X3
+# the split-bf16x3 contraction over stacks from an fp32 recursion (paired
+# stacks: complex H, v_x with H's pattern, n >= 256); stats["recursion_fp32"]
+GAMMA_BF16X3_FP32 = N.TBK_GAMMA_BF16X3_FP32
 
 
 def tuning_cache_save(path: str):
