@@ -368,7 +368,7 @@ def serve_leg(qps, n_requests, saturation_only=False):
             return srv.submit_dos(lat, r["N"], r["a"], 0.0, r["R"], r["seed"], kpm.FP32 if r["mixed"] else kpm.FP64,
                                   builder=Lm.BUILD_PERIODIC, extend=(1,) * r["dim"])
         return srv.submit_kubo_lattice(lat, r["N"], r["a"], 0.0, R=r["R"], seed=r["seed"],
-                                       precision=kpm.GAMMA_BF16X3 if r["mixed"] else kpm.GAMMA_FP64,
+                                       precision=kpm.GAMMA_BF16X3_FP32 if r["mixed"] else kpm.GAMMA_FP64,
                                        g=gweights(r["N"]))
 
     created = {}
@@ -448,7 +448,7 @@ def serve_leg(qps, n_requests, saturation_only=False):
         if r["kind"] == "dos":
             plain.append(plain_dos(r, lat, kpm.FP32 if r["mixed"] else kpm.FP64))
         else:
-            plain.append(plain_kubo(r, lat, kpm.GAMMA_BF16X3 if r["mixed"] else kpm.GAMMA_FP64))
+            plain.append(plain_kubo(r, lat, kpm.GAMMA_BF16X3_FP32 if r["mixed"] else kpm.GAMMA_FP64))
         if (i + 1) % 16 == 0:
             log("serve: plain path, %d of %d requests, %.1f s" % (i + 1, len(reqs), time.perf_counter() - t1))
     span_plain = time.perf_counter() - t1

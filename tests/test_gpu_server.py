@@ -182,21 +182,25 @@ def test_server_errors():
         assert info["state"] == Nn.TBK_REQ_DONE and mu.size == 10
 
 
-def test_presplit_kubo_request_matches_direct_path(gpu_ctx):
+@pytest.mark.parametrize("prec", [kpm.GAMMA_BF16X3, kpm.GAMMA_BF16X3_FP32])
+def test_presplit_kubo_request_matches_direct_path(gpu_ctx, prec):
     """A split-BF16x3 request with N >= 256 runs on pre-split stacks in the
-    server as in the direct path: the same Gamma bits."""
+    server as in the direct path, from the fp64 or the fp32 recursion: the
+    same Gamma bits."""
     Hs, vx, vy, g = landau_kubo(L0=10, N=256, R=2)
     N = g.size
-    ref, _ = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
-                            gpu_ctx.random_block(Hs.n, 2, seed=31), N, precision=kpm.GAMMA_BF16X3, g=g)
+    ref, st = kpm.kubo_gamma(gpu_ctx, gpu_ctx.matrix(Hs), gpu_ctx.matrix(vx), gpu_ctx.matrix(vy),
+                             gpu_ctx.random_block(Hs.n, 2, seed=31), N, precision=prec, g=g)
+    assert st["recursion_fp32"] == (1 if prec == kpm.GAMMA_BF16X3_FP32 else 0)
     with kpm.Server(0, lazy_arena=True) as srv:
-        t = srv.submit_kubo(Hs, vx, vy, N, R=2, seed=31, precision=kpm.GAMMA_BF16X3, g=g)
+        t = srv.submit_kubo(Hs, vx, vy, N, R=2, seed=31, precision=prec, g=g)
         assert srv.wait(t, 60.0)["state"] == Nn.TBK_REQ_DONE
         (z, G), info = srv.take(t)
     assert np.array_equal(G, ref)
 
 
-@pytest.mark.parametrize("prec,tol", [(kpm.GAMMA_FP64, 1e-12), (kpm.GAMMA_BF16X3, 1e-5)])
+@pytest.mark.parametrize("prec,tol", [(kpm.GAMMA_FP64, 1e-12), (kpm.GAMMA_BF16X3, 1e-5),
+                                      (kpm.GAMMA_BF16X3_FP32, 1e-5)])
 def test_kubo_request_over_budget_runs_in_vector_groups(gpu_ctx, prec, tol):
     """A sigma_xy request whose stacks exceed the server's budget runs its
     random vectors in groups (Gamma is a sum over vectors) instead of being

@@ -457,8 +457,10 @@ int64_t tbk_server::kubo_footprint(const Request& r) const {
     const int64_t stacks = 2 * N * entry, rolling = ps ? 5 * blk : r.row_chunks > 1 ? 5 * blk : 2 * blk;
     // a lattice request also holds the assembled H while H~ is formed
     // (H~ has the size of a velocity operator: one more entry per row)
-    const int64_t mats = r.lat.present ? mat_bytes(r.H, false) + 3 * mat_bytes(r.vx, false)
-                                       : mat_bytes(r.H, false) + mat_bytes(r.vx, false) + mat_bytes(r.vy, false);
+    int64_t mats = r.lat.present ? mat_bytes(r.H, false) + 3 * mat_bytes(r.vx, false)
+                                 : mat_bytes(r.H, false) + mat_bytes(r.vx, false) + mat_bytes(r.vy, false);
+    // the fp32 recursion's copies of H~'s and v_x's values (complex64)
+    if (r.precision == TBK_GAMMA_BF16X3_FP32) mats += 2 * (r.vx.hdr.nnz + 256) * 8;
     return mats + blk /* kets */ +
            stacks + rolling + static_cast<int64_t>(ctx->num_cu) * 32 * 16 /* step partials */ +
            static_cast<int64_t>(cs.nchunks) * N * N * 16 + 2 * N * N * 16 /* Gamma, its sum */ + N * 8 * 2 + N * 16 +
@@ -743,11 +745,19 @@ void tbk_server::start_kubo(Request* r, int32_t pin) {
     const void* vxv = fused_velocity(j->H, j->vx, pH, bstream) ? j->vx->val64.p : nullptr;
     // pre-split with a fused v_x: both chains as one block of 2 Rp columns in
     // W0, W1 (two blocks each) and S -- the five blocks the footprint counts
+    // TBK_GAMMA_BF16X3_FP32 with at least two columns per chain: the paired
+    // chains run in fp32 (fp32 copies of H's and v_x's values, in the lease)
     StepPlan pP{};
     bool paired = false;
     if (presplit && vxv && 2 * Rp <= 256) {
-        pP = make_plan(ctx, j->H, TBK_FP64, 2 * Rp, rows, b, true, tbk_tuning{});
+        const bool f32 = r->precision == TBK_GAMMA_BF16X3_FP32 && Rp >= 2;
+        if (f32) {
+            matrix_fp32(j->vx);
+            vxv = j->vx->val32.p;
+        }
+        pP = make_plan(ctx, j->H, f32 ? TBK_FP32 : TBK_FP64, 2 * Rp, rows, b, true, tbk_tuning{});
         paired = pP.kc.kch == 0;
+        if (!paired) vxv = j->vx->val64.p;
     }
     // fp64 stacks at a stride of ldk elements; word stacks k-blocked
     const int64_t ldk = stack_stride(K);
@@ -1256,7 +1266,8 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
             if (q->n_moments < 2) raise(TBK_E_INVALID, "n_moments must be at least 2");
             if (q->n_moments > s->cfg.max_moments) raise(TBK_E_UNSUPPORTED, "more moments than the server allows");
         } else {
-            if (q->precision != TBK_GAMMA_FP64 && q->precision != TBK_GAMMA_BF16X3)
+            if (q->precision != TBK_GAMMA_FP64 && q->precision != TBK_GAMMA_BF16X3 &&
+                q->precision != TBK_GAMMA_BF16X3_FP32)
                 raise(TBK_E_INVALID, "unknown contraction precision");
             if (q->H && (!q->vx || !q->vy)) raise(TBK_E_INVALID, "sigma_xy request without velocity operators");
             if (!q->H && (q->vx || q->vy))
