@@ -318,6 +318,30 @@ int tbk_cheb_seq_on_bases(tbk_context* ctx, const tbk_matrix* M, double a, doubl
  * per step.  Results do not depend on the tuning beyond rounding of the
  * moment sums (the recursion vectors are bit-identical across shapes).
  */
+/*
+ * One Chebyshev step on resident blocks, the step of the recursions above
+ * (the device side of Chebyshev::Cheb_first_kind_seq_view::next,
+ * numerical.h:346-361):
+ *   z == NULL:  y = H~ x,           z != NULL:  y = 2 H~ x - z,
+ * H~ = (M - b I)/a.  y may be z (in place); neither y nor z may be x.  All
+ * blocks share x's shape and precision.  Enqueued on the context's stream
+ * (tbk_block_download waits for it).
+ */
+int tbk_cheb_step(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* x,
+                  const tbk_block* z, tbk_block* y);
+
+/*
+ * Chebyshev expansion applied to a block:  out = sum_{k<n} c_k T_k(H~) v,
+ * c_k = coef[2k] + i coef[2k+1].  The terms are added in order of k, each as
+ * soon as T_k is formed (the recursion of tbk_cheb_seq_on_bases in two
+ * rolling blocks).  With v = columns of the identity and c_k = Im alpha_k(z)
+ * this is GreenFunction::direct_eval's sum (lattice.h:579-587) without
+ * storing the T_k.  out: v's shape and precision, not v.  kernel_ms
+ * (optional): device time of the steps and additions after the first term.
+ */
+int tbk_cheb_expand(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* v, int32_t n,
+                    const double* coef, tbk_block* out, float* kernel_ms);
+
 int tbk_autotune(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
                  int32_t steps, tbk_tuning* best, double* best_ms_per_step);
 /* flags of tbk_autotune_flags, besides TBK_FLAG_PLAIN (tune the

@@ -28,7 +28,7 @@ def build(tmp_path, name="dropin_dos"):
     return exe
 
 
-@pytest.mark.parametrize("name", ["dropin_dos", "kubo_xy_entry", "kubo_bastin"])
+@pytest.mark.parametrize("name", ["dropin_dos", "kubo_xy_entry", "kubo_bastin", "green_direct"])
 def test_cpp_dropin_compiles_and_links(tmp_path, name):
     assert os.path.exists(build(tmp_path, name))
 
@@ -211,3 +211,81 @@ def test_cpp_conductivity_kubo_bastin_matches_oracle(tmp_path):
         want.append(-1.0 / 4 / math.pi * float(np.sum(gl * np.imag(al) * mu)))
     want = np.array(want)
     assert np.max(np.abs(dosf[:, 1] - want)) <= 1e-9 * np.max(np.abs(want))
+
+
+def _alpha(z, N):
+    """1/(z - x) = sum alpha_n(z) T_n(x) (numerical.h:304-316)."""
+    s1 = np.sqrt(1.0 - z * z + 0j)
+    s2 = z - 1j * s1
+    a = np.empty(N, dtype=np.complex128)
+    a[0] = -1j / s1
+    sn = -2j / s1
+    for i in range(1, N):
+        sn = sn * s2
+        a[i] = sn
+    return a
+
+
+def _entries(lines, tag, n):
+    G = np.zeros((n, n), dtype=np.complex128)
+    for x in lines:
+        if x.startswith(tag + " "):
+            _, i, j, re, im = x.split()
+            G[int(i), int(j)] = complex(float(re), float(im))
+    return G
+
+
+@pytest.mark.gpu
+def test_cpp_small_system_green_function_and_view(tmp_path):
+    """GreenFunction(N, H) and direct_eval (lattice.h:570-592) and
+    Cheb_first_kind_seq_view (numerical.h:346-361) through the C++ layer.
+    direct_eval = sum_n Im(alpha_n(z~)) T_n(H~) / a: against the oracle's T_n
+    (identity kets) summed in order, and -- independently of any Chebyshev
+    code -- against the dense resolvent, whose anti-Hermitian part
+    (G - G^H) / 2i with G = (z~ - H~)^-1 is that sum once the series has
+    converged (Im z~ ~ 0.07 at N = 600: |s2|^N ~ 1e-19).  The view's blocks
+    are the oracle's T_i bit for bit."""
+    L, N = 6, 600
+    out = subprocess.run([build(tmp_path, "green_direct"), str(L), str(N)], capture_output=True, text=True,
+                         check=True, timeout=120)
+    lines = out.stdout.split("\n")
+    assert "Warning: Too big to implement direct eval." in out.stderr
+    assert "big_nnz 0" in lines
+    n = L * L
+    cell = O.model("square")
+    cell.add_hopping_nn("A", 1, 1.0)
+    lat = O.PartialLattice(cell, (L, L))
+    lat.set_scalar_field_indexed(O.uniform_disorder(1.0, 5))
+    Hr = lat.get_Hamiltonian_periodic_extend_bare_sparse((1, 1), False)
+    lat.gauge_field = O.landau_gauge(2 * math.pi / 6)
+    Hc = lat.get_Hamiltonian_sparse(True)
+    eye = np.eye(n, dtype=np.complex128)
+    for tag, H, z in (("c", Hc, complex(0.3, 0.3)), ("r", Hr, complex(-1.1, 0.4))):
+        a, b = [float(v) for x in lines if x.startswith("scale_%s " % tag) for v in x.split()[1:]]
+        ev = np.linalg.eigvalsh(H.to_dense())
+        assert abs(a - (ev[-1] - ev[0]) / 1.99) <= 1e-9 and abs(b - (ev[-1] + ev[0]) / 2) <= 1e-9
+        Hs = O.rescale(H, a, b)
+        got = _entries(lines, "g" + tag, n)
+        zt = complex((z.real - b) / a, z.imag / a)
+        al = _alpha(zt, N)
+        T = oracle.cheb_on_bases(Hs, N, eye)
+        want = np.zeros((n, n), dtype=np.complex128)
+        for k in range(N):
+            want = want + al[k].imag * T[k]
+        want = want / a
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+        Gz = np.linalg.inv(zt * np.eye(n) - Hs.to_dense())
+        anti = (Gz - Gz.conj().T) / 2j / a
+        assert np.max(np.abs(got - anti)) <= 1e-10 * np.max(np.abs(anti))
+        if tag == "r":
+            assert np.all(got.imag == 0)
+    # the view: T_0 .. T_5 of the seeded block on the complex H~
+    a, b = [float(v) for x in lines if x.startswith("scale_c ") for v in x.split()[1:]]
+    Hs = O.rescale(Hc, a, b)
+    v = cpp_kets(n, 3, 0x77)
+    rows = np.array([[float(t) for t in x.split()[1:]] for x in lines if x.startswith("view ")])
+    assert rows.shape == (6 * n, 4)
+    assert np.array_equal(rows[:, 0].astype(int), np.repeat(np.arange(6), n))
+    got = (rows[:, 2] + 1j * rows[:, 3]).reshape(6, n)
+    want = oracle.cheb_on_bases(Hs, 6, v)[:, :, 1]
+    assert np.array_equal(got, want)

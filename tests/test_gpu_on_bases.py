@@ -134,3 +134,72 @@ def test_errors(gpu_ctx):
         assert e.value.code == Nn.TBK_E_INVALID
     with pytest.raises(Nn.NativeError):
         gpu_ctx.cheb_on_bases(M, gpu_ctx.zeros_block(Hs.n + 1, 4), 2)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_single_steps_are_the_stored_blocks(gpu_ctx, cplx):
+    """tbk_cheb_step (the device side of Cheb_first_kind_seq_view::next,
+    numerical.h:346-361): y = H~ x, then y = 2 H~ x - z, in place over z as
+    the view rotates its blocks -- bit for bit the oracle's T_k."""
+    Hs = O.rescale(problem(cplx), 8.6, 0.0)
+    kets = O.random_phase_kets(Hs.n, 5, 17)
+    want = oracle.cheb_on_bases(Hs, 8, kets)
+    M = gpu_ctx.matrix(Hs)
+    v0, v1 = gpu_ctx.block(kets), gpu_ctx.zeros_block(Hs.n, 5)
+    gpu_ctx.cheb_step(M, v0, v1)
+    for k in range(8):
+        assert np.array_equal(v0.download(), want[k]), k
+        gpu_ctx.cheb_step(M, v1, v0, z=v0)  # T_{k+2} over T_k
+        v0, v1 = v1, v0
+    # separate output, fused rescale
+    a, b = 5.3, 0.2
+    Hr = O.rescale(problem(cplx), a, b)
+    w = oracle.cheb_on_bases(Hr, 3, kets)
+    P = gpu_ctx.matrix(problem(cplx))
+    x0, x1, x2 = gpu_ctx.block(kets), gpu_ctx.zeros_block(Hs.n, 5), gpu_ctx.zeros_block(Hs.n, 5)
+    gpu_ctx.cheb_step(P, x0, x1, a=a, b=b)
+    gpu_ctx.cheb_step(P, x1, x2, z=x0, a=a, b=b)
+    for k, x in ((1, x1), (2, x2)):
+        assert np.max(np.abs(x.download() - w[k])) <= 1e-13 * np.max(np.abs(w[k]))
+    with pytest.raises(Nn.NativeError) as e:
+        gpu_ctx.cheb_step(M, v0, v0)
+    assert e.value.code == Nn.TBK_E_INVALID
+    with pytest.raises(Nn.NativeError) as e:
+        gpu_ctx.cheb_step(M, v0, gpu_ctx.zeros_block(Hs.n, 4))
+    assert e.value.code == Nn.TBK_E_INVALID
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_expansion_is_the_ordered_sum_of_the_blocks(gpu_ctx, cplx):
+    """tbk_cheb_expand: out = sum_k c_k T_k(H~) v with complex c_k, each term
+    (c.re t.re - c.im t.im, c.re t.im + c.im t.re) added in order of k --
+    bit for bit the same sum over the oracle's T_k; real coefficients scale
+    each part."""
+    Hs = O.rescale(problem(cplx), 8.6, 0.0)
+    kets = O.random_phase_kets(Hs.n, 6, 29)
+    n = 40
+    T = oracle.cheb_on_bases(Hs, n, kets)
+    rng = np.random.default_rng(3)
+    M = gpu_ctx.matrix(Hs)
+    V = gpu_ctx.block(kets)
+    for c in (rng.standard_normal(n) + 1j * rng.standard_normal(n), rng.standard_normal(n) + 0j):
+        wr = np.zeros(T[0].shape)
+        wi = np.zeros(T[0].shape)
+        for k in range(n):
+            tr, ti = T[k].real, T[k].imag
+            if c[k].imag == 0.0:
+                pr, pi = c[k].real * tr, c[k].real * ti
+            else:
+                pr = c[k].real * tr - c[k].imag * ti
+                pi = c[k].real * ti + c[k].imag * tr
+            wr, wi = (pr, pi) if k == 0 else (wr + pr, wi + pi)
+        out, ms = gpu_ctx.cheb_expand(M, V, c)
+        assert ms >= 0.0
+        assert np.array_equal(out.download(), wr + 1j * wi)
+        out.close()
+    with pytest.raises(Nn.NativeError) as e:
+        gpu_ctx.cheb_expand(M, V, [1.0, float("nan")])
+    assert e.value.code == Nn.TBK_E_INVALID
+    with pytest.raises(Nn.NativeError) as e:
+        gpu_ctx.cheb_expand(M, V, [1.0], out=V)
+    assert e.value.code == Nn.TBK_E_INVALID

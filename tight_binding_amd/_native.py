@@ -285,6 +285,8 @@ KPM_SIGNATURES = {
     "tbk_cheb_first_kind_seq_braket": ([C.POINTER(tbk_sparse), I32, DP, DP, I32, I32, DP], I32),
     "tbk_cheb_first_kind_seq_on_bases": ([C.POINTER(tbk_sparse), I32, DP, I32, I32, DP], I32),
     "tbk_cheb_seq_on_bases": ([P, P, D, D, P, I32, PP, C.POINTER(C.c_float)], I32),
+    "tbk_cheb_step": ([P, P, D, D, P, P, P], I32),
+    "tbk_cheb_expand": ([P, P, D, D, P, I32, DP, P, C.POINTER(C.c_float)], I32),
     "tbk_context_create": ([I32, PP], I32),
     "tbk_context_destroy": ([P], I32),
     "tbk_context_synchronize": ([P], I32),

@@ -273,6 +273,35 @@ __global__ __launch_bounds__(kBlock) void slots_moments_kernel(const SlotDesc* _
     }
 }
 
+// out = c * t (init) or out + c * t, elementwise over n complex values, c
+// complex; the product is formed as (c.re t.re - c.im t.im, c.re t.im +
+// c.im t.re) in the block's precision.  With c.im = 0 each part is c.re times
+// the part, the scalar-times-matrix product of a real coefficient.
+template <typename V2>
+__global__ __launch_bounds__(kBlock) void block_axpy_kernel(V2* __restrict__ out, const V2* __restrict__ t,
+                                                            double cr, double ci, int64_t n, int32_t init) {
+    using E = decltype(t->x);
+    const E r = static_cast<E>(cr), im = static_cast<E>(ci);
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+        const V2 v = t[i];
+        V2 p;
+        if (ci == 0.0) {
+            p.x = r * v.x;
+            p.y = r * v.y;
+        } else {
+            p.x = r * v.x - im * v.y;
+            p.y = r * v.y + im * v.x;
+        }
+        if (!init) {
+            const V2 o = out[i];
+            p.x = o.x + p.x;
+            p.y = o.y + p.y;
+        }
+        out[i] = p;
+    }
+}
+
 // out = [a | b]: two blocks of half columns per row side by side in one of
 // 2 half columns (complex128 or rounded to complex64)
 template <typename V2>

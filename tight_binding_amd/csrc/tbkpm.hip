@@ -1682,6 +1682,67 @@ int tbk_cheb_seq_on_bases(tbk_context* ctx, const tbk_matrix* M, double a, doubl
     });
 }
 
+int tbk_cheb_step(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* x,
+                  const tbk_block* z, tbk_block* y) {
+    return guarded([&] {
+        check_on_bases_args(ctx, M, a, b, x, 1);
+        if (!y) raise(TBK_E_INVALID, "null argument");
+        auto alike = [&](const tbk_block* o) {
+            return o->ctx == ctx && o->rows == x->rows && o->R == x->R && o->Rp == x->Rp &&
+                   o->precision == x->precision;
+        };
+        if (!alike(y) || (z && !alike(z))) raise(TBK_E_INVALID, "block shape, precision or context differs from x");
+        if (y == x || z == x) raise(TBK_E_INVALID, "y and z must not be x");
+        set_device(ctx);
+        const StepPlan plan = make_plan(ctx, M, x->precision, x->Rp, x->rows, b, false);
+        ctx->partial.reserve(static_cast<size_t>(plan.nb) * sizeof(double2));
+        double2* part = ctx->partial.as<double2>();
+        if (!z)
+            launch_plan(plan, ctx->stream, tbk::kFirst, 1.0 / a, -b / a, x->data.p, nullptr, y->data.p, x->data.p,
+                        part);
+        else
+            launch_plan(plan, ctx->stream, tbk::kRecDbl, 2.0 / a, -2.0 * b / a, x->data.p, z->data.p, y->data.p,
+                        nullptr, part);
+    });
+}
+
+int tbk_cheb_expand(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* v, int32_t n,
+                    const double* coef, tbk_block* out, float* kernel_ms) {
+    return guarded([&] {
+        check_on_bases_args(ctx, M, a, b, v, n);
+        if (!coef || !out) raise(TBK_E_INVALID, "null argument");
+        if (out->ctx != ctx || out->rows != v->rows || out->R != v->R || out->Rp != v->Rp ||
+            out->precision != v->precision)
+            raise(TBK_E_INVALID, "out block shape, precision or context differs from v");
+        if (out == v) raise(TBK_E_INVALID, "out must not be v");
+        for (int64_t k = 0; k < 2 * static_cast<int64_t>(n); ++k)
+            if (!std::isfinite(coef[k])) raise(TBK_E_INVALID, "non-finite coefficient");
+        set_device(ctx);
+        const size_t block_bytes = static_cast<size_t>(v->rows) * v->Rp * v->elem_bytes();
+        ctx->work[0].reserve(block_bytes);
+        ctx->work[1].reserve(block_bytes);
+        const int64_t elems = static_cast<int64_t>(v->rows) * v->Rp;
+        const unsigned grid = static_cast<unsigned>(std::min<int64_t>((elems + tbk::kBlock - 1) / tbk::kBlock, 65536));
+        // T_k in v (k = 0) or the rolling work blocks; out += c_k T_k right
+        // after T_k is formed, in order of k
+        auto buf = [&](int64_t k) -> void* { return k == 0 ? v->data.p : ctx->work[k & 1].p; };
+        auto add = [&](int64_t k) {
+            const double cr = coef[2 * k], ci = coef[2 * k + 1];
+            if (v->precision == TBK_FP64)
+                hipLaunchKernelGGL(tbk::block_axpy_kernel<double2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
+                                   out->data.as<double2>(), static_cast<const double2*>(buf(k)), cr, ci, elems,
+                                   k == 0 ? 1 : 0);
+            else
+                hipLaunchKernelGGL(tbk::block_axpy_kernel<float2>, dim3(grid), dim3(tbk::kBlock), 0, ctx->stream,
+                                   out->data.as<float2>(), static_cast<const float2*>(buf(k)), cr, ci, elems,
+                                   k == 0 ? 1 : 0);
+            HIPC(hipGetLastError());
+        };
+        const double ms = enqueue_on_bases(ctx, M, a, b, v, n, buf, add);
+        if (kernel_ms) *kernel_ms = static_cast<float>(ms);
+    });
+}
+
 int tbk_autotune_flags(tbk_context* ctx, const tbk_matrix* M, double a, double b, const tbk_block* kets,
                        int32_t steps, int32_t flags, tbk_tuning* best, double* best_ms_per_step) {
     return guarded([&] {

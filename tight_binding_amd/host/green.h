@@ -1,6 +1,8 @@
 // KPM front end of PartialLattice: GreenFunction<Scalar> and its
 // GreenFunctionImag functor (lattice.h:532-652 of the reference), and
-// density_of_states_seq (functions.h:15-33).
+// density_of_states_seq (functions.h:15-33).  The small-system constructor
+// GreenFunction(N, H) and direct_eval (lattice.h:570-592) run the Chebyshev
+// sum on the device.
 //
 // GreenFunction(N, H, E_min, E_max) rescales H~ = (H - b I)/a with
 // a = (E_max - E_min)/2, b = (E_max + E_min)/2 over the union pattern of H
@@ -12,6 +14,7 @@
 #pragma once
 
 #include <cmath>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -34,6 +37,7 @@ struct PartialLattice<Dim>::GreenFunction {
     double a, b;
     double E_min, E_max;
     Mat scaled_HE;
+    bool direct_ok = false;  // direct_eval available (small-system constructor)
 
     GreenFunction(int N, const Mat& HE, double E_min, double E_max) : N(N), E_min(E_min), E_max(E_max) {
         a = (E_max - E_min) / 2;
@@ -47,6 +51,50 @@ struct PartialLattice<Dim>::GreenFunction {
         a = (E_max - E_min) / (2 - epsilon);
         b = (E_max + E_min) / 2;
         scaled_HE = rescale(HE, a, b);
+    }
+    // for small systems (lattice.h:570-577): the range of HE's own spectrum,
+    // and direct_eval below 5000 rows.  The reference stores T_n(H~) for
+    // n < N here; direct_eval below forms its sum on the device instead, so
+    // nothing is stored.
+    GreenFunction(int N, const Mat& HE) : GreenFunction(N, HE, HE) {
+        direct_ok = HE.rows() < 5000;
+        if (!direct_ok && PartialLattice<Dim>::auto_info)
+            std::cout << "GreenFunctionKP refuses to generate Chebyt seq of H (H is too big)." << std::endl;
+    }
+    // sum_{n<N} Im(alpha_n(z~)) T_n(H~) / a at z~ = (z - b)/a (lattice.h:579-587):
+    // the expansion applied to the identity on the device (tbk_cheb_expand),
+    // the terms added in order of n as the reference's loop adds them.  The
+    // result holds the entries that are not zero.  Outside the small-system
+    // constructor (or at 5000 rows and above) it warns and returns zero, as
+    // the reference's loop over an empty sequence does.  sgn is unused, as
+    // in the reference.
+    Mat direct_eval(Complex z, int sgn = 1) const {
+        (void)sgn;
+        Mat Gf(scaled_HE.rows(), scaled_HE.cols());
+        if (!direct_ok) {
+            std::cerr << "Warning: Too big to implement direct eval." << std::endl;
+            return Gf;
+        }
+        z = scale(z, Complex(1.0, 0.0));
+        const auto alpha = numerical::Chebyshev::expn_coef_alpha_seq(z, N);
+        std::vector<Complex> c(alpha.size());
+        for (size_t n = 0; n < alpha.size(); ++n) c[n] = Complex(alpha[n].imag(), 0.0);
+        const MatrixXcd sum = numerical::Chebyshev::Cheb_expand_on_identity(scaled_HE, c);
+        std::vector<Triplet<Scalar>> t;
+        for (int32_t j = 0; j < Gf.cols(); ++j)
+            for (int32_t i = 0; i < Gf.rows(); ++i) {
+                Scalar v;
+                if constexpr (is_complex_v<Scalar>) v = sum(i, j) / a;
+                else v = sum(i, j).real() / a;
+                if (v != Scalar(0)) t.emplace_back(i, j, v);
+            }
+        Gf.setFromTriplets(t.begin(), t.end());
+        return Gf;
+    }
+    // z = omega + i eta (lattice.h:589-592)
+    Mat direct_eval(double omega, double eta = 0.01) const {
+        const int sgn = eta > 0 ? 1 : -1;
+        return direct_eval(Complex(omega, eta), sgn);
     }
     double scale(double x) const { return (x - b) / a; }
     // (x - b id) / a, the reference's two-argument form (lattice.h:550)

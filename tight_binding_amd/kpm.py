@@ -255,6 +255,25 @@ class Context:
                                                   C.cast(arr, N.PP), C.byref(ms)))
         return out, ms.value
 
+    def cheb_step(self, M: Matrix, x: Block, y: Block, z: Block | None = None, a: float = 1.0, b: float = 0.0):
+        """One step on resident blocks (tbk_cheb_step): y = H~ x when z is None,
+        else y = 2 H~ x - z (y may be z), H~ = (M - b)/a."""
+        N.check_kpm(N.kpm().tbk_cheb_step(self._h, M._h, float(a), float(b), x._h,
+                                          z._h if z is not None else None, y._h))
+        return y
+
+    def cheb_expand(self, M: Matrix, v: Block, coef, a: float = 1.0, b: float = 0.0, out: Block | None = None):
+        """out = sum_k coef[k] T_k(H~) v over the given (complex) coefficients
+        (tbk_cheb_expand).  Returns (out, kernel ms)."""
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
+        n = c.size // 2
+        if out is None:
+            out = self.zeros_block(v.rows, v.R, v.precision)
+        ms = C.c_float()
+        N.check_kpm(N.kpm().tbk_cheb_expand(self._h, M._h, float(a), float(b), v._h, int(n), _dp(c), out._h,
+                                            C.byref(ms)))
+        return out, ms.value
+
     def moments(self, M: Matrix, kets: Block, n: int, bras: Block | None = None, a: float = 1.0,
                 b: float = 0.0, plain: bool = False, time_launches: bool = False, want_last: bool = False,
                 row_kernel: bool = False):
