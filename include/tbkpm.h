@@ -103,6 +103,9 @@ typedef struct tbk_tuning {
     int32_t kubo_stack_mib;  /* tbk_kubo_gamma: the most device memory (MiB) its vector stacks and
                                 work blocks may take; beyond it the random vectors run in groups
                                 (0: 95% of the free memory) */
+    int32_t gamma_tile_rows; /* split-bf16 contraction over pre-split stacks: Gamma rows per
+                                workgroup tile, 256 (0: default; 64 x 64 per wave) or 128 (64 x 32
+                                per wave); both give the same bits */
 } tbk_tuning;
 
 const char* tbk_version(void);

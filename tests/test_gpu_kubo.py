@@ -525,3 +525,42 @@ def test_fp32_recursion_sigma_error_against_fp64(gpu_ctx):
     gb, _ = kpm.kubo_gamma(gpu_ctx, *M, K1, N, precision=kpm.GAMMA_BF16X3, g=g)
     assert st1["recursion_fp32"] == 0 and np.array_equal(g1, gb)
     K1.close()
+
+
+@pytest.mark.parametrize("N,L0,R", [(300, 9, 3), (512, 12, 2), (256, 7, 5)])
+def test_wide_contraction_tiles_give_the_same_bits(gpu_ctx, N, L0, R):
+    """The pre-split contraction's 256 x 128 tiles (64 x 64 per wave, the
+    default) and its 128 x 128 tiles form the same sums: Gamma bit-identical
+    between the two and to the BF16x3 contraction of the oracle's fp64
+    stacks, at N not a multiple of the tile (rows past N clamped and not
+    stored) and with K = sites x R not a multiple of the 16-column stage (a
+    short last stage through registers)."""
+    c = L.UnitCell.model("square")
+    c.add_hopping_nn("A", 1, 1.0)
+    lat = L.PartialLattice(c, (L0, L0 + 1))
+    lat.set_gauge(L.GAUGE_LANDAU, 2 * np.pi / 9)
+    lat.set_disorder(1.0, 4)
+    Hs = L.rescale(lat.get_Hamiltonian_sparse(True), 5.0, 0.0)
+    vx = L.commutator_diag(lat.op_position(0), Hs)
+    vy = L.commutator_diag(lat.op_position(1), Hs)
+    saved = gpu_ctx.get_tuning()
+    try:
+        K = gpu_ctx.random_block(Hs.n, R, seed=41)
+        kets = K.download()
+        M = [gpu_ctx.matrix(m) for m in (Hs, vx, vy)]
+        gpu_ctx.set_tuning(**dict(saved, gamma_tile_rows=0))
+        g_wide, _ = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=kpm.GAMMA_BF16X3)
+        gpu_ctx.set_tuning(**dict(saved, gamma_tile_rows=128))
+        g_sq, _ = kpm.kubo_gamma(gpu_ctx, *M, K, N, precision=kpm.GAMMA_BF16X3)
+        assert np.array_equal(g_wide, g_sq)
+        TL = oracle.cheb_on_bases(Hs, N, kets)
+        w = oracle.cheb_on_bases(vy, 2, kets)[1]
+        TR = oracle.cheb_on_bases(Hs, N, w)
+        aR = np.stack([oracle.cheb_on_bases(vx, 2, TR[m])[1] for m in range(N)])
+        Rp = 1 << (R - 1).bit_length()
+        pad = lambda T: np.concatenate([T, np.zeros(T.shape[:2] + (Rp - R,))], axis=2) if Rp > R else T
+        ref, _ = kpm.stack_contract(gpu_ctx, pad(TL).reshape(N, -1), pad(aR).reshape(N, -1), kpm.GAMMA_BF16X3)
+        assert np.array_equal(g_wide, ref)
+        K.close()
+    finally:
+        gpu_ctx.set_tuning(**saved)

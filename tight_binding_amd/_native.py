@@ -130,6 +130,7 @@ class tbk_tuning(C.Structure):
         ("row_lane_bytes", C.c_int32),
         ("row_slab_cols", C.c_int32),
         ("kubo_stack_mib", C.c_int32),
+        ("gamma_tile_rows", C.c_int32),
     ]
 
 

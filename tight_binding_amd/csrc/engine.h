@@ -277,6 +277,8 @@ struct ContractionShape {
     int64_t kc;
     int nchunks;
     tbk::GammaMap map;
+    bool wide;               // pre-split words: the 256 x 128 kernel on wide_map
+    tbk::GammaMap wide_map;  // (same chunks as map)
 };
 ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int prec);
 void enqueue_contraction(const ContractionShape& cs, const double2* A, const double2* B, int N, int64_t K, int64_t ldk,

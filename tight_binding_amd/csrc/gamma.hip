@@ -136,12 +136,12 @@ __device__ __forceinline__ bool xcd_work_item(const GammaMap& m, int& tn, int& t
     const int64_t q = static_cast<int64_t>(blockIdx.x >> 3);
     if (q >= m.per_xcd) return false;
     const int64_t w = static_cast<int64_t>(x) * m.per_xcd + q;
-    const int64_t tiles = static_cast<int64_t>(m.tiles_n) * m.tiles_n;
+    const int64_t tiles = static_cast<int64_t>(m.tiles_n) * m.tiles_m;
     if (w >= tiles * m.chunks) return false;
     c = w / tiles;
     const int t = static_cast<int>(w - c * tiles);
-    tn = t / m.tiles_n;
-    tm = t - tn * m.tiles_n;
+    tn = t / m.tiles_m;
+    tm = t - tn * m.tiles_m;
     return true;
 }
 
@@ -604,6 +604,182 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
             }
 }
 
+// ---------------------------------------------------------------------------
+// Wide form of the pre-split contraction: each workgroup owns a 256 x 128
+// tile (8 waves, 64 x 64 each = 4 x 4 blocks of 16 x 16), so every staged
+// word feeds twice the products of the 128 x 128 form.  Per stage a wave
+// reads the hi and lo fragments of its 4 A and 4 B blocks once (256 bytes a
+// lane); for each half of its B blocks it makes the 24 Re products (hi.hi,
+// hi.lo, lo.hi), turns those B fragments into B' = (im, -re) in place and
+// makes the 24 Im products: the same sums as the 128 x 128 kernel (the same
+// fragments in the same k order, the same phases per accumulator), so Gamma
+// is the same bits.  LDS: two stage
+// buffers of a 256-row A tile and a 128-row B tile (rows of 128 bytes,
+// chunk c of row r at c ^ ((r / 2) % 8)), 96 KB; a full stage goes global ->
+// LDS by the load unit, a short last stage through registers, written at once.
+constexpr int kWideStageBytes = (kWideTile + kBigTile) * kRawRowBytes;  // 48 KB
+
+__global__ __launch_bounds__(512) void gamma_bf16x3_wide_kernel(const uint2* __restrict__ A,
+                                                                const uint2* __restrict__ B, int N, int64_t K,
+                                                                int64_t wbs, int64_t kc, GammaMap map,
+                                                                double2* __restrict__ P) {
+    __shared__ __align__(1024) uint8_t S0[kWideStageBytes];
+    __shared__ __align__(1024) uint8_t S1[kWideStageBytes];
+    int tn, tm;
+    int64_t c;
+    if (!xcd_work_item(map, tn, tm, c)) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int li = lane & 15, lk = lane >> 4;
+    const int wn = wave >> 1, wm = wave & 1;
+    const int n0 = tn * kWideTile, m0 = tm * kBigTile;
+    const int64_t kb = c * kc;
+    const int64_t ke = (kb + kc < K) ? kb + kc : K;
+
+    // copies: wave w moves 8-row groups 4w .. 4w + 3 of A (i = 0 .. 3) and
+    // 2w, 2w + 1 of B (i = 4, 5); lane L takes row r = 8 g + L / 8 and the
+    // logical chunk that belongs at byte 16 L of the group, (L % 8) ^ ((r / 2) % 8)
+    const int lr = lane >> 3;
+    uint32_t roff[6];  // words from a sub-stage's start to the lane's chunk
+    int lcol[6];       // the chunk's first column within the sub-stage
+    int dst_off[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const bool isb = i >= 4;
+        const int g = isb ? 2 * wave + (i - 4) : 4 * wave + i;
+        const int r = 8 * g + lr;
+        const int lci = (lane & 7) ^ ((r >> 1) & 7);
+        const int row = isb ? ((m0 + r < N) ? m0 + r : N - 1) : ((n0 + r < N) ? n0 + r : N - 1);
+        roff[i] = static_cast<uint32_t>(row) * kWordBlock + 2 * lci;
+        lcol[i] = 2 * lci;
+        dst_off[i] = (isb ? kWideTile * kRawRowBytes : 0) + 8 * g * kRawRowBytes;
+    }
+    // a short last stage goes through registers and is written at once (no
+    // registers held across the products)
+    auto fetch = [&](int64_t s, uint8_t* buf) {
+        const int64_t k0 = kb + s * kSplitKt;
+        const bool full = k0 + kSplitKt <= ke;
+        const int64_t o = word_offset(k0, wbs);
+        if (full) {
+            const uint2* pa = A + o;
+            const uint2* pb = B + o;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+                __builtin_amdgcn_global_load_lds((i >= 4 ? pb : pa) + roff[i],
+                                                 (__attribute__((address_space(3))) void*)(buf + dst_off[i]), 16, 0,
+                                                 0);
+#endif
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                const uint2* src = (i >= 4 ? B : A) + o + roff[i];
+                const int64_t e = k0 + lcol[i];
+                const uint2 w0 = ld_or_zero(src, e < ke), w1 = ld_or_zero(src + 1, e + 1 < ke);
+                *reinterpret_cast<uint4*>(buf + dst_off[i] + 16 * lane) = make_uint4(w0.x, w0.y, w1.x, w1.y);
+            }
+        }
+    };
+
+    v4f re[4][4], im[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            re[i][j] = v4f{0, 0, 0, 0};
+            im[i][j] = v4f{0, 0, 0, 0};
+        }
+    auto rd = [&](const uint8_t* tile, int r, uint4& x0, uint4& x1) {
+        const uint8_t* row = tile + r * kRawRowBytes;
+        const int sw = (r >> 1) & 7;
+        x0 = *reinterpret_cast<const uint4*>(row + (((2 * lk) ^ sw) << 4));
+        x1 = *reinterpret_cast<const uint4*>(row + (((2 * lk + 1) ^ sw) << 4));
+    };
+    // B fragments are read, used and turned into B' two 16-column blocks at a
+    // time (j = 2h, 2h + 1), so the A fragments, two B blocks and the 128
+    // accumulators fit the register budget of two waves per SIMD
+    auto half = [&](const uint8_t* Bt, int h, const v8bf (&ah)[4], const v8bf (&al)[4]) {
+        v8bf bh[2], bl[2];
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+            uint4 x0, x1;
+            rd(Bt, wm * 64 + 16 * (2 * h + jj) + li, x0, x1);
+            bh[jj] = as_v8bf(make_uint4(x0.x, x0.z, x1.x, x1.z));
+            bl[jj] = as_v8bf(make_uint4(x0.y, x0.w, x1.y, x1.w));
+        }
+        // Re: hi.hi, hi.lo, lo.hi
+#pragma unroll
+        for (int ph = 0; ph < 3; ++ph)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    const int j = 2 * h + jj;
+                    re[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ph == 2 ? al[i] : ah[i],
+                                                                       ph == 1 ? bl[jj] : bh[jj], re[i][j], 0, 0, 0);
+                }
+        // B' = (im, -re) per word, in place
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+            uint4 hw = __builtin_bit_cast(uint4, bh[jj]), lw = __builtin_bit_cast(uint4, bl[jj]);
+            hw = make_uint4(prime_word(hw.x), prime_word(hw.y), prime_word(hw.z), prime_word(hw.w));
+            lw = make_uint4(prime_word(lw.x), prime_word(lw.y), prime_word(lw.z), prime_word(lw.w));
+            bh[jj] = as_v8bf(hw);
+            bl[jj] = as_v8bf(lw);
+        }
+#pragma unroll
+        for (int ph = 0; ph < 3; ++ph)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    const int j = 2 * h + jj;
+                    im[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ph == 2 ? al[i] : ah[i],
+                                                                       ph == 1 ? bl[jj] : bh[jj], im[i][j], 0, 0, 0);
+                }
+    };
+    auto step = [&](int64_t st, int64_t stages, const uint8_t* cur, uint8_t* nxt) {
+        v8bf ah[4], al[4];
+        const uint8_t* At = cur;
+        const uint8_t* Bt = cur + kWideTile * kRawRowBytes;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint4 x0, x1;
+            rd(At, wn * 64 + 16 * i + li, x0, x1);
+            ah[i] = as_v8bf(make_uint4(x0.x, x0.z, x1.x, x1.z));
+            al[i] = as_v8bf(make_uint4(x0.y, x0.w, x1.y, x1.w));
+        }
+        const bool ahead = st + 1 < stages;
+        if (ahead) fetch(st + 1, nxt);
+        half(Bt, 0, ah, al);
+        half(Bt, 1, ah, al);
+        __syncthreads();
+    };
+
+    const int64_t stages = (ke - kb + kSplitKt - 1) / kSplitKt;
+    fetch(0, S0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int64_t st = 0; st < stages; st += 2) {
+        step(st, stages, S0, S1);
+        if (st + 1 < stages) step(st + 1, stages, S1, S0);
+    }
+
+    double2* Pc = P + c * static_cast<int64_t>(N) * N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = n0 + wn * 64 + 16 * i + 4 * lk + t;
+                const int cl = m0 + wm * 64 + 16 * j + li;
+                if (row < N && cl < N)
+                    Pc[static_cast<int64_t>(row) * N + cl] =
+                        make_double2(static_cast<double>(re[i][j][t]), static_cast<double>(im[i][j][t]));
+            }
+}
+
 __global__ __launch_bounds__(256) void gamma_reduce_kernel(const double2* __restrict__ P, int nchunks, int64_t NN,
                                                            double2* __restrict__ out) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < NN;
@@ -749,6 +925,12 @@ void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int
                                       const GammaMap& map, double2* P, hipStream_t st) {
     gamma_bf16x3_big_kernel<uint2><<<dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0, st>>>(
         A, B, N, K, K, wbs, kc, map, P);
+}
+
+void launch_gamma_bf16x3_wide_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t wbs, int64_t kc,
+                                       const GammaMap& map, double2* P, hipStream_t st) {
+    gamma_bf16x3_wide_kernel<<<dim3(static_cast<unsigned>(8 * map.per_xcd)), dim3(512), 0, st>>>(A, B, N, K, wbs, kc,
+                                                                                               map, P);
 }
 
 namespace {

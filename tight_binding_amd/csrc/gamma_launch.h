@@ -12,13 +12,19 @@ namespace tbk {
 constexpr int kGammaTile = 64;  // Gamma rows/cols per workgroup
 constexpr int kBigTile = 128;   // Gamma rows/cols per workgroup of the large-N kernels
 
-// Work placement of the large-N kernels: tiles_n^2 tiles x `chunks` k chunks,
-// per_xcd work items per XCD (the grid is 8 * per_xcd workgroups).
+// Work placement of the large-N kernels: tiles_n x tiles_m tiles x `chunks`
+// k chunks, per_xcd work items per XCD (the grid is 8 * per_xcd workgroups).
 struct GammaMap {
     int32_t tiles_n;
     int32_t chunks;
     int64_t per_xcd;
+    int32_t tiles_m;
+    int32_t pad_;
 };
+
+// Gamma rows (A entries) per workgroup of the wide pre-split kernel; its
+// columns (B entries) are kBigTile.
+constexpr int kWideTile = 256;
 
 // Split-K partial sums P[c][n][m] for k in chunk c = [c*kc, min(K, (c+1)*kc)),
 // fp64 matrix cores.  A, B: N rows of K complex128 each, row n at n * ldk
@@ -49,6 +55,11 @@ void launch_gamma_bf16x3_big(const double2* A, const double2* B, int N, int64_t 
 // stacks are k-blocked with block stride wbs = N * kWordBlock (word_offset).
 void launch_gamma_bf16x3_big_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t wbs, int64_t kc,
                                       const GammaMap& map, double2* P, hipStream_t st);
+
+// The same partial sums over pre-split stacks with 256 x 128 tiles, 64 x 64
+// per wave (wider reuse of every staged word; map.tiles_n counts 256-row tiles).
+void launch_gamma_bf16x3_wide_presplit(const uint2* A, const uint2* B, int N, int64_t K, int64_t wbs, int64_t kc,
+                                       const GammaMap& map, double2* P, hipStream_t st);
 
 // out[word_offset(i, bs)] = split words of in[i] (n complex values)
 void launch_split_block(const double2* in, uint2* out, int64_t n, int64_t bs, hipStream_t st);

@@ -700,7 +700,8 @@ constexpr int64_t kSplitChunkCols = 32768;
 void contraction_shape_big(const tbk_context* ctx, int N, int64_t K, int64_t max_cols, int64_t& kc,
                            tbk::GammaMap& map) {
     map.tiles_n = (N + tbk::kBigTile - 1) / tbk::kBigTile;
-    const int64_t tiles = static_cast<int64_t>(map.tiles_n) * map.tiles_n;
+    map.tiles_m = map.tiles_n;
+    const int64_t tiles = static_cast<int64_t>(map.tiles_n) * map.tiles_m;
     int64_t chunks = std::max<int64_t>(1, (static_cast<int64_t>(ctx->num_cu) + tiles / 2) / tiles);
     while ((tiles * chunks) % 8 != 0 && chunks < 8) ++chunks;
     if (max_cols > 0) chunks = std::max<int64_t>(chunks, (K + max_cols - 1) / max_cols);
@@ -727,6 +728,14 @@ ContractionShape contraction_plan(const tbk_context* ctx, int N, int64_t K, int 
     if (cs.big) {
         contraction_shape_big(ctx, N, K, prec == TBK_GAMMA_FP64 ? 0 : kSplitChunkCols, cs.kc, cs.map);
         cs.nchunks = cs.map.chunks;
+        // pre-split words: 256 x 128 tiles over the same k chunks (so the
+        // same partial sums), unless the context asks for 128 x 128
+        cs.wide = prec == TBK_GAMMA_BF16X3 && ctx->tuning.gamma_tile_rows != 128;
+        cs.wide_map.tiles_n = (N + tbk::kWideTile - 1) / tbk::kWideTile;
+        cs.wide_map.tiles_m = (N + tbk::kBigTile - 1) / tbk::kBigTile;
+        cs.wide_map.chunks = cs.map.chunks;
+        cs.wide_map.per_xcd =
+            (static_cast<int64_t>(cs.wide_map.tiles_n) * cs.wide_map.tiles_m * cs.wide_map.chunks + 7) / 8;
     } else {
         contraction_shape(ctx, N, K, cs.kc, cs.nchunks);
     }
@@ -946,8 +955,11 @@ void enqueue_kubo_stacks_paired(const StepPlan& pP, const StepPlan& pY, double a
 void enqueue_contraction_presplit(const ContractionShape& cs, const uint2* A, const uint2* B, int N, int64_t K,
                                   double2* P, double2* out, hipStream_t st) {
     if (!cs.big) raise(TBK_E_UNSUPPORTED, "pre-split contraction needs N >= 256");
-    tbk::launch_gamma_bf16x3_big_presplit(A, B, N, K, static_cast<int64_t>(N) * tbk::kWordBlock, cs.kc, cs.map, P,
-                                          st);
+    const int64_t wbs = static_cast<int64_t>(N) * tbk::kWordBlock;
+    if (cs.wide)
+        tbk::launch_gamma_bf16x3_wide_presplit(A, B, N, K, wbs, cs.kc, cs.wide_map, P, st);
+    else
+        tbk::launch_gamma_bf16x3_big_presplit(A, B, N, K, wbs, cs.kc, cs.map, P, st);
     HIPC(hipGetLastError());
     tbk::launch_gamma_reduce(P, cs.nchunks, static_cast<int64_t>(N) * N, out, st);
     HIPC(hipGetLastError());
@@ -1368,6 +1380,8 @@ int tbk_context_set_tuning(tbk_context* ctx, const tbk_tuning* t) {
         if (!ctx || !t) raise(TBK_E_INVALID, "null argument");
         if (t->slab_cols < 0 || t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || t->l2_budget_bytes < 0)
             raise(TBK_E_INVALID, "tuning value out of range");
+        if (t->gamma_tile_rows != 0 && t->gamma_tile_rows != 128 && t->gamma_tile_rows != 256)
+            raise(TBK_E_INVALID, "gamma_tile_rows must be 0, 128 or 256");
         ctx->tuning = *t;
     });
 }

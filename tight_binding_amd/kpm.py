@@ -142,11 +142,12 @@ class Context:
         N.check_kpm(N.kpm().tbk_context_synchronize(self._h))
 
     def set_tuning(self, slab_cols=0, blocks_per_cu=0, xcd_split=0, l2_budget_bytes=0, stream_nt=0,
-                   use_tiled=0, chunk_rows=0, chunk_period=0, row_lane_bytes=0, row_slab_cols=0, kubo_stack_mib=0):
+                   use_tiled=0, chunk_rows=0, chunk_period=0, row_lane_bytes=0, row_slab_cols=0, kubo_stack_mib=0,
+                   gamma_tile_rows=0):
         """Launch-shape choices (0 = default)."""
         t = N.tbk_tuning(int(slab_cols), int(blocks_per_cu), int(xcd_split), int(l2_budget_bytes),
                          int(stream_nt), int(use_tiled), int(chunk_rows), int(chunk_period), int(row_lane_bytes),
-                         int(row_slab_cols), int(kubo_stack_mib))
+                         int(row_slab_cols), int(kubo_stack_mib), int(gamma_tile_rows))
         N.check_kpm(N.kpm().tbk_context_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, M: "Matrix", kets: "Block", a=1.0, b=0.0, steps=16, plain=False, refresh=False):
