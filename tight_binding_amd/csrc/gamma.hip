@@ -619,6 +619,12 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
 // LDS by the load unit, a short last stage through registers, written at once.
 constexpr int kWideStageBytes = (kWideTile + kBigTile) * kRawRowBytes;  // 48 KB
 
+// Position of a row's 16-byte chunks: chunk c of row r sits at c ^ wide_swizzle(r).
+// Rows are 128 bytes, so a chunk's banks depend on its position (and, with
+// 64 banks, on the row's parity): any 8 consecutive rows of a fragment read
+// get distinct positions, and rows r, r + 8 of one 16-row read differ too.
+__device__ __forceinline__ int wide_swizzle(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+
 __global__ __launch_bounds__(512) void gamma_bf16x3_wide_kernel(const uint2* __restrict__ A,
                                                                 const uint2* __restrict__ B, int N, int64_t K,
                                                                 int64_t wbs, int64_t kc, GammaMap map,
@@ -647,7 +653,7 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_wide_kernel(const uint2* __r
         const bool isb = i >= 4;
         const int g = isb ? 2 * wave + (i - 4) : 4 * wave + i;
         const int r = 8 * g + lr;
-        const int lci = (lane & 7) ^ ((r >> 1) & 7);
+        const int lci = (lane & 7) ^ wide_swizzle(r);
         const int row = isb ? ((m0 + r < N) ? m0 + r : N - 1) : ((n0 + r < N) ? n0 + r : N - 1);
         roff[i] = static_cast<uint32_t>(row) * kWordBlock + 2 * lci;
         lcol[i] = 2 * lci;
@@ -691,7 +697,7 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_wide_kernel(const uint2* __r
         }
     auto rd = [&](const uint8_t* tile, int r, uint4& x0, uint4& x1) {
         const uint8_t* row = tile + r * kRawRowBytes;
-        const int sw = (r >> 1) & 7;
+        const int sw = wide_swizzle(r);
         x0 = *reinterpret_cast<const uint4*>(row + (((2 * lk) ^ sw) << 4));
         x1 = *reinterpret_cast<const uint4*>(row + (((2 * lk + 1) ^ sw) << 4));
     };
