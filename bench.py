@@ -214,6 +214,16 @@ def batch_leg(ctx, nprob, R=8):
     log("batch: kernel %.1f ms, wall %.1f ms, %.1f GB/s, shape cpl=%d bpc=%d, %d workgroups (create %.1f s, "
         "tune %.3f ms/step)" % (st["kernel_ms"], st["wall_ms"], gbps, st["cpl"], st["blocks_per_cu"], st["workgroups"],
                                 create_s, tune_ms))
+    # the same batch as persistent launches (one per 64-step chunk, problems
+    # on XCD queues), each chunk a hipGraph
+    mus_p, _ = B.run(graphs=True, persistent=True)
+    pruns = [B.run(graphs=True, persistent=True)[1] for _ in range(2)]
+    stp = min(pruns, key=lambda d: d["kernel_ms"])
+    gbps_p = stp["bytes_total"] / (stp["kernel_ms"] * 1e-3) / 1e9
+    dmu_p = max(float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(mus_p, mus))
+    log("batch persistent: kernel %.1f ms, wall %.1f ms, %.1f GB/s, %d launches of %d workgroups, "
+        "max |dmu|/max|mu| vs per-step launches %.2e" % (stp["kernel_ms"], stp["wall_ms"], gbps_p, stp["launches"],
+                                                         stp["workgroups"], dmu_p))
     B.close()
 
     # plain per-solve path: one moments call per problem, matrices resident
@@ -246,6 +256,10 @@ def batch_leg(ctx, nprob, R=8):
         "batched_GBps": gbps, "batched_frac_hbm": gbps / HBM_PEAK_GBS,
         "launch_shape": {"cpl": st["cpl"], "blocks_per_cu": st["blocks_per_cu"], "workgroups": st["workgroups"]},
         "graphs_per_run": st["graphs"], "step_launches": st["launches"],
+        "persistent": {"kernel_ms": stp["kernel_ms"], "wall_ms": stp["wall_ms"], "GBps": gbps_p,
+                       "frac_hbm": gbps_p / HBM_PEAK_GBS, "launches": stp["launches"],
+                       "workgroups": stp["workgroups"], "moments_per_s": moments / (stp["wall_ms"] * 1e-3),
+                       "max_rel_dmu_vs_per_step_launches": dmu_p},
         "per_solve_kernel_ms": k_ms, "per_solve_wall_ms": 1e3 * w_s,
         "per_solve_moments_per_s": moments / w_s,
         "speedup_wall": (1e3 * w_s) / st["wall_ms"],

@@ -489,12 +489,25 @@ int tbk_spectral_range(const tbk_sparse* M, double* e_min, double* e_max);
 /* ------------------------------------------- batched ragged moment problems */
 typedef struct tbk_batch tbk_batch;
 
-enum { TBK_BATCH_GRAPHS = 1 /* capture each chunk of 64 steps in a hipGraph, replay on later runs */ };
+enum {
+    TBK_BATCH_GRAPHS = 1,    /* capture each chunk of 64 steps in a hipGraph, replay on later runs */
+    /* each chunk of 64 steps as one persistent launch: every problem is placed
+     * on one XCD, its rows cut into shares of 4096; the workgroups on an XCD
+     * draw (step, problem, share) items from that XCD's queue in step order,
+     * an item waiting (bounded) until its problem's previous step is done.
+     * The moments are summed in a fixed order after the run, so a problem's
+     * moments do not depend on the rest of the batch; they differ from the
+     * per-step launches' only in the grouping of the partial sums.  A wait
+     * that gives up, or a queue left unserved, fails the run with
+     * TBK_E_DEVICE. */
+    TBK_BATCH_PERSISTENT = 2
+};
 
 typedef struct tbk_batch_stats {
     double kernel_ms;      /* device time from the first step launch to the last moment launch */
     double wall_ms;        /* host wall time of the call */
-    int64_t launches;      /* batched step launches (the largest step count in the batch) */
+    int64_t launches;      /* batched step launches (the largest step count in the batch), or
+                              persistent launches (one per chunk of 64 steps) */
     int64_t graphs;        /* chunk graphs launched (0 without TBK_BATCH_GRAPHS) */
     double bytes_total;    /* algorithmic HBM bytes of every problem's steps */
     int64_t rows_total;    /* rows of all problems */

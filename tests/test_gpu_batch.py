@@ -172,3 +172,55 @@ def test_batch_errors(gpu_ctx):
         B.set_shape(3, 4)
     assert e.value.code == N.TBK_E_UNSUPPORTED
     B.close()
+
+
+def test_persistent_batch_matches_oracle_and_per_step_launches(gpu_ctx):
+    """TBK_BATCH_PERSISTENT: each chunk of 64 steps as one persistent launch
+    (problems on XCD queues, items in step order).  The vectors are the
+    per-step launches' (the same step_row sums), the moments summed in
+    another grouping: within 1e-12 of the per-step batch and 1e-10 of the
+    oracle, over step counts that span several chunks (257 moments = 129
+    steps) and problems larger than one 4096-row share."""
+    probs = mixed_batch(7, base_seed=5)
+    H, a = model_problem("kagome", 40, 1.3, seed=77)  # 4800 rows: two shares
+    probs.append((H, a, 0.05, 257, 4242))
+    B = make_batch(gpu_ctx, probs, R=3)
+    mus_p, st = B.run(persistent=True)
+    assert st["launches"] == (max((p[3] + 1) // 2 for p in probs) + 63) // 64
+    mus_s, _ = B.run()
+    for (H, a, b, n, seed), mp, ms in zip(probs, mus_p, mus_s):
+        assert np.max(np.abs(mp - ms)) <= 1e-12 * np.max(np.abs(ms)), n
+        K = gpu_ctx.random_block(H.n, 3, seed=seed)
+        kets = K.download()
+        Hs = O.rescale(O.CSC(H.n, H.colptr, H.rowidx, H.values), a, b)
+        ref = oracle.cheb_braket(Hs, n, kets, kets)
+        assert np.max(np.abs(mp - ref)) <= 1e-10 * np.max(np.abs(ref)), n
+        K.close()
+    B.close()
+
+
+@pytest.mark.parametrize("precision", [kpm.FP64, kpm.FP32])
+def test_persistent_batch_alone_mixed_and_graphs_are_the_same_bits(gpu_ctx, precision):
+    """A problem's persistent-launch moments do not depend on the rest of
+    the batch (its own XCD queue placement changes, its partial grouping
+    does not), nor on hipGraph capture: the same bits alone, in a mix, and
+    over several graph replays with eager runs between them."""
+    probs = mixed_batch(9, base_seed=8)
+    B = make_batch(gpu_ctx, probs, R=5, precision=precision)
+    mixed, _ = B.run(persistent=True)
+    reps = []
+    for _ in range(3):
+        g, st = B.run(graphs=True, persistent=True)
+        assert st["graphs"] == st["launches"]
+        reps.append(g)
+        e, _ = B.run(persistent=True)
+        reps.append(e)
+    for r in reps:
+        for x, y in zip(mixed, r):
+            assert np.array_equal(x, y)
+    for i in (0, 4, 8):
+        A = make_batch(gpu_ctx, [probs[i]], R=5, precision=precision)
+        alone, _ = A.run(persistent=True)
+        assert np.array_equal(alone[0], mixed[i]), i
+        A.close()
+    B.close()

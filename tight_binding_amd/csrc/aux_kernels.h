@@ -273,7 +273,63 @@ __global__ __launch_bounds__(kBlock) void slots_moments_kernel(const SlotDesc* _
     }
 }
 
-// out = c * t (init) or out + c * t, elementwise over n complex values, c
+// Moments of the persistent batched recursion (one workgroup per slot):
+// for each of the slot's steps k, its nshare item partials at
+// part[pbase + k * nshare .. ] summed in share order, then mu_0, mu_1
+// (k = 0) or the doubling identities mu_2k = 2 s.x - mu_0,
+// mu_2k+1 = 2 s.y - mu_1.
+__global__ __launch_bounds__(kBlock) void slots_history_moments_kernel(const SlotDesc* __restrict__ slots,
+                                                                       const int32_t* __restrict__ nshare,
+                                                                       const int64_t* __restrict__ pbase,
+                                                                       const double2* __restrict__ part) {
+    const SlotDesc& d = slots[blockIdx.x];
+    const int32_t ns = nshare[blockIdx.x];
+    const double2* p = part + pbase[blockIdx.x];
+    double m0 = 0.0, m1 = 0.0;
+    for (int32_t j = 0; j < ns; ++j) {
+        m0 = m0 + p[j].x;
+        m1 = m1 + p[j].y;
+    }
+    double* m = d.mu;
+    for (int64_t k = threadIdx.x; k < d.steps; k += kBlock) {
+        if (k == 0) {
+            m[0] = m0;
+            m[1] = m1;
+            continue;
+        }
+        const double2* pk = p + k * ns;
+        double a0 = 0.0, a1 = 0.0;
+        for (int32_t j = 0; j < ns; ++j) {
+            a0 = a0 + pk[j].x;
+            a1 = a1 + pk[j].y;
+        }
+        if (2 * k < d.n) m[2 * k] = 2.0 * a0 - m0;
+        if (2 * k + 1 < d.n) m[2 * k + 1] = 2.0 * a1 - m1;
+    }
+}
+
+// Resets before a persistent launch (a kernel, not a memset, so that it can
+// be captured): the done counts of n slots, the nq ticket counters, and the
+// error flag when given.
+__global__ __launch_bounds__(kBlock) void persist_reset_kernel(int32_t* __restrict__ done, int32_t n,
+                                                               int32_t* __restrict__ counters, int32_t nq,
+                                                               int32_t* __restrict__ error) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    for (int64_t i = gid; i < n; i += static_cast<int64_t>(gridDim.x) * kBlock) done[i] = 0;
+    if (gid < nq) counters[gid] = 0;
+    if (gid == 0 && error) *error = 0;  // the first launch of a run
+}
+
+// After a persistent launch: *error if a queue was not served to its end (a
+// workgroup that draws a ticket works on it, so a served queue's counter
+// reaches its item count).
+__global__ void persist_check_kernel(const int32_t* __restrict__ counters, const int64_t* __restrict__ totals,
+                                     int32_t nq, int32_t* __restrict__ error) {
+    const int q = static_cast<int>(threadIdx.x);
+    if (q < nq && counters[q] < totals[q]) *error = 1;
+}
+
+// out = c * t (init) or out + c * t, elementwise over n complex values, c// out = c * t (init) or out + c * t, elementwise over n complex values, c
 // complex; the product is formed as (c.re t.re - c.im t.im, c.re t.im +
 // c.im t.re) in the block's precision.  With c.im = 0 each part is c.re times
 // the part, the scalar-times-matrix product of a real coefficient.

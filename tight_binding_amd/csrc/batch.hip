@@ -17,6 +17,15 @@
 //
 // Graphs.  With TBK_BATCH_GRAPHS the launches of each chunk of 64 steps are
 // captured once into a hipGraph (no memset nodes) and replayed on later runs.
+//
+// Persistent launches.  With TBK_BATCH_PERSISTENT each chunk of 64 steps is
+// one launch of cheb_slots_persistent_kernel (plus a reset and a check
+// kernel, captured together with TBK_BATCH_GRAPHS): every problem is placed
+// on one XCD's queue, its rows cut into shares, and the workgroups on that
+// XCD work through the queue's (step, problem, share) items in step order
+// (cheb_kernels.h).  The item partials of every step are kept and
+// slots_history_moments_kernel sums them per problem in share order at the
+// end, so a problem's moments again do not depend on the batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +43,10 @@ namespace {
 using namespace engine;
 
 constexpr int64_t kChunkSteps = 64;
+// persistent launches: XCD queues, rows per share, polls before a wait gives up
+constexpr int kPersistQueues = 8;
+constexpr int32_t kPersistShareRows = 4096;
+constexpr int32_t kPersistSpinLimit = 1 << 22;
 
 }  // namespace
 
@@ -57,10 +70,24 @@ struct tbk_batch {
     size_t elem() const { return precision == TBK_FP64 ? 16 : 8; }
     int32_t cpl_used() const { return slot_cpl(cpl, precision, Rp); }
 
+    // Persistent launches (TBK_BATCH_PERSISTENT): every problem on one XCD
+    // queue, items (step, problem, share) in step order; built on first use.
+    struct Persist {
+        bool built = false;
+        int32_t nchunks = 0;
+        int64_t grid = 0;
+        std::vector<int64_t> step_base;  // per chunk: offset of its per-queue step prefixes
+        DevBuf nshare, pbase, q_slots, q_off, q_share_pre, q_step_items, totals, counters, done, error, part;
+        std::vector<hipGraphExec_t> graphs;
+    } ps;
+
     void drop_graphs() {
         for (hipGraphExec_t g : graphs)
             if (g) (void)hipGraphExecDestroy(g);
         graphs.clear();
+        for (hipGraphExec_t g : ps.graphs)
+            if (g) (void)hipGraphExecDestroy(g);
+        ps.graphs.clear();
     }
     ~tbk_batch() { drop_graphs(); }
 
@@ -166,7 +193,132 @@ struct tbk_batch {
 
     int64_t device_bytes() const {
         return static_cast<int64_t>(rp.bytes + ci.bytes + val.bytes + mu.bytes + partial.bytes + desc.bytes +
-                                    wg_slot.bytes + buf[0].bytes + buf[1].bytes + buf[2].bytes);
+                                    wg_slot.bytes + buf[0].bytes + buf[1].bytes + buf[2].bytes + ps.part.bytes +
+                                    ps.q_step_items.bytes);
+    }
+
+    // The persistent plan: problems placed on the 8 XCD queues greedily by
+    // work (rows x steps, largest first), each queue's problems in layout
+    // order (decreasing steps, so the problems active at a step are a
+    // prefix); shares of kPersistShareRows rows; per chunk of kChunkSteps
+    // steps and queue, the items before each step.
+    void build_persist(hipStream_t st) {
+        constexpr int nq = kPersistQueues;
+        std::vector<int32_t> nsh(static_cast<size_t>(np));
+        std::vector<int64_t> pb(static_cast<size_t>(np));
+        int64_t acc = 0;
+        for (int32_t i = 0; i < np; ++i) {
+            nsh[i] = static_cast<int32_t>((rows[i] + kPersistShareRows - 1) / kPersistShareRows);
+            pb[i] = acc;
+            acc += steps[i] * nsh[i];
+        }
+        std::vector<int32_t> byw(static_cast<size_t>(np));
+        std::iota(byw.begin(), byw.end(), 0);
+        std::stable_sort(byw.begin(), byw.end(),
+                         [&](int32_t x, int32_t y) { return rows[x] * steps[x] > rows[y] * steps[y]; });
+        std::vector<std::vector<int32_t>> xq(nq);
+        std::vector<double> load(nq, 0.0);
+        for (int32_t p : byw) {
+            const int q = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            xq[q].push_back(p);
+            load[q] += static_cast<double>(rows[p]) * static_cast<double>(steps[p]);
+        }
+        std::vector<int32_t> qs, qo(nq + 1, 0);
+        std::vector<int64_t> pre;
+        for (int q = 0; q < nq; ++q) {
+            std::sort(xq[q].begin(), xq[q].end());
+            qo[q] = static_cast<int32_t>(qs.size());
+            int64_t a = 0;
+            pre.push_back(0);
+            for (int32_t p : xq[q]) {
+                qs.push_back(p);
+                a += nsh[p];
+                pre.push_back(a);
+            }
+        }
+        qo[nq] = static_cast<int32_t>(qs.size());
+        const int32_t nch = static_cast<int32_t>((max_steps + kChunkSteps - 1) / kChunkSteps);
+        std::vector<int64_t> items, tot;
+        ps.step_base.assign(static_cast<size_t>(nch), 0);
+        for (int32_t c = 0; c < nch; ++c) {
+            const int64_t k0 = static_cast<int64_t>(c) * kChunkSteps, k1 = std::min(max_steps, k0 + kChunkSteps);
+            ps.step_base[c] = static_cast<int64_t>(items.size());
+            for (int q = 0; q < nq; ++q) {
+                const int32_t n_q = qo[q + 1] - qo[q];
+                int64_t a = 0;
+                for (int64_t k = k0; k <= k1; ++k) {
+                    items.push_back(a);
+                    if (k == k1) break;
+                    int32_t act = 0;  // problems of the queue still active at step k (a prefix)
+                    while (act < n_q && steps[qs[qo[q] + act]] > k) ++act;
+                    a += pre[static_cast<size_t>(qo[q] + q + act)];
+                }
+                tot.push_back(a);
+            }
+        }
+        auto up = [&](DevBuf& d, const void* src, size_t bytes) {
+            d.reserve(std::max<size_t>(bytes, 16));
+            if (bytes) HIPC(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, st));
+        };
+        up(ps.nshare, nsh.data(), nsh.size() * sizeof(int32_t));
+        up(ps.pbase, pb.data(), pb.size() * sizeof(int64_t));
+        up(ps.q_slots, qs.data(), qs.size() * sizeof(int32_t));
+        up(ps.q_off, qo.data(), qo.size() * sizeof(int32_t));
+        up(ps.q_share_pre, pre.data(), pre.size() * sizeof(int64_t));
+        up(ps.q_step_items, items.data(), items.size() * sizeof(int64_t));
+        up(ps.totals, tot.data(), tot.size() * sizeof(int64_t));
+        ps.counters.reserve(nq * sizeof(int32_t));
+        ps.done.reserve(static_cast<size_t>(np) * sizeof(int32_t));
+        ps.error.reserve(sizeof(int32_t));
+        ps.part.reserve(std::max<size_t>(static_cast<size_t>(acc) * sizeof(double2), 16));
+        HIPC(hipStreamSynchronize(st));
+        ps.nchunks = nch;
+        ps.grid = static_cast<int64_t>(nq) * std::max(1, ctx->num_cu / nq) * std::max(1, bpc);
+        ps.built = true;
+    }
+
+    // chunk c: resets, the persistent launch, the check that every queue was served
+    void launch_persist_chunk(int32_t c, hipStream_t st) {
+        constexpr int nq = kPersistQueues;
+        const int64_t k0 = static_cast<int64_t>(c) * kChunkSteps, k1 = std::min(max_steps, k0 + kChunkSteps);
+        hipLaunchKernelGGL(tbk::persist_reset_kernel, dim3(static_cast<unsigned>((np + tbk::kBlock - 1) / tbk::kBlock)),
+                           dim3(tbk::kBlock), 0, st, ps.done.as<int32_t>(), np, ps.counters.as<int32_t>(), nq,
+                           c == 0 ? ps.error.as<int32_t>() : nullptr);
+        HIPC(hipGetLastError());
+        tbk::PersistSched S{};
+        S.slots = desc.as<const tbk::SlotDesc>();
+        S.nshare = ps.nshare.as<const int32_t>();
+        S.pbase = ps.pbase.as<const int64_t>();
+        S.q_slots = ps.q_slots.as<const int32_t>();
+        S.q_off = ps.q_off.as<const int32_t>();
+        S.q_share_pre = ps.q_share_pre.as<const int64_t>();
+        S.q_step_items = ps.q_step_items.as<const int64_t>() + ps.step_base[c];
+        S.counters = ps.counters.as<int32_t>();
+        S.done = ps.done.as<int32_t>();
+        S.error = ps.error.as<int32_t>();
+        S.part = ps.part.as<double2>();
+        S.k0 = static_cast<int32_t>(k0);
+        S.k1 = static_cast<int32_t>(k1);
+        S.share_rows = kPersistShareRows;
+        S.spin_limit = kPersistSpinLimit;
+        S.Rp = Rp;
+        // rows pass between workgroups through the XCD's L2: every access
+        // temporal (a non-temporal store does not leave the row in the L2
+        // the next step reads it from)
+        S.opts = 0;
+        for (int i = 0; i < 16; ++i) S.xcc_queue[i] = i < nq ? i : -1;
+        const dim3 grid(static_cast<unsigned>(ps.grid));
+        const int cp = cpl_used();
+        bool ok;
+        if (precision == TBK_FP64)
+            ok = cplx ? tbk::launch_persist_f64_cplx(Rp, cp, grid, st, S) : tbk::launch_persist_f64_real(Rp, cp, grid, st, S);
+        else
+            ok = cplx ? tbk::launch_persist_f32_cplx(Rp, cp, grid, st, S) : tbk::launch_persist_f32_real(Rp, cp, grid, st, S);
+        if (!ok) raise(TBK_E_UNSUPPORTED, "no persistent batched kernel for this width");
+        HIPC(hipGetLastError());
+        hipLaunchKernelGGL(tbk::persist_check_kernel, dim3(1), dim3(64), 0, st, ps.counters.as<const int32_t>(),
+                           ps.totals.as<const int64_t>() + static_cast<int64_t>(c) * nq, nq, ps.error.as<int32_t>());
+        HIPC(hipGetLastError());
     }
 };
 
@@ -274,9 +426,72 @@ void batch_run(tbk_batch* B, int32_t flags, double* mu_out, tbk_batch_stats* sta
     set_device(ctx);
     hipStream_t st = ctx->stream;
     const bool use_graphs = (flags & TBK_BATCH_GRAPHS) != 0;
+    const bool persistent = (flags & TBK_BATCH_PERSISTENT) != 0;
+    if (flags & ~(TBK_BATCH_GRAPHS | TBK_BATCH_PERSISTENT)) raise(TBK_E_INVALID, "unknown batch flag");
     ensure_events(ctx, 2);
+    if (persistent && !B->ps.built) {
+        const int64_t before = B->device_bytes();
+        B->build_persist(st);
+        ctx->held_bytes += B->device_bytes() - before;
+    }
     B->init_kets(st);
     HIPC(hipEventRecord(ctx->events[0], st));
+    if (persistent) {
+        int64_t graphs_launched = 0;
+        if (use_graphs && static_cast<int32_t>(B->ps.graphs.size()) != B->ps.nchunks)
+            B->ps.graphs.assign(static_cast<size_t>(B->ps.nchunks), nullptr);
+        for (int32_t c = 0; c < B->ps.nchunks; ++c) {
+            if (!use_graphs) {
+                B->launch_persist_chunk(c, st);
+                continue;
+            }
+            hipGraphExec_t& ex = B->ps.graphs[static_cast<size_t>(c)];
+            if (!ex) {
+                hipGraph_t g = nullptr;
+                HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                try {
+                    B->launch_persist_chunk(c, st);
+                } catch (...) {
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                HIPC(hipStreamEndCapture(st, &g));
+                const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIPC(e);
+            }
+            HIPC(hipGraphLaunch(ex, st));
+            ++graphs_launched;
+        }
+        hipLaunchKernelGGL(tbk::slots_history_moments_kernel, dim3(static_cast<unsigned>(B->np)), dim3(tbk::kBlock), 0,
+                           st, B->desc.as<const tbk::SlotDesc>(), B->ps.nshare.as<const int32_t>(),
+                           B->ps.pbase.as<const int64_t>(), B->ps.part.as<const double2>());
+        HIPC(hipGetLastError());
+        HIPC(hipEventRecord(ctx->events[1], st));
+        std::vector<double> mu(static_cast<size_t>(B->mu_total));
+        int32_t err = 0;
+        HIPC(hipMemcpyAsync(mu.data(), B->mu.p, mu.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&err, B->ps.error.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (err != 0)
+            raise(TBK_E_DEVICE, "persistent batch launch: a wait gave up or an XCD queue was not served");
+        std::memcpy(mu_out, mu.data(), mu.size() * sizeof(double));
+        if (stats) {
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+            stats->kernel_ms = ms;
+            stats->launches = B->ps.nchunks;
+            stats->graphs = graphs_launched;
+            stats->bytes_total = B->bytes_total();
+            stats->rows_total = B->rows_total;
+            stats->workgroups = B->ps.grid;
+            stats->cpl = B->cpl_used();
+            stats->blocks_per_cu = B->bpc;
+            stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return;
+    }
     const int64_t nchunks = (B->max_steps + kChunkSteps - 1) / kChunkSteps;
     int64_t graphs_launched = 0;
     if (use_graphs && static_cast<int64_t>(B->graphs.size()) != nchunks) {
@@ -341,6 +556,7 @@ void set_shape(tbk_batch* B, int32_t cpl, int32_t bpc) {
     B->cpl = cpl;
     B->bpc = bpc;
     B->drop_graphs();
+    B->ps.grid = static_cast<int64_t>(kPersistQueues) * std::max(1, B->ctx->num_cu / kPersistQueues) * std::max(1, bpc);
     B->build_descriptors(B->ctx->stream);
     B->ctx->held_bytes += B->device_bytes() - before;
 }

@@ -687,6 +687,173 @@ __global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __re
     }
 }
 
+// ------------------------------------------- persistent batched recursion
+// One launch runs steps [k0, k1) of every slot.  Each slot (problem) lives
+// on one XCD: its row tiles are dealt round robin to ns = ceil(rows /
+// share_rows) shares, and the XCD's queue lists the items (step k, slot,
+// share) step by step -- all
+// items of step k of the queue's slots, then those of step k + 1.  The
+// workgroups on an XCD (the XCC_ID hardware register) draw tickets from
+// its queue counter in order, one ahead of the item they work on; so a
+// slot's vectors, its counter and every reader and writer of them share one
+// L2.  An item of step k > k0 waits until every share of its slot's step
+// k - 1 is done (done[slot] counts the shares finished in this launch): all
+// of those have lower tickets of the same queue, drawn by workgroups that
+// never wait on higher ones, so every wait ends whatever the residency.  A
+// wait is still bounded: past spin_limit polls, or once another wait has
+// given up, it sets *error and goes on, and the host discards the results.
+//
+// The rows are loaded and stored with temporal accesses only (opts without
+// kOptStream): a non-temporal store does not leave the row in the L2 that
+// the next step gathers it from.
+//
+// Ordering within one XCD: after an item every wave waits until L2 has
+// acknowledged its row stores, the workgroup meets at a barrier, and thread
+// 0 adds to done[slot] (an L2 atomic); a waiting item polls with L2 loads
+// and takes an agent-scope acquire (its CU's L1 is invalidated) before the
+// barrier that releases its waves to gather.  Every item takes it, since the
+// CU's L1 may hold lines of the block from an earlier step (a workgroup-scope
+// invalidate, buffer_inv sc0, left stale rows in the moments).  Partial moments of item (k, slot, share) go to
+// part[pbase[slot] + k * nshare[slot] + share]; slots_history_moments_kernel
+// sums them in share order after the launch.
+struct PersistSched {
+    const SlotDesc* slots;         // per slot (layout position)
+    const int32_t* nshare;         // shares per slot
+    const int64_t* pbase;          // first partial of each slot
+    const int32_t* q_slots;        // slots of queue q at [q_off[q], q_off[q + 1]), decreasing step counts
+    const int32_t* q_off;
+    const int64_t* q_share_pre;    // per queue: prefix sums of nshare over its slots (length + 1 each),
+                                   // at q_off[q] + q
+    const int64_t* q_step_items;   // per queue: items before step k0 + i, i = 0 .. k1 - k0 (k1 - k0 + 1
+                                   // entries), at q * (k1 - k0 + 1)
+    int32_t* counters;             // ticket counter per queue
+    int32_t* done;                 // shares finished per slot in this launch
+    int32_t* error;
+    double2* part;
+    int32_t k0, k1, share_rows, spin_limit;
+    int32_t Rp, opts;
+    int32_t xcc_queue[16];         // queue of each XCC_ID value, -1: none
+};
+
+// XCC_ID hardware register (bits 3:0): the XCD the wave runs on
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
+
+// largest i in [0, n) with a[i] <= v (a ascending, a[0] <= v)
+__device__ __forceinline__ int32_t last_at_most(const int64_t* a, int32_t n, int64_t v) {
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) / 2;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename VT, bool CH, int LPR, int CPL>
+__global__ __launch_bounds__(kBlock) void cheb_slots_persistent_kernel(PersistSched s) {
+    using V2 = typename Vec2<VT>::type;
+    using H = typename HVal<VT, CH>::type;
+    constexpr int RPW = kWave / LPR;
+    constexpr int TR = kWavesPerBlock * RPW;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int sub = lane / LPR;
+    const int q = lane % LPR;
+    const int qid = s.xcc_queue[xcc_id()];
+    if (qid < 0) return;  // an XCD without a queue (the whole workgroup, before any barrier)
+    const int32_t nk = s.k1 - s.k0;
+    const int64_t* step_items = s.q_step_items + static_cast<int64_t>(qid) * (nk + 1);
+    const int64_t total = step_items[nk];
+    const int32_t qs0 = s.q_off[qid], nqs = s.q_off[qid + 1] - qs0;
+    const int64_t* share_pre = s.q_share_pre + qs0 + qid;
+    int32_t* counter = s.counters + qid;
+
+    __shared__ double red[2][kWavesPerBlock];
+    __shared__ int32_t ticket[2];
+    if (threadIdx.x == 0)
+        ticket[0] = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    int64_t cur = ticket[0];
+    while (cur < total) {
+        if (threadIdx.x == 0)
+            ticket[1] = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the item: step k0 + kk, the a-th slot of the queue, share j
+        const int32_t kk = last_at_most(step_items, nk, cur);
+        const int64_t i = cur - step_items[kk];
+        const int32_t a = last_at_most(share_pre, nqs, i);
+        const int32_t j = static_cast<int32_t>(i - share_pre[a]);
+        const int32_t slot = s.q_slots[qs0 + a];
+        const int64_t k = s.k0 + kk;
+        const int32_t ns = s.nshare[slot];
+        if (wave == 0) {
+            if (kk > 0) {
+                const int32_t need = kk * ns;
+                int32_t spins = 0;
+                for (;;) {
+                    const int32_t v = __hip_atomic_load(s.done + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (v >= need) break;
+                    if (++spins > s.spin_limit ||
+                        __hip_atomic_load(s.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        if (lane == 0) __hip_atomic_store(s.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+
+        const SlotDesc& d = s.slots[slot];
+        const V2* x = static_cast<const V2*>(d.blk[k % 3]);
+        const V2* z = static_cast<const V2*>(d.blk[(k + 2) % 3]);
+        V2* y = static_cast<V2*>(d.blk[(k + 1) % 3]);
+        const double ca = d.coef[k == 0 ? 0 : 2], cb = d.coef[k == 0 ? 1 : 3];
+        const VT alpha = static_cast<VT>(ca), beta = static_cast<VT>(cb);
+        const int32_t o = (cb != 0.0) ? (s.opts | kOptShift) : (s.opts & ~kOptShift);
+        const int32_t rows = d.rows;
+        // share j: row tiles j, j + ns, j + 2 ns, ... (TR rows each), so the
+        // workgroups on a slot sweep its rows side by side and a row's
+        // neighbours one lattice row away are gathered while still in L2
+        const int64_t ntiles = (static_cast<int64_t>(rows) + TR - 1) / TR;
+        double p0 = 0.0, p1 = 0.0;
+        for (int64_t t = j; t < ntiles; t += ns) {
+            const int64_t rr = t * TR + wave * RPW + sub;
+            if (rr >= rows) continue;
+            int r = static_cast<int>(rr);
+            if (LPR == kWave) r = __builtin_amdgcn_readfirstlane(r);
+            if (k == 0)
+                step_row<VT, CH, LPR, CPL, kFirst>(d.rp, d.ci, static_cast<const H*>(d.hv), x, z, y, x, r, q, s.Rp,
+                                                   alpha, beta, o, p0, p1);
+            else
+                step_row<VT, CH, LPR, CPL, kRecDbl>(d.rp, d.ci, static_cast<const H*>(d.hv), x, z, y, x, r, q,
+                                                    s.Rp, alpha, beta, o, p0, p1);
+        }
+        p0 = wave_sum(p0);
+        p1 = wave_sum(p1);
+        if (lane == 0) {
+            red[0][wave] = p0;
+            red[1][wave] = p1;
+        }
+        // every wave waits until L2 has acknowledged its row stores; the
+        // count then goes to the same L2
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s0 = red[0][0], s1 = red[1][0];
+            for (int w = 1; w < kWavesPerBlock; ++w) {
+                s0 = s0 + red[0][w];
+                s1 = s1 + red[1][w];
+            }
+            s.part[s.pbase[slot] + k * ns + j] = make_double2(s0, s1);
+            __hip_atomic_fetch_add(s.done + slot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();  // ticket[1] stored; red free again
+        cur = ticket[1];
+        __syncthreads();  // everyone has read ticket[1] before thread 0 overwrites it
+    }
+}
+
 // Tiled variant of cheb_step_kernel for matrices with at most KMAX = 4*KCH
 // entries per row (every lattice Hamiltonian of the presets: 4 to 13).
 // Per tile of TR consecutive rows the workgroup first stages the row

@@ -78,6 +78,42 @@ bool launch_slots(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a)
     return false;
 }
 
+bool launch_persist_f64_real(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSched& s);
+bool launch_persist_f64_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSched& s);
+bool launch_persist_f32_real(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSched& s);
+bool launch_persist_f32_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSched& s);
+
+// the persistent batched recursion for a slot class (the widths of cheb_slots_kernel)
+template <typename VT, bool CH>
+bool launch_persist(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSched& s) {
+#define TBK_PERSIST_CASE(RP, C)                                                                               \
+    if (Rp == RP && cpl == C) {                                                                               \
+        hipLaunchKernelGGL((cheb_slots_persistent_kernel<VT, CH, RP / C, C>), grid, dim3(kBlock), 0, st, s); \
+        return true;                                                                                          \
+    }
+    TBK_PERSIST_CASE(1, 1)
+    TBK_PERSIST_CASE(2, 1)
+    TBK_PERSIST_CASE(2, 2)
+    TBK_PERSIST_CASE(4, 1)
+    TBK_PERSIST_CASE(4, 2)
+    TBK_PERSIST_CASE(8, 1)
+    TBK_PERSIST_CASE(8, 2)
+    TBK_PERSIST_CASE(8, 4)
+    TBK_PERSIST_CASE(16, 1)
+    TBK_PERSIST_CASE(16, 2)
+    TBK_PERSIST_CASE(16, 4)
+    TBK_PERSIST_CASE(32, 1)
+    TBK_PERSIST_CASE(32, 2)
+    TBK_PERSIST_CASE(32, 4)
+    TBK_PERSIST_CASE(64, 1)
+    TBK_PERSIST_CASE(64, 2)
+    TBK_PERSIST_CASE(128, 2)
+    TBK_PERSIST_CASE(128, 4)
+    TBK_PERSIST_CASE(256, 4)
+#undef TBK_PERSIST_CASE
+    return false;
+}
+
 // Launch one step; returns false if no kernel exists for the combination.
 bool launch_f64_real(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a);
 bool launch_f64_cplx(int width, int kch, int cpl, int mode, dim3 grid, hipStream_t st, const StepArgs& a);

@@ -8,4 +8,7 @@ bool launch_f32_real(int width, int kch, int cpl, int mode, dim3 grid, hipStream
 bool launch_slots_f32_real(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a) {
     return launch_slots<float, false>(Rp, cpl, grid, st, a);
 }
+bool launch_persist_f32_real(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSched& s) {
+    return launch_persist<float, false>(Rp, cpl, grid, st, s);
+}
 }  // namespace tbk

@@ -8,4 +8,7 @@ bool launch_f64_cplx(int width, int kch, int cpl, int mode, dim3 grid, hipStream
 bool launch_slots_f64_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const SlotArgs& a) {
     return launch_slots<double, true>(Rp, cpl, grid, st, a);
 }
+bool launch_persist_f64_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSched& s) {
+    return launch_persist<double, true>(Rp, cpl, grid, st, s);
+}
 }  // namespace tbk

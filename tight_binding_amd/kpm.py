@@ -480,11 +480,14 @@ class Batch:
         self.ctx, self._h, self.R, self.precision = ctx, h, int(R), int(precision)
         self.offsets = np.concatenate([[0], np.cumsum(self.n_moments)])
 
-    def run(self, graphs: bool = False):
-        """Returns (list of moment arrays in problem order, stats)."""
+    def run(self, graphs: bool = False, persistent: bool = False):
+        """Returns (list of moment arrays in problem order, stats).  graphs:
+        capture each chunk of 64 steps in a hipGraph; persistent: one
+        persistent launch per chunk (problems on XCD queues)."""
         mu = np.zeros(int(self.offsets[-1]), dtype=np.float64)
         st = N.tbk_batch_stats()
-        N.check_kpm(N.kpm().tbk_batch_run(self._h, N.TBK_BATCH_GRAPHS if graphs else 0, _dp(mu), C.byref(st)))
+        flags = (N.TBK_BATCH_GRAPHS if graphs else 0) | (N.TBK_BATCH_PERSISTENT if persistent else 0)
+        N.check_kpm(N.kpm().tbk_batch_run(self._h, flags, _dp(mu), C.byref(st)))
         return [mu[self.offsets[i]:self.offsets[i + 1]] for i in range(self.n_moments.size)], st.as_dict()
 
     def set_shape(self, cpl=1, blocks_per_cu=4):
