@@ -75,7 +75,7 @@ struct tbk_batch {
     struct Persist {
         bool built = false;
         int32_t nchunks = 0;
-        int64_t grid = 0;
+        int64_t grid = 0, chunk = 64;
         std::vector<int64_t> step_base;  // per chunk: offset of its per-queue step prefixes
         DevBuf nshare, pbase, q_slots, q_off, q_share_pre, q_step_items, totals, counters, done, error, part;
         std::vector<hipGraphExec_t> graphs;
@@ -197,11 +197,11 @@ struct tbk_batch {
                                     ps.q_step_items.bytes);
     }
 
-    // The persistent plan: problems placed on the 8 XCD queues greedily by
-    // work (rows x steps, largest first), each queue's problems in layout
-    // order (decreasing steps, so the problems active at a step are a
-    // prefix); shares of kPersistShareRows rows; per chunk of kChunkSteps
-    // steps and queue, the items before each step.
+    // The persistent plan: problems placed on the 8 XCD queues (below), each
+    // queue's problems in layout order (decreasing steps, so the problems
+    // active at a step are a prefix); ceil(rows / kPersistShareRows) shares
+    // each; per chunk of kChunkSteps steps and queue, the items before each
+    // step.
     void build_persist(hipStream_t st) {
         constexpr int nq = kPersistQueues;
         std::vector<int32_t> nsh(static_cast<size_t>(np));
@@ -212,16 +212,22 @@ struct tbk_batch {
             pb[i] = acc;
             acc += steps[i] * nsh[i];
         }
+        // placement: problems in decreasing step count (larger rows first
+        // among equal counts), each to the queue holding the fewest rows so
+        // far.  The problems active at a step are the ones placed before
+        // some point of this order, so every step's rows are balanced over
+        // the XCDs, not only the total work.
         std::vector<int32_t> byw(static_cast<size_t>(np));
         std::iota(byw.begin(), byw.end(), 0);
-        std::stable_sort(byw.begin(), byw.end(),
-                         [&](int32_t x, int32_t y) { return rows[x] * steps[x] > rows[y] * steps[y]; });
+        std::stable_sort(byw.begin(), byw.end(), [&](int32_t x, int32_t y) {
+            return steps[x] != steps[y] ? steps[x] > steps[y] : rows[x] > rows[y];
+        });
         std::vector<std::vector<int32_t>> xq(nq);
         std::vector<double> load(nq, 0.0);
         for (int32_t p : byw) {
             const int q = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
             xq[q].push_back(p);
-            load[q] += static_cast<double>(rows[p]) * static_cast<double>(steps[p]);
+            load[q] += static_cast<double>(rows[p]);
         }
         std::vector<int32_t> qs, qo(nq + 1, 0);
         std::vector<int64_t> pre;
@@ -237,11 +243,13 @@ struct tbk_batch {
             }
         }
         qo[nq] = static_cast<int32_t>(qs.size());
-        const int32_t nch = static_cast<int32_t>((max_steps + kChunkSteps - 1) / kChunkSteps);
+        const int64_t pch = kChunkSteps;
+        ps.chunk = pch;
+        const int32_t nch = static_cast<int32_t>((max_steps + pch - 1) / pch);
         std::vector<int64_t> items, tot;
         ps.step_base.assign(static_cast<size_t>(nch), 0);
         for (int32_t c = 0; c < nch; ++c) {
-            const int64_t k0 = static_cast<int64_t>(c) * kChunkSteps, k1 = std::min(max_steps, k0 + kChunkSteps);
+            const int64_t k0 = static_cast<int64_t>(c) * pch, k1 = std::min(max_steps, k0 + pch);
             ps.step_base[c] = static_cast<int64_t>(items.size());
             for (int q = 0; q < nq; ++q) {
                 const int32_t n_q = qo[q + 1] - qo[q];
@@ -280,7 +288,7 @@ struct tbk_batch {
     // chunk c: resets, the persistent launch, the check that every queue was served
     void launch_persist_chunk(int32_t c, hipStream_t st) {
         constexpr int nq = kPersistQueues;
-        const int64_t k0 = static_cast<int64_t>(c) * kChunkSteps, k1 = std::min(max_steps, k0 + kChunkSteps);
+        const int64_t k0 = static_cast<int64_t>(c) * ps.chunk, k1 = std::min(max_steps, k0 + ps.chunk);
         hipLaunchKernelGGL(tbk::persist_reset_kernel, dim3(static_cast<unsigned>((np + tbk::kBlock - 1) / tbk::kBlock)),
                            dim3(tbk::kBlock), 0, st, ps.done.as<int32_t>(), np, ps.counters.as<int32_t>(), nq,
                            c == 0 ? ps.error.as<int32_t>() : nullptr);
