@@ -43,6 +43,7 @@
 #include <thread>
 #include <vector>
 
+#include "admission.h"
 #include "aux_kernels.h"
 #include "engine.h"
 #include "gamma_launch.h"
@@ -136,9 +137,12 @@ struct HostLattice {
     }
 };
 
-struct Slab;
+using serve::Arena;
+using serve::Slab;
 
-struct Request {
+// A submitted request; serve::Ticket holds its footprint, submission time
+// and arena placement.
+struct Request : serve::Ticket {
     int64_t ticket = 0;
     int32_t kind = TBK_REQ_DOS, precision = TBK_FP64, n = 0, R = 0, Rp = 0;
     uint64_t seed = 0;
@@ -149,29 +153,17 @@ struct Request {
     std::vector<double> g;
     int32_t Gw = 0;            // sigma_xy: random vectors per group (a power of two <= Rp; see kubo_footprint)
     int32_t row_chunks = 1;    // sigma_xy, fp64 stacks: row ranges the stacks hold one at a time
-    int64_t footprint = 0;
     std::atomic<bool> cancel{false};  // tbk_server_cancel on a running request
-    Slab* slab = nullptr;      // its span of the arena while admitted (null: a new slab is being made for it)
-    size_t arena_off = 0;
-    size_t new_slab = 0;       // bytes of that new slab
     int32_t state = TBK_REQ_QUEUED;
     int32_t error_code = 0;
     std::string error;
-    double t_submit = 0, t_admit = 0, t_done = 0;
+    double t_admit = 0, t_done = 0;
     std::vector<double> result;
 };
 
 }  // namespace
 struct tbk_server;
 namespace {
-
-// One slab of the arena: a device allocation carved into spans.
-struct Slab {
-    char* base = nullptr;
-    size_t cap = 0;
-    std::map<size_t, size_t> holes;  // offset -> bytes
-    int32_t live = 0;                // spans handed out and not returned
-};
 
 // The arena span a request's device buffers are carved from; handed back to
 // the server's arena when the slot or job holding it is destroyed, after
@@ -187,95 +179,18 @@ struct Lease {
     ~Lease();
 };
 
-// The server's device memory: the whole budget as one slab allocated at
-// creation, or (lazy_arena) slabs allocated as requests need them (the
-// first of 4 GiB, each later one at least as large as all before it, all
-// within the budget) and kept, so a server that has run a request of some
-// size holds the memory for the next one.  A request gets one span of one
-// slab: the smallest hole that fits, in address order.  Slabs with no span
-// handed out are freed only to make room for a larger slab.  Under the
-// server's mutex; the device allocations themselves happen outside it.
-struct Arena {
-    static constexpr size_t kGrain = 64 * 1024;
-    static constexpr size_t kFirstSlab = size_t{4} << 30;
-    std::vector<std::unique_ptr<Slab>> slabs;
-    size_t held = 0;     // bytes of the slabs
-    size_t pending = 0;  // bytes of slabs being allocated
-    static size_t round(size_t n) { return (n + kGrain - 1) / kGrain * kGrain; }
-    static void cut(Slab& sl, std::map<size_t, size_t>::iterator it, size_t n, size_t& off) {
-        off = it->first;
-        const size_t rest = it->second - n;
-        sl.holes.erase(it);
-        if (rest) sl.holes[off + n] = rest;
-        sl.live++;
-    }
-    // best fit over the holes of every slab
-    bool take(size_t n, Slab*& where, size_t& off) {
-        n = round(n);
-        Slab* best = nullptr;
-        std::map<size_t, size_t>::iterator bit;
-        for (auto& sl : slabs)
-            for (auto it = sl->holes.begin(); it != sl->holes.end(); ++it)
-                if (it->second >= n && (!best || it->second < bit->second)) {
-                    best = sl.get();
-                    bit = it;
-                }
-        if (!best) return false;
-        cut(*best, bit, n, off);
-        where = best;
-        return true;
-    }
-    // size of a new slab for a span of n bytes within `limit`, retiring idle
-    // slabs into `retired` when that makes the room; 0: none fits now
-    size_t plan_slab(size_t n, size_t limit, std::vector<Slab*>& retired) {
-        n = round(n);
-        auto size_for = [&](size_t room) -> size_t {
-            if (room < n) return 0;
-            const size_t want = std::max({n, kFirstSlab, held + pending});
-            return std::min(room, want) / kGrain * kGrain;
-        };
-        const size_t room = limit > held + pending ? limit - held - pending : 0;
-        if (size_t sz = size_for(room)) return sz;
-        size_t idle = 0;
-        for (auto& sl : slabs)
-            if (sl->live == 0) idle += sl->cap;
-        if (room + idle < n) return 0;
-        // retire idle slabs, smallest first, until the span fits
-        std::vector<Slab*> cand;
-        for (auto& sl : slabs)
-            if (sl->live == 0) cand.push_back(sl.get());
-        std::sort(cand.begin(), cand.end(), [](const Slab* a, const Slab* b) { return a->cap < b->cap; });
-        size_t freed = 0;
-        for (Slab* c : cand) {
-            if (room + freed >= n) break;
-            freed += c->cap;
-            retired.push_back(c);
+// Slabs of the arena from the device: a refused hipMalloc leaves no error
+// behind for the next HIP call.
+struct HipSlabMemory : serve::SlabMemory {
+    char* allocate(size_t bytes) override {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
         }
-        for (Slab* c : retired) {
-            held -= c->cap;
-            for (auto& sl : slabs)
-                if (sl.get() == c) sl.release();
-        }
-        slabs.erase(std::remove(slabs.begin(), slabs.end(), nullptr), slabs.end());
-        return size_for(room + freed);
+        return static_cast<char*>(p);
     }
-    void give(Slab* sl, size_t off, size_t n) {
-        n = round(n);
-        sl->live--;
-        auto next = sl->holes.lower_bound(off);
-        if (next != sl->holes.end() && off + n == next->first) {
-            n += next->second;
-            next = sl->holes.erase(next);
-        }
-        if (next != sl->holes.begin()) {
-            auto prev = std::prev(next);
-            if (prev->first + prev->second == off) {
-                prev->second += n;
-                return;
-            }
-        }
-        sl->holes[off] = n;
-    }
+    void release(char* base) override { (void)hipFree(base); }
 };
 
 // bytes lost to the 256-byte alignment of the buffers carved from a span
@@ -340,9 +255,8 @@ struct tbk_server {
     hipStream_t kstream = nullptr;
     hipStream_t bstream = nullptr;  // spectral-range Lanczos of auto-bounds requests (prep thread)
     tbk_server_config cfg{};
-    int64_t budget = 0;
-    Arena arena;                     // the budget's device memory (under mu)
-    std::vector<Slab*> retired_slabs;  // idle slabs to free (by the preparation thread)
+    serve::Admission adm;            // budget, bytes in use, the arena and its retired slabs (under mu)
+    HipSlabMemory slab_memory;
 
     // shared state, under mu
     std::mutex mu;
@@ -355,7 +269,6 @@ struct tbk_server {
     int64_t prep_gen = 0;                // bumped by every change the preparation thread acts on
     std::vector<int32_t> free_pin, free_kpin;
     int64_t next_ticket = 1;
-    int64_t used = 0;
     int32_t preparing = 0;               // admitted requests still being prepared
     bool stop = false, worker_done = false;
     std::string fatal;
@@ -475,14 +388,14 @@ void tbk_server::size_kubo(Request& r) const {
     r.Gw = r.Rp;
     r.row_chunks = 1;
     r.footprint = kubo_footprint(r);
-    while (r.footprint > budget && r.Gw > 1) {
+    while (r.footprint > adm.budget && r.Gw > 1) {
         r.Gw /= 2;
         r.footprint = kubo_footprint(r);
     }
     // fp64 stacks of one vector still too large: hold them a row range at a
     // time (the chains then run once per range)
     const bool ps = kubo_presplit(r.precision, r.n, r.H.hdr.is_complex != 0, r.vx.hdr.is_complex != 0);
-    while (r.footprint > budget && !ps && r.row_chunks < 64) {
+    while (r.footprint > adm.budget && !ps && r.row_chunks < 64) {
         ++r.row_chunks;
         r.footprint = kubo_footprint(r);
     }
@@ -521,7 +434,7 @@ void tbk_server::grow(DevBuf& b, size_t bytes) {
 // a span back to the arena (not under mu); queued requests may now fit
 void tbk_server::give_back(Slab* sl, size_t off, size_t bytes) {
     std::lock_guard<std::mutex> lk(mu);
-    arena.give(sl, off, bytes);
+    adm.give(sl, off, bytes);
     prep_gen++;
     cv_prep.notify_all();
 }
@@ -557,7 +470,7 @@ void tbk_server::finish(Request* r, int32_t code, const std::string& msg) {
     r->state = code == TBK_OK ? TBK_REQ_DONE : TBK_REQ_FAILED;
     r->error_code = code;
     r->error = msg;
-    used -= r->footprint;
+    adm.finished(*r);
     if (code == TBK_OK)
         st.completed++;
     else
@@ -574,84 +487,45 @@ void tbk_server::finish(Request* r, int32_t code, const std::string& msg) {
     cv_prep.notify_all();
 }
 
-// First-fit over the queue in arrival order (under mu); a request that has
-// waited more than max_wait_bypass_ms stops later ones from passing it.
-// Each admitted request takes its pinned result row.
+// First fit over the queue in arrival order (under mu,
+// serve::Admission::pick); a request that has waited more than
+// max_wait_bypass_ms stops later ones from passing it.  Each admitted
+// request takes its pinned result row.
 std::vector<std::pair<Request*, int32_t>> tbk_server::pick_admissions() {
+    std::vector<int32_t> rows;
+    auto pool = [&](const Request* r) -> std::vector<int32_t>& { return r->dos ? free_pin : free_kpin; };
+    const std::vector<Request*> picked =
+        adm.pick(queue, now_s(), [&](Request* r) { return !pool(r).empty(); },
+                 [&](Request* r) {
+                     rows.push_back(pool(r).back());
+                     pool(r).pop_back();
+                 });
     std::vector<std::pair<Request*, int32_t>> out;
     const double t = now_s();
-    for (auto it = queue.begin(); it != queue.end();) {
-        Request* r = *it;
-        const bool is_dos = r->kind == TBK_REQ_DOS;
-        std::vector<int32_t>& pins = is_dos ? free_pin : free_kpin;
-        size_t off = 0, nsl = 0;
-        Slab* sl = nullptr;
-        const bool fits = !pins.empty() && used + r->footprint <= budget &&
-                          (arena.take(static_cast<size_t>(r->footprint), sl, off) ||
-                           (nsl = arena.plan_slab(static_cast<size_t>(r->footprint), static_cast<size_t>(budget),
-                                                  retired_slabs)) > 0);
-        if (fits) {
-            r->slab = sl;
-            r->arena_off = off;
-            r->new_slab = nsl;
-            arena.pending += nsl;
-            used += r->footprint;
-            r->state = TBK_REQ_RUNNING;
-            r->t_admit = t;
-            if (is_dos)
-                st.running_dos++;
-            else
-                st.running_kubo++;
-            out.emplace_back(r, pins.back());
-            pins.pop_back();
-            it = queue.erase(it);
-            continue;
-        }
-        if (1e3 * (t - r->t_submit) > cfg.max_wait_bypass_ms) break;
-        ++it;
+    for (size_t i = 0; i < picked.size(); ++i) {
+        Request* r = picked[i];
+        r->state = TBK_REQ_RUNNING;
+        r->t_admit = t;
+        if (r->dos)
+            st.running_dos++;
+        else
+            st.running_kubo++;
+        out.emplace_back(r, rows[i]);
     }
     preparing += static_cast<int32_t>(out.size());
     st.queued = static_cast<int64_t>(queue.size());
-    st.used_bytes = used;
+    st.used_bytes = adm.used;
     return out;
 }
 
 // Outside mu: free the retired slabs, then allocate the new slab of every
-// admission that needs one and place its span at the slab's start.  An
-// allocation the device refuses fails its request.
-void tbk_server::make_slabs(std::vector<std::pair<Request*, int32_t>>& adm) {
-    std::vector<Slab*> dead;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        dead.swap(retired_slabs);
-    }
-    for (Slab* sl : dead) {
-        (void)hipFree(sl->base);
-        delete sl;
-    }
-    for (auto& pr : adm) {
-        Request* r = pr.first;
-        if (r->slab) continue;
-        auto sl = std::make_unique<Slab>();
-        void* p = nullptr;
-        const hipError_t e = hipMalloc(&p, r->new_slab);
-        std::lock_guard<std::mutex> lk(mu);
-        arena.pending -= r->new_slab;
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            r->slab = nullptr;
-            continue;  // start_* fails it
-        }
-        sl->base = static_cast<char*>(p);
-        sl->cap = r->new_slab;
-        sl->holes[0] = sl->cap;
-        size_t off = 0;
-        Arena::cut(*sl, sl->holes.begin(), Arena::round(static_cast<size_t>(r->footprint)), off);
-        r->slab = sl.get();
-        r->arena_off = off;
-        arena.held += sl->cap;
-        arena.slabs.push_back(std::move(sl));
-    }
+// admission that needs one and place its span at the slab's start
+// (serve::Admission::place).  An allocation the device refuses fails its
+// request (lease_to).
+void tbk_server::make_slabs(std::vector<std::pair<Request*, int32_t>>& picked) {
+    std::vector<Request*> reqs_;
+    for (auto& pr : picked) reqs_.push_back(pr.first);
+    adm.place(reqs_, slab_memory, mu);
 }
 
 Slot* tbk_server::start_dos(Request* r, int32_t pin) {
@@ -1207,23 +1081,14 @@ int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server**
         HIPC(hipStreamCreateWithFlags(&s->bstream, hipStreamNonBlocking));
         size_t free_b = 0, total_b = 0;
         HIPC(hipMemGetInfo(&free_b, &total_b));
-        s->budget = c.hbm_budget_bytes > 0 ? c.hbm_budget_bytes : static_cast<int64_t>(0.85 * static_cast<double>(free_b));
+        s->adm.budget =
+            c.hbm_budget_bytes > 0 ? c.hbm_budget_bytes : static_cast<int64_t>(0.85 * static_cast<double>(free_b));
+        s->adm.max_wait_bypass_ms = c.max_wait_bypass_ms;
         if (c.lazy_arena != 0 && c.lazy_arena != 1) raise(TBK_E_INVALID, "lazy_arena must be 0 or 1");
-        if (!c.lazy_arena) {
-            // the whole budget as one slab, now
-            auto sl = std::make_unique<Slab>();
-            sl->cap = static_cast<size_t>(s->budget) / Arena::kGrain * Arena::kGrain;
-            void* ap = nullptr;
-            if (hipMalloc(&ap, sl->cap) != hipSuccess) {
-                (void)hipGetLastError();
-                raise(TBK_E_OOM, "could not allocate the server's budget of " + std::to_string(s->budget / 1e9) +
-                                     " GB (" + std::to_string(static_cast<double>(free_b) / 1e9) + " GB free)");
-            }
-            sl->base = static_cast<char*>(ap);
-            sl->holes[0] = sl->cap;
-            s->arena.held = sl->cap;
-            s->arena.slabs.push_back(std::move(sl));
-        }
+        // the whole budget as one slab, now (else slabs as requests need them)
+        if (!c.lazy_arena && !s->adm.reserve_whole(s->slab_memory))
+            raise(TBK_E_OOM, "could not allocate the server's budget of " + std::to_string(s->adm.budget / 1e9) +
+                                 " GB (" + std::to_string(static_cast<double>(free_b) / 1e9) + " GB free)");
 
         void* p = nullptr;
         HIPC(hipHostMalloc(&p, static_cast<size_t>(c.max_slots) * c.max_moments * sizeof(double), hipHostMallocDefault));
@@ -1234,7 +1099,7 @@ int tbk_server_create(int32_t device, const tbk_server_config* cfg, tbk_server**
         HIPC(hipHostMalloc(&p, static_cast<size_t>(c.max_kubo) * kb * sizeof(double), hipHostMallocDefault));
         s->pin_kubo = static_cast<double*>(p);
         for (int32_t i = c.max_kubo - 1; i >= 0; --i) s->free_kpin.push_back(i);
-        s->st.budget_bytes = s->budget;
+        s->st.budget_bytes = s->adm.budget;
         tbk_server* raw = s.get();
         s->worker = std::thread([raw] { raw->run(); });
         s->prep = std::thread([raw] { raw->run_prep(); });
@@ -1252,6 +1117,7 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
         if (q->a == 0.0 && q->kind != TBK_REQ_DOS) raise(TBK_E_INVALID, "sigma_xy requests need a, b");
         auto r = std::make_unique<Request>();
         r->kind = q->kind;
+        r->dos = q->kind == TBK_REQ_DOS;
         r->precision = q->precision;
         r->n = q->n_moments;
         r->R = q->R;
@@ -1316,7 +1182,8 @@ int tbk_server_submit(tbk_server* s, const tbk_request* q, int64_t* ticket) {
         } else {
             r->footprint = s->dos_footprint(*r);
         }
-        if (r->footprint > s->budget) raise(TBK_E_OOM, "request needs more device memory than the server budget");
+        if (!s->adm.admissible(r->footprint))
+            raise(TBK_E_OOM, "request needs more device memory than the server budget");
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop) raise(TBK_E_INVALID, "server is shutting down");
         if (!s->fatal.empty()) raise(TBK_E_DEVICE, s->fatal);
@@ -1418,9 +1285,9 @@ int tbk_server_get_stats(tbk_server* s, tbk_server_stats* out) {
         if (!s || !out) raise(TBK_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(s->mu);
         *out = s->st;
-        out->arena_bytes = static_cast<int64_t>(s->arena.held);
-        out->arena_slabs = static_cast<int64_t>(s->arena.slabs.size());
-        out->used_bytes = s->used;
+        out->arena_bytes = static_cast<int64_t>(s->adm.arena.held);
+        out->arena_slabs = static_cast<int64_t>(s->adm.arena.slabs.size());
+        out->used_bytes = s->adm.used;
         out->queued = static_cast<int64_t>(s->queue.size());
     });
 }
@@ -1458,11 +1325,7 @@ int tbk_server_destroy(tbk_server* s) {
         for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
         if (s->pin_mu) (void)hipHostFree(s->pin_mu);
         if (s->pin_kubo) (void)hipHostFree(s->pin_kubo);
-        for (auto& sl : s->arena.slabs) (void)hipFree(sl->base);
-        for (Slab* sl : s->retired_slabs) {
-            (void)hipFree(sl->base);
-            delete sl;
-        }
+        s->adm.release_all(s->slab_memory);
         if (s->kstream) (void)hipStreamDestroy(s->kstream);
         if (s->bstream) (void)hipStreamDestroy(s->bstream);
         tbk_context* ctx = s->ctx;
