@@ -613,10 +613,11 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_big_kernel(const T* __restri
 // hi.lo, lo.hi), turns those B fragments into B' = (im, -re) in place and
 // makes the 24 Im products: the same sums as the 128 x 128 kernel (the same
 // fragments in the same k order, the same phases per accumulator), so Gamma
-// is the same bits.  LDS: two stage
+// is the same bits.  LDS: three stage
 // buffers of a 256-row A tile and a 128-row B tile (rows of 128 bytes,
-// chunk c of row r at c ^ ((r / 2) % 8)), 96 KB; a full stage goes global ->
-// LDS by the load unit, a short last stage through registers, written at once.
+// chunk c of row r at c ^ wide_swizzle(r)), 144 KB; a full stage goes global
+// -> LDS by the load unit two stages ahead of its products, a short last
+// stage through registers, written at once.
 constexpr int kWideStageBytes = (kWideTile + kBigTile) * kRawRowBytes;  // 48 KB
 
 // Position of a row's 16-byte chunks: chunk c of row r sits at c ^ wide_swizzle(r).
@@ -625,12 +626,32 @@ constexpr int kWideStageBytes = (kWideTile + kBigTile) * kRawRowBytes;  // 48 KB
 // get distinct positions, and rows r, r + 8 of one 16-row read differ too.
 __device__ __forceinline__ int wide_swizzle(int r) { return (r & 7) ^ ((r >> 3) & 1); }
 
+// End of a stage of the three-buffer pipeline: this wave's copies of the
+// next stage have landed, its LDS accesses are done, then the workgroup
+// barrier.  Copies complete in issue order, so with later_in_flight the 6
+// copies of the stage after the next one (issued last) may stay in flight.
+// The empty asm statements keep the compiler from moving memory accesses
+// across the barrier.  The waits are s_waitcnt builtins, not asm text, so
+// the compiler's own wait placement knows which copies have landed and adds
+// no wait before reading a landed buffer.  gfx9 encoding: vmcnt in bits 3:0
+// (and 15:14), expcnt in 6:4 (7: no wait), lgkmcnt in 11:8.
+__device__ __forceinline__ void wide_stage_barrier(bool later_in_flight) {
+    asm volatile("" ::: "memory");
+    if (later_in_flight)
+        __builtin_amdgcn_s_waitcnt(0x0076);  // vmcnt(6) lgkmcnt(0)
+    else
+        __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 __global__ __launch_bounds__(512) void gamma_bf16x3_wide_kernel(const uint2* __restrict__ A,
                                                                 const uint2* __restrict__ B, int N, int64_t K,
                                                                 int64_t wbs, int64_t kc, GammaMap map,
                                                                 double2* __restrict__ P) {
     __shared__ __align__(1024) uint8_t S0[kWideStageBytes];
     __shared__ __align__(1024) uint8_t S1[kWideStageBytes];
+    __shared__ __align__(1024) uint8_t S2[kWideStageBytes];
     int tn, tm;
     int64_t c;
     if (!xcd_work_item(map, tn, tm, c)) return;  // uniform over the workgroup
@@ -661,22 +682,27 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_wide_kernel(const uint2* __r
     }
     // a short last stage goes through registers and is written at once (no
     // registers held across the products)
-    auto fetch = [&](int64_t s, uint8_t* buf) {
-        const int64_t k0 = kb + s * kSplitKt;
-        const bool full = k0 + kSplitKt <= ke;
-        const int64_t o = word_offset(k0, wbs);
-        if (full) {
-            const uint2* pa = A + o;
-            const uint2* pb = B + o;
+    // stage counts are small (a chunk is at most a few thousand stages), so
+    // the stage tests are 32-bit scalar compares
+    const int nfull = static_cast<int>((ke - kb) / kSplitKt);
+    auto fetch_full = [&](int s, uint8_t* buf) {
+        const int64_t o = word_offset(kb + static_cast<int64_t>(s) * kSplitKt, wbs);
+        const uint2* pa = A + o;
+        const uint2* pb = B + o;
 #pragma unroll
-            for (int i = 0; i < 6; ++i) {
+        for (int i = 0; i < 6; ++i) {
 #if defined(__HIP_DEVICE_COMPILE__)
-                __builtin_amdgcn_global_load_lds((i >= 4 ? pb : pa) + roff[i],
-                                                 (__attribute__((address_space(3))) void*)(buf + dst_off[i]), 16, 0,
-                                                 0);
+            __builtin_amdgcn_global_load_lds((i >= 4 ? pb : pa) + roff[i],
+                                             (__attribute__((address_space(3))) void*)(buf + dst_off[i]), 16, 0, 0);
 #endif
-            }
+        }
+    };
+    auto fetch = [&](int s, uint8_t* buf) {
+        if (s < nfull) {
+            fetch_full(s, buf);
         } else {
+            const int64_t k0 = kb + static_cast<int64_t>(s) * kSplitKt;
+            const int64_t o = word_offset(k0, wbs);
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
                 const uint2* src = (i >= 4 ? B : A) + o + roff[i];
@@ -744,31 +770,52 @@ __global__ __launch_bounds__(512) void gamma_bf16x3_wide_kernel(const uint2* __r
                                                                        ph == 1 ? bl[jj] : bh[jj], im[i][j], 0, 0, 0);
                 }
     };
-    auto step = [&](int64_t st, int64_t stages, const uint8_t* cur, uint8_t* nxt) {
-        v8bf ah[4], al[4];
-        const uint8_t* At = cur;
-        const uint8_t* Bt = cur + kWideTile * kRawRowBytes;
+    const int stages = static_cast<int>((ke - kb + kSplitKt - 1) / kSplitKt);
+    {
+        // Copies two stages ahead: stage st + 2 goes into the buffer that
+        // stage st - 1 was read from (freed by the barrier that ended stage
+        // st - 1), so a stage's copies have a whole stage of products to land
+        // in; with two buffers they had only the products of the stage that
+        // issued them (38.6 against 37.7 ms at N = 512, K = 2^23).  A full
+        // stage is 6 copies a lane; a short one is written through registers
+        // before its fetch returns.
+        // Steps whose stage st + 2 is full (main loop) end with vmcnt(6):
+        // every path through them issues the 6 copies, so the compiler's wait
+        // placement can tell which buffer is still being filled.  The last
+        // few steps (a short or missing stage ahead) wait for everything.
+        auto step3 = [&](int st, const uint8_t* cur, uint8_t* nxt2, bool tail) {
+            v8bf ah[4], al[4];
+            const uint8_t* At = cur;
+            const uint8_t* Bt = cur + kWideTile * kRawRowBytes;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            uint4 x0, x1;
-            rd(At, wn * 64 + 16 * i + li, x0, x1);
-            ah[i] = as_v8bf(make_uint4(x0.x, x0.z, x1.x, x1.z));
-            al[i] = as_v8bf(make_uint4(x0.y, x0.w, x1.y, x1.w));
+            for (int i = 0; i < 4; ++i) {
+                uint4 x0, x1;
+                rd(At, wn * 64 + 16 * i + li, x0, x1);
+                ah[i] = as_v8bf(make_uint4(x0.x, x0.z, x1.x, x1.z));
+                al[i] = as_v8bf(make_uint4(x0.y, x0.w, x1.y, x1.w));
+            }
+            if (!tail)
+                fetch_full(st + 2, nxt2);
+            else if (st + 2 < stages)
+                fetch(st + 2, nxt2);
+            half(Bt, 0, ah, al);
+            half(Bt, 1, ah, al);
+            wide_stage_barrier(!tail);
+        };
+        fetch(0, S0);
+        if (stages > 1) fetch(1, S1);
+        wide_stage_barrier(false);
+        int st = 0;
+        for (; st + 4 < nfull; st += 3) {
+            step3(st, S0, S2, false);
+            step3(st + 1, S1, S0, false);
+            step3(st + 2, S2, S1, false);
         }
-        const bool ahead = st + 1 < stages;
-        if (ahead) fetch(st + 1, nxt);
-        half(Bt, 0, ah, al);
-        half(Bt, 1, ah, al);
-        __syncthreads();
-    };
-
-    const int64_t stages = (ke - kb + kSplitKt - 1) / kSplitKt;
-    fetch(0, S0);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    for (int64_t st = 0; st < stages; st += 2) {
-        step(st, stages, S0, S1);
-        if (st + 1 < stages) step(st + 1, stages, S1, S0);
+        for (; st < stages; st += 3) {
+            step3(st, S0, S2, true);
+            if (st + 1 < stages) step3(st + 1, S1, S0, true);
+            if (st + 2 < stages) step3(st + 2, S2, S1, true);
+        }
     }
 
     double2* Pc = P + c * static_cast<int64_t>(N) * N;
