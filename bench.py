@@ -33,28 +33,33 @@ HBM_PEAK_GBS = 8000.0
 # DRAM bytes per step of the default workload (1024 x 1024 cells, R = 64)
 # from rocprofv3 --pmc: TCC_EA0_RDREQ_DRAM_32B and TCC_EA0_WRREQ_WRITE_DRAM_32B
 # count 32-byte sectors and read 1.000x the traffic of stream kernels of known
-# size.  The figure holds for the launch shape of the profiled run only (other
+# size.  A figure holds for the launch shape it was profiled at only (other
 # shapes read 4.4-7.3 GB per step), so it is reported only when the tuner
-# installs that shape: (bytes, profiled shape, profile summary).
+# installs one of the profiled shapes.  The tuner's two usual choices differ
+# in xcd_split only (row ranges dealt per XCD, or not); both were profiled
+# in one run: [(bytes, shape, profile summary)].
+_C2_SHAPE = {"blocks_per_cu": (0, 4), "chunk_rows": (0, 256), "chunk_period": (0,), "stream_nt": (0, 1),
+             "use_tiled": (0,), "row_lane_bytes": (0, 16), "row_slab_cols": (0,), "slab_cols": (0,)}
 STEP_TRAFFIC = {
-    (1024, 64): (7.5821e9,
-                 {"blocks_per_cu": (0, 4), "chunk_rows": (0, 256), "chunk_period": (0,), "xcd_split": (0,),
-                  "stream_nt": (0, 1), "use_tiled": (0,), "row_lane_bytes": (0, 16), "row_slab_cols": (0,)},
-                 "profiles/cheb_step_fp64_honeycomb2M_R64_pmc_counters.md"),
+    (1024, 64): [
+        (7.794e9, dict(_C2_SHAPE, xcd_split=(0, 1)), "profiles/cheb_step_fp64_honeycomb2M_R64_pmc_counters.md"),
+        (8.016e9, dict(_C2_SHAPE, xcd_split=(-1,)), "profiles/cheb_step_fp64_honeycomb2M_R64_pmc_counters.md"),
+    ],
 }
 
 
 def step_traffic(cells, R, tuning):
-    """(bytes, source) of the profiled DRAM traffic when `tuning` is the
-    profiled launch shape, else (None, reason)."""
-    entry = STEP_TRAFFIC.get((cells, R))
-    if entry is None:
+    """(bytes, source) of the profiled DRAM traffic when `tuning` is one of
+    the profiled launch shapes, else (None, reason)."""
+    entries = STEP_TRAFFIC.get((cells, R))
+    if not entries:
         return None, None
-    nbytes, shape, source = entry
     t = tuning or {}
-    if all(t.get(k, 0) in allowed for k, allowed in shape.items()):
-        return nbytes, "rocprofv3 --pmc DRAM sector counters of this workload at this launch shape (%s)" % source
-    return None, "not profiled at this launch shape (%s holds the default shape)" % source
+    for nbytes, shape, source in entries:
+        if all(t.get(k, 0) in allowed for k, allowed in shape.items()):
+            return nbytes, ("rocprofv3 --pmc DRAM sector counters of this workload at this launch shape (%s)"
+                            % source)
+    return None, "not profiled at this launch shape (%s holds the profiled shapes)" % entries[0][2]
 
 
 def log(msg):
@@ -697,6 +702,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=3, help="time this many plain block copies")
     ap.add_argument("--no-extra", action="store_true", help="skip the fp32 and one-moment-per-step variants")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch shape")
+    ap.add_argument("--tuning", default=None,
+                    help="C2 leg: install this launch shape (JSON of tbk_tuning fields) instead of autotuning")
     ap.add_argument("--kubo-repeats", type=int, default=3, help="sigma_xy solves per precision (0: skip)")
     ap.add_argument("--kubo-cells", type=int, default=512)
     ap.add_argument("--kubo-N", type=int, default=512)
@@ -765,7 +772,11 @@ def main():
 
     # launch-shape autotuning for this matrix and block (outside the timed region)
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        tuning = dict(ctx.get_tuning(), **json.loads(args.tuning))
+        ctx.set_tuning(**tuning)
+        log("tuning installed: %s" % json.dumps(tuning))
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tuning, tms = ctx.autotune(M, K, a=a, steps=16)
         log("autotune: %.3f ms/step with %s (%.1f s)" % (tms, json.dumps(tuning), time.perf_counter() - t0))
