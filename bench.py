@@ -229,6 +229,15 @@ def batch_leg(ctx, nprob, R=8):
     log("batch persistent: kernel %.1f ms, wall %.1f ms, %.1f GB/s, %d launches of %d workgroups, "
         "max |dmu|/max|mu| vs per-step launches %.2e" % (stp["kernel_ms"], stp["wall_ms"], gbps_p, stp["launches"],
                                                          stp["workgroups"], dmu_p))
+    # one grid-synchronised persistent launch per chunk (TBK_BATCH_SYNCED)
+    mus_y, _ = B.run(synced=True)
+    yruns = [B.run(graphs=True, synced=True)[1] for _ in range(2)]
+    sty = min(yruns, key=lambda d: d["kernel_ms"])
+    gbps_y = sty["bytes_total"] / (sty["kernel_ms"] * 1e-3) / 1e9
+    dmu_y = max(float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(mus_y, mus))
+    log("batch synced: kernel %.1f ms, wall %.1f ms, %.1f GB/s, %d launches of %d workgroups, "
+        "max |dmu|/max|mu| vs per-step launches %.2e" % (sty["kernel_ms"], sty["wall_ms"], gbps_y, sty["launches"],
+                                                         sty["workgroups"], dmu_y))
     B.close()
 
     # plain per-solve path: one moments call per problem, matrices resident
@@ -265,6 +274,9 @@ def batch_leg(ctx, nprob, R=8):
                        "frac_hbm": gbps_p / HBM_PEAK_GBS, "launches": stp["launches"],
                        "workgroups": stp["workgroups"], "moments_per_s": moments / (stp["wall_ms"] * 1e-3),
                        "max_rel_dmu_vs_per_step_launches": dmu_p},
+        "synced": {"kernel_ms": sty["kernel_ms"], "wall_ms": sty["wall_ms"], "GBps": gbps_y,
+                   "frac_hbm": gbps_y / HBM_PEAK_GBS, "launches": sty["launches"], "workgroups": sty["workgroups"],
+                   "moments_per_s": moments / (sty["wall_ms"] * 1e-3), "max_rel_dmu_vs_per_step_launches": dmu_y},
         "per_solve_kernel_ms": k_ms, "per_solve_wall_ms": 1e3 * w_s,
         "per_solve_moments_per_s": moments / w_s,
         "speedup_wall": (1e3 * w_s) / st["wall_ms"],

@@ -500,7 +500,16 @@ enum {
      * per-step launches' only in the grouping of the partial sums.  A wait
      * that gives up, or a queue left unserved, fails the run with
      * TBK_E_DEVICE. */
-    TBK_BATCH_PERSISTENT = 2
+    TBK_BATCH_PERSISTENT = 2,
+    /* each chunk of 64 steps as one persistent launch over the whole GPU: the
+     * launch's workgroups (all resident) do the work of the per-step launches
+     * step after step, each wave on its own, with a grid-wide barrier
+     * between steps, and form each step's moments after its barrier from one
+     * partial per wave.  The vectors are the per-step launches'; the moments
+     * differ from theirs only in the grouping of the partial sums and do not
+     * depend on the rest of the batch.  A barrier that gives up fails the run
+     * with TBK_E_DEVICE.  Not combined with TBK_BATCH_PERSISTENT. */
+    TBK_BATCH_SYNCED = 4
 };
 
 typedef struct tbk_batch_stats {

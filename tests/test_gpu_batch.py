@@ -224,3 +224,71 @@ def test_persistent_batch_alone_mixed_and_graphs_are_the_same_bits(gpu_ctx, prec
         assert np.array_equal(alone[0], mixed[i]), i
         A.close()
     B.close()
+
+
+@pytest.mark.parametrize("precision", [kpm.FP64, kpm.FP32])
+def test_synced_batch_matches_per_step_launches_alone_mixed_and_graphs(gpu_ctx, precision):
+    """TBK_BATCH_SYNCED: each chunk of 64 steps as one persistent launch over
+    the whole GPU, grid barriers between steps, doing the per-step launches'
+    rows (the same vectors) with one moment partial per wave: moments within
+    1e-12 (fp64) / 1e-6 (fp32) of the per-step launches, (fp64) within 1e-10
+    of the oracle, over step counts that span several chunks (257 moments =
+    129 steps); the same bits eager and over several graph replays with eager
+    runs between them, and alone or in the mix."""
+    from tight_binding_amd import _native as N
+
+    probs = mixed_batch(9, base_seed=8)
+    H, a = model_problem("kagome", 40, 1.3, seed=77)
+    probs.append((H, a, 0.05, 257, 4242))
+    B = make_batch(gpu_ctx, probs, R=5, precision=precision)
+    ref, _ = B.run()
+    y, st = B.run(synced=True)
+    assert st["launches"] == (max((p[3] + 1) // 2 for p in probs) + 63) // 64
+    assert st["workgroups"] % 8 == 0 and st["workgroups"] > 0
+    tol = 1e-12 if precision == kpm.FP64 else 1e-6
+    for x, z in zip(ref, y):
+        assert np.max(np.abs(x - z)) <= tol * np.max(np.abs(x))
+    for _ in range(3):
+        g, stg = B.run(graphs=True, synced=True)
+        assert stg["graphs"] == stg["launches"]
+        e, _ = B.run(synced=True)
+        for r in (g, e):
+            for x, z in zip(y, r):
+                assert np.array_equal(x, z)
+    for i in (0, 4, 9):
+        A = make_batch(gpu_ctx, [probs[i]], R=5, precision=precision)
+        alone, _ = A.run(synced=True)
+        assert np.array_equal(alone[0], y[i]), i
+        A.close()
+    if precision == kpm.FP64:
+        for i in (0, 5, 9):
+            Hh, ah, bh, n, seed = probs[i]
+            K = gpu_ctx.random_block(Hh.n, 5, seed=seed)
+            kets = K.download()
+            Hs = O.rescale(O.CSC(Hh.n, Hh.colptr, Hh.rowidx, Hh.values), ah, bh)
+            want = oracle.cheb_braket(Hs, n, kets, kets)
+            assert np.max(np.abs(y[i] - want)) <= 1e-10 * np.max(np.abs(want)), i
+            K.close()
+    with pytest.raises(N.NativeError) as err:
+        B.run(persistent=True, synced=True)
+    assert err.value.code == N.TBK_E_INVALID
+    B.close()
+
+
+def test_synced_batch_follows_a_new_launch_shape(gpu_ctx):
+    """A synchronised run after the launch shape changed (set_shape: more
+    workgroups per problem) uses a plan rebuilt for the new shape: its
+    moments again match the per-step launches of that shape."""
+    probs = mixed_batch(6, base_seed=21)
+    B = make_batch(gpu_ctx, probs, R=4)
+    B.set_shape(1, 4)
+    B.run(synced=True)
+    for cpl, bpc in ((1, 8), (2, 4)):
+        B.set_shape(cpl, bpc)
+        ref, _ = B.run()
+        y, _ = B.run(synced=True)
+        g, _ = B.run(graphs=True, synced=True)
+        for x, z, w in zip(ref, y, g):
+            assert np.max(np.abs(x - z)) <= 1e-12 * np.max(np.abs(x))
+            assert np.array_equal(z, w)
+    B.close()

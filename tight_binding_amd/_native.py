@@ -189,6 +189,7 @@ class tbk_batch_stats(C.Structure):
 
 TBK_BATCH_GRAPHS = 1
 TBK_BATCH_PERSISTENT = 2
+TBK_BATCH_SYNCED = 4
 
 TBK_REQ_DOS, TBK_REQ_KUBO = 0, 1
 TBK_REQ_QUEUED, TBK_REQ_RUNNING, TBK_REQ_DONE, TBK_REQ_FAILED = 0, 1, 2, 3

@@ -241,36 +241,7 @@ __global__ __launch_bounds__(kBlock) void to_fp32_kernel(const double* __restric
 // 0) or the doubling identities mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
 __global__ __launch_bounds__(kBlock) void slots_moments_kernel(const SlotDesc* __restrict__ slots,
                                                                     int32_t s, const double2* __restrict__ partial) {
-    const SlotDesc& d = slots[blockIdx.x];
-    const int64_t k = d.k0 + s;
-    if (k >= d.steps) return;  // the whole workgroup
-    const double2* p = partial + d.wg0;
-    double a0 = 0.0, a1 = 0.0;
-    for (int i = threadIdx.x; i < d.nwg; i += kBlock) {
-        a0 = a0 + p[i].x;
-        a1 = a1 + p[i].y;
-    }
-    __shared__ double r0[kBlock], r1[kBlock];
-    r0[threadIdx.x] = a0;
-    r1[threadIdx.x] = a1;
-    __syncthreads();
-    for (int w = kBlock / 2; w >= 1; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w) {
-            r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
-            r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double* m = d.mu;
-        if (k == 0) {
-            m[0] = r0[0];
-            m[1] = r1[0];
-        } else {
-            if (2 * k < d.n) m[2 * k] = 2.0 * r0[0] - m[0];
-            if (2 * k + 1 < d.n) m[2 * k + 1] = 2.0 * r1[0] - m[1];
-        }
-    }
+    slot_step_moments(slots, static_cast<int32_t>(blockIdx.x), s, partial);
 }
 
 // Moments of the persistent batched recursion (one workgroup per slot):
@@ -329,7 +300,7 @@ __global__ void persist_check_kernel(const int32_t* __restrict__ counters, const
     if (q < nq && counters[q] < totals[q]) *error = 1;
 }
 
-// out = c * t (init) or out + c * t, elementwise over n complex values, c// out = c * t (init) or out + c * t, elementwise over n complex values, c
+// out = c * t (init) or out + c * t, elementwise over n complex values, c
 // complex; the product is formed as (c.re t.re - c.im t.im, c.re t.im +
 // c.im t.re) in the block's precision.  With c.im = 0 each part is c.re times
 // the part, the scalar-times-matrix product of a real coefficient.

@@ -26,6 +26,12 @@
 // (cheb_kernels.h).  The item partials of every step are kept and
 // slots_history_moments_kernel sums them per problem in share order at the
 // end, so a problem's moments again do not depend on the batch.
+//
+// Synchronised launches.  With TBK_BATCH_SYNCED each chunk of 64 steps is
+// one launch of cheb_slots_sync_kernel over the whole GPU: its resident
+// workgroups do the per-step launches' workgroup indices step after step,
+// one moment partial per wave, a grid barrier between steps, and form each
+// step's moments after its barrier (cheb_kernels.h).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,6 +87,16 @@ struct tbk_batch {
         std::vector<hipGraphExec_t> graphs;
     } ps;
 
+    // Grid-synchronised persistent launches (TBK_BATCH_SYNCED): per step,
+    // the workgroup count of its per-step launch and its active slots; the
+    // partials of two steps; the barrier counter and the error flag.
+    struct Synced {
+        bool built = false;
+        int64_t wmax = 0, grid = 0;
+        DevBuf step_wg, step_na, part, bar, error;
+        std::vector<hipGraphExec_t> graphs;
+    } sy;
+
     void drop_graphs() {
         for (hipGraphExec_t g : graphs)
             if (g) (void)hipGraphExecDestroy(g);
@@ -88,6 +104,9 @@ struct tbk_batch {
         for (hipGraphExec_t g : ps.graphs)
             if (g) (void)hipGraphExecDestroy(g);
         ps.graphs.clear();
+        for (hipGraphExec_t g : sy.graphs)
+            if (g) (void)hipGraphExecDestroy(g);
+        sy.graphs.clear();
     }
     ~tbk_batch() { drop_graphs(); }
 
@@ -194,7 +213,7 @@ struct tbk_batch {
     int64_t device_bytes() const {
         return static_cast<int64_t>(rp.bytes + ci.bytes + val.bytes + mu.bytes + partial.bytes + desc.bytes +
                                     wg_slot.bytes + buf[0].bytes + buf[1].bytes + buf[2].bytes + ps.part.bytes +
-                                    ps.q_step_items.bytes);
+                                    ps.q_step_items.bytes + sy.step_wg.bytes + sy.step_na.bytes + sy.part.bytes);
     }
 
     // The persistent plan: problems placed on the 8 XCD queues (below), each
@@ -328,6 +347,63 @@ struct tbk_batch {
                            ps.totals.as<const int64_t>() + static_cast<int64_t>(c) * nq, nq, ps.error.as<int32_t>());
         HIPC(hipGetLastError());
     }
+
+    // The synchronised plan: per step, the per-step launch's workgroups (the
+    // active prefix) and active slots.
+    void build_synced(hipStream_t st) {
+        std::vector<int64_t> wg(static_cast<size_t>(max_steps));
+        std::vector<int32_t> na(static_cast<size_t>(max_steps));
+        for (int64_t k = 0; k < max_steps; ++k) {
+            na[k] = active(k);
+            wg[k] = na[k] > 0 ? wg_end[na[k] - 1] : 0;
+        }
+        sy.wmax = wg_end.back();
+        sy.step_wg.reserve(wg.size() * sizeof(int64_t));
+        sy.step_na.reserve(na.size() * sizeof(int32_t));
+        sy.part.reserve(static_cast<size_t>(2 * tbk::kWavesPerBlock * sy.wmax) * sizeof(double2));
+        sy.bar.reserve(sizeof(int32_t));
+        sy.error.reserve(sizeof(int32_t));
+        HIPC(hipMemcpyAsync(sy.step_wg.p, wg.data(), wg.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(sy.step_na.p, na.data(), na.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPC(hipStreamSynchronize(st));
+        sy.built = true;
+    }
+
+    // chunk c: zero the barrier counter (and, for the first chunk, the error
+    // flag), then the synchronised launch of the chunk's steps
+    void launch_synced_chunk(int64_t c, hipStream_t st) {
+        if (!sy.built || sy.wmax != wg_end.back())
+            raise(TBK_E_INVALID, "synchronised plan does not match the launch shape");
+        const int64_t k0 = c * kChunkSteps, k1 = std::min(max_steps, k0 + kChunkSteps);
+        hipLaunchKernelGGL(tbk::persist_reset_kernel, dim3(1), dim3(tbk::kBlock), 0, st, sy.bar.as<int32_t>(), 1,
+                           nullptr, 0, c == 0 ? sy.error.as<int32_t>() : nullptr);
+        HIPC(hipGetLastError());
+        tbk::SyncSched S{};
+        S.slots = desc.as<const tbk::SlotDesc>();
+        S.wg_slot = wg_slot.as<const int32_t>();
+        S.step_wg = sy.step_wg.as<const int64_t>() + k0;
+        S.step_na = sy.step_na.as<const int32_t>() + k0;
+        S.partial = sy.part.as<double2>();
+        S.wmax = sy.wmax;
+        S.bar = sy.bar.as<int32_t>();
+        S.error = sy.error.as<int32_t>();
+        S.s0 = static_cast<int32_t>(k0);
+        S.nsteps = static_cast<int32_t>(k1 - k0);
+        S.Rp = Rp;
+        S.opts = ctx->tuning.stream_nt >= 0 ? tbk::kOptStream : 0;
+        S.spin_limit = kPersistSpinLimit;
+        const int cp = cpl_used();
+        int64_t g;
+        if (precision == TBK_FP64)
+            g = cplx ? tbk::launch_sync_f64_cplx(Rp, cp, bpc, ctx->num_cu, st, S)
+                     : tbk::launch_sync_f64_real(Rp, cp, bpc, ctx->num_cu, st, S);
+        else
+            g = cplx ? tbk::launch_sync_f32_cplx(Rp, cp, bpc, ctx->num_cu, st, S)
+                     : tbk::launch_sync_f32_real(Rp, cp, bpc, ctx->num_cu, st, S);
+        if (g == 0) raise(TBK_E_UNSUPPORTED, "no synchronised batched kernel for this width");
+        HIPC(hipGetLastError());
+        sy.grid = g;
+    }
 };
 
 namespace {
@@ -435,11 +511,18 @@ void batch_run(tbk_batch* B, int32_t flags, double* mu_out, tbk_batch_stats* sta
     hipStream_t st = ctx->stream;
     const bool use_graphs = (flags & TBK_BATCH_GRAPHS) != 0;
     const bool persistent = (flags & TBK_BATCH_PERSISTENT) != 0;
-    if (flags & ~(TBK_BATCH_GRAPHS | TBK_BATCH_PERSISTENT)) raise(TBK_E_INVALID, "unknown batch flag");
+    const bool synced = (flags & TBK_BATCH_SYNCED) != 0;
+    if (flags & ~(TBK_BATCH_GRAPHS | TBK_BATCH_PERSISTENT | TBK_BATCH_SYNCED)) raise(TBK_E_INVALID, "unknown batch flag");
+    if (persistent && synced) raise(TBK_E_INVALID, "TBK_BATCH_PERSISTENT and TBK_BATCH_SYNCED exclude each other");
     ensure_events(ctx, 2);
     if (persistent && !B->ps.built) {
         const int64_t before = B->device_bytes();
         B->build_persist(st);
+        ctx->held_bytes += B->device_bytes() - before;
+    }
+    if (synced && !B->sy.built) {
+        const int64_t before = B->device_bytes();
+        B->build_synced(st);
         ctx->held_bytes += B->device_bytes() - before;
     }
     B->init_kets(st);
@@ -502,22 +585,32 @@ void batch_run(tbk_batch* B, int32_t flags, double* mu_out, tbk_batch_stats* sta
     }
     const int64_t nchunks = (B->max_steps + kChunkSteps - 1) / kChunkSteps;
     int64_t graphs_launched = 0;
-    if (use_graphs && static_cast<int64_t>(B->graphs.size()) != nchunks) {
-        B->drop_graphs();
-        B->graphs.assign(static_cast<size_t>(nchunks), nullptr);
+    std::vector<hipGraphExec_t>& chunk_graphs = synced ? B->sy.graphs : B->graphs;
+    if (use_graphs && static_cast<int64_t>(chunk_graphs.size()) != nchunks) {
+        for (hipGraphExec_t g : chunk_graphs)
+            if (g) (void)hipGraphExecDestroy(g);
+        chunk_graphs.assign(static_cast<size_t>(nchunks), nullptr);
     }
-    for (int64_t c = 0; c < nchunks; ++c) {
+    // the launches of chunk c: one per step, or one synchronised launch
+    auto launch_chunk = [&](int64_t c) {
+        if (synced) {
+            B->launch_synced_chunk(c, st);
+            return;
+        }
         const int64_t k0 = c * kChunkSteps, k1 = std::min(B->max_steps, k0 + kChunkSteps);
+        for (int64_t k = k0; k < k1; ++k) B->launch_step(k, st);
+    };
+    for (int64_t c = 0; c < nchunks; ++c) {
         if (!use_graphs) {
-            for (int64_t k = k0; k < k1; ++k) B->launch_step(k, st);
+            launch_chunk(c);
             continue;
         }
-        hipGraphExec_t& ex = B->graphs[static_cast<size_t>(c)];
+        hipGraphExec_t& ex = chunk_graphs[static_cast<size_t>(c)];
         if (!ex) {
             hipGraph_t g = nullptr;
             HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                for (int64_t k = k0; k < k1; ++k) B->launch_step(k, st);
+                launch_chunk(c);
             } catch (...) {
                 (void)hipStreamEndCapture(st, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -533,18 +626,21 @@ void batch_run(tbk_batch* B, int32_t flags, double* mu_out, tbk_batch_stats* sta
     }
     HIPC(hipEventRecord(ctx->events[1], st));
     std::vector<double> mu(static_cast<size_t>(B->mu_total));
+    int32_t err = 0;
     HIPC(hipMemcpyAsync(mu.data(), B->mu.p, mu.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (synced) HIPC(hipMemcpyAsync(&err, B->sy.error.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
+    if (err != 0) raise(TBK_E_DEVICE, "synchronised batch launch: a grid barrier gave up");
     std::memcpy(mu_out, mu.data(), mu.size() * sizeof(double));
     if (stats) {
         float ms = 0.f;
         HIPC(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
         stats->kernel_ms = ms;
-        stats->launches = B->max_steps;
+        stats->launches = synced ? nchunks : B->max_steps;
         stats->graphs = graphs_launched;
         stats->bytes_total = B->bytes_total();
         stats->rows_total = B->rows_total;
-        stats->workgroups = B->wg_end.back();
+        stats->workgroups = synced ? B->sy.grid : B->wg_end.back();
         stats->cpl = B->cpl_used();
         stats->blocks_per_cu = B->bpc;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -566,6 +662,7 @@ void set_shape(tbk_batch* B, int32_t cpl, int32_t bpc) {
     B->drop_graphs();
     B->ps.grid = static_cast<int64_t>(kPersistQueues) * std::max(1, B->ctx->num_cu / kPersistQueues) * std::max(1, bpc);
     B->build_descriptors(B->ctx->stream);
+    B->sy.built = false;  // its per-step workgroup counts and partials follow the descriptors
     B->ctx->held_bytes += B->device_bytes() - before;
 }
 

@@ -621,22 +621,25 @@ struct SlotDesc {
     TileOrder order;
 };
 
+// Rows of workgroup index w of a slot launch at step offset s (step
+// d.k0 + s of the slot) that wave `wave` owns: the step on those rows, and
+// the wave's moment sums (all lanes hold them).  False (for the whole
+// workgroup) once the slot is done.  No barrier.
 template <typename VT, bool CH, int LPR, int CPL>
-__global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __restrict__ slots,
-                                                            const int32_t* __restrict__ wg_slot, int32_t s,
-                                                            int32_t Rp, int32_t opts, double2* __restrict__ partial) {
+__device__ __forceinline__ bool slots_wave_rows(const SlotDesc* __restrict__ slots,
+                                                const int32_t* __restrict__ wg_slot, int64_t w, int wave, int32_t s,
+                                                int32_t Rp, int32_t opts, double& p0, double& p1) {
     using V2 = typename Vec2<VT>::type;
     using H = typename HVal<VT, CH>::type;
     constexpr int RPW = kWave / LPR;
     constexpr int TR = kWavesPerBlock * RPW;
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
     const int sub = lane / LPR;
     const int q = lane % LPR;
 
-    const SlotDesc& d = slots[wg_slot[blockIdx.x]];
+    const SlotDesc& d = slots[wg_slot[w]];
     const int64_t k = d.k0 + s;
-    if (k >= d.steps) return;  // the whole workgroup, before any barrier
+    if (k >= d.steps) return false;
     const V2* x = static_cast<const V2*>(d.blk[k % 3]);
     const V2* z = static_cast<const V2*>(d.blk[(k + 2) % 3]);
     V2* y = static_cast<V2*>(d.blk[(k + 1) % 3]);
@@ -650,12 +653,13 @@ __global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __re
     const int32_t rows = d.rows;
 
     // this workgroup's tiles: as tile_range with xcd_split, within the slot
-    const int64_t j = static_cast<int64_t>(blockIdx.x) - d.wg0;
+    const int64_t j = w - d.wg0;
     const int64_t ntiles = static_cast<int64_t>(order.P) * order.Q * order.tpc;
     const int64_t xcd = j & 7, jj = j >> 3, per = d.nwg >> 3;
     const int64_t first = (ntiles * xcd) / 8 + jj, end = (ntiles * (xcd + 1)) / 8;
 
-    double p0 = 0.0, p1 = 0.0;
+    p0 = 0.0;
+    p1 = 0.0;
     for (int64_t t = first; t < end; t += per) {
         uint32_t slab_unused;
         const int64_t rt = visit_row_tile(order, static_cast<uint32_t>(t), slab_unused);
@@ -668,10 +672,24 @@ __global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __re
         else
             step_row<VT, CH, LPR, CPL, kRecDbl>(rp, ci, hv, x, z, y, x, r, q, Rp, alpha, beta, o, p0, p1);
     }
-
-    __shared__ double red[2][kWavesPerBlock];
     p0 = wave_sum(p0);
     p1 = wave_sum(p1);
+    return true;
+}
+
+// Workgroup w of a slot launch at step offset s: its share of the slot's rows
+// for step d.k0 + s, and the workgroup's moment partials in partial[w]
+// (waves' sums added in wave order).  Nothing (the whole workgroup, before
+// any barrier) once the slot is done.
+template <typename VT, bool CH, int LPR, int CPL>
+__device__ __forceinline__ void slots_workgroup_step(const SlotDesc* __restrict__ slots,
+                                                     const int32_t* __restrict__ wg_slot, int64_t w, int32_t s,
+                                                     int32_t Rp, int32_t opts, double2* __restrict__ partial) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    double p0, p1;
+    if (!slots_wave_rows<VT, CH, LPR, CPL>(slots, wg_slot, w, wave, s, Rp, opts, p0, p1)) return;
+    __shared__ double red[2][kWavesPerBlock];
     if (lane == 0) {
         red[0][wave] = p0;
         red[1][wave] = p1;
@@ -679,11 +697,139 @@ __global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __re
     __syncthreads();
     if (threadIdx.x == 0) {
         double s0 = red[0][0], s1 = red[1][0];
-        for (int w = 1; w < kWavesPerBlock; ++w) {
-            s0 = s0 + red[0][w];
-            s1 = s1 + red[1][w];
+        for (int i = 1; i < kWavesPerBlock; ++i) {
+            s0 = s0 + red[0][i];
+            s1 = s1 + red[1][i];
         }
-        partial[blockIdx.x] = make_double2(s0, s1);
+        partial[w] = make_double2(s0, s1);
+    }
+}
+
+template <typename VT, bool CH, int LPR, int CPL>
+__global__ __launch_bounds__(kBlock) void cheb_slots_kernel(const SlotDesc* __restrict__ slots,
+                                                            const int32_t* __restrict__ wg_slot, int32_t s,
+                                                            int32_t Rp, int32_t opts, double2* __restrict__ partial) {
+    slots_workgroup_step<VT, CH, LPR, CPL>(slots, wg_slot, blockIdx.x, s, Rp, opts, partial);
+}
+
+// Moments of slot `slot` at step d.k0 + s: its workgroups' partials
+// (per_wg of them per workgroup, from partial[d.wg0 * per_wg]) summed in a
+// fixed order, then mu_0, mu_1 (step 0) or
+// the doubling identities mu_2k = 2 s.x - mu_0, mu_2k+1 = 2 s.y - mu_1.
+// Nothing once the slot is done; ends with a workgroup barrier.
+__device__ __forceinline__ void slot_step_moments(const SlotDesc* __restrict__ slots, int32_t slot, int32_t s,
+                                                  const double2* __restrict__ partial, int per_wg = 1) {
+    const SlotDesc& d = slots[slot];
+    const int64_t k = d.k0 + s;
+    if (k >= d.steps) return;  // the whole workgroup
+    const double2* p = partial + d.wg0 * per_wg;
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t i = threadIdx.x; i < static_cast<int64_t>(d.nwg) * per_wg; i += kBlock) {
+        a0 = a0 + p[i].x;
+        a1 = a1 + p[i].y;
+    }
+    __shared__ double r0[kBlock], r1[kBlock];
+    r0[threadIdx.x] = a0;
+    r1[threadIdx.x] = a1;
+    __syncthreads();
+    for (int w = kBlock / 2; w >= 1; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
+            r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* m = d.mu;
+        if (k == 0) {
+            m[0] = r0[0];
+            m[1] = r1[0];
+        } else {
+            if (2 * k < d.n) m[2 * k] = 2.0 * r0[0] - m[0];
+            if (2 * k + 1 < d.n) m[2 * k + 1] = 2.0 * r1[0] - m[1];
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------- grid-synchronised batched recursion
+// One launch runs steps [k0, k0 + nsteps) of every slot with the work of the
+// per-step launches: at each step the launch's G workgroups (all resident:
+// the host sizes G from the kernel's occupancy) take the step's workgroup
+// indices w = blockIdx.x, blockIdx.x + G, ... below step_wg[i] (the active
+// slots' workgroups), so each slot's rows are split and visited exactly as
+// in cheb_slots_kernel; with G a multiple of 8, index w runs on XCD w % 8 as
+// it does there.  The waves of a workgroup go through those indices on their
+// own, each writing its own moment partial (no workgroup barrier between
+// indices), so the moments group the same products per wave rather than per
+// workgroup.  A grid barrier ends the step; after it the step's moments are
+// formed (slot j by workgroup j mod G) while the next step runs.  The
+// partials of step i are in half i % 2 of `partial`, so the moments of step
+// i are read before step i + 2 (after the next barrier) overwrites them.
+//
+// Barrier: every workgroup's waves finish their stores (the workgroup
+// barrier waits for them), then thread 0 adds to *bar with an agent-scope
+// release (the XCD's L2 is written back) and polls *bar until all G
+// workgroups of this step have arrived, then takes an agent-scope acquire
+// (its CU's caches are invalidated), so rows written on other XCDs are read
+// from memory.  The poll is bounded: past spin_limit polls, or once another
+// barrier has given up, it sets *error and every workgroup leaves the loop;
+// the host then discards the results.
+struct SyncSched {
+    const SlotDesc* slots;
+    const int32_t* wg_slot;
+    const int64_t* step_wg;  // per step of the launch: workgroup indices of that step (active slots' prefix)
+    const int32_t* step_na;  // per step of the launch: active slots
+    double2* partial;        // two halves of kWavesPerBlock * wmax partials (one per wave)
+    int64_t wmax;
+    int32_t* bar;            // barrier arrivals of this launch (zeroed before it)
+    int32_t* error;
+    int32_t s0, nsteps;      // step offsets s0 .. s0 + nsteps - 1 (slot step d.k0 + s)
+    int32_t Rp, opts, spin_limit;
+};
+
+// Grid barrier number i (1-based) of the launch; false if it gave up.
+__device__ __forceinline__ bool grid_barrier(int32_t* bar, int32_t target, int32_t spin_limit, int32_t* error) {
+    __shared__ int32_t passed;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t ok = 1, polls = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++polls > spin_limit ||
+                ((polls & 63) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        passed = ok;
+    }
+    __syncthreads();
+    return passed != 0;
+}
+
+template <typename VT, bool CH, int LPR, int CPL>
+__global__ __launch_bounds__(kBlock) void cheb_slots_sync_kernel(SyncSched a) {
+    const int64_t G = gridDim.x;
+    const int wave = threadIdx.x / kWave;
+    for (int32_t i = 0; i < a.nsteps; ++i) {
+        const int32_t s = a.s0 + i;
+        double2* part = a.partial + (s & 1) * (kWavesPerBlock * a.wmax);
+        const int64_t W = a.step_wg[i];
+        // each wave goes through its workgroup indices on its own (no
+        // workgroup barrier between them); one partial per wave
+        for (int64_t w = blockIdx.x; w < W; w += G) {
+            double p0, p1;
+            if (slots_wave_rows<VT, CH, LPR, CPL>(a.slots, a.wg_slot, w, wave, s, a.Rp, a.opts, p0, p1) &&
+                (threadIdx.x & (kWave - 1)) == 0)
+                part[kWavesPerBlock * w + wave] = make_double2(p0, p1);
+        }
+        if (!grid_barrier(a.bar, (i + 1) * static_cast<int32_t>(G), a.spin_limit, a.error)) return;
+        for (int32_t j = static_cast<int32_t>(blockIdx.x); j < a.step_na[i]; j += static_cast<int32_t>(G))
+            slot_step_moments(a.slots, j, s, part, kWavesPerBlock);
     }
 }
 

@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cheb_kernels.h"
 
 namespace tbk {
@@ -112,6 +114,57 @@ bool launch_persist(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSch
     TBK_PERSIST_CASE(256, 4)
 #undef TBK_PERSIST_CASE
     return false;
+}
+
+// The grid-synchronised batched recursion (cheb_slots_sync_kernel) for a
+// slot class: grid = min(bpc, resident workgroups per CU of the kernel)
+// workgroups per CU times num_cu, rounded down to a multiple of 8 (all of
+// them resident, so that every grid barrier can complete).  Returns the
+// grid launched, 0 if no kernel exists for the width.
+int64_t launch_sync_f64_real(int Rp, int cpl, int bpc, int num_cu, hipStream_t st, const SyncSched& s);
+int64_t launch_sync_f64_cplx(int Rp, int cpl, int bpc, int num_cu, hipStream_t st, const SyncSched& s);
+int64_t launch_sync_f32_real(int Rp, int cpl, int bpc, int num_cu, hipStream_t st, const SyncSched& s);
+int64_t launch_sync_f32_cplx(int Rp, int cpl, int bpc, int num_cu, hipStream_t st, const SyncSched& s);
+
+template <typename VT, bool CH, int LPR, int CPL>
+int64_t launch_sync_one(int bpc, int num_cu, hipStream_t st, const SyncSched& s) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cheb_slots_sync_kernel<VT, CH, LPR, CPL>, kBlock, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        return 0;
+    const int64_t grid = (static_cast<int64_t>(std::min(bpc, per_cu)) * num_cu / 8) * 8;
+    if (grid < 8) return 0;
+    hipLaunchKernelGGL((cheb_slots_sync_kernel<VT, CH, LPR, CPL>), dim3(static_cast<unsigned>(grid)), dim3(kBlock), 0,
+                       st, s);
+    return grid;
+}
+
+template <typename VT, bool CH>
+int64_t launch_sync(int Rp, int cpl, int bpc, int num_cu, hipStream_t st, const SyncSched& s) {
+#define TBK_SYNC_CASE(RP, C) \
+    if (Rp == RP && cpl == C) return launch_sync_one<VT, CH, RP / C, C>(bpc, num_cu, st, s);
+    TBK_SYNC_CASE(1, 1)
+    TBK_SYNC_CASE(2, 1)
+    TBK_SYNC_CASE(2, 2)
+    TBK_SYNC_CASE(4, 1)
+    TBK_SYNC_CASE(4, 2)
+    TBK_SYNC_CASE(8, 1)
+    TBK_SYNC_CASE(8, 2)
+    TBK_SYNC_CASE(8, 4)
+    TBK_SYNC_CASE(16, 1)
+    TBK_SYNC_CASE(16, 2)
+    TBK_SYNC_CASE(16, 4)
+    TBK_SYNC_CASE(32, 1)
+    TBK_SYNC_CASE(32, 2)
+    TBK_SYNC_CASE(32, 4)
+    TBK_SYNC_CASE(64, 1)
+    TBK_SYNC_CASE(64, 2)
+    TBK_SYNC_CASE(128, 2)
+    TBK_SYNC_CASE(128, 4)
+    TBK_SYNC_CASE(256, 4)
+#undef TBK_SYNC_CASE
+    return 0;
 }
 
 // Launch one step; returns false if no kernel exists for the combination.

@@ -11,4 +11,7 @@ bool launch_slots_f32_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const Slo
 bool launch_persist_f32_cplx(int Rp, int cpl, dim3 grid, hipStream_t st, const PersistSched& s) {
     return launch_persist<float, true>(Rp, cpl, grid, st, s);
 }
+int64_t launch_sync_f32_cplx(int Rp, int cpl, int bpc, int num_cu, hipStream_t st, const SyncSched& s) {
+    return launch_sync<float, true>(Rp, cpl, bpc, num_cu, st, s);
+}
 }  // namespace tbk

@@ -480,13 +480,16 @@ class Batch:
         self.ctx, self._h, self.R, self.precision = ctx, h, int(R), int(precision)
         self.offsets = np.concatenate([[0], np.cumsum(self.n_moments)])
 
-    def run(self, graphs: bool = False, persistent: bool = False):
+    def run(self, graphs: bool = False, persistent: bool = False, synced: bool = False):
         """Returns (list of moment arrays in problem order, stats).  graphs:
         capture each chunk of 64 steps in a hipGraph; persistent: one
-        persistent launch per chunk (problems on XCD queues)."""
+        persistent launch per chunk (problems on XCD queues); synced: one
+        persistent launch per chunk over the whole GPU, grid barriers between
+        steps (the per-step launches' work and moments)."""
         mu = np.zeros(int(self.offsets[-1]), dtype=np.float64)
         st = N.tbk_batch_stats()
-        flags = (N.TBK_BATCH_GRAPHS if graphs else 0) | (N.TBK_BATCH_PERSISTENT if persistent else 0)
+        flags = ((N.TBK_BATCH_GRAPHS if graphs else 0) | (N.TBK_BATCH_PERSISTENT if persistent else 0)
+                 | (N.TBK_BATCH_SYNCED if synced else 0))
         N.check_kpm(N.kpm().tbk_batch_run(self._h, flags, _dp(mu), C.byref(st)))
         return [mu[self.offsets[i]:self.offsets[i + 1]] for i in range(self.n_moments.size)], st.as_dict()
 
