@@ -229,10 +229,19 @@ def batch_leg(ctx, nprob, R=8):
     log("batch persistent: kernel %.1f ms, wall %.1f ms, %.1f GB/s, %d launches of %d workgroups, "
         "max |dmu|/max|mu| vs per-step launches %.2e" % (stp["kernel_ms"], stp["wall_ms"], gbps_p, stp["launches"],
                                                          stp["workgroups"], dmu_p))
-    # one grid-synchronised persistent launch per chunk (TBK_BATCH_SYNCED)
-    mus_y, _ = B.run(synced=True)
-    yruns = [B.run(graphs=True, synced=True)[1] for _ in range(2)]
-    sty = min(yruns, key=lambda d: d["kernel_ms"])
+    # one grid-synchronised persistent launch per chunk (TBK_BATCH_SYNCED), at
+    # the tuned shape and at 4 values per lane, 4 workgroups per CU (its best
+    # on this batch); the tuned shape is restored afterwards
+    tuned_shape = (st["cpl"], st["blocks_per_cu"])
+    sty, mus_y = None, None
+    for shape in dict.fromkeys([tuned_shape, (4, 4)]):
+        B.set_shape(*shape)
+        m_y, _ = B.run(synced=True)
+        for _ in range(2):
+            r = B.run(graphs=True, synced=True)[1]
+            if sty is None or r["kernel_ms"] < sty["kernel_ms"]:
+                sty, mus_y = dict(r, shape=shape), m_y
+    B.set_shape(*tuned_shape)
     gbps_y = sty["bytes_total"] / (sty["kernel_ms"] * 1e-3) / 1e9
     dmu_y = max(float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(mus_y, mus))
     log("batch synced: kernel %.1f ms, wall %.1f ms, %.1f GB/s, %d launches of %d workgroups, "
@@ -275,6 +284,7 @@ def batch_leg(ctx, nprob, R=8):
                        "workgroups": stp["workgroups"], "moments_per_s": moments / (stp["wall_ms"] * 1e-3),
                        "max_rel_dmu_vs_per_step_launches": dmu_p},
         "synced": {"kernel_ms": sty["kernel_ms"], "wall_ms": sty["wall_ms"], "GBps": gbps_y,
+                   "shape": {"cpl": sty["shape"][0], "blocks_per_cu": sty["shape"][1]},
                    "frac_hbm": gbps_y / HBM_PEAK_GBS, "launches": sty["launches"], "workgroups": sty["workgroups"],
                    "moments_per_s": moments / (sty["wall_ms"] * 1e-3), "max_rel_dmu_vs_per_step_launches": dmu_y},
         "per_solve_kernel_ms": k_ms, "per_solve_wall_ms": 1e3 * w_s,
