@@ -361,7 +361,7 @@ struct tbk_batch {
         sy.step_wg.reserve(wg.size() * sizeof(int64_t));
         sy.step_na.reserve(na.size() * sizeof(int32_t));
         sy.part.reserve(static_cast<size_t>(2 * tbk::kWavesPerBlock * sy.wmax) * sizeof(double2));
-        sy.bar.reserve(sizeof(int32_t));
+        sy.bar.reserve(tbk::kSyncWords * sizeof(int32_t));
         sy.error.reserve(sizeof(int32_t));
         HIPC(hipMemcpyAsync(sy.step_wg.p, wg.data(), wg.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
         HIPC(hipMemcpyAsync(sy.step_na.p, na.data(), na.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -375,8 +375,8 @@ struct tbk_batch {
         if (!sy.built || sy.wmax != wg_end.back())
             raise(TBK_E_INVALID, "synchronised plan does not match the launch shape");
         const int64_t k0 = c * kChunkSteps, k1 = std::min(max_steps, k0 + kChunkSteps);
-        hipLaunchKernelGGL(tbk::persist_reset_kernel, dim3(1), dim3(tbk::kBlock), 0, st, sy.bar.as<int32_t>(), 1,
-                           nullptr, 0, c == 0 ? sy.error.as<int32_t>() : nullptr);
+        hipLaunchKernelGGL(tbk::persist_reset_kernel, dim3(1), dim3(tbk::kBlock), 0, st, sy.bar.as<int32_t>(),
+                           tbk::kSyncWords, nullptr, 0, c == 0 ? sy.error.as<int32_t>() : nullptr);
         HIPC(hipGetLastError());
         tbk::SyncSched S{};
         S.slots = desc.as<const tbk::SlotDesc>();

@@ -752,6 +752,9 @@ __device__ __forceinline__ void slot_step_moments(const SlotDesc* __restrict__ s
     __syncthreads();
 }
 
+// XCC_ID hardware register (bits 3:0): the XCD the wave runs on
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
+
 // ------------------------------------------- grid-synchronised batched recursion
 // One launch runs steps [k0, k0 + nsteps) of every slot with the work of the
 // per-step launches: at each step the launch's G workgroups (all resident:
@@ -767,14 +770,13 @@ __device__ __forceinline__ void slot_step_moments(const SlotDesc* __restrict__ s
 // partials of step i are in half i % 2 of `partial`, so the moments of step
 // i are read before step i + 2 (after the next barrier) overwrites them.
 //
-// Barrier: every workgroup's waves finish their stores (the workgroup
-// barrier waits for them), then thread 0 adds to *bar with an agent-scope
-// release (the XCD's L2 is written back) and polls *bar until all G
-// workgroups of this step have arrived, then takes an agent-scope acquire
-// (its CU's caches are invalidated), so rows written on other XCDs are read
-// from memory.  The poll is bounded: past spin_limit polls, or once another
-// barrier has given up, it sets *error and every workgroup leaves the loop;
-// the host then discards the results.
+// Barriers (grid_barrier below): the workgroups of an XCD count their
+// arrivals, the last of them writes the XCD's L2 back once and signals the
+// grid counter, and every workgroup waits for all XCDs and then invalidates
+// its caches, so rows written on other XCDs are read from memory.  Polls are
+// bounded: past spin_limit polls, or once another wait has given up, a wait
+// sets *error and every workgroup leaves the loop; the host then discards
+// the results.
 struct SyncSched {
     const SlotDesc* slots;
     const int32_t* wg_slot;
@@ -782,30 +784,74 @@ struct SyncSched {
     const int32_t* step_na;  // per step of the launch: active slots
     double2* partial;        // two halves of kWavesPerBlock * wmax partials (one per wave)
     int64_t wmax;
-    int32_t* bar;            // barrier arrivals of this launch (zeroed before it)
+    int32_t* bar;            // kSyncWords counters of this launch (zeroed before it): [0] leaders'
+                             // arrivals, [1 + x] workgroups on XCD x, [1 + kXcds + x] arrivals on XCD x
     int32_t* error;
     int32_t s0, nsteps;      // step offsets s0 .. s0 + nsteps - 1 (slot step d.k0 + s)
     int32_t Rp, opts, spin_limit;
 };
 
-// Grid barrier number i (1-based) of the launch; false if it gave up.
-__device__ __forceinline__ bool grid_barrier(int32_t* bar, int32_t target, int32_t spin_limit, int32_t* error) {
+constexpr int kXcds = 16;                 // XCC_ID values
+constexpr int kSyncWords = 1 + 2 * kXcds;  // counters of a synchronised launch
+
+// Bounded poll of *ctr until it reaches target (thread 0); false, with
+// *error set, past spin_limit polls or once another wait has given up.
+__device__ __forceinline__ bool sync_poll(const int32_t* ctr, int32_t target, int32_t spin_limit, int32_t* error) {
+    int32_t polls = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++polls > spin_limit ||
+            ((polls & 63) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+            __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+// The launch's first barrier: every workgroup counts itself on its XCD,
+// then a plain grid barrier (each workgroup's thread 0 adds to bar[0]).
+// Afterwards *per_xcd is the number of workgroups on this workgroup's XCD
+// and *nxcd the number of XCDs with workgroups.
+__device__ __forceinline__ bool sync_register(int32_t* bar, int xcd, int32_t G, int32_t spin_limit, int32_t* error,
+                                              int32_t* per_xcd, int32_t* nxcd) {
+    __shared__ int32_t sh[3];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar + 1 + xcd, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const bool ok = sync_poll(bar, G, spin_limit, error);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        int32_t n = 0;
+        for (int x = 0; x < kXcds; ++x)
+            n += __hip_atomic_load(bar + 1 + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0 ? 1 : 0;
+        sh[0] = ok ? 1 : 0;
+        sh[1] = __hip_atomic_load(bar + 1 + xcd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh[2] = n;
+    }
+    __syncthreads();
+    *per_xcd = sh[1];
+    *nxcd = sh[2];
+    return sh[0] != 0;
+}
+
+// Grid barrier i (1-based, after the first) of the launch; false if it gave
+// up.  The workgroups of an XCD count their arrivals on that XCD (their
+// stores are in its L2: the workgroup barrier waited for them); the last of
+// them writes the XCD's L2 back once (agent-scope release) and adds to
+// bar[0]; every workgroup waits until all XCDs have, then takes an
+// agent-scope acquire (its CU's caches and its XCD's L2 invalidated).
+__device__ __forceinline__ bool grid_barrier(int32_t* bar, int xcd, int32_t per_xcd, int32_t nxcd, int32_t G,
+                                             int32_t i, int32_t spin_limit, int32_t* error) {
     __shared__ int32_t passed;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        int32_t ok = 1, polls = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++polls > spin_limit ||
-                ((polls & 63) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-                __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
+        const int32_t old =
+            __hip_atomic_fetch_add(bar + 1 + kXcds + xcd, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old + 1 == i * per_xcd) __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const bool ok = sync_poll(bar, G + i * nxcd, spin_limit, error);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        passed = ok;
+        passed = ok ? 1 : 0;
     }
     __syncthreads();
     return passed != 0;
@@ -815,6 +861,9 @@ template <typename VT, bool CH, int LPR, int CPL>
 __global__ __launch_bounds__(kBlock) void cheb_slots_sync_kernel(SyncSched a) {
     const int64_t G = gridDim.x;
     const int wave = threadIdx.x / kWave;
+    const int xcd = xcc_id();
+    int32_t per_xcd, nxcd;
+    if (!sync_register(a.bar, xcd, static_cast<int32_t>(G), a.spin_limit, a.error, &per_xcd, &nxcd)) return;
     for (int32_t i = 0; i < a.nsteps; ++i) {
         const int32_t s = a.s0 + i;
         double2* part = a.partial + (s & 1) * (kWavesPerBlock * a.wmax);
@@ -827,7 +876,7 @@ __global__ __launch_bounds__(kBlock) void cheb_slots_sync_kernel(SyncSched a) {
                 (threadIdx.x & (kWave - 1)) == 0)
                 part[kWavesPerBlock * w + wave] = make_double2(p0, p1);
         }
-        if (!grid_barrier(a.bar, (i + 1) * static_cast<int32_t>(G), a.spin_limit, a.error)) return;
+        if (!grid_barrier(a.bar, xcd, per_xcd, nxcd, static_cast<int32_t>(G), i + 1, a.spin_limit, a.error)) return;
         for (int32_t j = static_cast<int32_t>(blockIdx.x); j < a.step_na[i]; j += static_cast<int32_t>(G))
             slot_step_moments(a.slots, j, s, part, kWavesPerBlock);
     }
@@ -881,8 +930,6 @@ struct PersistSched {
     int32_t xcc_queue[16];         // queue of each XCC_ID value, -1: none
 };
 
-// XCC_ID hardware register (bits 3:0): the XCD the wave runs on
-__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
 
 // largest i in [0, n) with a[i] <= v (a ascending, a[0] <= v)
 __device__ __forceinline__ int32_t last_at_most(const int64_t* a, int32_t n, int64_t v) {
