@@ -292,3 +292,38 @@ def test_synced_batch_follows_a_new_launch_shape(gpu_ctx):
             assert np.max(np.abs(x - z)) <= 1e-12 * np.max(np.abs(x))
             assert np.array_equal(z, w)
     B.close()
+
+
+@pytest.mark.parametrize("precision", [kpm.FP64, kpm.FP32])
+def test_persistent_forms_with_complex_hamiltonians(gpu_ctx, precision):
+    """Complex Hamiltonians (open square and honeycomb lattices in a Landau
+    gauge) through both persistent forms: moments within 1e-12 (fp64) /
+    1e-6 (fp32) of the per-step launches, and (fp64) within 1e-10 of the
+    oracle."""
+    rng = np.random.default_rng(31)
+    probs = []
+    for i, (model, Lc) in enumerate((("square", 14), ("honeycomb", 9), ("square", 21), ("honeycomb", 17))):
+        c = L.UnitCell.model(model)
+        c.add_hopping_nn("A", 1, 1.0)
+        lat = L.PartialLattice(c, (Lc, Lc + 2))
+        lat.set_gauge(L.GAUGE_LANDAU, 2 * np.pi / 7)
+        lat.set_disorder(float(rng.uniform(0, 2)), 40 + i)
+        H = lat.get_Hamiltonian_sparse(True)
+        probs.append((H, 1.02 * (COORD[model] + 1.0), 0.0, [64, 130, 257, 30][i], 600 + i))
+    B = make_batch(gpu_ctx, probs, R=6, precision=precision)
+    ref, _ = B.run()
+    tol = 1e-12 if precision == kpm.FP64 else 1e-6
+    for run in (dict(persistent=True), dict(synced=True), dict(graphs=True, synced=True)):
+        got, _ = B.run(**run)
+        for x, z in zip(ref, got):
+            assert np.max(np.abs(x - z)) <= tol * np.max(np.abs(x)), run
+    if precision == kpm.FP64:
+        got, _ = B.run(synced=True)
+        for (Hh, ah, bh, n, seed), y in zip(probs, got):
+            K = gpu_ctx.random_block(Hh.n, 6, seed=seed)
+            kets = K.download()
+            Hs = O.rescale(O.CSC(Hh.n, Hh.colptr, Hh.rowidx, Hh.values), ah, bh)
+            want = oracle.cheb_braket(Hs, n, kets, kets)
+            assert np.max(np.abs(y - want)) <= 1e-10 * np.max(np.abs(want))
+            K.close()
+    B.close()
