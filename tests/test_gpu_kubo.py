@@ -527,14 +527,16 @@ def test_fp32_recursion_sigma_error_against_fp64(gpu_ctx):
     K1.close()
 
 
-@pytest.mark.parametrize("N,L0,R", [(300, 9, 3), (512, 12, 2), (256, 7, 5)])
+@pytest.mark.parametrize("N,L0,R", [(300, 9, 3), (512, 12, 2), (256, 7, 5), (256, 2, 1), (256, 4, 3)])
 def test_wide_contraction_tiles_give_the_same_bits(gpu_ctx, N, L0, R):
     """The pre-split contraction's 256 x 128 tiles (64 x 64 per wave, the
     default) and its 128 x 128 tiles form the same sums: Gamma bit-identical
     between the two and to the BF16x3 contraction of the oracle's fp64
     stacks, at N not a multiple of the tile (rows past N clamped and not
     stored) and with K = sites x R not a multiple of the 16-column stage (a
-    short last stage through registers)."""
+    short last stage through registers); K = 6 (one short stage) and K = 80
+    (five full stages: one pass of the three-buffer main loop, then its
+    last steps) cover the pipeline's ends."""
     c = L.UnitCell.model("square")
     c.add_hopping_nn("A", 1, 1.0)
     lat = L.PartialLattice(c, (L0, L0 + 1))
